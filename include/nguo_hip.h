@@ -1,0 +1,175 @@
+/*
+ * nguo_hip.h — C ABI of the MI355X (gfx950) implicit-IPC inner loop.
+ *
+ * One shared library, libnguo_hip.so, exports everything declared here. The boundary
+ * carries plain pointers and sizes only: the caller owns every host buffer it passes,
+ * the library owns every device buffer it creates. Every entry point returns an int
+ * status (NGUO_OK == 0, negative on error); nguo_last_error() returns the message of the
+ * last failure on the calling thread. A handle is used from one thread at a time and may
+ * be moved between threads (the reference's solver objects are Send, not Sync).
+ *
+ * Reference interfaces each group replaces (paths under core-engine/crates/):
+ *   nguo_ls_*    engine-core/src/linsolve.rs:21-28   trait LinearSolver { solve, last_stats }
+ *                engine-gpu/src/pcg.rs:737-774       GpuLinearSolver (the wgpu PCG)
+ *   nguo_step_*  engine-gpu/src/gpu_implicit.rs:159-659  GpuImplicitSolver::{new,step,positions}
+ *                engine-core/src/implicit.rs:1951-2191   ImplicitSolver::step (the CPU tier)
+ *   nguo_lbvh_*  engine-gpu/src/morton.rs:61-230, radix.rs:102-135, karras.rs:91-262,
+ *                bvh_aabb.rs, bvh_traverse.rs:77-260 (GpuMorton/GpuRadixSort/GpuKarras/...)
+ *   nguo_scene_* engine-api/src/scene.rs:89-229 (Scene::lower + build_implicit): the
+ *                synthetic workload generators that stand in for the scene JSON.
+ */
+#ifndef NGUO_HIP_H
+#define NGUO_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NGUO_OK 0
+#define NGUO_E_INVALID (-1)     /* bad argument / malformed scene (Result::Err in the reference) */
+#define NGUO_E_DEVICE (-2)      /* HIP runtime error or no device */
+#define NGUO_E_NOMEM (-3)       /* device allocation failed */
+#define NGUO_E_CAPACITY (-4)    /* a fixed-capacity list overflowed (contacts, candidates) */
+#define NGUO_E_UNSUPPORTED (-5) /* input the device path does not handle (stated in the message) */
+
+#define NGUO_SHELL_STVK 0
+#define NGUO_SHELL_ARAP 1
+#define NGUO_BARRIER_LOG 0
+#define NGUO_BARRIER_CUBIC 1
+#define NGUO_PRECOND_BLOCK 0   /* 3x3 block Jacobi (engine-gpu/src/pcg.rs GpuPrecond::Block) */
+#define NGUO_PRECOND_SCHWARZ 1 /* additive Schwarz over vertex domains (GpuPrecond::SchwarzMl) */
+
+#define NGUO_COLLIDER_HALFSPACE 0
+#define NGUO_COLLIDER_SPHERE 1
+
+/* engine-core/src/contact.rs:21-31 Collider. HalfSpace: a = point, b = unit normal.
+ * Sphere: a = centre, radius. */
+typedef struct nguo_collider {
+  int32_t kind;
+  float a[3];
+  float b[3];
+  float radius;
+} nguo_collider;
+
+/* The subset of engine-core/src/implicit.rs:30-229 ImplicitParams the path reads. */
+typedef struct nguo_params {
+  float dt;
+  float gravity[3];
+  uint32_t newton_iters;
+  float bend_stiffness;   /* hinge bending k_b (weight w = 1 per hinge) */
+  float shell_mu;         /* isotropic shell Lame mu */
+  float shell_lambda;     /* isotropic shell Lame lambda */
+  int32_t shell_model;    /* NGUO_SHELL_* */
+  float eiganalysis_eps;  /* degenerate-eigenvalue guard of the shell eigensystem */
+  float contact_dhat;     /* barrier activation distance */
+  float contact_kappa;    /* barrier base stiffness (fallback of the Rayleigh stiffness) */
+  float friction;         /* Coulomb mu for contact stencils */
+  float friction_eps;     /* slip regularisation */
+  int32_t barrier_kind;   /* NGUO_BARRIER_* */
+  int32_t self_collision; /* vertex-triangle + edge-edge contact (incl. static mesh colliders) */
+  float max_dx;           /* uniform search-direction cap, INFINITY = off */
+  float target_toi;       /* Newton termination target, 0 = off */
+  uint32_t min_newton_steps;
+  uint32_t ccd_max_iter;
+  float cg_tol;           /* relative residual ||b-Ax||/||b|| of each Newton solve */
+  uint32_t cg_max_iter;
+  float floor_y;          /* -INFINITY = no floor */
+} nguo_params;
+
+/* A scene in the solver's vertex space (engine-api/src/scene.rs:89-152 `lower` output plus
+ * the ImplicitSolver setters implicit.rs:819-1563). All arrays are caller-owned. */
+typedef struct nguo_scene_desc {
+  uint32_t n_verts;
+  const float *positions;        /* [3*n_verts] */
+  const float *velocities;       /* [3*n_verts] or NULL (zero) */
+  const float *inv_mass;         /* [n_verts], 0 = pinned */
+  uint32_t n_tris;
+  const uint32_t *tris;          /* [3*n_tris] all triangles: collision + hinge topology */
+  const uint8_t *static_collider;/* [n_verts] static mesh collider mask or NULL */
+  uint32_t n_shell_faces;
+  const uint32_t *shell_faces;   /* [3*n_shell_faces] faces carrying the shell model */
+  uint32_t n_rods;
+  const uint32_t *rod_verts;     /* [2*n_rods] */
+  const float *rod_rest;         /* [n_rods] */
+  const float *rod_k;            /* [n_rods] */
+  uint32_t n_rod_hinges;
+  const uint32_t *rod_hinge_verts; /* [3*n_rod_hinges] (j, i = centre, k) */
+  const float *rod_hinge_rest;   /* [n_rod_hinges] rest angle, pi = straight */
+  const float *rod_hinge_stiff;  /* [n_rod_hinges] */
+  uint32_t n_colliders;
+  const nguo_collider *colliders;
+  nguo_params params;
+} nguo_scene_desc;
+
+/* engine-core/src/linsolve.rs:13-18 LinearStats (+ refinement sweeps). */
+typedef struct nguo_stats {
+  uint32_t iters;        /* inner PCG iterations, summed over refinement sweeps */
+  uint32_t sweeps;       /* FP64 refinement sweeps */
+  double residual;       /* final ||b - A x|| / ||b||, evaluated in FP64 */
+} nguo_stats;
+
+/* engine-core/src/metrics.rs:12-35 StepMetrics (the fields the device path fills). */
+typedef struct nguo_step_metrics {
+  double time;
+  double max_reresid;
+  uint32_t max_cg_iters;
+  uint32_t newton_iters;
+  uint32_t num_stencils;    /* active contact stencils at the last Newton iteration */
+  float max_velocity;
+  int32_t penetration_free; /* 1 = no edge pierces a triangle and all finite */
+  int32_t all_finite;
+  float last_toi;           /* CCD step fraction of the last Newton iteration */
+} nguo_step_metrics;
+
+typedef struct nguo_ctx nguo_ctx;
+typedef struct nguo_ls nguo_ls;
+typedef struct nguo_step nguo_step;
+typedef struct nguo_scene_buf nguo_scene_buf;
+
+const char *nguo_last_error(void);
+
+/* ---- device context (engine-gpu/src/lib.rs:79-196 GpuContext::new) */
+int nguo_ctx_create(int device, nguo_ctx **out);
+void nguo_ctx_destroy(nguo_ctx *ctx);
+/* Device name and CU count, for reports. */
+int nguo_ctx_info(nguo_ctx *ctx, char *name, size_t name_len, int *cu_count);
+
+/* ---- LinearSolver seam (linsolve.rs:21-28; pcg.rs:737-774 GpuLinearSolver::solve).
+ * solve(): A is the scalar CSR the reference assembles (n rows, a multiple of 3 for the
+ * block preconditioner), b the right-hand side, tol relative to ||b||. x_out receives the
+ * solution; stats (may be NULL) receives what last_stats() would return. */
+int nguo_ls_create(nguo_ctx *ctx, int precond, uint32_t schwarz_domain, float schwarz_weight, nguo_ls **out);
+void nguo_ls_destroy(nguo_ls *ls);
+int nguo_ls_solve(nguo_ls *ls, uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx,
+                  const float *values, const float *b, float tol, uint32_t max_iter,
+                  float *x_out, nguo_stats *stats);
+int nguo_ls_last_stats(nguo_ls *ls, nguo_stats *out);
+/* y = A x through the device 3x3 BSR SpMV (the PCG's operator), for parity checks. */
+int nguo_ls_spmv(nguo_ls *ls, uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx,
+                 const float *values, const float *x, float *y);
+
+/* ---- whole implicit step (GpuImplicitSolver / ImplicitSolver::step) */
+int nguo_step_create(nguo_ctx *ctx, const nguo_scene_desc *scene, nguo_step **out);
+void nguo_step_destroy(nguo_step *st);
+int nguo_step_advance(nguo_step *st, uint32_t nsteps);
+int nguo_step_get_positions(nguo_step *st, float *out /* [3*n_verts] */);
+int nguo_step_get_velocities(nguo_step *st, float *out /* [3*n_verts] */);
+int nguo_step_set_state(nguo_step *st, const float *positions, const float *velocities);
+int nguo_step_get_metrics(nguo_step *st, nguo_step_metrics *out);
+
+/* ---- scene generators for the benchmark workloads (host only, no device needed).
+ * config: 0 sheet on icosphere, 1 tube garment on a capsule body with seam rods,
+ * 2 two tangled concentric tubes, 3 graded tube (param = circumference factor),
+ * 100 small sheet on icosphere (tests). */
+int nguo_scene_generate(int config, uint64_t seed, double param, nguo_scene_buf **out);
+int nguo_scene_view(const nguo_scene_buf *buf, nguo_scene_desc *out);
+void nguo_scene_free(nguo_scene_buf *buf);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NGUO_HIP_H */

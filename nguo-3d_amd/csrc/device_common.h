@@ -1,0 +1,72 @@
+// Device-side building blocks shared by the kernels of libnguo_hip: wavefront (64-lane)
+// and workgroup reductions with a fixed combination order (bitwise-deterministic results
+// for a fixed launch shape), and the SELL-64 3x3-block matrix view the PCG runs on.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nguo {
+
+constexpr int WAVE = 64;
+constexpr int BLOCK = 256;  // 4 wavefronts per workgroup
+
+template <class T> __device__ inline T wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+  return v;
+}
+template <class T> __device__ inline T wave_max(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) { T w = __shfl_xor(v, o, WAVE); v = v > w ? v : w; }
+  return v;
+}
+template <class T> __device__ inline T wave_min(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) { T w = __shfl_xor(v, o, WAVE); v = v < w ? v : w; }
+  return v;
+}
+
+// Sum over a 256-thread workgroup; the result is valid in thread 0. Fixed order.
+template <class T> __device__ inline T block_sum(T v, T *lds4) {
+  v = wave_sum(v);
+  int w = threadIdx.x / WAVE;
+  if ((threadIdx.x & (WAVE - 1)) == 0) lds4[w] = v;
+  __syncthreads();
+  T r = 0;
+  if (threadIdx.x == 0) r = (lds4[0] + lds4[1]) + (lds4[2] + lds4[3]);
+  __syncthreads();
+  return r;
+}
+
+// A 3x3-block matrix in SELL-64 layout: block rows are grouped in slices of 64; inside a
+// slice the k-th block of every row sits at slot slice_ptr[s] + 64k + lane, so one wavefront
+// reads 64 consecutive column indices and, per block entry e, 64 consecutive floats
+// (value plane e of the slot group). Padding slots hold col = row and value 0.
+struct SellView {
+  uint32_t nb;                    // block rows (= free vertices)
+  const uint32_t *slice_ptr;      // [nslices + 1], in slots (multiples of 64)
+  const uint32_t *row_len;        // [nb] blocks in each row
+  const uint32_t *col;            // [slots]
+  const float *val;               // [slots * 9], plane layout (see val_index)
+};
+
+__host__ __device__ inline size_t sell_val_index(uint32_t slot, int e) {
+  return (size_t)(slot & ~63u) * 9u + (size_t)e * 64u + (slot & 63u);
+}
+
+// Matrix-free contact term of the Newton system. Stencil s couples its (up to) four
+// vertices with H_s = hp_s g g^T + lambda_s (w w^T) (x) (I - n n^T) where g = [g_0..g_3]
+// (distance gradient), w the proximity weights and n the contact normal.
+struct ContactView {
+  uint32_t ns;                    // stencils (0 = no contact term)
+  const uint32_t *dof;            // [4 ns] free-vertex index or 0xffffffff (pinned)
+  const float *g;                 // [12 ns]
+  const float *w;                 // [4 ns]
+  const float *coef;              // [2 ns] hp, lambda
+  const float *n;                 // [3 ns]
+  float *sums;                    // [4 ns] per-apply: g.p (1) and sum w p (3)
+  const uint32_t *inc_ptr;        // [nb + 1] incidence of free vertices
+  const uint32_t *inc;            // [..] 4 s + a
+};
+
+}  // namespace nguo

@@ -1,0 +1,95 @@
+// Shared host-side plumbing of libnguo_hip: error reporting (thread-local last error, the
+// reference's GpuError / Result<(), String> split, engine-gpu/src/lib.rs:58-66), HIP call
+// checking and an owning device buffer.
+#pragma once
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "nguo_hip.h"
+
+namespace nguo {
+
+// Records `msg` as the calling thread's last error and returns `code`.
+int fail(int code, const std::string &msg);
+void clear_error();
+
+}  // namespace nguo
+
+#ifdef __HIPCC__
+#include <hip/hip_runtime.h>
+
+namespace nguo {
+
+struct HipError {
+  int code;
+  std::string msg;
+};
+
+#define NGUO_HIP(call)                                                                      \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      throw ::nguo::HipError{NGUO_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)}; \
+  } while (0)
+
+#define NGUO_REQUIRE(cond, code, msg)                      \
+  do {                                                     \
+    if (!(cond)) throw ::nguo::HipError{(code), (msg)};    \
+  } while (0)
+
+// Runs `body`, converting thrown HipError into a status code + last error message.
+template <class F> int guarded(F &&body) {
+  try {
+    clear_error();
+    return body();
+  } catch (const HipError &e) {
+    return fail(e.code, e.msg);
+  } catch (const std::bad_alloc &) {
+    return fail(NGUO_E_NOMEM, "host allocation failed");
+  }
+}
+
+// Owning device allocation of T[n] (hipMalloc / hipFree). Grows, never shrinks.
+template <class T> struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void reserve(size_t count) {
+    if (count <= n) return;
+    release();
+    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    hipError_t e = hipMalloc((void **)&p, bytes);
+    if (e != hipSuccess) {
+      p = nullptr;
+      throw HipError{NGUO_E_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e)};
+    }
+    n = count;
+  }
+  void upload(const T *src, size_t count, hipStream_t s) {
+    reserve(count);
+    if (count) NGUO_HIP(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  void upload(const std::vector<T> &v, hipStream_t s) { upload(v.data(), v.size(), s); }
+  void download(T *dst, size_t count, hipStream_t s) const {
+    if (count) NGUO_HIP(hipMemcpyAsync(dst, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
+  }
+  void zero(size_t count, hipStream_t s) {
+    reserve(count);
+    if (count) NGUO_HIP(hipMemsetAsync(p, 0, count * sizeof(T), s));
+  }
+};
+
+inline unsigned div_up(size_t a, unsigned b) { return (unsigned)((a + b - 1) / b); }
+
+}  // namespace nguo
+#endif

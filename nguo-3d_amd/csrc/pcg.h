@@ -1,0 +1,58 @@
+// FP32 preconditioned CG with FP64 iterative refinement on a SELL-64 3x3-block matrix plus
+// an optional matrix-free contact term. Used by the LinearSolver seam (nguo_ls_*) and by
+// the resident Newton step.
+#pragma once
+#include <vector>
+
+#include "device_common.h"
+#include "nguo_internal.h"
+
+namespace nguo {
+
+// Pattern + values of a SELL-64 3x3-block matrix on the device, with its host pattern.
+struct SellMatrix {
+  uint32_t nb = 0, nslices = 0, nslots = 0, nnzb = 0;
+  std::vector<uint32_t> h_slice_ptr, h_row_len, h_col;
+  DevBuf<uint32_t> slice_ptr, row_len, col, diag_slot;
+  DevBuf<float> val;
+  // cols[i] = sorted block columns of block row i (must contain i).
+  void build_pattern(uint32_t nblocks, const std::vector<std::vector<uint32_t>> &cols, hipStream_t s);
+  uint32_t slot(uint32_t row, uint32_t k) const { return h_slice_ptr[row / 64] + 64u * k + (row & 63u); }
+  // slot of block (row, col) from the host pattern, or 0xffffffff.
+  uint32_t find(uint32_t row, uint32_t colj) const;
+  SellView view() const { return SellView{nb, slice_ptr.p, row_len.p, col.p, val.p}; }
+  // Algorithmic bytes of one y = A x (f32 values, u32 column index; 3x3 blocks).
+  double spmv_bytes() const { return (double)nnzb * 40.0 + (double)nb * 28.0 + 4.0; }
+};
+
+struct PcgStats {
+  uint32_t iters = 0, sweeps = 0;
+  double residual = 0;
+  bool breakdown = false;
+};
+
+class Pcg {
+ public:
+  explicit Pcg(hipStream_t s) : s_(s) {}
+  ~Pcg();
+  // Solve (A + C) x = b to ||b - (A+C) x|| <= tol ||b|| (FP64 residual). b is a device
+  // f32 vector [3 nb]; the solution lands in x_out (device f32 [3 nb]).
+  PcgStats solve(const SellMatrix &M, const ContactView &C, const float *b, float tol, uint32_t max_iter, float *x_out);
+  // y = (A + C) x in f32, for parity checks and the SpMV benchmark.
+  void apply(const SellView &A, const ContactView &C, const float *x, float *y);
+  // Block-Jacobi inverse of (A + C)'s diagonal blocks (FP64 inversion, scalar fallback).
+  void build_precond(const SellMatrix &M, const ContactView &C);
+  uint32_t chunk = 8;  // iterations per host convergence check
+
+ private:
+  void ensure(uint32_t nb, uint32_t ns);
+  hipStream_t s_;
+  uint32_t cap_nb_ = 0, cap_ns_ = 0;
+  DevBuf<float> r_, z_, p_, q_, d_, minv_, bf_;
+  DevBuf<double> x64_, r64_, part_, scal_, sums64_;
+  DevBuf<uint32_t> flag_;
+  uint32_t *h_flag_ = nullptr;
+  double *h_scal_ = nullptr;
+};
+
+}  // namespace nguo

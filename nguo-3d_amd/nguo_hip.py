@@ -1,0 +1,243 @@
+"""ctypes binding of libnguo_hip.so (the C ABI in include/nguo_hip.h).
+
+This is the Python-side stand-in for the reference's Rust FFI crate: the same entry points a
+Rust `extern "C"` block would bind, used by the tests, the benchmark
+and the smoke check. It loads the in-tree library and fails loudly when it is missing; there
+is no fallback path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libnguo_hip.so")
+
+NGUO_OK = 0
+NGUO_E_INVALID = -1
+NGUO_E_DEVICE = -2
+NGUO_E_NOMEM = -3
+NGUO_E_CAPACITY = -4
+NGUO_E_UNSUPPORTED = -5
+
+EXPORTED = [
+    "nguo_last_error", "nguo_ctx_create", "nguo_ctx_destroy", "nguo_ctx_info",
+    "nguo_ls_create", "nguo_ls_destroy", "nguo_ls_solve", "nguo_ls_last_stats", "nguo_ls_spmv",
+    "nguo_step_create", "nguo_step_destroy", "nguo_step_advance", "nguo_step_get_positions",
+    "nguo_step_get_velocities", "nguo_step_set_state", "nguo_step_get_metrics",
+    "nguo_scene_generate", "nguo_scene_view", "nguo_scene_free",
+]
+
+
+class Collider(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("a", C.c_float * 3), ("b", C.c_float * 3), ("radius", C.c_float)]
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("dt", C.c_float), ("gravity", C.c_float * 3), ("newton_iters", C.c_uint32),
+        ("bend_stiffness", C.c_float), ("shell_mu", C.c_float), ("shell_lambda", C.c_float),
+        ("shell_model", C.c_int32), ("eiganalysis_eps", C.c_float), ("contact_dhat", C.c_float),
+        ("contact_kappa", C.c_float), ("friction", C.c_float), ("friction_eps", C.c_float),
+        ("barrier_kind", C.c_int32), ("self_collision", C.c_int32), ("max_dx", C.c_float),
+        ("target_toi", C.c_float), ("min_newton_steps", C.c_uint32), ("ccd_max_iter", C.c_uint32),
+        ("cg_tol", C.c_float), ("cg_max_iter", C.c_uint32), ("floor_y", C.c_float),
+    ]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [
+        ("n_verts", C.c_uint32), ("positions", C.POINTER(C.c_float)), ("velocities", C.POINTER(C.c_float)),
+        ("inv_mass", C.POINTER(C.c_float)), ("n_tris", C.c_uint32), ("tris", C.POINTER(C.c_uint32)),
+        ("static_collider", C.POINTER(C.c_uint8)), ("n_shell_faces", C.c_uint32),
+        ("shell_faces", C.POINTER(C.c_uint32)), ("n_rods", C.c_uint32), ("rod_verts", C.POINTER(C.c_uint32)),
+        ("rod_rest", C.POINTER(C.c_float)), ("rod_k", C.POINTER(C.c_float)), ("n_rod_hinges", C.c_uint32),
+        ("rod_hinge_verts", C.POINTER(C.c_uint32)), ("rod_hinge_rest", C.POINTER(C.c_float)),
+        ("rod_hinge_stiff", C.POINTER(C.c_float)), ("n_colliders", C.c_uint32),
+        ("colliders", C.POINTER(Collider)), ("params", Params),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [("iters", C.c_uint32), ("sweeps", C.c_uint32), ("residual", C.c_double)]
+
+
+class StepMetrics(C.Structure):
+    _fields_ = [
+        ("time", C.c_double), ("max_reresid", C.c_double), ("max_cg_iters", C.c_uint32),
+        ("newton_iters", C.c_uint32), ("num_stencils", C.c_uint32), ("max_velocity", C.c_float),
+        ("penetration_free", C.c_int32), ("all_finite", C.c_int32), ("last_toi", C.c_float),
+    ]
+
+
+class NguoError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"nguo error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load():
+    """Loads the in-tree libnguo_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    lib = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    vp = C.c_void_p
+    lib.nguo_last_error.restype = C.c_char_p
+    lib.nguo_ctx_create.argtypes = [C.c_int, P(vp)]
+    lib.nguo_ctx_destroy.argtypes = [vp]
+    lib.nguo_ctx_destroy.restype = None
+    lib.nguo_ctx_info.argtypes = [vp, C.c_char_p, C.c_size_t, P(C.c_int)]
+    lib.nguo_ls_create.argtypes = [vp, C.c_int, C.c_uint32, C.c_float, P(vp)]
+    lib.nguo_ls_destroy.argtypes = [vp]
+    lib.nguo_ls_destroy.restype = None
+    lib.nguo_ls_solve.argtypes = [vp, C.c_uint32, P(C.c_uint32), P(C.c_uint32), P(C.c_float), P(C.c_float),
+                                  C.c_float, C.c_uint32, P(C.c_float), P(Stats)]
+    lib.nguo_ls_last_stats.argtypes = [vp, P(Stats)]
+    lib.nguo_ls_spmv.argtypes = [vp, C.c_uint32, P(C.c_uint32), P(C.c_uint32), P(C.c_float), P(C.c_float), P(C.c_float)]
+    lib.nguo_scene_generate.argtypes = [C.c_int, C.c_uint64, C.c_double, P(vp)]
+    lib.nguo_scene_view.argtypes = [vp, P(SceneDesc)]
+    lib.nguo_scene_free.argtypes = [vp]
+    lib.nguo_scene_free.restype = None
+    if hasattr(lib, "nguo_step_create"):
+        lib.nguo_step_create.argtypes = [vp, P(SceneDesc), P(vp)]
+        lib.nguo_step_destroy.argtypes = [vp]
+        lib.nguo_step_destroy.restype = None
+        lib.nguo_step_advance.argtypes = [vp, C.c_uint32]
+        lib.nguo_step_get_positions.argtypes = [vp, P(C.c_float)]
+        lib.nguo_step_get_velocities.argtypes = [vp, P(C.c_float)]
+        lib.nguo_step_set_state.argtypes = [vp, P(C.c_float), P(C.c_float)]
+        lib.nguo_step_get_metrics.argtypes = [vp, P(StepMetrics)]
+    _lib = lib
+    return lib
+
+
+def check(code):
+    if code != NGUO_OK:
+        raise NguoError(code, load().nguo_last_error().decode())
+
+
+def fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def uptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+class Scene:
+    """A generated workload: owns the native buffer and exposes numpy views of it."""
+
+    def __init__(self, config, seed=0, param=1.0):
+        lib = load()
+        self._buf = C.c_void_p()
+        check(lib.nguo_scene_generate(int(config), int(seed), float(param), C.byref(self._buf)))
+        self.desc = SceneDesc()
+        check(lib.nguo_scene_view(self._buf, C.byref(self.desc)))
+        d = self.desc
+        n = d.n_verts
+        self.n_verts = n
+        self.positions = np.ctypeslib.as_array(d.positions, shape=(n, 3))
+        self.inv_mass = np.ctypeslib.as_array(d.inv_mass, shape=(n,))
+        self.tris = np.ctypeslib.as_array(d.tris, shape=(d.n_tris, 3))
+        self.static = np.ctypeslib.as_array(d.static_collider, shape=(n,))
+        self.params = d.params
+
+    def __del__(self):
+        if getattr(self, "_buf", None) and _lib is not None:
+            _lib.nguo_scene_free(self._buf)
+            self._buf = None
+
+
+class Context:
+    def __init__(self, device=0):
+        lib = load()
+        self.h = C.c_void_p()
+        check(lib.nguo_ctx_create(int(device), C.byref(self.h)))
+
+    def info(self):
+        name = C.create_string_buffer(128)
+        cu = C.c_int()
+        check(load().nguo_ctx_info(self.h, name, 128, C.byref(cu)))
+        return name.value.decode(), cu.value
+
+    def close(self):
+        if self.h:
+            load().nguo_ctx_destroy(self.h)
+            self.h = None
+
+
+class LinearSolver:
+    """The LinearSolver seam (engine-core/src/linsolve.rs:21-28) on the device."""
+
+    def __init__(self, ctx, precond=0):
+        self.h = C.c_void_p()
+        check(load().nguo_ls_create(ctx.h, int(precond), 0, 0.0, C.byref(self.h)))
+
+    def solve(self, row_ptr, col_idx, values, b, tol, max_iter):
+        row_ptr = np.ascontiguousarray(row_ptr, dtype=np.uint32)
+        col_idx = np.ascontiguousarray(col_idx, dtype=np.uint32)
+        values = np.ascontiguousarray(values, dtype=np.float32)
+        b = np.ascontiguousarray(b, dtype=np.float32)
+        x = np.zeros_like(b)
+        st = Stats()
+        check(load().nguo_ls_solve(self.h, len(b), uptr(row_ptr), uptr(col_idx), fptr(values), fptr(b),
+                                   float(tol), int(max_iter), fptr(x), C.byref(st)))
+        return x, st
+
+    def spmv(self, row_ptr, col_idx, values, x):
+        row_ptr = np.ascontiguousarray(row_ptr, dtype=np.uint32)
+        col_idx = np.ascontiguousarray(col_idx, dtype=np.uint32)
+        values = np.ascontiguousarray(values, dtype=np.float32)
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros_like(x)
+        check(load().nguo_ls_spmv(self.h, len(x), uptr(row_ptr), uptr(col_idx), fptr(values), fptr(x), fptr(y)))
+        return y
+
+    def close(self):
+        if self.h:
+            load().nguo_ls_destroy(self.h)
+            self.h = None
+
+
+class Stepper:
+    """The whole implicit step (GpuImplicitSolver / ImplicitSolver::step) on the device."""
+
+    def __init__(self, ctx, scene):
+        self.h = C.c_void_p()
+        self.n = scene.n_verts
+        check(load().nguo_step_create(ctx.h, C.byref(scene.desc), C.byref(self.h)))
+
+    def advance(self, nsteps=1):
+        check(load().nguo_step_advance(self.h, int(nsteps)))
+
+    def positions(self):
+        out = np.zeros((self.n, 3), dtype=np.float32)
+        check(load().nguo_step_get_positions(self.h, fptr(out)))
+        return out
+
+    def velocities(self):
+        out = np.zeros((self.n, 3), dtype=np.float32)
+        check(load().nguo_step_get_velocities(self.h, fptr(out)))
+        return out
+
+    def set_state(self, x, v):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        v = np.ascontiguousarray(v, dtype=np.float32)
+        check(load().nguo_step_set_state(self.h, fptr(x), fptr(v)))
+
+    def metrics(self):
+        m = StepMetrics()
+        check(load().nguo_step_get_metrics(self.h, C.byref(m)))
+        return m
+
+    def close(self):
+        if self.h:
+            load().nguo_step_destroy(self.h)
+            self.h = None

@@ -1,0 +1,234 @@
+"""ctypes loader of oracle/liboracle.so, the CPU restatement of the reference (test-only).
+
+Only the tests, the smoke check and the benchmark's CPU-baseline leg use this module: it is
+the checker for the HIP path, never part of it.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "oracle", "liboracle.so")
+_lib = None
+
+D = C.POINTER(C.c_double)
+F = C.POINTER(C.c_float)
+U = C.POINTER(C.c_uint32)
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise FileNotFoundError(f"{LIB} missing: run `make -C oracle`")
+        lib = C.CDLL(LIB)
+        vp = C.c_void_p
+        lib.oracle_solver_create.restype = vp
+        lib.oracle_solver_create.argtypes = [vp, C.c_int]
+        lib.oracle_solver_destroy.argtypes = [vp]
+        lib.oracle_solver_nverts.restype = C.c_uint32
+        lib.oracle_solver_nverts.argtypes = [vp]
+        lib.oracle_solver_step.argtypes = [vp, C.c_uint32]
+        lib.oracle_solver_positions.argtypes = [vp, F]
+        lib.oracle_solver_positions_d.argtypes = [vp, D]
+        lib.oracle_solver_velocities.argtypes = [vp, F]
+        lib.oracle_solver_set_state.argtypes = [vp, F, F]
+        lib.oracle_solver_metrics.argtypes = [vp, D]
+        lib.oracle_solver_newton_trace.restype = C.c_int
+        lib.oracle_solver_newton_trace.argtypes = [vp, C.c_uint32, F, D]
+        lib.oracle_solver_stencils.restype = C.c_int64
+        lib.oracle_solver_stencils.argtypes = [vp, F, C.c_uint32, U, F, F]
+        lib.oracle_solver_collider_sides.restype = C.c_int
+        lib.oracle_solver_collider_sides.argtypes = [vp, F, F]
+        lib.oracle_solver_assemble.restype = C.c_int64
+        lib.oracle_solver_assemble.argtypes = [vp, F, F, C.c_double, F, C.c_uint32]
+        lib.oracle_solver_csr.argtypes = [vp, U, U, F, F]
+        lib.oracle_solver_gradient.argtypes = [vp, F, F, C.c_double, D]
+        lib.oracle_solver_ccd.restype = C.c_double
+        lib.oracle_solver_ccd.argtypes = [vp, F, F]
+        lib.oracle_solver_self_intersects.restype = C.c_int
+        lib.oracle_solver_self_intersects.argtypes = [vp, F]
+        lib.oracle_solver_newton_iteration.restype = C.c_double
+        lib.oracle_solver_newton_iteration.argtypes = [vp, F, F, D, U]
+        lib.oracle_solver_x_tilde.argtypes = [vp, F]
+        lib.oracle_cg.argtypes = [C.c_uint32, U, U, F, F, C.c_float, C.c_uint32, F, U, D]
+        lib.oracle_spmv.argtypes = [C.c_uint32, U, U, F, F, F]
+        lib.oracle_point_tri_distance_grad.restype = C.c_double
+        lib.oracle_point_tri_distance_grad.argtypes = [C.c_int, D, D]
+        lib.oracle_segment_distance_grad.restype = C.c_double
+        lib.oracle_segment_distance_grad.argtypes = [C.c_int, D, D]
+        lib.oracle_segment_distance.restype = C.c_double
+        lib.oracle_segment_distance.argtypes = [C.c_int, D]
+        lib.oracle_point_tri_distance.restype = C.c_double
+        lib.oracle_point_tri_distance.argtypes = [C.c_int, D]
+        lib.oracle_edge_triangle_intersect.restype = C.c_int
+        lib.oracle_edge_triangle_intersect.argtypes = [C.c_int, D]
+        lib.oracle_accd_toi.restype = C.c_double
+        lib.oracle_accd_toi.argtypes = [C.c_int, C.c_int, D, D, C.c_double, C.c_double, C.c_double, C.c_uint32]
+        lib.oracle_barrier.restype = C.c_int
+        lib.oracle_barrier.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, D]
+        lib.oracle_rayleigh_stiffness.restype = C.c_double
+        lib.oracle_rayleigh_stiffness.argtypes = [C.c_int, D, D, D, C.c_int, D, C.c_double]
+        lib.oracle_friction.restype = C.c_int
+        lib.oracle_friction.argtypes = [C.c_int, D, D, D, C.c_double, C.c_double, D, D]
+        lib.oracle_lame.argtypes = [C.c_double, C.c_double, D, D]
+        lib.oracle_shell_element.restype = C.c_double
+        lib.oracle_shell_element.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, D, D, D, D]
+        lib.oracle_dihedral.restype = C.c_double
+        lib.oracle_dihedral.argtypes = [C.c_int, D, D]
+        lib.oracle_strand.restype = C.c_double
+        lib.oracle_strand.argtypes = [C.c_int, D, C.c_double, D, D]
+        lib.oracle_rod_stretch.argtypes = [C.c_int, D, C.c_double, C.c_double, D, D]
+        lib.oracle_morton_codes.argtypes = [C.c_uint32, F, C.c_uint32, U, U]
+        lib.oracle_sort_pairs.argtypes = [C.c_uint32, U, U]
+        lib.oracle_karras.argtypes = [C.c_uint32, U, U, U]
+        lib.oracle_propagate_aabb.argtypes = [C.c_uint32, U, U, F, F]
+        _lib = lib
+    return _lib
+
+
+def dp(a):
+    return a.ctypes.data_as(D)
+
+
+def fp(a):
+    return a.ctypes.data_as(F)
+
+
+def up(a):
+    return a.ctypes.data_as(U)
+
+
+def d64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+class OracleSolver:
+    """The oracle ImplicitSolver on a generated scene (fp64=True gives the FP64 twin)."""
+
+    def __init__(self, scene, fp64=False):
+        self.lib = load()
+        self.scene = scene  # keeps the arrays alive
+        self.h = self.lib.oracle_solver_create(C.byref(scene.desc), 1 if fp64 else 0)
+        self.n = self.lib.oracle_solver_nverts(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.oracle_solver_destroy(self.h)
+            self.h = None
+
+    def step(self, k=1):
+        self.lib.oracle_solver_step(self.h, k)
+
+    def positions(self):
+        out = np.zeros((self.n, 3), np.float32)
+        self.lib.oracle_solver_positions(self.h, fp(out))
+        return out
+
+    def positions_d(self):
+        out = np.zeros((self.n, 3), np.float64)
+        self.lib.oracle_solver_positions_d(self.h, dp(out))
+        return out
+
+    def velocities(self):
+        out = np.zeros((self.n, 3), np.float32)
+        self.lib.oracle_solver_velocities(self.h, fp(out))
+        return out
+
+    def set_state(self, x, v):
+        x, v = f32(x), f32(v)
+        self.lib.oracle_solver_set_state(self.h, fp(x), fp(v))
+
+    def metrics(self):
+        out = np.zeros(8)
+        self.lib.oracle_solver_metrics(self.h, dp(out))
+        keys = ["time", "max_reresid", "max_cg_iters", "newton_iters", "last_toi", "penetration_free", "all_finite", "n_edges"]
+        return dict(zip(keys, out))
+
+    def newton_trace(self):
+        xs, tois = [], []
+        k = 0
+        while True:
+            out = np.zeros((self.n, 3), np.float32)
+            t = C.c_double()
+            r = self.lib.oracle_solver_newton_trace(self.h, k, fp(out), C.byref(t))
+            if r < 0:
+                break
+            xs.append(out)
+            tois.append(t.value)
+            k += 1
+        return xs, tois
+
+    def stencils(self, x):
+        x = f32(x)
+        cnt = self.lib.oracle_solver_stencils(self.h, fp(x), 0, None, None, None)
+        idx = np.zeros((max(cnt, 1), 4), np.uint32)
+        d = np.zeros(max(cnt, 1), np.float32)
+        g = np.zeros((max(cnt, 1), 12), np.float32)
+        self.lib.oracle_solver_stencils(self.h, fp(x), cnt, up(idx), fp(d), fp(g))
+        return idx[:cnt], d[:cnt], g[:cnt]
+
+    def collider_sides(self, x):
+        x = f32(x)
+        side = np.ones(self.n, np.float32)
+        ok = self.lib.oracle_solver_collider_sides(self.h, fp(x), fp(side))
+        return side if ok else None
+
+    def assemble(self, x, xt, inv_h2):
+        x, xt = f32(x), f32(xt)
+        nnz = self.lib.oracle_solver_assemble(self.h, fp(x), fp(xt), inv_h2, None, 0)
+        n3 = 3 * self.n
+        rp = np.zeros(n3 + 1, np.uint32)
+        ci = np.zeros(nnz, np.uint32)
+        va = np.zeros(nnz, np.float32)
+        rhs = np.zeros(n3, np.float32)
+        self.lib.oracle_solver_csr(self.h, up(rp), up(ci), fp(va), fp(rhs))
+        return rp, ci, va, rhs
+
+    def x_tilde(self):
+        out = np.zeros((self.n, 3), np.float32)
+        self.lib.oracle_solver_x_tilde(self.h, fp(out))
+        return out
+
+    def ccd(self, x, dx):
+        x, dx = f32(x), f32(dx)
+        return self.lib.oracle_solver_ccd(self.h, fp(x), fp(dx))
+
+    def self_intersects(self, x):
+        x = f32(x)
+        return bool(self.lib.oracle_solver_self_intersects(self.h, fp(x)))
+
+    def newton_iteration(self, x, xt):
+        x = f32(x).copy()
+        xt = f32(xt)
+        res = C.c_double()
+        it = C.c_uint32()
+        t = self.lib.oracle_solver_newton_iteration(self.h, fp(x), fp(xt), C.byref(res), C.byref(it))
+        return x, t, res.value, it.value
+
+
+def cg(row_ptr, col_idx, values, b, tol, max_iter):
+    lib = load()
+    row_ptr, col_idx, values, b = u32(row_ptr), u32(col_idx), f32(values), f32(b)
+    x = np.zeros_like(b)
+    it = C.c_uint32()
+    res = C.c_double()
+    lib.oracle_cg(len(b), up(row_ptr), up(col_idx), fp(values), fp(b), tol, max_iter, fp(x), C.byref(it), C.byref(res))
+    return x, it.value, res.value
+
+
+def spmv(row_ptr, col_idx, values, x):
+    lib = load()
+    row_ptr, col_idx, values, x = u32(row_ptr), u32(col_idx), f32(values), f32(x)
+    y = np.zeros_like(x)
+    lib.oracle_spmv(len(x), up(row_ptr), up(col_idx), fp(values), fp(x), fp(y))
+    return y

@@ -1,0 +1,83 @@
+"""LinearSolver seam on the device (nguo_ls_*) against the oracle's f64 block-Jacobi CG
+(engine-core/src/linsolve.rs:350-411) and f32 CSR SpMV (csr.rs:98-110)."""
+import numpy as np
+import pytest
+
+from conftest import nguo_hip
+import oracle_py as O
+
+
+def laplacian_1d(n):
+    # reference manufactured system: tridiagonal 2/-1 (linsolve.rs:855-867), in 3 dof blocks
+    rp, ci, va = [0], [], []
+    for i in range(n):
+        for j, v in ((i - 1, -1.0), (i, 2.0), (i + 1, -1.0)):
+            if 0 <= j < n:
+                ci.append(j)
+                va.append(v)
+        rp.append(len(ci))
+    return np.array(rp, np.uint32), np.array(ci, np.uint32), np.array(va, np.float32)
+
+
+def newton_system(cfg, steps_before=0):
+    s = nguo_hip.Scene(cfg, seed=cfg)
+    o = O.OracleSolver(s)
+    if steps_before:
+        o.step(steps_before)
+    x = o.positions()
+    xt = o.x_tilde()
+    dt = s.params.dt
+    return o.assemble(x, xt, 1.0 / (dt * dt))
+
+
+@pytest.mark.gpu
+def test_spmv_matches_oracle(ctx):
+    rp, ci, va, rhs = newton_system(0)
+    ls = nguo_hip.LinearSolver(ctx)
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(len(rhs)).astype(np.float32)
+    y = ls.spmv(rp, ci, va, x)
+    y_ref = O.spmv(rp, ci, va, x)
+    y64 = np.zeros(len(rhs))
+    for r in range(len(rhs)):
+        y64[r] = np.dot(va[rp[r]:rp[r + 1]].astype(np.float64), x[ci[rp[r]:rp[r + 1]]].astype(np.float64))
+    scale = np.abs(va).max() * np.abs(x).max() * 16
+    assert np.abs(y - y64).max() <= 1e-6 * scale
+    assert np.abs(y - y_ref).max() <= 1e-6 * scale
+    ls.close()
+
+
+@pytest.mark.gpu
+def test_pcg_manufactured_laplacian(ctx):
+    n = 3 * 200
+    rp, ci, va = laplacian_1d(n)
+    xs = np.linspace(-1, 1, n).astype(np.float32)
+    b = O.spmv(rp, ci, va, xs)
+    ls = nguo_hip.LinearSolver(ctx)
+    x, st = ls.solve(rp, ci, va, b, 1e-6, 100000)
+    assert st.residual <= 1e-6
+    assert np.abs(x - xs).max() < 1e-2
+    x0, st0 = ls.solve(rp, ci, va, np.zeros(n, np.float32), 1e-6, 1000)
+    assert np.all(x0 == 0)  # zero rhs -> zero solution (engine-gpu/src/pcg.rs:838-845)
+    ls.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cfg,steps", [(0, 0), (0, 14)])
+def test_pcg_matches_oracle_cg(ctx, cfg, steps):
+    rp, ci, va, rhs = newton_system(cfg, steps)
+    x_ref, it_ref, res_ref = O.cg(rp, ci, va, rhs, 1e-6, 10000)
+    ls = nguo_hip.LinearSolver(ctx)
+    x, st = ls.solve(rp, ci, va, rhs, 1e-6, 10000)
+    # FP64 residual of the device solution, evaluated on the host
+    n = len(rhs)
+    r = rhs.astype(np.float64).copy()
+    for row in range(n):
+        sl = slice(rp[row], rp[row + 1])
+        r[row] -= np.dot(va[sl].astype(np.float64), x[ci[sl]].astype(np.float64))
+    rel = np.linalg.norm(r) / np.linalg.norm(rhs.astype(np.float64))
+    assert rel <= 2e-6, (rel, st.residual)
+    assert st.residual <= 1e-6
+    scale = np.abs(x_ref).max()
+    assert np.abs(x - x_ref).max() <= 1e-3 * scale, (np.abs(x - x_ref).max(), scale)
+    ls.close()
