@@ -159,6 +159,33 @@ int nguo_step_get_positions(nguo_step *st, float *out /* [3*n_verts] */);
 int nguo_step_get_velocities(nguo_step *st, float *out /* [3*n_verts] */);
 int nguo_step_set_state(nguo_step *st, const float *positions, const float *velocities);
 int nguo_step_get_metrics(nguo_step *st, nguo_step_metrics *out);
+/* Positions after Newton iteration k of the last advanced step (the oracle's per-iteration
+ * parity point); *count receives the number of iterations taken, toi their CCD fraction.
+ * out == NULL only queries the count. */
+int nguo_step_get_newton_trace(nguo_step *st, uint32_t k, float *out, uint32_t *count, float *toi);
+/* The active contact stencils at positions x, as implicit.rs:678-753
+ * self_collision_stencils returns them (vertex-triangle sorted by (v, t), then edge-edge
+ * sorted by (ei, ej)): idx [4 cap], gap d [cap], distance gradient g [12 cap] and the
+ * Rayleigh stiffness kappa [cap] (implicit.rs:755-795) against the elastic Hessian at x.
+ * *count receives the number of stencils; at most cap are written. State is unchanged. */
+int nguo_step_stencils(nguo_step *st, const float *x, uint32_t cap, uint32_t *idx, float *d, float *g, float *kappa, uint32_t *count);
+/* implicit.rs:1431-1476 collider_sides at positions x: -1 inside the static body, +1 else. */
+int nguo_step_collider_sides(nguo_step *st, const float *x, float *side);
+
+/* ---- measurement (no reference counterpart: the reference has no GPU timestamps,
+ * core-engine/docs/WGPU_PIN.md:142-144). Average duration of the PCG operator y = (A+C) x on
+ * the current Newton system, timed with HIP events on the library stream; *bytes receives
+ * the algorithmic bytes of one 3x3-BSR SpMV (nnzb*40 + nb*28 + 4). */
+int nguo_step_bench_spmv(nguo_step *st, uint32_t reps, double *avg_ms, double *bytes);
+/* The same operator on a synthetic nx x ny cloth-grid BSR (13 blocks per interior row). */
+int nguo_bench_spmv_grid(nguo_ctx *ctx, uint32_t nx, uint32_t ny, uint32_t reps, double *avg_ms, double *bytes, double *nnzb);
+
+/* ---- LBVH stages on a triangle set (bit-exact checks; engine-gpu/src/morton.rs:61-230,
+ * radix.rs:112-135, karras.rs:91-262, bvh_aabb.rs). Host buffers in and out. */
+int nguo_lbvh_morton(nguo_ctx *ctx, uint32_t nv, const float *pos, uint32_t nt, const uint32_t *tris, uint32_t *codes);
+int nguo_lbvh_sort_pairs(nguo_ctx *ctx, uint32_t n, uint32_t *keys, uint32_t *vals);
+int nguo_lbvh_karras(nguo_ctx *ctx, uint32_t n, const uint32_t *codes, uint32_t *left, uint32_t *right);
+int nguo_lbvh_aabb(nguo_ctx *ctx, uint32_t n, const uint32_t *left, const uint32_t *right, const float *leaf_box, float *node_box);
 
 /* ---- scene generators for the benchmark workloads (host only, no device needed).
  * config: 0 sheet on icosphere, 1 tube garment on a capsule body with seam rods,

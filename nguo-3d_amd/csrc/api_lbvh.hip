@@ -1,0 +1,101 @@
+// C ABI of the LBVH stages on a triangle set, so each stage can be checked bit for bit:
+// engine-gpu/src/morton.rs:61-230 (GpuMorton::codes), radix.rs:112-135 (sort_pairs),
+// karras.rs:91-262 (GpuKarras::build_tree), bvh_aabb.rs (node AABB propagation).
+#include "lbvh.h"
+
+using namespace nguo;
+
+extern "C" {
+
+int nguo_lbvh_morton(nguo_ctx *ctx, uint32_t nv, const float *pos, uint32_t nt, const uint32_t *tris, uint32_t *codes) {
+  return guarded([&]() -> int {
+    NGUO_REQUIRE(ctx && pos && tris && codes, NGUO_E_INVALID, "nguo_lbvh_morton: null argument");
+    for (size_t i = 0; i < 3 * (size_t)nt; ++i) NGUO_REQUIRE(tris[i] < nv, NGUO_E_INVALID, "triangle index out of range");
+    if (nt == 0) return NGUO_OK;
+    hipStream_t s = ctx->stream;
+    DevBuf<float> p, b, c;
+    DevBuf<uint32_t> t, k;
+    p.upload(pos, 3 * (size_t)nv, s);
+    t.upload(tris, 3 * (size_t)nt, s);
+    b.reserve(6 * (size_t)div_up(nt, 256) + 6);
+    c.reserve(3 * (size_t)nt);
+    k.reserve(nt);
+    morton_codes(0, nt, nullptr, t.p, p.p, b.p, c.p, k.p, s);
+    k.download(codes, nt, s);
+    NGUO_HIP(hipStreamSynchronize(s));
+    return NGUO_OK;
+  });
+}
+
+int nguo_lbvh_sort_pairs(nguo_ctx *ctx, uint32_t n, uint32_t *keys, uint32_t *vals) {
+  return guarded([&]() -> int {
+    NGUO_REQUIRE(ctx && keys && vals, NGUO_E_INVALID, "nguo_lbvh_sort_pairs: null argument");
+    if (n == 0) return NGUO_OK;
+    hipStream_t s = ctx->stream;
+    DevBuf<uint32_t> k, v;
+    k.upload(keys, n, s);
+    v.upload(vals, n, s);
+    RadixSort rs;
+    rs.sort_u32(n, k.p, v.p, {0, 8, 16, 24}, s);
+    k.download(keys, n, s);
+    v.download(vals, n, s);
+    NGUO_HIP(hipStreamSynchronize(s));
+    return NGUO_OK;
+  });
+}
+
+int nguo_lbvh_karras(nguo_ctx *ctx, uint32_t n, const uint32_t *codes, uint32_t *left, uint32_t *right) {
+  return guarded([&]() -> int {
+    NGUO_REQUIRE(ctx && codes && left && right, NGUO_E_INVALID, "nguo_lbvh_karras: null argument");
+    if (n < 2) return NGUO_OK;
+    hipStream_t s = ctx->stream;
+    DevBuf<uint32_t> c, l, r;
+    c.upload(codes, n, s);
+    l.reserve(n);
+    r.reserve(n);
+    karras_build(n, c.p, l.p, r.p, s);
+    l.download(left, n - 1, s);
+    r.download(right, n - 1, s);
+    NGUO_HIP(hipStreamSynchronize(s));
+    return NGUO_OK;
+  });
+}
+
+int nguo_lbvh_aabb(nguo_ctx *ctx, uint32_t n, const uint32_t *left, const uint32_t *right, const float *leaf_box, float *node_box) {
+  return guarded([&]() -> int {
+    NGUO_REQUIRE(ctx && left && right && leaf_box && node_box, NGUO_E_INVALID, "nguo_lbvh_aabb: null argument");
+    if (n < 2) return NGUO_OK;
+    for (uint32_t i = 0; i + 1 < n; ++i) {
+      uint32_t a = left[i], b = right[i];
+      NGUO_REQUIRE(((a & LEAF) ? (a & ~LEAF) < n : a < n - 1) && ((b & LEAF) ? (b & ~LEAF) < n : b < n - 1), NGUO_E_INVALID,
+                   "nguo_lbvh_aabb: child index out of range");
+    }
+    hipStream_t s = ctx->stream;
+    // parents from the child lists (host), then the device climb
+    std::vector<uint32_t> parent(2 * (size_t)n - 1, 0xffffffffu);
+    for (uint32_t i = 0; i + 1 < n; ++i)
+      for (uint32_t c : {left[i], right[i]}) parent[(c & LEAF) ? (n - 1 + (c & ~LEAF)) : c] = i;
+    for (size_t i = 1; i < parent.size(); ++i) {
+      NGUO_REQUIRE(parent[i] != 0xffffffffu, NGUO_E_INVALID, "nguo_lbvh_aabb: not a tree");
+      // every node must reach the root within n steps (no cycles)
+      size_t node = i;
+      uint32_t hops = 0;
+      while (node != 0 && hops <= n) { node = parent[node]; ++hops; }
+      NGUO_REQUIRE(node == 0, NGUO_E_INVALID, "nguo_lbvh_aabb: child lists contain a cycle");
+    }
+    DevBuf<uint32_t> l, r, p, v;
+    DevBuf<float> lb, nb;
+    l.upload(left, n - 1, s);
+    r.upload(right, n - 1, s);
+    p.upload(parent, s);
+    v.reserve(n);
+    lb.upload(leaf_box, 6 * (size_t)n, s);
+    nb.reserve(6 * (size_t)n);
+    aabb_build(n, l.p, r.p, p.p, v.p, lb.p, nb.p, s);
+    nb.download(node_box, 6 * (size_t)(n - 1), s);
+    NGUO_HIP(hipStreamSynchronize(s));
+    return NGUO_OK;
+  });
+}
+
+}  // extern "C"

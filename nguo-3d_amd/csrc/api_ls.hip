@@ -20,12 +20,6 @@ int fail(int code, const std::string &msg) {
 void clear_error() { g_last_error.clear(); }
 }  // namespace nguo
 
-struct nguo_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  hipDeviceProp_t prop{};
-};
-
 struct nguo_ls {
   nguo_ctx *ctx = nullptr;
   int precond = NGUO_PRECOND_BLOCK;
@@ -180,6 +174,61 @@ int nguo_ls_spmv(nguo_ls *ls, uint32_t n, const uint32_t *row_ptr, const uint32_
     ls->pcg->apply(ls->mat.view(), no_contact(), ls->b.p, ls->y.p);
     ls->y.download(y, n, s);
     NGUO_HIP(hipStreamSynchronize(s));
+    return NGUO_OK;
+  });
+}
+
+// A synthetic cloth BSR larger than the last-level cache: an nx x ny triangulated grid whose
+// block rows couple every vertex to its 1-ring plus the hinge-opposite vertices (13 blocks
+// for an interior row, the count the cloth Hessians have), SPD values. Times `reps` SpMV
+// launches between two HIP events on the context stream.
+int nguo_bench_spmv_grid(nguo_ctx *ctx, uint32_t nx, uint32_t ny, uint32_t reps, double *avg_ms, double *bytes, double *nnzb_out) {
+  return guarded([&]() -> int {
+    NGUO_REQUIRE(ctx && avg_ms && bytes && nx >= 2 && ny >= 2 && reps > 0, NGUO_E_INVALID, "nguo_bench_spmv_grid: bad argument");
+    hipStream_t s = ctx->stream;
+    uint32_t nb = nx * ny;
+    std::vector<std::vector<uint32_t>> cols(nb);
+    const int off[13][2] = {{0, 0}, {1, 0}, {-1, 0}, {0, 1}, {0, -1}, {1, -1}, {-1, 1}, {1, 1}, {-1, -1}, {2, -1}, {-2, 1}, {1, -2}, {-1, 2}};
+    for (uint32_t j = 0; j < ny; ++j)
+      for (uint32_t i = 0; i < nx; ++i) {
+        auto &c = cols[j * nx + i];
+        for (auto &o : off) {
+          int a = (int)i + o[0], b = (int)j + o[1];
+          if (a >= 0 && b >= 0 && a < (int)nx && b < (int)ny) c.push_back((uint32_t)(b * (int)nx + a));
+        }
+        std::sort(c.begin(), c.end());
+      }
+    SellMatrix m;
+    m.build_pattern(nb, cols, s);
+    std::vector<float> val((size_t)m.nslots * 9, 0.0f);
+    for (uint32_t r = 0; r < nb; ++r)
+      for (uint32_t k = 0; k < m.h_row_len[r]; ++k) {
+        uint32_t sl = m.slot(r, k);
+        bool diag = m.h_col[sl] == r;
+        for (int e = 0; e < 9; ++e) val[sell_val_index(sl, e)] = diag ? (e % 4 == 0 ? 20.0f : 0.1f) : (e % 4 == 0 ? -1.0f : 0.05f);
+      }
+    m.val.upload(val, s);
+    DevBuf<float> x, y;
+    std::vector<float> hx(3 * (size_t)nb, 1.0f);
+    x.upload(hx, s);
+    y.reserve(3 * (size_t)nb);
+    Pcg pcg(s);
+    ContactView none{};
+    pcg.apply(m.view(), none, x.p, y.p);
+    hipEvent_t e0, e1;
+    NGUO_HIP(hipEventCreate(&e0));
+    NGUO_HIP(hipEventCreate(&e1));
+    NGUO_HIP(hipEventRecord(e0, s));
+    for (uint32_t r = 0; r < reps; ++r) pcg.apply(m.view(), none, x.p, y.p);
+    NGUO_HIP(hipEventRecord(e1, s));
+    NGUO_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    NGUO_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_ms = (double)ms / reps;
+    *bytes = m.spmv_bytes();
+    if (nnzb_out) *nnzb_out = (double)m.nnzb;
     return NGUO_OK;
   });
 }

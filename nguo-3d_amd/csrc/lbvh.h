@@ -1,0 +1,100 @@
+// LBVH over triangles or edges: 30-bit Morton keys of primitive centroids normalised by the
+// centroid bounds, a stable LSD radix sort (8-bit digits, wavefront multisplit ranking in
+// LDS), Karras (2012) internal nodes and a single-pass bottom-up AABB union.
+//
+// Reference stages (engine-gpu/src/): morton.rs:12-58 + morton.wgsl, radix.rs:102-135 +
+// radix.wgsl (serial per-block scatter, n <= 65 536), scan.rs, karras.rs:24-88 + karras.wgsl,
+// bvh_aabb.rs:28-55 + bvh_aabb.wgsl (host-driven level passes), bvh_traverse.rs:77-260.
+#pragma once
+#include <vector>
+
+#include "devmath.h"
+#include "nguo_internal.h"
+
+namespace nguo {
+
+constexpr uint32_t LEAF = 0x80000000u;
+
+// Stable ascending sort of (key, value) pairs on the device.
+class RadixSort {
+ public:
+  // keys/vals are sorted in place (ping-pong through internal buffers). `shifts` lists the
+  // digit positions to sort by (least significant first), e.g. {0, 8, 16, 24} for u32 keys.
+  void sort_u32(uint32_t n, uint32_t *keys, uint32_t *vals, const std::vector<int> &shifts, hipStream_t s);
+  void sort_u64(uint32_t n, uint64_t *keys, uint32_t *vals, const std::vector<int> &shifts, hipStream_t s);
+
+ private:
+  DevBuf<uint32_t> k32_, v_, hist_;
+  DevBuf<uint64_t> k64_;
+};
+
+// Flat view of a built tree for device traversal.
+struct BvhView {
+  uint32_t n;             // leaves
+  const uint32_t *left;   // [n-1] child refs (LEAF bit marks a leaf slot)
+  const uint32_t *right;  // [n-1]
+  const float *node_box;  // [6 (n-1)] lo.xyz hi.xyz
+  const float *leaf_box;  // [6 n] in sorted order
+  const uint32_t *prim;   // [n] primitive id of sorted leaf slot
+};
+
+class Lbvh {
+ public:
+  // kind 0: triangles (3 indices per primitive), 1: edges (2 indices). `ids` selects the
+  // primitives of this tree (device array of primitive indices into `idx`).
+  void build(int kind, uint32_t n, const uint32_t *ids, const uint32_t *idx, const float *pos, hipStream_t s);
+  BvhView view() const { return BvhView{n_, left_.p, right_.p, node_box_.p, leaf_box_.p, prim_.p}; }
+  uint32_t size() const { return n_; }
+  // Stage outputs for parity checks.
+  const uint32_t *codes() const { return keys_.p; }   // sorted Morton keys
+  RadixSort sorter;
+
+ private:
+  uint32_t n_ = 0;
+  DevBuf<uint32_t> keys_, prim_, left_, right_, parent_, visit_;
+  DevBuf<float> leaf_box_, node_box_, bounds_, cent_;
+};
+
+// Kernels exposed for the stage-by-stage C ABI test entry points.
+void morton_codes(int kind, uint32_t n, const uint32_t *ids, const uint32_t *idx, const float *pos, float *bounds_scratch,
+                  float *cent_scratch, uint32_t *codes, hipStream_t s);
+void karras_build(uint32_t n, const uint32_t *codes, uint32_t *left, uint32_t *right, hipStream_t s);
+void aabb_build(uint32_t n, const uint32_t *left, const uint32_t *right, uint32_t *parent, uint32_t *visit, const float *leaf_box,
+                float *node_box, hipStream_t s);
+
+// Stack traversal: calls visit(prim) for every leaf whose box passes `overlap(box_lo,
+// box_hi)` (a conservative test). A stack deeper than TRAV_STACK sets *err (reported to the
+// caller as NGUO_E_CAPACITY) instead of silently dropping a subtree.
+constexpr int TRAV_STACK = 96;
+template <class Overlap, class Visit>
+__device__ inline void bvh_traverse(const BvhView &t, uint32_t *err, Overlap overlap, Visit visit) {
+  if (t.n == 0) return;
+  if (t.n == 1) {
+    const float *b = t.leaf_box;
+    if (overlap(f3{b[0], b[1], b[2]}, f3{b[3], b[4], b[5]})) visit(t.prim[0]);
+    return;
+  }
+  uint32_t stack[TRAV_STACK];
+  int sp = 0;
+  stack[sp++] = 0;  // root
+  while (sp > 0) {
+    uint32_t node = stack[--sp];
+    uint32_t ch[2] = {t.left[node], t.right[node]};
+    for (int c = 0; c < 2; ++c) {
+      uint32_t r = ch[c];
+      if (r & LEAF) {
+        uint32_t li = r & ~LEAF;
+        const float *b = t.leaf_box + 6 * (size_t)li;
+        if (overlap(f3{b[0], b[1], b[2]}, f3{b[3], b[4], b[5]})) visit(t.prim[li]);
+      } else {
+        const float *b = t.node_box + 6 * (size_t)r;
+        if (overlap(f3{b[0], b[1], b[2]}, f3{b[3], b[4], b[5]})) {
+          if (sp < TRAV_STACK) stack[sp++] = r;
+          else atomicOr(err, 1u);
+        }
+      }
+    }
+  }
+}
+
+}  // namespace nguo

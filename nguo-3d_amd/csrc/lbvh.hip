@@ -1,0 +1,360 @@
+// LBVH build kernels: Morton keys, stable LSD radix sort, Karras internal nodes, bottom-up
+// AABB union. See lbvh.h for the reference stages each replaces.
+#include <algorithm>
+
+#include "device_common.h"
+#include "lbvh.h"
+
+namespace nguo {
+
+constexpr uint32_t TILE = 1024;  // items per radix workgroup (4 chunks of 256)
+
+// ---------------------------------------------------------------- radix sort
+template <class K>
+__global__ __launch_bounds__(BLOCK) void k_radix_hist(uint32_t n, const K *__restrict__ keys, int shift, uint32_t *hist, uint32_t nblocks) {
+  __shared__ uint32_t cnt[256];
+  cnt[threadIdx.x] = 0;
+  __syncthreads();
+  uint32_t base = blockIdx.x * TILE;
+  for (uint32_t c = 0; c < TILE / BLOCK; ++c) {
+    uint32_t i = base + c * BLOCK + threadIdx.x;
+    if (i < n) atomicAdd(&cnt[(uint32_t)(keys[i] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// Exclusive scan of `total` counters by one workgroup (contiguous segment per thread).
+__global__ __launch_bounds__(BLOCK) void k_scan_excl(uint32_t total, uint32_t *a) {
+  __shared__ uint32_t part[BLOCK];
+  uint32_t seg = (total + BLOCK - 1) / BLOCK;
+  uint32_t lo = threadIdx.x * seg, hi = min(total, lo + seg);
+  uint32_t s = 0;
+  for (uint32_t i = lo; i < hi; ++i) s += a[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t acc = 0;
+    for (int t = 0; t < BLOCK; ++t) { uint32_t v = part[t]; part[t] = acc; acc += v; }
+  }
+  __syncthreads();
+  uint32_t acc = part[threadIdx.x];
+  for (uint32_t i = lo; i < hi; ++i) { uint32_t v = a[i]; a[i] = acc; acc += v; }
+}
+
+// Stable scatter: rank of an item among equal digits = items of that digit in earlier
+// chunks + in earlier wavefronts of this chunk + earlier lanes (8-ballot multisplit).
+template <class K>
+__global__ __launch_bounds__(BLOCK) void k_radix_scatter(uint32_t n, const K *__restrict__ kin, const uint32_t *__restrict__ vin,
+                                                        K *__restrict__ kout, uint32_t *__restrict__ vout, int shift,
+                                                        const uint32_t *__restrict__ hist, uint32_t nblocks) {
+  __shared__ uint32_t base[256];
+  __shared__ uint32_t wcnt[4][256];
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+  base[tid] = hist[(size_t)tid * nblocks + blockIdx.x];
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  for (uint32_t c = 0; c < TILE / BLOCK; ++c) {
+    for (int q = 0; q < 4; ++q) wcnt[q][tid] = 0;
+    __syncthreads();
+    uint32_t i = blockIdx.x * TILE + c * BLOCK + tid;
+    bool valid = i < n;
+    K key = valid ? kin[i] : K(0);
+    uint32_t val = valid ? vin[i] : 0u;
+    uint32_t digit = (uint32_t)(key >> shift) & 255u;
+    unsigned long long mask = __ballot(valid);
+    for (int b = 0; b < 8; ++b) {
+      bool bit = (digit >> b) & 1u;
+      unsigned long long bal = __ballot(bit);
+      mask &= bit ? bal : ~bal;
+    }
+    uint32_t rank = (uint32_t)__popcll(mask & lt);
+    if (valid && rank == 0) wcnt[w][digit] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    {
+      uint32_t s = base[tid];
+      for (int q = 0; q < 4; ++q) { uint32_t t = wcnt[q][tid]; wcnt[q][tid] = s; s += t; }
+      base[tid] = s;
+    }
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = wcnt[w][digit] + rank;
+      kout[pos] = key;
+      vout[pos] = val;
+    }
+    __syncthreads();
+  }
+}
+
+template <class K>
+static void radix_sort(uint32_t n, K *keys, uint32_t *vals, K *ktmp, uint32_t *vtmp, uint32_t *hist, const std::vector<int> &shifts,
+                       hipStream_t s) {
+  if (n <= 1) return;
+  uint32_t nblocks = div_up(n, TILE);
+  K *ka = keys, *kb = ktmp;
+  uint32_t *va = vals, *vb = vtmp;
+  for (int sh : shifts) {
+    hipLaunchKernelGGL((k_radix_hist<K>), dim3(nblocks), dim3(BLOCK), 0, s, n, (const K *)ka, sh, hist, nblocks);
+    hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(BLOCK), 0, s, 256u * nblocks, hist);
+    hipLaunchKernelGGL((k_radix_scatter<K>), dim3(nblocks), dim3(BLOCK), 0, s, n, (const K *)ka, (const uint32_t *)va, kb, vb, sh,
+                       (const uint32_t *)hist, nblocks);
+    std::swap(ka, kb);
+    std::swap(va, vb);
+  }
+  if (ka != keys) {
+    NGUO_HIP(hipMemcpyAsync(keys, ka, (size_t)n * sizeof(K), hipMemcpyDeviceToDevice, s));
+    NGUO_HIP(hipMemcpyAsync(vals, va, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+  }
+  NGUO_HIP(hipGetLastError());
+}
+
+void RadixSort::sort_u32(uint32_t n, uint32_t *keys, uint32_t *vals, const std::vector<int> &shifts, hipStream_t s) {
+  if (n <= 1) return;
+  k32_.reserve(n);
+  v_.reserve(n);
+  hist_.reserve(256 * (size_t)div_up(n, TILE));
+  radix_sort<uint32_t>(n, keys, vals, k32_.p, v_.p, hist_.p, shifts, s);
+}
+
+void RadixSort::sort_u64(uint32_t n, uint64_t *keys, uint32_t *vals, const std::vector<int> &shifts, hipStream_t s) {
+  if (n <= 1) return;
+  k64_.reserve(n);
+  v_.reserve(n);
+  hist_.reserve(256 * (size_t)div_up(n, TILE));
+  radix_sort<uint64_t>(n, keys, vals, k64_.p, v_.p, hist_.p, shifts, s);
+}
+
+// ---------------------------------------------------------------- Morton
+__device__ inline uint32_t expand_bits(uint32_t v) {
+  uint32_t x = v & 0x3ffu;
+  x = (x | (x << 16)) & 0x030000ffu;
+  x = (x | (x << 8)) & 0x0300f00fu;
+  x = (x | (x << 4)) & 0x030c30c3u;
+  x = (x | (x << 2)) & 0x09249249u;
+  return x;
+}
+// `as u32` of the reference: truncation, saturating, NaN -> 0.
+__device__ inline uint32_t f2u_sat(float v) {
+  if (!(v > 0.0f)) return 0u;
+  if (v >= 4294967296.0f) return 0xffffffffu;
+  return (uint32_t)v;
+}
+__device__ inline uint32_t morton_q(float v) { return f2u_sat(clampv(clampv(v, 0.0f, 1.0f) * 1023.0f, 0.0f, 1023.0f)); }
+
+__device__ inline f3 prim_centroid(int kind, uint32_t p, const uint32_t *idx, const float *pos) {
+  if (kind == 0) {
+    f3 a = ld3(pos, idx[3 * (size_t)p]), b = ld3(pos, idx[3 * (size_t)p + 1]), c = ld3(pos, idx[3 * (size_t)p + 2]);
+    return (a + b + c) / 3.0f;
+  }
+  f3 a = ld3(pos, idx[2 * (size_t)p]), b = ld3(pos, idx[2 * (size_t)p + 1]);
+  return (a + b) / 2.0f;
+}
+
+// Per-workgroup min/max of the centroids; bounds[6 * block] = lo.xyz, hi.xyz.
+__global__ __launch_bounds__(BLOCK) void k_centroid_bounds(int kind, uint32_t n, const uint32_t *ids, const uint32_t *idx, const float *pos,
+                                                          float *cent, float *part) {
+  __shared__ float lds[6][4];
+  uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  f3 lo{INFINITY, INFINITY, INFINITY}, hi{-INFINITY, -INFINITY, -INFINITY};
+  if (i < n) {
+    f3 c = prim_centroid(kind, ids ? ids[i] : i, idx, pos);
+    st3(cent, i, c);
+    lo = c;
+    hi = c;
+  }
+  float v[6] = {lo.x, lo.y, lo.z, hi.x, hi.y, hi.z};
+  for (int k = 0; k < 3; ++k) { v[k] = wave_min(v[k]); v[3 + k] = wave_max(v[3 + k]); }
+  int w = threadIdx.x / WAVE;
+  if ((threadIdx.x & 63) == 0)
+    for (int k = 0; k < 6; ++k) lds[k][w] = v[k];
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    int k = threadIdx.x;
+    float r = lds[k][0];
+    for (int q = 1; q < 4; ++q) r = k < 3 ? fminf(r, lds[k][q]) : fmaxf(r, lds[k][q]);
+    part[6 * (size_t)blockIdx.x + k] = r;
+  }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_bounds_final(uint32_t nparts, float *part) {
+  __shared__ float lds[6][4];
+  float v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  for (uint32_t i = threadIdx.x; i < nparts; i += BLOCK)
+    for (int k = 0; k < 6; ++k) v[k] = k < 3 ? fminf(v[k], part[6 * (size_t)i + k]) : fmaxf(v[k], part[6 * (size_t)i + k]);
+  for (int k = 0; k < 3; ++k) { v[k] = wave_min(v[k]); v[3 + k] = wave_max(v[3 + k]); }
+  int w = threadIdx.x / WAVE;
+  if ((threadIdx.x & 63) == 0)
+    for (int k = 0; k < 6; ++k) lds[k][w] = v[k];
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    int k = threadIdx.x;
+    float r = lds[k][0];
+    for (int q = 1; q < 4; ++q) r = k < 3 ? fminf(r, lds[k][q]) : fmaxf(r, lds[k][q]);
+    part[k] = r;  // final lo/hi in part[0..6)
+  }
+}
+
+// engine-gpu/src/morton.rs:22-58: code of ((c - lo) * inv), inv = 1/extent (0 if flat).
+__global__ __launch_bounds__(BLOCK) void k_morton(uint32_t n, const float *cent, const float *bounds, uint32_t *codes, uint32_t *order) {
+  uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= n) return;
+  f3 lo{bounds[0], bounds[1], bounds[2]}, hi{bounds[3], bounds[4], bounds[5]};
+  f3 ext = hi - lo;
+  f3 inv{ext.x > 0.0f ? 1.0f / ext.x : 0.0f, ext.y > 0.0f ? 1.0f / ext.y : 0.0f, ext.z > 0.0f ? 1.0f / ext.z : 0.0f};
+  f3 d = ld3(cent, i) - lo;
+  f3 nrm{d.x * inv.x, d.y * inv.y, d.z * inv.z};
+  codes[i] = expand_bits(morton_q(nrm.x)) * 4u + expand_bits(morton_q(nrm.y)) * 2u + expand_bits(morton_q(nrm.z));
+  if (order) order[i] = i;
+}
+
+void morton_codes(int kind, uint32_t n, const uint32_t *ids, const uint32_t *idx, const float *pos, float *bounds_scratch, float *cent_scratch,
+                  uint32_t *codes, hipStream_t s) {
+  if (n == 0) return;
+  unsigned g = div_up(n, BLOCK);
+  hipLaunchKernelGGL(k_centroid_bounds, dim3(g), dim3(BLOCK), 0, s, kind, n, ids, idx, pos, cent_scratch, bounds_scratch);
+  hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(BLOCK), 0, s, (uint32_t)g, bounds_scratch);
+  hipLaunchKernelGGL(k_morton, dim3(g), dim3(BLOCK), 0, s, n, (const float *)cent_scratch, (const float *)bounds_scratch, codes, (uint32_t *)nullptr);
+  NGUO_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------- Karras
+__device__ inline int kdelta(const uint32_t *codes, int n, int i, int j) {
+  if (j < 0 || j >= n) return -1;
+  uint32_t x = codes[i] ^ codes[j];
+  if (x == 0) return 32 + __clz((int)((uint32_t)i ^ (uint32_t)j));
+  return __clz((int)x);
+}
+
+// engine-gpu/src/karras.rs:24-88 (Karras 2012, section 4).
+__global__ __launch_bounds__(BLOCK) void k_karras(uint32_t n, const uint32_t *codes, uint32_t *left, uint32_t *right, uint32_t *parent) {
+  int i = (int)(blockIdx.x * BLOCK + threadIdx.x);
+  int nn = (int)n;
+  if (i >= nn - 1) return;
+  int d = (kdelta(codes, nn, i, i + 1) - kdelta(codes, nn, i, i - 1)) < 0 ? -1 : 1;
+  int dmin = kdelta(codes, nn, i, i - d);
+  int lmax = 2;
+  while (kdelta(codes, nn, i, i + lmax * d) > dmin) lmax *= 2;
+  int l = 0;
+  for (int t = lmax / 2; t >= 1; t /= 2)
+    if (kdelta(codes, nn, i, i + (l + t) * d) > dmin) l += t;
+  int j = i + l * d;
+  int first = min(i, j), last = max(i, j);
+  int nd = kdelta(codes, nn, first, last);
+  int split = first, step = last - first;
+  while (true) {
+    step = (step + 1) / 2;
+    int ns = split + step;
+    if (ns < last && kdelta(codes, nn, first, ns) > nd) split = ns;
+    if (step <= 1) break;
+  }
+  int gamma = split;
+  uint32_t lc = first == gamma ? ((uint32_t)gamma | LEAF) : (uint32_t)gamma;
+  uint32_t rc = last == gamma + 1 ? ((uint32_t)(gamma + 1) | LEAF) : (uint32_t)(gamma + 1);
+  left[i] = lc;
+  right[i] = rc;
+  if (parent) {
+    // parent[0 .. n-1) internal, parent[n-1 + leaf] leaves
+    parent[(lc & LEAF) ? (n - 1 + (lc & ~LEAF)) : lc] = (uint32_t)i;
+    parent[(rc & LEAF) ? (n - 1 + (rc & ~LEAF)) : rc] = (uint32_t)i;
+  }
+}
+
+void karras_build(uint32_t n, const uint32_t *codes, uint32_t *left, uint32_t *right, hipStream_t s) {
+  if (n < 2) return;
+  hipLaunchKernelGGL(k_karras, dim3(div_up(n - 1, BLOCK)), dim3(BLOCK), 0, s, n, codes, left, right, (uint32_t *)nullptr);
+  NGUO_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------- AABB
+__global__ __launch_bounds__(BLOCK) void k_fill_u32(uint32_t n, uint32_t *a, uint32_t v) {
+  uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+// Leaf boxes in sorted order: leaf_box[6 i] = AABB of primitive prim[i].
+__global__ __launch_bounds__(BLOCK) void k_leaf_boxes(int kind, uint32_t n, const uint32_t *prim, const uint32_t *idx, const float *pos, float *box) {
+  uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= n) return;
+  uint32_t p = prim[i];
+  f3 lo, hi;
+  if (kind == 0) {
+    f3 a = ld3(pos, idx[3 * (size_t)p]), b = ld3(pos, idx[3 * (size_t)p + 1]), c = ld3(pos, idx[3 * (size_t)p + 2]);
+    lo = vmin(vmin(a, b), c);
+    hi = vmax(vmax(a, b), c);
+  } else {
+    f3 a = ld3(pos, idx[2 * (size_t)p]), b = ld3(pos, idx[2 * (size_t)p + 1]);
+    lo = vmin(a, b);
+    hi = vmax(a, b);
+  }
+  float *o = box + 6 * (size_t)i;
+  o[0] = lo.x; o[1] = lo.y; o[2] = lo.z; o[3] = hi.x; o[4] = hi.y; o[5] = hi.z;
+}
+
+// Bottom-up union (one thread per leaf climbs; the second arrival at a node merges).
+__global__ __launch_bounds__(BLOCK) void k_aabb_climb(uint32_t n, const uint32_t *left, const uint32_t *right, const uint32_t *parent,
+                                                     uint32_t *visit, const float *leaf_box, float *node_box) {
+  uint32_t li = blockIdx.x * BLOCK + threadIdx.x;
+  if (li >= n) return;
+  uint32_t node = parent[n - 1 + li];
+  while (true) {
+    __threadfence();
+    uint32_t prev = atomicAdd(&visit[node], 1u);
+    if (prev == 0) return;  // the sibling subtree is not finished; it will continue
+    __threadfence();
+    uint32_t ch[2] = {left[node], right[node]};
+    float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (int c = 0; c < 2; ++c) {
+      const float *src = (ch[c] & LEAF) ? leaf_box + 6 * (size_t)(ch[c] & ~LEAF) : node_box + 6 * (size_t)ch[c];
+      float v[6];
+      for (int k = 0; k < 6; ++k) v[k] = src[k];
+      for (int k = 0; k < 3; ++k) { b[k] = fminf(b[k], v[k]); b[3 + k] = fmaxf(b[3 + k], v[3 + k]); }
+    }
+    float *o = node_box + 6 * (size_t)node;
+    for (int k = 0; k < 6; ++k) o[k] = b[k];
+    if (node == 0) return;
+    node = parent[node];
+  }
+}
+
+void aabb_build(uint32_t n, const uint32_t *left, const uint32_t *right, uint32_t *parent, uint32_t *visit, const float *leaf_box, float *node_box,
+                hipStream_t s) {
+  if (n < 2) return;
+  hipLaunchKernelGGL(k_fill_u32, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, s, n, visit, 0u);
+  hipLaunchKernelGGL(k_aabb_climb, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, s, n, left, right, (const uint32_t *)parent, visit, leaf_box, node_box);
+  NGUO_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------- build
+__global__ __launch_bounds__(BLOCK) void k_gather_prim(uint32_t n, const uint32_t *order, const uint32_t *ids, uint32_t *prim) {
+  uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i < n) prim[i] = ids ? ids[order[i]] : order[i];
+}
+
+void Lbvh::build(int kind, uint32_t n, const uint32_t *ids, const uint32_t *idx, const float *pos, hipStream_t s) {
+  n_ = n;
+  if (n == 0) return;
+  unsigned g = div_up(n, BLOCK);
+  keys_.reserve(n);
+  prim_.reserve(n);
+  cent_.reserve(3 * (size_t)n);
+  bounds_.reserve(6 * (size_t)g + 6);
+  leaf_box_.reserve(6 * (size_t)n);
+  left_.reserve(n);
+  right_.reserve(n);
+  parent_.reserve(2 * (size_t)n);
+  visit_.reserve(n);
+  node_box_.reserve(6 * (size_t)n);
+  DevBuf<uint32_t> &order = visit_;  // reused as the sort payload before the climb
+  hipLaunchKernelGGL(k_centroid_bounds, dim3(g), dim3(BLOCK), 0, s, kind, n, ids, idx, pos, cent_.p, bounds_.p);
+  hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(BLOCK), 0, s, (uint32_t)g, bounds_.p);
+  hipLaunchKernelGGL(k_morton, dim3(g), dim3(BLOCK), 0, s, n, (const float *)cent_.p, (const float *)bounds_.p, keys_.p, order.p);
+  sorter.sort_u32(n, keys_.p, order.p, {0, 8, 16, 24}, s);
+  hipLaunchKernelGGL(k_gather_prim, dim3(g), dim3(BLOCK), 0, s, n, (const uint32_t *)order.p, ids, prim_.p);
+  hipLaunchKernelGGL(k_leaf_boxes, dim3(g), dim3(BLOCK), 0, s, kind, n, (const uint32_t *)prim_.p, idx, pos, leaf_box_.p);
+  if (n >= 2) {
+    hipLaunchKernelGGL(k_karras, dim3(div_up(n - 1, BLOCK)), dim3(BLOCK), 0, s, n, (const uint32_t *)keys_.p, left_.p, right_.p, parent_.p);
+    aabb_build(n, left_.p, right_.p, parent_.p, visit_.p, leaf_box_.p, node_box_.p, s);
+  }
+  NGUO_HIP(hipGetLastError());
+}
+
+}  // namespace nguo

@@ -92,4 +92,11 @@ template <class T> struct DevBuf {
 inline unsigned div_up(size_t a, unsigned b) { return (unsigned)((a + b - 1) / b); }
 
 }  // namespace nguo
+
+// Device context: one HIP device and the (default-priority) stream all work is queued on.
+struct nguo_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipDeviceProp_t prop{};
+};
 #endif

@@ -1,0 +1,1719 @@
+// The resident implicit step: engine-core/src/implicit.rs:1951-2191 ImplicitSolver::step
+// (and the wgpu GpuImplicitSolver it generalises, engine-gpu/src/gpu_implicit.rs:94-659)
+// on one MI355X. Per Newton iteration, all on the device:
+//   1. element kernels (shells, hinges, rods, rod bends) write local blocks;
+//   2. a fixed-order gather sums them into the SELL-64 3x3 BSR over the free vertices
+//      (push order of implicit.rs:2392-2740: inertia, rods, rod bends, shells, hinges,
+//      analytic colliders) - no float atomics, bitwise deterministic;
+//   3. LBVH queries produce the active vertex-triangle and edge-edge stencils, sorted by
+//      (v, t) / (ei, ej) exactly as implicit.rs:678-753 orders them;
+//   4. per stencil: Rayleigh stiffness from the assembled elastic blocks, barrier (FP64),
+//      lagged friction; the contact Hessian stays matrix-free (rank one per stencil plus
+//      the friction projector) and the block-Jacobi diagonal absorbs it;
+//   5. FP32 PCG with FP64 refinement to the requested relative residual;
+//   6. the ACCD line search (FP64 time of impact) over LBVH candidates with the reference's
+//      widening search radius (implicit.rs:1834-1943).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+
+#include "device_common.h"
+#include "devmath.h"
+#include "lbvh.h"
+#include "pcg.h"
+
+namespace nguo {
+
+constexpr uint32_t NONE = 0xffffffffu;
+constexpr uint32_t C_DENSE = 0u, C_ROD_POS = 1u, C_ROD_NEG = 2u, C_HINGE = 3u;
+__host__ __device__ inline uint32_t code(uint32_t type, uint32_t payload) { return (type << 30) | payload; }
+
+// ---------------------------------------------------------------- element sets
+struct ElemDev {
+  uint32_t ns, nh, nr, nrh;
+  const uint32_t *sh_v;  const float *sh_dm; const float *sh_area;
+  const uint32_t *hg_v;  const float *hg_rest;
+  const uint32_t *rd_v;  const float *rd_rest; const float *rd_k;
+  const uint32_t *rh_v;  const float *rh_rest; const float *rh_stiff;
+  float *out_dense;  // 81 floats per rod bend, then per shell (9 blocks of 9)
+  float *out_dg;     // 9 floats per rod bend, then per shell
+  float *out_rod;    // 12 per rod: hk (9) + ga (3)
+  float *out_hinge;  // 14 per hinge: kwt, kwt*(theta-rest), dtheta/dx (12)
+};
+
+struct Params {
+  int shell_model;
+  float shell_mu, shell_lambda, eig_eps, kb;
+  float inv_h2;
+  float dhat, kappa, mu, feps;
+  int barrier_kind;
+  uint32_t ncoll;
+  nguo_collider coll[4];
+};
+
+__global__ __launch_bounds__(BLOCK) void k_shells(ElemDev E, Params P, const float *__restrict__ x) {
+  uint32_t e = blockIdx.x * BLOCK + threadIdx.x;
+  if (e >= E.ns) return;
+  const uint32_t *v = E.sh_v + 3 * (size_t)e;
+  f3 p[3] = {ld3(x, v[0]), ld3(x, v[1]), ld3(x, v[2])};
+  const float *dm = E.sh_dm + 4 * (size_t)e;
+  Mat2 m{dm[0], dm[1], dm[2], dm[3]};
+  f3 g[3];
+  float h[81];
+  shell_element(p, m, E.sh_area[e], P.shell_model, P.shell_mu, P.shell_lambda, P.eig_eps, g, h);
+  float *ob = E.out_dense + 81 * ((size_t)E.nrh + e);
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) ob[(a * 3 + b) * 9 + 3 * r + c] = h[(3 * a + r) * 9 + 3 * b + c];
+  float *og = E.out_dg + 9 * ((size_t)E.nrh + e);
+  for (int a = 0; a < 3; ++a) { og[3 * a] = g[a].x; og[3 * a + 1] = g[a].y; og[3 * a + 2] = g[a].z; }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_hinges(ElemDev E, Params P, const float *__restrict__ x) {
+  uint32_t e = blockIdx.x * BLOCK + threadIdx.x;
+  if (e >= E.nh) return;
+  const uint32_t *v = E.hg_v + 4 * (size_t)e;
+  f3 p[4] = {ld3(x, v[0]), ld3(x, v[1]), ld3(x, v[2]), ld3(x, v[3])};
+  float theta = dihedral_angle(p);
+  f3 g[4];
+  dihedral_grad(p, g);
+  float kwt = P.kb * 1.0f;  // hinge weight w = 1 (bending.rs:151-155)
+  float *o = E.out_hinge + 14 * (size_t)e;
+  o[0] = kwt;
+  o[1] = kwt * (theta - E.hg_rest[e]);
+  for (int a = 0; a < 4; ++a) { o[2 + 3 * a] = g[a].x; o[3 + 3 * a] = g[a].y; o[4 + 3 * a] = g[a].z; }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_rods(ElemDev E, const float *__restrict__ x) {
+  uint32_t e = blockIdx.x * BLOCK + threadIdx.x;
+  if (e >= E.nr) return;
+  uint32_t a = E.rd_v[2 * (size_t)e], b = E.rd_v[2 * (size_t)e + 1];
+  float h[9];
+  f3 ga;
+  rod_stretch_block(ld3(x, a) - ld3(x, b), E.rd_rest[e], E.rd_k[e], h, ga);
+  float *o = E.out_rod + 12 * (size_t)e;
+  for (int i = 0; i < 9; ++i) o[i] = h[i];
+  o[9] = ga.x; o[10] = ga.y; o[11] = ga.z;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_rod_hinges(ElemDev E, const float *__restrict__ x) {
+  uint32_t e = blockIdx.x * BLOCK + threadIdx.x;
+  if (e >= E.nrh) return;
+  const uint32_t *v = E.rh_v + 3 * (size_t)e;
+  f3 g[3];
+  float h[81];
+  strand_force_hessian(ld3(x, v[0]), ld3(x, v[1]), ld3(x, v[2]), E.rh_rest[e], g, h);
+  float st = E.rh_stiff[e];
+  float *ob = E.out_dense + 81 * (size_t)e;
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) ob[(a * 3 + b) * 9 + 3 * r + c] = st * h[(3 * a + r) * 9 + 3 * b + c];
+  float *og = E.out_dg + 9 * (size_t)e;
+  for (int a = 0; a < 3; ++a) { og[3 * a] = st * g[a].x; og[3 * a + 1] = st * g[a].y; og[3 * a + 2] = st * g[a].z; }
+}
+
+// contact.rs:34-49 Collider::distance.
+__device__ inline void collider_distance(const nguo_collider &c, f3 x, float &d, f3 &n) {
+  if (c.kind == NGUO_COLLIDER_HALFSPACE) {
+    f3 p{c.a[0], c.a[1], c.a[2]}, nn{c.b[0], c.b[1], c.b[2]};
+    d = dot(x - p, nn);
+    n = nn;
+  } else {
+    f3 ctr{c.a[0], c.a[1], c.a[2]};
+    f3 v = x - ctr;
+    float l = len(v);
+    if (l < 1.0e-9f) { d = -c.radius; n = f3{0, 1, 0}; }
+    else { d = l - c.radius; n = v / l; }
+  }
+}
+
+// contact.rs:613-643 friction: returns lambda (0 when inactive), gradient in g.
+__device__ inline float friction(f3 f_c, f3 dx, f3 n, float mu, float min_dx, f3 &g) {
+  g = f3{0, 0, 0};
+  if (mu <= 0.0f) return 0.0f;
+  float contact = -dot(n, f_c);
+  f3 pdx = dx - dot(n, dx) * n;
+  float eps = fmaxf(min_dx, 1.0e-12f);
+  float denom = fmaxf(len(pdx), eps);
+  float lambda = mu * contact / denom;
+  if (lambda <= 0.0f) return 0.0f;
+  g = pdx * lambda;
+  return lambda;
+}
+
+// contact.rs:1082-1125 vertex_contact_full for the analytic colliders (kappa fixed,
+// contact_dynamic = 0): gradient and 3x3 Hessian added to g / h.
+__device__ inline void collider_terms(const Params &P, f3 x, f3 xs, f3 &g, float h[9]) {
+  for (uint32_t k = 0; k < P.ncoll; ++k) {
+    float d;
+    f3 n;
+    collider_distance(P.coll[k], x, d, n);
+    double bo[3];
+    if (!barrier_of(P.barrier_kind, (double)d, (double)P.dhat, (double)P.kappa, bo)) continue;
+    f3 f_c = n * (float)bo[1];
+    g = g + f_c;
+    float hp = fmaxf((float)bo[2], 0.0f);
+    float nn[3] = {n.x, n.y, n.z};
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) h[3 * r + c] += hp * nn[r] * nn[c];
+    if (P.mu > 0.0f) {
+      f3 fg;
+      float lam = friction(f_c, x - xs, n, P.mu, P.feps, fg);
+      if (lam > 0.0f) {
+        g = g + fg;
+        for (int r = 0; r < 3; ++r)
+          for (int c = 0; c < 3; ++c) h[3 * r + c] += lam * ((r == c ? 1.0f : 0.0f) - nn[r] * nn[c]);
+      }
+    }
+  }
+}
+
+// Fixed-order gather of the element blocks into the BSR slots (+ inertia and analytic
+// colliders on diagonal slots of the free system).
+__global__ __launch_bounds__(BLOCK) void k_gather_hess(uint32_t nslots, const uint32_t *__restrict__ slot_row, const uint32_t *__restrict__ col,
+                                                      const uint32_t *__restrict__ hc_ptr, const uint32_t *__restrict__ hc, ElemDev E,
+                                                      const uint32_t *__restrict__ row_vert, const float *__restrict__ im, int active,
+                                                      Params P, const float *__restrict__ x, const float *__restrict__ xs, float *__restrict__ val) {
+  uint32_t sl = blockIdx.x * BLOCK + threadIdx.x;
+  if (sl >= nslots) return;
+  float acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  uint32_t row = slot_row[sl];
+  if (row != NONE) {
+    bool diag = col[sl] == row;
+    uint32_t vtx = row_vert[row];
+    if (active && diag) {
+      float m_over_h2 = (1.0f / im[vtx]) * P.inv_h2;
+      acc[0] = m_over_h2; acc[4] = m_over_h2; acc[8] = m_over_h2;
+    }
+    for (uint32_t k = hc_ptr[sl]; k < hc_ptr[sl + 1]; ++k) {
+      uint32_t c = hc[k], type = c >> 30, pl = c & 0x3fffffffu;
+      if (type == C_DENSE) {
+        const float *b = E.out_dense + 9 * (size_t)pl;
+        for (int e = 0; e < 9; ++e) acc[e] += b[e];
+      } else if (type == C_ROD_POS) {
+        const float *b = E.out_rod + 12 * (size_t)pl;
+        for (int e = 0; e < 9; ++e) acc[e] += b[e];
+      } else if (type == C_ROD_NEG) {
+        const float *b = E.out_rod + 12 * (size_t)pl;
+        for (int e = 0; e < 9; ++e) acc[e] += -b[e];
+      } else {
+        uint32_t h = pl >> 4, a = (pl >> 2) & 3u, bb = pl & 3u;
+        const float *o = E.out_hinge + 14 * (size_t)h;
+        const float *ga = o + 2 + 3 * a, *gb = o + 2 + 3 * bb;
+        for (int r = 0; r < 3; ++r)
+          for (int cc = 0; cc < 3; ++cc) acc[3 * r + cc] += (o[0] * ga[r]) * gb[cc];
+      }
+    }
+    if (active && diag && P.ncoll) {
+      f3 g{0, 0, 0};
+      collider_terms(P, ld3(x, vtx), ld3(xs, vtx), g, acc);
+    }
+  }
+  for (int e = 0; e < 9; ++e) val[sell_val_index(sl, e)] = acc[e];
+}
+
+// Elastic (+ inertia, colliders) gradient per free vertex, in push order.
+__global__ __launch_bounds__(BLOCK) void k_gather_grad(uint32_t nf, const uint32_t *__restrict__ free_list, const uint32_t *__restrict__ gc_ptr,
+                                                      const uint32_t *__restrict__ gc, ElemDev E, const float *__restrict__ im, Params P,
+                                                      const float *__restrict__ x, const float *__restrict__ xt, const float *__restrict__ xs,
+                                                      float *__restrict__ grad) {
+  uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= nf) return;
+  uint32_t v = free_list[i];
+  float m_over_h2 = (1.0f / im[v]) * P.inv_h2;
+  f3 xv = ld3(x, v);
+  f3 diff = xv - ld3(xt, v);
+  f3 g{m_over_h2 * diff.x, m_over_h2 * diff.y, m_over_h2 * diff.z};
+  for (uint32_t k = gc_ptr[i]; k < gc_ptr[i + 1]; ++k) {
+    uint32_t c = gc[k], type = c >> 30, pl = c & 0x3fffffffu;
+    if (type == C_DENSE) {
+      const float *b = E.out_dg + 3 * (size_t)pl;
+      g.x += b[0]; g.y += b[1]; g.z += b[2];
+    } else if (type == C_ROD_POS) {
+      const float *b = E.out_rod + 12 * (size_t)pl + 9;
+      g.x += b[0]; g.y += b[1]; g.z += b[2];
+    } else if (type == C_ROD_NEG) {
+      const float *b = E.out_rod + 12 * (size_t)pl + 9;
+      g.x -= b[0]; g.y -= b[1]; g.z -= b[2];
+    } else {
+      uint32_t h = pl >> 2, a = pl & 3u;
+      const float *o = E.out_hinge + 14 * (size_t)h;
+      g.x += o[1] * o[2 + 3 * a]; g.y += o[1] * o[3 + 3 * a]; g.z += o[1] * o[4 + 3 * a];
+    }
+  }
+  if (P.ncoll) {
+    float hdummy[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    f3 gc2{0, 0, 0};
+    collider_terms(P, xv, ld3(xs, v), gc2, hdummy);
+    g = g + gc2;
+  }
+  st3(grad, i, g);
+}
+
+// ---------------------------------------------------------------- contact detection
+struct Topo {
+  uint32_t V;
+  const float *im;
+  const uint32_t *tris, *edges;
+  const uint32_t *ex_ptr, *ex;
+  const uint8_t *static_v;  // null = no mesh collider
+  const float *side;        // null = no mesh collider
+  const float *pn;          // [3V] pseudonormals
+  const uint32_t *vdof, *sdof;
+};
+
+__device__ inline bool movable(const Topo &T, uint32_t v) { return T.im[v] > 0.0f; }
+__device__ inline bool excluded(const Topo &T, uint32_t v, uint32_t t0, uint32_t t1, uint32_t t2) {
+  for (uint32_t k = T.ex_ptr[v]; k < T.ex_ptr[v + 1]; ++k) {
+    uint32_t e = T.ex[k];
+    if (e == t0 || e == t1 || e == t2) return true;
+  }
+  return false;
+}
+__device__ inline bool inside_padded(f3 p, f3 lo, f3 hi, float pad) {
+  f3 l = lo - f3{pad, pad, pad}, h = hi + f3{pad, pad, pad};
+  return !(p.x < l.x || p.y < l.y || p.z < l.z || p.x > h.x || p.y > h.y || p.z > h.z);
+}
+
+// implicit.rs:1431-1476 collider_sides: the side of each movable non-collider vertex from
+// its closest collider facet within dhat (ties: lowest facet index); +1 otherwise.
+__global__ __launch_bounds__(BLOCK) void k_sides(Topo T, BvhView coll, const float *__restrict__ x, float dhat, float *side, uint32_t *err) {
+  uint32_t v = blockIdx.x * BLOCK + threadIdx.x;
+  if (v >= T.V) return;
+  float s = 1.0f;
+  if (movable(T, v) && !T.static_v[v]) {
+    f3 p = ld3(x, v);
+    float best = INFINITY;
+    uint32_t best_t = NONE;
+    float best_side = 1.0f;
+    bvh_traverse(coll, err, [&](f3 lo, f3 hi) { return inside_padded(p, lo, hi, dhat); },
+                 [&](uint32_t t) {
+                   uint32_t t0 = T.tris[3 * (size_t)t], t1 = T.tris[3 * (size_t)t + 1], t2 = T.tris[3 * (size_t)t + 2];
+                   f3 a = ld3(x, t0), b = ld3(x, t1), c = ld3(x, t2);
+                   if (!inside_padded(p, vmin(vmin(a, b), c), vmax(vmax(a, b), c), dhat)) return;
+                   if (excluded(T, v, t0, t1, t2)) return;
+                   float l[3];
+                   f3 q = closest_point_bary(p, a, b, c, l);
+                   f3 e = p - q;
+                   float d2 = dot(e, e);
+                   if (d2 > best || (d2 == best && t > best_t)) return;
+                   best = d2;
+                   best_t = t;
+                   f3 n = surface_normal_at(l, normalize_or_zero(cross(b - a, c - a)), ld3(T.pn, t0), ld3(T.pn, t1), ld3(T.pn, t2));
+                   best_side = dot(e, n) < 0.0f ? -1.0f : 1.0f;
+                 });
+    if (best < dhat * dhat) s = best_side;
+  }
+  side[v] = s;
+}
+
+struct StencilOut {
+  uint32_t cap;
+  uint32_t *count;
+  uint64_t *key;
+  uint32_t *order;  // payload for the sort: slot index
+  float *d;         // [cap]
+  float *g;         // [12 cap]
+};
+
+__device__ inline void emit(const StencilOut &O, uint64_t key, float d, const f3 g[4]) {
+  uint32_t slot = atomicAdd(O.count, 1u);
+  if (slot >= O.cap) return;
+  O.key[slot] = key;
+  O.order[slot] = slot;
+  O.d[slot] = d;
+  float *o = O.g + 12 * (size_t)slot;
+  for (int a = 0; a < 4; ++a) { o[3 * a] = g[a].x; o[3 * a + 1] = g[a].y; o[3 * a + 2] = g[a].z; }
+}
+
+// implicit.rs:683-717 vertex-triangle stencils: every (v, t) with v in AABB(t) + dhat, not
+// excluded, not both immovable, with (signed for collider facets) distance < dhat.
+__global__ __launch_bounds__(BLOCK) void k_vt_stencils(Topo T, BvhView dyn, BvhView stat, const float *__restrict__ x, float dhat,
+                                                      StencilOut O, uint32_t *err) {
+  uint32_t v = blockIdx.x * BLOCK + threadIdx.x;
+  if (v >= T.V) return;
+  f3 p = ld3(x, v);
+  bool mv = movable(T, v);
+  auto leaf = [&](uint32_t t) {
+    uint32_t t0 = T.tris[3 * (size_t)t], t1 = T.tris[3 * (size_t)t + 1], t2 = T.tris[3 * (size_t)t + 2];
+    f3 a = ld3(x, t0), b = ld3(x, t1), c = ld3(x, t2);
+    if (!inside_padded(p, vmin(vmin(a, b), c), vmax(vmax(a, b), c), dhat)) return;
+    bool tm = movable(T, t0) || movable(T, t1) || movable(T, t2);
+    if (!tm && !mv) return;
+    if (excluded(T, v, t0, t1, t2)) return;
+    f3 g[4];
+    float d = point_tri_distance_grad(p, a, b, c, g);
+    if (T.static_v && T.static_v[t0] && T.static_v[t1] && T.static_v[t2] && T.side[v] < 0.0f) {
+      d = -d;
+      for (int k = 0; k < 4; ++k) g[k] = -g[k];
+    }
+    if (d >= dhat) return;
+    emit(O, ((uint64_t)v << 32) | t, d, g);
+  };
+  auto ov = [&](f3 lo, f3 hi) { return inside_padded(p, lo, hi, dhat); };
+  bvh_traverse(dyn, err, ov, leaf);
+  if (mv) bvh_traverse(stat, err, ov, leaf);
+}
+
+// implicit.rs:719-751 edge-edge stencils. Query edges are padded, tree edges are not; a
+// movable query pairs only with lower-indexed tree edges, a static query with any.
+__global__ __launch_bounds__(BLOCK) void k_ee_stencils(Topo T, BvhView dyn, const uint32_t *qids, uint32_t nq, int query_dyn,
+                                                      const float *__restrict__ x, float pad, float ghat, StencilOut O, uint32_t *err) {
+  uint32_t q = blockIdx.x * BLOCK + threadIdx.x;
+  if (q >= nq) return;
+  uint32_t ej = qids[q];
+  uint32_t b0 = T.edges[2 * (size_t)ej], b1 = T.edges[2 * (size_t)ej + 1];
+  f3 pb = ld3(x, b0), qb = ld3(x, b1);
+  f3 blo = vmin(pb, qb) - f3{pad, pad, pad}, bhi = vmax(pb, qb) + f3{pad, pad, pad};
+  auto ov = [&](f3 lo, f3 hi) { return !(lo.x > bhi.x || lo.y > bhi.y || lo.z > bhi.z || hi.x < blo.x || hi.y < blo.y || hi.z < blo.z); };
+  bvh_traverse(dyn, err, ov, [&](uint32_t ei) {
+    if (ei == ej || (query_dyn && ei > ej)) return;
+    uint32_t a0 = T.edges[2 * (size_t)ei], a1 = T.edges[2 * (size_t)ei + 1];
+    if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) return;
+    f3 pa = ld3(x, a0), qa = ld3(x, a1);
+    if (!ov(vmin(pa, qa), vmax(pa, qa))) return;
+    uint32_t lo_e = min(ei, ej), hi_e = max(ei, ej);
+    uint32_t i0 = T.edges[2 * (size_t)lo_e], i1 = T.edges[2 * (size_t)lo_e + 1], i2 = T.edges[2 * (size_t)hi_e], i3 = T.edges[2 * (size_t)hi_e + 1];
+    f3 g[4];
+    float d = segment_segment_distance_grad(ld3(x, i0), ld3(x, i1), ld3(x, i2), ld3(x, i3), g);
+    if (d - 0.0f >= ghat) return;
+    emit(O, ((uint64_t)lo_e << 32) | hi_e, d - 0.0f, g);
+  });
+}
+
+// Gathers the sorted stencils (V-T block then E-E block) into one array.
+__global__ __launch_bounds__(BLOCK) void k_stencil_gather(uint32_t nvt, uint32_t nee, const uint64_t *vt_key, const uint32_t *vt_ord,
+                                                         const float *vt_d, const float *vt_g, const uint64_t *ee_key, const uint32_t *ee_ord,
+                                                         const float *ee_d, const float *ee_g, const uint32_t *tris, const uint32_t *edges,
+                                                         uint32_t *sidx, float *sd, float *sg) {
+  uint32_t s = blockIdx.x * BLOCK + threadIdx.x;
+  if (s >= nvt + nee) return;
+  uint32_t id[4];
+  uint32_t src;
+  const float *dd, *gg;
+  if (s < nvt) {
+    uint64_t k = vt_key[s];
+    uint32_t v = (uint32_t)(k >> 32), t = (uint32_t)k;
+    id[0] = v; id[1] = tris[3 * (size_t)t]; id[2] = tris[3 * (size_t)t + 1]; id[3] = tris[3 * (size_t)t + 2];
+    src = vt_ord[s]; dd = vt_d; gg = vt_g;
+  } else {
+    uint64_t k = ee_key[s - nvt];
+    uint32_t ei = (uint32_t)(k >> 32), ej = (uint32_t)k;
+    id[0] = edges[2 * (size_t)ei]; id[1] = edges[2 * (size_t)ei + 1]; id[2] = edges[2 * (size_t)ej]; id[3] = edges[2 * (size_t)ej + 1];
+    src = ee_ord[s - nvt]; dd = ee_d; gg = ee_g;
+  }
+  for (int a = 0; a < 4; ++a) sidx[4 * (size_t)s + a] = id[a];
+  sd[s] = dd[src];
+  for (int k = 0; k < 12; ++k) sg[12 * (size_t)s + k] = gg[12 * (size_t)src + k];
+}
+
+// Block (va, vb) of the elastic Hessian gathered so far (the reference's fixed_hess_in):
+// from the free system for two free vertices, from the static table for two pinned ones.
+__device__ inline void lookup_block(const Topo &T, const SellView &K, const SellView &S, uint32_t va, uint32_t vb, float out[9]) {
+  for (int e = 0; e < 9; ++e) out[e] = 0.0f;
+  const SellView *M = nullptr;
+  uint32_t r, c;
+  if (T.vdof[va] != NONE && T.vdof[vb] != NONE) { M = &K; r = T.vdof[va]; c = T.vdof[vb]; }
+  else if (T.sdof && T.sdof[va] != NONE && T.sdof[vb] != NONE) { M = &S; r = T.sdof[va]; c = T.sdof[vb]; }
+  if (!M || M->nb == 0) return;
+  uint32_t base = M->slice_ptr[r >> 6], lane = r & 63u, lo = 0, hi = M->row_len[r];
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (M->col[base + 64u * mid + lane] < c) lo = mid + 1; else hi = mid;
+  }
+  if (lo < M->row_len[r] && M->col[base + 64u * lo + lane] == c) {
+    uint32_t sl = base + 64u * lo + lane;
+    for (int e = 0; e < 9; ++e) out[e] = M->val[sell_val_index(sl, e)];
+  }
+}
+
+struct ContactOut {
+  uint32_t *dof;
+  float *g, *w, *coef, *n, *cgrad;  // cgrad: 24 per stencil (barrier part, friction part)
+  float *kappa;
+};
+
+// implicit.rs:3026-3103 per-stencil barrier + friction, implicit.rs:755-795 Rayleigh stiffness.
+__global__ __launch_bounds__(BLOCK) void k_stencil_eval(uint32_t ns, const uint32_t *__restrict__ sidx, const float *__restrict__ sd,
+                                                       const float *__restrict__ sg, float ghat_vt, float ghat_ee, uint32_t nvt, Topo T,
+                                                       SellView K, SellView S, Params P, const float *__restrict__ x, const float *__restrict__ xs,
+                                                       ContactOut C) {
+  uint32_t s = blockIdx.x * BLOCK + threadIdx.x;
+  if (s >= ns) return;
+  uint32_t id[4];
+  f3 g[4];
+  for (int a = 0; a < 4; ++a) {
+    id[a] = sidx[4 * (size_t)s + a];
+    const float *q = sg + 12 * (size_t)s + 3 * a;
+    g[a] = f3{q[0], q[1], q[2]};
+  }
+  float d = sd[s];
+  float ghat = s < nvt ? ghat_vt : ghat_ee;
+  // Rayleigh stiffness (contact.rs:750-832) along the weighted normal
+  float k = P.kappa;
+  float nlen = len(g[0]);
+  f3 nrm{0, 0, 0};
+  float weights[4] = {0, 0, 0, 0};
+  if (!(nlen < 1.0e-12f)) {
+    nrm = g[0] / nlen;
+    float mass[4];
+    for (int a = 0; a < 4; ++a) {
+      weights[a] = dot(g[a], nrm);
+      float im = T.im[id[a]];
+      mass[a] = im > 0.0f ? 1.0f / im : 0.0f;
+    }
+    f3 e = d * nrm;
+    float gap = len(e) - 0.0f;
+    float kk = 0.0f;
+    if (gap > 0.0f) {
+      float sqr_x = gap * gap;
+      float m_max = 0.0f;
+      for (int a = 0; a < 4; ++a) m_max = fmaxf(m_max, mass[a]);
+      float w[12];
+      for (int a = 0; a < 4; ++a) { f3 v = weights[a] * e; w[3 * a] = v.x; w[3 * a + 1] = v.y; w[3 * a + 2] = v.z; }
+      float ss = 0.0f;
+      for (int i = 0; i < 12; ++i) ss += w[i] * w[i];
+      float norm = sqrtf(ss);
+      if (norm > 0.0f) {
+        for (int i = 0; i < 12; ++i) w[i] /= norm;
+        float acc = 0.0f;
+        for (int ii = 0; ii < 4; ++ii) {
+          float m_ii = mass[ii] > 0.0f ? mass[ii] : m_max;
+          for (int jj = 0; jj < 4; ++jj) {
+            float blk[9];
+            lookup_block(T, K, S, id[ii], id[jj], blk);
+            for (int r = 0; r < 3; ++r) {
+              float row = 0.0f;
+              for (int c = 0; c < 3; ++c) {
+                float v = blk[3 * r + c];
+                if (ii == jj && r == c) v += m_ii / sqr_x;
+                row += v * w[3 * jj + c];
+              }
+              acc += w[3 * ii + r] * row;
+            }
+          }
+        }
+        kk = acc;
+      }
+    }
+    k = kk > 0.0f ? kk : P.kappa;
+  }
+  C.kappa[s] = k;
+  int kind = d < 0.0f ? 1 : P.barrier_kind;
+  double bo[3] = {0, 0, 0};
+  bool act = barrier_of(kind, (double)d, (double)ghat, (double)k, bo);
+  float bg = act ? (float)bo[1] : 0.0f;
+  float hp = act ? fmaxf((float)bo[2], 0.0f) : 0.0f;
+  float lam = 0.0f;
+  f3 fn{0, 0, 0};
+  f3 fg{0, 0, 0};
+  float wv[4] = {0, 0, 0, 0};
+  if (act && P.mu > 0.0f && !(nlen < 1.0e-12f)) {
+    // contact.rs:658-706 stencil_friction
+    f3 slip{0, 0, 0};
+    for (int a = 0; a < 4; ++a) {
+      wv[a] = dot(g[a], nrm);
+      slip = slip + (ld3(x, id[a]) - ld3(xs, id[a])) * wv[a];
+    }
+    f3 f_c = nrm * bg;
+    lam = friction(f_c, slip, nrm, P.mu, P.feps, fg);
+    if (lam > 0.0f && fg.x == 0.0f && fg.y == 0.0f && fg.z == 0.0f) lam = 0.0f;
+    fn = nrm;
+  }
+  for (int a = 0; a < 4; ++a) {
+    C.dof[4 * (size_t)s + a] = T.vdof[id[a]];
+    C.w[4 * (size_t)s + a] = wv[a];
+    float *cg = C.cgrad + 24 * (size_t)s + 3 * a;
+    cg[0] = bg * g[a].x; cg[1] = bg * g[a].y; cg[2] = bg * g[a].z;
+    float *cf = C.cgrad + 24 * (size_t)s + 12 + 3 * a;
+    f3 f = lam > 0.0f ? fg * wv[a] : f3{0, 0, 0};
+    cf[0] = f.x; cf[1] = f.y; cf[2] = f.z;
+    for (int c = 0; c < 3; ++c) C.g[12 * (size_t)s + 3 * a + c] = comp(g[a], c);
+  }
+  C.coef[2 * (size_t)s] = hp;
+  C.coef[2 * (size_t)s + 1] = lam;
+  C.n[3 * (size_t)s] = fn.x; C.n[3 * (size_t)s + 1] = fn.y; C.n[3 * (size_t)s + 2] = fn.z;
+}
+
+// Incidence keys: (free dof << 32) | (4 s + a); pinned participants get dof = nf (sort last).
+__global__ __launch_bounds__(BLOCK) void k_inc_keys(uint32_t ns, const uint32_t *dof, uint32_t nf, uint64_t *key, uint32_t *val) {
+  uint32_t e = blockIdx.x * BLOCK + threadIdx.x;
+  if (e >= 4 * ns) return;
+  uint32_t d = dof[e];
+  key[e] = ((uint64_t)(d == NONE ? nf : d) << 32) | e;
+  val[e] = e;
+}
+__global__ __launch_bounds__(BLOCK) void k_inc_ptr(uint32_t nf, uint32_t m, const uint64_t *key, uint32_t *ptr, uint32_t *inc) {
+  uint32_t r = blockIdx.x * BLOCK + threadIdx.x;
+  if (r < m) inc[r] = (uint32_t)key[r];
+  if (r > nf) return;
+  uint64_t target = (uint64_t)r << 32;
+  uint32_t lo = 0, hi = m;
+  while (lo < hi) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (key[mid] < target) lo = mid + 1; else hi = mid;
+  }
+  ptr[r] = lo;
+}
+
+// rhs = -(elastic gradient + contact gradient in stencil order).
+__global__ __launch_bounds__(BLOCK) void k_rhs(uint32_t nf, const float *__restrict__ eg, const uint32_t *inc_ptr, const uint32_t *inc,
+                                              const float *cgrad, int has_contact, float *rhs) {
+  uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= nf) return;
+  f3 g = ld3(eg, i);
+  if (has_contact) {
+    for (uint32_t k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k) {
+      uint32_t e = inc[k], s = e >> 2, a = e & 3u;
+      const float *b = cgrad + 24 * (size_t)s + 3 * a;
+      const float *f = cgrad + 24 * (size_t)s + 12 + 3 * a;
+      g.x += b[0]; g.y += b[1]; g.z += b[2];
+      g.x += f[0]; g.y += f[1]; g.z += f[2];
+    }
+  }
+  st3(rhs, i, f3{-g.x, -g.y, -g.z});
+}
+
+// ---------------------------------------------------------------- step kernels
+__global__ __launch_bounds__(BLOCK) void k_predict(uint32_t V, const float *im, const float *x0, const float *v, float h, f3 grav, float *xt, float *x) {
+  uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= V) return;
+  f3 p = ld3(x0, i);
+  f3 t = im[i] > 0.0f ? p + ld3(v, i) * h + grav * (h * h) : p;
+  st3(xt, i, t);
+  st3(x, i, p);
+}
+
+// dx on all vertices (0 on pinned) and per-block max |dx|.
+__global__ __launch_bounds__(BLOCK) void k_scatter_dx(uint32_t V, const uint32_t *vdof, const float *dxf, float *dx, float *part) {
+  __shared__ float lds[4];
+  uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  float m = 0.0f;
+  if (i < V) {
+    uint32_t d = vdof[i];
+    f3 v = d == NONE ? f3{0, 0, 0} : ld3(dxf, d);
+    st3(dx, i, v);
+    m = len(v);
+  }
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = fmaxf(fmaxf(lds[0], lds[1]), fmaxf(lds[2], lds[3]));
+}
+
+__global__ __launch_bounds__(BLOCK) void k_max_final(uint32_t n, const float *part, float *out) {
+  __shared__ float lds[4];
+  float m = 0.0f;
+  for (uint32_t i = threadIdx.x; i < n; i += BLOCK) m = fmaxf(m, part[i]);
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) *out = fmaxf(fmaxf(lds[0], lds[1]), fmaxf(lds[2], lds[3]));
+}
+
+__global__ __launch_bounds__(BLOCK) void k_scale(uint32_t n, float *a, float s) {
+  uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i < n) a[i] = a[i] * s;
+}
+
+// Analytic-collider CCD clamp (implicit.rs:1792-1817), min into tmin (double bits).
+__global__ __launch_bounds__(BLOCK) void k_ccd_colliders(uint32_t V, const float *im, const float *x, const float *dx, Params P,
+                                                        unsigned long long *tmin) {
+  uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= V || !(im[i] > 0.0f)) return;
+  const float GAP = 0.2f;
+  float t = 1.0f;
+  f3 xi = ld3(x, i), di = ld3(dx, i);
+  for (uint32_t k = 0; k < P.ncoll; ++k) {
+    float d0;
+    f3 n;
+    collider_distance(P.coll[k], xi, d0, n);
+    if (d0 <= 0.0f) continue;
+    if (P.coll[k].kind == NGUO_COLLIDER_HALFSPACE) {
+      float rate = dot(di, n);
+      if (rate < 0.0f) t = fminf(t, (GAP - 1.0f) * d0 / rate);
+    } else {
+      float l = len(di);
+      if (l > 1.0e-12f) t = fminf(t, (1.0f - GAP) * d0 / l);
+    }
+  }
+  if (t < 1.0f) atomicMin(tmin, (unsigned long long)__double_as_longlong((double)t));
+}
+
+// Vertex-triangle ACCD over the padded-AABB candidates (implicit.rs:1856-1900), FP64 TOI.
+__global__ __launch_bounds__(BLOCK) void k_ccd_vt(Topo T, BvhView dyn, BvhView stat, const float *__restrict__ x, const float *__restrict__ dx,
+                                                 float pad, uint32_t iters, unsigned long long *tmin, uint32_t *err) {
+  uint32_t v = blockIdx.x * BLOCK + threadIdx.x;
+  if (v >= T.V) return;
+  f3 p = ld3(x, v);
+  bool mv = movable(T, v);
+  double best = 1.0;
+  auto leaf = [&](uint32_t t) {
+    uint32_t t0 = T.tris[3 * (size_t)t], t1 = T.tris[3 * (size_t)t + 1], t2 = T.tris[3 * (size_t)t + 2];
+    f3 a = ld3(x, t0), b = ld3(x, t1), c = ld3(x, t2);
+    if (!inside_padded(p, vmin(vmin(a, b), c), vmax(vmax(a, b), c), pad)) return;
+    bool tm = movable(T, t0) || movable(T, t1) || movable(T, t2);
+    if (!tm && !mv) return;
+    if (excluded(T, v, t0, t1, t2)) return;
+    if (T.static_v && T.static_v[t0] && T.static_v[t1] && T.static_v[t2] && T.side[v] < 0.0f) return;
+    d3 x0[4] = {to_d(p), to_d(a), to_d(b), to_d(c)};
+    d3 d4[4] = {to_d(ld3(dx, v)), to_d(ld3(dx, t0)), to_d(ld3(dx, t1)), to_d(ld3(dx, t2))};
+    double toi = accd_toi(x0, d4, 0, 0.0, 1.25, 0.01, iters);
+    best = fmin(best, toi / 1.25);
+  };
+  auto ov = [&](f3 lo, f3 hi) { return inside_padded(p, lo, hi, pad); };
+  bvh_traverse(dyn, err, ov, leaf);
+  if (mv) bvh_traverse(stat, err, ov, leaf);
+  if (best < 1.0) atomicMin(tmin, (unsigned long long)__double_as_longlong(best));
+}
+
+// Edge-edge ACCD (implicit.rs:1902-1936).
+__global__ __launch_bounds__(BLOCK) void k_ccd_ee(Topo T, BvhView dyn, const uint32_t *qids, uint32_t nq, int query_dyn, const float *__restrict__ x,
+                                                 const float *__restrict__ dx, float pad, uint32_t iters, unsigned long long *tmin, uint32_t *err) {
+  uint32_t q = blockIdx.x * BLOCK + threadIdx.x;
+  if (q >= nq) return;
+  uint32_t ej = qids[q];
+  uint32_t b0 = T.edges[2 * (size_t)ej], b1 = T.edges[2 * (size_t)ej + 1];
+  f3 pb = ld3(x, b0), qb = ld3(x, b1);
+  f3 blo = vmin(pb, qb) - f3{pad, pad, pad}, bhi = vmax(pb, qb) + f3{pad, pad, pad};
+  auto ov = [&](f3 lo, f3 hi) { return !(lo.x > bhi.x || lo.y > bhi.y || lo.z > bhi.z || hi.x < blo.x || hi.y < blo.y || hi.z < blo.z); };
+  double best = 1.0;
+  bvh_traverse(dyn, err, ov, [&](uint32_t ei) {
+    if (ei == ej || (query_dyn && ei > ej)) return;
+    uint32_t a0 = T.edges[2 * (size_t)ei], a1 = T.edges[2 * (size_t)ei + 1];
+    if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) return;
+    f3 pa = ld3(x, a0), qa = ld3(x, a1);
+    if (!ov(vmin(pa, qa), vmax(pa, qa))) return;
+    uint32_t lo_e = min(ei, ej), hi_e = max(ei, ej);
+    uint32_t i0 = T.edges[2 * (size_t)lo_e], i1 = T.edges[2 * (size_t)lo_e + 1], i2 = T.edges[2 * (size_t)hi_e], i3 = T.edges[2 * (size_t)hi_e + 1];
+    d3 x0[4] = {to_d(ld3(x, i0)), to_d(ld3(x, i1)), to_d(ld3(x, i2)), to_d(ld3(x, i3))};
+    d3 d4[4] = {to_d(ld3(dx, i0)), to_d(ld3(dx, i1)), to_d(ld3(dx, i2)), to_d(ld3(dx, i3))};
+    double toi = accd_toi(x0, d4, 1, 0.0, 1.25, 0.01, iters);
+    best = fmin(best, toi / 1.25);
+  });
+  if (best < 1.0) atomicMin(tmin, (unsigned long long)__double_as_longlong(best));
+}
+
+// x += t dx; per-block max |t dx|.
+__global__ __launch_bounds__(BLOCK) void k_apply(uint32_t V, float *x, const float *dx, float t, float *part) {
+  __shared__ float lds[4];
+  uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  float m = 0.0f;
+  if (i < V) {
+    f3 d = ld3(dx, i) * t;
+    st3(x, i, ld3(x, i) + d);
+    m = len(d);
+  }
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = fmaxf(fmaxf(lds[0], lds[1]), fmaxf(lds[2], lds[3]));
+}
+
+// implicit.rs:2032-2064: floor projection, finiteness guard, velocity update, commit.
+__global__ __launch_bounds__(BLOCK) void k_finite(uint32_t V, const float *x, uint32_t *bad) {
+  uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= V) return;
+  f3 p = ld3(x, i);
+  if (!(isfin(p.x) && isfin(p.y) && isfin(p.z))) atomicOr(bad, 1u);
+}
+__global__ __launch_bounds__(BLOCK) void k_commit(uint32_t V, const float *im, float *x, float *x0, float *v, float inv_h, float floor_y,
+                                                 const uint32_t *bad, float *part) {
+  __shared__ float lds[4];
+  uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  float m = 0.0f;
+  if (i < V) {
+    f3 p = ld3(x, i), p0 = ld3(x0, i);
+    bool free_v = im[i] > 0.0f;
+    if (free_v && isfin(floor_y) && p.y < floor_y) p.y = floor_y;
+    if (*bad == 0u) {
+      f3 vel = free_v ? (p - p0) * inv_h : f3{0, 0, 0};
+      st3(v, i, vel);
+      st3(x0, i, p);
+      st3(x, i, p);
+      if (free_v) m = len(vel);
+    } else {
+      st3(v, i, f3{0, 0, 0});
+      st3(x, i, p0);
+    }
+  }
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = fmaxf(fmaxf(lds[0], lds[1]), fmaxf(lds[2], lds[3]));
+}
+
+// implicit.rs:618-650 positions_self_intersect: any non-inert, non-adjacent (edge, triangle)
+// with a transverse pierce. Each edge queries both triangle trees (movable edges) or the
+// dynamic one (static edges).
+__global__ __launch_bounds__(BLOCK) void k_pierce(Topo T, BvhView dyn, BvhView stat, uint32_t E, const float *__restrict__ x, uint32_t *hit,
+                                                 uint32_t *err) {
+  uint32_t ei = blockIdx.x * BLOCK + threadIdx.x;
+  if (ei >= E) return;
+  uint32_t e0 = T.edges[2 * (size_t)ei], e1 = T.edges[2 * (size_t)ei + 1];
+  f3 p = ld3(x, e0), q = ld3(x, e1);
+  f3 lo = vmin(p, q), hi = vmax(p, q);
+  bool em = movable(T, e0) || movable(T, e1);
+  auto ov = [&](f3 l, f3 h) { return !(l.x > hi.x || l.y > hi.y || l.z > hi.z || h.x < lo.x || h.y < lo.y || h.z < lo.z); };
+  auto leaf = [&](uint32_t t) {
+    uint32_t t0 = T.tris[3 * (size_t)t], t1 = T.tris[3 * (size_t)t + 1], t2 = T.tris[3 * (size_t)t + 2];
+    if (e0 == t0 || e0 == t1 || e0 == t2 || e1 == t0 || e1 == t1 || e1 == t2) return;
+    if (edge_triangle_intersect(p, q, ld3(x, t0), ld3(x, t1), ld3(x, t2))) atomicOr(hit, 1u);
+  };
+  bvh_traverse(dyn, err, ov, leaf);
+  if (em) bvh_traverse(stat, err, ov, leaf);
+}
+
+}  // namespace nguo
+
+// ================================================================ host side
+using namespace nguo;
+
+struct ElemHost {
+  std::vector<uint32_t> sh_v, hg_v, rd_v, rh_v;
+  std::vector<float> sh_dm, sh_area, hg_rest, rd_rest, rd_k, rh_rest, rh_stiff;
+  uint32_t ns() const { return (uint32_t)sh_area.size(); }
+  uint32_t nh() const { return (uint32_t)hg_rest.size(); }
+  uint32_t nr() const { return (uint32_t)rd_rest.size(); }
+  uint32_t nrh() const { return (uint32_t)rh_rest.size(); }
+};
+
+struct ElemSet {
+  uint32_t ns = 0, nh = 0, nr = 0, nrh = 0;
+  DevBuf<uint32_t> sh_v, hg_v, rd_v, rh_v;
+  DevBuf<float> sh_dm, sh_area, hg_rest, rd_rest, rd_k, rh_rest, rh_stiff;
+  DevBuf<float> out_dense, out_dg, out_rod, out_hinge;
+  SellMatrix M;
+  DevBuf<uint32_t> slot_row, hc_ptr, hc, gc_ptr, gc, row_vert;
+  ElemDev dev() const {
+    return ElemDev{ns, nh, nr, nrh, sh_v.p, sh_dm.p, sh_area.p, hg_v.p, hg_rest.p, rd_v.p, rd_rest.p, rd_k.p,
+                   rh_v.p, rh_rest.p, rh_stiff.p, out_dense.p, out_dg.p, out_rod.p, out_hinge.p};
+  }
+};
+
+struct nguo_step {
+  nguo_ctx *ctx = nullptr;
+  hipStream_t s = nullptr;
+  nguo_params prm{};
+  Params P{};
+  uint32_t V = 0, NF = 0, F = 0, E = 0;
+  std::vector<uint32_t> h_vdof, h_free, h_tris, h_edges;
+  std::vector<float> h_im;
+  std::vector<uint8_t> h_static;
+  bool has_collider = false;
+  float char_len = 1.0f;
+  double time = 0.0;
+  // device state
+  DevBuf<float> x, v, x0, xt, dx, dxf, im, rhs, egrad, side, pn, part;
+  DevBuf<uint32_t> vdof, sdof, free_list, tris, edges, ex_ptr, ex, err, flag;
+  DevBuf<uint8_t> static_v;
+  DevBuf<unsigned long long> tmin;
+  // trees and their primitive id lists
+  DevBuf<uint32_t> tri_dyn_ids, tri_stat_ids, tri_coll_ids, edge_dyn_ids, edge_stat_ids;
+  uint32_t n_tri_dyn = 0, n_tri_stat = 0, n_tri_coll = 0, n_edge_dyn = 0, n_edge_stat = 0;
+  Lbvh tree_tri_dyn, tree_tri_stat, tree_tri_coll, tree_edge_dyn;
+  // elements
+  ElemSet act, stat;
+  // contact buffers
+  uint32_t cap = 0;
+  DevBuf<uint32_t> vt_count, vt_ord, ee_ord;
+  DevBuf<uint64_t> vt_key, ee_key, inc_key;
+  DevBuf<float> vt_d, vt_g, ee_d, ee_g;
+  DevBuf<uint32_t> sidx, inc_val, inc, inc_ptr, cdof;
+  DevBuf<float> sd, sg, cg, cw, ccoef, cn, cgrad, csums, ckappa;
+  uint32_t ns = 0, nvt = 0, nee = 0;
+  RadixSort sorter;
+  std::unique_ptr<Pcg> pcg;
+  // host scratch (pinned)
+  float *hf = nullptr;
+  uint32_t *hu = nullptr;
+  unsigned long long *hl = nullptr;
+  nguo_step_metrics metrics{};
+  DevBuf<float> trace;  // x after every Newton iteration of the last step
+  std::vector<float> trace_toi;
+  ~nguo_step() {
+    if (hf) (void)hipHostFree(hf);
+    if (hu) (void)hipHostFree(hu);
+    if (hl) (void)hipHostFree(hl);
+  }
+};
+
+namespace {
+
+std::vector<int> digit_shifts(uint64_t maxval_lo, uint64_t maxval_hi) {
+  std::vector<int> sh;
+  int lob = 0, hib = 0;
+  while (lob < 32 && (maxval_lo >> lob) != 0) lob++;
+  while (hib < 32 && (maxval_hi >> hib) != 0) hib++;
+  for (int b = 0; b < lob; b += 8) sh.push_back(b);
+  for (int b = 0; b < hib; b += 8) sh.push_back(32 + b);
+  if (sh.empty()) sh.push_back(0);
+  return sh;
+}
+
+// shell_model.rs:309-349 build_shells (rest frame, Dm^-1, area) on the host.
+bool shell_rest(const float *p0, const float *p1, const float *p2, float dm_out[4], float &area) {
+  f3 a{p0[0], p0[1], p0[2]}, b{p1[0], p1[1], p1[2]}, c{p2[0], p2[1], p2[2]};
+  f3 e1 = b - a, e2 = c - a;
+  f3 n = cross(e1, e2);
+  area = 0.5f * std::sqrt(dot(n, n));
+  if (area <= 1.0e-12f) return false;
+  f3 ax = e1 * (1.0f / std::sqrt(dot(e1, e1)));
+  f3 t = e2 - dot(e2, ax) * ax;
+  f3 ay = t * (1.0f / std::sqrt(dot(t, t)));
+  float m00 = dot(e1, ax), m10 = dot(e1, ay), m01 = dot(e2, ax), m11 = dot(e2, ay);
+  float det = m00 * m11 - m10 * m01;
+  if (std::fabs(det) < 1e-20f) return false;
+  float inv_det = 1.0f / det;
+  // inverse, column-major (xa = (m11, -m10) / det, ya = (-m01, m00) / det)
+  dm_out[0] = m11 * inv_det;
+  dm_out[1] = -m10 * inv_det;
+  dm_out[2] = -m01 * inv_det;
+  dm_out[3] = m00 * inv_det;
+  return true;
+}
+
+// bending.rs:34-48 dihedral angle on the host (rest angles).
+float dihedral_host(const float *p0, const float *p1, const float *p2, const float *p3) {
+  f3 a{p0[0], p0[1], p0[2]}, b{p1[0], p1[1], p1[2]}, c{p2[0], p2[1], p2[2]}, d{p3[0], p3[1], p3[2]};
+  f3 n1 = cross(b - a, c - a), n2 = cross(b - c, d - c);
+  f3 e = normalize_or_zero(c - b);
+  f3 n1n = normalize_or_zero(n1), n2n = normalize_or_zero(n2);
+  float sn = dot(cross(n1n, n2n), e);
+  float cs = clampv(dot(n1n, n2n), -1.0f, 1.0f);
+  return std::atan2(sn, cs);
+}
+
+// Builds the pattern, slot->row map and the fixed-order contribution lists of one element
+// set over `rows` (vertex -> row map, NONE = not in this system).
+void build_gather(ElemSet &S, const ElemHost &H, uint32_t nrows, const std::vector<uint32_t> &vmap, const std::vector<uint32_t> &row_vert,
+                  bool with_grad, hipStream_t s) {
+  std::vector<std::vector<uint32_t>> cols(nrows);
+  for (uint32_t r = 0; r < nrows; ++r) cols[r].push_back(r);
+  auto couple = [&](const uint32_t *v, int k) {
+    for (int a = 0; a < k; ++a)
+      for (int b = 0; b < k; ++b) {
+        uint32_t ra = vmap[v[a]], rb = vmap[v[b]];
+        if (ra != NONE && rb != NONE) cols[ra].push_back(rb);
+      }
+  };
+  for (uint32_t e = 0; e < H.nr(); ++e) couple(&H.rd_v[2 * e], 2);
+  for (uint32_t e = 0; e < H.nrh(); ++e) couple(&H.rh_v[3 * e], 3);
+  for (uint32_t e = 0; e < H.ns(); ++e) couple(&H.sh_v[3 * e], 3);
+  for (uint32_t e = 0; e < H.nh(); ++e) couple(&H.hg_v[4 * e], 4);
+  for (auto &c : cols) { std::sort(c.begin(), c.end()); c.erase(std::unique(c.begin(), c.end()), c.end()); }
+  S.M.build_pattern(nrows, cols, s);
+  // slot -> row
+  std::vector<uint32_t> srow(S.M.nslots, NONE);
+  for (uint32_t r = 0; r < nrows; ++r)
+    for (uint32_t k = 0; k < S.M.h_row_len[r]; ++k) srow[S.M.slot(r, k)] = r;
+  // contributions (slot, code) in push order
+  std::vector<std::pair<uint32_t, uint32_t>> hlist;
+  std::vector<std::pair<uint32_t, uint32_t>> glist;
+  const uint32_t nrh = H.nrh();
+  for (uint32_t e = 0; e < H.nr(); ++e) {
+    uint32_t a = H.rd_v[2 * e], b = H.rd_v[2 * e + 1];
+    uint32_t ra = vmap[a], rb = vmap[b];
+    if (ra != NONE) hlist.push_back({S.M.find(ra, ra), code(C_ROD_POS, e)});
+    if (rb != NONE) hlist.push_back({S.M.find(rb, rb), code(C_ROD_POS, e)});
+    if (ra != NONE && rb != NONE) {
+      hlist.push_back({S.M.find(ra, rb), code(C_ROD_NEG, e)});
+      hlist.push_back({S.M.find(rb, ra), code(C_ROD_NEG, e)});
+    }
+    if (ra != NONE) glist.push_back({ra, code(C_ROD_POS, e)});
+    if (rb != NONE) glist.push_back({rb, code(C_ROD_NEG, e)});
+  }
+  auto dense = [&](const uint32_t *v, uint32_t elem_base) {
+    for (int a = 0; a < 3; ++a) {
+      uint32_t ra = vmap[v[a]];
+      for (int b = 0; b < 3; ++b) {
+        uint32_t rb = vmap[v[b]];
+        if (ra != NONE && rb != NONE) hlist.push_back({S.M.find(ra, rb), code(C_DENSE, elem_base * 9 + 3 * a + b)});
+      }
+    }
+    for (int a = 0; a < 3; ++a)
+      if (vmap[v[a]] != NONE) glist.push_back({vmap[v[a]], code(C_DENSE, elem_base * 3 + a)});
+  };
+  for (uint32_t e = 0; e < nrh; ++e) dense(&H.rh_v[3 * e], e);
+  for (uint32_t e = 0; e < H.ns(); ++e) dense(&H.sh_v[3 * e], nrh + e);
+  for (uint32_t e = 0; e < H.nh(); ++e) {
+    const uint32_t *v = &H.hg_v[4 * e];
+    for (int a = 0; a < 4; ++a) {
+      uint32_t ra = vmap[v[a]];
+      for (int b = 0; b < 4; ++b) {
+        uint32_t rb = vmap[v[b]];
+        if (ra != NONE && rb != NONE) hlist.push_back({S.M.find(ra, rb), code(C_HINGE, e * 16 + 4 * a + b)});
+      }
+    }
+    for (int a = 0; a < 4; ++a)
+      if (vmap[v[a]] != NONE) glist.push_back({vmap[v[a]], code(C_HINGE, e * 4 + a)});
+  }
+  auto to_csr = [](const std::vector<std::pair<uint32_t, uint32_t>> &lst, uint32_t nkeys, std::vector<uint32_t> &ptr, std::vector<uint32_t> &val) {
+    ptr.assign(nkeys + 1, 0);
+    for (auto &p : lst) ptr[p.first + 1]++;
+    for (uint32_t i = 0; i < nkeys; ++i) ptr[i + 1] += ptr[i];
+    val.assign(lst.size(), 0);
+    std::vector<uint32_t> cur(ptr.begin(), ptr.end() - 1);
+    for (auto &p : lst) val[cur[p.first]++] = p.second;  // stable: keeps push order
+  };
+  std::vector<uint32_t> hp, hv, gp, gv;
+  to_csr(hlist, S.M.nslots, hp, hv);
+  S.slot_row.upload(srow, s);
+  S.hc_ptr.upload(hp, s);
+  S.hc.upload(hv.empty() ? std::vector<uint32_t>{0} : hv, s);
+  if (with_grad) {
+    to_csr(glist, nrows, gp, gv);
+    S.gc_ptr.upload(gp, s);
+    S.gc.upload(gv.empty() ? std::vector<uint32_t>{0} : gv, s);
+  }
+  S.row_vert.upload(row_vert.empty() ? std::vector<uint32_t>{0} : row_vert, s);
+}
+
+void upload_elems(ElemSet &S, const ElemHost &H, hipStream_t s) {
+  S.ns = H.ns(); S.nh = H.nh(); S.nr = H.nr(); S.nrh = H.nrh();
+  auto up = [&](auto &buf, const auto &vec) {
+    using T = typename std::decay_t<decltype(vec)>::value_type;
+    if (vec.empty()) buf.reserve(1);
+    else buf.upload(vec.data(), vec.size(), s);
+    (void)sizeof(T);
+  };
+  up(S.sh_v, H.sh_v); up(S.sh_dm, H.sh_dm); up(S.sh_area, H.sh_area);
+  up(S.hg_v, H.hg_v); up(S.hg_rest, H.hg_rest);
+  up(S.rd_v, H.rd_v); up(S.rd_rest, H.rd_rest); up(S.rd_k, H.rd_k);
+  up(S.rh_v, H.rh_v); up(S.rh_rest, H.rh_rest); up(S.rh_stiff, H.rh_stiff);
+  S.out_dense.reserve(81 * ((size_t)S.nrh + S.ns) + 1);
+  S.out_dg.reserve(9 * ((size_t)S.nrh + S.ns) + 1);
+  S.out_rod.reserve(12 * (size_t)S.nr + 1);
+  S.out_hinge.reserve(14 * (size_t)S.nh + 1);
+}
+
+void run_elements(ElemSet &S, const Params &P, const float *x, hipStream_t s) {
+  ElemDev E = S.dev();
+  if (S.nr) hipLaunchKernelGGL(k_rods, dim3(div_up(S.nr, BLOCK)), dim3(BLOCK), 0, s, E, x);
+  if (S.nrh) hipLaunchKernelGGL(k_rod_hinges, dim3(div_up(S.nrh, BLOCK)), dim3(BLOCK), 0, s, E, x);
+  if (S.ns) hipLaunchKernelGGL(k_shells, dim3(div_up(S.ns, BLOCK)), dim3(BLOCK), 0, s, E, P, x);
+  if (S.nh) hipLaunchKernelGGL(k_hinges, dim3(div_up(S.nh, BLOCK)), dim3(BLOCK), 0, s, E, P, x);
+  NGUO_HIP(hipGetLastError());
+}
+
+void gather(ElemSet &S, const Params &P, int active, const float *im, const float *x, const float *xs, hipStream_t s) {
+  if (S.M.nslots == 0) return;
+  hipLaunchKernelGGL(k_gather_hess, dim3(div_up(S.M.nslots, BLOCK)), dim3(BLOCK), 0, s, S.M.nslots, (const uint32_t *)S.slot_row.p,
+                     (const uint32_t *)S.M.col.p, (const uint32_t *)S.hc_ptr.p, (const uint32_t *)S.hc.p, S.dev(), (const uint32_t *)S.row_vert.p,
+                     im, active, P, x, xs, S.M.val.p);
+  NGUO_HIP(hipGetLastError());
+}
+
+Topo topo(nguo_step *st) {
+  Topo T{};
+  T.V = st->V;
+  T.im = st->im.p;
+  T.tris = st->tris.p;
+  T.edges = st->edges.p;
+  T.ex_ptr = st->ex_ptr.p;
+  T.ex = st->ex.p;
+  T.static_v = st->has_collider ? st->static_v.p : nullptr;
+  T.side = st->has_collider ? st->side.p : nullptr;
+  T.pn = st->pn.p;
+  T.vdof = st->vdof.p;
+  T.sdof = st->sdof.p;
+  return T;
+}
+
+ContactView contact_view(nguo_step *st) {
+  ContactView C{};
+  C.ns = st->ns;
+  C.dof = st->cdof.p;
+  C.g = st->cg.p;
+  C.w = st->cw.p;
+  C.coef = st->ccoef.p;
+  C.n = st->cn.p;
+  C.sums = st->csums.p;
+  C.inc_ptr = st->inc_ptr.p;
+  C.inc = st->inc.p;
+  return C;
+}
+
+void ensure_contact_cap(nguo_step *st, uint32_t want) {
+  if (want <= st->cap) return;
+  uint32_t c = std::max<uint32_t>(want, 1u << 14);
+  st->vt_key.reserve(c); st->vt_ord.reserve(c); st->vt_d.reserve(c); st->vt_g.reserve(12 * (size_t)c);
+  st->ee_key.reserve(c); st->ee_ord.reserve(c); st->ee_d.reserve(c); st->ee_g.reserve(12 * (size_t)c);
+  size_t c2 = 2 * (size_t)c;
+  st->sidx.reserve(4 * c2); st->sd.reserve(c2); st->sg.reserve(12 * c2);
+  st->cdof.reserve(4 * c2); st->cg.reserve(12 * c2); st->cw.reserve(4 * c2); st->ccoef.reserve(2 * c2); st->cn.reserve(3 * c2);
+  st->cgrad.reserve(24 * c2); st->csums.reserve(4 * c2); st->ckappa.reserve(c2);
+  st->inc_key.reserve(4 * c2); st->inc_val.reserve(4 * c2); st->inc.reserve(4 * c2);
+  st->cap = c;
+}
+
+uint32_t read_u32(nguo_step *st, const uint32_t *dev) {
+  NGUO_HIP(hipMemcpyAsync(st->hu, dev, 4, hipMemcpyDeviceToHost, st->s));
+  NGUO_HIP(hipStreamSynchronize(st->s));
+  return st->hu[0];
+}
+
+// Active stencils at the current x, sorted by (v, t) then (ei, ej), gathered into sidx/sd/sg.
+void detect(nguo_step *st) {
+  hipStream_t s = st->s;
+  st->ns = st->nvt = st->nee = 0;
+  if (!st->prm.self_collision) return;
+  Topo T = topo(st);
+  float dhat = st->prm.contact_dhat;
+  // dynamic trees at x
+  st->tree_tri_dyn.build(0, st->n_tri_dyn, st->tri_dyn_ids.p, st->tris.p, st->x.p, s);
+  st->tree_edge_dyn.build(1, st->n_edge_dyn, st->edge_dyn_ids.p, st->edges.p, st->x.p, s);
+  if (st->has_collider)
+    hipLaunchKernelGGL(k_sides, dim3(div_up(st->V, BLOCK)), dim3(BLOCK), 0, s, T, st->tree_tri_coll.view(), (const float *)st->x.p, dhat,
+                       st->side.p, st->err.p);
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    NGUO_HIP(hipMemsetAsync(st->vt_count.p, 0, 2 * sizeof(uint32_t), s));
+    StencilOut ovt{st->cap, st->vt_count.p, st->vt_key.p, st->vt_ord.p, st->vt_d.p, st->vt_g.p};
+    StencilOut oee{st->cap, st->vt_count.p + 1, st->ee_key.p, st->ee_ord.p, st->ee_d.p, st->ee_g.p};
+    hipLaunchKernelGGL(k_vt_stencils, dim3(div_up(st->V, BLOCK)), dim3(BLOCK), 0, s, T, st->tree_tri_dyn.view(), st->tree_tri_stat.view(),
+                       (const float *)st->x.p, dhat, ovt, st->err.p);
+    if (st->n_edge_dyn)
+      hipLaunchKernelGGL(k_ee_stencils, dim3(div_up(st->n_edge_dyn, BLOCK)), dim3(BLOCK), 0, s, T, st->tree_edge_dyn.view(),
+                         (const uint32_t *)st->edge_dyn_ids.p, st->n_edge_dyn, 1, (const float *)st->x.p, dhat, dhat, oee, st->err.p);
+    if (st->n_edge_stat && st->n_edge_dyn)
+      hipLaunchKernelGGL(k_ee_stencils, dim3(div_up(st->n_edge_stat, BLOCK)), dim3(BLOCK), 0, s, T, st->tree_edge_dyn.view(),
+                         (const uint32_t *)st->edge_stat_ids.p, st->n_edge_stat, 0, (const float *)st->x.p, dhat, dhat, oee, st->err.p);
+    NGUO_HIP(hipGetLastError());
+    NGUO_HIP(hipMemcpyAsync(st->hu, st->vt_count.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    NGUO_HIP(hipMemcpyAsync(st->hu + 2, st->err.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    NGUO_HIP(hipStreamSynchronize(s));
+    NGUO_REQUIRE(st->hu[2] == 0, NGUO_E_CAPACITY, "LBVH traversal stack overflow");
+    uint32_t nvt = st->hu[0], nee = st->hu[1];
+    if (nvt <= st->cap && nee <= st->cap) { st->nvt = nvt; st->nee = nee; break; }
+    ensure_contact_cap(st, (uint32_t)(std::max(nvt, nee) * 1.5) + 1024);
+    NGUO_REQUIRE(attempt < 3, NGUO_E_CAPACITY, "contact stencil list did not fit after regrowth");
+  }
+  st->ns = st->nvt + st->nee;
+  if (st->ns == 0) return;
+  st->sorter.sort_u64(st->nvt, st->vt_key.p, st->vt_ord.p, digit_shifts(st->F, st->V), s);
+  st->sorter.sort_u64(st->nee, st->ee_key.p, st->ee_ord.p, digit_shifts(st->E, st->E), s);
+  hipLaunchKernelGGL(k_stencil_gather, dim3(div_up(st->ns, BLOCK)), dim3(BLOCK), 0, s, st->nvt, st->nee, (const uint64_t *)st->vt_key.p,
+                     (const uint32_t *)st->vt_ord.p, (const float *)st->vt_d.p, (const float *)st->vt_g.p, (const uint64_t *)st->ee_key.p,
+                     (const uint32_t *)st->ee_ord.p, (const float *)st->ee_d.p, (const float *)st->ee_g.p, (const uint32_t *)st->tris.p,
+                     (const uint32_t *)st->edges.p, st->sidx.p, st->sd.p, st->sg.p);
+  NGUO_HIP(hipGetLastError());
+}
+
+// Detection + per-stencil evaluation + vertex incidence: the contact view of the system.
+void contacts(nguo_step *st) {
+  hipStream_t s = st->s;
+  detect(st);
+  if (st->ns == 0) return;
+  Topo T = topo(st);
+  float dhat = st->prm.contact_dhat;
+  ContactOut CO{st->cdof.p, st->cg.p, st->cw.p, st->ccoef.p, st->cn.p, st->cgrad.p, st->ckappa.p};
+  hipLaunchKernelGGL(k_stencil_eval, dim3(div_up(st->ns, BLOCK)), dim3(BLOCK), 0, s, st->ns, (const uint32_t *)st->sidx.p, (const float *)st->sd.p,
+                     (const float *)st->sg.p, dhat, 0.5f * (dhat + dhat), st->nvt, T, st->act.M.view(), st->stat.M.view(), st->P,
+                     (const float *)st->x.p, (const float *)st->x0.p, CO);
+  uint32_t m = 4 * st->ns;
+  hipLaunchKernelGGL(k_inc_keys, dim3(div_up(m, BLOCK)), dim3(BLOCK), 0, s, st->ns, (const uint32_t *)st->cdof.p, st->NF, st->inc_key.p, st->inc_val.p);
+  st->sorter.sort_u64(m, st->inc_key.p, st->inc_val.p, digit_shifts(m, st->NF + 1), s);
+  hipLaunchKernelGGL(k_inc_ptr, dim3(div_up(std::max(m, st->NF + 1), BLOCK)), dim3(BLOCK), 0, s, st->NF, m, (const uint64_t *)st->inc_key.p,
+                     st->inc_ptr.p, st->inc.p);
+  NGUO_HIP(hipGetLastError());
+}
+
+float reduce_max(nguo_step *st, uint32_t nparts) {
+  hipLaunchKernelGGL(k_max_final, dim3(1), dim3(BLOCK), 0, st->s, nparts, (const float *)st->part.p, st->part.p + st->part.n - 1);
+  NGUO_HIP(hipMemcpyAsync(st->hf, st->part.p + st->part.n - 1, sizeof(float), hipMemcpyDeviceToHost, st->s));
+  NGUO_HIP(hipStreamSynchronize(st->s));
+  return st->hf[0];
+}
+
+// implicit.rs:1690-1947 ccd_max_step.
+float ccd_max_step(nguo_step *st, float max_disp) {
+  hipStream_t s = st->s;
+  bool has_self = st->prm.self_collision != 0;
+  if (st->P.ncoll == 0 && !has_self) return 1.0f;
+  double t = 1.0;
+  Topo T = topo(st);
+  auto run_min = [&](auto launch) -> double {
+    st->hl[0] = (unsigned long long)__builtin_bit_cast(unsigned long long, 1.0);
+    NGUO_HIP(hipMemcpyAsync(st->tmin.p, st->hl, sizeof(unsigned long long), hipMemcpyHostToDevice, s));
+    launch();
+    NGUO_HIP(hipGetLastError());
+    NGUO_HIP(hipMemcpyAsync(st->hl, st->tmin.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    NGUO_HIP(hipStreamSynchronize(s));
+    return __builtin_bit_cast(double, st->hl[0]);
+  };
+  if (st->P.ncoll) {
+    t = std::min(t, run_min([&] {
+      hipLaunchKernelGGL(k_ccd_colliders, dim3(div_up(st->V, BLOCK)), dim3(BLOCK), 0, s, st->V, (const float *)st->im.p, (const float *)st->x.p,
+                         (const float *)st->dx.p, st->P, st->tmin.p);
+    }));
+  }
+  if (has_self && std::isfinite(max_disp)) {
+    uint32_t iters = std::max(st->prm.ccd_max_iter, 1u);
+    float fixed = st->prm.contact_dhat;
+    float allowed = std::max(std::min(0.5f * st->char_len, max_disp), 1.0e-9f);
+    for (int pass = 0; pass < 64; ++pass) {
+      bool full = allowed >= max_disp;
+      float search = fixed + 2.0f * allowed;
+      double t_self = run_min([&] {
+        hipLaunchKernelGGL(k_ccd_vt, dim3(div_up(st->V, BLOCK)), dim3(BLOCK), 0, s, T, st->tree_tri_dyn.view(), st->tree_tri_stat.view(),
+                           (const float *)st->x.p, (const float *)st->dx.p, search, iters, st->tmin.p, st->err.p);
+        if (st->n_edge_dyn)
+          hipLaunchKernelGGL(k_ccd_ee, dim3(div_up(st->n_edge_dyn, BLOCK)), dim3(BLOCK), 0, s, T, st->tree_edge_dyn.view(),
+                             (const uint32_t *)st->edge_dyn_ids.p, st->n_edge_dyn, 1, (const float *)st->x.p, (const float *)st->dx.p, search, iters,
+                             st->tmin.p, st->err.p);
+        if (st->n_edge_stat && st->n_edge_dyn)
+          hipLaunchKernelGGL(k_ccd_ee, dim3(div_up(st->n_edge_stat, BLOCK)), dim3(BLOCK), 0, s, T, st->tree_edge_dyn.view(),
+                             (const uint32_t *)st->edge_stat_ids.p, st->n_edge_stat, 0, (const float *)st->x.p, (const float *)st->dx.p, search,
+                             iters, st->tmin.p, st->err.p);
+      });
+      if (full || (float)t_self * max_disp <= allowed) { t = std::min(t, t_self); break; }
+      allowed = std::min(allowed * 4.0f, max_disp);
+    }
+    NGUO_REQUIRE(read_u32(st, st->err.p) == 0, NGUO_E_CAPACITY, "LBVH traversal stack overflow");
+  }
+  return (float)std::min(std::max(t, 0.0), 1.0);
+}
+
+void assemble_and_solve(nguo_step *st, nguo_stats &stats) {
+  hipStream_t s = st->s;
+  run_elements(st->act, st->P, st->x.p, s);
+  gather(st->act, st->P, 1, st->im.p, st->x.p, st->x0.p, s);
+  if (st->NF)
+    hipLaunchKernelGGL(k_gather_grad, dim3(div_up(st->NF, BLOCK)), dim3(BLOCK), 0, s, st->NF, (const uint32_t *)st->free_list.p,
+                       (const uint32_t *)st->act.gc_ptr.p, (const uint32_t *)st->act.gc.p, st->act.dev(), (const float *)st->im.p, st->P,
+                       (const float *)st->x.p, (const float *)st->xt.p, (const float *)st->x0.p, st->egrad.p);
+  contacts(st);
+  if (st->NF)
+    hipLaunchKernelGGL(k_rhs, dim3(div_up(st->NF, BLOCK)), dim3(BLOCK), 0, s, st->NF, (const float *)st->egrad.p, (const uint32_t *)st->inc_ptr.p,
+                       (const uint32_t *)st->inc.p, (const float *)st->cgrad.p, st->ns > 0 ? 1 : 0, st->rhs.p);
+  NGUO_HIP(hipGetLastError());
+  if (st->NF == 0) { stats = nguo_stats{}; return; }
+  PcgStats ps = st->pcg->solve(st->act.M, contact_view(st), st->rhs.p, st->prm.cg_tol, st->prm.cg_max_iter, st->dxf.p);
+  stats.iters = ps.iters;
+  stats.sweeps = ps.sweeps;
+  stats.residual = ps.residual;
+}
+
+void advance_one(nguo_step *st) {
+  hipStream_t s = st->s;
+  const uint32_t V = st->V;
+  const unsigned GV = div_up(V, BLOCK);
+  float h = st->prm.dt;
+  st->P.inv_h2 = 1.0f / (h * h);
+  hipLaunchKernelGGL(k_predict, dim3(GV), dim3(BLOCK), 0, s, V, (const float *)st->im.p, (const float *)st->x0.p, (const float *)st->v.p, h,
+                     f3{st->prm.gravity[0], st->prm.gravity[1], st->prm.gravity[2]}, st->xt.p, st->x.p);
+  uint32_t cap_it = std::max(st->prm.newton_iters, 1u), taken = 0;
+  st->trace_toi.clear();
+  bool final_step = false;
+  double max_res = 0;
+  uint32_t max_it = 0;
+  float last_t = 1.0f;
+  for (uint32_t it = 0; it < cap_it; ++it) {
+    taken++;
+    nguo_stats ls{};
+    assemble_and_solve(st, ls);
+    max_res = std::max(max_res, ls.residual);
+    max_it = std::max(max_it, ls.iters);
+    hipLaunchKernelGGL(k_scatter_dx, dim3(GV), dim3(BLOCK), 0, s, V, (const uint32_t *)st->vdof.p, (const float *)st->dxf.p, st->dx.p, st->part.p);
+    float max_norm = reduce_max(st, GV);
+    if (max_norm > 0.0f) {
+      float rescale = std::min(st->prm.max_dx / max_norm, 1.0f);
+      if (rescale < 1.0f) {
+        hipLaunchKernelGGL(k_scale, dim3(div_up(3 * (size_t)V, BLOCK)), dim3(BLOCK), 0, s, 3 * V, st->dx.p, rescale);
+        max_norm = max_norm * rescale;
+      }
+    }
+    float t = ccd_max_step(st, max_norm);
+    last_t = t;
+    hipLaunchKernelGGL(k_apply, dim3(GV), dim3(BLOCK), 0, s, V, st->x.p, (const float *)st->dx.p, t, st->part.p);
+    st->trace.reserve(3 * (size_t)V * cap_it);
+    NGUO_HIP(hipMemcpyAsync(st->trace.p + 3 * (size_t)V * it, st->x.p, 3 * (size_t)V * sizeof(float), hipMemcpyDeviceToDevice, s));
+    st->trace_toi.resize(it + 1);
+    st->trace_toi[it] = t;
+    float max_step = reduce_max(st, GV);
+    if (max_step < 1.0e-7f) break;
+    if (final_step) break;
+    if (st->prm.target_toi > 0.0f && t >= st->prm.target_toi && taken > st->prm.min_newton_steps) final_step = true;
+  }
+  NGUO_HIP(hipMemsetAsync(st->flag.p, 0, 2 * sizeof(uint32_t), s));
+  hipLaunchKernelGGL(k_finite, dim3(GV), dim3(BLOCK), 0, s, V, (const float *)st->x.p, st->flag.p);
+  hipLaunchKernelGGL(k_commit, dim3(GV), dim3(BLOCK), 0, s, V, (const float *)st->im.p, st->x.p, st->x0.p, st->v.p, 1.0f / h, st->prm.floor_y,
+                     (const uint32_t *)st->flag.p, st->part.p);
+  float vmax = reduce_max(st, GV);
+  uint32_t bad = read_u32(st, st->flag.p);
+  st->time += (double)h;
+  // penetration_free: finite, analytic colliders respected, no edge pierces a triangle
+  uint32_t hit = 0;
+  if (st->prm.self_collision && !bad) {
+    st->tree_tri_dyn.build(0, st->n_tri_dyn, st->tri_dyn_ids.p, st->tris.p, st->x.p, s);
+    Topo T = topo(st);
+    hipLaunchKernelGGL(k_pierce, dim3(div_up(st->E, BLOCK)), dim3(BLOCK), 0, s, T, st->tree_tri_dyn.view(), st->tree_tri_stat.view(), st->E,
+                       (const float *)st->x.p, st->flag.p + 1, st->err.p);
+    NGUO_HIP(hipGetLastError());
+    hit = read_u32(st, st->flag.p + 1);
+  }
+  nguo_step_metrics &m = st->metrics;
+  m.time = st->time;
+  m.max_reresid = max_res;
+  m.max_cg_iters = max_it;
+  m.newton_iters = taken;
+  m.num_stencils = st->ns;
+  m.max_velocity = vmax;
+  m.all_finite = bad ? 0 : 1;
+  m.penetration_free = (!bad && !hit) ? 1 : 0;
+  m.last_toi = last_t;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nguo_step_create(nguo_ctx *ctx, const nguo_scene_desc *sc, nguo_step **out) {
+  return guarded([&]() -> int {
+    NGUO_REQUIRE(ctx && sc && out, NGUO_E_INVALID, "nguo_step_create: null argument");
+    NGUO_REQUIRE(sc->n_verts > 0 && sc->positions && sc->inv_mass, NGUO_E_INVALID, "nguo_step_create: empty scene");
+    NGUO_HIP(hipSetDevice(ctx->device));
+    std::unique_ptr<nguo_step> st(new nguo_step());
+    st->ctx = ctx;
+    st->s = ctx->stream;
+    hipStream_t s = st->s;
+    const uint32_t V = sc->n_verts;
+    st->V = V;
+    st->prm = sc->params;
+    NGUO_REQUIRE(sc->n_colliders <= 4, NGUO_E_UNSUPPORTED, "at most 4 analytic colliders");
+    // validation (implicit.rs:826-838 style)
+    for (uint32_t i = 0; i < 3 * sc->n_tris; ++i) NGUO_REQUIRE(sc->tris[i] < V, NGUO_E_INVALID, "triangle index out of range");
+    for (uint32_t i = 0; i < 3 * sc->n_shell_faces; ++i) NGUO_REQUIRE(sc->shell_faces[i] < V, NGUO_E_INVALID, "shell face index out of range");
+    for (uint32_t r = 0; r < sc->n_rods; ++r) {
+      uint32_t a = sc->rod_verts[2 * r], b = sc->rod_verts[2 * r + 1];
+      NGUO_REQUIRE(a < V && b < V && a != b, NGUO_E_INVALID, "rod out of range or degenerate");
+      NGUO_REQUIRE(sc->rod_rest[r] > 0.0f && sc->rod_k[r] > 0.0f, NGUO_E_INVALID, "rod rest length and stiffness must be > 0");
+    }
+    for (uint32_t h = 0; h < sc->n_rod_hinges; ++h) {
+      const uint32_t *v = sc->rod_hinge_verts + 3 * h;
+      NGUO_REQUIRE(v[0] < V && v[1] < V && v[2] < V, NGUO_E_INVALID, "rod hinge out of range");
+      NGUO_REQUIRE(v[0] != v[1] && v[1] != v[2] && v[0] != v[2], NGUO_E_INVALID, "rod hinge repeats a vertex");
+    }
+    st->h_im.assign(sc->inv_mass, sc->inv_mass + V);
+    st->h_vdof.assign(V, NONE);
+    for (uint32_t i = 0; i < V; ++i)
+      if (st->h_im[i] > 0.0f) { st->h_vdof[i] = (uint32_t)st->h_free.size(); st->h_free.push_back(i); }
+    st->NF = (uint32_t)st->h_free.size();
+    st->F = sc->n_tris;
+    st->h_tris.assign(sc->tris, sc->tris + 3 * (size_t)sc->n_tris);
+    // Params
+    Params &P = st->P;
+    P.shell_model = st->prm.shell_model;
+    P.shell_mu = st->prm.shell_mu;
+    P.shell_lambda = st->prm.shell_lambda;
+    P.eig_eps = st->prm.eiganalysis_eps;
+    P.kb = st->prm.bend_stiffness;
+    P.dhat = st->prm.contact_dhat;
+    P.kappa = st->prm.contact_kappa;
+    P.mu = st->prm.friction;
+    P.feps = st->prm.friction_eps;
+    P.barrier_kind = st->prm.barrier_kind;
+    P.ncoll = sc->n_colliders;
+    P.inv_h2 = 1.0f / (st->prm.dt * st->prm.dt);
+    for (uint32_t c = 0; c < sc->n_colliders; ++c) {
+      P.coll[c] = sc->colliders[c];
+      if (P.coll[c].kind == NGUO_COLLIDER_HALFSPACE) {  // scene.rs:24-27 normalize_or_zero
+        f3 nn{P.coll[c].b[0], P.coll[c].b[1], P.coll[c].b[2]};
+        nn = normalize_or_zero(nn);
+        P.coll[c].b[0] = nn.x; P.coll[c].b[1] = nn.y; P.coll[c].b[2] = nn.z;
+      }
+    }
+    const float *X = sc->positions;
+    auto pinned = [&](uint32_t v) { return !(st->h_im[v] > 0.0f); };
+    // elements, split into active (all free) and static (all pinned); mixed ones are not
+    // representable in the free-vertex system's gather
+    ElemHost ha, hs;
+    auto classify = [&](const uint32_t *v, int k) -> int {
+      int np = 0;
+      for (int a = 0; a < k; ++a) np += pinned(v[a]) ? 1 : 0;
+      if (np == 0) return 0;
+      if (np == k) return 1;
+      return 2;
+    };
+    if (!sc->shell_faces || !(st->prm.shell_mu > 0.0f || st->prm.shell_lambda > 0.0f)) {
+      // shells contribute nothing (implicit.rs:2608-2611)
+    } else {
+      for (uint32_t f = 0; f < sc->n_shell_faces; ++f) {
+        const uint32_t *t = sc->shell_faces + 3 * f;
+        float dm[4], area;
+        if (!shell_rest(X + 3 * t[0], X + 3 * t[1], X + 3 * t[2], dm, area)) continue;
+        int c = classify(t, 3);
+        NGUO_REQUIRE(c != 2, NGUO_E_UNSUPPORTED, "a shell face mixes pinned and free vertices");
+        ElemHost &H = c == 0 ? ha : hs;
+        H.sh_v.insert(H.sh_v.end(), t, t + 3);
+        H.sh_dm.insert(H.sh_dm.end(), dm, dm + 4);
+        H.sh_area.push_back(area);
+      }
+    }
+    if (st->prm.bend_stiffness > 0.0f) {
+      // bending.rs:128-158 build_hinges over all triangles
+      std::vector<std::pair<uint64_t, uint32_t>> ek;  // (edge key, opposite) in face order
+      ek.reserve(3 * (size_t)st->F);
+      for (uint32_t f = 0; f < st->F; ++f) {
+        const uint32_t *t = &st->h_tris[3 * f];
+        uint32_t tri[3][3] = {{t[1], t[2], t[0]}, {t[0], t[2], t[1]}, {t[0], t[1], t[2]}};
+        for (auto &e : tri) {
+          uint32_t a = std::min(e[0], e[1]), b = std::max(e[0], e[1]);
+          ek.push_back({((uint64_t)a << 32) | b, e[2]});
+        }
+      }
+      std::stable_sort(ek.begin(), ek.end(), [](const auto &p, const auto &q) { return p.first < q.first; });
+      for (size_t i = 0; i < ek.size();) {
+        size_t j = i;
+        while (j < ek.size() && ek[j].first == ek[i].first) ++j;
+        if (j - i == 2) {
+          uint32_t v[4] = {ek[i].second, (uint32_t)(ek[i].first >> 32), (uint32_t)ek[i].first, ek[i + 1].second};
+          int c = classify(v, 4);
+          NGUO_REQUIRE(c != 2, NGUO_E_UNSUPPORTED, "a bending hinge mixes pinned and free vertices");
+          ElemHost &H = c == 0 ? ha : hs;
+          H.hg_v.insert(H.hg_v.end(), v, v + 4);
+          H.hg_rest.push_back(dihedral_host(X + 3 * v[0], X + 3 * v[1], X + 3 * v[2], X + 3 * v[3]));
+        }
+        i = j;
+      }
+    }
+    for (uint32_t r = 0; r < sc->n_rods; ++r) {
+      const uint32_t *v = sc->rod_verts + 2 * r;
+      int c = classify(v, 2);
+      NGUO_REQUIRE(c != 2, NGUO_E_UNSUPPORTED, "a rod segment mixes pinned and free vertices");
+      ElemHost &H = c == 0 ? ha : hs;
+      H.rd_v.insert(H.rd_v.end(), v, v + 2);
+      H.rd_rest.push_back(sc->rod_rest[r]);
+      H.rd_k.push_back(sc->rod_k[r]);
+    }
+    for (uint32_t h = 0; h < sc->n_rod_hinges; ++h) {
+      if (!(sc->rod_hinge_stiff[h] > 0.0f)) continue;
+      const uint32_t *v = sc->rod_hinge_verts + 3 * h;
+      int c = classify(v, 3);
+      NGUO_REQUIRE(c != 2, NGUO_E_UNSUPPORTED, "a rod bend mixes pinned and free vertices");
+      ElemHost &H = c == 0 ? ha : hs;
+      H.rh_v.insert(H.rh_v.end(), v, v + 3);
+      H.rh_rest.push_back(sc->rod_hinge_rest[h]);
+      H.rh_stiff.push_back(sc->rod_hinge_stiff[h]);
+    }
+    // static rows: pinned vertices touched by static elements
+    std::vector<uint32_t> sdof(V, NONE), srow_vert;
+    auto touch = [&](const std::vector<uint32_t> &vv) {
+      for (uint32_t v : vv)
+        if (sdof[v] == NONE) { sdof[v] = (uint32_t)srow_vert.size(); srow_vert.push_back(v); }
+    };
+    touch(hs.rd_v); touch(hs.rh_v); touch(hs.sh_v); touch(hs.hg_v);
+    // device uploads
+    st->x.upload(X, 3 * (size_t)V, s);
+    st->x0.upload(X, 3 * (size_t)V, s);
+    if (sc->velocities) st->v.upload(sc->velocities, 3 * (size_t)V, s);
+    else st->v.zero(3 * (size_t)V, s);
+    st->xt.reserve(3 * (size_t)V);
+    st->dx.reserve(3 * (size_t)V);
+    st->im.upload(st->h_im, s);
+    st->vdof.upload(st->h_vdof, s);
+    st->sdof.upload(sdof, s);
+    st->free_list.upload(st->h_free.empty() ? std::vector<uint32_t>{0} : st->h_free, s);
+    st->dxf.reserve(3 * (size_t)std::max(st->NF, 1u));
+    st->rhs.reserve(3 * (size_t)std::max(st->NF, 1u));
+    st->egrad.reserve(3 * (size_t)std::max(st->NF, 1u));
+    st->part.reserve(div_up(V, BLOCK) + 2);
+    st->err.zero(1, s);
+    st->flag.zero(4, s);
+    st->tmin.reserve(1);
+    st->vt_count.zero(2, s);
+    st->inc_ptr.zero((size_t)st->NF + 2, s);
+    upload_elems(st->act, ha, s);
+    upload_elems(st->stat, hs, s);
+    build_gather(st->act, ha, st->NF, st->h_vdof, st->h_free, true, s);
+    build_gather(st->stat, hs, (uint32_t)srow_vert.size(), sdof, srow_vert, false, s);
+    // static element blocks once (the body never moves)
+    if (st->stat.M.nb) {
+      run_elements(st->stat, P, st->x.p, s);
+      gather(st->stat, P, 0, st->im.p, st->x.p, st->x0.p, s);
+    }
+    // collision topology: unique edges (+ rods), 1-ring exclusion, trees
+    std::vector<std::pair<uint32_t, uint32_t>> es;
+    for (uint32_t f = 0; f < st->F; ++f) {
+      const uint32_t *t = &st->h_tris[3 * f];
+      for (int k = 0; k < 3; ++k) {
+        uint32_t i = t[k], j = t[(k + 1) % 3];
+        es.push_back({std::min(i, j), std::max(i, j)});
+      }
+    }
+    std::sort(es.begin(), es.end());
+    es.erase(std::unique(es.begin(), es.end()), es.end());
+    std::vector<std::pair<uint32_t, uint32_t>> seen = es;
+    for (auto &e : es) { st->h_edges.push_back(e.first); st->h_edges.push_back(e.second); }
+    for (uint32_t r = 0; r < sc->n_rods; ++r) {
+      uint32_t a = sc->rod_verts[2 * r], b = sc->rod_verts[2 * r + 1];
+      auto key = std::make_pair(std::min(a, b), std::max(a, b));
+      auto it = std::lower_bound(seen.begin(), seen.end(), key);
+      if (it != seen.end() && *it == key) continue;
+      seen.insert(it, key);
+      st->h_edges.push_back(a);
+      st->h_edges.push_back(b);
+    }
+    st->E = (uint32_t)(st->h_edges.size() / 2);
+    {
+      double sum = 0;
+      float fsum = 0.0f;
+      for (uint32_t e = 0; e < st->E; ++e) {
+        const float *a = X + 3 * st->h_edges[2 * e], *b = X + 3 * st->h_edges[2 * e + 1];
+        f3 d{a[0] - b[0], a[1] - b[1], a[2] - b[2]};
+        fsum += std::sqrt(dot(d, d));
+      }
+      (void)sum;
+      st->char_len = st->E ? std::max(fsum / (float)st->E, 1e-6f) : 1.0f;
+    }
+    std::vector<std::vector<uint32_t>> exl(V);
+    for (uint32_t i = 0; i < V; ++i) exl[i].push_back(i);
+    for (uint32_t f = 0; f < st->F; ++f) {
+      const uint32_t *t = &st->h_tris[3 * f];
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+          if (a != b) exl[t[a]].push_back(t[b]);
+    }
+    std::vector<uint32_t> exp(V + 1, 0), exv;
+    for (uint32_t i = 0; i < V; ++i) {
+      std::sort(exl[i].begin(), exl[i].end());
+      exl[i].erase(std::unique(exl[i].begin(), exl[i].end()), exl[i].end());
+      exp[i + 1] = exp[i] + (uint32_t)exl[i].size();
+      exv.insert(exv.end(), exl[i].begin(), exl[i].end());
+    }
+    st->tris.upload(st->h_tris.empty() ? std::vector<uint32_t>{0, 0, 0} : st->h_tris, s);
+    st->edges.upload(st->h_edges.empty() ? std::vector<uint32_t>{0, 0} : st->h_edges, s);
+    st->ex_ptr.upload(exp, s);
+    st->ex.upload(exv, s);
+    // static mesh collider (implicit.rs:1258-1322): winding check, pseudonormals
+    std::vector<uint32_t> coll_ids;
+    if (sc->static_collider) {
+      float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+      for (uint32_t f = 0; f < st->F; ++f) {
+        const uint32_t *t = &st->h_tris[3 * f];
+        if (!(sc->static_collider[t[0]] && sc->static_collider[t[1]] && sc->static_collider[t[2]])) continue;
+        coll_ids.push_back(f);
+        for (int k = 0; k < 3; ++k)
+          for (int c = 0; c < 3; ++c) { lo[c] = std::fmin(lo[c], X[3 * t[k] + c]); hi[c] = std::fmax(hi[c], X[3 * t[k] + c]); }
+      }
+      if (!coll_ids.empty()) {
+        f3 mid{0.5f * (lo[0] + hi[0]), 0.5f * (lo[1] + hi[1]), 0.5f * (lo[2] + hi[2])};
+        double box = (double)((hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]));
+        double vol = 0;
+        for (uint32_t f : coll_ids) {
+          const uint32_t *t = &st->h_tris[3 * f];
+          f3 a = f3{X[3 * t[0]], X[3 * t[0] + 1], X[3 * t[0] + 2]} - mid;
+          f3 b = f3{X[3 * t[1]], X[3 * t[1] + 1], X[3 * t[1] + 2]} - mid;
+          f3 c = f3{X[3 * t[2]], X[3 * t[2] + 1], X[3 * t[2] + 2]} - mid;
+          vol += (double)dot(cross(a, b), c) / 6.0;
+        }
+        if (vol < -0.05 * box) coll_ids.clear();  // wound inward: unsigned contact only
+      }
+    }
+    st->has_collider = !coll_ids.empty();
+    std::vector<float> pn(3 * (size_t)V, 0.0f);
+    if (st->has_collider) {
+      st->h_static.assign(sc->static_collider, sc->static_collider + V);
+      // contact.rs:943-981 pseudonormals: welded on exact bits (-0 folded), angle-weighted
+      std::vector<std::pair<std::array<uint32_t, 3>, uint32_t>> keys(V);
+      auto bits = [](float c) { uint32_t u; float z = c == 0.0f ? 0.0f : c; std::memcpy(&u, &z, 4); return u; };
+      for (uint32_t i = 0; i < V; ++i) keys[i] = {{bits(X[3 * i]), bits(X[3 * i + 1]), bits(X[3 * i + 2])}, i};
+      std::sort(keys.begin(), keys.end());
+      std::vector<uint32_t> of(V);
+      uint32_t nw = 0;
+      for (uint32_t i = 0; i < V; ++i) {
+        if (i > 0 && keys[i].first != keys[i - 1].first) nw++;
+        of[keys[i].second] = nw;
+      }
+      std::vector<f3> acc(nw + 1, f3{0, 0, 0});
+      for (uint32_t f : coll_ids) {
+        const uint32_t *t = &st->h_tris[3 * f];
+        f3 p[3];
+        for (int k = 0; k < 3; ++k) p[k] = f3{X[3 * t[k]], X[3 * t[k] + 1], X[3 * t[k] + 2]};
+        f3 face = normalize_or_zero(cross(p[1] - p[0], p[2] - p[0]));
+        if (face.x == 0.0f && face.y == 0.0f && face.z == 0.0f) continue;
+        for (int k = 0; k < 3; ++k) {
+          f3 u = normalize_or_zero(p[(k + 1) % 3] - p[k]);
+          f3 w = normalize_or_zero(p[(k + 2) % 3] - p[k]);
+          acc[of[t[k]]] = acc[of[t[k]]] + face * std::acos(clampv(dot(u, w), -1.0f, 1.0f));
+        }
+      }
+      for (uint32_t i = 0; i < V; ++i) {
+        f3 n = normalize_or_zero(acc[of[i]]);
+        pn[3 * i] = n.x; pn[3 * i + 1] = n.y; pn[3 * i + 2] = n.z;
+      }
+      st->static_v.upload(st->h_static, s);
+    } else {
+      st->static_v.zero(V, s);
+    }
+    st->pn.upload(pn, s);
+    st->side.reserve(V);
+    {
+      std::vector<float> ones(V, 1.0f);
+      st->side.upload(ones, s);
+    }
+    // tree membership
+    std::vector<uint32_t> tdyn, tstat, edyn, estat;
+    for (uint32_t f = 0; f < st->F; ++f) {
+      const uint32_t *t = &st->h_tris[3 * f];
+      (pinned(t[0]) && pinned(t[1]) && pinned(t[2]) ? tstat : tdyn).push_back(f);
+    }
+    for (uint32_t e = 0; e < st->E; ++e) (pinned(st->h_edges[2 * e]) && pinned(st->h_edges[2 * e + 1]) ? estat : edyn).push_back(e);
+    auto upl = [&](DevBuf<uint32_t> &b, const std::vector<uint32_t> &v, uint32_t &n) {
+      n = (uint32_t)v.size();
+      b.upload(v.empty() ? std::vector<uint32_t>{0} : v, s);
+    };
+    upl(st->tri_dyn_ids, tdyn, st->n_tri_dyn);
+    upl(st->tri_stat_ids, tstat, st->n_tri_stat);
+    upl(st->tri_coll_ids, coll_ids, st->n_tri_coll);
+    upl(st->edge_dyn_ids, edyn, st->n_edge_dyn);
+    upl(st->edge_stat_ids, estat, st->n_edge_stat);
+    st->tree_tri_stat.build(0, st->n_tri_stat, st->tri_stat_ids.p, st->tris.p, st->x.p, s);
+    if (st->has_collider) st->tree_tri_coll.build(0, st->n_tri_coll, st->tri_coll_ids.p, st->tris.p, st->x.p, s);
+    ensure_contact_cap(st.get(), std::max<uint32_t>(4 * V, 1u << 16));
+    st->pcg.reset(new Pcg(s));
+    NGUO_HIP(hipHostMalloc((void **)&st->hf, 16 * sizeof(float), hipHostMallocDefault));
+    NGUO_HIP(hipHostMalloc((void **)&st->hu, 16 * sizeof(uint32_t), hipHostMallocDefault));
+    NGUO_HIP(hipHostMalloc((void **)&st->hl, 4 * sizeof(unsigned long long), hipHostMallocDefault));
+    NGUO_HIP(hipStreamSynchronize(s));
+    *out = st.release();
+    return NGUO_OK;
+  });
+}
+
+void nguo_step_destroy(nguo_step *st) { delete st; }
+
+int nguo_step_advance(nguo_step *st, uint32_t nsteps) {
+  return guarded([&]() -> int {
+    NGUO_REQUIRE(st, NGUO_E_INVALID, "nguo_step_advance: null handle");
+    for (uint32_t k = 0; k < nsteps; ++k) advance_one(st);
+    NGUO_HIP(hipStreamSynchronize(st->s));
+    return NGUO_OK;
+  });
+}
+
+int nguo_step_get_positions(nguo_step *st, float *out) {
+  return guarded([&]() -> int {
+    NGUO_REQUIRE(st && out, NGUO_E_INVALID, "nguo_step_get_positions: null argument");
+    st->x0.download(out, 3 * (size_t)st->V, st->s);
+    NGUO_HIP(hipStreamSynchronize(st->s));
+    return NGUO_OK;
+  });
+}
+
+int nguo_step_get_velocities(nguo_step *st, float *out) {
+  return guarded([&]() -> int {
+    NGUO_REQUIRE(st && out, NGUO_E_INVALID, "nguo_step_get_velocities: null argument");
+    st->v.download(out, 3 * (size_t)st->V, st->s);
+    NGUO_HIP(hipStreamSynchronize(st->s));
+    return NGUO_OK;
+  });
+}
+
+int nguo_step_set_state(nguo_step *st, const float *positions, const float *velocities) {
+  return guarded([&]() -> int {
+    NGUO_REQUIRE(st && positions, NGUO_E_INVALID, "nguo_step_set_state: null argument");
+    st->x0.upload(positions, 3 * (size_t)st->V, st->s);
+    st->x.upload(positions, 3 * (size_t)st->V, st->s);
+    if (velocities) st->v.upload(velocities, 3 * (size_t)st->V, st->s);
+    NGUO_HIP(hipStreamSynchronize(st->s));
+    return NGUO_OK;
+  });
+}
+
+int nguo_step_get_newton_trace(nguo_step *st, uint32_t k, float *out, uint32_t *count, float *toi) {
+  return guarded([&]() -> int {
+    NGUO_REQUIRE(st, NGUO_E_INVALID, "nguo_step_get_newton_trace: null handle");
+    uint32_t n = (uint32_t)st->trace_toi.size();
+    if (count) *count = n;
+    if (!out) return NGUO_OK;
+    NGUO_REQUIRE(k < n, NGUO_E_INVALID, "nguo_step_get_newton_trace: iteration index out of range");
+    NGUO_HIP(hipMemcpyAsync(out, st->trace.p + 3 * (size_t)st->V * k, 3 * (size_t)st->V * sizeof(float), hipMemcpyDeviceToHost, st->s));
+    NGUO_HIP(hipStreamSynchronize(st->s));
+    if (toi) *toi = st->trace_toi[k];
+    return NGUO_OK;
+  });
+}
+
+int nguo_step_stencils(nguo_step *st, const float *x, uint32_t cap, uint32_t *idx, float *d, float *g, float *kappa, uint32_t *count) {
+  return guarded([&]() -> int {
+    NGUO_REQUIRE(st && x && count, NGUO_E_INVALID, "nguo_step_stencils: null argument");
+    hipStream_t s = st->s;
+    st->x.upload(x, 3 * (size_t)st->V, s);
+    run_elements(st->act, st->P, st->x.p, s);
+    gather(st->act, st->P, 1, st->im.p, st->x.p, st->x0.p, s);
+    contacts(st);
+    *count = st->ns;
+    uint32_t m = std::min(cap, st->ns);
+    if (m) {
+      if (idx) st->sidx.download(idx, 4 * (size_t)m, s);
+      if (d) st->sd.download(d, m, s);
+      if (g) st->sg.download(g, 12 * (size_t)m, s);
+      if (kappa) st->ckappa.download(kappa, m, s);
+    }
+    NGUO_HIP(hipMemcpyAsync(st->x.p, st->x0.p, 3 * (size_t)st->V * sizeof(float), hipMemcpyDeviceToDevice, s));
+    NGUO_HIP(hipStreamSynchronize(s));
+    return NGUO_OK;
+  });
+}
+
+int nguo_step_collider_sides(nguo_step *st, const float *x, float *side) {
+  return guarded([&]() -> int {
+    NGUO_REQUIRE(st && x && side, NGUO_E_INVALID, "nguo_step_collider_sides: null argument");
+    hipStream_t s = st->s;
+    if (!st->has_collider) {
+      std::fill(side, side + st->V, 1.0f);
+      return NGUO_OK;
+    }
+    st->x.upload(x, 3 * (size_t)st->V, s);
+    hipLaunchKernelGGL(k_sides, dim3(div_up(st->V, BLOCK)), dim3(BLOCK), 0, s, topo(st), st->tree_tri_coll.view(), (const float *)st->x.p,
+                       st->prm.contact_dhat, st->side.p, st->err.p);
+    NGUO_HIP(hipGetLastError());
+    st->side.download(side, st->V, s);
+    NGUO_HIP(hipMemcpyAsync(st->x.p, st->x0.p, 3 * (size_t)st->V * sizeof(float), hipMemcpyDeviceToDevice, s));
+    NGUO_HIP(hipStreamSynchronize(s));
+    return NGUO_OK;
+  });
+}
+
+int nguo_step_bench_spmv(nguo_step *st, uint32_t reps, double *avg_ms, double *bytes) {
+  return guarded([&]() -> int {
+    NGUO_REQUIRE(st && avg_ms && bytes && reps > 0, NGUO_E_INVALID, "nguo_step_bench_spmv: bad argument");
+    hipStream_t s = st->s;
+    if (st->NF == 0) { *avg_ms = 0; *bytes = 0; return NGUO_OK; }
+    // the system of the current state (elastic blocks + contact term) and a probe vector
+    run_elements(st->act, st->P, st->x.p, s);
+    gather(st->act, st->P, 1, st->im.p, st->x.p, st->x0.p, s);
+    contacts(st);
+    ContactView C = contact_view(st);
+    float *xin = st->egrad.p, *yout = st->rhs.p;
+    NGUO_HIP(hipMemcpyAsync(xin, st->x.p, 3 * (size_t)st->NF * sizeof(float), hipMemcpyDeviceToDevice, s));
+    st->pcg->apply(st->act.M.view(), C, xin, yout);  // warm-up
+    hipEvent_t e0, e1;
+    NGUO_HIP(hipEventCreate(&e0));
+    NGUO_HIP(hipEventCreate(&e1));
+    NGUO_HIP(hipEventRecord(e0, s));
+    for (uint32_t r = 0; r < reps; ++r) st->pcg->apply(st->act.M.view(), C, xin, yout);
+    NGUO_HIP(hipEventRecord(e1, s));
+    NGUO_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    NGUO_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_ms = (double)ms / reps;
+    *bytes = st->act.M.spmv_bytes();
+    return NGUO_OK;
+  });
+}
+
+int nguo_step_get_metrics(nguo_step *st, nguo_step_metrics *out) {
+  if (!st || !out) return fail(NGUO_E_INVALID, "nguo_step_get_metrics: null argument");
+  *out = st->metrics;
+  return NGUO_OK;
+}
+
+}  // extern "C"

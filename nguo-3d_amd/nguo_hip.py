@@ -26,6 +26,9 @@ EXPORTED = [
     "nguo_step_create", "nguo_step_destroy", "nguo_step_advance", "nguo_step_get_positions",
     "nguo_step_get_velocities", "nguo_step_set_state", "nguo_step_get_metrics",
     "nguo_scene_generate", "nguo_scene_view", "nguo_scene_free",
+    "nguo_step_get_newton_trace", "nguo_step_stencils", "nguo_step_collider_sides",
+    "nguo_lbvh_morton", "nguo_lbvh_sort_pairs", "nguo_lbvh_karras", "nguo_lbvh_aabb",
+    "nguo_step_bench_spmv", "nguo_bench_spmv_grid",
 ]
 
 
@@ -114,6 +117,16 @@ def load():
         lib.nguo_step_get_velocities.argtypes = [vp, P(C.c_float)]
         lib.nguo_step_set_state.argtypes = [vp, P(C.c_float), P(C.c_float)]
         lib.nguo_step_get_metrics.argtypes = [vp, P(StepMetrics)]
+        lib.nguo_step_get_newton_trace.argtypes = [vp, C.c_uint32, P(C.c_float), P(C.c_uint32), P(C.c_float)]
+        lib.nguo_step_stencils.argtypes = [vp, P(C.c_float), C.c_uint32, P(C.c_uint32), P(C.c_float), P(C.c_float),
+                                           P(C.c_float), P(C.c_uint32)]
+        lib.nguo_step_collider_sides.argtypes = [vp, P(C.c_float), P(C.c_float)]
+        lib.nguo_lbvh_morton.argtypes = [vp, C.c_uint32, P(C.c_float), C.c_uint32, P(C.c_uint32), P(C.c_uint32)]
+        lib.nguo_lbvh_sort_pairs.argtypes = [vp, C.c_uint32, P(C.c_uint32), P(C.c_uint32)]
+        lib.nguo_lbvh_karras.argtypes = [vp, C.c_uint32, P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
+        lib.nguo_lbvh_aabb.argtypes = [vp, C.c_uint32, P(C.c_uint32), P(C.c_uint32), P(C.c_float), P(C.c_float)]
+        lib.nguo_step_bench_spmv.argtypes = [vp, C.c_uint32, P(C.c_double), P(C.c_double)]
+        lib.nguo_bench_spmv_grid.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32, P(C.c_double), P(C.c_double), P(C.c_double)]
     _lib = lib
     return lib
 
@@ -143,10 +156,11 @@ class Scene:
         d = self.desc
         n = d.n_verts
         self.n_verts = n
-        self.positions = np.ctypeslib.as_array(d.positions, shape=(n, 3))
-        self.inv_mass = np.ctypeslib.as_array(d.inv_mass, shape=(n,))
-        self.tris = np.ctypeslib.as_array(d.tris, shape=(d.n_tris, 3))
-        self.static = np.ctypeslib.as_array(d.static_collider, shape=(n,))
+        # copies: a numpy view must not outlive the native buffer it points into
+        self.positions = np.ctypeslib.as_array(d.positions, shape=(n, 3)).copy()
+        self.inv_mass = np.ctypeslib.as_array(d.inv_mass, shape=(n,)).copy()
+        self.tris = np.ctypeslib.as_array(d.tris, shape=(d.n_tris, 3)).copy()
+        self.static = np.ctypeslib.as_array(d.static_collider, shape=(n,)).copy()
         self.params = d.params
 
     def __del__(self):
@@ -206,6 +220,46 @@ class LinearSolver:
             self.h = None
 
 
+def lbvh_morton(ctx, pos, tris):
+    pos = np.ascontiguousarray(pos, dtype=np.float32)
+    tris = np.ascontiguousarray(tris, dtype=np.uint32)
+    codes = np.zeros(len(tris), np.uint32)
+    check(load().nguo_lbvh_morton(ctx.h, len(pos), fptr(pos), len(tris), uptr(tris), uptr(codes)))
+    return codes
+
+
+def lbvh_sort_pairs(ctx, keys, vals):
+    keys = np.ascontiguousarray(keys, dtype=np.uint32).copy()
+    vals = np.ascontiguousarray(vals, dtype=np.uint32).copy()
+    check(load().nguo_lbvh_sort_pairs(ctx.h, len(keys), uptr(keys), uptr(vals)))
+    return keys, vals
+
+
+def lbvh_karras(ctx, codes):
+    codes = np.ascontiguousarray(codes, dtype=np.uint32)
+    n = len(codes)
+    left = np.zeros(max(n - 1, 1), np.uint32)
+    right = np.zeros(max(n - 1, 1), np.uint32)
+    check(load().nguo_lbvh_karras(ctx.h, n, uptr(codes), uptr(left), uptr(right)))
+    return left[: max(n - 1, 0)], right[: max(n - 1, 0)]
+
+
+def lbvh_aabb(ctx, left, right, leaf_box):
+    left = np.ascontiguousarray(left, dtype=np.uint32)
+    right = np.ascontiguousarray(right, dtype=np.uint32)
+    leaf_box = np.ascontiguousarray(leaf_box, dtype=np.float32)
+    n = len(leaf_box)
+    node = np.zeros((max(n - 1, 1), 6), np.float32)
+    check(load().nguo_lbvh_aabb(ctx.h, n, uptr(left), uptr(right), fptr(leaf_box), fptr(node)))
+    return node[: max(n - 1, 0)]
+
+
+def bench_spmv_grid(ctx, nx, ny, reps=20):
+    ms, by, nnzb = C.c_double(), C.c_double(), C.c_double()
+    check(load().nguo_bench_spmv_grid(ctx.h, nx, ny, reps, C.byref(ms), C.byref(by), C.byref(nnzb)))
+    return ms.value, by.value, nnzb.value
+
+
 class Stepper:
     """The whole implicit step (GpuImplicitSolver / ImplicitSolver::step) on the device."""
 
@@ -236,6 +290,44 @@ class Stepper:
         m = StepMetrics()
         check(load().nguo_step_get_metrics(self.h, C.byref(m)))
         return m
+
+    def newton_trace(self):
+        """Positions after every Newton iteration of the last step, and their CCD fractions."""
+        cnt = C.c_uint32()
+        check(load().nguo_step_get_newton_trace(self.h, 0, None, C.byref(cnt), None))
+        xs, tois = [], []
+        for k in range(cnt.value):
+            out = np.zeros((self.n, 3), dtype=np.float32)
+            t = C.c_float()
+            check(load().nguo_step_get_newton_trace(self.h, k, fptr(out), None, C.byref(t)))
+            xs.append(out)
+            tois.append(t.value)
+        return xs, tois
+
+    def stencils(self, x):
+        """Active contact stencils at x: (idx [n,4], d [n], g [n,12], kappa [n])."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        cnt = C.c_uint32()
+        check(load().nguo_step_stencils(self.h, fptr(x), 0, None, None, None, None, C.byref(cnt)))
+        n = cnt.value
+        idx = np.zeros((max(n, 1), 4), np.uint32)
+        d = np.zeros(max(n, 1), np.float32)
+        g = np.zeros((max(n, 1), 12), np.float32)
+        k = np.zeros(max(n, 1), np.float32)
+        check(load().nguo_step_stencils(self.h, fptr(x), n, uptr(idx), fptr(d), fptr(g), fptr(k), C.byref(cnt)))
+        return idx[:n], d[:n], g[:n], k[:n]
+
+    def bench_spmv(self, reps=200):
+        """(average ms per launch, algorithmic bytes per launch) of the PCG operator."""
+        ms, by = C.c_double(), C.c_double()
+        check(load().nguo_step_bench_spmv(self.h, int(reps), C.byref(ms), C.byref(by)))
+        return ms.value, by.value
+
+    def collider_sides(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        side = np.zeros(self.n, np.float32)
+        check(load().nguo_step_collider_sides(self.h, fptr(x), fptr(side)))
+        return side
 
     def close(self):
         if self.h:

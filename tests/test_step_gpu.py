@@ -1,0 +1,115 @@
+"""The resident implicit step on the device against the oracle (engine-core/src/implicit.rs
+ImplicitSolver::step restated in oracle/): per-step parity from identical inputs, bit-exact
+sorted contact stencil lists, collider sides, closed-form free fall, invariants."""
+import numpy as np
+import pytest
+
+from conftest import nguo_hip
+import oracle_py as O
+
+POS_TOL = 1e-5  # |x_gpu - x_oracle|_inf / max(1, |x|_inf) after each Newton iteration / step
+
+
+def rel_err(a, b):
+    return np.abs(a - b).max() / max(1.0, np.abs(b).max())
+
+
+def make(ctx, cfg):
+    s = nguo_hip.Scene(cfg, seed=cfg)
+    return s, nguo_hip.Stepper(ctx, s)
+
+
+@pytest.mark.gpu
+def test_free_fall_closed_form(ctx):
+    s = nguo_hip.Scene(100, seed=0)
+    s.desc.params.self_collision = 0
+    st = nguo_hip.Stepper(ctx, s)
+    x0 = st.positions().astype(np.float64)
+    st.advance(1)
+    x1 = st.positions().astype(np.float64)
+    h = s.params.dt
+    free = s.inv_mass > 0
+    want = x0.copy()
+    want[free, 1] += h * h * -9.8
+    assert np.abs(x1 - want).max() < 2e-6
+    m = st.metrics()
+    assert m.all_finite == 1 and m.max_reresid <= 1e-6
+    st.close()
+
+
+def oracle_state(cfg, steps):
+    s = nguo_hip.Scene(cfg, seed=cfg)
+    o = O.OracleSolver(s)
+    if steps:
+        o.step(steps)
+    return s, o
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cfg,steps", [(100, 30), (0, 16), (0, 30), (101, 25)])
+def test_stencils_bitexact(ctx, cfg, steps):
+    s, o = oracle_state(cfg, steps)
+    x = o.positions()
+    idx, d, g = o.stencils(x)
+    assert len(idx) > 0, "the state must be in contact"
+    st = nguo_hip.Stepper(ctx, s)
+    st.set_state(x, o.velocities())
+    gidx, gd, gg, gk = st.stencils(x)
+    assert np.array_equal(gidx, idx)
+    assert np.array_equal(gd, d)  # same FP32 operation order: bitwise equal gaps
+    assert np.array_equal(gg, g)
+    assert np.all(np.isfinite(gk)) and np.all(gk > 0)
+    side = o.collider_sides(x)
+    assert np.array_equal(st.collider_sides(x), side)
+    st.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cfg,steps", [(100, 0), (100, 12), (100, 30), (0, 12), (0, 20), (0, 33), (101, 25)])
+def test_one_step_parity(ctx, cfg, steps):
+    s, o = oracle_state(cfg, steps)
+    x, v = o.positions(), o.velocities()
+    st = nguo_hip.Stepper(ctx, s)
+    st.set_state(x, v)
+    o.step(1)
+    st.advance(1)
+    ox, otoi = o.newton_trace()
+    gx, gtoi = st.newton_trace()
+    assert len(gx) == len(ox), (len(gx), len(ox))
+    for k in range(len(ox)):
+        assert rel_err(gx[k], ox[k]) <= POS_TOL, (k, rel_err(gx[k], ox[k]))
+    assert rel_err(st.positions(), o.positions()) <= POS_TOL
+    m = st.metrics()
+    assert m.all_finite == 1 and m.penetration_free == 1
+    assert m.max_reresid <= 1e-6
+    st.close()
+
+
+@pytest.mark.gpu
+def test_sheet_drape_invariants_36_steps(ctx):
+    s, st = make(ctx, 0)
+    for k in range(36):
+        st.advance(1)
+        m = st.metrics()
+        assert m.all_finite == 1 and m.penetration_free == 1, k
+        assert m.max_reresid <= 1e-6
+    x = st.positions()
+    free = s.inv_mass > 0
+    assert x[free, 1].min() < 0.0  # draped over the sphere
+    assert np.all(st.collider_sides(x) == 1)
+    st.close()
+
+
+@pytest.mark.gpu
+def test_tube_garment_step_parity_from_device_state(ctx):
+    s, st = make(ctx, 1)
+    st.advance(6)
+    x, v = st.positions(), st.velocities()
+    o = O.OracleSolver(s)
+    o.set_state(x, v)
+    o.step(1)
+    st.advance(1)
+    assert rel_err(st.positions(), o.positions()) <= POS_TOL
+    m = st.metrics()
+    assert m.all_finite == 1 and m.penetration_free == 1 and m.max_reresid <= 1e-6
+    st.close()
