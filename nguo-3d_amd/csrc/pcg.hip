@@ -242,6 +242,83 @@ __global__ __launch_bounds__(BLOCK) void k_pupdate(uint32_t nb, const float *__r
   p[i] = z[i] + beta * p[i];
 }
 
+// Fused iteration (small systems, <= FUSE_MAX_PARTS workgroups): every workgroup reduces the
+// few partials it needs itself, in the same fixed order, so the scalars never round-trip
+// through a single-workgroup kernel. Partials of r.z / r.r alternate between two buffers by
+// iteration parity: `cur` holds rz_i, the update writes rz_{i+1} into `nxt`.
+constexpr uint32_t FUSE_MAX_PARTS = 1024;
+
+__device__ inline double reduce_parts_bcast(const double *part, uint32_t n, double *lds, double *bcast) {
+  double s = reduce_parts(part, n, lds);
+  if (threadIdx.x == 0) *bcast = s;
+  __syncthreads();
+  double v = *bcast;
+  __syncthreads();
+  return v;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_update_fused(uint32_t nb, const float *__restrict__ p, const float *__restrict__ q,
+                                                       float *__restrict__ d, float *__restrict__ r, float *__restrict__ z,
+                                                       const float *__restrict__ minv, const double *pap_part, const double *cur,
+                                                       double *nxt, double *scal, uint32_t *flag, uint32_t npart) {
+  __shared__ double lds[4];
+  __shared__ double bc;
+  if (flag[0]) return;
+  double pap = reduce_parts_bcast(pap_part, npart, lds, &bc);
+  double rz = reduce_parts_bcast(cur, npart, lds, &bc);
+  if (!(pap > 0.0)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) flag[0] = 2;
+    return;
+  }
+  float alpha = (float)(rz / pap);
+  if (blockIdx.x == 0 && threadIdx.x == 0) { scal[1] = pap; scal[2] = rz / pap; }
+  uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  double rzn = 0, rr = 0;
+  if (i < nb) {
+    size_t o = 3 * (size_t)i;
+    float r0 = r[o] - alpha * q[o], r1 = r[o + 1] - alpha * q[o + 1], r2 = r[o + 2] - alpha * q[o + 2];
+    d[o] += alpha * p[o]; d[o + 1] += alpha * p[o + 1]; d[o + 2] += alpha * p[o + 2];
+    r[o] = r0; r[o + 1] = r1; r[o + 2] = r2;
+    const float *m = minv + 9 * (size_t)i;
+    float z0 = m[0] * r0 + m[1] * r1 + m[2] * r2;
+    float z1 = m[3] * r0 + m[4] * r1 + m[5] * r2;
+    float z2 = m[6] * r0 + m[7] * r1 + m[8] * r2;
+    z[o] = z0; z[o + 1] = z1; z[o + 2] = z2;
+    rzn = (double)r0 * z0 + (double)r1 * z1 + (double)r2 * z2;
+    rr = (double)r0 * r0 + (double)r1 * r1 + (double)r2 * r2;
+  }
+  double a = block_sum(rzn, lds);
+  double b = block_sum(rr, lds);
+  if (threadIdx.x == 0) { nxt[blockIdx.x] = a; nxt[npart + blockIdx.x] = b; }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_pupdate_fused(uint32_t nb, const float *__restrict__ z, float *__restrict__ p, const double *cur,
+                                                        const double *nxt, double *scal, uint32_t *flag, uint32_t npart) {
+  __shared__ double lds[4];
+  __shared__ double bc;
+  if (flag[0]) return;
+  double rz_old = reduce_parts_bcast(cur, npart, lds, &bc);
+  double rz_new = reduce_parts_bcast(nxt, npart, lds, &bc);
+  double rr = reduce_parts_bcast(nxt + npart, npart, lds, &bc);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    uint32_t it = ++flag[1];
+    scal[0] = rz_new;
+    scal[3] = rz_new / rz_old;
+    scal[4] = rr;
+    if (rr <= scal[5]) flag[0] = 1;
+    else if (it >= flag[2]) flag[0] = 3;
+  }
+  if (rr <= scal[5]) return;  // converged: p is no longer needed
+  float beta = (float)(rz_new / rz_old);
+  uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i < nb) {
+    size_t o = 3 * (size_t)i;
+    p[o] = z[o] + beta * p[o];
+    p[o + 1] = z[o + 1] + beta * p[o + 1];
+    p[o + 2] = z[o + 2] + beta * p[o + 2];
+  }
+}
+
 // Start of a sweep: r = (f32) r64, d = 0, z = Minv r, p = z; partials of r.z and r.r.
 __global__ __launch_bounds__(BLOCK) void k_init(uint32_t nb, const double *__restrict__ r64, float *__restrict__ r, float *__restrict__ d,
                                                float *__restrict__ z, float *__restrict__ p, const float *__restrict__ minv, double *part) {
@@ -361,7 +438,7 @@ void Pcg::ensure(uint32_t nb, uint32_t ns) {
     r_.reserve(n3); z_.reserve(n3); p_.reserve(n3); q_.reserve(n3); d_.reserve(n3); bf_.reserve(n3);
     minv_.reserve(9 * (size_t)nb);
     x64_.reserve(n3); r64_.reserve(n3);
-    part_.reserve(2 * (size_t)div_up(n3, BLOCK) + 8);
+    part_.reserve(6 * (size_t)div_up(n3, BLOCK) + 8);  // pAp, two parity buffers of (rz, rr), spares
     cap_nb_ = nb;
   }
   scal_.reserve(8);
@@ -405,9 +482,29 @@ PcgStats Pcg::solve(const SellMatrix &M, const ContactView &C, const float *b, f
     double res = std::sqrt(rr / bb);
     if (res <= (double)tol || st.iters >= max_iter) break;
     double eta = std::min(0.5, std::max(0.25 * (double)tol / res, 1.0e-5));
-    hipLaunchKernelGGL(k_init, dim3(G), dim3(BLOCK), 0, s_, nb, (const double *)r64_.p, r_.p, d_.p, z_.p, p_.p, (const float *)minv_.p, part_.p);
-    hipLaunchKernelGGL(k_init_scal, dim3(1), dim3(BLOCK), 0, s_, scal_.p, flag_.p, (const double *)part_.p, G, eta * eta, max_iter - st.iters);
+    const bool fused = G <= FUSE_MAX_PARTS;
+    double *pap_part = part_.p, *bufA = part_.p + G, *bufB = part_.p + 3 * (size_t)G;
+    double *init_part = fused ? bufA : part_.p;
+    hipLaunchKernelGGL(k_init, dim3(G), dim3(BLOCK), 0, s_, nb, (const double *)r64_.p, r_.p, d_.p, z_.p, p_.p, (const float *)minv_.p, init_part);
+    hipLaunchKernelGGL(k_init_scal, dim3(1), dim3(BLOCK), 0, s_, scal_.p, flag_.p, (const double *)init_part, G, eta * eta, max_iter - st.iters);
+    uint64_t launched = 0;
     while (true) {
+      if (fused) {
+        for (uint32_t c = 0; c < chunk; ++c, ++launched) {
+          double *cur = (launched & 1) ? bufB : bufA, *nxt = (launched & 1) ? bufA : bufB;
+          if (C.ns) hipLaunchKernelGGL((k_contact_sums<float, float>), dim3(GS), dim3(BLOCK), 0, s_, C, (const float *)p_.p, C.sums, (const uint32_t *)flag_.p);
+          hipLaunchKernelGGL(k_spmv, dim3(G), dim3(BLOCK), 0, s_, A, C, (const float *)p_.p, q_.p, pap_part, (const uint32_t *)flag_.p);
+          hipLaunchKernelGGL(k_update_fused, dim3(G), dim3(BLOCK), 0, s_, nb, (const float *)p_.p, (const float *)q_.p, d_.p, r_.p, z_.p,
+                             (const float *)minv_.p, (const double *)pap_part, (const double *)cur, nxt, scal_.p, flag_.p, G);
+          hipLaunchKernelGGL(k_pupdate_fused, dim3(G), dim3(BLOCK), 0, s_, nb, (const float *)z_.p, p_.p, (const double *)cur, (const double *)nxt,
+                             scal_.p, flag_.p, G);
+        }
+        NGUO_HIP(hipGetLastError());
+        NGUO_HIP(hipMemcpyAsync(h_flag_, flag_.p, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, s_));
+        NGUO_HIP(hipStreamSynchronize(s_));
+        if (h_flag_[0]) break;
+        continue;
+      }
       for (uint32_t c = 0; c < chunk; ++c) {
         if (C.ns) hipLaunchKernelGGL((k_contact_sums<float, float>), dim3(GS), dim3(BLOCK), 0, s_, C, (const float *)p_.p, C.sums, (const uint32_t *)flag_.p);
         hipLaunchKernelGGL(k_spmv, dim3(G), dim3(BLOCK), 0, s_, A, C, (const float *)p_.p, q_.p, part_.p, (const uint32_t *)flag_.p);

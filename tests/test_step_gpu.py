@@ -37,8 +37,10 @@ def test_free_fall_closed_form(ctx):
     st.close()
 
 
-def oracle_state(cfg, steps):
+def oracle_state(cfg, steps, cg_tol=None):
     s = nguo_hip.Scene(cfg, seed=cfg)
+    if cg_tol is not None:
+        s.desc.params.cg_tol = cg_tol
     o = O.OracleSolver(s)
     if steps:
         o.step(steps)
@@ -64,10 +66,13 @@ def test_stencils_bitexact(ctx, cfg, steps):
     st.close()
 
 
+# Both sides solve each Newton system to 1e-10 relative residual here, so the comparison
+# measures the implementation (assembly, contact, CCD) and not the solver tolerance: at the
+# production 1e-6 two correct solvers already differ by up to cond(H) * 1e-6 in dx.
 @pytest.mark.gpu
 @pytest.mark.parametrize("cfg,steps", [(100, 0), (100, 12), (100, 30), (0, 12), (0, 20), (0, 33), (101, 25)])
 def test_one_step_parity(ctx, cfg, steps):
-    s, o = oracle_state(cfg, steps)
+    s, o = oracle_state(cfg, steps, cg_tol=1e-10)
     x, v = o.positions(), o.velocities()
     st = nguo_hip.Stepper(ctx, s)
     st.set_state(x, v)
@@ -81,7 +86,21 @@ def test_one_step_parity(ctx, cfg, steps):
     assert rel_err(st.positions(), o.positions()) <= POS_TOL
     m = st.metrics()
     assert m.all_finite == 1 and m.penetration_free == 1
-    assert m.max_reresid <= 1e-6
+    assert m.max_reresid <= 1e-9
+    st.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cfg,steps", [(0, 33), (101, 25)])
+def test_one_step_parity_production_tolerance(ctx, cfg, steps):
+    # at the workload's 1e-6: L2-relative position agreement after the step
+    s, o = oracle_state(cfg, steps)
+    st = nguo_hip.Stepper(ctx, s)
+    st.set_state(o.positions(), o.velocities())
+    o.step(1)
+    st.advance(1)
+    a, b = st.positions().astype(np.float64), o.positions().astype(np.float64)
+    assert np.linalg.norm(a - b) / np.linalg.norm(b) <= POS_TOL
     st.close()
 
 
@@ -102,7 +121,9 @@ def test_sheet_drape_invariants_36_steps(ctx):
 
 @pytest.mark.gpu
 def test_tube_garment_step_parity_from_device_state(ctx):
-    s, st = make(ctx, 1)
+    s = nguo_hip.Scene(1, seed=1)
+    s.desc.params.cg_tol = 1e-10
+    st = nguo_hip.Stepper(ctx, s)
     st.advance(6)
     x, v = st.positions(), st.velocities()
     o = O.OracleSolver(s)
@@ -111,5 +132,5 @@ def test_tube_garment_step_parity_from_device_state(ctx):
     st.advance(1)
     assert rel_err(st.positions(), o.positions()) <= POS_TOL
     m = st.metrics()
-    assert m.all_finite == 1 and m.penetration_free == 1 and m.max_reresid <= 1e-6
+    assert m.all_finite == 1 and m.penetration_free == 1 and m.max_reresid <= 1e-9
     st.close()
