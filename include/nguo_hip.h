@@ -77,6 +77,8 @@ typedef struct nguo_params {
   float cg_tol;           /* relative residual ||b-Ax||/||b|| of each Newton solve */
   uint32_t cg_max_iter;
   float floor_y;          /* -INFINITY = no floor */
+  int32_t precond;        /* NGUO_PRECOND_*: 3x3 block Jacobi or one-level Schwarz */
+  uint32_t schwarz_domain;/* vertices per Schwarz domain (1..32) */
 } nguo_params;
 
 /* A scene in the solver's vertex space (engine-api/src/scene.rs:89-152 `lower` output plus
@@ -147,6 +149,9 @@ int nguo_ls_solve(nguo_ls *ls, uint32_t n, const uint32_t *row_ptr, const uint32
                   const float *values, const float *b, float tol, uint32_t max_iter,
                   float *x_out, nguo_stats *stats);
 int nguo_ls_last_stats(nguo_ls *ls, nguo_stats *out);
+/* z = M^-1 r with the solver's preconditioner built on this matrix (parity checks). */
+int nguo_ls_precond_apply(nguo_ls *ls, uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx,
+                          const float *values, const float *r, float *z);
 /* y = A x through the device 3x3 BSR SpMV (the PCG's operator), for parity checks. */
 int nguo_ls_spmv(nguo_ls *ls, uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx,
                  const float *values, const float *x, float *y);

@@ -28,6 +28,8 @@ struct nguo_ls {
   std::vector<uint32_t> last_row_ptr, last_col;
   std::vector<uint32_t> val_slot;  // scalar CSR entry -> value index in the SELL planes
   nguo::DevBuf<float> b, x, y;
+  uint32_t domain = 32;
+  std::unique_ptr<nguo::Schwarz> sz;
   std::vector<float> h_val;
   nguo_stats last{};
 };
@@ -66,6 +68,11 @@ void load_csr(nguo_ls *ls, uint32_t n, const uint32_t *row_ptr, const uint32_t *
       }
     ls->last_row_ptr.assign(row_ptr, row_ptr + n + 1);
     ls->last_col.assign(col_idx, col_idx + nnz);
+    if (ls->precond == NGUO_PRECOND_SCHWARZ) {
+      ls->sz.reset(new Schwarz());
+      ls->sz->build(ls->mat, ls->domain, s);
+      ls->pcg->set_schwarz(ls->sz.get());
+    }
   }
   ls->h_val.assign((size_t)ls->mat.nslots * 9, 0.0f);
   for (uint32_t k = 0; k < nnz; ++k) ls->h_val[ls->val_slot[k]] += values[k];
@@ -122,11 +129,13 @@ int nguo_ls_create(nguo_ctx *ctx, int precond, uint32_t schwarz_domain, float sc
   (void)schwarz_weight;
   return guarded([&]() -> int {
     NGUO_REQUIRE(ctx && out, NGUO_E_INVALID, "nguo_ls_create: null argument");
-    NGUO_REQUIRE(precond == NGUO_PRECOND_BLOCK, NGUO_E_UNSUPPORTED,
-                 "nguo_ls_create: only the 3x3 block-Jacobi preconditioner is available on this path");
+    NGUO_REQUIRE(precond == NGUO_PRECOND_BLOCK || precond == NGUO_PRECOND_SCHWARZ, NGUO_E_INVALID, "nguo_ls_create: unknown preconditioner");
+    NGUO_REQUIRE(precond != NGUO_PRECOND_SCHWARZ || (schwarz_domain >= 1 && schwarz_domain <= SCHWARZ_MAX_DOMAIN), NGUO_E_UNSUPPORTED,
+                 "nguo_ls_create: Schwarz domains hold 1..32 vertices");
     auto *ls = new nguo_ls();
     ls->ctx = ctx;
     ls->precond = precond;
+    ls->domain = schwarz_domain;
     ls->pcg.reset(new Pcg(ctx->stream));
     *out = ls;
     return NGUO_OK;
@@ -160,6 +169,24 @@ int nguo_ls_last_stats(nguo_ls *ls, nguo_stats *out) {
   if (!ls || !out) return fail(NGUO_E_INVALID, "nguo_ls_last_stats: null argument");
   *out = ls->last;
   return NGUO_OK;
+}
+
+int nguo_ls_precond_apply(nguo_ls *ls, uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx, const float *values, const float *r,
+                          float *z) {
+  return guarded([&]() -> int {
+    NGUO_REQUIRE(ls && row_ptr && col_idx && values && r && z, NGUO_E_INVALID, "nguo_ls_precond_apply: null argument");
+    NGUO_REQUIRE(n % 3 == 0 && n > 0, NGUO_E_UNSUPPORTED, "nguo_ls_precond_apply: n must be a positive multiple of 3");
+    NGUO_HIP(hipSetDevice(ls->ctx->device));
+    hipStream_t s = ls->ctx->stream;
+    load_csr(ls, n, row_ptr, col_idx, values);
+    ls->b.upload(r, n, s);
+    ls->y.reserve(n);
+    ls->pcg->build_precond(ls->mat, no_contact());
+    ls->pcg->apply_precond(ls->mat, ls->b.p, ls->y.p);
+    ls->y.download(z, n, s);
+    NGUO_HIP(hipStreamSynchronize(s));
+    return NGUO_OK;
+  });
 }
 
 int nguo_ls_spmv(nguo_ls *ls, uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx, const float *values, const float *x, float *y) {

@@ -25,11 +25,32 @@ struct SellMatrix {
   double spmv_bytes() const { return (double)nnzb * 40.0 + (double)nb * 28.0 + 4.0; }
 };
 
+// One-level additive Schwarz over disjoint vertex domains (engine-core/src/linsolve.rs:126-258
+// SchwarzTopology): greedy BFS aggregation of the block graph into domains of `domain`
+// vertices (at most 32, so a domain block is at most 96 x 96); per solve, one workgroup per
+// domain assembles its dense block (elastic blocks + contact terms) in LDS, adds the
+// diagonal floor 1e-6 tr/m + 1e-8, factors it in FP64 and writes the explicit FP32 inverse
+// A_d^-1 = L^-T L^-1; the apply is one dense 96 x 96 product per domain.
+constexpr uint32_t SCHWARZ_MAX_DOMAIN = 32;
+constexpr uint32_t SCHWARZ_MAX_DOF = 3 * SCHWARZ_MAX_DOMAIN;
+
+struct Schwarz {
+  uint32_t nd = 0, domain = SCHWARZ_MAX_DOMAIN;
+  std::vector<uint32_t> h_dom_ptr, h_dom_row;
+  DevBuf<uint32_t> dom_ptr, dom_row, row_dom, row_pos, fails;
+  DevBuf<float> ainv;  // nd blocks of SCHWARZ_MAX_DOF^2, row-major with stride SCHWARZ_MAX_DOF
+  void build(const SellMatrix &M, uint32_t domain_size, hipStream_t s);
+};
+
 struct PcgStats {
   uint32_t iters = 0, sweeps = 0;
   double residual = 0;
   bool breakdown = false;
 };
+
+void launch_schwarz_init(hipStream_t s, const Schwarz &sz, const double *r64, float *r, float *z, float *dsol, float *p, double *part);
+void launch_schwarz_update(hipStream_t s, const Schwarz &sz, float *r, float *z, float *dsol, float *p, const float *q, const double *pap_part,
+                           uint32_t npap, const double *cur, double *nxt, double *scal, uint32_t *flag);
 
 class Pcg {
  public:
@@ -42,11 +63,18 @@ class Pcg {
   void apply(const SellView &A, const ContactView &C, const float *x, float *y);
   // Block-Jacobi inverse of (A + C)'s diagonal blocks (FP64 inversion, scalar fallback).
   void build_precond(const SellMatrix &M, const ContactView &C);
+  // Use `sz` (built over the same matrix pattern) instead of 3x3 block Jacobi; null = Jacobi.
+  void set_schwarz(Schwarz *sz) { sz_ = sz; }
+  Schwarz *schwarz() const { return sz_; }
+  // z = M^-1 r with the current preconditioner (after build_precond), for parity checks.
+  void apply_precond(const SellMatrix &M, const float *r, float *z);
   uint32_t chunk = 8;  // iterations per host convergence check
 
  private:
   void ensure(uint32_t nb, uint32_t ns);
+  void launch_minv(const SellMatrix &M, const ContactView &C);
   hipStream_t s_;
+  Schwarz *sz_ = nullptr;
   uint32_t cap_nb_ = 0, cap_ns_ = 0;
   DevBuf<float> r_, z_, p_, q_, d_, minv_, bf_;
   DevBuf<double> x64_, r64_, part_, scal_, sums64_;

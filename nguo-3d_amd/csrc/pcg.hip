@@ -246,7 +246,7 @@ __global__ __launch_bounds__(BLOCK) void k_pupdate(uint32_t nb, const float *__r
 // few partials it needs itself, in the same fixed order, so the scalars never round-trip
 // through a single-workgroup kernel. Partials of r.z / r.r alternate between two buffers by
 // iteration parity: `cur` holds rz_i, the update writes rz_{i+1} into `nxt`.
-constexpr uint32_t FUSE_MAX_PARTS = 1024;
+constexpr uint32_t FUSE_MAX_PARTS = 4096;
 
 __device__ inline double reduce_parts_bcast(const double *part, uint32_t n, double *lds, double *bcast) {
   double s = reduce_parts(part, n, lds);
@@ -429,6 +429,11 @@ Pcg::~Pcg() {
 }
 
 void Pcg::ensure(uint32_t nb, uint32_t ns) {
+  {
+    // pAp partials (G), then two parity buffers of (r.z, r.r) partials of max(G, domains) each
+    size_t G = div_up(nb, BLOCK), nz = std::max<size_t>(G, sz_ ? sz_->nd : 0);
+    part_.reserve(G + 4 * nz + 2 * (size_t)div_up(3 * (size_t)nb, BLOCK) + 8);
+  }
   if (!h_flag_) {
     NGUO_HIP(hipHostMalloc((void **)&h_flag_, 4 * sizeof(uint32_t), hipHostMallocDefault));
     NGUO_HIP(hipHostMalloc((void **)&h_scal_, 8 * sizeof(double), hipHostMallocDefault));
@@ -438,7 +443,6 @@ void Pcg::ensure(uint32_t nb, uint32_t ns) {
     r_.reserve(n3); z_.reserve(n3); p_.reserve(n3); q_.reserve(n3); d_.reserve(n3); bf_.reserve(n3);
     minv_.reserve(9 * (size_t)nb);
     x64_.reserve(n3); r64_.reserve(n3);
-    part_.reserve(6 * (size_t)div_up(n3, BLOCK) + 8);  // pAp, two parity buffers of (rz, rr), spares
     cap_nb_ = nb;
   }
   scal_.reserve(8);
@@ -449,9 +453,29 @@ void Pcg::ensure(uint32_t nb, uint32_t ns) {
   }
 }
 
-void Pcg::build_precond(const SellMatrix &M, const ContactView &C) {
-  ensure(M.nb, C.ns);
+void Pcg::launch_minv(const SellMatrix &M, const ContactView &C) {
   hipLaunchKernelGGL(k_minv, dim3(div_up(M.nb, BLOCK)), dim3(BLOCK), 0, s_, M.view(), C, M.diag_slot.p, minv_.p);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_bj_apply(uint32_t nb, const float *minv, const float *r, float *z) {
+  uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= nb) return;
+  const float *m = minv + 9 * (size_t)i;
+  size_t o = 3 * (size_t)i;
+  float r0 = r[o], r1 = r[o + 1], r2 = r[o + 2];
+  z[o] = m[0] * r0 + m[1] * r1 + m[2] * r2;
+  z[o + 1] = m[3] * r0 + m[4] * r1 + m[5] * r2;
+  z[o + 2] = m[6] * r0 + m[7] * r1 + m[8] * r2;
+}
+
+void Pcg::apply_precond(const SellMatrix &M, const float *r, float *z) {
+  const uint32_t nb = M.nb, n3 = 3 * nb;
+  if (sz_) {
+    hipLaunchKernelGGL(k_f32_to_f64, dim3(div_up(n3, BLOCK)), dim3(BLOCK), 0, s_, n3, r, r64_.p);
+    launch_schwarz_init(s_, *sz_, r64_.p, r_.p, z, d_.p, p_.p, part_.p);
+  } else {
+    hipLaunchKernelGGL(k_bj_apply, dim3(div_up(nb, BLOCK)), dim3(BLOCK), 0, s_, nb, (const float *)minv_.p, r, z);
+  }
   NGUO_HIP(hipGetLastError());
 }
 
@@ -482,13 +506,31 @@ PcgStats Pcg::solve(const SellMatrix &M, const ContactView &C, const float *b, f
     double res = std::sqrt(rr / bb);
     if (res <= (double)tol || st.iters >= max_iter) break;
     double eta = std::min(0.5, std::max(0.25 * (double)tol / res, 1.0e-5));
-    const bool fused = G <= FUSE_MAX_PARTS;
-    double *pap_part = part_.p, *bufA = part_.p + G, *bufB = part_.p + 3 * (size_t)G;
+    const uint32_t nz = sz_ ? sz_->nd : G;  // partials of r.z / r.r: per domain or per row block
+    const bool fused = G <= FUSE_MAX_PARTS && nz <= FUSE_MAX_PARTS;
+    NGUO_REQUIRE(fused || !sz_, NGUO_E_UNSUPPORTED, "Schwarz preconditioner: too many domains for the fused iteration");
+    double *pap_part = part_.p, *bufA = part_.p + G, *bufB = part_.p + G + 2 * (size_t)std::max(G, nz);
     double *init_part = fused ? bufA : part_.p;
-    hipLaunchKernelGGL(k_init, dim3(G), dim3(BLOCK), 0, s_, nb, (const double *)r64_.p, r_.p, d_.p, z_.p, p_.p, (const float *)minv_.p, init_part);
-    hipLaunchKernelGGL(k_init_scal, dim3(1), dim3(BLOCK), 0, s_, scal_.p, flag_.p, (const double *)init_part, G, eta * eta, max_iter - st.iters);
+    if (sz_) launch_schwarz_init(s_, *sz_, r64_.p, r_.p, z_.p, d_.p, p_.p, init_part);
+    else hipLaunchKernelGGL(k_init, dim3(G), dim3(BLOCK), 0, s_, nb, (const double *)r64_.p, r_.p, d_.p, z_.p, p_.p, (const float *)minv_.p, init_part);
+    hipLaunchKernelGGL(k_init_scal, dim3(1), dim3(BLOCK), 0, s_, scal_.p, flag_.p, (const double *)init_part, nz, eta * eta, max_iter - st.iters);
     uint64_t launched = 0;
     while (true) {
+      if (fused && sz_) {
+        for (uint32_t c = 0; c < chunk; ++c, ++launched) {
+          double *cur = (launched & 1) ? bufB : bufA, *nxt = (launched & 1) ? bufA : bufB;
+          if (C.ns) hipLaunchKernelGGL((k_contact_sums<float, float>), dim3(GS), dim3(BLOCK), 0, s_, C, (const float *)p_.p, C.sums, (const uint32_t *)flag_.p);
+          hipLaunchKernelGGL(k_spmv, dim3(G), dim3(BLOCK), 0, s_, A, C, (const float *)p_.p, q_.p, pap_part, (const uint32_t *)flag_.p);
+          launch_schwarz_update(s_, *sz_, r_.p, z_.p, d_.p, p_.p, q_.p, pap_part, G, cur, nxt, scal_.p, flag_.p);
+          hipLaunchKernelGGL(k_pupdate_fused, dim3(G), dim3(BLOCK), 0, s_, nb, (const float *)z_.p, p_.p, (const double *)cur, (const double *)nxt,
+                             scal_.p, flag_.p, nz);
+        }
+        NGUO_HIP(hipGetLastError());
+        NGUO_HIP(hipMemcpyAsync(h_flag_, flag_.p, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, s_));
+        NGUO_HIP(hipStreamSynchronize(s_));
+        if (h_flag_[0]) break;
+        continue;
+      }
       if (fused) {
         for (uint32_t c = 0; c < chunk; ++c, ++launched) {
           double *cur = (launched & 1) ? bufB : bufA, *nxt = (launched & 1) ? bufA : bufB;

@@ -219,6 +219,8 @@ nguo_params default_params() {
   p.cg_tol = 1.0e-6f;
   p.cg_max_iter = 10000;
   p.floor_y = -INFINITY;
+  p.precond = NGUO_PRECOND_SCHWARZ;
+  p.schwarz_domain = 32;
   return p;
 }
 

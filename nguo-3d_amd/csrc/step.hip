@@ -828,6 +828,7 @@ struct nguo_step {
   uint32_t ns = 0, nvt = 0, nee = 0;
   RadixSort sorter;
   std::unique_ptr<Pcg> pcg;
+  std::unique_ptr<Schwarz> sz;
   // host scratch (pinned)
   float *hf = nullptr;
   uint32_t *hu = nullptr;
@@ -1576,6 +1577,14 @@ int nguo_step_create(nguo_ctx *ctx, const nguo_scene_desc *sc, nguo_step **out) 
     if (st->has_collider) st->tree_tri_coll.build(0, st->n_tri_coll, st->tri_coll_ids.p, st->tris.p, st->x.p, s);
     ensure_contact_cap(st.get(), std::max<uint32_t>(4 * V, 1u << 16));
     st->pcg.reset(new Pcg(s));
+    NGUO_REQUIRE(st->prm.precond == NGUO_PRECOND_BLOCK || st->prm.precond == NGUO_PRECOND_SCHWARZ, NGUO_E_INVALID, "unknown preconditioner");
+    if (st->prm.precond == NGUO_PRECOND_SCHWARZ && st->NF) {
+      NGUO_REQUIRE(st->prm.schwarz_domain >= 1 && st->prm.schwarz_domain <= SCHWARZ_MAX_DOMAIN, NGUO_E_UNSUPPORTED,
+                   "Schwarz domains hold 1..32 vertices");
+      st->sz.reset(new Schwarz());
+      st->sz->build(st->act.M, st->prm.schwarz_domain, s);
+      st->pcg->set_schwarz(st->sz.get());
+    }
     NGUO_HIP(hipHostMalloc((void **)&st->hf, 16 * sizeof(float), hipHostMallocDefault));
     NGUO_HIP(hipHostMalloc((void **)&st->hu, 16 * sizeof(uint32_t), hipHostMallocDefault));
     NGUO_HIP(hipHostMalloc((void **)&st->hl, 4 * sizeof(unsigned long long), hipHostMallocDefault));

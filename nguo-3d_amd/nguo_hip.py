@@ -28,7 +28,7 @@ EXPORTED = [
     "nguo_scene_generate", "nguo_scene_view", "nguo_scene_free",
     "nguo_step_get_newton_trace", "nguo_step_stencils", "nguo_step_collider_sides",
     "nguo_lbvh_morton", "nguo_lbvh_sort_pairs", "nguo_lbvh_karras", "nguo_lbvh_aabb",
-    "nguo_step_bench_spmv", "nguo_bench_spmv_grid",
+    "nguo_step_bench_spmv", "nguo_bench_spmv_grid", "nguo_ls_precond_apply",
 ]
 
 
@@ -45,6 +45,7 @@ class Params(C.Structure):
         ("barrier_kind", C.c_int32), ("self_collision", C.c_int32), ("max_dx", C.c_float),
         ("target_toi", C.c_float), ("min_newton_steps", C.c_uint32), ("ccd_max_iter", C.c_uint32),
         ("cg_tol", C.c_float), ("cg_max_iter", C.c_uint32), ("floor_y", C.c_float),
+        ("precond", C.c_int32), ("schwarz_domain", C.c_uint32),
     ]
 
 
@@ -104,6 +105,7 @@ def load():
                                   C.c_float, C.c_uint32, P(C.c_float), P(Stats)]
     lib.nguo_ls_last_stats.argtypes = [vp, P(Stats)]
     lib.nguo_ls_spmv.argtypes = [vp, C.c_uint32, P(C.c_uint32), P(C.c_uint32), P(C.c_float), P(C.c_float), P(C.c_float)]
+    lib.nguo_ls_precond_apply.argtypes = [vp, C.c_uint32, P(C.c_uint32), P(C.c_uint32), P(C.c_float), P(C.c_float), P(C.c_float)]
     lib.nguo_scene_generate.argtypes = [C.c_int, C.c_uint64, C.c_double, P(vp)]
     lib.nguo_scene_view.argtypes = [vp, P(SceneDesc)]
     lib.nguo_scene_free.argtypes = [vp]
@@ -190,9 +192,18 @@ class Context:
 class LinearSolver:
     """The LinearSolver seam (engine-core/src/linsolve.rs:21-28) on the device."""
 
-    def __init__(self, ctx, precond=0):
+    def __init__(self, ctx, precond=0, domain=32):
         self.h = C.c_void_p()
-        check(load().nguo_ls_create(ctx.h, int(precond), 0, 0.0, C.byref(self.h)))
+        check(load().nguo_ls_create(ctx.h, int(precond), int(domain), 0.0, C.byref(self.h)))
+
+    def precond_apply(self, row_ptr, col_idx, values, r):
+        row_ptr = np.ascontiguousarray(row_ptr, dtype=np.uint32)
+        col_idx = np.ascontiguousarray(col_idx, dtype=np.uint32)
+        values = np.ascontiguousarray(values, dtype=np.float32)
+        r = np.ascontiguousarray(r, dtype=np.float32)
+        z = np.zeros_like(r)
+        check(load().nguo_ls_precond_apply(self.h, len(r), uptr(row_ptr), uptr(col_idx), fptr(values), fptr(r), fptr(z)))
+        return z
 
     def solve(self, row_ptr, col_idx, values, b, tol, max_iter):
         row_ptr = np.ascontiguousarray(row_ptr, dtype=np.uint32)

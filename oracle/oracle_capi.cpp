@@ -228,6 +228,49 @@ void oracle_cg(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx, con
   if (residual) *residual = st.residual;
 }
 
+// CG with the one-level Schwarz preconditioner (linsolve.rs:463-520 SchwarzCpu semantics).
+void oracle_cg_schwarz(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx, const float *values, const float *b, float tol,
+                       uint32_t max_iter, uint32_t domain, float *x_out, uint32_t *iters, double *residual) {
+  Csr a;
+  a.n = n;
+  a.row_ptr.assign(row_ptr, row_ptr + n + 1);
+  a.col_idx.assign(col_idx, col_idx + row_ptr[n]);
+  a.values.assign(values, values + row_ptr[n]);
+  std::vector<float> bv(b, b + n);
+  CgStats st;
+  auto x = cg_schwarz(a, bv, tol, max_iter, domain, &st);
+  std::memcpy(x_out, x.data(), n * 4);
+  if (iters) *iters = st.iters;
+  if (residual) *residual = st.residual;
+}
+
+// Schwarz domains (vertex index lists, concatenated) and z = M^-1 r (f64).
+uint32_t oracle_schwarz(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx, const float *values, uint32_t domain,
+                        uint32_t *dom_ptr, uint32_t *dom_vert, const double *r, double *z) {
+  Csr a;
+  a.n = n;
+  a.row_ptr.assign(row_ptr, row_ptr + n + 1);
+  a.col_idx.assign(col_idx, col_idx + row_ptr[n]);
+  a.values.assign(values, values + row_ptr[n]);
+  Schwarz sz;
+  sz.domains = Schwarz::build_domains(a, domain);
+  if (dom_ptr) {
+    uint32_t o = 0;
+    dom_ptr[0] = 0;
+    for (size_t d = 0; d < sz.domains.size(); ++d) {
+      for (uint32_t v : sz.domains[d]) dom_vert[o++] = v;
+      dom_ptr[d + 1] = o;
+    }
+  }
+  if (r && z) {
+    sz.factorize(a);
+    std::vector<double> rv(r, r + n), zv(n, 0.0);
+    sz.apply(rv, zv);
+    std::memcpy(z, zv.data(), n * 8);
+  }
+  return (uint32_t)sz.domains.size();
+}
+
 // f32 CSR SpMV (csr.rs:98-110 spmv_cpu).
 void oracle_spmv(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx, const float *values, const float *x, float *y) {
   for (uint32_t r = 0; r < n; ++r) {

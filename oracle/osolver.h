@@ -158,6 +158,162 @@ inline std::vector<float> cg_block_jacobi(const Csr &a, const std::vector<float>
   return out;
 }
 
+// linsolve.rs:80-99 cholesky (lower, row-major); false if a pivot is <= 1e-14.
+inline bool cholesky(const std::vector<double> &a, int m, std::vector<double> &l) {
+  l.assign((size_t)m * m, 0.0);
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j <= i; ++j) {
+      double s = a[(size_t)i * m + j];
+      for (int k = 0; k < j; ++k) s -= l[(size_t)i * m + k] * l[(size_t)j * m + k];
+      if (i == j) {
+        if (s <= 1e-14) return false;
+        l[(size_t)i * m + i] = std::sqrt(s);
+      } else {
+        l[(size_t)i * m + j] = s / l[(size_t)j * m + j];
+      }
+    }
+  return true;
+}
+
+// linsolve.rs:102-118 chol_solve.
+inline void chol_solve(const std::vector<double> &l, int m, const double *b, double *out) {
+  std::vector<double> y(m);
+  for (int i = 0; i < m; ++i) {
+    double s = b[i];
+    for (int k = 0; k < i; ++k) s -= l[(size_t)i * m + k] * y[k];
+    y[i] = s / l[(size_t)i * m + i];
+  }
+  for (int i = m - 1; i >= 0; --i) {
+    double s = y[i];
+    for (int k = i + 1; k < m; ++k) s -= l[(size_t)k * m + i] * out[k];
+    out[i] = s / l[(size_t)i * m + i];
+  }
+}
+
+// linsolve.rs:126-258 SchwarzTopology: greedy BFS aggregation of the block graph into
+// disjoint domains of `domain` vertices; per-domain dense block with the diagonal floor
+// 1e-6 tr/m + 1e-8, f64 Cholesky (diagonal fallback), block solve.
+struct Schwarz {
+  std::vector<std::vector<uint32_t>> domains;
+  std::vector<std::vector<double>> factors;
+  std::vector<int> dims;
+  static std::vector<std::vector<uint32_t>> build_domains(const Csr &a, uint32_t domain) {
+    uint32_t nv = a.n / 3;
+    std::vector<std::vector<uint32_t>> adj(nv);
+    for (uint32_t row = 0; row < a.n; ++row) {
+      uint32_t i = row / 3;
+      for (uint32_t k = a.row_ptr[row]; k < a.row_ptr[row + 1]; ++k) {
+        uint32_t j = a.col_idx[k] / 3;
+        if (j != i) adj[i].push_back(j);
+      }
+    }
+    for (auto &v : adj) { std::sort(v.begin(), v.end()); v.erase(std::unique(v.begin(), v.end()), v.end()); }
+    std::vector<uint8_t> assigned(nv, 0);
+    std::vector<std::vector<uint32_t>> doms;
+    for (uint32_t seed = 0; seed < nv; ++seed) {
+      if (assigned[seed]) continue;
+      std::vector<uint32_t> dom{seed};
+      assigned[seed] = 1;
+      size_t fi = 0;
+      while (dom.size() < domain && fi < dom.size()) {
+        uint32_t v = dom[fi++];
+        for (uint32_t nb : adj[v])
+          if (!assigned[nb] && dom.size() < domain) { assigned[nb] = 1; dom.push_back(nb); }
+      }
+      doms.push_back(dom);
+    }
+    return doms;
+  }
+  void factorize(const Csr &a) {
+    factors.clear();
+    dims.clear();
+    for (auto &dom : domains) {
+      int m = 3 * (int)dom.size();
+      std::vector<uint32_t> dofs;
+      for (uint32_t v : dom) { dofs.push_back(3 * v); dofs.push_back(3 * v + 1); dofs.push_back(3 * v + 2); }
+      std::vector<double> ad((size_t)m * m, 0.0);
+      for (int p = 0; p < m; ++p)
+        for (uint32_t k = a.row_ptr[dofs[p]]; k < a.row_ptr[dofs[p] + 1]; ++k)
+          for (int q = 0; q < m; ++q)
+            if (a.col_idx[k] == dofs[q]) ad[(size_t)p * m + q] = (double)a.values[k];
+      double tr = 0;
+      for (int k = 0; k < m; ++k) tr += ad[(size_t)k * m + k];
+      double floor = 1e-6 * (tr / m) + 1e-8;
+      for (int k = 0; k < m; ++k) ad[(size_t)k * m + k] += floor;
+      std::vector<double> l;
+      if (!cholesky(ad, m, l)) {
+        l.assign((size_t)m * m, 0.0);
+        for (int p = 0; p < m; ++p) {
+          double v = ad[(size_t)p * m + p];
+          l[(size_t)p * m + p] = v > 1e-14 ? std::sqrt(v) : 1.0;
+        }
+      }
+      factors.push_back(l);
+      dims.push_back(m);
+    }
+  }
+  void apply(const std::vector<double> &r, std::vector<double> &z) const {
+    for (size_t d = 0; d < domains.size(); ++d) {
+      const auto &dom = domains[d];
+      int m = dims[d];
+      std::vector<double> rd(m), xd(m);
+      for (size_t p = 0; p < dom.size(); ++p)
+        for (int c = 0; c < 3; ++c) rd[3 * p + c] = r[3 * dom[p] + c];
+      chol_solve(factors[d], m, rd.data(), xd.data());
+      for (size_t p = 0; p < dom.size(); ++p)
+        for (int c = 0; c < 3; ++c) z[3 * dom[p] + c] = xd[3 * p + c];
+    }
+  }
+};
+
+// linsolve.rs:272-411 with Precond::Schwarz (and the cg_run loop of cg_block_jacobi).
+inline std::vector<float> cg_schwarz(const Csr &a, const std::vector<float> &b, float tol, uint32_t max_iter, uint32_t domain, CgStats *st) {
+  const uint32_t n = a.n;
+  Schwarz sz;
+  sz.domains = Schwarz::build_domains(a, domain);
+  sz.factorize(a);
+  auto spmv = [&](const std::vector<double> &x, std::vector<double> &out) {
+    for (uint32_t row = 0; row < n; ++row) {
+      double s = 0;
+      for (uint32_t k = a.row_ptr[row]; k < a.row_ptr[row + 1]; ++k) s += (double)a.values[k] * x[a.col_idx[k]];
+      out[row] = s;
+    }
+  };
+  auto dotv = [](const std::vector<double> &u, const std::vector<double> &v) {
+    double s = 0;
+    for (size_t i = 0; i < u.size(); ++i) s += u[i] * v[i];
+    return s;
+  };
+  std::vector<double> bf(n);
+  for (uint32_t i = 0; i < n; ++i) bf[i] = b[i];
+  double bnorm = std::fmax(std::sqrt(dotv(bf, bf)), 1e-30);
+  std::vector<double> x(n, 0.0), r = bf, z(n), p, ap(n);
+  sz.apply(r, z);
+  p = z;
+  double rz = dotv(r, z);
+  uint32_t iters = 0;
+  double residual = std::sqrt(dotv(r, r)) / bnorm;
+  while (iters < max_iter) {
+    spmv(p, ap);
+    double pap = dotv(p, ap);
+    if (pap <= 0.0 || std::isnan(pap)) break;
+    double alpha = rz / pap;
+    for (uint32_t i = 0; i < n; ++i) { x[i] += alpha * p[i]; r[i] -= alpha * ap[i]; }
+    iters++;
+    residual = std::sqrt(dotv(r, r)) / bnorm;
+    if (residual <= (double)tol) break;
+    sz.apply(r, z);
+    double rz_new = dotv(r, z);
+    double beta = rz_new / rz;
+    for (uint32_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+    rz = rz_new;
+  }
+  std::vector<float> out(n);
+  for (uint32_t i = 0; i < n; ++i) out[i] = (float)x[i];
+  if (st) { st->iters = iters; st->residual = residual; }
+  return out;
+}
+
 template <class T> class Solver {
  public:
   using V = V3<T>;

@@ -494,3 +494,64 @@ def test_oracle_drape_invariants_small():
     assert np.linalg.norm(x[free], axis=1).min() < 0.3 + 2e-3
     assert np.all(o.collider_sides(x) == 1)
     assert len(o.stencils(x)[0]) > 0
+
+
+def grid_system(nx, ny, mass=1e-2, k=1.0):
+    """Block CSR of a mass + spring grid (3 dofs per vertex), SPD."""
+    n = nx * ny
+    rows = {}
+    def add(i, j, blk):
+        rows.setdefault(i, {}).setdefault(j, np.zeros((3, 3)))
+        rows[i][j] += blk
+    for v in range(n):
+        add(v, v, mass * np.eye(3))
+    for y in range(ny):
+        for x in range(nx):
+            v = y * nx + x
+            for dx, dy in ((1, 0), (0, 1), (1, 1)):
+                if x + dx < nx and y + dy < ny:
+                    w = v + dy * nx + dx
+                    K = k * np.eye(3)
+                    add(v, v, K); add(w, w, K); add(v, w, -K); add(w, v, -K)
+    rp, ci, va = [0], [], []
+    for i in range(n):
+        cols = sorted(rows[i])
+        for r in range(3):
+            for j in cols:
+                for c in range(3):
+                    ci.append(3 * j + c)
+                    va.append(rows[i][j][r, c])
+            rp.append(len(ci))
+    return np.array(rp, np.uint32), np.array(ci, np.uint32), np.array(va, np.float32)
+
+
+def test_schwarz_beats_block_jacobi_on_a_grid():
+    # ec/linsolve.rs:915-968: same solution, fewer iterations
+    rp, ci, va = grid_system(24, 24)
+    b = np.random.default_rng(12).standard_normal(len(rp) - 1).astype(np.float32)
+    xj, itj, rj = O.cg(rp, ci, va, b, 1e-6, 100000)
+    x1, it1, r1 = O.cg_schwarz(rp, ci, va, b, 1e-6, 100000, 1)
+    assert it1 == itj  # one-vertex domains are the block-Jacobi preconditioner
+    xs, its, rs = O.cg_schwarz(rp, ci, va, b, 1e-6, 100000, 16)
+    assert its < 0.7 * itj, (its, itj)
+    assert np.abs(xs - xj).max() < 1e-3 * np.abs(xj).max()
+
+
+def test_schwarz_domains_partition_and_single_vertex_limit():
+    rp, ci, va = grid_system(10, 7)
+    doms, _ = O.schwarz(rp, ci, va, 16)
+    allv = np.sort(np.concatenate(doms))
+    assert np.array_equal(allv, np.arange(70))
+    assert max(len(d) for d in doms) <= 16 and doms[0][0] == 0
+    # domain of one vertex: the apply is the 3x3 diagonal-block inverse (up to the 1e-6 floor)
+    r = np.random.default_rng(13).standard_normal(210)
+    _, z = O.schwarz(rp, ci, va, 1, r)
+    n = 210
+    for v in range(70):
+        blk = np.zeros((3, 3))
+        for rr in range(3):
+            row = 3 * v + rr
+            for k in range(rp[row], rp[row + 1]):
+                if ci[k] // 3 == v:
+                    blk[rr, ci[k] % 3] = va[k]
+        assert np.allclose(z[3 * v:3 * v + 3], np.linalg.solve(blk, r[3 * v:3 * v + 3]), rtol=1e-5)
