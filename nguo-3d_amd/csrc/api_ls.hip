@@ -235,25 +235,12 @@ int nguo_bench_spmv_grid(nguo_ctx *ctx, uint32_t nx, uint32_t ny, uint32_t reps,
         for (int e = 0; e < 9; ++e) val[sell_val_index(sl, e)] = diag ? (e % 4 == 0 ? 20.0f : 0.1f) : (e % 4 == 0 ? -1.0f : 0.05f);
       }
     m.val.upload(val, s);
-    DevBuf<float> x, y;
+    DevBuf<float> x;
     std::vector<float> hx(3 * (size_t)nb, 1.0f);
     x.upload(hx, s);
-    y.reserve(3 * (size_t)nb);
     Pcg pcg(s);
     ContactView none{};
-    pcg.apply(m.view(), none, x.p, y.p);
-    hipEvent_t e0, e1;
-    NGUO_HIP(hipEventCreate(&e0));
-    NGUO_HIP(hipEventCreate(&e1));
-    NGUO_HIP(hipEventRecord(e0, s));
-    for (uint32_t r = 0; r < reps; ++r) pcg.apply(m.view(), none, x.p, y.p);
-    NGUO_HIP(hipEventRecord(e1, s));
-    NGUO_HIP(hipEventSynchronize(e1));
-    float ms = 0;
-    NGUO_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    *avg_ms = (double)ms / reps;
+    *avg_ms = pcg.time_spmv(m.view(), none, x.p, reps);
     *bytes = m.spmv_bytes();
     if (nnzb_out) *nnzb_out = (double)m.nnzb;
     return NGUO_OK;

@@ -40,8 +40,10 @@ template <class T> __device__ inline T block_sum(T v, T *lds4) {
 
 // A 3x3-block matrix in SELL-64 layout: block rows are grouped in slices of 64; inside a
 // slice the k-th block of every row sits at slot slice_ptr[s] + 64k + lane, so one wavefront
-// reads 64 consecutive column indices and, per block entry e, 64 consecutive floats
-// (value plane e of the slot group). Padding slots hold col = row and value 0.
+// reads 64 consecutive column indices. The 9 values of the 64 blocks of a slot group are
+// stored as three planes: entries 0-3 as 64 float4, entries 4-7 as 64 float4, entry 8 as
+// 64 floats (576 floats, no padding), so a wavefront reads a block with two 16-byte and one
+// 4-byte coalesced loads. Padding slots hold col = row and value 0.
 struct SellView {
   uint32_t nb;                    // block rows (= free vertices)
   const uint32_t *slice_ptr;      // [nslices + 1], in slots (multiples of 64)
@@ -51,7 +53,22 @@ struct SellView {
 };
 
 __host__ __device__ inline size_t sell_val_index(uint32_t slot, int e) {
-  return (size_t)(slot & ~63u) * 9u + (size_t)e * 64u + (slot & 63u);
+  const size_t g = (size_t)(slot & ~63u) * 9u, lane = slot & 63u;
+  return e < 4 ? g + 4 * lane + (size_t)e : (e < 8 ? g + 256 + 4 * lane + (size_t)(e - 4) : g + 512 + lane);
+}
+
+// The 9 values of the block in slot group `group` (a multiple of 64), lane `lane`.
+struct BlockVals {
+  float4 a, b;  // entries 0-3, 4-7 (row-major 3x3)
+  float c;      // entry 8
+};
+__device__ inline BlockVals load_block(const float *val, uint32_t group, uint32_t lane) {
+  const float *g = val + (size_t)group * 9u;
+  BlockVals v;
+  v.a = reinterpret_cast<const float4 *>(g)[lane];
+  v.b = reinterpret_cast<const float4 *>(g + 256)[lane];
+  v.c = g[512 + lane];
+  return v;
 }
 
 // Matrix-free contact term of the Newton system. Stencil s couples its (up to) four

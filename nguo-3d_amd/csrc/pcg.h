@@ -48,9 +48,12 @@ struct PcgStats {
   bool breakdown = false;
 };
 
-void launch_schwarz_init(hipStream_t s, const Schwarz &sz, const double *r64, float *r, float *z, float *dsol, float *p, double *part);
-void launch_schwarz_update(hipStream_t s, const Schwarz &sz, float *r, float *z, float *dsol, float *p, const float *q, const double *pap_part,
-                           uint32_t npap, const double *cur, double *nxt, double *scal, uint32_t *flag);
+// Additive Schwarz preconditioner kernel of the PCG iteration (one workgroup per domain).
+// init: r = (f32) r64, d = 0, z = M^-1 r, p = z. Otherwise: alpha = rz_j / pAp from the
+// partials (npap SpMV partials, nd r.z partials in cur), r -= alpha q, d += alpha p,
+// z = M^-1 r. Writes the r.z | r.r partials of the new z to nxt (2 nd).
+void launch_schwarz_pcg(hipStream_t s, const Schwarz &sz, bool init, const double *r64, float *r, float *z, float *dsol, float *p,
+                        const float *q, const double *pap_part, uint32_t npap, const double *cur, double *nxt, double *scal, uint32_t *flag);
 
 class Pcg {
  public:
@@ -59,8 +62,11 @@ class Pcg {
   // Solve (A + C) x = b to ||b - (A+C) x|| <= tol ||b|| (FP64 residual). b is a device
   // f32 vector [3 nb]; the solution lands in x_out (device f32 [3 nb]).
   PcgStats solve(const SellMatrix &M, const ContactView &C, const float *b, float tol, uint32_t max_iter, float *x_out);
-  // y = (A + C) x in f32, for parity checks and the SpMV benchmark.
+  // y = (A + C) x in f32 (stride-3 vectors), for parity checks.
   void apply(const SellView &A, const ContactView &C, const float *x, float *y);
+  // Average duration (ms) of the PCG's SpMV kernel on (A + C) x over `reps` back-to-back
+  // launches, between two HIP events on the solver stream.
+  double time_spmv(const SellView &A, const ContactView &C, const float *x, uint32_t reps);
   // Block-Jacobi inverse of (A + C)'s diagonal blocks (FP64 inversion, scalar fallback).
   void build_precond(const SellMatrix &M, const ContactView &C);
   // Use `sz` (built over the same matrix pattern) instead of 3x3 block Jacobi; null = Jacobi.
@@ -73,10 +79,12 @@ class Pcg {
  private:
   void ensure(uint32_t nb, uint32_t ns);
   void launch_minv(const SellMatrix &M, const ContactView &C);
+  void launch_precond_init(const SellMatrix &M, double *part);
+  void launch_spmv_plain(const SellView &A, const ContactView &C);
   hipStream_t s_;
   Schwarz *sz_ = nullptr;
   uint32_t cap_nb_ = 0, cap_ns_ = 0;
-  DevBuf<float> r_, z_, p_, q_, d_, minv_, bf_;
+  DevBuf<float> r_, z_, p_, p2_, q_, d_, minv_;
   DevBuf<double> x64_, r64_, part_, scal_, sums64_;
   DevBuf<uint32_t> flag_;
   uint32_t *h_flag_ = nullptr;

@@ -11,6 +11,7 @@
 #include <cstring>
 
 #include "pcg.h"
+#include "pcg_device.h"
 
 namespace nguo {
 
@@ -66,23 +67,6 @@ uint32_t SellMatrix::find(uint32_t row, uint32_t colj) const {
 }
 
 // ---------------------------------------------------------------- kernels
-__device__ inline void contact_gather(const ContactView &C, uint32_t i, const float *sums, float &y0, float &y1, float &y2) {
-  for (uint32_t k = C.inc_ptr[i]; k < C.inc_ptr[i + 1]; ++k) {
-    uint32_t e = C.inc[k], s = e >> 2, a = e & 3u;
-    float hp = C.coef[2 * s], lam = C.coef[2 * s + 1];
-    const float *sm = sums + 4 * (size_t)s;
-    float t = hp * sm[0];
-    const float *g = C.g + 12 * (size_t)s + 3 * a;
-    y0 += t * g[0]; y1 += t * g[1]; y2 += t * g[2];
-    if (lam != 0.0f) {
-      const float *n = C.n + 3 * (size_t)s;
-      float nd = n[0] * sm[1] + n[1] * sm[2] + n[2] * sm[3];
-      float pw = lam * C.w[4 * (size_t)s + a];
-      y0 += pw * (sm[1] - n[0] * nd); y1 += pw * (sm[2] - n[1] * nd); y2 += pw * (sm[3] - n[2] * nd);
-    }
-  }
-}
-
 __device__ inline void contact_gather64(const ContactView &C, uint32_t i, const double *sums, double &y0, double &y1, double &y2) {
   for (uint32_t k = C.inc_ptr[i]; k < C.inc_ptr[i + 1]; ++k) {
     uint32_t e = C.inc[k], s = e >> 2, a = e & 3u;
@@ -118,33 +102,196 @@ __global__ __launch_bounds__(BLOCK) void k_contact_sums(ContactView C, const TX 
   sums[4 * (size_t)s] = s1; sums[4 * (size_t)s + 1] = a0; sums[4 * (size_t)s + 2] = a1; sums[4 * (size_t)s + 3] = a2;
 }
 
-// y = (A + C) x; part[block] = sum x.y over the block's rows (when part != null).
-__global__ __launch_bounds__(BLOCK) void k_spmv(SellView A, ContactView C, const float *__restrict__ x, float *__restrict__ y,
-                                               double *part, const uint32_t *flag) {
-  __shared__ double lds[4];
-  if (flag && flag[0]) return;
-  uint32_t row = blockIdx.x * BLOCK + threadIdx.x;
-  double dp = 0;
-  if (row < A.nb) {
-    uint32_t base = A.slice_ptr[row >> 6], len = A.row_len[row], lane = row & 63u;
-    float y0 = 0.f, y1 = 0.f, y2 = 0.f;
-    for (uint32_t k = 0; k < len; ++k) {
-      uint32_t sl = base + 64u * k + lane;
-      uint32_t c = A.col[sl];
-      const float *v = A.val + (size_t)(base + 64u * k) * 9u + lane;
-      float x0 = x[3 * (size_t)c], x1 = x[3 * (size_t)c + 1], x2 = x[3 * (size_t)c + 2];
-      y0 += v[0] * x0 + v[64] * x1 + v[128] * x2;
-      y1 += v[192] * x0 + v[256] * x1 + v[320] * x2;
-      y2 += v[384] * x0 + v[448] * x1 + v[512] * x2;
+// ---------------------------------------------------------------- PCG iteration kernels
+// PCG vectors are stored with a stride of 4 floats per block row (x, y, z, 0): a gathered
+// neighbour is one 16-byte load, and a row's three components are one 16-byte store. The
+// fourth component is zero and never enters the arithmetic.
+constexpr uint32_t SPMV_GROUP = 16;  // slots whose loads are in flight together
+enum SpmvMode { SPMV_PLAIN = 0, SPMV_FIRST = 1, SPMV_NEXT = 2 };
+
+// At these sizes a launch costs a handful of dependent memory round trips, not bandwidth,
+// so the kernels below issue every load that does not depend on a reduced scalar before
+// they reduce it, and apply the scalar afterwards.
+
+template <int MODE> __device__ inline float4 pval4(const float4 *z, const float4 *pold, float beta, size_t i) {
+  const float4 pp = pold[i];
+  if (MODE != SPMV_NEXT) return pp;
+  const float4 zz = z[i];
+  return make_float4(zz.x + beta * pp.x, zz.y + beta * pp.y, zz.z + beta * pp.z, 0.0f);
+}
+
+// Loads of a group of slots: the block values and the gathered z / pold of the column.
+template <int MODE> struct SlotGroup {
+  BlockVals v[SPMV_GROUP];
+  float4 zc[SPMV_GROUP], pc[SPMV_GROUP];
+  __device__ inline void load(const SellView &A, const float4 *z, const float4 *pold, uint32_t base, uint32_t k0, uint32_t len, uint32_t lane) {
+    uint32_t cc[SPMV_GROUP];
+#pragma unroll
+    for (uint32_t u = 0; u < SPMV_GROUP; ++u) cc[u] = k0 + u < len ? A.col[base + 64u * (k0 + u) + lane] : 0u;
+#pragma unroll
+    for (uint32_t u = 0; u < SPMV_GROUP; ++u) {
+      if (k0 + u < len) {
+        v[u] = load_block(A.val, base + 64u * (k0 + u), lane);
+        pc[u] = pold[cc[u]];
+        if (MODE == SPMV_NEXT) zc[u] = z[cc[u]];
+      }
     }
-    if (C.ns) contact_gather(C, row, C.sums, y0, y1, y2);
-    y[3 * (size_t)row] = y0; y[3 * (size_t)row + 1] = y1; y[3 * (size_t)row + 2] = y2;
-    if (part) dp = (double)x[3 * (size_t)row] * y0 + (double)x[3 * (size_t)row + 1] * y1 + (double)x[3 * (size_t)row + 2] * y2;
   }
-  if (part) {
-    double t = block_sum(dp, lds);
-    if (threadIdx.x == 0) part[blockIdx.x] = t;
+  // y += sum over the group's slots of B p (p = pold, or z + beta pold), slots ascending
+  __device__ inline void accumulate(float &y0, float &y1, float &y2, float beta, uint32_t k0, uint32_t len) const {
+#pragma unroll
+    for (uint32_t u = 0; u < SPMV_GROUP; ++u) {
+      if (k0 + u < len) {
+        float x0 = pc[u].x, x1 = pc[u].y, x2 = pc[u].z;
+        if (MODE == SPMV_NEXT) { x0 = zc[u].x + beta * x0; x1 = zc[u].y + beta * x1; x2 = zc[u].z + beta * x2; }
+        y0 += v[u].a.x * x0 + v[u].a.y * x1 + v[u].a.z * x2;
+        y1 += v[u].a.w * x0 + v[u].b.x * x1 + v[u].b.y * x2;
+        y2 += v[u].b.z * x0 + v[u].b.w * x1 + v[u].c * x2;
+      }
+    }
   }
+};
+
+// q = (A + C) p, one SELL slice (64 block rows, one per lane) per wavefront, four slices per
+// workgroup; SPMV_GROUP slots at a time: column indices, then every value and gather load
+// of the group, then the FMAs in slot order. PLAIN: p = pold, no reduction. FIRST: p_0 =
+// pold. NEXT: the workgroup also reduces rz_{j-1} (prev), rz_j and rr_j (cur) from the
+// partials of the last preconditioner launch - after the first group's loads are in
+// flight - stops on convergence or the cap (workgroup 0 records the state), and evaluates
+// p_j = z + beta pold wherever it reads p; the row's owner also stores p_j to pnew, and
+// every reader of a neighbour's p evaluates the same expression, so they agree bitwise.
+// The contact term evaluates each incident stencil's g.p and sum w p inline from its four
+// participants. part[workgroup] = p.q over its rows.
+template <int MODE>
+__global__ __launch_bounds__(BLOCK) void k_pcg_spmv(SellView A, ContactView C, const float4 *__restrict__ z, const float4 *__restrict__ pold,
+                                                   float4 *__restrict__ pnew, float4 *__restrict__ q, double *part, const double *prev,
+                                                   const double *cur, uint32_t nz, double *scal, uint32_t *flag, uint32_t j) {
+  __shared__ double lds[3 * BLOCK / WAVE];
+  const uint32_t state = MODE != SPMV_PLAIN ? flag[0] : 0u;
+  const uint32_t lane = threadIdx.x & (WAVE - 1), slice = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE, row = slice * 64u + lane;
+  const bool active = row < A.nb;
+  uint32_t base = 0, width = 0, len = 0;
+  if (active) {
+    base = A.slice_ptr[slice];
+    width = (A.slice_ptr[slice + 1] - base) / 64u;
+    len = A.row_len[row];
+  }
+  SlotGroup<MODE> g;
+  g.load(A, z, pold, base, 0, len, lane);
+  if (state) return;
+  float beta = 0.0f;
+  if (MODE == SPMV_NEXT) {
+    const double *src[3] = {prev, cur, cur + nz};
+    const uint32_t n[3] = {nz, nz, nz};
+    double v[3];
+    sum_parts_all<BLOCK, 3>(src, n, v, lds);
+    const bool conv = v[2] <= scal[5], capped = j >= flag[2];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      flag[1] = j;
+      scal[0] = v[1];
+      scal[3] = v[1] / v[0];
+      scal[4] = v[2];
+      if (conv) flag[0] = 1;
+      else if (capped) flag[0] = 3;
+    }
+    if (conv || capped) return;
+    beta = (float)(v[1] / v[0]);
+  }
+  double dp = 0.0;
+  if (active) {
+    float y0 = 0.0f, y1 = 0.0f, y2 = 0.0f;
+    g.accumulate(y0, y1, y2, beta, 0, len);
+    for (uint32_t k0 = SPMV_GROUP; k0 < width; k0 += SPMV_GROUP) {
+      g.load(A, z, pold, base, k0, len, lane);
+      g.accumulate(y0, y1, y2, beta, k0, len);
+    }
+    if (C.ns) {
+      for (uint32_t k = C.inc_ptr[row]; k < C.inc_ptr[row + 1]; ++k) {
+        const uint32_t e = C.inc[k], s = e >> 2, a = e & 3u;
+        float s1 = 0.f, a0 = 0.f, a1 = 0.f, a2 = 0.f;
+        for (int b = 0; b < 4; ++b) {
+          const uint32_t db = C.dof[4 * (size_t)s + b];
+          if (db == NONE) continue;
+          const float4 x = pval4<MODE>(z, pold, beta, db);
+          const float *gs = C.g + 12 * (size_t)s + 3 * b;
+          s1 += gs[0] * x.x + gs[1] * x.y + gs[2] * x.z;
+          const float w = C.w[4 * (size_t)s + b];
+          a0 += w * x.x; a1 += w * x.y; a2 += w * x.z;
+        }
+        const float hp = C.coef[2 * s], lam = C.coef[2 * s + 1];
+        const float t = hp * s1;
+        const float *ga = C.g + 12 * (size_t)s + 3 * a;
+        y0 += t * ga[0]; y1 += t * ga[1]; y2 += t * ga[2];
+        if (lam != 0.0f) {
+          const float *n = C.n + 3 * (size_t)s;
+          const float nd = n[0] * a0 + n[1] * a1 + n[2] * a2;
+          const float pw = lam * C.w[4 * (size_t)s + a];
+          y0 += pw * (a0 - n[0] * nd); y1 += pw * (a1 - n[1] * nd); y2 += pw * (a2 - n[2] * nd);
+        }
+      }
+    }
+    q[row] = make_float4(y0, y1, y2, 0.0f);
+    if (MODE != SPMV_PLAIN) {
+      const float4 pr = pval4<MODE>(z, pold, beta, row);
+      if (MODE == SPMV_NEXT) pnew[row] = pr;
+      dp = (double)pr.x * y0 + (double)pr.y * y1 + (double)pr.z * y2;
+    }
+  }
+  if (MODE == SPMV_PLAIN) return;
+  double v[1] = {dp};
+  wg_sum_all<BLOCK, 1>(v, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+}
+
+// 3x3 block-Jacobi preconditioner step, one block row per thread (vectors of stride 4,
+// r64 of stride 3). INIT: r = (f32) r64,
+// d = 0, z = Minv r, p = z. Otherwise the workgroup reduces pAp (npap SpMV partials) and
+// rz_j (cur), stops on breakdown, and applies r -= alpha q, d += alpha p, z = Minv r.
+// Partials r.z -> nxt[block], r.r -> nxt[gridDim + block].
+template <bool INIT>
+__global__ __launch_bounds__(BLOCK) void k_jacobi_pcg(uint32_t nb, const float *__restrict__ minv, const double *__restrict__ r64,
+                                                     float *__restrict__ r, float *__restrict__ z, float *__restrict__ d,
+                                                     float *__restrict__ p, const float *__restrict__ q, const double *pap_part,
+                                                     uint32_t npap, const double *cur, double *nxt, double *scal, uint32_t *flag) {
+  __shared__ double lds[2 * BLOCK / WAVE];
+  float alpha = 0.0f;
+  if (!INIT) {
+    if (flag[0]) return;
+    const double *src[2] = {pap_part, cur};
+    const uint32_t n[2] = {npap, gridDim.x};
+    double v[2];
+    sum_parts_all<BLOCK, 2>(src, n, v, lds);
+    if (!(v[0] > 0.0)) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) flag[0] = 2;
+      return;
+    }
+    alpha = (float)(v[1] / v[0]);
+    if (blockIdx.x == 0 && threadIdx.x == 0) { scal[1] = v[0]; scal[2] = v[1] / v[0]; }
+  }
+  const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  double v[2] = {0.0, 0.0};
+  if (i < nb) {
+    const size_t o = 4 * (size_t)i;
+    float r0, r1, r2;
+    if (INIT) {
+      r0 = (float)r64[3 * (size_t)i]; r1 = (float)r64[3 * (size_t)i + 1]; r2 = (float)r64[3 * (size_t)i + 2];
+      d[o] = 0.f; d[o + 1] = 0.f; d[o + 2] = 0.f;
+    } else {
+      r0 = r[o] - alpha * q[o]; r1 = r[o + 1] - alpha * q[o + 1]; r2 = r[o + 2] - alpha * q[o + 2];
+      d[o] += alpha * p[o]; d[o + 1] += alpha * p[o + 1]; d[o + 2] += alpha * p[o + 2];
+    }
+    r[o] = r0; r[o + 1] = r1; r[o + 2] = r2;
+    const float *m = minv + 9 * (size_t)i;
+    const float z0 = m[0] * r0 + m[1] * r1 + m[2] * r2;
+    const float z1 = m[3] * r0 + m[4] * r1 + m[5] * r2;
+    const float z2 = m[6] * r0 + m[7] * r1 + m[8] * r2;
+    z[o] = z0; z[o + 1] = z1; z[o + 2] = z2;
+    if (INIT) { p[o] = z0; p[o + 1] = z1; p[o + 2] = z2; }
+    v[0] = (double)r0 * z0 + (double)r1 * z1 + (double)r2 * z2;
+    v[1] = (double)r0 * r0 + (double)r1 * r1 + (double)r2 * r2;
+  }
+  wg_sum_all<BLOCK, 2>(v, lds);
+  if (threadIdx.x == 0) { nxt[blockIdx.x] = v[0]; nxt[gridDim.x + blockIdx.x] = v[1]; }
 }
 
 // r = b - (A + C) x in FP64; part = r.r per block.
@@ -158,11 +305,11 @@ __global__ __launch_bounds__(BLOCK) void k_residual64(SellView A, ContactView C,
     double y0 = 0, y1 = 0, y2 = 0;
     for (uint32_t k = 0; k < len; ++k) {
       uint32_t c = A.col[base + 64u * k + lane];
-      const float *v = A.val + (size_t)(base + 64u * k) * 9u + lane;
+      const BlockVals v = load_block(A.val, base + 64u * k, lane);
       double x0 = x[3 * (size_t)c], x1 = x[3 * (size_t)c + 1], x2 = x[3 * (size_t)c + 2];
-      y0 += (double)v[0] * x0 + (double)v[64] * x1 + (double)v[128] * x2;
-      y1 += (double)v[192] * x0 + (double)v[256] * x1 + (double)v[320] * x2;
-      y2 += (double)v[384] * x0 + (double)v[448] * x1 + (double)v[512] * x2;
+      y0 += (double)v.a.x * x0 + (double)v.a.y * x1 + (double)v.a.z * x2;
+      y1 += (double)v.a.w * x0 + (double)v.b.x * x1 + (double)v.b.y * x2;
+      y2 += (double)v.b.z * x0 + (double)v.b.w * x1 + (double)v.c * x2;
     }
     if (C.ns) contact_gather64(C, row, sums64, y0, y1, y2);
     double r0 = (double)b[3 * (size_t)row] - y0, r1 = (double)b[3 * (size_t)row + 1] - y1, r2 = (double)b[3 * (size_t)row + 2] - y2;
@@ -173,177 +320,14 @@ __global__ __launch_bounds__(BLOCK) void k_residual64(SellView A, ContactView C,
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
-// Sums `n` partials in a fixed order (single workgroup).
+// Sums `n` partials in a fixed order (single workgroup); valid in thread 0.
 __device__ inline double reduce_parts(const double *part, uint32_t n, double *lds) {
   double acc = 0;
   for (uint32_t i = threadIdx.x; i < n; i += BLOCK) acc += part[i];
   return block_sum(acc, lds);
 }
 
-// scal: 0 rz, 1 pAp, 2 alpha, 3 beta, 4 rr, 5 threshold, 6 rr0.  flag: 0 state, 1 iters, 2 cap.
-__global__ __launch_bounds__(BLOCK) void k_alpha(double *scal, uint32_t *flag, const double *part, uint32_t npart) {
-  __shared__ double lds[4];
-  if (flag[0]) return;
-  double pap = reduce_parts(part, npart, lds);
-  if (threadIdx.x == 0) {
-    scal[1] = pap;
-    if (!(pap > 0.0)) { flag[0] = 2; scal[2] = 0.0; }
-    else scal[2] = scal[0] / pap;
-  }
-}
-
-__global__ __launch_bounds__(BLOCK) void k_update(uint32_t nb, const float *__restrict__ p, const float *__restrict__ q,
-                                                 float *__restrict__ d, float *__restrict__ r, float *__restrict__ z,
-                                                 const float *__restrict__ minv, const double *scal, double *part, const uint32_t *flag) {
-  __shared__ double lds[4];
-  if (flag[0]) return;
-  uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-  float alpha = (float)scal[2];
-  double rz = 0, rr = 0;
-  if (i < nb) {
-    size_t o = 3 * (size_t)i;
-    float r0 = r[o] - alpha * q[o], r1 = r[o + 1] - alpha * q[o + 1], r2 = r[o + 2] - alpha * q[o + 2];
-    d[o] += alpha * p[o]; d[o + 1] += alpha * p[o + 1]; d[o + 2] += alpha * p[o + 2];
-    r[o] = r0; r[o + 1] = r1; r[o + 2] = r2;
-    const float *m = minv + 9 * (size_t)i;
-    float z0 = m[0] * r0 + m[1] * r1 + m[2] * r2;
-    float z1 = m[3] * r0 + m[4] * r1 + m[5] * r2;
-    float z2 = m[6] * r0 + m[7] * r1 + m[8] * r2;
-    z[o] = z0; z[o + 1] = z1; z[o + 2] = z2;
-    rz = (double)r0 * z0 + (double)r1 * z1 + (double)r2 * z2;
-    rr = (double)r0 * r0 + (double)r1 * r1 + (double)r2 * r2;
-  }
-  double a = block_sum(rz, lds);
-  double b = block_sum(rr, lds);
-  if (threadIdx.x == 0) { part[blockIdx.x] = a; part[gridDim.x + blockIdx.x] = b; }
-}
-
-__global__ __launch_bounds__(BLOCK) void k_beta(double *scal, uint32_t *flag, const double *part, uint32_t npart) {
-  __shared__ double lds[4];
-  if (flag[0]) return;
-  double rz_new = reduce_parts(part, npart, lds);
-  double rr = reduce_parts(part + npart, npart, lds);
-  if (threadIdx.x == 0) {
-    uint32_t it = ++flag[1];
-    scal[4] = rr;
-    scal[3] = rz_new / scal[0];
-    scal[0] = rz_new;
-    if (rr <= scal[5]) flag[0] = 1;
-    else if (it >= flag[2]) flag[0] = 3;
-  }
-}
-
-__global__ __launch_bounds__(BLOCK) void k_pupdate(uint32_t nb, const float *__restrict__ z, float *__restrict__ p, const double *scal,
-                                                  const uint32_t *flag) {
-  if (flag[0]) return;
-  uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-  if (i >= 3 * nb) return;
-  float beta = (float)scal[3];
-  p[i] = z[i] + beta * p[i];
-}
-
-// Fused iteration (small systems, <= FUSE_MAX_PARTS workgroups): every workgroup reduces the
-// few partials it needs itself, in the same fixed order, so the scalars never round-trip
-// through a single-workgroup kernel. Partials of r.z / r.r alternate between two buffers by
-// iteration parity: `cur` holds rz_i, the update writes rz_{i+1} into `nxt`.
-constexpr uint32_t FUSE_MAX_PARTS = 4096;
-
-__device__ inline double reduce_parts_bcast(const double *part, uint32_t n, double *lds, double *bcast) {
-  double s = reduce_parts(part, n, lds);
-  if (threadIdx.x == 0) *bcast = s;
-  __syncthreads();
-  double v = *bcast;
-  __syncthreads();
-  return v;
-}
-
-__global__ __launch_bounds__(BLOCK) void k_update_fused(uint32_t nb, const float *__restrict__ p, const float *__restrict__ q,
-                                                       float *__restrict__ d, float *__restrict__ r, float *__restrict__ z,
-                                                       const float *__restrict__ minv, const double *pap_part, const double *cur,
-                                                       double *nxt, double *scal, uint32_t *flag, uint32_t npart) {
-  __shared__ double lds[4];
-  __shared__ double bc;
-  if (flag[0]) return;
-  double pap = reduce_parts_bcast(pap_part, npart, lds, &bc);
-  double rz = reduce_parts_bcast(cur, npart, lds, &bc);
-  if (!(pap > 0.0)) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) flag[0] = 2;
-    return;
-  }
-  float alpha = (float)(rz / pap);
-  if (blockIdx.x == 0 && threadIdx.x == 0) { scal[1] = pap; scal[2] = rz / pap; }
-  uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-  double rzn = 0, rr = 0;
-  if (i < nb) {
-    size_t o = 3 * (size_t)i;
-    float r0 = r[o] - alpha * q[o], r1 = r[o + 1] - alpha * q[o + 1], r2 = r[o + 2] - alpha * q[o + 2];
-    d[o] += alpha * p[o]; d[o + 1] += alpha * p[o + 1]; d[o + 2] += alpha * p[o + 2];
-    r[o] = r0; r[o + 1] = r1; r[o + 2] = r2;
-    const float *m = minv + 9 * (size_t)i;
-    float z0 = m[0] * r0 + m[1] * r1 + m[2] * r2;
-    float z1 = m[3] * r0 + m[4] * r1 + m[5] * r2;
-    float z2 = m[6] * r0 + m[7] * r1 + m[8] * r2;
-    z[o] = z0; z[o + 1] = z1; z[o + 2] = z2;
-    rzn = (double)r0 * z0 + (double)r1 * z1 + (double)r2 * z2;
-    rr = (double)r0 * r0 + (double)r1 * r1 + (double)r2 * r2;
-  }
-  double a = block_sum(rzn, lds);
-  double b = block_sum(rr, lds);
-  if (threadIdx.x == 0) { nxt[blockIdx.x] = a; nxt[npart + blockIdx.x] = b; }
-}
-
-__global__ __launch_bounds__(BLOCK) void k_pupdate_fused(uint32_t nb, const float *__restrict__ z, float *__restrict__ p, const double *cur,
-                                                        const double *nxt, double *scal, uint32_t *flag, uint32_t npart) {
-  __shared__ double lds[4];
-  __shared__ double bc;
-  if (flag[0]) return;
-  double rz_old = reduce_parts_bcast(cur, npart, lds, &bc);
-  double rz_new = reduce_parts_bcast(nxt, npart, lds, &bc);
-  double rr = reduce_parts_bcast(nxt + npart, npart, lds, &bc);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    uint32_t it = ++flag[1];
-    scal[0] = rz_new;
-    scal[3] = rz_new / rz_old;
-    scal[4] = rr;
-    if (rr <= scal[5]) flag[0] = 1;
-    else if (it >= flag[2]) flag[0] = 3;
-  }
-  if (rr <= scal[5]) return;  // converged: p is no longer needed
-  float beta = (float)(rz_new / rz_old);
-  uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-  if (i < nb) {
-    size_t o = 3 * (size_t)i;
-    p[o] = z[o] + beta * p[o];
-    p[o + 1] = z[o + 1] + beta * p[o + 1];
-    p[o + 2] = z[o + 2] + beta * p[o + 2];
-  }
-}
-
-// Start of a sweep: r = (f32) r64, d = 0, z = Minv r, p = z; partials of r.z and r.r.
-__global__ __launch_bounds__(BLOCK) void k_init(uint32_t nb, const double *__restrict__ r64, float *__restrict__ r, float *__restrict__ d,
-                                               float *__restrict__ z, float *__restrict__ p, const float *__restrict__ minv, double *part) {
-  __shared__ double lds[4];
-  uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-  double rz = 0, rr = 0;
-  if (i < nb) {
-    size_t o = 3 * (size_t)i;
-    float r0 = (float)r64[o], r1 = (float)r64[o + 1], r2 = (float)r64[o + 2];
-    r[o] = r0; r[o + 1] = r1; r[o + 2] = r2;
-    d[o] = 0.f; d[o + 1] = 0.f; d[o + 2] = 0.f;
-    const float *m = minv + 9 * (size_t)i;
-    float z0 = m[0] * r0 + m[1] * r1 + m[2] * r2;
-    float z1 = m[3] * r0 + m[4] * r1 + m[5] * r2;
-    float z2 = m[6] * r0 + m[7] * r1 + m[8] * r2;
-    z[o] = z0; z[o + 1] = z1; z[o + 2] = z2;
-    p[o] = z0; p[o + 1] = z1; p[o + 2] = z2;
-    rz = (double)r0 * z0 + (double)r1 * z1 + (double)r2 * z2;
-    rr = (double)r0 * r0 + (double)r1 * r1 + (double)r2 * r2;
-  }
-  double a = block_sum(rz, lds);
-  double b = block_sum(rr, lds);
-  if (threadIdx.x == 0) { part[blockIdx.x] = a; part[gridDim.x + blockIdx.x] = b; }
-}
-
+// Opens a sweep from the (r.z | r.r) partials of the preconditioner's INIT launch.
 __global__ __launch_bounds__(BLOCK) void k_init_scal(double *scal, uint32_t *flag, const double *part, uint32_t npart, double eta2,
                                                     uint32_t cap) {
   __shared__ double lds[4];
@@ -371,9 +355,22 @@ __global__ void k_fill64(uint32_t n, double *a, double v) {
   uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
   if (i < n) a[i] = v;
 }
+// x (stride 3, FP64) += d (stride 4)
 __global__ void k_accum(uint32_t n, double *x, const float *d) {
   uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-  if (i < n) x[i] += (double)d[i];
+  if (i < n) x[i] += (double)d[4 * (i / 3) + i % 3];
+}
+// stride 3 <-> stride 4 copies of a vector of nb rows (fourth component zero)
+__global__ void k_pack4(uint32_t nb, const float *a, float4 *b) {
+  uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i < nb) b[i] = make_float4(a[3 * (size_t)i], a[3 * (size_t)i + 1], a[3 * (size_t)i + 2], 0.0f);
+}
+__global__ void k_unpack4(uint32_t nb, const float4 *a, float *b) {
+  uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i < nb) {
+    const float4 v = a[i];
+    b[3 * (size_t)i] = v.x; b[3 * (size_t)i + 1] = v.y; b[3 * (size_t)i + 2] = v.z;
+  }
 }
 __global__ void k_to_f32(uint32_t n, const double *x, float *out) {
   uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
@@ -430,17 +427,19 @@ Pcg::~Pcg() {
 
 void Pcg::ensure(uint32_t nb, uint32_t ns) {
   {
-    // pAp partials (G), then two parity buffers of (r.z, r.r) partials of max(G, domains) each
-    size_t G = div_up(nb, BLOCK), nz = std::max<size_t>(G, sz_ ? sz_->nd : 0);
-    part_.reserve(G + 4 * nz + 2 * (size_t)div_up(3 * (size_t)nb, BLOCK) + 8);
+    // SpMV partials (one per slice), then two parity buffers of (r.z | r.r) partials of the
+    // preconditioner kernel (per row block or per domain); the FP64 residual reuses the front
+    size_t S = div_up(div_up(nb, 64), BLOCK / WAVE), G = div_up(nb, BLOCK), nz = std::max<size_t>(G, sz_ ? sz_->nd : 0);
+    part_.reserve(std::max(S, G) + 4 * nz + 8);
   }
   if (!h_flag_) {
     NGUO_HIP(hipHostMalloc((void **)&h_flag_, 4 * sizeof(uint32_t), hipHostMallocDefault));
     NGUO_HIP(hipHostMalloc((void **)&h_scal_, 8 * sizeof(double), hipHostMallocDefault));
   }
   if (nb > cap_nb_) {
-    size_t n3 = 3 * (size_t)nb;
-    r_.reserve(n3); z_.reserve(n3); p_.reserve(n3); q_.reserve(n3); d_.reserve(n3); bf_.reserve(n3);
+    size_t n3 = 3 * (size_t)nb, n4 = 4 * (size_t)nb;
+    // stride-4 vectors: the fourth components stay zero
+    r_.zero(n4, s_); z_.zero(n4, s_); p_.zero(n4, s_); p2_.zero(n4, s_); q_.zero(n4, s_); d_.zero(n4, s_);
     minv_.reserve(9 * (size_t)nb);
     x64_.reserve(n3); r64_.reserve(n3);
     cap_nb_ = nb;
@@ -468,11 +467,21 @@ __global__ __launch_bounds__(BLOCK) void k_bj_apply(uint32_t nb, const float *mi
   z[o + 2] = m[6] * r0 + m[7] * r1 + m[8] * r2;
 }
 
+// Opens a sweep: r = (f32) r64, d = 0, z = M^-1 r, p_ = z; (r.z | r.r) partials -> part.
+void Pcg::launch_precond_init(const SellMatrix &M, double *part) {
+  if (sz_) launch_schwarz_pcg(s_, *sz_, true, r64_.p, r_.p, z_.p, d_.p, p_.p, q_.p, nullptr, 0, nullptr, part, scal_.p, flag_.p);
+  else hipLaunchKernelGGL(k_jacobi_pcg<true>, dim3(div_up(M.nb, BLOCK)), dim3(BLOCK), 0, s_, M.nb, (const float *)minv_.p,
+                          (const double *)r64_.p, r_.p, z_.p, d_.p, p_.p, (const float *)q_.p, (const double *)nullptr, 0u,
+                          (const double *)nullptr, part, scal_.p, flag_.p);
+}
+
 void Pcg::apply_precond(const SellMatrix &M, const float *r, float *z) {
   const uint32_t nb = M.nb, n3 = 3 * nb;
+  ensure(nb, 0);
   if (sz_) {
     hipLaunchKernelGGL(k_f32_to_f64, dim3(div_up(n3, BLOCK)), dim3(BLOCK), 0, s_, n3, r, r64_.p);
-    launch_schwarz_init(s_, *sz_, r64_.p, r_.p, z, d_.p, p_.p, part_.p);
+    launch_precond_init(M, part_.p);
+    hipLaunchKernelGGL(k_unpack4, dim3(div_up(nb, BLOCK)), dim3(BLOCK), 0, s_, nb, (const float4 *)z_.p, z);
   } else {
     hipLaunchKernelGGL(k_bj_apply, dim3(div_up(nb, BLOCK)), dim3(BLOCK), 0, s_, nb, (const float *)minv_.p, r, z);
   }
@@ -485,7 +494,7 @@ PcgStats Pcg::solve(const SellMatrix &M, const ContactView &C, const float *b, f
   ensure(nb, C.ns);
   const SellView A = M.view();
   const uint32_t n3 = 3 * nb;
-  const unsigned G = div_up(nb, BLOCK), G3 = div_up(n3, BLOCK), GS = div_up(C.ns, BLOCK);
+  const unsigned G = div_up(nb, BLOCK), G3 = div_up(n3, BLOCK), GS = div_up(C.ns, BLOCK), NS = div_up(div_up(nb, 64), BLOCK / WAVE);
   build_precond(M, C);
   // ||b||^2 through the FP64 residual at x = 0
   hipLaunchKernelGGL(k_fill64, dim3(G3), dim3(BLOCK), 0, s_, n3, x64_.p, 0.0);
@@ -502,59 +511,30 @@ PcgStats Pcg::solve(const SellMatrix &M, const ContactView &C, const float *b, f
     st.residual = 0.0;
     return st;
   }
+  const uint32_t nz = sz_ ? sz_->nd : G;  // preconditioner partials: per domain or per row block
+  double *part_spmv = part_.p, *bufA = part_.p + std::max(NS, G), *bufB = bufA + 2 * (size_t)nz;
   for (uint32_t sweep = 0; sweep < 12; ++sweep) {
     double res = std::sqrt(rr / bb);
     if (res <= (double)tol || st.iters >= max_iter) break;
     double eta = std::min(0.5, std::max(0.25 * (double)tol / res, 1.0e-5));
-    const uint32_t nz = sz_ ? sz_->nd : G;  // partials of r.z / r.r: per domain or per row block
-    const bool fused = G <= FUSE_MAX_PARTS && nz <= FUSE_MAX_PARTS;
-    NGUO_REQUIRE(fused || !sz_, NGUO_E_UNSUPPORTED, "Schwarz preconditioner: too many domains for the fused iteration");
-    double *pap_part = part_.p, *bufA = part_.p + G, *bufB = part_.p + G + 2 * (size_t)std::max(G, nz);
-    double *init_part = fused ? bufA : part_.p;
-    if (sz_) launch_schwarz_init(s_, *sz_, r64_.p, r_.p, z_.p, d_.p, p_.p, init_part);
-    else hipLaunchKernelGGL(k_init, dim3(G), dim3(BLOCK), 0, s_, nb, (const double *)r64_.p, r_.p, d_.p, z_.p, p_.p, (const float *)minv_.p, init_part);
-    hipLaunchKernelGGL(k_init_scal, dim3(1), dim3(BLOCK), 0, s_, scal_.p, flag_.p, (const double *)init_part, nz, eta * eta, max_iter - st.iters);
-    uint64_t launched = 0;
+    launch_precond_init(M, bufA);
+    hipLaunchKernelGGL(k_init_scal, dim3(1), dim3(BLOCK), 0, s_, scal_.p, flag_.p, (const double *)bufA, nz, eta * eta, max_iter - st.iters);
+    uint32_t j = 0;
     while (true) {
-      if (fused && sz_) {
-        for (uint32_t c = 0; c < chunk; ++c, ++launched) {
-          double *cur = (launched & 1) ? bufB : bufA, *nxt = (launched & 1) ? bufA : bufB;
-          if (C.ns) hipLaunchKernelGGL((k_contact_sums<float, float>), dim3(GS), dim3(BLOCK), 0, s_, C, (const float *)p_.p, C.sums, (const uint32_t *)flag_.p);
-          hipLaunchKernelGGL(k_spmv, dim3(G), dim3(BLOCK), 0, s_, A, C, (const float *)p_.p, q_.p, pap_part, (const uint32_t *)flag_.p);
-          launch_schwarz_update(s_, *sz_, r_.p, z_.p, d_.p, p_.p, q_.p, pap_part, G, cur, nxt, scal_.p, flag_.p);
-          hipLaunchKernelGGL(k_pupdate_fused, dim3(G), dim3(BLOCK), 0, s_, nb, (const float *)z_.p, p_.p, (const double *)cur, (const double *)nxt,
-                             scal_.p, flag_.p, nz);
-        }
-        NGUO_HIP(hipGetLastError());
-        NGUO_HIP(hipMemcpyAsync(h_flag_, flag_.p, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, s_));
-        NGUO_HIP(hipStreamSynchronize(s_));
-        if (h_flag_[0]) break;
-        continue;
-      }
-      if (fused) {
-        for (uint32_t c = 0; c < chunk; ++c, ++launched) {
-          double *cur = (launched & 1) ? bufB : bufA, *nxt = (launched & 1) ? bufA : bufB;
-          if (C.ns) hipLaunchKernelGGL((k_contact_sums<float, float>), dim3(GS), dim3(BLOCK), 0, s_, C, (const float *)p_.p, C.sums, (const uint32_t *)flag_.p);
-          hipLaunchKernelGGL(k_spmv, dim3(G), dim3(BLOCK), 0, s_, A, C, (const float *)p_.p, q_.p, pap_part, (const uint32_t *)flag_.p);
-          hipLaunchKernelGGL(k_update_fused, dim3(G), dim3(BLOCK), 0, s_, nb, (const float *)p_.p, (const float *)q_.p, d_.p, r_.p, z_.p,
-                             (const float *)minv_.p, (const double *)pap_part, (const double *)cur, nxt, scal_.p, flag_.p, G);
-          hipLaunchKernelGGL(k_pupdate_fused, dim3(G), dim3(BLOCK), 0, s_, nb, (const float *)z_.p, p_.p, (const double *)cur, (const double *)nxt,
-                             scal_.p, flag_.p, G);
-        }
-        NGUO_HIP(hipGetLastError());
-        NGUO_HIP(hipMemcpyAsync(h_flag_, flag_.p, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, s_));
-        NGUO_HIP(hipStreamSynchronize(s_));
-        if (h_flag_[0]) break;
-        continue;
-      }
-      for (uint32_t c = 0; c < chunk; ++c) {
-        if (C.ns) hipLaunchKernelGGL((k_contact_sums<float, float>), dim3(GS), dim3(BLOCK), 0, s_, C, (const float *)p_.p, C.sums, (const uint32_t *)flag_.p);
-        hipLaunchKernelGGL(k_spmv, dim3(G), dim3(BLOCK), 0, s_, A, C, (const float *)p_.p, q_.p, part_.p, (const uint32_t *)flag_.p);
-        hipLaunchKernelGGL(k_alpha, dim3(1), dim3(BLOCK), 0, s_, scal_.p, flag_.p, (const double *)part_.p, G);
-        hipLaunchKernelGGL(k_update, dim3(G), dim3(BLOCK), 0, s_, nb, (const float *)p_.p, (const float *)q_.p, d_.p, r_.p, z_.p,
-                           (const float *)minv_.p, (const double *)scal_.p, part_.p, (const uint32_t *)flag_.p);
-        hipLaunchKernelGGL(k_beta, dim3(1), dim3(BLOCK), 0, s_, scal_.p, flag_.p, (const double *)part_.p, G);
-        hipLaunchKernelGGL(k_pupdate, dim3(G3), dim3(BLOCK), 0, s_, nb, (const float *)z_.p, p_.p, (const double *)scal_.p, (const uint32_t *)flag_.p);
+      for (uint32_t c = 0; c < chunk; ++c, ++j) {
+        // iteration j: p_j in p_ (even j) or p2_ (odd j); rz_j partials in cur, rz_{j+1} -> nxt
+        float *pj = (j & 1) ? p2_.p : p_.p, *pprev = (j & 1) ? p_.p : p2_.p;
+        double *cur = (j & 1) ? bufB : bufA, *nxt = (j & 1) ? bufA : bufB;
+        if (j == 0)
+          hipLaunchKernelGGL(k_pcg_spmv<SPMV_FIRST>, dim3(NS), dim3(BLOCK), 0, s_, A, C, (const float4 *)z_.p, (const float4 *)pj,
+                             (float4 *)nullptr, (float4 *)q_.p, part_spmv, (const double *)nullptr, (const double *)nullptr, nz, scal_.p,
+                             flag_.p, j);
+        else
+          hipLaunchKernelGGL(k_pcg_spmv<SPMV_NEXT>, dim3(NS), dim3(BLOCK), 0, s_, A, C, (const float4 *)z_.p, (const float4 *)pprev,
+                             (float4 *)pj, (float4 *)q_.p, part_spmv, (const double *)nxt, (const double *)cur, nz, scal_.p, flag_.p, j);
+        if (sz_) launch_schwarz_pcg(s_, *sz_, false, nullptr, r_.p, z_.p, d_.p, pj, q_.p, part_spmv, NS, cur, nxt, scal_.p, flag_.p);
+        else hipLaunchKernelGGL(k_jacobi_pcg<false>, dim3(G), dim3(BLOCK), 0, s_, nb, (const float *)minv_.p, (const double *)nullptr, r_.p,
+                                z_.p, d_.p, pj, (const float *)q_.p, (const double *)part_spmv, NS, (const double *)cur, nxt, scal_.p, flag_.p);
       }
       NGUO_HIP(hipGetLastError());
       NGUO_HIP(hipMemcpyAsync(h_flag_, flag_.p, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, s_));
@@ -581,9 +561,38 @@ PcgStats Pcg::solve(const SellMatrix &M, const ContactView &C, const float *b, f
 
 void Pcg::apply(const SellView &A, const ContactView &C, const float *x, float *y) {
   ensure(A.nb, C.ns);
-  if (C.ns) hipLaunchKernelGGL((k_contact_sums<float, float>), dim3(div_up(C.ns, BLOCK)), dim3(BLOCK), 0, s_, C, x, C.sums, nullptr);
-  hipLaunchKernelGGL(k_spmv, dim3(div_up(A.nb, BLOCK)), dim3(BLOCK), 0, s_, A, C, x, y, nullptr, nullptr);
+  if (!A.nb) return;
+  const unsigned G = div_up(A.nb, BLOCK);
+  hipLaunchKernelGGL(k_pack4, dim3(G), dim3(BLOCK), 0, s_, A.nb, x, (float4 *)p2_.p);
+  launch_spmv_plain(A, C);
+  hipLaunchKernelGGL(k_unpack4, dim3(G), dim3(BLOCK), 0, s_, A.nb, (const float4 *)q_.p, y);
   NGUO_HIP(hipGetLastError());
+}
+
+double Pcg::time_spmv(const SellView &A, const ContactView &C, const float *x, uint32_t reps) {
+  ensure(A.nb, C.ns);
+  if (!A.nb || !reps) return 0.0;
+  hipLaunchKernelGGL(k_pack4, dim3(div_up(A.nb, BLOCK)), dim3(BLOCK), 0, s_, A.nb, x, (float4 *)p2_.p);
+  launch_spmv_plain(A, C);  // warm-up
+  hipEvent_t e0, e1;
+  NGUO_HIP(hipEventCreate(&e0));
+  NGUO_HIP(hipEventCreate(&e1));
+  NGUO_HIP(hipEventRecord(e0, s_));
+  for (uint32_t r = 0; r < reps; ++r) launch_spmv_plain(A, C);
+  NGUO_HIP(hipEventRecord(e1, s_));
+  NGUO_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  NGUO_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  NGUO_HIP(hipGetLastError());
+  return (double)ms / reps;
+}
+
+void Pcg::launch_spmv_plain(const SellView &A, const ContactView &C) {
+  hipLaunchKernelGGL(k_pcg_spmv<SPMV_PLAIN>, dim3(div_up(div_up(A.nb, 64), BLOCK / WAVE)), dim3(BLOCK), 0, s_, A, C, (const float4 *)nullptr,
+                     (const float4 *)p2_.p, (float4 *)nullptr, (float4 *)q_.p, (double *)nullptr, (const double *)nullptr,
+                     (const double *)nullptr, 0u, (double *)nullptr, (uint32_t *)nullptr, 0u);
 }
 
 }  // namespace nguo

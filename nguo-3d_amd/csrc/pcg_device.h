@@ -1,0 +1,48 @@
+// Device helpers of the PCG iteration kernels (k_pcg_spmv and the block-Jacobi / additive
+// Schwarz preconditioner kernels): workgroup sums with a fixed combination order, and the
+// redundant per-workgroup reduction of the previous launch's partials.
+//
+// One PCG iteration is two launches. Each launch stores one partial sum per workgroup; the
+// next launch's workgroups each reduce those partials themselves, in the same fixed order,
+// so every workgroup holds bitwise the same scalar without a single-workgroup launch or a
+// cross-workgroup handshake in between (kernel boundaries order all the memory traffic).
+//
+// scal: 0 rz_j, 1 pAp, 2 alpha, 3 beta, 4 rr, 5 threshold (eta^2 rr0), 6 rr0, 7 host-read
+//       FP64 residual.
+// flag: 0 state (0 running, 1 converged, 2 breakdown, 3 iteration cap), 1 iterations done,
+//       2 cap.
+#pragma once
+#include "device_common.h"
+
+namespace nguo {
+
+// Sum of K values over an NT-thread workgroup, broadcast to every thread. Fixed order.
+template <int NT, int K> __device__ inline void wg_sum_all(double (&v)[K], double *lds /* [NT / WAVE * K] */) {
+  constexpr int NW = NT / WAVE;
+  const int w = threadIdx.x / WAVE;
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
+  if ((threadIdx.x & (WAVE - 1)) == 0)
+    for (int k = 0; k < K; ++k) lds[w * K + k] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double s = lds[k];
+    for (int i = 1; i < NW; ++i) s += lds[i * K + k];
+    v[k] = s;
+  }
+  __syncthreads();
+}
+
+// v[k] = sum_{i < n[k]} src[k][i] for K partial arrays, every thread gets the sums.
+template <int NT, int K>
+__device__ inline void sum_parts_all(const double *const (&src)[K], const uint32_t (&n)[K], double (&v)[K], double *lds) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    v[k] = 0.0;
+    for (uint32_t i = threadIdx.x; i < n[k]; i += NT) v[k] += src[k][i];
+  }
+  wg_sum_all<NT, K>(v, lds);
+}
+
+}  // namespace nguo

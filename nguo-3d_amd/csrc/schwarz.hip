@@ -7,6 +7,7 @@
 #include <algorithm>
 
 #include "pcg.h"
+#include "pcg_device.h"
 
 namespace nguo {
 
@@ -131,8 +132,9 @@ __global__ __launch_bounds__(BLOCK) void k_schwarz_build(SellView A, ContactView
   if (failed) {
     // diagonal fallback: factor sqrt(a_pp) (or 1) -> inverse 1 / a_pp
     if (t == 0) atomicAdd(fails, 1u);
-    for (uint32_t i = t; i < m * m; i += BLOCK) {
-      uint32_t p = i / m, q = i % m;
+    const uint32_t m4 = (m + 3) & ~3u;
+    for (uint32_t i = t; i < m * m4; i += BLOCK) {
+      uint32_t p = i / m4, q = i % m4;
       double v = diag0[p];
       out[p * MD + q] = p == q ? (float)(v > 1e-14 ? 1.0 / v : 1.0) : 0.0f;
     }
@@ -148,9 +150,11 @@ __global__ __launch_bounds__(BLOCK) void k_schwarz_build(SellView A, ContactView
     }
   }
   __syncthreads();
-  // A^-1 = W^T W
-  for (uint32_t i = t; i < m * m; i += BLOCK) {
-    uint32_t p = i / m, q = i % m;
+  // A^-1 = W^T W; columns m..m4-1 of rows < m are zero (the apply reads float4 groups)
+  const uint32_t m4 = (m + 3) & ~3u;
+  for (uint32_t i = t; i < m * m4; i += BLOCK) {
+    uint32_t p = i / m4, q = i % m4;
+    if (q >= m) { out[p * MD + q] = 0.0f; continue; }
     uint32_t k0 = p > q ? p : q;
     double s = 0.0;
     for (uint32_t k = k0; k < m; ++k) s += W[k * m + p] * W[k * m + q];
@@ -170,88 +174,121 @@ void Pcg::build_precond(const SellMatrix &M, const ContactView &C) {
   NGUO_HIP(hipGetLastError());
 }
 
-// z = A_d^-1 r over each domain; with `alpha` the PCG update r -= alpha q, d += alpha p is
-// folded in first. Partials per domain: r.z -> part[d], r.r -> part[nd + d].
-template <bool UPDATE>
-__global__ __launch_bounds__(BLOCK) void k_schwarz_apply(uint32_t nd, const uint32_t *__restrict__ dom_ptr, const uint32_t *__restrict__ dom_row,
-                                                        const float *__restrict__ ainv, const double *r64, float *__restrict__ r,
-                                                        float *__restrict__ z, float *__restrict__ dsol, float *__restrict__ p,
-                                                        const float *__restrict__ q, const double *pap_part, uint32_t npap,
-                                                        const double *cur, double *part, double *scal, uint32_t *flag, int init) {
+// Preconditioner kernel of the PCG iteration, one workgroup per domain. Not INIT: the
+// workgroup reduces pAp (npap SpMV partials) and rz_j (cur) itself, stops on breakdown,
+// then applies the PCG update to the domain's rows (r -= alpha q, d += alpha p); INIT:
+// r = (f32) r64, d = 0, p = z. Then z = A_d^-1 r: the domain's FP32 inverse is read as rows
+// of 24 float4 (coalesced 384-byte rows) by 240 threads that split its k range into 10
+// interleaved groups, all of a thread's loads issued before its FMAs; the groups' partial z
+// are summed from LDS in a fixed order. Each row belongs to exactly one domain, so r, d
+// and z have a single writer. Vectors have stride 4 per block row, r64 stride 3.
+// Partials r.z -> nxt[d], r.r -> nxt[nd + d].
+constexpr uint32_t SCH_KG = 10;       // k groups
+constexpr uint32_t SCH_C4 = MD / 4;   // float4 columns of a row
+constexpr uint32_t SCH_KPER = (MD + SCH_KG - 1) / SCH_KG;  // rows per k group
+static_assert(SCH_KG * SCH_C4 <= (uint32_t)BLOCK, "k groups x float4 columns must fit the workgroup");
+
+template <bool INIT>
+__global__ __launch_bounds__(BLOCK) void k_schwarz_pcg(uint32_t nd, const uint32_t *__restrict__ dom_ptr, const uint32_t *__restrict__ dom_row,
+                                                      const float *__restrict__ ainv, const double *__restrict__ r64, float *__restrict__ r,
+                                                      float *__restrict__ z, float *__restrict__ dsol, float *__restrict__ p,
+                                                      const float *__restrict__ q, const double *pap_part, uint32_t npap, const double *cur,
+                                                      double *nxt, double *scal, uint32_t *flag) {
   __shared__ float rs[MD];
-  __shared__ double lds[4];
-  __shared__ double bc;
-  if (flag && flag[0]) return;
+  __shared__ float4 zp[SCH_KG][SCH_C4];
+  __shared__ double lds[2 * BLOCK / WAVE];
   const uint32_t d = blockIdx.x, t = threadIdx.x;
-  const uint32_t base = dom_ptr[d], m = 3 * (dom_ptr[d + 1] - base);
-  float alpha = 0.0f;
-  if (UPDATE) {
-    double pap = 0, rz = 0;
-    {
-      double acc = 0;
-      for (uint32_t i = t; i < npap; i += BLOCK) acc += pap_part[i];
-      pap = block_sum(acc, lds);
-      if (t == 0) bc = pap;
-      __syncthreads();
-      pap = bc;
-      __syncthreads();
-      acc = 0;
-      for (uint32_t i = t; i < nd; i += BLOCK) acc += cur[i];
-      rz = block_sum(acc, lds);
-      if (t == 0) bc = rz;
-      __syncthreads();
-      rz = bc;
-      __syncthreads();
+  const uint32_t state = INIT ? 0u : flag[0];
+  // loads that need no scalar: the inverse rows of this thread's k group (rows < MD of the
+  // domain's block: in bounds whatever m is; rows >= m are masked at the FMA), the domain
+  // rows and this thread's r / q / p / d entries
+  const uint32_t c4 = t % SCH_C4, kg = t / SCH_C4;
+  const bool mac = kg < SCH_KG;
+  float4 w[SCH_KPER];
+  {
+    const float4 *a4 = reinterpret_cast<const float4 *>(ainv + (size_t)d * MD * MD) + c4;
+#pragma unroll
+    for (uint32_t i = 0; i < SCH_KPER; ++i) {
+      const uint32_t k = kg + SCH_KG * i;
+      w[i] = mac && k < MD ? a4[(size_t)k * SCH_C4] : make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    if (!(pap > 0.0)) {
-      if (d == 0 && t == 0) flag[0] = 2;
+  }
+  const uint32_t base = dom_ptr[d], m = 3 * (dom_ptr[d + 1] - base);
+  size_t o = 0;
+  float r0 = 0.f, q0 = 0.f, p0 = 0.f, d0 = 0.f;
+  if (t < m) {
+    const size_t row = dom_row[base + t / 3];
+    o = 4 * row + t % 3;
+    if (INIT) {
+      r0 = (float)r64[3 * row + t % 3];
+    } else {
+      r0 = r[o]; q0 = q[o]; p0 = p[o]; d0 = dsol[o];
+    }
+  }
+  if (state) return;
+  float alpha = 0.0f;
+  if (!INIT) {
+    const double *src[2] = {pap_part, cur};
+    const uint32_t n[2] = {npap, nd};
+    double v[2];
+    sum_parts_all<BLOCK, 2>(src, n, v, lds);
+    if (!(v[0] > 0.0)) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) flag[0] = 2;
       return;
     }
-    alpha = (float)(rz / pap);
-    if (d == 0 && t == 0) { scal[1] = pap; scal[2] = rz / pap; }
+    alpha = (float)(v[1] / v[0]);
+    if (blockIdx.x == 0 && threadIdx.x == 0) { scal[1] = v[0]; scal[2] = v[1] / v[0]; }
   }
-  size_t o = 0;
   float rv = 0.0f;
   if (t < m) {
-    uint32_t row = dom_row[base + t / 3];
-    o = 3 * (size_t)row + t % 3;
-    if (init) {
-      rv = (float)r64[o];
+    if (INIT) {
+      rv = r0;
       dsol[o] = 0.0f;
     } else {
-      rv = r[o] - alpha * q[o];
-      dsol[o] += alpha * p[o];
+      rv = r0 - alpha * q0;
+      dsol[o] = d0 + alpha * p0;
     }
     r[o] = rv;
     rs[t] = rv;
   }
   __syncthreads();
-  double rzp = 0, rrp = 0;
-  if (t < m) {
-    const float *col = ainv + (size_t)d * MD * MD + t;
-    float zv = 0.0f;
-    for (uint32_t k = 0; k < m; ++k) zv += col[(size_t)k * MD] * rs[k];
-    z[o] = zv;
-    if (init) p[o] = zv;
-    rzp = (double)rv * zv;
-    rrp = (double)rv * rv;
+  const uint32_t m4 = (m + 3) / 4;
+  if (mac && c4 < m4) {
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (uint32_t i = 0; i < SCH_KPER; ++i) {
+      const uint32_t k = kg + SCH_KG * i;
+      if (k < m) {
+        const float rk = rs[k];
+        acc.x += w[i].x * rk; acc.y += w[i].y * rk; acc.z += w[i].z * rk; acc.w += w[i].w * rk;
+      }
+    }
+    zp[kg][c4] = acc;
   }
-  double a = block_sum(rzp, lds);
-  double b = block_sum(rrp, lds);
-  if (t == 0) { part[d] = a; part[nd + d] = b; }
+  __syncthreads();
+  double v[2] = {0.0, 0.0};
+  if (t < m) {
+    const float *zf = reinterpret_cast<const float *>(&zp[0][0]);
+    float zv = zf[t];
+    for (uint32_t g = 1; g < SCH_KG; ++g) zv += zf[g * MD + t];
+    z[o] = zv;
+    if (INIT) p[o] = zv;
+    v[0] = (double)rv * zv;
+    v[1] = (double)rv * rv;
+  }
+  wg_sum_all<BLOCK, 2>(v, lds);
+  if (t == 0) { nxt[d] = v[0]; nxt[nd + d] = v[1]; }
 }
 
-void launch_schwarz_init(hipStream_t s, const Schwarz &sz, const double *r64, float *r, float *z, float *dsol, float *p, double *part) {
-  hipLaunchKernelGGL((k_schwarz_apply<false>), dim3(sz.nd), dim3(BLOCK), 0, s, sz.nd, (const uint32_t *)sz.dom_ptr.p,
-                     (const uint32_t *)sz.dom_row.p, (const float *)sz.ainv.p, r64, r, z, dsol, p, (const float *)nullptr,
-                     (const double *)nullptr, 0u, (const double *)nullptr, part, (double *)nullptr, (uint32_t *)nullptr, 1);
-}
-
-void launch_schwarz_update(hipStream_t s, const Schwarz &sz, float *r, float *z, float *dsol, float *p, const float *q, const double *pap_part,
-                           uint32_t npap, const double *cur, double *nxt, double *scal, uint32_t *flag) {
-  hipLaunchKernelGGL((k_schwarz_apply<true>), dim3(sz.nd), dim3(BLOCK), 0, s, sz.nd, (const uint32_t *)sz.dom_ptr.p,
-                     (const uint32_t *)sz.dom_row.p, (const float *)sz.ainv.p, (const double *)nullptr, r, z, dsol, p, q, pap_part, npap, cur,
-                     nxt, scal, flag, 0);
+void launch_schwarz_pcg(hipStream_t s, const Schwarz &sz, bool init, const double *r64, float *r, float *z, float *dsol, float *p,
+                        const float *q, const double *pap_part, uint32_t npap, const double *cur, double *nxt, double *scal, uint32_t *flag) {
+  if (!sz.nd) return;
+  if (init)
+    hipLaunchKernelGGL(k_schwarz_pcg<true>, dim3(sz.nd), dim3(BLOCK), 0, s, sz.nd, (const uint32_t *)sz.dom_ptr.p, (const uint32_t *)sz.dom_row.p,
+                       (const float *)sz.ainv.p, r64, r, z, dsol, p, q, pap_part, npap, cur, nxt, scal, flag);
+  else
+    hipLaunchKernelGGL(k_schwarz_pcg<false>, dim3(sz.nd), dim3(BLOCK), 0, s, sz.nd, (const uint32_t *)sz.dom_ptr.p, (const uint32_t *)sz.dom_row.p,
+                       (const float *)sz.ainv.p, r64, r, z, dsol, p, q, pap_part, npap, cur, nxt, scal, flag);
 }
 
 }  // namespace nguo

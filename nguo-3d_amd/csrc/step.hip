@@ -1699,21 +1699,9 @@ int nguo_step_bench_spmv(nguo_step *st, uint32_t reps, double *avg_ms, double *b
     gather(st->act, st->P, 1, st->im.p, st->x.p, st->x0.p, s);
     contacts(st);
     ContactView C = contact_view(st);
-    float *xin = st->egrad.p, *yout = st->rhs.p;
+    float *xin = st->egrad.p;
     NGUO_HIP(hipMemcpyAsync(xin, st->x.p, 3 * (size_t)st->NF * sizeof(float), hipMemcpyDeviceToDevice, s));
-    st->pcg->apply(st->act.M.view(), C, xin, yout);  // warm-up
-    hipEvent_t e0, e1;
-    NGUO_HIP(hipEventCreate(&e0));
-    NGUO_HIP(hipEventCreate(&e1));
-    NGUO_HIP(hipEventRecord(e0, s));
-    for (uint32_t r = 0; r < reps; ++r) st->pcg->apply(st->act.M.view(), C, xin, yout);
-    NGUO_HIP(hipEventRecord(e1, s));
-    NGUO_HIP(hipEventSynchronize(e1));
-    float ms = 0;
-    NGUO_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    *avg_ms = (double)ms / reps;
+    *avg_ms = st->pcg->time_spmv(st->act.M.view(), C, xin, reps);
     *bytes = st->act.M.spmv_bytes();
     return NGUO_OK;
   });

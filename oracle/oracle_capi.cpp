@@ -230,7 +230,7 @@ void oracle_cg(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx, con
 
 // CG with the one-level Schwarz preconditioner (linsolve.rs:463-520 SchwarzCpu semantics).
 void oracle_cg_schwarz(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx, const float *values, const float *b, float tol,
-                       uint32_t max_iter, uint32_t domain, float *x_out, uint32_t *iters, double *residual) {
+                       uint32_t max_iter, uint32_t domain, double weight, float *x_out, uint32_t *iters, double *residual) {
   Csr a;
   a.n = n;
   a.row_ptr.assign(row_ptr, row_ptr + n + 1);
@@ -238,37 +238,37 @@ void oracle_cg_schwarz(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_
   a.values.assign(values, values + row_ptr[n]);
   std::vector<float> bv(b, b + n);
   CgStats st;
-  auto x = cg_schwarz(a, bv, tol, max_iter, domain, &st);
+  auto x = cg_schwarz(a, bv, tol, max_iter, domain, weight, &st);
   std::memcpy(x_out, x.data(), n * 4);
   if (iters) *iters = st.iters;
   if (residual) *residual = st.residual;
 }
 
-// Schwarz domains (vertex index lists, concatenated) and z = M^-1 r (f64).
-uint32_t oracle_schwarz(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx, const float *values, uint32_t domain,
+// Schwarz domains (vertex index lists, concatenated) and z = M^-1 r (f64); weight 0 = one
+// level, otherwise the two-level form with that coarse weight.
+uint32_t oracle_schwarz(uint32_t n, const uint32_t *row_ptr, const uint32_t *col_idx, const float *values, uint32_t domain, double weight,
                         uint32_t *dom_ptr, uint32_t *dom_vert, const double *r, double *z) {
   Csr a;
   a.n = n;
   a.row_ptr.assign(row_ptr, row_ptr + n + 1);
   a.col_idx.assign(col_idx, col_idx + row_ptr[n]);
   a.values.assign(values, values + row_ptr[n]);
-  Schwarz sz;
-  sz.domains = Schwarz::build_domains(a, domain);
+  Schwarz2 sz;
+  sz.build(a, domain, weight);
   if (dom_ptr) {
     uint32_t o = 0;
     dom_ptr[0] = 0;
-    for (size_t d = 0; d < sz.domains.size(); ++d) {
-      for (uint32_t v : sz.domains[d]) dom_vert[o++] = v;
+    for (size_t d = 0; d < sz.l0.domains.size(); ++d) {
+      for (uint32_t v : sz.l0.domains[d]) dom_vert[o++] = v;
       dom_ptr[d + 1] = o;
     }
   }
   if (r && z) {
-    sz.factorize(a);
     std::vector<double> rv(r, r + n), zv(n, 0.0);
     sz.apply(rv, zv);
     std::memcpy(z, zv.data(), n * 8);
   }
-  return (uint32_t)sz.domains.size();
+  return (uint32_t)sz.l0.domains.size();
 }
 
 // f32 CSR SpMV (csr.rs:98-110 spmv_cpu).

@@ -266,12 +266,82 @@ struct Schwarz {
   }
 };
 
-// linsolve.rs:272-411 with Precond::Schwarz (and the cg_run loop of cg_block_jacobi).
-inline std::vector<float> cg_schwarz(const Csr &a, const std::vector<float> &b, float tol, uint32_t max_iter, uint32_t domain, CgStats *st) {
+// linsolve.rs:522-657 MultilevelSchwarz (levels = 2): z = S0 r + w C1^T S1 (C1 r). The
+// coarse node of a fine vertex is its level-0 domain; C1 sums a domain's rows (restriction
+// by summation, prolongation by injection); A1 = C1 A C1^T is the Galerkin coarse operator,
+// assembled by summing every fine entry into (3 agg(row/3) + row%3, 3 agg(col/3) + col%3) in
+// CSR order; S1 is a one-level Schwarz over A1 with the same domain size. Weight 0 leaves
+// the one-level preconditioner S0.
+struct Schwarz2 {
+  Schwarz l0, l1;
+  std::vector<uint32_t> agg0;
+  uint32_t n_coarse = 0;
+  double weight = 0.0;
+  Csr coarse;
+  void build(const Csr &a, uint32_t domain, double w) {
+    weight = w;
+    l0.domains = Schwarz::build_domains(a, domain);
+    l0.factorize(a);
+    if (weight == 0.0) return;
+    uint32_t nv = a.n / 3;
+    agg0.assign(nv, 0);
+    for (size_t d = 0; d < l0.domains.size(); ++d)
+      for (uint32_t v : l0.domains[d]) agg0[v] = (uint32_t)d;
+    n_coarse = (uint32_t)l0.domains.size();
+    // coarse CSR pattern: sorted unique coarse columns per coarse row
+    const uint32_t nc = 3 * n_coarse;
+    std::vector<std::vector<uint32_t>> cols(nc);
+    for (uint32_t row = 0; row < a.n; ++row) {
+      uint32_t cr = 3 * agg0[row / 3] + row % 3;
+      for (uint32_t k = a.row_ptr[row]; k < a.row_ptr[row + 1]; ++k) {
+        uint32_t col = a.col_idx[k];
+        cols[cr].push_back(3 * agg0[col / 3] + col % 3);
+      }
+    }
+    coarse.n = nc;
+    coarse.row_ptr.assign(1, 0);
+    coarse.col_idx.clear();
+    for (uint32_t r = 0; r < nc; ++r) {
+      auto &c = cols[r];
+      std::sort(c.begin(), c.end());
+      c.erase(std::unique(c.begin(), c.end()), c.end());
+      coarse.col_idx.insert(coarse.col_idx.end(), c.begin(), c.end());
+      coarse.row_ptr.push_back((uint32_t)coarse.col_idx.size());
+    }
+    // f32 values summed in fine CSR order (linsolve.rs:613-625 coarse_matrix)
+    coarse.values.assign(coarse.col_idx.size(), 0.0f);
+    for (uint32_t row = 0; row < a.n; ++row) {
+      uint32_t cr = 3 * agg0[row / 3] + row % 3;
+      for (uint32_t k = a.row_ptr[row]; k < a.row_ptr[row + 1]; ++k) {
+        uint32_t col = a.col_idx[k];
+        uint32_t cc = 3 * agg0[col / 3] + col % 3;
+        auto it = std::lower_bound(coarse.col_idx.begin() + coarse.row_ptr[cr], coarse.col_idx.begin() + coarse.row_ptr[cr + 1], cc);
+        coarse.values[it - coarse.col_idx.begin()] += a.values[k];
+      }
+    }
+    l1.domains = Schwarz::build_domains(coarse, domain);
+    l1.factorize(coarse);
+  }
+  void apply(const std::vector<double> &r, std::vector<double> &z) const {
+    l0.apply(r, z);
+    if (weight == 0.0) return;
+    const uint32_t nv = (uint32_t)agg0.size();
+    std::vector<double> rc(3 * (size_t)n_coarse, 0.0), ec(3 * (size_t)n_coarse, 0.0);
+    for (uint32_t i = 0; i < nv; ++i)
+      for (int c = 0; c < 3; ++c) rc[3 * agg0[i] + c] += r[3 * i + c];
+    l1.apply(rc, ec);
+    for (uint32_t i = 0; i < nv; ++i)
+      for (int c = 0; c < 3; ++c) z[3 * i + c] += weight * ec[3 * agg0[i] + c];
+  }
+};
+
+// linsolve.rs:272-411 with Precond::Schwarz / SchwarzMl (and the cg_run loop of
+// cg_block_jacobi): weight 0 = one level, otherwise the two-level form above.
+inline std::vector<float> cg_schwarz(const Csr &a, const std::vector<float> &b, float tol, uint32_t max_iter, uint32_t domain, double weight,
+                                     CgStats *st) {
   const uint32_t n = a.n;
-  Schwarz sz;
-  sz.domains = Schwarz::build_domains(a, domain);
-  sz.factorize(a);
+  Schwarz2 sz;
+  sz.build(a, domain, weight);
   auto spmv = [&](const std::vector<double> &x, std::vector<double> &out) {
     for (uint32_t row = 0; row < n; ++row) {
       double s = 0;

@@ -54,9 +54,9 @@ def load():
         lib.oracle_solver_x_tilde.argtypes = [vp, F]
         lib.oracle_cg.argtypes = [C.c_uint32, U, U, F, F, C.c_float, C.c_uint32, F, U, D]
         lib.oracle_spmv.argtypes = [C.c_uint32, U, U, F, F, F]
-        lib.oracle_cg_schwarz.argtypes = [C.c_uint32, U, U, F, F, C.c_float, C.c_uint32, C.c_uint32, F, U, D]
+        lib.oracle_cg_schwarz.argtypes = [C.c_uint32, U, U, F, F, C.c_float, C.c_uint32, C.c_uint32, C.c_double, F, U, D]
         lib.oracle_schwarz.restype = C.c_uint32
-        lib.oracle_schwarz.argtypes = [C.c_uint32, U, U, F, C.c_uint32, U, U, D, D]
+        lib.oracle_schwarz.argtypes = [C.c_uint32, U, U, F, C.c_uint32, C.c_double, U, U, D, D]
         lib.oracle_point_tri_distance_grad.restype = C.c_double
         lib.oracle_point_tri_distance_grad.argtypes = [C.c_int, D, D]
         lib.oracle_segment_distance_grad.restype = C.c_double
@@ -237,17 +237,17 @@ def spmv(row_ptr, col_idx, values, x):
     return y
 
 
-def cg_schwarz(row_ptr, col_idx, values, b, tol, max_iter, domain):
+def cg_schwarz(row_ptr, col_idx, values, b, tol, max_iter, domain, weight=0.0):
     lib = load()
     row_ptr, col_idx, values, b = u32(row_ptr), u32(col_idx), f32(values), f32(b)
     x = np.zeros_like(b)
     it = C.c_uint32()
     res = C.c_double()
-    lib.oracle_cg_schwarz(len(b), up(row_ptr), up(col_idx), fp(values), fp(b), tol, max_iter, domain, fp(x), C.byref(it), C.byref(res))
+    lib.oracle_cg_schwarz(len(b), up(row_ptr), up(col_idx), fp(values), fp(b), tol, max_iter, domain, weight, fp(x), C.byref(it), C.byref(res))
     return x, it.value, res.value
 
 
-def schwarz(row_ptr, col_idx, values, domain, r=None):
+def schwarz(row_ptr, col_idx, values, domain, r=None, weight=0.0):
     """(domains as a list of vertex arrays, z = M^-1 r or None)."""
     lib = load()
     row_ptr, col_idx, values = u32(row_ptr), u32(col_idx), f32(values)
@@ -256,7 +256,7 @@ def schwarz(row_ptr, col_idx, values, domain, r=None):
     dp = np.zeros(nv + 1, np.uint32)
     dv = np.zeros(nv, np.uint32)
     z = np.zeros(n) if r is not None else None
-    nd = lib.oracle_schwarz(n, up(row_ptr), up(col_idx), fp(values), domain, up(dp), up(dv),
+    nd = lib.oracle_schwarz(n, up(row_ptr), up(col_idx), fp(values), domain, weight, up(dp), up(dv),
                             None if r is None else d64(r).ctypes.data_as(D), None if r is None else z.ctypes.data_as(D))
     doms = [dv[dp[i]:dp[i + 1]] for i in range(nd)]
     return doms, z
