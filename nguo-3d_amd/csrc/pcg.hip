@@ -170,11 +170,12 @@ __global__ __launch_bounds__(BLOCK) void k_pcg_spmv(SellView A, ContactView C, c
   const uint32_t state = MODE != SPMV_PLAIN ? flag[0] : 0u;
   const uint32_t lane = threadIdx.x & (WAVE - 1), slice = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE, row = slice * 64u + lane;
   const bool active = row < A.nb;
-  uint32_t base = 0, width = 0, len = 0;
+  uint32_t base = 0, width = 0, len = 0, ib = 0, ie = 0;
   if (active) {
     base = A.slice_ptr[slice];
     width = (A.slice_ptr[slice + 1] - base) / 64u;
     len = A.row_len[row];
+    if (C.ns) { ib = C.inc_ptr[row]; ie = C.inc_ptr[row + 1]; }
   }
   SlotGroup<MODE> g;
   g.load(A, z, pold, base, 0, len, lane);
@@ -205,8 +206,8 @@ __global__ __launch_bounds__(BLOCK) void k_pcg_spmv(SellView A, ContactView C, c
       g.load(A, z, pold, base, k0, len, lane);
       g.accumulate(y0, y1, y2, beta, k0, len);
     }
-    if (C.ns) {
-      for (uint32_t k = C.inc_ptr[row]; k < C.inc_ptr[row + 1]; ++k) {
+    {
+      for (uint32_t k = ib; k < ie; ++k) {
         const uint32_t e = C.inc[k], s = e >> 2, a = e & 3u;
         float s1 = 0.f, a0 = 0.f, a1 = 0.f, a2 = 0.f;
         for (int b = 0; b < 4; ++b) {
@@ -421,6 +422,8 @@ __global__ __launch_bounds__(BLOCK) void k_minv(SellView A, ContactView C, const
 
 // ---------------------------------------------------------------- host driver
 Pcg::~Pcg() {
+  for (auto &e : ev_)
+    if (e) (void)hipEventDestroy(e);
   if (h_flag_) (void)hipHostFree(h_flag_);
   if (h_scal_) (void)hipHostFree(h_scal_);
 }
@@ -433,7 +436,8 @@ void Pcg::ensure(uint32_t nb, uint32_t ns) {
     part_.reserve(std::max(S, G) + 4 * nz + 8);
   }
   if (!h_flag_) {
-    NGUO_HIP(hipHostMalloc((void **)&h_flag_, 4 * sizeof(uint32_t), hipHostMallocDefault));
+    NGUO_HIP(hipHostMalloc((void **)&h_flag_, 8 * sizeof(uint32_t), hipHostMallocDefault));
+    for (auto &e : ev_) NGUO_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     NGUO_HIP(hipHostMalloc((void **)&h_scal_, 8 * sizeof(double), hipHostMallocDefault));
   }
   if (nb > cap_nb_) {
@@ -519,8 +523,11 @@ PcgStats Pcg::solve(const SellMatrix &M, const ContactView &C, const float *b, f
     double eta = std::min(0.5, std::max(0.25 * (double)tol / res, 1.0e-5));
     launch_precond_init(M, bufA);
     hipLaunchKernelGGL(k_init_scal, dim3(1), dim3(BLOCK), 0, s_, scal_.p, flag_.p, (const double *)bufA, nz, eta * eta, max_iter - st.iters);
-    uint32_t j = 0;
-    while (true) {
+    // Chunks of `chunk` iterations are queued one ahead of the one whose state the host
+    // reads, so the device never idles on the convergence check; iterations queued after
+    // convergence exit at their first instruction.
+    uint32_t j = 0, queued = 0, waited = 0;
+    auto queue_chunk = [&]() {
       for (uint32_t c = 0; c < chunk; ++c, ++j) {
         // iteration j: p_j in p_ (even j) or p2_ (odd j); rz_j partials in cur, rz_{j+1} -> nxt
         float *pj = (j & 1) ? p2_.p : p_.p, *pprev = (j & 1) ? p_.p : p2_.p;
@@ -537,13 +544,23 @@ PcgStats Pcg::solve(const SellMatrix &M, const ContactView &C, const float *b, f
                                 z_.p, d_.p, pj, (const float *)q_.p, (const double *)part_spmv, NS, (const double *)cur, nxt, scal_.p, flag_.p);
       }
       NGUO_HIP(hipGetLastError());
-      NGUO_HIP(hipMemcpyAsync(h_flag_, flag_.p, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, s_));
-      NGUO_HIP(hipStreamSynchronize(s_));
-      if (h_flag_[0]) break;
+      const uint32_t sl = queued & 1u;
+      NGUO_HIP(hipMemcpyAsync(h_flag_ + 4 * sl, flag_.p, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, s_));
+      NGUO_HIP(hipEventRecord(ev_[sl], s_));
+      ++queued;
+    };
+    const uint32_t *fl = nullptr;
+    queue_chunk();
+    while (true) {
+      queue_chunk();
+      const uint32_t sl = waited & 1u;
+      NGUO_HIP(hipEventSynchronize(ev_[sl]));
+      ++waited;
+      if (h_flag_[4 * sl]) { fl = h_flag_ + 4 * sl; break; }
     }
-    st.iters += h_flag_[1];
+    st.iters += fl[1];
     st.sweeps += 1;
-    if (h_flag_[0] == 2) st.breakdown = true;
+    if (fl[0] == 2) st.breakdown = true;
     hipLaunchKernelGGL(k_accum, dim3(G3), dim3(BLOCK), 0, s_, n3, x64_.p, (const float *)d_.p);
     if (C.ns) hipLaunchKernelGGL((k_contact_sums<double, double>), dim3(GS), dim3(BLOCK), 0, s_, C, (const double *)x64_.p, sums64_.p, nullptr);
     hipLaunchKernelGGL(k_residual64, dim3(G), dim3(BLOCK), 0, s_, A, C, (const double *)sums64_.p, b, (const double *)x64_.p, r64_.p, part_.p);
@@ -551,7 +568,7 @@ PcgStats Pcg::solve(const SellMatrix &M, const ContactView &C, const float *b, f
     NGUO_HIP(hipMemcpyAsync(h_scal_ + 7, scal_.p + 7, sizeof(double), hipMemcpyDeviceToHost, s_));
     NGUO_HIP(hipStreamSynchronize(s_));
     rr = h_scal_[7];
-    if (st.breakdown && h_flag_[1] == 0) break;
+    if (st.breakdown && fl[1] == 0) break;
   }
   hipLaunchKernelGGL(k_to_f32, dim3(G3), dim3(BLOCK), 0, s_, n3, (const double *)x64_.p, x_out);
   NGUO_HIP(hipGetLastError());

@@ -59,27 +59,35 @@ __global__ __launch_bounds__(BLOCK) void k_schwarz_build(SellView A, ContactView
   __shared__ double L[MD * MD];
   __shared__ double W[MD * MD];
   __shared__ double diag0[MD];
-  __shared__ double red[BLOCK];
+  __shared__ double colj[MD];
+  __shared__ double floor_v;
   __shared__ int failed;
   const uint32_t d = blockIdx.x, t = threadIdx.x;
+  // 16 x 16 thread grid for the dense loops (row index ti, column index tj)
+  const uint32_t ti = t / 16, tj = t % 16;
   const uint32_t base = dom_ptr[d], nv = dom_ptr[d + 1] - base, m = 3 * nv;
   for (uint32_t i = t; i < m * m; i += BLOCK) L[i] = 0.0;
   if (t == 0) failed = 0;
   __syncthreads();
-  // elastic blocks of the domain rows (one thread per domain row: no write conflicts)
-  for (uint32_t p = t; p < nv; p += BLOCK) {
-    uint32_t row = dom_row[base + p];
-    uint32_t sbase = A.slice_ptr[row >> 6], lane = row & 63u;
-    for (uint32_t k = 0; k < A.row_len[row]; ++k) {
-      uint32_t sl = sbase + 64u * k + lane;
-      uint32_t c = A.col[sl];
+  // elastic blocks of the domain rows: 8 threads per row, one writer per block
+  for (uint32_t p = t / 8; p < nv; p += BLOCK / 8) {
+    const uint32_t row = dom_row[base + p];
+    const uint32_t sbase = A.slice_ptr[row >> 6], lane = row & 63u, len = A.row_len[row];
+    for (uint32_t k = t % 8; k < len; k += 8) {
+      const uint32_t sl = sbase + 64u * k + lane;
+      const uint32_t c = A.col[sl];
       if (row_dom[c] != d) continue;
-      uint32_t q = row_pos[c];
+      const uint32_t q = row_pos[c];
       for (int r = 0; r < 3; ++r)
         for (int cc = 0; cc < 3; ++cc) L[(3 * p + r) * m + 3 * q + cc] = (double)A.val[sell_val_index(sl, 3 * r + cc)];
     }
-    // contact: hp g_a g_b^T + lambda w_a w_b (I - n n^T) for partners inside the domain
-    if (C.ns) {
+  }
+  __syncthreads();
+  // contact: hp g_a g_b^T + lambda w_a w_b (I - n n^T) for partners inside the domain (one
+  // thread per row: the row's incidences are added in list order)
+  if (C.ns) {
+    for (uint32_t p = t; p < nv; p += BLOCK) {
+      const uint32_t row = dom_row[base + p];
       for (uint32_t k = C.inc_ptr[row]; k < C.inc_ptr[row + 1]; ++k) {
         uint32_t e = C.inc[k], s = e >> 2, a = e & 3u;
         double hp = C.coef[2 * s], lam = C.coef[2 * s + 1];
@@ -99,40 +107,46 @@ __global__ __launch_bounds__(BLOCK) void k_schwarz_build(SellView A, ContactView
         }
       }
     }
+    __syncthreads();
+  }
+  // floor 1e-6 tr/m + 1e-8 (linsolve.rs:232-238)
+  if (t == 0) {
+    double tr = 0.0;
+    for (uint32_t k = 0; k < m; ++k) tr += L[k * m + k];
+    floor_v = 1e-6 * (tr / (double)m) + 1e-8;
   }
   __syncthreads();
-  // floor 1e-6 tr/m + 1e-8 (linsolve.rs:232-238)
-  double tr = 0.0;
-  if (t == 0)
-    for (uint32_t k = 0; k < m; ++k) tr += L[k * m + k];
-  if (t == 0) red[0] = 1e-6 * (tr / (double)m) + 1e-8;
+  for (uint32_t k = t; k < m; k += BLOCK) { L[k * m + k] += floor_v; diag0[k] = L[k * m + k]; }
   __syncthreads();
-  double fl = red[0];
-  __syncthreads();
-  for (uint32_t k = t; k < m; k += BLOCK) { L[k * m + k] += fl; diag0[k] = L[k * m + k]; }
-  __syncthreads();
-  // left-looking Cholesky, column j (linsolve.rs:80-99 order: sums over k ascending)
+  // Cholesky, right-looking: after column j is scaled, its outer product leaves the trailing
+  // lower triangle. Entry (i, jj) receives its updates for k = 0, 1, ... in order, so each
+  // entry sees the same operations in the same order as the left-looking form
+  // s = a_ij - sum_{k<j} l_ik l_jk of linsolve.rs:80-99. The scaled column is also kept in
+  // colj, so the update reads it without strided LDS accesses.
   for (uint32_t j = 0; j < m; ++j) {
     if (t == 0) {
-      double s = L[j * m + j];
-      for (uint32_t k = 0; k < j; ++k) s -= L[j * m + k] * L[j * m + k];
+      const double s = L[j * m + j];
       if (s <= 1e-14) failed = 1;
       else L[j * m + j] = sqrt(s);
     }
     __syncthreads();
     if (failed) break;
+    const double ljj = L[j * m + j];
     for (uint32_t i = j + 1 + t; i < m; i += BLOCK) {
-      double s = L[i * m + j];
-      for (uint32_t k = 0; k < j; ++k) s -= L[i * m + k] * L[j * m + k];
-      L[i * m + j] = s / L[j * m + j];
+      const double v = L[i * m + j] / ljj;
+      L[i * m + j] = v;
+      colj[i] = v;
     }
+    __syncthreads();
+    for (uint32_t i = j + 1 + ti; i < m; i += 16)
+      for (uint32_t jj = j + 1 + tj; jj <= i; jj += 16) L[i * m + jj] -= colj[i] * colj[jj];
     __syncthreads();
   }
   float *out = ainv + (size_t)d * MD * MD;
+  const uint32_t m4 = (m + 3) & ~3u;
   if (failed) {
     // diagonal fallback: factor sqrt(a_pp) (or 1) -> inverse 1 / a_pp
     if (t == 0) atomicAdd(fails, 1u);
-    const uint32_t m4 = (m + 3) & ~3u;
     for (uint32_t i = t; i < m * m4; i += BLOCK) {
       uint32_t p = i / m4, q = i % m4;
       double v = diag0[p];
@@ -140,26 +154,31 @@ __global__ __launch_bounds__(BLOCK) void k_schwarz_build(SellView A, ContactView
     }
     return;
   }
-  // W = L^-1 (lower), one column per thread
-  for (uint32_t j = t; j < m; j += BLOCK) {
-    for (uint32_t i = 0; i < j; ++i) W[i * m + j] = 0.0;
-    for (uint32_t i = j; i < m; ++i) {
-      double s = i == j ? 1.0 : 0.0;
-      for (uint32_t k = j; k < i; ++k) s -= L[i * m + k] * W[k * m + j];
-      W[i * m + j] = s / L[i * m + i];
-    }
-  }
+  // W = L^-1 (lower) by forward substitution on all columns at once: row i is scaled by
+  // 1 / l_ii, then its multiples leave the rows below (entry (r, c) accumulates
+  // -l_rk w_kc for k = c .. r-1 in order, then the division, as a column-by-column
+  // substitution would).
+  for (uint32_t e = t; e < m * m; e += BLOCK) W[e] = (e / m == e % m) ? 1.0 : 0.0;
   __syncthreads();
-  // A^-1 = W^T W; columns m..m4-1 of rows < m are zero (the apply reads float4 groups)
-  const uint32_t m4 = (m + 3) & ~3u;
-  for (uint32_t i = t; i < m * m4; i += BLOCK) {
-    uint32_t p = i / m4, q = i % m4;
-    if (q >= m) { out[p * MD + q] = 0.0f; continue; }
-    uint32_t k0 = p > q ? p : q;
-    double s = 0.0;
-    for (uint32_t k = k0; k < m; ++k) s += W[k * m + p] * W[k * m + q];
-    out[p * MD + q] = (float)s;
+  for (uint32_t i = 0; i < m; ++i) {
+    const double lii = L[i * m + i];
+    for (uint32_t c = t; c <= i; c += BLOCK) W[i * m + c] = W[i * m + c] / lii;
+    __syncthreads();
+    for (uint32_t r = i + 1 + ti; r < m; r += 16) {
+      const double lri = L[r * m + i];
+      for (uint32_t c = tj; c <= i; c += 16) W[r * m + c] -= lri * W[i * m + c];
+    }
+    __syncthreads();
   }
+  // A^-1 = W^T W; columns m..m4-1 of rows < m are zero (the apply reads float4 groups)
+  for (uint32_t p = ti; p < m; p += 16)
+    for (uint32_t q = tj; q < m4; q += 16) {
+      if (q >= m) { out[p * MD + q] = 0.0f; continue; }
+      const uint32_t k0 = p > q ? p : q;
+      double s = 0.0;
+      for (uint32_t k = k0; k < m; ++k) s += W[k * m + p] * W[k * m + q];
+      out[p * MD + q] = (float)s;
+    }
 }
 
 void Pcg::build_precond(const SellMatrix &M, const ContactView &C) {
