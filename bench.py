@@ -38,6 +38,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps timed for the CPU baseline (0 = skip)")
     ap.add_argument("--grid", type=int, default=2048, help="side of the synthetic SpMV grid (0 = skip)")
+    ap.add_argument("--traffic-bytes", type=float, default=None,
+                    help="HBM bytes per launch of the roofline kernel from the rocprofv3 FETCH_SIZE/WRITE_SIZE passes")
+    ap.add_argument("--copy-calib-mib", type=int, default=0,
+                    help="after the bench, copy a buffer of this many MiB once (counter calibration for --pmc passes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus}).", file=sys.stderr)
@@ -71,11 +75,15 @@ def main():
     elapsed = time.perf_counter() - t0
     ms = 1000.0 * elapsed / args.steps
 
-    spmv_ms, spmv_bytes = st.bench_spmv(reps=500)
+    # the dominant kernel: the SpMV instance of the PCG iteration on the final state's system
+    spmv_ms, spmv_bytes = st.bench_spmv(reps=500, iteration_form=True)
     achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9
-    roofline = {"kernel": "k_spmv (SELL-64 3x3 BSR + matrix-free contact)", "bound": "hbm", "achieved": round(achieved, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "bytes_per_launch": spmv_bytes, "avg_launch_us": round(spmv_ms * 1000, 3)}
+    plain_ms, plain_bytes = st.bench_spmv(reps=500, iteration_form=False)
+    roofline = {"kernel": "k_pcg_spmv<SPMV_NEXT> (SELL-64 3x3 BSR + matrix-free contact, p = z + beta p fused)", "bound": "hbm",
+                "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": args.traffic_bytes, "bytes_per_launch": spmv_bytes, "avg_launch_us": round(spmv_ms * 1000, 3),
+                "plain_spmv": {"avg_launch_us": round(plain_ms * 1000, 3), "bytes_per_launch": plain_bytes,
+                               "achieved": round(plain_bytes / (plain_ms * 1e-3) / 1e9, 1)}}
     grid = None
     if args.grid:
         g_ms, g_bytes, g_nnzb = nh.bench_spmv_grid(ctx, args.grid, args.grid, reps=20)
@@ -120,6 +128,13 @@ def main():
     print(json.dumps(out))
     st.close()
     ctx.close()
+    if args.copy_calib_mib:
+        import torch
+        src = torch.ones(args.copy_calib_mib * 262144, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
 
 
 if __name__ == "__main__":

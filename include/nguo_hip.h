@@ -179,9 +179,12 @@ int nguo_step_collider_sides(nguo_step *st, const float *x, float *side);
 
 /* ---- measurement (no reference counterpart: the reference has no GPU timestamps,
  * core-engine/docs/WGPU_PIN.md:142-144). Average duration of the PCG operator y = (A+C) x on
- * the current Newton system, timed with HIP events on the library stream; *bytes receives
- * the algorithmic bytes of one 3x3-BSR SpMV (nnzb*40 + nb*28 + 4). */
-int nguo_step_bench_spmv(nguo_step *st, uint32_t reps, double *avg_ms, double *bytes);
+ * the current Newton system, timed with HIP events on the library stream over `reps`
+ * back-to-back launches. iteration_form = 0: y = (A+C) x alone, *bytes = nnzb*40 + nb*28 + 4;
+ * iteration_form = 1: the kernel instance the PCG iteration launches (also gathers z,
+ * evaluates and stores p = z + beta p_old, reduces the previous partials),
+ * *bytes = nnzb*40 + nb*52 + 4. */
+int nguo_step_bench_spmv(nguo_step *st, uint32_t reps, int iteration_form, double *avg_ms, double *bytes);
 /* The same operator on a synthetic nx x ny cloth-grid BSR (13 blocks per interior row). */
 int nguo_bench_spmv_grid(nguo_ctx *ctx, uint32_t nx, uint32_t ny, uint32_t reps, double *avg_ms, double *bytes, double *nnzb);
 

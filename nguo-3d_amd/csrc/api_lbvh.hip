@@ -83,15 +83,43 @@ int nguo_lbvh_aabb(nguo_ctx *ctx, uint32_t n, const uint32_t *left, const uint32
       while (node != 0 && hops <= n) { node = parent[node]; ++hops; }
       NGUO_REQUIRE(node == 0, NGUO_E_INVALID, "nguo_lbvh_aabb: child lists contain a cycle");
     }
-    DevBuf<uint32_t> l, r, p, v;
-    DevBuf<float> lb, nb;
-    l.upload(left, n - 1, s);
-    r.upload(right, n - 1, s);
-    p.upload(parent, s);
-    v.reserve(n);
+    // leaf ranges bottom-up (children before parents: reverse BFS order from the root);
+    // a tree whose subtrees cover contiguous sorted leaves (every Karras tree) takes the
+    // range path, any other binary tree the climb
+    std::vector<uint32_t> order{0};
+    for (size_t k = 0; k < order.size(); ++k)
+      for (uint32_t c : {left[order[k]], right[order[k]]})
+        if (!(c & LEAF)) order.push_back(c);
+    std::vector<uint32_t> range(2 * (size_t)(n - 1));
+    bool contiguous = true;
+    auto span = [&](uint32_t c, uint32_t &a, uint32_t &b) {
+      if (c & LEAF) { a = b = c & ~LEAF; }
+      else { a = range[2 * c]; b = range[2 * c + 1]; }
+    };
+    for (size_t k = order.size(); k-- > 0;) {
+      const uint32_t i = order[k];
+      uint32_t la, lb2, ra, rb;
+      span(left[i], la, lb2);
+      span(right[i], ra, rb);
+      if (lb2 + 1 != ra) contiguous = false;
+      range[2 * i] = la;
+      range[2 * i + 1] = rb;
+    }
+    DevBuf<float> lb, nb, table;
     lb.upload(leaf_box, 6 * (size_t)n, s);
     nb.reserve(6 * (size_t)n);
-    aabb_build(n, l.p, r.p, p.p, v.p, lb.p, nb.p, s);
+    if (contiguous) {
+      DevBuf<uint32_t> rg;
+      rg.upload(range, s);
+      aabb_from_ranges(n, rg.p, lb.p, nb.p, table, s);
+    } else {
+      DevBuf<uint32_t> l, r, p, v;
+      l.upload(left, n - 1, s);
+      r.upload(right, n - 1, s);
+      p.upload(parent, s);
+      v.reserve(n);
+      aabb_build(n, l.p, r.p, p.p, v.p, lb.p, nb.p, s);
+    }
     nb.download(node_box, 6 * (size_t)(n - 1), s);
     NGUO_HIP(hipStreamSynchronize(s));
     return NGUO_OK;

@@ -240,7 +240,7 @@ int nguo_bench_spmv_grid(nguo_ctx *ctx, uint32_t nx, uint32_t ny, uint32_t reps,
     x.upload(hx, s);
     Pcg pcg(s);
     ContactView none{};
-    *avg_ms = pcg.time_spmv(m.view(), none, x.p, reps);
+    *avg_ms = pcg.time_spmv(m.view(), none, x.p, reps, false);
     *bytes = m.spmv_bytes();
     if (nnzb_out) *nnzb_out = (double)m.nnzb;
     return NGUO_OK;

@@ -84,6 +84,8 @@ struct ContactView {
   float *sums;                    // [4 ns] per-apply: g.p (1) and sum w p (3)
   const uint32_t *inc_ptr;        // [nb + 1] incidence of free vertices
   const uint32_t *inc;            // [..] 4 s + a
+  const uint32_t *pos;            // [4 ns] incidence slot of participant (s, a), or ~0u (pinned)
+  float4 *contrib;                // [..] per-apply: H_s p restricted to the incidence's vertex
 };
 
 }  // namespace nguo

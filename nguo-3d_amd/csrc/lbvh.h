@@ -1,6 +1,7 @@
 // LBVH over triangles or edges: 30-bit Morton keys of primitive centroids normalised by the
 // centroid bounds, a stable LSD radix sort (8-bit digits, wavefront multisplit ranking in
-// LDS), Karras (2012) internal nodes and a single-pass bottom-up AABB union.
+// LDS), Karras (2012) internal nodes with their leaf ranges, and node boxes as unions over
+// those ranges (sparse table of leaf-box unions).
 //
 // Reference stages (engine-gpu/src/): morton.rs:12-58 + morton.wgsl, radix.rs:102-135 +
 // radix.wgsl (serial per-block scatter, n <= 65 536), scan.rs, karras.rs:24-88 + karras.wgsl,
@@ -36,6 +37,9 @@ struct BvhView {
   const float *node_box;  // [6 (n-1)] lo.xyz hi.xyz
   const float *leaf_box;  // [6 n] in sorted order
   const uint32_t *prim;   // [n] primitive id of sorted leaf slot
+  // [4 (n-1)] traversal nodes: both child boxes and child refs of internal node i in one
+  // 64-byte record (lo_l.xyz hi_l.x | hi_l.yz lo_r.xy | lo_r.z hi_r.xyz | left right - -)
+  const float4 *trav;
 };
 
 class Lbvh {
@@ -43,7 +47,7 @@ class Lbvh {
   // kind 0: triangles (3 indices per primitive), 1: edges (2 indices). `ids` selects the
   // primitives of this tree (device array of primitive indices into `idx`).
   void build(int kind, uint32_t n, const uint32_t *ids, const uint32_t *idx, const float *pos, hipStream_t s);
-  BvhView view() const { return BvhView{n_, left_.p, right_.p, node_box_.p, leaf_box_.p, prim_.p}; }
+  BvhView view() const { return BvhView{n_, left_.p, right_.p, node_box_.p, leaf_box_.p, prim_.p, (const float4 *)trav_.p}; }
   uint32_t size() const { return n_; }
   // Stage outputs for parity checks.
   const uint32_t *codes() const { return keys_.p; }   // sorted Morton keys
@@ -51,8 +55,8 @@ class Lbvh {
 
  private:
   uint32_t n_ = 0;
-  DevBuf<uint32_t> keys_, prim_, left_, right_, parent_, visit_;
-  DevBuf<float> leaf_box_, node_box_, bounds_, cent_;
+  DevBuf<uint32_t> keys_, prim_, left_, right_, range_, visit_;
+  DevBuf<float> leaf_box_, node_box_, bounds_, cent_, table_, trav_;
 };
 
 // Kernels exposed for the stage-by-stage C ABI test entry points.
@@ -61,6 +65,9 @@ void morton_codes(int kind, uint32_t n, const uint32_t *ids, const uint32_t *idx
 void karras_build(uint32_t n, const uint32_t *codes, uint32_t *left, uint32_t *right, hipStream_t s);
 void aabb_build(uint32_t n, const uint32_t *left, const uint32_t *right, uint32_t *parent, uint32_t *visit, const float *leaf_box,
                 float *node_box, hipStream_t s);
+// Node boxes of a tree whose internal node i covers the contiguous sorted leaves
+// [range[2i], range[2i+1]] (every Karras tree); `table` is scratch.
+void aabb_from_ranges(uint32_t n, const uint32_t *range, const float *leaf_box, float *node_box, DevBuf<float> &table, hipStream_t s);
 
 // Stack traversal: calls visit(prim) for every leaf whose box passes `overlap(box_lo,
 // box_hi)` (a conservative test). A stack deeper than TRAV_STACK sets *err (reported to the
@@ -78,20 +85,21 @@ __device__ inline void bvh_traverse(const BvhView &t, uint32_t *err, Overlap ove
   int sp = 0;
   stack[sp++] = 0;  // root
   while (sp > 0) {
-    uint32_t node = stack[--sp];
-    uint32_t ch[2] = {t.left[node], t.right[node]};
+    const uint32_t node = stack[--sp];
+    const float4 *rec = t.trav + 4 * (size_t)node;
+    const float4 q0 = rec[0], q1 = rec[1], q2 = rec[2], q3 = rec[3];
+    const uint32_t ch[2] = {__float_as_uint(q3.x), __float_as_uint(q3.y)};
+    const f3 lo[2] = {f3{q0.x, q0.y, q0.z}, f3{q1.z, q1.w, q2.x}};
+    const f3 hi[2] = {f3{q0.w, q1.x, q1.y}, f3{q2.y, q2.z, q2.w}};
     for (int c = 0; c < 2; ++c) {
-      uint32_t r = ch[c];
+      const uint32_t r = ch[c];
+      if (!overlap(lo[c], hi[c])) continue;
       if (r & LEAF) {
-        uint32_t li = r & ~LEAF;
-        const float *b = t.leaf_box + 6 * (size_t)li;
-        if (overlap(f3{b[0], b[1], b[2]}, f3{b[3], b[4], b[5]})) visit(t.prim[li]);
+        visit(t.prim[r & ~LEAF]);
+      } else if (sp < TRAV_STACK) {
+        stack[sp++] = r;
       } else {
-        const float *b = t.node_box + 6 * (size_t)r;
-        if (overlap(f3{b[0], b[1], b[2]}, f3{b[3], b[4], b[5]})) {
-          if (sp < TRAV_STACK) stack[sp++] = r;
-          else atomicOr(err, 1u);
-        }
+        atomicOr(err, 1u);
       }
     }
   }

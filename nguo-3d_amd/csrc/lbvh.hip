@@ -225,7 +225,8 @@ __device__ inline int kdelta(const uint32_t *codes, int n, int i, int j) {
 }
 
 // engine-gpu/src/karras.rs:24-88 (Karras 2012, section 4).
-__global__ __launch_bounds__(BLOCK) void k_karras(uint32_t n, const uint32_t *codes, uint32_t *left, uint32_t *right, uint32_t *parent) {
+__global__ __launch_bounds__(BLOCK) void k_karras(uint32_t n, const uint32_t *codes, uint32_t *left, uint32_t *right, uint32_t *parent,
+                                                 uint32_t *range) {
   int i = (int)(blockIdx.x * BLOCK + threadIdx.x);
   int nn = (int)n;
   if (i >= nn - 1) return;
@@ -251,6 +252,7 @@ __global__ __launch_bounds__(BLOCK) void k_karras(uint32_t n, const uint32_t *co
   uint32_t rc = last == gamma + 1 ? ((uint32_t)(gamma + 1) | LEAF) : (uint32_t)(gamma + 1);
   left[i] = lc;
   right[i] = rc;
+  if (range) { range[2 * i] = (uint32_t)first; range[2 * i + 1] = (uint32_t)last; }
   if (parent) {
     // parent[0 .. n-1) internal, parent[n-1 + leaf] leaves
     parent[(lc & LEAF) ? (n - 1 + (lc & ~LEAF)) : lc] = (uint32_t)i;
@@ -260,7 +262,8 @@ __global__ __launch_bounds__(BLOCK) void k_karras(uint32_t n, const uint32_t *co
 
 void karras_build(uint32_t n, const uint32_t *codes, uint32_t *left, uint32_t *right, hipStream_t s) {
   if (n < 2) return;
-  hipLaunchKernelGGL(k_karras, dim3(div_up(n - 1, BLOCK)), dim3(BLOCK), 0, s, n, codes, left, right, (uint32_t *)nullptr);
+  hipLaunchKernelGGL(k_karras, dim3(div_up(n - 1, BLOCK)), dim3(BLOCK), 0, s, n, codes, left, right, (uint32_t *)nullptr,
+                     (uint32_t *)nullptr);
   NGUO_HIP(hipGetLastError());
 }
 
@@ -289,7 +292,8 @@ __global__ __launch_bounds__(BLOCK) void k_leaf_boxes(int kind, uint32_t n, cons
   o[0] = lo.x; o[1] = lo.y; o[2] = lo.z; o[3] = hi.x; o[4] = hi.y; o[5] = hi.z;
 }
 
-// Bottom-up union (one thread per leaf climbs; the second arrival at a node merges).
+// Bottom-up union for a general binary tree (one thread per leaf climbs; the second arrival
+// at a node merges). Used where leaf ranges are not contiguous.
 __global__ __launch_bounds__(BLOCK) void k_aabb_climb(uint32_t n, const uint32_t *left, const uint32_t *right, const uint32_t *parent,
                                                      uint32_t *visit, const float *leaf_box, float *node_box) {
   uint32_t li = blockIdx.x * BLOCK + threadIdx.x;
@@ -313,6 +317,64 @@ __global__ __launch_bounds__(BLOCK) void k_aabb_climb(uint32_t n, const uint32_t
     if (node == 0) return;
     node = parent[node];
   }
+}
+
+// Node boxes from leaf ranges. The leaves of a Karras node form one contiguous range
+// [first, last] of the sorted order, so its box is the union of those leaf boxes. A sparse
+// table level k holds the union of the 2^k leaves starting at each index; a node's box is
+// the union of two (overlapping) level-k entries covering its range. Min and max are exact,
+// so the result equals the bottom-up climb's, without cross-workgroup fences on the
+// critical path: about log2(n) short launches.
+__global__ __launch_bounds__(BLOCK) void k_union_level(uint32_t count, uint32_t half, const float *in, float *out) {
+  uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= count) return;
+  const float *a = in + 6 * (size_t)i, *b = in + 6 * ((size_t)i + half);
+  float *o = out + 6 * (size_t)i;
+  for (int k = 0; k < 3; ++k) { o[k] = fminf(a[k], b[k]); o[3 + k] = fmaxf(a[3 + k], b[3 + k]); }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_node_boxes(uint32_t n, const uint32_t *range, const float *leaf_box, const float *table,
+                                                     float *node_box) {
+  uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i + 1 >= n) return;
+  const uint32_t a = range[2 * i], b = range[2 * i + 1], len = b - a + 1;
+  const int k = 31 - __clz(len);
+  const float *lv = k == 0 ? leaf_box : table + (size_t)(k - 1) * 6 * n;
+  const float *x = lv + 6 * (size_t)a, *y = lv + 6 * ((size_t)b + 1 - (1u << k));
+  float *o = node_box + 6 * (size_t)i;
+  for (int c = 0; c < 3; ++c) { o[c] = fminf(x[c], y[c]); o[3 + c] = fmaxf(x[3 + c], y[3 + c]); }
+}
+
+void aabb_from_ranges(uint32_t n, const uint32_t *range, const float *leaf_box, float *node_box, DevBuf<float> &table, hipStream_t s) {
+  if (n < 2) return;
+  int levels = 0;
+  while ((2u << levels) <= n) ++levels;  // floor(log2 n)
+  table.reserve((size_t)std::max(levels, 1) * 6 * n);
+  for (int k = 1; k <= levels; ++k) {
+    const uint32_t half = 1u << (k - 1), count = n - (1u << k) + 1;
+    const float *in = k == 1 ? leaf_box : table.p + (size_t)(k - 2) * 6 * n;
+    hipLaunchKernelGGL(k_union_level, dim3(div_up(count, BLOCK)), dim3(BLOCK), 0, s, count, half, in, table.p + (size_t)(k - 1) * 6 * n);
+  }
+  hipLaunchKernelGGL(k_node_boxes, dim3(div_up(n - 1, BLOCK)), dim3(BLOCK), 0, s, n, range, leaf_box, (const float *)table.p, node_box);
+  NGUO_HIP(hipGetLastError());
+}
+
+// Traversal records: child boxes and refs of each internal node in 64 bytes.
+__global__ __launch_bounds__(BLOCK) void k_pack_trav(uint32_t n, const uint32_t *left, const uint32_t *right, const float *leaf_box,
+                                                    const float *node_box, float4 *trav) {
+  uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i + 1 >= n) return;
+  const uint32_t ch[2] = {left[i], right[i]};
+  float b[2][6];
+  for (int c = 0; c < 2; ++c) {
+    const float *src = (ch[c] & LEAF) ? leaf_box + 6 * (size_t)(ch[c] & ~LEAF) : node_box + 6 * (size_t)ch[c];
+    for (int k = 0; k < 6; ++k) b[c][k] = src[k];
+  }
+  float4 *o = trav + 4 * (size_t)i;
+  o[0] = make_float4(b[0][0], b[0][1], b[0][2], b[0][3]);
+  o[1] = make_float4(b[0][4], b[0][5], b[1][0], b[1][1]);
+  o[2] = make_float4(b[1][2], b[1][3], b[1][4], b[1][5]);
+  o[3] = make_float4(__uint_as_float(ch[0]), __uint_as_float(ch[1]), 0.0f, 0.0f);
 }
 
 void aabb_build(uint32_t n, const uint32_t *left, const uint32_t *right, uint32_t *parent, uint32_t *visit, const float *leaf_box, float *node_box,
@@ -340,7 +402,7 @@ void Lbvh::build(int kind, uint32_t n, const uint32_t *ids, const uint32_t *idx,
   leaf_box_.reserve(6 * (size_t)n);
   left_.reserve(n);
   right_.reserve(n);
-  parent_.reserve(2 * (size_t)n);
+  range_.reserve(2 * (size_t)n);
   visit_.reserve(n);
   node_box_.reserve(6 * (size_t)n);
   DevBuf<uint32_t> &order = visit_;  // reused as the sort payload before the climb
@@ -351,8 +413,12 @@ void Lbvh::build(int kind, uint32_t n, const uint32_t *ids, const uint32_t *idx,
   hipLaunchKernelGGL(k_gather_prim, dim3(g), dim3(BLOCK), 0, s, n, (const uint32_t *)order.p, ids, prim_.p);
   hipLaunchKernelGGL(k_leaf_boxes, dim3(g), dim3(BLOCK), 0, s, kind, n, (const uint32_t *)prim_.p, idx, pos, leaf_box_.p);
   if (n >= 2) {
-    hipLaunchKernelGGL(k_karras, dim3(div_up(n - 1, BLOCK)), dim3(BLOCK), 0, s, n, (const uint32_t *)keys_.p, left_.p, right_.p, parent_.p);
-    aabb_build(n, left_.p, right_.p, parent_.p, visit_.p, leaf_box_.p, node_box_.p, s);
+    hipLaunchKernelGGL(k_karras, dim3(div_up(n - 1, BLOCK)), dim3(BLOCK), 0, s, n, (const uint32_t *)keys_.p, left_.p, right_.p,
+                       (uint32_t *)nullptr, range_.p);
+    aabb_from_ranges(n, range_.p, leaf_box_.p, node_box_.p, table_, s);
+    trav_.reserve(16 * (size_t)n);
+    hipLaunchKernelGGL(k_pack_trav, dim3(div_up(n - 1, BLOCK)), dim3(BLOCK), 0, s, n, (const uint32_t *)left_.p, (const uint32_t *)right_.p,
+                       (const float *)leaf_box_.p, (const float *)node_box_.p, (float4 *)trav_.p);
   }
   NGUO_HIP(hipGetLastError());
 }

@@ -21,8 +21,12 @@ struct SellMatrix {
   // slot of block (row, col) from the host pattern, or 0xffffffff.
   uint32_t find(uint32_t row, uint32_t colj) const;
   SellView view() const { return SellView{nb, slice_ptr.p, row_len.p, col.p, val.p}; }
-  // Algorithmic bytes of one y = A x (f32 values, u32 column index; 3x3 blocks).
-  double spmv_bytes() const { return (double)nnzb * 40.0 + (double)nb * 28.0 + 4.0; }
+  // Algorithmic bytes of one y = A x (f32 values, u32 column index per 3x3 block; row
+  // length, x and y per block row). The PCG iteration's instance also reads z and writes p
+  // (24 more bytes per block row).
+  double spmv_bytes(bool iteration_form = false) const {
+    return (double)nnzb * 40.0 + (double)nb * (iteration_form ? 52.0 : 28.0) + 4.0;
+  }
 };
 
 // One-level additive Schwarz over disjoint vertex domains (engine-core/src/linsolve.rs:126-258
@@ -38,6 +42,7 @@ struct Schwarz {
   uint32_t nd = 0, domain = SCHWARZ_MAX_DOMAIN;
   std::vector<uint32_t> h_dom_ptr, h_dom_row;
   DevBuf<uint32_t> dom_ptr, dom_row, row_dom, row_pos, fails;
+  DevBuf<uint32_t> dom_dof;  // [nd * 96]: stride-4 vector index of dof t of domain d, or ~0u past m
   DevBuf<float> ainv;  // nd blocks of SCHWARZ_MAX_DOF^2, row-major with stride SCHWARZ_MAX_DOF
   void build(const SellMatrix &M, uint32_t domain_size, hipStream_t s);
 };
@@ -65,8 +70,10 @@ class Pcg {
   // y = (A + C) x in f32 (stride-3 vectors), for parity checks.
   void apply(const SellView &A, const ContactView &C, const float *x, float *y);
   // Average duration (ms) of the PCG's SpMV kernel on (A + C) x over `reps` back-to-back
-  // launches, between two HIP events on the solver stream.
-  double time_spmv(const SellView &A, const ContactView &C, const float *x, uint32_t reps);
+  // launches, between two HIP events on the solver stream. iteration_form: the instance the
+  // PCG iteration runs (also gathers z, evaluates p = z + beta p_old, stores p, reduces the
+  // previous partials and p.q); otherwise y = (A + C) x alone.
+  double time_spmv(const SellView &A, const ContactView &C, const float *x, uint32_t reps, bool iteration_form);
   // Block-Jacobi inverse of (A + C)'s diagonal blocks (FP64 inversion, scalar fallback).
   void build_precond(const SellMatrix &M, const ContactView &C);
   // Use `sz` (built over the same matrix pattern) instead of 3x3 block Jacobi; null = Jacobi.
@@ -81,6 +88,7 @@ class Pcg {
   void launch_minv(const SellMatrix &M, const ContactView &C);
   void launch_precond_init(const SellMatrix &M, double *part);
   void launch_spmv_plain(const SellView &A, const ContactView &C);
+  void launch_csums_plain(const ContactView &C);
   hipStream_t s_;
   Schwarz *sz_ = nullptr;
   uint32_t cap_nb_ = 0, cap_ns_ = 0;

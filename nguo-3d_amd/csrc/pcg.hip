@@ -152,6 +152,55 @@ template <int MODE> struct SlotGroup {
   }
 };
 
+// Contact term of this iteration's p, one stencil per thread: the sums g . p (over the four
+// participants) and sum_a w_a p_a, participants in order, p evaluated exactly as
+// k_pcg_spmv evaluates it (the same reduced beta, the same expression); then each free
+// participant's block of H_s p, hp (g.p) g_a + lambda w_a (s_w - n (n . s_w)), goes to its
+// incidence slot for the row sum in k_pcg_spmv. NEXT also stops where the SpMV launch after
+// it will stop.
+template <int MODE>
+__global__ __launch_bounds__(BLOCK) void k_csums(ContactView C, const float4 *__restrict__ z, const float4 *__restrict__ pold, const double *prev,
+                                                const double *cur, uint32_t nz, const double *scal, const uint32_t *flag, uint32_t j) {
+  __shared__ double lds[3 * BLOCK / WAVE];
+  if (MODE != SPMV_PLAIN && flag[0]) return;
+  float beta = 0.0f;
+  if (MODE == SPMV_NEXT) {
+    const double *src[3] = {prev, cur, cur + nz};
+    const uint32_t n[3] = {nz, nz, nz};
+    double v[3];
+    sum_parts_all<BLOCK, 3>(src, n, v, lds);
+    if (v[2] <= scal[5] || j >= flag[2]) return;
+    beta = (float)(v[1] / v[0]);
+  }
+  const uint32_t s = blockIdx.x * BLOCK + threadIdx.x;
+  if (s >= C.ns) return;
+  float s1 = 0.f, a0 = 0.f, a1 = 0.f, a2 = 0.f;
+  for (int b = 0; b < 4; ++b) {
+    const uint32_t db = C.dof[4 * (size_t)s + b];
+    if (db == NONE) continue;
+    const float4 x = pval4<MODE>(z, pold, beta, db);
+    const float *gs = C.g + 12 * (size_t)s + 3 * b;
+    s1 += gs[0] * x.x + gs[1] * x.y + gs[2] * x.z;
+    const float w = C.w[4 * (size_t)s + b];
+    a0 += w * x.x; a1 += w * x.y; a2 += w * x.z;
+  }
+  const float hp = C.coef[2 * (size_t)s], lam = C.coef[2 * (size_t)s + 1];
+  const float t = hp * s1;
+  const float *n = C.n + 3 * (size_t)s;
+  const float nd = n[0] * a0 + n[1] * a1 + n[2] * a2;
+  for (int a = 0; a < 4; ++a) {
+    const uint32_t k = C.pos[4 * (size_t)s + a];
+    if (k == NONE) continue;
+    const float *ga = C.g + 12 * (size_t)s + 3 * a;
+    float c0 = t * ga[0], c1 = t * ga[1], c2 = t * ga[2];
+    if (lam != 0.0f) {
+      const float pw = lam * C.w[4 * (size_t)s + a];
+      c0 += pw * (a0 - n[0] * nd); c1 += pw * (a1 - n[1] * nd); c2 += pw * (a2 - n[2] * nd);
+    }
+    C.contrib[k] = make_float4(c0, c1, c2, 0.0f);
+  }
+}
+
 // q = (A + C) p, one SELL slice (64 block rows, one per lane) per wavefront, four slices per
 // workgroup; SPMV_GROUP slots at a time: column indices, then every value and gather load
 // of the group, then the FMAs in slot order. PLAIN: p = pold, no reduction. FIRST: p_0 =
@@ -160,8 +209,8 @@ template <int MODE> struct SlotGroup {
 // flight - stops on convergence or the cap (workgroup 0 records the state), and evaluates
 // p_j = z + beta pold wherever it reads p; the row's owner also stores p_j to pnew, and
 // every reader of a neighbour's p evaluates the same expression, so they agree bitwise.
-// The contact term evaluates each incident stencil's g.p and sum w p inline from its four
-// participants. part[workgroup] = p.q over its rows.
+// The contact term adds the per-incidence blocks k_csums wrote for the same p.
+// part[workgroup] = p.q over its rows.
 template <int MODE>
 __global__ __launch_bounds__(BLOCK) void k_pcg_spmv(SellView A, ContactView C, const float4 *__restrict__ z, const float4 *__restrict__ pold,
                                                    float4 *__restrict__ pnew, float4 *__restrict__ q, double *part, const double *prev,
@@ -206,30 +255,16 @@ __global__ __launch_bounds__(BLOCK) void k_pcg_spmv(SellView A, ContactView C, c
       g.load(A, z, pold, base, k0, len, lane);
       g.accumulate(y0, y1, y2, beta, k0, len);
     }
-    {
-      for (uint32_t k = ib; k < ie; ++k) {
-        const uint32_t e = C.inc[k], s = e >> 2, a = e & 3u;
-        float s1 = 0.f, a0 = 0.f, a1 = 0.f, a2 = 0.f;
-        for (int b = 0; b < 4; ++b) {
-          const uint32_t db = C.dof[4 * (size_t)s + b];
-          if (db == NONE) continue;
-          const float4 x = pval4<MODE>(z, pold, beta, db);
-          const float *gs = C.g + 12 * (size_t)s + 3 * b;
-          s1 += gs[0] * x.x + gs[1] * x.y + gs[2] * x.z;
-          const float w = C.w[4 * (size_t)s + b];
-          a0 += w * x.x; a1 += w * x.y; a2 += w * x.z;
-        }
-        const float hp = C.coef[2 * s], lam = C.coef[2 * s + 1];
-        const float t = hp * s1;
-        const float *ga = C.g + 12 * (size_t)s + 3 * a;
-        y0 += t * ga[0]; y1 += t * ga[1]; y2 += t * ga[2];
-        if (lam != 0.0f) {
-          const float *n = C.n + 3 * (size_t)s;
-          const float nd = n[0] * a0 + n[1] * a1 + n[2] * a2;
-          const float pw = lam * C.w[4 * (size_t)s + a];
-          y0 += pw * (a0 - n[0] * nd); y1 += pw * (a1 - n[1] * nd); y2 += pw * (a2 - n[2] * nd);
-        }
-      }
+    // contact term: the incidences' contributions k_csums wrote for this iteration's p,
+    // contiguous per row, eight loads in flight at a time, added in incidence order
+    for (uint32_t k0 = ib; k0 < ie; k0 += 8) {
+      float4 cv[8];
+#pragma unroll
+      for (uint32_t u = 0; u < 8; ++u)
+        if (k0 + u < ie) cv[u] = C.contrib[k0 + u];
+#pragma unroll
+      for (uint32_t u = 0; u < 8; ++u)
+        if (k0 + u < ie) { y0 += cv[u].x; y1 += cv[u].y; y2 += cv[u].z; }
     }
     q[row] = make_float4(y0, y1, y2, 0.0f);
     if (MODE != SPMV_PLAIN) {
@@ -532,6 +567,14 @@ PcgStats Pcg::solve(const SellMatrix &M, const ContactView &C, const float *b, f
         // iteration j: p_j in p_ (even j) or p2_ (odd j); rz_j partials in cur, rz_{j+1} -> nxt
         float *pj = (j & 1) ? p2_.p : p_.p, *pprev = (j & 1) ? p_.p : p2_.p;
         double *cur = (j & 1) ? bufB : bufA, *nxt = (j & 1) ? bufA : bufB;
+        if (C.ns) {
+          if (j == 0)
+            hipLaunchKernelGGL(k_csums<SPMV_FIRST>, dim3(GS), dim3(BLOCK), 0, s_, C, (const float4 *)z_.p, (const float4 *)pj,
+                               (const double *)nullptr, (const double *)nullptr, nz, (const double *)scal_.p, (const uint32_t *)flag_.p, j);
+          else
+            hipLaunchKernelGGL(k_csums<SPMV_NEXT>, dim3(GS), dim3(BLOCK), 0, s_, C, (const float4 *)z_.p, (const float4 *)pprev,
+                               (const double *)nxt, (const double *)cur, nz, (const double *)scal_.p, (const uint32_t *)flag_.p, j);
+        }
         if (j == 0)
           hipLaunchKernelGGL(k_pcg_spmv<SPMV_FIRST>, dim3(NS), dim3(BLOCK), 0, s_, A, C, (const float4 *)z_.p, (const float4 *)pj,
                              (float4 *)nullptr, (float4 *)q_.p, part_spmv, (const double *)nullptr, (const double *)nullptr, nz, scal_.p,
@@ -581,21 +624,52 @@ void Pcg::apply(const SellView &A, const ContactView &C, const float *x, float *
   if (!A.nb) return;
   const unsigned G = div_up(A.nb, BLOCK);
   hipLaunchKernelGGL(k_pack4, dim3(G), dim3(BLOCK), 0, s_, A.nb, x, (float4 *)p2_.p);
+  if (C.ns) launch_csums_plain(C);
   launch_spmv_plain(A, C);
   hipLaunchKernelGGL(k_unpack4, dim3(G), dim3(BLOCK), 0, s_, A.nb, (const float4 *)q_.p, y);
   NGUO_HIP(hipGetLastError());
 }
 
-double Pcg::time_spmv(const SellView &A, const ContactView &C, const float *x, uint32_t reps) {
+// State under which the iteration-form SpMV runs without stopping: no convergence threshold
+// (rr <= -1 never holds), no cap, every partial 1 (beta = 1).
+__global__ void k_bench_state(uint32_t *flag, double *scal, double *part, uint32_t n) {
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) part[i] = 1.0;
+  if (threadIdx.x == 0) {
+    flag[0] = 0u; flag[1] = 0u; flag[2] = 0xffffffffu;
+    scal[5] = -1.0;
+  }
+}
+
+double Pcg::time_spmv(const SellView &A, const ContactView &C, const float *x, uint32_t reps, bool iteration_form) {
   ensure(A.nb, C.ns);
   if (!A.nb || !reps) return 0.0;
+  const unsigned NS = div_up(div_up(A.nb, 64), BLOCK / WAVE);
+  const uint32_t nz = 64;  // partial count the iteration form reduces (state set to 1 each)
   hipLaunchKernelGGL(k_pack4, dim3(div_up(A.nb, BLOCK)), dim3(BLOCK), 0, s_, A.nb, x, (float4 *)p2_.p);
-  launch_spmv_plain(A, C);  // warm-up
+  hipLaunchKernelGGL(k_pack4, dim3(div_up(A.nb, BLOCK)), dim3(BLOCK), 0, s_, A.nb, x, (float4 *)z_.p);
+  double *prev = part_.p + NS, *cur = prev + 2 * nz;
+  if (iteration_form) hipLaunchKernelGGL(k_bench_state, dim3(1), dim3(BLOCK), 0, s_, flag_.p, scal_.p, prev, 4 * nz);
+  if (C.ns) {
+    // the contact sums of p (fixed over the timed launches)
+    if (iteration_form)
+      hipLaunchKernelGGL(k_csums<SPMV_NEXT>, dim3(div_up(C.ns, BLOCK)), dim3(BLOCK), 0, s_, C, (const float4 *)z_.p, (const float4 *)p2_.p,
+                         (const double *)prev, (const double *)cur, nz, (const double *)scal_.p, (const uint32_t *)flag_.p, 1u);
+    else
+      launch_csums_plain(C);
+  }
+  auto launch = [&]() {
+    if (iteration_form)
+      hipLaunchKernelGGL(k_pcg_spmv<SPMV_NEXT>, dim3(NS), dim3(BLOCK), 0, s_, A, C, (const float4 *)z_.p, (const float4 *)p2_.p, (float4 *)p_.p,
+                         (float4 *)q_.p, part_.p, (const double *)prev, (const double *)cur, nz, scal_.p, flag_.p, 1u);
+    else
+      launch_spmv_plain(A, C);
+  };
+  launch();  // warm-up
   hipEvent_t e0, e1;
   NGUO_HIP(hipEventCreate(&e0));
   NGUO_HIP(hipEventCreate(&e1));
   NGUO_HIP(hipEventRecord(e0, s_));
-  for (uint32_t r = 0; r < reps; ++r) launch_spmv_plain(A, C);
+  for (uint32_t r = 0; r < reps; ++r) launch();
   NGUO_HIP(hipEventRecord(e1, s_));
   NGUO_HIP(hipEventSynchronize(e1));
   float ms = 0;
@@ -604,6 +678,11 @@ double Pcg::time_spmv(const SellView &A, const ContactView &C, const float *x, u
   (void)hipEventDestroy(e1);
   NGUO_HIP(hipGetLastError());
   return (double)ms / reps;
+}
+
+void Pcg::launch_csums_plain(const ContactView &C) {
+  hipLaunchKernelGGL(k_csums<SPMV_PLAIN>, dim3(div_up(C.ns, BLOCK)), dim3(BLOCK), 0, s_, C, (const float4 *)nullptr, (const float4 *)p2_.p,
+                     (const double *)nullptr, (const double *)nullptr, 0u, (const double *)nullptr, (const uint32_t *)nullptr, 0u);
 }
 
 void Pcg::launch_spmv_plain(const SellView &A, const ContactView &C) {

@@ -34,13 +34,29 @@ template <int NT, int K> __device__ inline void wg_sum_all(double (&v)[K], doubl
   __syncthreads();
 }
 
-// v[k] = sum_{i < n[k]} src[k][i] for K partial arrays, every thread gets the sums.
+// v[k] = sum_{i < n[k]} src[k][i] for K partial arrays, every thread gets the sums. Each
+// thread adds its strided share in index order; the loads of up to 4 strides are issued
+// before the adds, so a launch with up to 4 NT partials per array pays one memory round trip.
 template <int NT, int K>
 __device__ inline void sum_parts_all(const double *const (&src)[K], const uint32_t (&n)[K], double (&v)[K], double *lds) {
+  constexpr uint32_t U = 4;
+  uint32_t nmax = 0;
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
-    v[k] = 0.0;
-    for (uint32_t i = threadIdx.x; i < n[k]; i += NT) v[k] += src[k][i];
+  for (int k = 0; k < K; ++k) { v[k] = 0.0; nmax = n[k] > nmax ? n[k] : nmax; }
+  for (uint32_t b = 0; b < nmax; b += U * NT) {
+    double x[K][U];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u) {
+        const uint32_t i = b + u * NT + threadIdx.x;
+        x[k][u] = i < n[k] ? src[k][i] : 0.0;
+      }
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u)
+        if (b + u * NT + threadIdx.x < n[k]) v[k] += x[k][u];
   }
   wg_sum_all<NT, K>(v, lds);
 }

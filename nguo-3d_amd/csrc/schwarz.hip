@@ -12,6 +12,7 @@
 namespace nguo {
 
 constexpr uint32_t NONE_S = 0xffffffffu;
+typedef double double4_t __attribute__((ext_vector_type(4)));
 constexpr uint32_t MD = SCHWARZ_MAX_DOF;
 
 void Schwarz::build(const SellMatrix &M, uint32_t domain_size, hipStream_t s) {
@@ -43,6 +44,11 @@ void Schwarz::build(const SellMatrix &M, uint32_t domain_size, hipStream_t s) {
   std::vector<uint32_t> rd(nv, NONE_S), rp(nv, 0);
   for (uint32_t d = 0; d < nd; ++d)
     for (uint32_t k = h_dom_ptr[d]; k < h_dom_ptr[d + 1]; ++k) { rd[h_dom_row[k]] = d; rp[h_dom_row[k]] = k - h_dom_ptr[d]; }
+  std::vector<uint32_t> dd((size_t)std::max(nd, 1u) * MD, NONE_S);
+  for (uint32_t d = 0; d < nd; ++d)
+    for (uint32_t k = h_dom_ptr[d]; k < h_dom_ptr[d + 1]; ++k)
+      for (uint32_t c = 0; c < 3; ++c) dd[(size_t)d * MD + 3 * (k - h_dom_ptr[d]) + c] = 4 * h_dom_row[k] + c;
+  dom_dof.upload(dd, s);
   dom_ptr.upload(h_dom_ptr, s);
   dom_row.upload(h_dom_row.empty() ? std::vector<uint32_t>{0} : h_dom_row, s);
   row_dom.upload(rd.empty() ? std::vector<uint32_t>{0} : rd, s);
@@ -170,15 +176,34 @@ __global__ __launch_bounds__(BLOCK) void k_schwarz_build(SellView A, ContactView
     }
     __syncthreads();
   }
-  // A^-1 = W^T W; columns m..m4-1 of rows < m are zero (the apply reads float4 groups)
-  for (uint32_t p = ti; p < m; p += 16)
-    for (uint32_t q = tj; q < m4; q += 16) {
-      if (q >= m) { out[p * MD + q] = 0.0f; continue; }
-      const uint32_t k0 = p > q ? p : q;
-      double s = 0.0;
-      for (uint32_t k = k0; k < m; ++k) s += W[k * m + p] * W[k * m + q];
-      out[p * MD + q] = (float)s;
+  // A^-1 = W^T W on the FP64 matrix cores: 16 x 16 output tiles, k in steps of 4
+  // (v_mfma_f64_16x16x4f64; A operand lane a = (row a % 16, k a / 16), B operand lane
+  // b = (k b / 16, column b % 16), result lane l register v = (row 4 v + l / 16, column
+  // l % 16)). Only tiles with P <= Q are formed (the product is symmetric) and W being lower
+  // triangular, k starts at 16 max(P, Q). Entries outside m x m read as zero, so columns
+  // m..m4-1 of rows < m come out zero (the apply reads float4 groups).
+  const uint32_t wave = t / WAVE, lane = t % WAVE;
+  auto wget = [&](uint32_t k, uint32_t c) -> double { return (k < m && c < m) ? W[k * m + c] : 0.0; };
+  for (uint32_t tile = wave; tile < 21; tile += BLOCK / WAVE) {
+    // tile -> (P, Q), P <= Q < 6
+    uint32_t P = 0, rem = tile;
+    while (rem >= 6 - P) { rem -= 6 - P; ++P; }
+    const uint32_t Q = P + rem;
+    if (16 * P >= m4 || 16 * Q >= m4) continue;
+    double4_t acc = {0.0, 0.0, 0.0, 0.0};
+    for (uint32_t kb = 4 * Q; kb < 24 && 4 * kb < m; ++kb) {
+      const uint32_t k = 4 * kb + lane / 16;
+      const double a = wget(k, 16 * P + lane % 16);
+      const double b = wget(k, 16 * Q + lane % 16);
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
     }
+    for (uint32_t v = 0; v < 4; ++v) {
+      const uint32_t r = 16 * P + 4 * v + lane / 16, c = 16 * Q + lane % 16;
+      const float val = (float)acc[v];
+      if (r < m && c < m4) out[r * MD + c] = val;
+      if (P != Q && c < m && r < m4) out[c * MD + r] = val;
+    }
+  }
 }
 
 void Pcg::build_precond(const SellMatrix &M, const ContactView &C) {
@@ -208,7 +233,7 @@ constexpr uint32_t SCH_KPER = (MD + SCH_KG - 1) / SCH_KG;  // rows per k group
 static_assert(SCH_KG * SCH_C4 <= (uint32_t)BLOCK, "k groups x float4 columns must fit the workgroup");
 
 template <bool INIT>
-__global__ __launch_bounds__(BLOCK) void k_schwarz_pcg(uint32_t nd, const uint32_t *__restrict__ dom_ptr, const uint32_t *__restrict__ dom_row,
+__global__ __launch_bounds__(BLOCK) void k_schwarz_pcg(uint32_t nd, const uint32_t *__restrict__ dom_ptr, const uint32_t *__restrict__ dom_dof,
                                                       const float *__restrict__ ainv, const double *__restrict__ r64, float *__restrict__ r,
                                                       float *__restrict__ z, float *__restrict__ dsol, float *__restrict__ p,
                                                       const float *__restrict__ q, const double *pap_part, uint32_t npap, const double *cur,
@@ -233,13 +258,12 @@ __global__ __launch_bounds__(BLOCK) void k_schwarz_pcg(uint32_t nd, const uint32
     }
   }
   const uint32_t base = dom_ptr[d], m = 3 * (dom_ptr[d + 1] - base);
-  size_t o = 0;
+  const uint32_t od = t < MD ? dom_dof[(size_t)d * MD + t] : NONE_S;
+  const size_t o = od;
   float r0 = 0.f, q0 = 0.f, p0 = 0.f, d0 = 0.f;
-  if (t < m) {
-    const size_t row = dom_row[base + t / 3];
-    o = 4 * row + t % 3;
+  if (od != NONE_S) {
     if (INIT) {
-      r0 = (float)r64[3 * row + t % 3];
+      r0 = (float)r64[3 * (o >> 2) + (o & 3)];
     } else {
       r0 = r[o]; q0 = q[o]; p0 = p[o]; d0 = dsol[o];
     }
@@ -303,10 +327,10 @@ void launch_schwarz_pcg(hipStream_t s, const Schwarz &sz, bool init, const doubl
                         const float *q, const double *pap_part, uint32_t npap, const double *cur, double *nxt, double *scal, uint32_t *flag) {
   if (!sz.nd) return;
   if (init)
-    hipLaunchKernelGGL(k_schwarz_pcg<true>, dim3(sz.nd), dim3(BLOCK), 0, s, sz.nd, (const uint32_t *)sz.dom_ptr.p, (const uint32_t *)sz.dom_row.p,
+    hipLaunchKernelGGL(k_schwarz_pcg<true>, dim3(sz.nd), dim3(BLOCK), 0, s, sz.nd, (const uint32_t *)sz.dom_ptr.p, (const uint32_t *)sz.dom_dof.p,
                        (const float *)sz.ainv.p, r64, r, z, dsol, p, q, pap_part, npap, cur, nxt, scal, flag);
   else
-    hipLaunchKernelGGL(k_schwarz_pcg<false>, dim3(sz.nd), dim3(BLOCK), 0, s, sz.nd, (const uint32_t *)sz.dom_ptr.p, (const uint32_t *)sz.dom_row.p,
+    hipLaunchKernelGGL(k_schwarz_pcg<false>, dim3(sz.nd), dim3(BLOCK), 0, s, sz.nd, (const uint32_t *)sz.dom_ptr.p, (const uint32_t *)sz.dom_dof.p,
                        (const float *)sz.ainv.p, r64, r, z, dsol, p, q, pap_part, npap, cur, nxt, scal, flag);
 }
 

@@ -540,6 +540,16 @@ __global__ __launch_bounds__(BLOCK) void k_stencil_eval(uint32_t ns, const uint3
 }
 
 // Incidence keys: (free dof << 32) | (4 s + a); pinned participants get dof = nf (sort last).
+// pos[4 s + a] = slot of (s, a) in the vertex-sorted incidence list (~0u for a pinned
+// participant, which has no incidence).
+__global__ __launch_bounds__(BLOCK) void k_inc_pos(uint32_t nf, uint32_t m, const uint32_t *dof, const uint32_t *inc_ptr, const uint32_t *inc,
+                                                  uint32_t *pos) {
+  uint32_t k = blockIdx.x * BLOCK + threadIdx.x;
+  if (k >= m) return;
+  if (dof[k] == 0xffffffffu) pos[k] = 0xffffffffu;
+  if (k < inc_ptr[nf]) pos[inc[k]] = k;
+}
+
 __global__ __launch_bounds__(BLOCK) void k_inc_keys(uint32_t ns, const uint32_t *dof, uint32_t nf, uint64_t *key, uint32_t *val) {
   uint32_t e = blockIdx.x * BLOCK + threadIdx.x;
   if (e >= 4 * ns) return;
@@ -824,6 +834,8 @@ struct nguo_step {
   DevBuf<uint64_t> vt_key, ee_key, inc_key;
   DevBuf<float> vt_d, vt_g, ee_d, ee_g;
   DevBuf<uint32_t> sidx, inc_val, inc, inc_ptr, cdof;
+  DevBuf<uint32_t> inc_pos;  // incidence slot of each (stencil, participant)
+  DevBuf<float> contrib;     // per-incidence contact blocks of the PCG's p (float4 each)
   DevBuf<float> sd, sg, cg, cw, ccoef, cn, cgrad, csums, ckappa;
   uint32_t ns = 0, nvt = 0, nee = 0;
   RadixSort sorter;
@@ -1036,6 +1048,8 @@ ContactView contact_view(nguo_step *st) {
   C.sums = st->csums.p;
   C.inc_ptr = st->inc_ptr.p;
   C.inc = st->inc.p;
+  C.pos = st->inc_pos.p;
+  C.contrib = (float4 *)st->contrib.p;
   return C;
 }
 
@@ -1049,6 +1063,7 @@ void ensure_contact_cap(nguo_step *st, uint32_t want) {
   st->cdof.reserve(4 * c2); st->cg.reserve(12 * c2); st->cw.reserve(4 * c2); st->ccoef.reserve(2 * c2); st->cn.reserve(3 * c2);
   st->cgrad.reserve(24 * c2); st->csums.reserve(4 * c2); st->ckappa.reserve(c2);
   st->inc_key.reserve(4 * c2); st->inc_val.reserve(4 * c2); st->inc.reserve(4 * c2);
+  st->inc_pos.reserve(4 * c2); st->contrib.reserve(16 * c2);
   st->cap = c;
 }
 
@@ -1120,6 +1135,8 @@ void contacts(nguo_step *st) {
   st->sorter.sort_u64(m, st->inc_key.p, st->inc_val.p, digit_shifts(m, st->NF + 1), s);
   hipLaunchKernelGGL(k_inc_ptr, dim3(div_up(std::max(m, st->NF + 1), BLOCK)), dim3(BLOCK), 0, s, st->NF, m, (const uint64_t *)st->inc_key.p,
                      st->inc_ptr.p, st->inc.p);
+  hipLaunchKernelGGL(k_inc_pos, dim3(div_up(m, BLOCK)), dim3(BLOCK), 0, s, st->NF, m, (const uint32_t *)st->cdof.p,
+                     (const uint32_t *)st->inc_ptr.p, (const uint32_t *)st->inc.p, st->inc_pos.p);
   NGUO_HIP(hipGetLastError());
 }
 
@@ -1689,7 +1706,7 @@ int nguo_step_collider_sides(nguo_step *st, const float *x, float *side) {
   });
 }
 
-int nguo_step_bench_spmv(nguo_step *st, uint32_t reps, double *avg_ms, double *bytes) {
+int nguo_step_bench_spmv(nguo_step *st, uint32_t reps, int iteration_form, double *avg_ms, double *bytes) {
   return guarded([&]() -> int {
     NGUO_REQUIRE(st && avg_ms && bytes && reps > 0, NGUO_E_INVALID, "nguo_step_bench_spmv: bad argument");
     hipStream_t s = st->s;
@@ -1701,8 +1718,8 @@ int nguo_step_bench_spmv(nguo_step *st, uint32_t reps, double *avg_ms, double *b
     ContactView C = contact_view(st);
     float *xin = st->egrad.p;
     NGUO_HIP(hipMemcpyAsync(xin, st->x.p, 3 * (size_t)st->NF * sizeof(float), hipMemcpyDeviceToDevice, s));
-    *avg_ms = st->pcg->time_spmv(st->act.M.view(), C, xin, reps);
-    *bytes = st->act.M.spmv_bytes();
+    *avg_ms = st->pcg->time_spmv(st->act.M.view(), C, xin, reps, iteration_form != 0);
+    *bytes = st->act.M.spmv_bytes(iteration_form != 0);
     return NGUO_OK;
   });
 }

@@ -127,7 +127,7 @@ def load():
         lib.nguo_lbvh_sort_pairs.argtypes = [vp, C.c_uint32, P(C.c_uint32), P(C.c_uint32)]
         lib.nguo_lbvh_karras.argtypes = [vp, C.c_uint32, P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
         lib.nguo_lbvh_aabb.argtypes = [vp, C.c_uint32, P(C.c_uint32), P(C.c_uint32), P(C.c_float), P(C.c_float)]
-        lib.nguo_step_bench_spmv.argtypes = [vp, C.c_uint32, P(C.c_double), P(C.c_double)]
+        lib.nguo_step_bench_spmv.argtypes = [vp, C.c_uint32, C.c_int, P(C.c_double), P(C.c_double)]
         lib.nguo_bench_spmv_grid.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32, P(C.c_double), P(C.c_double), P(C.c_double)]
     _lib = lib
     return lib
@@ -328,10 +328,11 @@ class Stepper:
         check(load().nguo_step_stencils(self.h, fptr(x), n, uptr(idx), fptr(d), fptr(g), fptr(k), C.byref(cnt)))
         return idx[:n], d[:n], g[:n], k[:n]
 
-    def bench_spmv(self, reps=200):
-        """(average ms per launch, algorithmic bytes per launch) of the PCG operator."""
+    def bench_spmv(self, reps=200, iteration_form=True):
+        """(average ms per launch, algorithmic bytes per launch) of the PCG operator; the
+        iteration form is the kernel instance the PCG iteration launches."""
         ms, by = C.c_double(), C.c_double()
-        check(load().nguo_step_bench_spmv(self.h, int(reps), C.byref(ms), C.byref(by)))
+        check(load().nguo_step_bench_spmv(self.h, int(reps), int(bool(iteration_form)), C.byref(ms), C.byref(by)))
         return ms.value, by.value
 
     def collider_sides(self, x):
