@@ -38,8 +38,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps timed for the CPU baseline (0 = skip)")
     ap.add_argument("--grid", type=int, default=2048, help="side of the synthetic SpMV grid (0 = skip)")
-    ap.add_argument("--traffic-bytes", type=float, default=None,
-                    help="HBM bytes per launch of the roofline kernel from the rocprofv3 FETCH_SIZE/WRITE_SIZE passes")
+    # 14.16 MB read + 0.61 MB written per launch of the iteration-form SpMV on this workload,
+    # from the L2 request counters (profiles/hbm_traffic_pcg_spmv_tube_garment.md)
+    ap.add_argument("--traffic-bytes", type=float, default=14.77e6,
+                    help="HBM bytes per launch of the roofline kernel, measured with rocprofv3 counter passes")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="after the bench, copy a buffer of this many MiB once (counter calibration for --pmc passes)")
     args = ap.parse_args()
@@ -61,6 +63,7 @@ def main():
     for _ in range(args.warmup):
         st.advance(1)
     sync()
+    x_w, v_w = st.positions(), st.velocities()  # start state of the timed steps (CPU sample starts here too)
     t0 = time.perf_counter()
     iters, newton, stencils, res = [], [], [], 0.0
     for _ in range(args.steps):
@@ -98,11 +101,13 @@ def main():
         import oracle_py  # the CPU restatement of the reference (checker), timed as the baseline
         cscene = nh.Scene(1, seed=1)
         o = oracle_py.OracleSolver(cscene)
+        o.set_state(x_w, v_w)
         tc = time.perf_counter()
         o.step(args.cpu_steps)
         cpu_s = time.perf_counter() - tc
         cpu = {"value": round(1000.0 * cpu_s / args.cpu_steps, 1), "unit": "ms/implicit step", "cores": 1, "kind": "port",
-               "sample": f"first {args.cpu_steps} step(s) of the same tube-garment workload, oracle FP32 restatement, 1 thread"}
+               "sample": f"{args.cpu_steps} step(s) of the same tube-garment workload from the state the timed GPU steps start at "
+                         f"(after {args.warmup} steps), oracle FP32 restatement of the reference CPU tier, 1 thread"}
 
     out = {
         "metric": "ms/implicit step @1e-6 PCG residual (plain single garment); SpMV GB/s vs HBM peak",
@@ -130,10 +135,12 @@ def main():
     ctx.close()
     if args.copy_calib_mib:
         import torch
+        # counter calibration: a fill (writes N MiB), then an elementwise add (reads N MiB,
+        # writes N MiB), each one kernel of known traffic
         src = torch.ones(args.copy_calib_mib * 262144, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         torch.cuda.synchronize()
-        dst.copy_(src)
+        torch.add(src, 1.0, out=dst)
         torch.cuda.synchronize()
 
 

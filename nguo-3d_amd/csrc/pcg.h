@@ -54,11 +54,11 @@ struct PcgStats {
 };
 
 // Additive Schwarz preconditioner kernel of the PCG iteration (one workgroup per domain).
-// init: r = (f32) r64, d = 0, z = M^-1 r, p = z. Otherwise: alpha = rz_j / pAp from the
-// partials (npap SpMV partials, nd r.z partials in cur), r -= alpha q, d += alpha p,
-// z = M^-1 r. Writes the r.z | r.r partials of the new z to nxt (2 nd).
+// init: r = (f32) r64, d = 0, z = M^-1 r, p = z. Otherwise iteration j: alpha = rz_j / pAp
+// (npap SpMV partials; rz_j in scal[8 + (j & 1)]), r -= alpha q, d += alpha p, z = M^-1 r.
+// Writes the r.z | r.r partials of the new z to nxt (2 nd).
 void launch_schwarz_pcg(hipStream_t s, const Schwarz &sz, bool init, const double *r64, float *r, float *z, float *dsol, float *p,
-                        const float *q, const double *pap_part, uint32_t npap, const double *cur, double *nxt, double *scal, uint32_t *flag);
+                        const float *q, const double *pap_part, uint32_t npap, double *nxt, double *scal, uint32_t *flag, uint32_t j);
 
 class Pcg {
  public:
@@ -88,7 +88,7 @@ class Pcg {
   void launch_minv(const SellMatrix &M, const ContactView &C);
   void launch_precond_init(const SellMatrix &M, double *part);
   void launch_spmv_plain(const SellView &A, const ContactView &C);
-  void launch_csums_plain(const ContactView &C);
+  void launch_pre_plain(const ContactView &C);
   hipStream_t s_;
   Schwarz *sz_ = nullptr;
   uint32_t cap_nb_ = 0, cap_ns_ = 0;

@@ -2,13 +2,13 @@
 // Schwarz preconditioner kernels): workgroup sums with a fixed combination order, and the
 // redundant per-workgroup reduction of the previous launch's partials.
 //
-// One PCG iteration is two launches. Each launch stores one partial sum per workgroup; the
-// next launch's workgroups each reduce those partials themselves, in the same fixed order,
-// so every workgroup holds bitwise the same scalar without a single-workgroup launch or a
-// cross-workgroup handshake in between (kernel boundaries order all the memory traffic).
+// Each launch stores one partial sum per workgroup; the next launch's workgroups each
+// reduce those partials themselves, in the same fixed order, so every workgroup holds
+// bitwise the same scalar without a single-workgroup launch or a cross-workgroup handshake
+// in between (kernel boundaries order all the memory traffic).
 //
-// scal: 0 rz_j, 1 pAp, 2 alpha, 3 beta, 4 rr, 5 threshold (eta^2 rr0), 6 rr0, 7 host-read
-//       FP64 residual.
+// scal: 1 pAp, 2 alpha, 3 beta, 4 rr, 5 threshold (eta^2 rr0), 6 rr0, 7 host-read FP64
+//       residual, 8 / 9 rz_j for even / odd j.
 // flag: 0 state (0 running, 1 converged, 2 breakdown, 3 iteration cap), 1 iterations done,
 //       2 cap.
 #pragma once

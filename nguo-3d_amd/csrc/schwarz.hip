@@ -219,7 +219,8 @@ void Pcg::build_precond(const SellMatrix &M, const ContactView &C) {
 }
 
 // Preconditioner kernel of the PCG iteration, one workgroup per domain. Not INIT: the
-// workgroup reduces pAp (npap SpMV partials) and rz_j (cur) itself, stops on breakdown,
+// workgroup reduces pAp (npap SpMV partials) itself, takes rz_j from scal[8 + (j & 1)],
+// stops on breakdown,
 // then applies the PCG update to the domain's rows (r -= alpha q, d += alpha p); INIT:
 // r = (f32) r64, d = 0, p = z. Then z = A_d^-1 r: the domain's FP32 inverse is read as rows
 // of 24 float4 (coalesced 384-byte rows) by 240 threads that split its k range into 10
@@ -236,8 +237,8 @@ template <bool INIT>
 __global__ __launch_bounds__(BLOCK) void k_schwarz_pcg(uint32_t nd, const uint32_t *__restrict__ dom_ptr, const uint32_t *__restrict__ dom_dof,
                                                       const float *__restrict__ ainv, const double *__restrict__ r64, float *__restrict__ r,
                                                       float *__restrict__ z, float *__restrict__ dsol, float *__restrict__ p,
-                                                      const float *__restrict__ q, const double *pap_part, uint32_t npap, const double *cur,
-                                                      double *nxt, double *scal, uint32_t *flag) {
+                                                      const float *__restrict__ q, const double *pap_part, uint32_t npap, double *nxt,
+                                                      double *scal, uint32_t *flag, uint32_t j) {
   __shared__ float rs[MD];
   __shared__ float4 zp[SCH_KG][SCH_C4];
   __shared__ double lds[2 * BLOCK / WAVE];
@@ -271,16 +272,17 @@ __global__ __launch_bounds__(BLOCK) void k_schwarz_pcg(uint32_t nd, const uint32
   if (state) return;
   float alpha = 0.0f;
   if (!INIT) {
-    const double *src[2] = {pap_part, cur};
-    const uint32_t n[2] = {npap, nd};
-    double v[2];
-    sum_parts_all<BLOCK, 2>(src, n, v, lds);
+    const double *src[1] = {pap_part};
+    const uint32_t n[1] = {npap};
+    double v[1];
+    sum_parts_all<BLOCK, 1>(src, n, v, lds);
     if (!(v[0] > 0.0)) {
       if (blockIdx.x == 0 && threadIdx.x == 0) flag[0] = 2;
       return;
     }
-    alpha = (float)(v[1] / v[0]);
-    if (blockIdx.x == 0 && threadIdx.x == 0) { scal[1] = v[0]; scal[2] = v[1] / v[0]; }
+    const double a = scal[8 + (j & 1u)] / v[0];
+    alpha = (float)a;
+    if (blockIdx.x == 0 && threadIdx.x == 0) { scal[1] = v[0]; scal[2] = a; }
   }
   float rv = 0.0f;
   if (t < m) {
@@ -324,14 +326,14 @@ __global__ __launch_bounds__(BLOCK) void k_schwarz_pcg(uint32_t nd, const uint32
 }
 
 void launch_schwarz_pcg(hipStream_t s, const Schwarz &sz, bool init, const double *r64, float *r, float *z, float *dsol, float *p,
-                        const float *q, const double *pap_part, uint32_t npap, const double *cur, double *nxt, double *scal, uint32_t *flag) {
+                        const float *q, const double *pap_part, uint32_t npap, double *nxt, double *scal, uint32_t *flag, uint32_t j) {
   if (!sz.nd) return;
   if (init)
     hipLaunchKernelGGL(k_schwarz_pcg<true>, dim3(sz.nd), dim3(BLOCK), 0, s, sz.nd, (const uint32_t *)sz.dom_ptr.p, (const uint32_t *)sz.dom_dof.p,
-                       (const float *)sz.ainv.p, r64, r, z, dsol, p, q, pap_part, npap, cur, nxt, scal, flag);
+                       (const float *)sz.ainv.p, r64, r, z, dsol, p, q, pap_part, npap, nxt, scal, flag, j);
   else
     hipLaunchKernelGGL(k_schwarz_pcg<false>, dim3(sz.nd), dim3(BLOCK), 0, s, sz.nd, (const uint32_t *)sz.dom_ptr.p, (const uint32_t *)sz.dom_dof.p,
-                       (const float *)sz.ainv.p, r64, r, z, dsol, p, q, pap_part, npap, cur, nxt, scal, flag);
+                       (const float *)sz.ainv.p, r64, r, z, dsol, p, q, pap_part, npap, nxt, scal, flag, j);
 }
 
 }  // namespace nguo
