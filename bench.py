@@ -39,7 +39,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps timed for the CPU baseline (0 = skip)")
     ap.add_argument("--grid", type=int, default=2048, help="side of the synthetic SpMV grid (0 = skip)")
     # 14.16 MB read + 0.61 MB written per launch of the iteration-form SpMV on this workload,
-    # from the L2 request counters (profiles/hbm_traffic_pcg_spmv_tube_garment.md)
+    # from rocprofv3 passes over the L2-to-fabric request counters (128 B / 64 B requests)
     ap.add_argument("--traffic-bytes", type=float, default=14.77e6,
                     help="HBM bytes per launch of the roofline kernel, measured with rocprofv3 counter passes")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
