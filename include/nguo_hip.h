@@ -77,8 +77,10 @@ typedef struct nguo_params {
   float cg_tol;           /* relative residual ||b-Ax||/||b|| of each Newton solve */
   uint32_t cg_max_iter;
   float floor_y;          /* -INFINITY = no floor */
-  int32_t precond;        /* NGUO_PRECOND_*: 3x3 block Jacobi or one-level Schwarz */
+  int32_t precond;        /* NGUO_PRECOND_*: 3x3 block Jacobi or additive Schwarz */
   uint32_t schwarz_domain;/* vertices per Schwarz domain (1..32) */
+  float schwarz_weight;   /* coarse weight of the two-level Schwarz form; 0 = one level
+                             (engine-core/src/linsolve.rs:522-657, GpuPrecond::SchwarzMl) */
 } nguo_params;
 
 /* A scene in the solver's vertex space (engine-api/src/scene.rs:89-152 `lower` output plus
@@ -140,6 +142,9 @@ void nguo_ctx_destroy(nguo_ctx *ctx);
 int nguo_ctx_info(nguo_ctx *ctx, char *name, size_t name_len, int *cu_count);
 
 /* ---- LinearSolver seam (linsolve.rs:21-28; pcg.rs:737-774 GpuLinearSolver::solve).
+ * create(): precond NGUO_PRECOND_BLOCK or NGUO_PRECOND_SCHWARZ with schwarz_domain vertices
+ * per domain (1..32); schwarz_weight 0 = one level, > 0 = the two-level form with that coarse
+ * weight (GpuPrecond::{Block, SchwarzMl{domain, weight}}, pcg.rs:32-39).
  * solve(): A is the scalar CSR the reference assembles (n rows, a multiple of 3 for the
  * block preconditioner), b the right-hand side, tol relative to ||b||. x_out receives the
  * solution; stats (may be NULL) receives what last_stats() would return. */

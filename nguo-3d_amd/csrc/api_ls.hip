@@ -5,6 +5,7 @@
 // 3x3 blocks, and runs the FP32 PCG with FP64 refinement to `tol` relative residual. The
 // reference's GpuLinearSolver keeps its device buffers while (n, nnz) is unchanged; here the
 // block pattern is kept while the CSR pattern is unchanged and only values are re-uploaded.
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -29,6 +30,7 @@ struct nguo_ls {
   std::vector<uint32_t> val_slot;  // scalar CSR entry -> value index in the SELL planes
   nguo::DevBuf<float> b, x, y;
   uint32_t domain = 32;
+  float weight = 0.0f;  // coarse weight of the two-level Schwarz form (0: one level)
   std::unique_ptr<nguo::Schwarz> sz;
   std::vector<float> h_val;
   nguo_stats last{};
@@ -70,7 +72,7 @@ void load_csr(nguo_ls *ls, uint32_t n, const uint32_t *row_ptr, const uint32_t *
     ls->last_col.assign(col_idx, col_idx + nnz);
     if (ls->precond == NGUO_PRECOND_SCHWARZ) {
       ls->sz.reset(new Schwarz());
-      ls->sz->build(ls->mat, ls->domain, s);
+      ls->sz->build(ls->mat, ls->domain, s, ls->weight);
       ls->pcg->set_schwarz(ls->sz.get());
     }
   }
@@ -125,17 +127,17 @@ int nguo_ctx_info(nguo_ctx *ctx, char *name, size_t name_len, int *cu_count) {
 }
 
 int nguo_ls_create(nguo_ctx *ctx, int precond, uint32_t schwarz_domain, float schwarz_weight, nguo_ls **out) {
-  (void)schwarz_domain;
-  (void)schwarz_weight;
   return guarded([&]() -> int {
     NGUO_REQUIRE(ctx && out, NGUO_E_INVALID, "nguo_ls_create: null argument");
     NGUO_REQUIRE(precond == NGUO_PRECOND_BLOCK || precond == NGUO_PRECOND_SCHWARZ, NGUO_E_INVALID, "nguo_ls_create: unknown preconditioner");
     NGUO_REQUIRE(precond != NGUO_PRECOND_SCHWARZ || (schwarz_domain >= 1 && schwarz_domain <= SCHWARZ_MAX_DOMAIN), NGUO_E_UNSUPPORTED,
                  "nguo_ls_create: Schwarz domains hold 1..32 vertices");
+    NGUO_REQUIRE(schwarz_weight >= 0.0f && std::isfinite(schwarz_weight), NGUO_E_INVALID, "nguo_ls_create: coarse weight must be finite and >= 0");
     auto *ls = new nguo_ls();
     ls->ctx = ctx;
     ls->precond = precond;
     ls->domain = schwarz_domain;
+    ls->weight = schwarz_weight;
     ls->pcg.reset(new Pcg(ctx->stream));
     *out = ls;
     return NGUO_OK;

@@ -44,7 +44,17 @@ struct Schwarz {
   DevBuf<uint32_t> dom_ptr, dom_row, row_dom, row_pos, fails;
   DevBuf<uint32_t> dom_dof;  // [nd * 96]: stride-4 vector index of dof t of domain d, or ~0u past m
   DevBuf<float> ainv;  // nd blocks of SCHWARZ_MAX_DOF^2, row-major with stride SCHWARZ_MAX_DOF
-  void build(const SellMatrix &M, uint32_t domain_size, hipStream_t s);
+  // Two-level form (engine-core/src/linsolve.rs:522-657 MultilevelSchwarz): the coarse node of
+  // a vertex is its domain; the Galerkin coarse operator A1 = C A C^T sums the blocks between
+  // two domains; a one-level Schwarz over A1 (coarse domains: BFS over the coarse graph, same
+  // size) is the coarse smoother; z = S0 r + weight C^T S1 C r. weight 0 = one level.
+  float weight = 0.0f;
+  uint32_t nd1 = 0;
+  std::vector<uint32_t> h_cdom_ptr, h_cdom_node;
+  DevBuf<uint32_t> cdom_ptr, cdom_node, node_cdom, node_cpos;
+  DevBuf<float> ainv1;  // nd1 coarse blocks, as ainv
+  DevBuf<float> rc;     // [4 nd] restricted residual (sum of a domain's rows), per apply
+  void build(const SellMatrix &M, uint32_t domain_size, hipStream_t s, float coarse_weight = 0.0f);
 };
 
 struct PcgStats {
@@ -59,6 +69,9 @@ struct PcgStats {
 // Writes the r.z | r.r partials of the new z to nxt (2 nd).
 void launch_schwarz_pcg(hipStream_t s, const Schwarz &sz, bool init, const double *r64, float *r, float *z, float *dsol, float *p,
                         const float *q, const double *pap_part, uint32_t npap, double *nxt, double *scal, uint32_t *flag, uint32_t j);
+// Two-level form only, right after launch_schwarz_pcg: z += weight C^T S1 (C r) on every
+// domain's rows (and p = z for init); replaces the r.z partials in nxt.
+void launch_schwarz_coarse(hipStream_t s, const Schwarz &sz, bool init, const float *r, float *z, float *p, double *nxt, const uint32_t *flag);
 
 class Pcg {
  public:

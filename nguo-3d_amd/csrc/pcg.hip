@@ -510,10 +510,14 @@ __global__ __launch_bounds__(BLOCK) void k_bj_apply(uint32_t nb, const float *mi
 
 // Opens a sweep: r = (f32) r64, d = 0, z = M^-1 r, p_ = z; (r.z | r.r) partials -> part.
 void Pcg::launch_precond_init(const SellMatrix &M, double *part) {
-  if (sz_) launch_schwarz_pcg(s_, *sz_, true, r64_.p, r_.p, z_.p, d_.p, p_.p, q_.p, nullptr, 0, part, scal_.p, flag_.p, 0);
-  else hipLaunchKernelGGL(k_jacobi_pcg<true>, dim3(div_up(M.nb, BLOCK)), dim3(BLOCK), 0, s_, M.nb, (const float *)minv_.p,
-                          (const double *)r64_.p, r_.p, z_.p, d_.p, p_.p, (const float *)q_.p, (const double *)nullptr, 0u, part, scal_.p,
-                          flag_.p, 0u);
+  if (sz_) {
+    launch_schwarz_pcg(s_, *sz_, true, r64_.p, r_.p, z_.p, d_.p, p_.p, q_.p, nullptr, 0, part, scal_.p, flag_.p, 0);
+    launch_schwarz_coarse(s_, *sz_, true, r_.p, z_.p, p_.p, part, flag_.p);
+    return;
+  }
+  hipLaunchKernelGGL(k_jacobi_pcg<true>, dim3(div_up(M.nb, BLOCK)), dim3(BLOCK), 0, s_, M.nb, (const float *)minv_.p,
+                     (const double *)r64_.p, r_.p, z_.p, d_.p, p_.p, (const float *)q_.p, (const double *)nullptr, 0u, part, scal_.p, flag_.p,
+                     0u);
 }
 
 void Pcg::apply_precond(const SellMatrix &M, const float *r, float *z) {
@@ -581,8 +585,10 @@ PcgStats Pcg::solve(const SellMatrix &M, const ContactView &C, const float *b, f
           hipLaunchKernelGGL(k_pcg_spmv<SPMV_NEXT>, dim3(NS), dim3(WAVE), 0, s_, A, C, (const float4 *)z_.p, (const float4 *)pprev,
                              (float4 *)pj, (float4 *)q_.p, part_spmv, (const double *)scal_.p, (const uint32_t *)flag_.p);
         }
-        if (sz_) launch_schwarz_pcg(s_, *sz_, false, nullptr, r_.p, z_.p, d_.p, pj, q_.p, part_spmv, NS, part_pc, scal_.p, flag_.p, j);
-        else hipLaunchKernelGGL(k_jacobi_pcg<false>, dim3(G), dim3(BLOCK), 0, s_, nb, (const float *)minv_.p, (const double *)nullptr, r_.p,
+        if (sz_) {
+          launch_schwarz_pcg(s_, *sz_, false, nullptr, r_.p, z_.p, d_.p, pj, q_.p, part_spmv, NS, part_pc, scal_.p, flag_.p, j);
+          launch_schwarz_coarse(s_, *sz_, false, r_.p, z_.p, pj, part_pc, flag_.p);
+        } else hipLaunchKernelGGL(k_jacobi_pcg<false>, dim3(G), dim3(BLOCK), 0, s_, nb, (const float *)minv_.p, (const double *)nullptr, r_.p,
                                 z_.p, d_.p, pj, (const float *)q_.p, (const double *)part_spmv, NS, part_pc, scal_.p, flag_.p, j);
       }
       NGUO_HIP(hipGetLastError());

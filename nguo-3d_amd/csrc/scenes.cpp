@@ -221,6 +221,7 @@ nguo_params default_params() {
   p.floor_y = -INFINITY;
   p.precond = NGUO_PRECOND_SCHWARZ;
   p.schwarz_domain = 32;
+  p.schwarz_weight = 0.0f;
   return p;
 }
 

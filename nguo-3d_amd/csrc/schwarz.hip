@@ -15,8 +15,9 @@ constexpr uint32_t NONE_S = 0xffffffffu;
 typedef double double4_t __attribute__((ext_vector_type(4)));
 constexpr uint32_t MD = SCHWARZ_MAX_DOF;
 
-void Schwarz::build(const SellMatrix &M, uint32_t domain_size, hipStream_t s) {
+void Schwarz::build(const SellMatrix &M, uint32_t domain_size, hipStream_t s, float coarse_weight) {
   domain = std::max(1u, std::min(domain_size, SCHWARZ_MAX_DOMAIN));
+  weight = coarse_weight;
   const uint32_t nv = M.nb;
   // linsolve.rs:134-170: adjacency = off-diagonal block columns (sorted), greedy BFS from
   // seeds in index order.
@@ -55,66 +56,52 @@ void Schwarz::build(const SellMatrix &M, uint32_t domain_size, hipStream_t s) {
   row_pos.upload(rp.empty() ? std::vector<uint32_t>{0} : rp, s);
   ainv.reserve((size_t)std::max(nd, 1u) * MD * MD);
   fails.zero(1, s);
+  nd1 = 0;
+  if (weight > 0.0f && nd > 0) {
+    // coarse graph: domains G ~ H when a block of a row of G lies in a column of H
+    // (linsolve.rs:573-605: the coarse sparsity of every fine entry); BFS as for level 0
+    std::vector<std::vector<uint32_t>> adj(nd);
+    for (uint32_t i = 0; i < nv; ++i)
+      for (uint32_t k = 0; k < M.h_row_len[i]; ++k) {
+        const uint32_t H = rd[M.h_col[M.slot(i, k)]];
+        if (H != rd[i]) adj[rd[i]].push_back(H);
+      }
+    for (auto &a : adj) { std::sort(a.begin(), a.end()); a.erase(std::unique(a.begin(), a.end()), a.end()); }
+    std::vector<uint8_t> cassigned(nd, 0);
+    h_cdom_ptr.assign(1, 0);
+    h_cdom_node.clear();
+    for (uint32_t seed = 0; seed < nd; ++seed) {
+      if (cassigned[seed]) continue;
+      dom.assign(1, seed);
+      cassigned[seed] = 1;
+      size_t fi = 0;
+      while (dom.size() < domain && fi < dom.size()) {
+        const uint32_t g = dom[fi++];
+        for (uint32_t h : adj[g])
+          if (!cassigned[h] && dom.size() < domain) { cassigned[h] = 1; dom.push_back(h); }
+      }
+      h_cdom_node.insert(h_cdom_node.end(), dom.begin(), dom.end());
+      h_cdom_ptr.push_back((uint32_t)h_cdom_node.size());
+    }
+    nd1 = (uint32_t)h_cdom_ptr.size() - 1;
+    std::vector<uint32_t> ncd(nd), ncp(nd);
+    for (uint32_t D = 0; D < nd1; ++D)
+      for (uint32_t k = h_cdom_ptr[D]; k < h_cdom_ptr[D + 1]; ++k) { ncd[h_cdom_node[k]] = D; ncp[h_cdom_node[k]] = k - h_cdom_ptr[D]; }
+    cdom_ptr.upload(h_cdom_ptr, s);
+    cdom_node.upload(h_cdom_node, s);
+    node_cdom.upload(ncd, s);
+    node_cpos.upload(ncp, s);
+    ainv1.reserve((size_t)nd1 * MD * MD);
+    rc.zero(4 * (size_t)nd, s);
+  }
 }
 
-// Dense block of domain d (elastic + contact), FP64 Cholesky with the reference's floor and
-// diagonal fallback, explicit inverse A^-1 = L^-T L^-1 written as FP32.
-__global__ __launch_bounds__(BLOCK) void k_schwarz_build(SellView A, ContactView C, const uint32_t *__restrict__ dom_ptr,
-                                                        const uint32_t *__restrict__ dom_row, const uint32_t *__restrict__ row_dom,
-                                                        const uint32_t *__restrict__ row_pos, float *__restrict__ ainv, uint32_t *fails) {
-  __shared__ double L[MD * MD];
-  __shared__ double W[MD * MD];
-  __shared__ double diag0[MD];
-  __shared__ double colj[MD];
-  __shared__ double floor_v;
-  __shared__ int failed;
-  const uint32_t d = blockIdx.x, t = threadIdx.x;
-  // 16 x 16 thread grid for the dense loops (row index ti, column index tj)
-  const uint32_t ti = t / 16, tj = t % 16;
-  const uint32_t base = dom_ptr[d], nv = dom_ptr[d + 1] - base, m = 3 * nv;
-  for (uint32_t i = t; i < m * m; i += BLOCK) L[i] = 0.0;
-  if (t == 0) failed = 0;
-  __syncthreads();
-  // elastic blocks of the domain rows: 8 threads per row, one writer per block
-  for (uint32_t p = t / 8; p < nv; p += BLOCK / 8) {
-    const uint32_t row = dom_row[base + p];
-    const uint32_t sbase = A.slice_ptr[row >> 6], lane = row & 63u, len = A.row_len[row];
-    for (uint32_t k = t % 8; k < len; k += 8) {
-      const uint32_t sl = sbase + 64u * k + lane;
-      const uint32_t c = A.col[sl];
-      if (row_dom[c] != d) continue;
-      const uint32_t q = row_pos[c];
-      for (int r = 0; r < 3; ++r)
-        for (int cc = 0; cc < 3; ++cc) L[(3 * p + r) * m + 3 * q + cc] = (double)A.val[sell_val_index(sl, 3 * r + cc)];
-    }
-  }
-  __syncthreads();
-  // contact: hp g_a g_b^T + lambda w_a w_b (I - n n^T) for partners inside the domain (one
-  // thread per row: the row's incidences are added in list order)
-  if (C.ns) {
-    for (uint32_t p = t; p < nv; p += BLOCK) {
-      const uint32_t row = dom_row[base + p];
-      for (uint32_t k = C.inc_ptr[row]; k < C.inc_ptr[row + 1]; ++k) {
-        uint32_t e = C.inc[k], s = e >> 2, a = e & 3u;
-        double hp = C.coef[2 * s], lam = C.coef[2 * s + 1];
-        const float *ga = C.g + 12 * (size_t)s + 3 * a;
-        const float *n = C.n + 3 * (size_t)s;
-        double wa = C.w[4 * (size_t)s + a];
-        for (uint32_t b = 0; b < 4; ++b) {
-          uint32_t db = C.dof[4 * (size_t)s + b];
-          if (db == NONE_S || row_dom[db] != d) continue;
-          uint32_t q = row_pos[db];
-          const float *gb = C.g + 12 * (size_t)s + 3 * b;
-          double wb = C.w[4 * (size_t)s + b];
-          for (int r = 0; r < 3; ++r)
-            for (int cc = 0; cc < 3; ++cc)
-              L[(3 * p + r) * m + 3 * q + cc] +=
-                  hp * (double)ga[r] * (double)gb[cc] + lam * wa * wb * ((r == cc ? 1.0 : 0.0) - (double)n[r] * (double)n[cc]);
-        }
-      }
-    }
-    __syncthreads();
-  }
+// Floor, FP64 Cholesky (diagonal fallback) and explicit FP32 inverse of the dense m x m
+// block in L (row stride m), written to out (row stride MD; columns m..m4-1 zero). All
+// threads of the workgroup call it; the LDS arrays are the caller's.
+__device__ void factor_invert(double *L, double *W, double *diag0, double *colj, double &floor_v, int &failed, uint32_t m, float *out,
+                              uint32_t *fails) {
+  const uint32_t t = threadIdx.x, ti = t / 16, tj = t % 16;
   // floor 1e-6 tr/m + 1e-8 (linsolve.rs:232-238)
   if (t == 0) {
     double tr = 0.0;
@@ -148,7 +135,6 @@ __global__ __launch_bounds__(BLOCK) void k_schwarz_build(SellView A, ContactView
       for (uint32_t jj = j + 1 + tj; jj <= i; jj += 16) L[i * m + jj] -= colj[i] * colj[jj];
     __syncthreads();
   }
-  float *out = ainv + (size_t)d * MD * MD;
   const uint32_t m4 = (m + 3) & ~3u;
   if (failed) {
     // diagonal fallback: factor sqrt(a_pp) (or 1) -> inverse 1 / a_pp
@@ -206,12 +192,196 @@ __global__ __launch_bounds__(BLOCK) void k_schwarz_build(SellView A, ContactView
   }
 }
 
+// Dense block of domain d (elastic + contact), FP64 Cholesky with the reference's floor and
+// diagonal fallback, explicit inverse A^-1 = L^-T L^-1 written as FP32.
+__global__ __launch_bounds__(BLOCK) void k_schwarz_build(SellView A, ContactView C, const uint32_t *__restrict__ dom_ptr,
+                                                        const uint32_t *__restrict__ dom_row, const uint32_t *__restrict__ row_dom,
+                                                        const uint32_t *__restrict__ row_pos, float *__restrict__ ainv, uint32_t *fails) {
+  __shared__ double L[MD * MD];
+  __shared__ double W[MD * MD];
+  __shared__ double diag0[MD];
+  __shared__ double colj[MD];
+  __shared__ double floor_v;
+  __shared__ int failed;
+  const uint32_t d = blockIdx.x, t = threadIdx.x;
+  const uint32_t base = dom_ptr[d], nv = dom_ptr[d + 1] - base, m = 3 * nv;
+  for (uint32_t i = t; i < m * m; i += BLOCK) L[i] = 0.0;
+  if (t == 0) failed = 0;
+  __syncthreads();
+  // elastic blocks of the domain rows: 8 threads per row, one writer per block
+  for (uint32_t p = t / 8; p < nv; p += BLOCK / 8) {
+    const uint32_t row = dom_row[base + p];
+    const uint32_t sbase = A.slice_ptr[row >> 6], lane = row & 63u, len = A.row_len[row];
+    for (uint32_t k = t % 8; k < len; k += 8) {
+      const uint32_t sl = sbase + 64u * k + lane;
+      const uint32_t c = A.col[sl];
+      if (row_dom[c] != d) continue;
+      const uint32_t q = row_pos[c];
+      for (int r = 0; r < 3; ++r)
+        for (int cc = 0; cc < 3; ++cc) L[(3 * p + r) * m + 3 * q + cc] = (double)A.val[sell_val_index(sl, 3 * r + cc)];
+    }
+  }
+  __syncthreads();
+  // contact: hp g_a g_b^T + lambda w_a w_b (I - n n^T) for partners inside the domain (one
+  // thread per row: the row's incidences are added in list order)
+  if (C.ns) {
+    for (uint32_t p = t; p < nv; p += BLOCK) {
+      const uint32_t row = dom_row[base + p];
+      for (uint32_t k = C.inc_ptr[row]; k < C.inc_ptr[row + 1]; ++k) {
+        uint32_t e = C.inc[k], s = e >> 2, a = e & 3u;
+        double hp = C.coef[2 * s], lam = C.coef[2 * s + 1];
+        const float *ga = C.g + 12 * (size_t)s + 3 * a;
+        const float *n = C.n + 3 * (size_t)s;
+        double wa = C.w[4 * (size_t)s + a];
+        for (uint32_t b = 0; b < 4; ++b) {
+          uint32_t db = C.dof[4 * (size_t)s + b];
+          if (db == NONE_S || row_dom[db] != d) continue;
+          uint32_t q = row_pos[db];
+          const float *gb = C.g + 12 * (size_t)s + 3 * b;
+          double wb = C.w[4 * (size_t)s + b];
+          for (int r = 0; r < 3; ++r)
+            for (int cc = 0; cc < 3; ++cc)
+              L[(3 * p + r) * m + 3 * q + cc] +=
+                  hp * (double)ga[r] * (double)gb[cc] + lam * wa * wb * ((r == cc ? 1.0 : 0.0) - (double)n[r] * (double)n[cc]);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  factor_invert(L, W, diag0, colj, floor_v, failed, m, ainv + (size_t)d * MD * MD, fails);
+}
+
+// Coarse block of coarse domain D (linsolve.rs:613-625 coarse_matrix, 230-258 factorize):
+// entry (3a + r, 3b + c) sums the r,c entries of every block (i, j), i in domain G_a, j in
+// domain G_b, elastic and contact, G_a and G_b members a, b of D. Thread 3a + r owns coarse
+// row 3a + r and walks the rows of G_a in domain order (no write conflicts, fixed order);
+// then the same floor, FP64 factorisation and FP32 inverse as level 0.
+__global__ __launch_bounds__(BLOCK) void k_schwarz_build_coarse(SellView A, ContactView C, const uint32_t *__restrict__ dom_ptr,
+                                                               const uint32_t *__restrict__ dom_row, const uint32_t *__restrict__ row_dom,
+                                                               const uint32_t *__restrict__ cdom_ptr, const uint32_t *__restrict__ cdom_node,
+                                                               const uint32_t *__restrict__ node_cdom, const uint32_t *__restrict__ node_cpos,
+                                                               float *__restrict__ ainv1, uint32_t *fails) {
+  __shared__ double L[MD * MD];
+  __shared__ double W[MD * MD];
+  __shared__ double diag0[MD];
+  __shared__ double colj[MD];
+  __shared__ double floor_v;
+  __shared__ int failed;
+  const uint32_t D = blockIdx.x, t = threadIdx.x;
+  const uint32_t cb = cdom_ptr[D], n1 = cdom_ptr[D + 1] - cb, m = 3 * n1;
+  for (uint32_t i = t; i < m * m; i += BLOCK) L[i] = 0.0;
+  if (t == 0) failed = 0;
+  __syncthreads();
+  if (t < m) {
+    const uint32_t a = t / 3, r = t % 3, G = cdom_node[cb + a];
+    double *lrow = L + (size_t)t * m;
+    for (uint32_t k = dom_ptr[G]; k < dom_ptr[G + 1]; ++k) {
+      const uint32_t row = dom_row[k];
+      const uint32_t sbase = A.slice_ptr[row >> 6], lane = row & 63u, len = A.row_len[row];
+      for (uint32_t kk = 0; kk < len; ++kk) {
+        const uint32_t sl = sbase + 64u * kk + lane;
+        const uint32_t H = row_dom[A.col[sl]];
+        if (node_cdom[H] != D) continue;
+        const uint32_t b = node_cpos[H];
+        for (int c = 0; c < 3; ++c) lrow[3 * b + c] += (double)A.val[sell_val_index(sl, 3 * r + c)];
+      }
+      if (C.ns) {
+        for (uint32_t kk = C.inc_ptr[row]; kk < C.inc_ptr[row + 1]; ++kk) {
+          const uint32_t e = C.inc[kk], s = e >> 2, ai = e & 3u;
+          const double hp = C.coef[2 * s], lam = C.coef[2 * s + 1];
+          const double gar = C.g[12 * (size_t)s + 3 * ai + r], nr = C.n[3 * (size_t)s + r];
+          const double wa = C.w[4 * (size_t)s + ai];
+          for (uint32_t bi = 0; bi < 4; ++bi) {
+            const uint32_t db = C.dof[4 * (size_t)s + bi];
+            if (db == NONE_S) continue;
+            const uint32_t H = row_dom[db];
+            if (node_cdom[H] != D) continue;
+            const uint32_t b = node_cpos[H];
+            const double wb = C.w[4 * (size_t)s + bi];
+            for (int c = 0; c < 3; ++c)
+              lrow[3 * b + c] += hp * gar * (double)C.g[12 * (size_t)s + 3 * bi + c] +
+                                 lam * wa * wb * ((r == (uint32_t)c ? 1.0 : 0.0) - nr * (double)C.n[3 * (size_t)s + c]);
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  factor_invert(L, W, diag0, colj, floor_v, failed, m, ainv1 + (size_t)D * MD * MD, fails);
+}
+
+// Coarse correction, one workgroup per domain d (member a of coarse domain D): e =
+// (A1_D^-1 rc_D) rows 3a..3a+2, then z += weight e on d's rows (p = z too for init), and the
+// r.z partial of the corrected z.
+constexpr int COARSE_THREADS = 128;
+template <bool INIT>
+__global__ __launch_bounds__(COARSE_THREADS) void k_schwarz_coarse(uint32_t nd, const uint32_t *__restrict__ dom_dof,
+                                                                  const uint32_t *__restrict__ cdom_ptr, const uint32_t *__restrict__ cdom_node,
+                                                                  const uint32_t *__restrict__ node_cdom, const uint32_t *__restrict__ node_cpos,
+                                                                  const float *__restrict__ ainv1, const float4 *__restrict__ rc, float weight,
+                                                                  const float *__restrict__ r, float *__restrict__ z, float *__restrict__ p,
+                                                                  double *nxt, const uint32_t *flag) {
+  __shared__ float rcs[MD];
+  __shared__ float ec[3];
+  __shared__ double lds[COARSE_THREADS / WAVE];
+  if (!INIT && flag[0]) return;
+  const uint32_t d = blockIdx.x, t = threadIdx.x;
+  const uint32_t D = node_cdom[d], a = node_cpos[d], cb = cdom_ptr[D], n1 = cdom_ptr[D + 1] - cb, m1 = 3 * n1;
+  const uint32_t o = t < MD ? dom_dof[(size_t)d * MD + t] : NONE_S;
+  float r0 = 0.f, z0 = 0.f;
+  if (o != NONE_S) { r0 = r[o]; z0 = z[o]; }
+  if (t < n1) {
+    const float4 v = rc[cdom_node[cb + t]];
+    rcs[3 * t] = v.x; rcs[3 * t + 1] = v.y; rcs[3 * t + 2] = v.z;
+  }
+  __syncthreads();
+  // rows 3a + c of the coarse inverse: 32 lanes per row, fixed-order lane sums
+  if (t < 3 * 32) {
+    const uint32_t c = t / 32, l = t % 32;
+    const float *row = ainv1 + (size_t)D * MD * MD + (size_t)(3 * a + c) * MD;
+    float acc = 0.0f;
+    for (uint32_t k = l; k < m1; k += 32) acc += row[k] * rcs[k];
+#pragma unroll
+    for (int off = 16; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 32);
+    if (l == 0) ec[c] = acc;
+  }
+  __syncthreads();
+  double v = 0.0;
+  if (o != NONE_S) {
+    const float zn = z0 + weight * ec[t % 3];
+    z[o] = zn;
+    if (INIT) p[o] = zn;
+    v = (double)r0 * zn;
+  }
+  v = wave_sum(v);
+  if ((t & (WAVE - 1)) == 0) lds[t / WAVE] = v;
+  __syncthreads();
+  if (t == 0) nxt[d] = lds[0] + lds[1];
+}
+
+void launch_schwarz_coarse(hipStream_t s, const Schwarz &sz, bool init, const float *r, float *z, float *p, double *nxt, const uint32_t *flag) {
+  if (!sz.nd1) return;
+  if (init)
+    hipLaunchKernelGGL(k_schwarz_coarse<true>, dim3(sz.nd), dim3(COARSE_THREADS), 0, s, sz.nd, (const uint32_t *)sz.dom_dof.p,
+                       (const uint32_t *)sz.cdom_ptr.p, (const uint32_t *)sz.cdom_node.p, (const uint32_t *)sz.node_cdom.p,
+                       (const uint32_t *)sz.node_cpos.p, (const float *)sz.ainv1.p, (const float4 *)sz.rc.p, sz.weight, r, z, p, nxt, flag);
+  else
+    hipLaunchKernelGGL(k_schwarz_coarse<false>, dim3(sz.nd), dim3(COARSE_THREADS), 0, s, sz.nd, (const uint32_t *)sz.dom_dof.p,
+                       (const uint32_t *)sz.cdom_ptr.p, (const uint32_t *)sz.cdom_node.p, (const uint32_t *)sz.node_cdom.p,
+                       (const uint32_t *)sz.node_cpos.p, (const float *)sz.ainv1.p, (const float4 *)sz.rc.p, sz.weight, r, z, p, nxt, flag);
+}
+
 void Pcg::build_precond(const SellMatrix &M, const ContactView &C) {
   ensure(M.nb, C.ns);
   if (sz_) {
     if (sz_->nd) hipLaunchKernelGGL(k_schwarz_build, dim3(sz_->nd), dim3(BLOCK), 0, s_, M.view(), C, (const uint32_t *)sz_->dom_ptr.p,
                                     (const uint32_t *)sz_->dom_row.p, (const uint32_t *)sz_->row_dom.p, (const uint32_t *)sz_->row_pos.p,
                                     sz_->ainv.p, sz_->fails.p);
+    if (sz_->nd1)
+      hipLaunchKernelGGL(k_schwarz_build_coarse, dim3(sz_->nd1), dim3(BLOCK), 0, s_, M.view(), C, (const uint32_t *)sz_->dom_ptr.p,
+                         (const uint32_t *)sz_->dom_row.p, (const uint32_t *)sz_->row_dom.p, (const uint32_t *)sz_->cdom_ptr.p,
+                         (const uint32_t *)sz_->cdom_node.p, (const uint32_t *)sz_->node_cdom.p, (const uint32_t *)sz_->node_cpos.p,
+                         sz_->ainv1.p, sz_->fails.p);
   } else {
     launch_minv(M, C);
   }
@@ -238,7 +408,7 @@ __global__ __launch_bounds__(BLOCK) void k_schwarz_pcg(uint32_t nd, const uint32
                                                       const float *__restrict__ ainv, const double *__restrict__ r64, float *__restrict__ r,
                                                       float *__restrict__ z, float *__restrict__ dsol, float *__restrict__ p,
                                                       const float *__restrict__ q, const double *pap_part, uint32_t npap, double *nxt,
-                                                      double *scal, uint32_t *flag, uint32_t j) {
+                                                      double *scal, uint32_t *flag, uint32_t j, float4 *rc) {
   __shared__ float rs[MD];
   __shared__ float4 zp[SCH_KG][SCH_C4];
   __shared__ double lds[2 * BLOCK / WAVE];
@@ -297,6 +467,12 @@ __global__ __launch_bounds__(BLOCK) void k_schwarz_pcg(uint32_t nd, const uint32
     rs[t] = rv;
   }
   __syncthreads();
+  if (rc && t < 3) {
+    // restriction for the coarse level: the domain's rows summed (domain order)
+    float acc = 0.0f;
+    for (uint32_t k = t; k < m; k += 3) acc += rs[k];
+    reinterpret_cast<float *>(rc)[4 * (size_t)d + t] = acc;
+  }
   const uint32_t m4 = (m + 3) / 4;
   if (mac && c4 < m4) {
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -330,10 +506,12 @@ void launch_schwarz_pcg(hipStream_t s, const Schwarz &sz, bool init, const doubl
   if (!sz.nd) return;
   if (init)
     hipLaunchKernelGGL(k_schwarz_pcg<true>, dim3(sz.nd), dim3(BLOCK), 0, s, sz.nd, (const uint32_t *)sz.dom_ptr.p, (const uint32_t *)sz.dom_dof.p,
-                       (const float *)sz.ainv.p, r64, r, z, dsol, p, q, pap_part, npap, nxt, scal, flag, j);
+                       (const float *)sz.ainv.p, r64, r, z, dsol, p, q, pap_part, npap, nxt, scal, flag, j,
+                       sz.nd1 ? (float4 *)sz.rc.p : (float4 *)nullptr);
   else
     hipLaunchKernelGGL(k_schwarz_pcg<false>, dim3(sz.nd), dim3(BLOCK), 0, s, sz.nd, (const uint32_t *)sz.dom_ptr.p, (const uint32_t *)sz.dom_dof.p,
-                       (const float *)sz.ainv.p, r64, r, z, dsol, p, q, pap_part, npap, nxt, scal, flag, j);
+                       (const float *)sz.ainv.p, r64, r, z, dsol, p, q, pap_part, npap, nxt, scal, flag, j,
+                       sz.nd1 ? (float4 *)sz.rc.p : (float4 *)nullptr);
 }
 
 }  // namespace nguo

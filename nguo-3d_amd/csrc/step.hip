@@ -1599,7 +1599,9 @@ int nguo_step_create(nguo_ctx *ctx, const nguo_scene_desc *sc, nguo_step **out) 
       NGUO_REQUIRE(st->prm.schwarz_domain >= 1 && st->prm.schwarz_domain <= SCHWARZ_MAX_DOMAIN, NGUO_E_UNSUPPORTED,
                    "Schwarz domains hold 1..32 vertices");
       st->sz.reset(new Schwarz());
-      st->sz->build(st->act.M, st->prm.schwarz_domain, s);
+      NGUO_REQUIRE(st->prm.schwarz_weight >= 0.0f && std::isfinite(st->prm.schwarz_weight), NGUO_E_INVALID,
+                   "Schwarz coarse weight must be finite and >= 0");
+      st->sz->build(st->act.M, st->prm.schwarz_domain, s, st->prm.schwarz_weight);
       st->pcg->set_schwarz(st->sz.get());
     }
     NGUO_HIP(hipHostMalloc((void **)&st->hf, 16 * sizeof(float), hipHostMallocDefault));

@@ -45,7 +45,7 @@ class Params(C.Structure):
         ("barrier_kind", C.c_int32), ("self_collision", C.c_int32), ("max_dx", C.c_float),
         ("target_toi", C.c_float), ("min_newton_steps", C.c_uint32), ("ccd_max_iter", C.c_uint32),
         ("cg_tol", C.c_float), ("cg_max_iter", C.c_uint32), ("floor_y", C.c_float),
-        ("precond", C.c_int32), ("schwarz_domain", C.c_uint32),
+        ("precond", C.c_int32), ("schwarz_domain", C.c_uint32), ("schwarz_weight", C.c_float),
     ]
 
 
@@ -192,9 +192,9 @@ class Context:
 class LinearSolver:
     """The LinearSolver seam (engine-core/src/linsolve.rs:21-28) on the device."""
 
-    def __init__(self, ctx, precond=0, domain=32):
+    def __init__(self, ctx, precond=0, domain=32, weight=0.0):
         self.h = C.c_void_p()
-        check(load().nguo_ls_create(ctx.h, int(precond), int(domain), 0.0, C.byref(self.h)))
+        check(load().nguo_ls_create(ctx.h, int(precond), int(domain), float(weight), C.byref(self.h)))
 
     def precond_apply(self, row_ptr, col_idx, values, r):
         row_ptr = np.ascontiguousarray(row_ptr, dtype=np.uint32)

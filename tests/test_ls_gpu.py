@@ -125,3 +125,37 @@ def test_schwarz_pcg_matches_oracle(ctx, cfg, steps):
     assert st.iters < it_bj, (st.iters, it_bj)
     assert st.iters <= 1.5 * it_ref + 10, (st.iters, it_ref)  # FP32 sweeps + FP64 refinement
     ls.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cfg,steps,domain", [(0, 14, 32), (101, 0, 32), (101, 20, 16)])
+def test_two_level_schwarz_apply_matches_oracle(ctx, cfg, steps, domain):
+    # z = S0 r + w C^T S1 (C r) (linsolve.rs:522-657); coarse operator, coarse domains and
+    # restriction as the oracle's Schwarz2, FP32 inverses and sums on the device
+    rp, ci, va, rhs = newton_system(cfg, steps)
+    r = np.random.default_rng(22).standard_normal(len(rhs)).astype(np.float32)
+    _, z_ref = O.schwarz(rp, ci, va, domain, r, weight=0.7)
+    _, z1 = O.schwarz(rp, ci, va, domain, r, weight=0.0)
+    assert np.linalg.norm(z_ref - z1) > 1e-3 * np.linalg.norm(z_ref)  # the coarse term is not negligible here
+    ls = nguo_hip.LinearSolver(ctx, precond=1, domain=domain, weight=0.7)
+    z = ls.precond_apply(rp, ci, va, r)
+    err = np.linalg.norm(z - z_ref) / np.linalg.norm(z_ref)
+    assert err < 5e-4, err
+    ls.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cfg,steps", [(101, 0), (101, 20)])
+def test_two_level_schwarz_pcg(ctx, cfg, steps):
+    rp, ci, va, rhs = newton_system(cfg, steps)
+    x_ref, it_ref, _ = O.cg_schwarz(rp, ci, va, rhs, 1e-6, 10000, 32, 0.7)
+    _, it_one, _ = O.cg_schwarz(rp, ci, va, rhs, 1e-6, 10000, 32, 0.0)
+    ls = nguo_hip.LinearSolver(ctx, precond=1, domain=32, weight=0.7)
+    x, st = ls.solve(rp, ci, va, rhs, 1e-6, 10000)
+    rel, rel_ref = host_residual(rp, ci, va, rhs, x), host_residual(rp, ci, va, rhs, x_ref)
+    assert rel <= max(2e-6, 2.0 * rel_ref), (rel, rel_ref, st.residual)
+    assert st.residual <= 1e-6
+    assert np.abs(x - x_ref).max() <= 1e-3 * np.abs(x_ref).max()
+    assert it_ref < it_one  # the oracle's two-level count beats one level on the tube
+    assert st.iters <= 1.5 * it_ref + 10, (st.iters, it_ref)
+    ls.close()

@@ -555,3 +555,31 @@ def test_schwarz_domains_partition_and_single_vertex_limit():
                 if ci[k] // 3 == v:
                     blk[rr, ci[k] % 3] = va[k]
         assert np.allclose(z[3 * v:3 * v + 3], np.linalg.solve(blk, r[3 * v:3 * v + 3]), rtol=1e-5)
+
+
+def test_two_level_schwarz_fewer_iterations_on_a_stiff_chain():
+    # MultilevelSchwarz (linsolve.rs:522-657): the coarse correction couples domains, so a
+    # long stiff chain (rod-like) needs fewer iterations with it; weight 0 is one level
+    n = 400
+    rows = {}
+    def add(i, j, blk):
+        rows.setdefault(i, {}).setdefault(j, np.zeros((3, 3)))
+        rows[i][j] += blk
+    for v in range(n):
+        add(v, v, 1e-6 * np.eye(3))
+    for v in range(n - 1):
+        K = np.eye(3)
+        add(v, v, K); add(v + 1, v + 1, K); add(v, v + 1, -K); add(v + 1, v, -K)
+    rp, ci, va = [0], [], []
+    for i in range(n):
+        for r in range(3):
+            for j in sorted(rows[i]):
+                for c in range(3):
+                    ci.append(3 * j + c); va.append(rows[i][j][r, c])
+            rp.append(len(ci))
+    rp, ci, va = np.array(rp, np.uint32), np.array(ci, np.uint32), np.array(va, np.float32)
+    b = np.random.default_rng(14).standard_normal(3 * n).astype(np.float32)
+    x1, it1, _ = O.cg_schwarz(rp, ci, va, b, 1e-6, 100000, 8, 0.0)
+    x2, it2, _ = O.cg_schwarz(rp, ci, va, b, 1e-6, 100000, 8, 0.7)
+    assert it2 < 0.8 * it1, (it2, it1)
+    assert np.abs(x2 - x1).max() < 1e-3 * np.abs(x1).max()
