@@ -38,6 +38,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps timed for the CPU baseline (0 = skip)")
     ap.add_argument("--grid", type=int, default=2048, help="side of the synthetic SpMV grid (0 = skip)")
+    ap.add_argument("--c3-steps", type=int, default=3, help="timed steps of the 8-size graded batch vs sequential (0 = skip)")
+    ap.add_argument("--c3-warmup", type=int, default=2)
+    ap.add_argument("--schwarz-domain", type=int, default=None, help="override the scene's Schwarz domain size")
+    ap.add_argument("--schwarz-weight", type=float, default=None, help="override the scene's two-level coarse weight (0 = one level)")
     # 14.16 MB read + 0.61 MB written per launch of the iteration-form SpMV on this workload,
     # from rocprofv3 passes over the L2-to-fabric request counters (128 B / 64 B requests)
     ap.add_argument("--traffic-bytes", type=float, default=14.77e6,
@@ -59,6 +63,10 @@ def main():
     ctx = nh.Context(0)
     name, cus = ctx.info()
     scene = nh.Scene(1, seed=1)
+    if args.schwarz_domain is not None:
+        scene.desc.params.schwarz_domain = args.schwarz_domain
+    if args.schwarz_weight is not None:
+        scene.desc.params.schwarz_weight = args.schwarz_weight
     st = nh.Stepper(ctx, scene)
     for _ in range(args.warmup):
         st.advance(1)
@@ -95,6 +103,37 @@ def main():
                 "unit": "GB/s", "frac": round(g_ach / HBM_PEAK_GBS, 4), "avg_launch_us": round(g_ms * 1000, 2),
                 "bytes_per_launch": g_bytes}
 
+    c3 = None
+    if args.c3_steps > 0:
+        # graded sizes: circumference factors 0.85 .. 1.20 of the tube garment, each on its
+        # scaled body; all 8 advanced concurrently vs the same 8 advanced one after another
+        factors = [0.85, 0.90, 0.95, 1.00, 1.05, 1.10, 1.15, 1.20]
+        scenes = [nh.Scene(3, seed=3, param=f) for f in factors]
+        seq_ms = []
+        for sc in scenes:
+            s1 = nh.Stepper(ctx, sc)
+            s1.advance(args.c3_warmup)
+            sync()
+            t1 = time.perf_counter()
+            s1.advance(args.c3_steps)
+            sync()
+            seq_ms.append(1000.0 * (time.perf_counter() - t1) / args.c3_steps)
+            s1.close()
+        batch = nh.Batch(ctx, scenes)
+        batch.advance(args.c3_warmup)
+        sync()
+        t1 = time.perf_counter()
+        batch.advance(args.c3_steps)
+        sync()
+        b_ms = 1000.0 * (time.perf_counter() - t1) / args.c3_steps
+        free = [int((sc.inv_mass > 0).sum()) for sc in scenes]
+        c3 = {"workload": f"8 graded tubes, circumference x{factors[0]}..x{factors[-1]}, {min(free)}..{max(free)} free vertices "
+                          f"({sum(free)} total), steps {args.c3_warmup + 1}..{args.c3_warmup + args.c3_steps}",
+              "batched_ms_per_step": round(b_ms, 3), "sequential_ms_per_step": round(sum(seq_ms), 3),
+              "per_size_ms": [round(v, 3) for v in seq_ms], "speedup": round(sum(seq_ms) / b_ms, 3),
+              "form": "one stream and host thread per instance, kernels interleaved on the device"}
+        batch.close()
+
     cpu = None
     if args.cpu_steps > 0:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -123,11 +162,13 @@ def main():
         "dtype": "f32 (PCG vectors, elements) + f64 (refinement residual, CCD time of impact, PCG scalars)",
         "data": "synthetic",
         "config": {"workload": "tube garment 19950 v / 39600 t + 3x200-segment rods on a 12-capsule body (50688 t), 60-step drape config",
-                   "dt": 1.0 / 60.0, "newton_iters": 4, "cg_tol": 1e-6, "device": name, "cus": cus},
+                   "dt": 1.0 / 60.0, "newton_iters": 4, "cg_tol": 1e-6, "device": name, "cus": cus,
+                   "precond": {"schwarz_domain": int(scene.desc.params.schwarz_domain), "coarse_weight": float(scene.desc.params.schwarz_weight)}},
         "stats": {"mean_pcg_iters": sum(iters) / len(iters), "mean_newton": sum(newton) / len(newton),
                   "mean_stencils": sum(stencils) / len(stencils), "max_reresid": res},
         "roofline": roofline,
         "spmv_hbm": grid,
+        "batched_c3": c3,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

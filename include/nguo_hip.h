@@ -182,6 +182,18 @@ int nguo_step_stencils(nguo_step *st, const float *x, uint32_t cap, uint32_t *id
 /* implicit.rs:1431-1476 collider_sides at positions x: -1 inside the static body, +1 else. */
 int nguo_step_collider_sides(nguo_step *st, const float *x, float *side);
 
+/* ---- batch of independent scenes (the graded-size form, C3). Each instance is a full step
+ * object on its own stream; nguo_batch_advance advances all of them concurrently (one host
+ * thread each), every instance taking exactly the steps nguo_step_advance would. Instances are
+ * reached through nguo_batch_instance for positions, metrics and the other nguo_step_* calls
+ * (not concurrently with nguo_batch_advance). No reference counterpart: the reference steps
+ * one scene per ImplicitSolver (engine-core/src/implicit.rs:1951). */
+typedef struct nguo_batch nguo_batch;
+int nguo_batch_create(nguo_ctx *ctx, uint32_t n, const nguo_scene_desc *scenes, nguo_batch **out);
+void nguo_batch_destroy(nguo_batch *batch);
+int nguo_batch_advance(nguo_batch *batch, uint32_t nsteps);
+int nguo_batch_instance(nguo_batch *batch, uint32_t i, nguo_step **out);
+
 /* ---- measurement (no reference counterpart: the reference has no GPU timestamps,
  * core-engine/docs/WGPU_PIN.md:142-144). Average duration of the PCG operator y = (A+C) x on
  * the current Newton system, timed with HIP events on the library stream over `reps`

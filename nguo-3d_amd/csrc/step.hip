@@ -18,6 +18,9 @@
 #include <cstring>
 #include <memory>
 #include <numeric>
+#include <string>
+#include <thread>
+#include <vector>
 
 #include "device_common.h"
 #include "devmath.h"
@@ -848,10 +851,13 @@ struct nguo_step {
   nguo_step_metrics metrics{};
   DevBuf<float> trace;  // x after every Newton iteration of the last step
   std::vector<float> trace_toi;
+  bool own_stream = false;
   ~nguo_step() {
+    if (own_stream && s) (void)hipStreamSynchronize(s);
     if (hf) (void)hipHostFree(hf);
     if (hu) (void)hipHostFree(hu);
     if (hl) (void)hipHostFree(hl);
+    if (own_stream && s) (void)hipStreamDestroy(s);
   }
 };
 
@@ -1297,7 +1303,9 @@ int nguo_step_create(nguo_ctx *ctx, const nguo_scene_desc *sc, nguo_step **out) 
     NGUO_HIP(hipSetDevice(ctx->device));
     std::unique_ptr<nguo_step> st(new nguo_step());
     st->ctx = ctx;
-    st->s = ctx->stream;
+    // each step object owns a stream (default priority), so several can advance concurrently
+    NGUO_HIP(hipStreamCreateWithFlags(&st->s, hipStreamNonBlocking));
+    st->own_stream = true;
     hipStream_t s = st->s;
     const uint32_t V = sc->n_verts;
     st->V = V;
@@ -1724,6 +1732,72 @@ int nguo_step_bench_spmv(nguo_step *st, uint32_t reps, int iteration_form, doubl
     *bytes = st->act.M.spmv_bytes(iteration_form != 0);
     return NGUO_OK;
   });
+}
+
+// ---- batch of independent scenes advanced concurrently (C3, graded sizes)
+struct nguo_batch {
+  nguo_ctx *ctx = nullptr;
+  std::vector<nguo_step *> inst;
+  ~nguo_batch() {
+    for (auto *st : inst) delete st;
+  }
+};
+
+int nguo_batch_create(nguo_ctx *ctx, uint32_t n, const nguo_scene_desc *scenes, nguo_batch **out) {
+  return guarded([&]() -> int {
+    NGUO_REQUIRE(ctx && scenes && out && n > 0, NGUO_E_INVALID, "nguo_batch_create: bad argument");
+    std::unique_ptr<nguo_batch> b(new nguo_batch());
+    b->ctx = ctx;
+    for (uint32_t i = 0; i < n; ++i) {
+      nguo_step *st = nullptr;
+      int rc = nguo_step_create(ctx, &scenes[i], &st);
+      if (rc != NGUO_OK) return fail(rc, "nguo_batch_create: instance " + std::to_string(i) + ": " + std::string(nguo_last_error()));
+      b->inst.push_back(st);
+    }
+    *out = b.release();
+    return NGUO_OK;
+  });
+}
+
+void nguo_batch_destroy(nguo_batch *b) { delete b; }
+
+// Every instance advances on its own stream from its own host thread: the instances' kernels
+// and their host-side convergence reads interleave on the device. Each instance runs exactly
+// the single-scene step (per-instance Newton loop, tolerance, CCD step), so its trajectory is
+// the one nguo_step_advance gives.
+int nguo_batch_advance(nguo_batch *b, uint32_t nsteps) {
+  return guarded([&]() -> int {
+    NGUO_REQUIRE(b, NGUO_E_INVALID, "nguo_batch_advance: null handle");
+    const size_t n = b->inst.size();
+    std::vector<int> code(n, NGUO_OK);
+    std::vector<std::string> msg(n);
+    std::vector<std::thread> th;
+    th.reserve(n);
+    for (size_t i = 0; i < n; ++i)
+      th.emplace_back([&, i]() {
+        try {
+          NGUO_HIP(hipSetDevice(b->ctx->device));
+          for (uint32_t k = 0; k < nsteps; ++k) advance_one(b->inst[i]);
+          NGUO_HIP(hipStreamSynchronize(b->inst[i]->s));
+        } catch (const HipError &e) {
+          code[i] = e.code;
+          msg[i] = e.msg;
+        } catch (const std::bad_alloc &) {
+          code[i] = NGUO_E_NOMEM;
+          msg[i] = "host allocation failed";
+        }
+      });
+    for (auto &t : th) t.join();
+    for (size_t i = 0; i < n; ++i)
+      if (code[i] != NGUO_OK) return fail(code[i], "nguo_batch_advance: instance " + std::to_string(i) + ": " + msg[i]);
+    return NGUO_OK;
+  });
+}
+
+int nguo_batch_instance(nguo_batch *b, uint32_t i, nguo_step **out) {
+  if (!b || !out || i >= b->inst.size()) return fail(NGUO_E_INVALID, "nguo_batch_instance: bad argument");
+  *out = b->inst[i];
+  return NGUO_OK;
 }
 
 int nguo_step_get_metrics(nguo_step *st, nguo_step_metrics *out) {

@@ -29,6 +29,7 @@ EXPORTED = [
     "nguo_step_get_newton_trace", "nguo_step_stencils", "nguo_step_collider_sides",
     "nguo_lbvh_morton", "nguo_lbvh_sort_pairs", "nguo_lbvh_karras", "nguo_lbvh_aabb",
     "nguo_step_bench_spmv", "nguo_bench_spmv_grid", "nguo_ls_precond_apply",
+    "nguo_batch_create", "nguo_batch_destroy", "nguo_batch_advance", "nguo_batch_instance",
 ]
 
 
@@ -127,6 +128,11 @@ def load():
         lib.nguo_lbvh_sort_pairs.argtypes = [vp, C.c_uint32, P(C.c_uint32), P(C.c_uint32)]
         lib.nguo_lbvh_karras.argtypes = [vp, C.c_uint32, P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
         lib.nguo_lbvh_aabb.argtypes = [vp, C.c_uint32, P(C.c_uint32), P(C.c_uint32), P(C.c_float), P(C.c_float)]
+        lib.nguo_batch_create.argtypes = [vp, C.c_uint32, P(SceneDesc), P(vp)]
+        lib.nguo_batch_destroy.argtypes = [vp]
+        lib.nguo_batch_destroy.restype = None
+        lib.nguo_batch_advance.argtypes = [vp, C.c_uint32]
+        lib.nguo_batch_instance.argtypes = [vp, C.c_uint32, P(vp)]
         lib.nguo_step_bench_spmv.argtypes = [vp, C.c_uint32, C.c_int, P(C.c_double), P(C.c_double)]
         lib.nguo_bench_spmv_grid.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32, P(C.c_double), P(C.c_double), P(C.c_double)]
     _lib = lib
@@ -279,6 +285,15 @@ class Stepper:
         self.n = scene.n_verts
         check(load().nguo_step_create(ctx.h, C.byref(scene.desc), C.byref(self.h)))
 
+    @classmethod
+    def from_handle(cls, handle, scene):
+        """A view of a step object owned elsewhere (a batch instance)."""
+        st = cls.__new__(cls)
+        st.h = handle
+        st.n = scene.n_verts
+        st._borrowed = True
+        return st
+
     def advance(self, nsteps=1):
         check(load().nguo_step_advance(self.h, int(nsteps)))
 
@@ -342,6 +357,31 @@ class Stepper:
         return side
 
     def close(self):
-        if self.h:
+        if self.h and not getattr(self, "_borrowed", False):
             load().nguo_step_destroy(self.h)
+            self.h = None
+
+
+class Batch:
+    """Independent scenes advanced concurrently, one stream each (nguo_batch_*)."""
+
+    def __init__(self, ctx, scenes):
+        self.scenes = list(scenes)  # keeps the scene buffers alive
+        arr = (SceneDesc * len(self.scenes))(*[sc.desc for sc in self.scenes])
+        self.h = C.c_void_p()
+        check(load().nguo_batch_create(ctx.h, len(self.scenes), arr, C.byref(self.h)))
+        self.steppers = []
+        for i, sc in enumerate(self.scenes):
+            hs = C.c_void_p()
+            check(load().nguo_batch_instance(self.h, i, C.byref(hs)))
+            self.steppers.append(Stepper.from_handle(hs, sc))
+
+    def advance(self, steps=1):
+        check(load().nguo_batch_advance(self.h, int(steps)))
+
+    def close(self):
+        if self.h:
+            for st in self.steppers:
+                st.h = None  # owned by the batch
+            load().nguo_batch_destroy(self.h)
             self.h = None

@@ -155,3 +155,21 @@ def test_tube_garment_step_parity_from_device_state(ctx):
     m = st.metrics()
     assert m.all_finite == 1 and m.penetration_free == 1 and m.max_reresid <= 1e-9
     st.close()
+
+
+@pytest.mark.gpu
+def test_batch_matches_independent_steppers(ctx):
+    # graded sizes advanced concurrently (nguo_batch_*): every instance takes exactly the
+    # steps a lone nguo_step takes, bit for bit
+    scenes = [nguo_hip.Scene(100, seed=0), nguo_hip.Scene(101, seed=101), nguo_hip.Scene(100, seed=5)]
+    b = nguo_hip.Batch(ctx, scenes)
+    b.advance(4)
+    for sc, bst in zip(scenes, b.steppers):
+        st = nguo_hip.Stepper(ctx, sc)
+        st.advance(4)
+        assert np.array_equal(bst.positions(), st.positions())
+        assert np.array_equal(bst.velocities(), st.velocities())
+        mb, ms = bst.metrics(), st.metrics()
+        assert mb.newton_iters == ms.newton_iters and mb.max_cg_iters == ms.max_cg_iters
+        st.close()
+    b.close()
