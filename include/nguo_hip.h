@@ -182,6 +182,25 @@ int nguo_step_stencils(nguo_step *st, const float *x, uint32_t cap, uint32_t *id
 /* implicit.rs:1431-1476 collider_sides at positions x: -1 inside the static body, +1 else. */
 int nguo_step_collider_sides(nguo_step *st, const float *x, float *side);
 
+/* ---- drape adjoint (the differentiation form, C4). No reference counterpart: the reference
+ * steps a fixed rest shape (engine-core/src/shell_model.rs:309-349 build_shells) and has no
+ * gradient. Parameters are per shell face rest-shape scales s_f: face f's rest shape is
+ * scaled by s_f (Dm^-1 / s_f, area s_f^2); default 1. nfaces = scene.n_shell_faces; scales
+ * must be finite and > 0. Faces that build_shells skips (degenerate) get gradient 0. */
+int nguo_step_set_rest_scale(nguo_step *st, const float *face_scale, uint32_t nfaces);
+/* Gradient of a loss L(x*) through the equilibrium x* of the last advanced step: with
+ * g(x; s) the step's gradient (inertia to its target, elastic terms, contact and lagged
+ * friction), g(x*; s) = 0, so dL/ds_f = -lambda . dg/ds_f with H lambda = dL/dx, H the Newton
+ * Hessian the step assembles (PSD-projected elastic blocks, rank-one barrier and friction
+ * terms; hessian = NGUO_ADJ_EXACT_ELASTIC leaves the shell and rod-segment blocks unprojected),
+ * solved by the same FP32 PCG with FP64 refinement to relative residual `tol`. A PCG
+ * breakdown (H not positive definite at x*) returns NGUO_E_UNSUPPORTED.
+ * dLdx [3 n_verts] (pinned rows ignored); dLds [n_shell_faces]. State is unchanged; the last
+ * step must not have been followed by set_state or set_rest_scale. */
+#define NGUO_ADJ_NEWTON 0        /* H as the Newton iteration assembles it */
+#define NGUO_ADJ_EXACT_ELASTIC 1 /* shells and rod segments unprojected (their exact second derivative) */
+int nguo_step_adjoint(nguo_step *st, const float *dLdx, float tol, int hessian, float *dLds, nguo_stats *stats);
+
 /* ---- batch of independent scenes (the graded-size form, C3). Each instance is a full step
  * object on its own stream; nguo_batch_advance advances all of them concurrently (one host
  * thread each), every instance taking exactly the steps nguo_step_advance would. Instances are

@@ -349,7 +349,9 @@ __device__ inline void shell_derivs(const float s[2], int model, float mu, float
 }
 
 // shell_model.rs:367-436: gradient (3 vertices) and 9x9 PSD Hessian of one shell element.
-__device__ inline void shell_element(const f3 p[3], Mat2 dm, float area, int model, float mu, float lambda, float eps, f3 g[3], float *h81) {
+// project = false keeps negative mode eigenvalues: the exact second derivative.
+__device__ inline void shell_element(const f3 p[3], Mat2 dm, float area, int model, float mu, float lambda, float eps, f3 g[3], float *h81,
+                                     bool project = true) {
   // F = [p1-p0, p2-p0] * Dm^-1
   f3 e1 = p[1] - p[0], e2 = p[2] - p[0];
   f3 f0 = e1 * dm.a + e2 * dm.b;
@@ -391,7 +393,7 @@ __device__ inline void shell_element(const f3 p[3], Mat2 dm, float area, int mod
   float a6[36];
   for (int i = 0; i < 36; ++i) a6[i] = 0.0f;
   for (int k = 0; k < 6; ++k) {
-    float l = fmaxf(lamk[k], 0.0f);
+    float l = project ? fmaxf(lamk[k], 0.0f) : lamk[k];
     if (l == 0.0f) continue;
     f3 qc0 = u0 * q[k][0] + u1 * q[k][1] + u2 * q[k][2];
     f3 qc1 = u0 * q[k][3] + u1 * q[k][4] + u2 * q[k][5];
@@ -474,8 +476,9 @@ __device__ inline void dihedral_grad(const f3 p[4], f3 g[4]) {
   g[3] = g3;
 }
 
-// implicit.rs:3287-3308 rod_stretch_block.
-__device__ inline void rod_stretch_block(f3 d, float rest, float k, float h[9], f3 &ga) {
+// implicit.rs:3287-3308 rod_stretch_block (project = false: compressed segments keep their
+// negative transverse stiffness, the exact second derivative).
+__device__ inline void rod_stretch_block(f3 d, float rest, float k, float h[9], f3 &ga, bool project = true) {
   float l = len(d);
   if (l < 1.0e-9f) {
     for (int i = 0; i < 9; ++i) h[i] = 0.0f;
@@ -486,7 +489,7 @@ __device__ inline void rod_stretch_block(f3 d, float rest, float k, float h[9], 
   f3 nrm = d / l;
   float r = (l - rest) / l;
   float c_axial = 1.0f - r;
-  float c_trans = fmaxf(r, 0.0f);
+  float c_trans = project ? fmaxf(r, 0.0f) : r;
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) {
       float iden = i == j ? 1.0f : 0.0f;

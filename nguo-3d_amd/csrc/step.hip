@@ -37,6 +37,7 @@ __host__ __device__ inline uint32_t code(uint32_t type, uint32_t payload) { retu
 struct ElemDev {
   uint32_t ns, nh, nr, nrh;
   const uint32_t *sh_v;  const float *sh_dm; const float *sh_area;
+  const float *sh_scale;  // rest-shape scale per shell (Dm^-1 / s, area s^2)
   const uint32_t *hg_v;  const float *hg_rest;
   const uint32_t *rd_v;  const float *rd_rest; const float *rd_k;
   const uint32_t *rh_v;  const float *rh_rest; const float *rh_stiff;
@@ -52,6 +53,7 @@ struct Params {
   float inv_h2;
   float dhat, kappa, mu, feps;
   int barrier_kind;
+  int project;  // PSD-project element Hessians (Newton); 0 = exact second derivatives (adjoint)
   uint32_t ncoll;
   nguo_collider coll[4];
 };
@@ -62,10 +64,11 @@ __global__ __launch_bounds__(BLOCK) void k_shells(ElemDev E, Params P, const flo
   const uint32_t *v = E.sh_v + 3 * (size_t)e;
   f3 p[3] = {ld3(x, v[0]), ld3(x, v[1]), ld3(x, v[2])};
   const float *dm = E.sh_dm + 4 * (size_t)e;
-  Mat2 m{dm[0], dm[1], dm[2], dm[3]};
+  const float sc = E.sh_scale[e];
+  Mat2 m{dm[0] / sc, dm[1] / sc, dm[2] / sc, dm[3] / sc};
   f3 g[3];
   float h[81];
-  shell_element(p, m, E.sh_area[e], P.shell_model, P.shell_mu, P.shell_lambda, P.eig_eps, g, h);
+  shell_element(p, m, E.sh_area[e] * sc * sc, P.shell_model, P.shell_mu, P.shell_lambda, P.eig_eps, g, h, P.project != 0);
   float *ob = E.out_dense + 81 * ((size_t)E.nrh + e);
   for (int a = 0; a < 3; ++a)
     for (int b = 0; b < 3; ++b)
@@ -73,6 +76,48 @@ __global__ __launch_bounds__(BLOCK) void k_shells(ElemDev E, Params P, const flo
         for (int c = 0; c < 3; ++c) ob[(a * 3 + b) * 9 + 3 * r + c] = h[(3 * a + r) * 9 + 3 * b + c];
   float *og = E.out_dg + 9 * ((size_t)E.nrh + e);
   for (int a = 0; a < 3; ++a) { og[3 * a] = g[a].x; og[3 * a + 1] = g[a].y; og[3 * a + 2] = g[a].z; }
+}
+
+// Drape adjoint, per active shell e with scale s: the derivative of its gradient in s.
+// With Dm'^-1 = Dm^-1 / s and area' = s^2 area, F' = F / s leaves U and V unchanged and
+// scales the stretches, so d g / d s = (1/s) area' U diag(c) V^T Dm'^-T (the gradient's
+// assembly) with c_i = dPsi/dsigma_i - sum_j d2Psi/dsigma_i dsigma_j sigma_j. Writes
+// out[face] = -lambda . dg/ds over the element's vertices (lambda on free dofs by vdof).
+__global__ __launch_bounds__(BLOCK) void k_shell_dgds(ElemDev E, Params P, const float *__restrict__ x, const uint32_t *__restrict__ vdof,
+                                                      const float *__restrict__ lam, const uint32_t *__restrict__ face, float *out) {
+  uint32_t e = blockIdx.x * BLOCK + threadIdx.x;
+  if (e >= E.ns) return;
+  const uint32_t *v = E.sh_v + 3 * (size_t)e;
+  f3 p[3] = {ld3(x, v[0]), ld3(x, v[1]), ld3(x, v[2])};
+  const float *dm = E.sh_dm + 4 * (size_t)e;
+  const float sc = E.sh_scale[e];
+  Mat2 m{dm[0] / sc, dm[1] / sc, dm[2] / sc, dm[3] / sc};
+  const float area = E.sh_area[e] * sc * sc;
+  f3 e1 = p[1] - p[0], e2 = p[2] - p[0];
+  f3 f0 = e1 * m.a + e2 * m.b;
+  f3 f1 = e1 * m.c + e2 * m.d;
+  f3 u0, u1;
+  float sg[2], v0[2], v1[2];
+  svd3x2(f0, f1, u0, u1, sg, v0, v1);
+  float deda[2], d2[4];
+  shell_derivs(sg, P.shell_model, P.shell_mu, P.shell_lambda, deda, d2);
+  const float c0s = deda[0] - (d2[0] * sg[0] + d2[1] * sg[1]);
+  const float c1s = deda[1] - (d2[2] * sg[0] + d2[3] * sg[1]);
+  f3 c0 = u0 * c0s, c1 = u1 * c1s;
+  f3 pk0 = c0 * v0[0] + c1 * v1[0];
+  f3 pk1 = c0 * v0[1] + c1 * v1[1];
+  f3 h0 = pk0 * m.a + pk1 * m.c;
+  f3 h1 = pk0 * m.b + pk1 * m.d;
+  const float k = area / sc;
+  f3 g[3] = {(h0 + h1) * (-k), h0 * k, h1 * k};
+  float acc = 0.0f;
+  for (int a = 0; a < 3; ++a) {
+    uint32_t d = vdof[v[a]];
+    if (d == 0xffffffffu) continue;
+    f3 l = ld3(lam, d);
+    acc += dot(l, g[a]);
+  }
+  out[face[e]] = -acc;
 }
 
 __global__ __launch_bounds__(BLOCK) void k_hinges(ElemDev E, Params P, const float *__restrict__ x) {
@@ -90,13 +135,13 @@ __global__ __launch_bounds__(BLOCK) void k_hinges(ElemDev E, Params P, const flo
   for (int a = 0; a < 4; ++a) { o[2 + 3 * a] = g[a].x; o[3 + 3 * a] = g[a].y; o[4 + 3 * a] = g[a].z; }
 }
 
-__global__ __launch_bounds__(BLOCK) void k_rods(ElemDev E, const float *__restrict__ x) {
+__global__ __launch_bounds__(BLOCK) void k_rods(ElemDev E, int project, const float *__restrict__ x) {
   uint32_t e = blockIdx.x * BLOCK + threadIdx.x;
   if (e >= E.nr) return;
   uint32_t a = E.rd_v[2 * (size_t)e], b = E.rd_v[2 * (size_t)e + 1];
   float h[9];
   f3 ga;
-  rod_stretch_block(ld3(x, a) - ld3(x, b), E.rd_rest[e], E.rd_k[e], h, ga);
+  rod_stretch_block(ld3(x, a) - ld3(x, b), E.rd_rest[e], E.rd_k[e], h, ga, project != 0);
   float *o = E.out_rod + 12 * (size_t)e;
   for (int i = 0; i < 9; ++i) o[i] = h[i];
   o[9] = ga.x; o[10] = ga.y; o[11] = ga.z;
@@ -788,6 +833,7 @@ using namespace nguo;
 
 struct ElemHost {
   std::vector<uint32_t> sh_v, hg_v, rd_v, rh_v;
+  std::vector<uint32_t> sh_face;  // scene shell face index of each shell
   std::vector<float> sh_dm, sh_area, hg_rest, rd_rest, rd_k, rh_rest, rh_stiff;
   uint32_t ns() const { return (uint32_t)sh_area.size(); }
   uint32_t nh() const { return (uint32_t)hg_rest.size(); }
@@ -799,11 +845,15 @@ struct ElemSet {
   uint32_t ns = 0, nh = 0, nr = 0, nrh = 0;
   DevBuf<uint32_t> sh_v, hg_v, rd_v, rh_v;
   DevBuf<float> sh_dm, sh_area, hg_rest, rd_rest, rd_k, rh_rest, rh_stiff;
+  DevBuf<float> sh_scale;
+  DevBuf<uint32_t> sh_face;
+  std::vector<uint32_t> h_sh_face;
+  std::vector<float> h_sh_scale;  // host copy (kept alive for the asynchronous upload)
   DevBuf<float> out_dense, out_dg, out_rod, out_hinge;
   SellMatrix M;
   DevBuf<uint32_t> slot_row, hc_ptr, hc, gc_ptr, gc, row_vert;
   ElemDev dev() const {
-    return ElemDev{ns, nh, nr, nrh, sh_v.p, sh_dm.p, sh_area.p, hg_v.p, hg_rest.p, rd_v.p, rd_rest.p, rd_k.p,
+    return ElemDev{ns, nh, nr, nrh, sh_v.p, sh_dm.p, sh_area.p, sh_scale.p, hg_v.p, hg_rest.p, rd_v.p, rd_rest.p, rd_k.p,
                    rh_v.p, rh_rest.p, rh_stiff.p, out_dense.p, out_dg.p, out_rod.p, out_hinge.p};
   }
 };
@@ -850,6 +900,11 @@ struct nguo_step {
   unsigned long long *hl = nullptr;
   nguo_step_metrics metrics{};
   DevBuf<float> trace;  // x after every Newton iteration of the last step
+  // drape adjoint: start of the last step (its friction reference), adjoint vector and rhs
+  uint32_t n_shell_faces = 0;
+  DevBuf<float> xprev, lam, adj_b, dgds;
+  std::vector<float> h_adj_b;
+  bool adj_valid = false;  // the last step can be differentiated (cleared by set_state / set_rest_scale)
   std::vector<float> trace_toi;
   bool own_stream = false;
   ~nguo_step() {
@@ -1001,6 +1056,10 @@ void upload_elems(ElemSet &S, const ElemHost &H, hipStream_t s) {
     (void)sizeof(T);
   };
   up(S.sh_v, H.sh_v); up(S.sh_dm, H.sh_dm); up(S.sh_area, H.sh_area);
+  S.h_sh_face = H.sh_face;
+  S.h_sh_scale.assign(H.ns(), 1.0f);
+  up(S.sh_scale, S.h_sh_scale);
+  up(S.sh_face, S.h_sh_face);
   up(S.hg_v, H.hg_v); up(S.hg_rest, H.hg_rest);
   up(S.rd_v, H.rd_v); up(S.rd_rest, H.rd_rest); up(S.rd_k, H.rd_k);
   up(S.rh_v, H.rh_v); up(S.rh_rest, H.rh_rest); up(S.rh_stiff, H.rh_stiff);
@@ -1012,7 +1071,7 @@ void upload_elems(ElemSet &S, const ElemHost &H, hipStream_t s) {
 
 void run_elements(ElemSet &S, const Params &P, const float *x, hipStream_t s) {
   ElemDev E = S.dev();
-  if (S.nr) hipLaunchKernelGGL(k_rods, dim3(div_up(S.nr, BLOCK)), dim3(BLOCK), 0, s, E, x);
+  if (S.nr) hipLaunchKernelGGL(k_rods, dim3(div_up(S.nr, BLOCK)), dim3(BLOCK), 0, s, E, P.project, x);
   if (S.nrh) hipLaunchKernelGGL(k_rod_hinges, dim3(div_up(S.nrh, BLOCK)), dim3(BLOCK), 0, s, E, x);
   if (S.ns) hipLaunchKernelGGL(k_shells, dim3(div_up(S.ns, BLOCK)), dim3(BLOCK), 0, s, E, P, x);
   if (S.nh) hipLaunchKernelGGL(k_hinges, dim3(div_up(S.nh, BLOCK)), dim3(BLOCK), 0, s, E, P, x);
@@ -1228,6 +1287,8 @@ void advance_one(nguo_step *st) {
   const unsigned GV = div_up(V, BLOCK);
   float h = st->prm.dt;
   st->P.inv_h2 = 1.0f / (h * h);
+  st->adj_valid = false;
+  NGUO_HIP(hipMemcpyAsync(st->xprev.p, st->x0.p, 3 * (size_t)V * sizeof(float), hipMemcpyDeviceToDevice, s));
   hipLaunchKernelGGL(k_predict, dim3(GV), dim3(BLOCK), 0, s, V, (const float *)st->im.p, (const float *)st->x0.p, (const float *)st->v.p, h,
                      f3{st->prm.gravity[0], st->prm.gravity[1], st->prm.gravity[2]}, st->xt.p, st->x.p);
   uint32_t cap_it = std::max(st->prm.newton_iters, 1u), taken = 0;
@@ -1290,6 +1351,7 @@ void advance_one(nguo_step *st) {
   m.all_finite = bad ? 0 : 1;
   m.penetration_free = (!bad && !hit) ? 1 : 0;
   m.last_toi = last_t;
+  st->adj_valid = !bad;
 }
 
 }  // namespace
@@ -1343,6 +1405,7 @@ int nguo_step_create(nguo_ctx *ctx, const nguo_scene_desc *sc, nguo_step **out) 
     P.mu = st->prm.friction;
     P.feps = st->prm.friction_eps;
     P.barrier_kind = st->prm.barrier_kind;
+    P.project = 1;
     P.ncoll = sc->n_colliders;
     P.inv_h2 = 1.0f / (st->prm.dt * st->prm.dt);
     for (uint32_t c = 0; c < sc->n_colliders; ++c) {
@@ -1378,6 +1441,7 @@ int nguo_step_create(nguo_ctx *ctx, const nguo_scene_desc *sc, nguo_step **out) 
         H.sh_v.insert(H.sh_v.end(), t, t + 3);
         H.sh_dm.insert(H.sh_dm.end(), dm, dm + 4);
         H.sh_area.push_back(area);
+        H.sh_face.push_back(f);
       }
     }
     if (st->prm.bend_stiffness > 0.0f) {
@@ -1440,6 +1504,8 @@ int nguo_step_create(nguo_ctx *ctx, const nguo_scene_desc *sc, nguo_step **out) 
     else st->v.zero(3 * (size_t)V, s);
     st->xt.reserve(3 * (size_t)V);
     st->dx.reserve(3 * (size_t)V);
+    st->xprev.reserve(3 * (size_t)V);
+    st->n_shell_faces = sc->shell_faces ? sc->n_shell_faces : 0;
     st->im.upload(st->h_im, s);
     st->vdof.upload(st->h_vdof, s);
     st->sdof.upload(sdof, s);
@@ -1656,6 +1722,7 @@ int nguo_step_set_state(nguo_step *st, const float *positions, const float *velo
     st->x0.upload(positions, 3 * (size_t)st->V, st->s);
     st->x.upload(positions, 3 * (size_t)st->V, st->s);
     if (velocities) st->v.upload(velocities, 3 * (size_t)st->V, st->s);
+    st->adj_valid = false;
     NGUO_HIP(hipStreamSynchronize(st->s));
     return NGUO_OK;
   });
@@ -1712,6 +1779,83 @@ int nguo_step_collider_sides(nguo_step *st, const float *x, float *side) {
     st->side.download(side, st->V, s);
     NGUO_HIP(hipMemcpyAsync(st->x.p, st->x0.p, 3 * (size_t)st->V * sizeof(float), hipMemcpyDeviceToDevice, s));
     NGUO_HIP(hipStreamSynchronize(s));
+    return NGUO_OK;
+  });
+}
+
+int nguo_step_set_rest_scale(nguo_step *st, const float *face_scale, uint32_t nfaces) {
+  return guarded([&]() -> int {
+    NGUO_REQUIRE(st && face_scale, NGUO_E_INVALID, "nguo_step_set_rest_scale: null argument");
+    NGUO_REQUIRE(nfaces == st->n_shell_faces, NGUO_E_INVALID, "nguo_step_set_rest_scale: nfaces must equal the scene's shell face count");
+    for (uint32_t f = 0; f < nfaces; ++f)
+      NGUO_REQUIRE(std::isfinite(face_scale[f]) && face_scale[f] > 0.0f, NGUO_E_INVALID, "nguo_step_set_rest_scale: scales must be finite and > 0");
+    hipStream_t s = st->s;
+    NGUO_HIP(hipStreamSynchronize(s));  // the host copies below may still feed an earlier upload
+    for (ElemSet *S : {&st->act, &st->stat}) {
+      if (!S->ns) continue;
+      for (uint32_t e = 0; e < S->ns; ++e) S->h_sh_scale[e] = face_scale[S->h_sh_face[e]];
+      S->sh_scale.upload(S->h_sh_scale, s);
+    }
+    // the static element blocks are evaluated once; refresh them for the new rest shapes
+    if (st->stat.M.nb) {
+      run_elements(st->stat, st->P, st->x.p, s);
+      gather(st->stat, st->P, 0, st->im.p, st->x.p, st->x0.p, s);
+    }
+    st->adj_valid = false;
+    NGUO_HIP(hipStreamSynchronize(s));
+    return NGUO_OK;
+  });
+}
+
+int nguo_step_adjoint(nguo_step *st, const float *dLdx, float tol, int hessian, float *dLds, nguo_stats *stats) {
+  return guarded([&]() -> int {
+    NGUO_REQUIRE(st && dLdx && dLds, NGUO_E_INVALID, "nguo_step_adjoint: null argument");
+    NGUO_REQUIRE(st->adj_valid, NGUO_E_INVALID,
+                 "nguo_step_adjoint: no step to differentiate (advance first; set_state and set_rest_scale clear the last step)");
+    NGUO_REQUIRE(tol > 0.0f, NGUO_E_INVALID, "nguo_step_adjoint: tol must be > 0");
+    NGUO_REQUIRE(hessian == NGUO_ADJ_NEWTON || hessian == NGUO_ADJ_EXACT_ELASTIC, NGUO_E_INVALID, "nguo_step_adjoint: unknown hessian mode");
+    hipStream_t s = st->s;
+    const uint32_t nfc = st->n_shell_faces;
+    if (nfc) std::memset(dLds, 0, nfc * sizeof(float));
+    if (st->NF == 0 || nfc == 0) {
+      if (stats) *stats = nguo_stats{};
+      return NGUO_OK;
+    }
+    // H at x* (= x = x0 after the commit) with the last step's inertial target (xt) and
+    // friction reference (its start, xprev)
+    NGUO_HIP(hipMemcpyAsync(st->x0.p, st->xprev.p, 3 * (size_t)st->V * sizeof(float), hipMemcpyDeviceToDevice, s));
+    run_elements(st->act, st->P, st->x.p, s);
+    gather(st->act, st->P, 1, st->im.p, st->x.p, st->x0.p, s);
+    contacts(st);  // contact stiffnesses from the step's (projected) elastic blocks
+    if (hessian == NGUO_ADJ_EXACT_ELASTIC) {
+      Params PA = st->P;
+      PA.project = 0;
+      run_elements(st->act, PA, st->x.p, s);
+      gather(st->act, PA, 1, st->im.p, st->x.p, st->x0.p, s);
+    }
+    // rhs: dL/dx on the free dofs
+    std::vector<float> &hb = st->h_adj_b;
+    hb.resize(3 * (size_t)st->NF);
+    for (uint32_t k = 0; k < st->NF; ++k)
+      for (int c = 0; c < 3; ++c) hb[3 * (size_t)k + c] = dLdx[3 * (size_t)st->h_free[k] + c];
+    st->adj_b.upload(hb, s);
+    st->lam.reserve(3 * (size_t)st->NF);
+    PcgStats ps = st->pcg->solve(st->act.M, contact_view(st), st->adj_b.p, tol, st->prm.cg_max_iter, st->lam.p);
+    NGUO_HIP(hipMemcpyAsync(st->x0.p, st->x.p, 3 * (size_t)st->V * sizeof(float), hipMemcpyDeviceToDevice, s));
+    NGUO_REQUIRE(!ps.breakdown, NGUO_E_UNSUPPORTED,
+                 "nguo_step_adjoint: the adjoint Hessian is not positive definite at x* (PCG breakdown); use the Newton Hessian");
+    st->dgds.zero(nfc, s);
+    if (st->act.ns)
+      hipLaunchKernelGGL(k_shell_dgds, dim3(div_up(st->act.ns, BLOCK)), dim3(BLOCK), 0, s, st->act.dev(), st->P, (const float *)st->x.p,
+                         (const uint32_t *)st->vdof.p, (const float *)st->lam.p, (const uint32_t *)st->act.sh_face.p, st->dgds.p);
+    NGUO_HIP(hipGetLastError());
+    st->dgds.download(dLds, nfc, s);
+    NGUO_HIP(hipStreamSynchronize(s));
+    if (stats) {
+      stats->iters = ps.iters;
+      stats->sweeps = ps.sweeps;
+      stats->residual = ps.residual;
+    }
     return NGUO_OK;
   });
 }

@@ -30,6 +30,7 @@ EXPORTED = [
     "nguo_lbvh_morton", "nguo_lbvh_sort_pairs", "nguo_lbvh_karras", "nguo_lbvh_aabb",
     "nguo_step_bench_spmv", "nguo_bench_spmv_grid", "nguo_ls_precond_apply",
     "nguo_batch_create", "nguo_batch_destroy", "nguo_batch_advance", "nguo_batch_instance",
+    "nguo_step_set_rest_scale", "nguo_step_adjoint",
 ]
 
 
@@ -134,6 +135,8 @@ def load():
         lib.nguo_batch_advance.argtypes = [vp, C.c_uint32]
         lib.nguo_batch_instance.argtypes = [vp, C.c_uint32, P(vp)]
         lib.nguo_step_bench_spmv.argtypes = [vp, C.c_uint32, C.c_int, P(C.c_double), P(C.c_double)]
+        lib.nguo_step_set_rest_scale.argtypes = [vp, P(C.c_float), C.c_uint32]
+        lib.nguo_step_adjoint.argtypes = [vp, P(C.c_float), C.c_float, C.c_int, P(C.c_float), P(Stats)]
         lib.nguo_bench_spmv_grid.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32, P(C.c_double), P(C.c_double), P(C.c_double)]
     _lib = lib
     return lib
@@ -277,12 +280,35 @@ def bench_spmv_grid(ctx, nx, ny, reps=20):
     return ms.value, by.value, nnzb.value
 
 
+def drape_panels(scene, n_around=8, n_height=8, axes=None):
+    """Panel index [0, n_around * n_height) of every scene shell face, from the face centroids
+    of the rest shape: n_around sectors of the angle about the vertical (y) axis through the
+    cloth centroid times n_height bands of the height. axes = (i, j) bins coordinates i and j
+    instead (flat sheets). These panels carry the drape adjoint's rest-scale parameters."""
+    nf = scene.desc.n_shell_faces
+    faces = np.ctypeslib.as_array(scene.desc.shell_faces, (3 * nf,)).reshape(nf, 3)
+    pos = np.ctypeslib.as_array(scene.desc.positions, (3 * scene.n_verts,)).reshape(-1, 3).astype(np.float64)
+    cen = pos[faces].mean(axis=1)
+
+    def bins(c, n):
+        lo, hi = c.min(), c.max()
+        return np.clip(((c - lo) / max(hi - lo, 1e-12) * n).astype(np.int64), 0, n - 1)
+
+    if axes is not None:
+        return bins(cen[:, axes[0]], n_around) * n_height + bins(cen[:, axes[1]], n_height)
+    mid = cen.mean(axis=0)
+    ang = np.arctan2(cen[:, 2] - mid[2], cen[:, 0] - mid[0])
+    sector = np.clip(((ang + np.pi) / (2 * np.pi) * n_around).astype(np.int64), 0, n_around - 1)
+    return sector * n_height + bins(cen[:, 1], n_height)
+
+
 class Stepper:
     """The whole implicit step (GpuImplicitSolver / ImplicitSolver::step) on the device."""
 
     def __init__(self, ctx, scene):
         self.h = C.c_void_p()
         self.n = scene.n_verts
+        self.nfaces = scene.desc.n_shell_faces
         check(load().nguo_step_create(ctx.h, C.byref(scene.desc), C.byref(self.h)))
 
     @classmethod
@@ -291,6 +317,7 @@ class Stepper:
         st = cls.__new__(cls)
         st.h = handle
         st.n = scene.n_verts
+        st.nfaces = scene.desc.n_shell_faces
         st._borrowed = True
         return st
 
@@ -349,6 +376,20 @@ class Stepper:
         ms, by = C.c_double(), C.c_double()
         check(load().nguo_step_bench_spmv(self.h, int(reps), int(bool(iteration_form)), C.byref(ms), C.byref(by)))
         return ms.value, by.value
+
+    def set_rest_scale(self, face_scale):
+        """Rest-shape scale per scene shell face (the drape adjoint's parameters)."""
+        sc = np.ascontiguousarray(face_scale, dtype=np.float32)
+        check(load().nguo_step_set_rest_scale(self.h, fptr(sc), len(sc)))
+
+    def adjoint(self, dLdx, tol=1e-6, exact_elastic=False):
+        """dL/ds per shell face through the last step's equilibrium, given dL/dx [n, 3], and
+        the adjoint solve's stats. exact_elastic: unprojected shell / rod-segment blocks."""
+        g = np.ascontiguousarray(dLdx, dtype=np.float32).reshape(-1)
+        out = np.zeros(self.nfaces, np.float32)
+        st = Stats()
+        check(load().nguo_step_adjoint(self.h, fptr(g), float(tol), 1 if exact_elastic else 0, fptr(out), C.byref(st)))
+        return out, st
 
     def collider_sides(self, x):
         x = np.ascontiguousarray(x, dtype=np.float32)

@@ -126,7 +126,8 @@ inline void shell_eigenpairs(const T s[2], const T deda[2], const T d2[4], T eps
 
 // shell_model.rs:262-288 shell_d2psi_df2_from_derivs: sum max(lam_k,0) q_k q_k^T.
 template <class T>
-inline void shell_d2psi_df2(const M32<T> &u, const T s[2], const M2<T> &v, const T deda[2], const T d2[4], T eps, T out[36]) {
+inline void shell_d2psi_df2(const M32<T> &u, const T s[2], const M2<T> &v, const T deda[2], const T d2[4], T eps, T out[36],
+                            bool project = true) {
   M32<T> q[6];
   T lam[6];
   shell_eigenpairs(s, deda, d2, eps, q, lam);
@@ -134,7 +135,7 @@ inline void shell_d2psi_df2(const M32<T> &u, const T s[2], const M2<T> &v, const
   for (int i = 0; i < 36; ++i) out[i] = 0;
   M2<T> vt = v.transpose();
   for (int k = 0; k < 6; ++k) {
-    T l = std::fmax(lam[k], T(0));
+    T l = project ? std::fmax(lam[k], T(0)) : lam[k];
     if (l == T(0)) continue;
     auto col = [&](V3<T> c) { return u.c0 * c.x + u.c1 * c.y + u2 * c.z; };
     M32<T> uq(col(q[k].c0), col(q[k].c1));
@@ -149,13 +150,15 @@ template <class T> struct ShellElem {
   uint32_t v[3];
   M2<T> dm_inv;
   T area;
+  uint32_t face = 0;  // index among the faces handed to build_shells
 };
 
 // shell_model.rs:309-349 build_shells (intrinsic in-plane frame; skips degenerate faces).
 template <class T>
 inline std::vector<ShellElem<T>> build_shells(const std::vector<std::array<uint32_t, 3>> &tris, const std::vector<V3<T>> &rest, T min_area) {
   std::vector<ShellElem<T>> out;
-  for (auto &t : tris) {
+  for (uint32_t fi = 0; fi < (uint32_t)tris.size(); ++fi) {
+    const auto &t = tris[fi];
     V3<T> p0 = rest[t[0]], p1 = rest[t[1]], p2 = rest[t[2]];
     V3<T> e1 = p1 - p0, e2 = p2 - p0;
     V3<T> n = cross(e1, e2);
@@ -169,6 +172,7 @@ inline std::vector<ShellElem<T>> build_shells(const std::vector<std::array<uint3
     el.v[0] = t[0]; el.v[1] = t[1]; el.v[2] = t[2];
     el.dm_inv = dm.inverse();
     el.area = area;
+    el.face = fi;
     out.push_back(el);
   }
   return out;
@@ -217,7 +221,9 @@ template <class T> inline void shell_dfdx(const M2<T> &dm_inv, T b[54]) {
 }
 
 // shell_model.rs:403-436 shell_hessian_analytic: area * B^T A6 B.
-template <class T> inline void shell_hessian(const V3<T> p[3], const ShellElem<T> &el, int model, T mu, T lambda, T eps, T h[81]) {
+// project = false: the exact second derivative (negative mode eigenvalues kept).
+template <class T>
+inline void shell_hessian(const V3<T> p[3], const ShellElem<T> &el, int model, T mu, T lambda, T eps, T h[81], bool project = true) {
   M32<T> f = shell_F(p, el.dm_inv), u;
   T s[2];
   M2<T> v;
@@ -225,7 +231,7 @@ template <class T> inline void shell_hessian(const V3<T> p[3], const ShellElem<T
   T deda[2], d2[4], a6[36], b[54], ab[54];
   shell_dpsi(s, model, mu, lambda, deda);
   shell_d2psi(s, model, mu, lambda, d2);
-  shell_d2psi_df2(u, s, v, deda, d2, eps, a6);
+  shell_d2psi_df2(u, s, v, deda, d2, eps, a6, project);
   shell_dfdx(el.dm_inv, b);
   for (int r = 0; r < 6; ++r)
     for (int c = 0; c < 9; ++c) {
@@ -319,7 +325,7 @@ inline std::vector<Hinge<T>> build_hinges(const std::vector<std::array<uint32_t,
 
 // ---------------------------------------------------------------- rods
 // implicit.rs:3287-3308 rod_stretch_block: 3x3 block (row-major) and gradient on a.
-template <class T> inline void rod_stretch_block(V3<T> d, T rest, T k, T h[9], V3<T> &ga) {
+template <class T> inline void rod_stretch_block(V3<T> d, T rest, T k, T h[9], V3<T> &ga, bool project = true) {
   T l = length(d);
   if (l < T(1.0e-9)) {
     for (int i = 0; i < 9; ++i) h[i] = 0;
@@ -330,7 +336,7 @@ template <class T> inline void rod_stretch_block(V3<T> d, T rest, T k, T h[9], V
   V3<T> nrm = d / l;
   T r = (l - rest) / l;
   T c_axial = T(1) - r;
-  T c_trans = std::fmax(r, T(0));
+  T c_trans = project ? std::fmax(r, T(0)) : r;
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) {
       T iden = i == j ? T(1) : T(0);

@@ -75,6 +75,32 @@ void oracle_solver_set_state(void *p, const float *x, const float *v) {
   else { h->s32->pos = load<float>(x, n); if (v) h->s32->vel = load<float>(v, n); }
 }
 
+// Per shell face rest scale (nfaces = the scene's shell face count).
+void oracle_solver_set_rest_scale(void *p, const double *scale, uint32_t nfaces) {
+  auto *h = (Handle *)p;
+  auto go = [&](auto &s, auto tag) {
+    using T = decltype(tag);
+    s.face_scale.assign(nfaces, T(1));
+    for (uint32_t f = 0; f < nfaces; ++f) s.face_scale[f] = T(scale[f]);
+  };
+  if (h->fp64) go(*h->s64, 0.0); else go(*h->s32, 0.0f);
+}
+
+// Adjoint of the last step's equilibrium: dLds[f] for every shell face given dL/dx (3 n).
+// exact: unprojected shell / rod-segment blocks; xstar (may be null): evaluation point.
+uint32_t oracle_solver_adjoint(void *p, const double *dLdx, uint32_t nfaces, double cg_tol, int exact, const double *xstar, double *dLds,
+                               double *residual) {
+  auto *h = (Handle *)p;
+  uint32_t n = oracle_solver_nverts(p);
+  std::vector<double> g(dLdx, dLdx + 3 * (size_t)n);
+  CgStats st;
+  std::vector<double> out = h->fp64 ? h->s64->adjoint(g, nfaces, cg_tol, &st, exact != 0, xstar)
+                                    : h->s32->adjoint(g, nfaces, cg_tol, &st, exact != 0, xstar);
+  for (uint32_t f = 0; f < nfaces; ++f) dLds[f] = out[f];
+  if (residual) *residual = st.residual;
+  return st.iters;
+}
+
 // out: time, max_reresid, max_cg_iters, newton_iters, last_toi, penetration_free, all_finite, n_edges
 void oracle_solver_metrics(void *p, double *out) {
   auto *h = (Handle *)p;
@@ -162,6 +188,25 @@ void oracle_solver_gradient(void *p, const float *x, const float *xt, double inv
     std::vector<float> rhs;
     std::vector<T> gr;
     s.assemble(load<T>(x, n), load<T>(xt, n), T(inv_h2), rhs, nullptr, nullptr, &gr);
+    for (size_t i = 0; i < gr.size(); ++i) grad[i] = (double)gr[i];
+  };
+  if (h->fp64) go(*h->s64, 0.0); else go(*h->s32, 0.0f);
+}
+
+// As oracle_solver_gradient with FP64 positions and inertial target (derivative checks).
+void oracle_solver_gradient_d(void *p, const double *x, const double *xt, double inv_h2, double *grad) {
+  auto *h = (Handle *)p;
+  uint32_t n = oracle_solver_nverts(p);
+  auto go = [&](auto &s, auto tag) {
+    using T = decltype(tag);
+    std::vector<V3<T>> xv(n), tv(n);
+    for (uint32_t i = 0; i < n; ++i) {
+      xv[i] = V3<T>(T(x[3 * i]), T(x[3 * i + 1]), T(x[3 * i + 2]));
+      tv[i] = V3<T>(T(xt[3 * i]), T(xt[3 * i + 1]), T(xt[3 * i + 2]));
+    }
+    std::vector<float> rhs;
+    std::vector<T> gr;
+    s.assemble(xv, tv, T(inv_h2), rhs, nullptr, nullptr, &gr);
     for (size_t i = 0; i < gr.size(); ++i) grad[i] = (double)gr[i];
   };
   if (h->fp64) go(*h->s64, 0.0); else go(*h->s32, 0.0f);

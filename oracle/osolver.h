@@ -64,8 +64,8 @@ inline bool invert3x3(const double m[3][3], double out[3][3]) {
 }
 
 // linsolve.rs:272-411 cg_cpu_precond(Block) + cg_run: f64 PCG, residual checked every
-// iteration, f32 result.
-inline std::vector<float> cg_block_jacobi(const Csr &a, const std::vector<float> &b, float tol, uint32_t max_iter, CgStats *st) {
+// iteration; FP64 right-hand side and result.
+inline std::vector<double> cg_block_jacobi64(const Csr &a, const std::vector<double> &bf, double tol, uint32_t max_iter, uint32_t *iters_out) {
   const uint32_t n = a.n;
   bool block3 = n % 3 == 0;
   std::vector<double> inv_diag(n, 1.0);
@@ -116,8 +116,6 @@ inline std::vector<float> cg_block_jacobi(const Csr &a, const std::vector<float>
     for (size_t i = 0; i < u.size(); ++i) s += u[i] * v[i];
     return s;
   };
-  std::vector<double> bf(n);
-  for (uint32_t i = 0; i < n; ++i) bf[i] = b[i];
   double bnorm = std::fmax(std::sqrt(dotv(bf, bf)), 1e-30);
   std::vector<double> x(n, 0.0), r = bf, z(n), p, ap(n);
   precond(r, z);
@@ -133,13 +131,24 @@ inline std::vector<float> cg_block_jacobi(const Csr &a, const std::vector<float>
     for (uint32_t i = 0; i < n; ++i) { x[i] += alpha * p[i]; r[i] -= alpha * ap[i]; }
     iters++;
     residual = std::sqrt(dotv(r, r)) / bnorm;
-    if (residual <= (double)tol) break;
+    if (residual <= tol) break;
     precond(r, z);
     double rz_new = dotv(r, z);
     double beta = rz_new / rz;
     for (uint32_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
     rz = rz_new;
   }
+  if (iters_out) *iters_out = iters;
+  return x;
+}
+
+// The same with an f32 right-hand side and result (the reference's solve seam).
+inline std::vector<float> cg_block_jacobi(const Csr &a, const std::vector<float> &b, float tol, uint32_t max_iter, CgStats *st) {
+  const uint32_t n = a.n;
+  std::vector<double> bf(n);
+  for (uint32_t i = 0; i < n; ++i) bf[i] = b[i];
+  uint32_t iters = 0;
+  std::vector<double> x = cg_block_jacobi64(a, bf, (double)tol, max_iter, &iters);
   std::vector<float> out(n);
   for (uint32_t i = 0; i < n; ++i) out[i] = (float)x[i];
   if (st) {
@@ -399,6 +408,19 @@ template <class T> class Solver {
   std::vector<Tri> tris;
   std::vector<Hinge<T>> hinges;
   std::vector<ShellElem<T>> shells;
+  // Per shell face rest-shape scale s (empty = 1): the rest shape of the face is scaled by s,
+  // i.e. Dm^-1 / s and area s^2 (the parameters of the drape adjoint).
+  std::vector<T> face_scale;
+  std::vector<V> last_xt;  // inertial target of the last step
+  bool project_elastic = true;  // PSD-project shell / rod-segment Hessians (false: exact)
+  ShellElem<T> scaled(const ShellElem<T> &el) const {
+    if (face_scale.empty()) return el;
+    const T sc = face_scale[el.face];
+    ShellElem<T> e = el;
+    e.dm_inv = M2<T>(V2<T>(el.dm_inv.xa.x / sc, el.dm_inv.xa.y / sc), V2<T>(el.dm_inv.ya.x / sc, el.dm_inv.ya.y / sc));
+    e.area = el.area * sc * sc;
+    return e;
+  }
   struct Rod { uint32_t a, b; T rest, k; };
   struct RodHinge { uint32_t j, i, k; T rest, stiff; };
   std::vector<Rod> rods;
@@ -716,10 +738,11 @@ template <class T> class Solver {
   }
 
   // implicit.rs:2280-3155 assemble -> (Dirichlet-filtered CSR, rhs). `stencils_out` (may be
-  // null) receives the active stencils used; `kappa_out` their stiffnesses.
+  // null) receives the active stencils used; `kappa_out` their stiffnesses; `kappa_in` (may
+  // be null) supplies them instead of the Rayleigh stiffness of this assembly's blocks.
   Csr assemble(const std::vector<V> &x, const std::vector<V> &xt, T inv_h2, std::vector<float> &rhs_out,
                std::vector<Stencil<T>> *stencils_out = nullptr, std::vector<T> *kappa_out = nullptr,
-               std::vector<T> *grad_out = nullptr) const {
+               std::vector<T> *grad_out = nullptr, const std::vector<T> *kappa_in = nullptr) const {
     const uint32_t n = (uint32_t)x.size();
     std::vector<T> grad(3 * n, T(0));
     Blocks blk;
@@ -740,7 +763,7 @@ template <class T> class Solver {
     for (auto &rd : rods) {
       T hk[9];
       V ga;
-      rod_stretch_block(x[rd.a] - x[rd.b], rd.rest, rd.k, hk, ga);
+      rod_stretch_block(x[rd.a] - x[rd.b], rd.rest, rd.k, hk, ga, project_elastic);
       for (int c = 0; c < 3; ++c) { grad[3 * rd.a + c] += ga[c]; grad[3 * rd.b + c] -= ga[c]; }
       for (int r = 0; r < 3; ++r)
         for (int cc = 0; cc < 3; ++cc) {
@@ -771,12 +794,13 @@ template <class T> class Solver {
     // isotropic shells (implicit.rs:2608-2652)
     if (!shells.empty() && (prm.shell_mu > 0.0f || prm.shell_lambda > 0.0f)) {
       T smu = T(prm.shell_mu), slam = T(prm.shell_lambda);
-      for (auto &el : shells) {
+      for (auto &el0 : shells) {
+        const ShellElem<T> el = scaled(el0);
         V p[3] = {x[el.v[0]], x[el.v[1]], x[el.v[2]]};
         V eg[3];
         T eh[81];
         shell_gradient(p, el, prm.shell_model, smu, slam, eg);
-        shell_hessian(p, el, prm.shell_model, smu, slam, T(prm.eiganalysis_eps), eh);
+        shell_hessian(p, el, prm.shell_model, smu, slam, T(prm.eiganalysis_eps), eh, project_elastic);
         for (int a = 0; a < 3; ++a) {
           uint32_t va = el.v[a];
           for (int c = 0; c < 3; ++c) grad[3 * va + c] += eg[a][c];
@@ -854,8 +878,10 @@ template <class T> class Solver {
     // self / body contact (implicit.rs:3026-3103)
     if (!sc.empty()) {
       Blocks snap = blk;
+      size_t si = 0;
       for (auto &s : sc) {
-        T k = self_contact_rayleigh(snap, s);
+        T k = kappa_in ? (*kappa_in)[si] : self_contact_rayleigh(snap, s);
+        ++si;
         if (kappa_out) kappa_out->push_back(k);
         int kind = s.d < T(0) ? BARRIER_CUBIC : prm.barrier_kind;
         T bo[3];
@@ -1082,6 +1108,79 @@ template <class T> class Solver {
     return t;
   }
 
+  // Adjoint of the last step's equilibrium (the drape adjoint's reference). At the converged
+  // end state x* the step gradient vanishes, grad E_step(x*; s) = 0, so for a loss L(x*):
+  //   dL/ds_f = -lambda^T d(grad E_step)/ds_f,  H lambda = dL/dx,
+  // with H the Newton Hessian the step assembles at x* (inertial target and friction reference
+  // of that step). Only the shells depend on s; their derivative is taken by central
+  // differences of the element gradient in s (relative step 1e-6). dLdx is per vertex (3 n);
+  // pinned rows are ignored. exact: unprojected shell / rod-segment blocks. xstar (3 n, may be
+  // null) evaluates at that point instead of the step's own end state. Returns d L / d s per
+  // scene shell face (nfaces).
+  std::vector<double> adjoint(const std::vector<double> &dLdx, uint32_t nfaces, double cg_tol, CgStats *st, bool exact = false,
+                              const double *xstar = nullptr) {
+    const uint32_t n = (uint32_t)pos.size();
+    const T h = T(prm.dt), inv_h2 = T(1) / (h * h);
+    std::vector<V> xs = pos;
+    if (xstar)
+      for (uint32_t i = 0; i < n; ++i) xs[i] = V(T(xstar[3 * i]), T(xstar[3 * i + 1]), T(xstar[3 * i + 2]));
+    std::vector<V> keep = pos;
+    pos = prev;  // friction reference of the last step
+    std::vector<float> rhs;
+    std::vector<T> kap;
+    Csr a = assemble(xs, last_xt, inv_h2, rhs, nullptr, &kap);
+    if (exact) {
+      // contact stiffnesses stay those of the step (Rayleigh against the projected blocks)
+      project_elastic = false;
+      a = assemble(xs, last_xt, inv_h2, rhs, nullptr, nullptr, nullptr, &kap);
+      project_elastic = true;
+    }
+    pos = keep;
+    std::vector<double> b(3 * n, 0.0);
+    for (uint32_t i = 0; i < n; ++i)
+      if (inv_mass[i] > 0.0f)
+        for (int c = 0; c < 3; ++c) b[3 * i + c] = dLdx[3 * i + c];
+    uint32_t iters = 0;
+    std::vector<double> lam = cg_block_jacobi64(a, b, cg_tol, 100000, &iters);
+    if (st) {
+      // FP64 relative residual of the solve
+      double num = 0, den = 0;
+      for (uint32_t row = 0; row < a.n; ++row) {
+        double sacc = 0;
+        for (uint32_t k = a.row_ptr[row]; k < a.row_ptr[row + 1]; ++k) sacc += (double)a.values[k] * lam[a.col_idx[k]];
+        num += (b[row] - sacc) * (b[row] - sacc);
+        den += b[row] * b[row];
+      }
+      st->iters = iters;
+      st->residual = std::sqrt(num) / std::fmax(std::sqrt(den), 1e-300);
+    }
+    std::vector<double> out(nfaces, 0.0);
+    if (!(prm.shell_mu > 0.0f || prm.shell_lambda > 0.0f)) return out;
+    const T smu = T(prm.shell_mu), slam = T(prm.shell_lambda);
+    for (auto &el0 : shells) {
+      const T s0 = face_scale.empty() ? T(1) : face_scale[el0.face];
+      const T ds = T(1e-6) * s0;
+      V p[3] = {xs[el0.v[0]], xs[el0.v[1]], xs[el0.v[2]]};
+      V gp[3], gm[3];
+      ShellElem<T> ep = el0, em = el0;
+      const T sp = s0 + ds, sm = s0 - ds;
+      ep.dm_inv = M2<T>(V2<T>(el0.dm_inv.xa.x / sp, el0.dm_inv.xa.y / sp), V2<T>(el0.dm_inv.ya.x / sp, el0.dm_inv.ya.y / sp));
+      ep.area = el0.area * sp * sp;
+      em.dm_inv = M2<T>(V2<T>(el0.dm_inv.xa.x / sm, el0.dm_inv.xa.y / sm), V2<T>(el0.dm_inv.ya.x / sm, el0.dm_inv.ya.y / sm));
+      em.area = el0.area * sm * sm;
+      shell_gradient(p, ep, prm.shell_model, smu, slam, gp);
+      shell_gradient(p, em, prm.shell_model, smu, slam, gm);
+      double acc = 0.0;
+      for (int k = 0; k < 3; ++k) {
+        const uint32_t v = el0.v[k];
+        if (inv_mass[v] <= 0.0f) continue;
+        for (int c = 0; c < 3; ++c) acc += lam[3 * v + c] * (double)((gp[k][c] - gm[k][c]) / (sp - sm));
+      }
+      out[el0.face] -= acc;
+    }
+    return out;
+  }
+
   // implicit.rs:1951-2191 step.
   void step() {
     const uint32_t n = (uint32_t)pos.size();
@@ -1090,6 +1189,7 @@ template <class T> class Solver {
     V g(T(prm.gravity[0]), T(prm.gravity[1]), T(prm.gravity[2]));
     std::vector<V> x0 = pos, xt(n);
     for (uint32_t i = 0; i < n; ++i) xt[i] = inv_mass[i] > 0.0f ? x0[i] + vel[i] * h + g * (h * h) : x0[i];
+    last_xt = xt;
     std::vector<V> x = x0;
     last_max_reresid = 0;
     last_max_cg_iters = 0;

@@ -34,6 +34,10 @@ def load():
         lib.oracle_solver_positions_d.argtypes = [vp, D]
         lib.oracle_solver_velocities.argtypes = [vp, F]
         lib.oracle_solver_set_state.argtypes = [vp, F, F]
+        lib.oracle_solver_set_rest_scale.argtypes = [vp, D, C.c_uint32]
+        lib.oracle_solver_set_rest_scale.restype = None
+        lib.oracle_solver_adjoint.argtypes = [vp, D, C.c_uint32, C.c_double, C.c_int, D, D, D]
+        lib.oracle_solver_adjoint.restype = C.c_uint32
         lib.oracle_solver_metrics.argtypes = [vp, D]
         lib.oracle_solver_newton_trace.restype = C.c_int
         lib.oracle_solver_newton_trace.argtypes = [vp, C.c_uint32, F, D]
@@ -45,6 +49,8 @@ def load():
         lib.oracle_solver_assemble.argtypes = [vp, F, F, C.c_double, F, C.c_uint32]
         lib.oracle_solver_csr.argtypes = [vp, U, U, F, F]
         lib.oracle_solver_gradient.argtypes = [vp, F, F, C.c_double, D]
+        lib.oracle_solver_gradient_d.argtypes = [vp, D, D, C.c_double, D]
+        lib.oracle_solver_gradient_d.restype = None
         lib.oracle_solver_ccd.restype = C.c_double
         lib.oracle_solver_ccd.argtypes = [vp, F, F]
         lib.oracle_solver_self_intersects.restype = C.c_int
@@ -150,6 +156,21 @@ class OracleSolver:
     def set_state(self, x, v):
         x, v = f32(x), f32(v)
         self.lib.oracle_solver_set_state(self.h, fp(x), fp(v))
+
+    def set_rest_scale(self, face_scale):
+        """Per shell face rest-shape scale (one value per scene shell face)."""
+        sc = d64(face_scale)
+        self.lib.oracle_solver_set_rest_scale(self.h, dp(sc), len(sc))
+
+    def adjoint(self, dLdx, nfaces, cg_tol=1e-10, exact=False, xstar=None):
+        """dL/ds per shell face for the last step's equilibrium, given dL/dx [n, 3]; exact =
+        unprojected elastic blocks; xstar = evaluation point (default: the step's end state)."""
+        g = d64(dLdx).reshape(-1)
+        out = np.zeros(nfaces)
+        res = np.zeros(1)
+        xs = None if xstar is None else d64(xstar).reshape(-1)
+        it = self.lib.oracle_solver_adjoint(self.h, dp(g), nfaces, cg_tol, 1 if exact else 0, None if xs is None else dp(xs), dp(out), dp(res))
+        return out, int(it), float(res[0])
 
     def metrics(self):
         out = np.zeros(8)

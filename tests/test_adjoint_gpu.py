@@ -1,0 +1,119 @@
+"""The device drape adjoint (nguo_step_set_rest_scale / nguo_step_adjoint) against the
+oracle: scaled rest shapes step like the oracle's, the adjoint matches the oracle's FP64
+reference adjoint at the same end state, and FP64 central finite differences of the loss."""
+import numpy as np
+import pytest
+
+from conftest import nguo_hip
+import drape_case as D
+import oracle_py as O
+
+REF_REL = 1e-4  # |g_device - g_fp64| / |g_fp64| (2-norm over the 64 panels)
+FD_REL = 1e-4   # per checked parameter, against FP64 central differences (h = 1e-4)
+
+
+@pytest.mark.gpu
+def test_unit_rest_scale_is_bitwise_identity(ctx):
+    s = nguo_hip.Scene(101, seed=1)
+    a, b = nguo_hip.Stepper(ctx, s), nguo_hip.Stepper(ctx, s)
+    b.set_rest_scale(np.ones(s.desc.n_shell_faces, np.float32))
+    a.advance(2)
+    b.advance(2)
+    assert np.array_equal(a.positions(), b.positions())
+    a.close()
+    b.close()
+
+
+@pytest.mark.gpu
+def test_scaled_rest_shape_step_matches_oracle(ctx):
+    s = nguo_hip.Scene(101, seed=1)
+    theta, _, _ = D.draws()
+    scale = theta[nguo_hip.drape_panels(s)]
+    o32, o64 = O.OracleSolver(s), O.OracleSolver(s, fp64=True)
+    st = nguo_hip.Stepper(ctx, s)
+    for o in (o32, o64):
+        o.set_rest_scale(scale)
+    st.set_rest_scale(scale.astype(np.float32))
+    for _ in range(3):
+        o32.step(1)
+        o64.step(1)
+        st.advance(1)
+        x, x32, x64 = st.positions(), o32.positions(), o64.positions()
+        spread = np.abs(x32 - x64).max()
+        bar = max(1e-5, 1.5 * spread)
+        assert np.abs(x - x32).max() <= bar
+    st.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("exact", [False, True])
+def test_adjoint_matches_fp64_reference_in_contact(ctx, exact):
+    # Small tube in contact with the hip capsule: one step from a drape state on both sides,
+    # then the device adjoint and the oracle's FP64 adjoint at the device's end state with the
+    # same dL/dx.
+    s = D.tube_scene(newton_iters=8, cg_tol=1e-6)
+    panel = nguo_hip.drape_panels(s)
+    theta, _, _ = D.draws()
+    o = O.OracleSolver(s, fp64=True)
+    o.step(6)
+    xs, vs = o.positions(), o.velocities()
+    st = nguo_hip.Stepper(ctx, s)
+    for obj in (o, st):
+        obj.set_rest_scale(theta[panel].astype(np.float32) if obj is st else theta[panel])
+        obj.set_state(xs, vs)
+    o.step(1)
+    st.advance(1)
+    assert st.metrics().num_stencils > 0, "the state must be in contact"
+    x = st.positions()
+    rng = np.random.default_rng(11)
+    free = s.inv_mass > 0
+    dLdx = np.zeros((s.n_verts, 3))
+    dLdx[free] = rng.normal(size=(free.sum(), 3)) / free.sum()
+    g_dev, stats = st.adjoint(dLdx, tol=1e-7, exact_elastic=exact)
+    assert stats.residual <= 1e-7
+    g_ref, _, res = o.adjoint(dLdx, s.desc.n_shell_faces, 1e-13, exact=exact, xstar=x.astype(np.float64))
+    assert res < 1e-10
+    pd = np.bincount(panel, weights=g_dev.astype(np.float64), minlength=64)
+    pr = np.bincount(panel, weights=g_ref, minlength=64)
+    rel = np.linalg.norm(pd - pr) / np.linalg.norm(pr)
+    print(f"panels rel {rel:.3e}, faces rel {np.linalg.norm(g_dev - g_ref) / np.linalg.norm(g_ref):.3e}, iters {stats.iters}")
+    assert rel <= REF_REL
+    st.close()
+
+
+@pytest.mark.gpu
+def test_adjoint_matches_central_fd(ctx):
+    s = D.sheet_scene()
+    theta, theta_t, pick = D.draws()
+    panel = nguo_hip.drape_panels(s, axes=(0, 2))
+    dr = D.OracleDrape(s, 5, panel, theta_t)
+    _, x64 = dr.loss(theta)
+    st = nguo_hip.Stepper(ctx, s)
+    st.set_rest_scale(theta[panel].astype(np.float32))
+    st.set_state(dr.xs, dr.vs)
+    st.advance(1)
+    assert np.abs(st.positions() - x64).max() <= 1e-5
+    g_face, stats = st.adjoint(dr.dloss_dx(x64), tol=1e-7, exact_elastic=True)
+    g = np.bincount(panel, weights=g_face.astype(np.float64), minlength=64)
+    fd = np.array([dr.central_fd(theta, k) for k in pick])
+    rel = np.abs(g[pick] - fd) / np.abs(fd)
+    print("fd rel", rel)
+    assert rel.max() <= FD_REL
+    st.close()
+
+
+@pytest.mark.gpu
+def test_adjoint_needs_a_step(ctx):
+    s = nguo_hip.Scene(100, seed=1)
+    st = nguo_hip.Stepper(ctx, s)
+    g = np.zeros((s.n_verts, 3), np.float32)
+    with pytest.raises(nguo_hip.NguoError):
+        st.adjoint(g)
+    st.advance(1)
+    st.adjoint(g)  # zero loss gradient: fine after a step
+    st.set_rest_scale(np.ones(s.desc.n_shell_faces, np.float32))
+    with pytest.raises(nguo_hip.NguoError):
+        st.adjoint(g)
+    with pytest.raises(nguo_hip.NguoError):
+        st.set_rest_scale(np.ones(3, np.float32))
+    st.close()
