@@ -10,7 +10,9 @@ line search. A step is one `nguo_step_advance(1)`; inputs are resident on the de
 Prints one JSON line: ms per implicit step (lower is better), the roofline of the PCG
 operator (3x3-BSR SpMV + matrix-free contact) timed with HIP events on the library stream,
 the same operator on a synthetic 2048 x 2048 cloth BSR that exceeds the last-level cache,
-and the CPU baseline: the oracle restatement (FP32, one core) on a bounded sample.
+the graded-size batch (8 tubes advanced concurrently vs one after another), 20 timed
+gradient-descent iterations of the drape adjoint, and the CPU baseline: the oracle
+restatement (FP32, one core) on a bounded sample.
 """
 import argparse
 import importlib.util
@@ -40,6 +42,9 @@ def main():
     ap.add_argument("--grid", type=int, default=2048, help="side of the synthetic SpMV grid (0 = skip)")
     ap.add_argument("--c3-steps", type=int, default=3, help="timed steps of the 8-size graded batch vs sequential (0 = skip)")
     ap.add_argument("--c3-warmup", type=int, default=2)
+    ap.add_argument("--c4-iters", type=int, default=20, help="timed gradient-descent iterations of the drape adjoint (0 = skip)")
+    ap.add_argument("--c4-hessian", choices=["exact", "newton"], default="newton",
+                    help="adjoint Hessian: exact elastic second derivatives, or the Newton iteration's projected blocks")
     ap.add_argument("--schwarz-domain", type=int, default=None, help="override the scene's Schwarz domain size")
     ap.add_argument("--schwarz-weight", type=float, default=None, help="override the scene's two-level coarse weight (0 = one level)")
     # 14.16 MB read + 0.61 MB written per launch of the iteration-form SpMV on this workload,
@@ -85,6 +90,7 @@ def main():
     sync()
     elapsed = time.perf_counter() - t0
     ms = 1000.0 * elapsed / args.steps
+    x_e, v_e = st.positions(), st.velocities()  # the drape after warmup + timed steps (start of the adjoint loop)
 
     # the dominant kernel: the SpMV instance of the PCG iteration on the final state's system
     spmv_ms, spmv_bytes = st.bench_spmv(reps=500, iteration_form=True)
@@ -134,6 +140,65 @@ def main():
               "form": "one stream and host thread per instance, kernels interleaved on the device"}
         batch.close()
 
+    c4 = None
+    if args.c4_iters > 0:
+        # differentiable drape: 64 panel rest scales ~ N(1, 0.02) (seed 4), target = the drape
+        # of a second draw (seed 5) from the same start state (the drape after the timed
+        # steps); each gradient-descent iteration = one implicit step from that state + the
+        # adjoint solve
+        import numpy as np
+        panel = nh.drape_panels(scene)
+        free = scene.inv_mass > 0
+        theta = np.random.default_rng(4).normal(1.0, 0.02, 64)
+        theta_t = np.random.default_rng(5).normal(1.0, 0.02, 64)
+
+        def forward(th):
+            st.set_rest_scale(th[panel].astype(np.float32))
+            st.set_state(x_e, v_e)
+            st.advance(1)
+            return st.positions().astype(np.float64)
+
+        target = forward(theta_t)
+        exact = args.c4_hessian == "exact"
+        losses, fwd_ms, adj_ms, adj_iters, fallbacks = [], [], [], [], 0
+        dist0 = float(np.linalg.norm(theta - theta_t))
+        step = 0.004
+        sync()
+        t1 = time.perf_counter()
+        for it in range(args.c4_iters):
+            ta = time.perf_counter()
+            x = forward(theta)
+            tb = time.perf_counter()
+            d = (x - target)[free]
+            losses.append(float((d * d).sum(axis=1).mean()))
+            dLdx = np.zeros_like(x)
+            dLdx[free] = 2.0 * d / free.sum()
+            try:
+                g_face, stats = st.adjoint(dLdx, tol=1e-6, exact_elastic=exact)
+            except nh.NguoError:
+                fallbacks += 1  # H not positive definite at this end state: the Newton blocks
+                g_face, stats = st.adjoint(dLdx, tol=1e-6, exact_elastic=False)
+            tc = time.perf_counter()
+            fwd_ms.append(1000.0 * (tb - ta))
+            adj_ms.append(1000.0 * (tc - tb))
+            adj_iters.append(stats.iters)
+            g = np.bincount(panel, weights=g_face.astype(np.float64), minlength=64)
+            gmax = np.abs(g).max()
+            if gmax > 0:
+                theta = theta - step * g / gmax  # normalised descent step, 0.4 % per panel at most
+            step *= 0.9
+        sync()
+        total = time.perf_counter() - t1
+        c4 = {"workload": "tube garment (config above) with 64 panel rest scales (8 sectors x 8 height bands), theta ~ N(1, 0.02) "
+                          "seed 4, target drape from seed 5; loss = mean |x* - target|^2 over free vertices; "
+                          f"one implicit step from the drape after {args.warmup + args.steps} steps + one adjoint solve per iteration",
+              "iters": args.c4_iters, "ms_per_iter": round(1000.0 * total / args.c4_iters, 3),
+              "forward_ms": round(sum(fwd_ms) / len(fwd_ms), 3), "adjoint_ms": round(sum(adj_ms) / len(adj_ms), 3),
+              "adjoint_pcg_iters": round(sum(adj_iters) / len(adj_iters), 1), "hessian": args.c4_hessian,
+              "newton_fallbacks": fallbacks, "loss_first": losses[0], "loss_last": losses[-1],
+              "theta_dist_first": round(dist0, 6), "theta_dist_last": round(float(np.linalg.norm(theta - theta_t)), 6)}
+        st.set_rest_scale(np.ones(scene.desc.n_shell_faces, np.float32))
+
     cpu = None
     if args.cpu_steps > 0:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -169,6 +234,7 @@ def main():
         "roofline": roofline,
         "spmv_hbm": grid,
         "batched_c3": c3,
+        "adjoint_c4": c4,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
