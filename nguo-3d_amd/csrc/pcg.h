@@ -6,6 +6,7 @@
 
 #include "device_common.h"
 #include "nguo_internal.h"
+#include "pcg_device.h"
 
 namespace nguo {
 
@@ -69,9 +70,24 @@ struct PcgStats {
 // Writes the r.z | r.r partials of the new z to nxt (2 nd).
 void launch_schwarz_pcg(hipStream_t s, const Schwarz &sz, bool init, const double *r64, float *r, float *z, float *dsol, float *p,
                         const float *q, const double *pap_part, uint32_t npap, double *nxt, double *scal, uint32_t *flag, uint32_t j);
+// Iteration form of launch_schwarz_pcg over several one-level systems in one launch.
+void launch_schwarz_pcg_batch(hipStream_t s, const BatchOf<SchArgs> &B, uint32_t j);
 // Two-level form only, right after launch_schwarz_pcg: z += weight C^T S1 (C r) on every
 // domain's rows (and p = z for init); replaces the r.z partials in nxt.
 void launch_schwarz_coarse(hipStream_t s, const Schwarz &sz, bool init, const float *r, float *z, float *p, double *nxt, const uint32_t *flag);
+
+class Pcg;
+// One system of a batched solve (Pcg::solve_batch); stats receives its result.
+struct PcgBatchItem {
+  Pcg *pcg;
+  const SellMatrix *M;
+  ContactView C;
+  const float *b;
+  float tol;
+  uint32_t max_iter;
+  float *x_out;
+  PcgStats stats;
+};
 
 class Pcg {
  public:
@@ -80,6 +96,14 @@ class Pcg {
   // Solve (A + C) x = b to ||b - (A+C) x|| <= tol ||b|| (FP64 residual). b is a device
   // f32 vector [3 nb]; the solution lands in x_out (device f32 [3 nb]).
   PcgStats solve(const SellMatrix &M, const ContactView &C, const float *b, float tol, uint32_t max_iter, float *x_out);
+  // Solves n <= MAX_BATCH systems, each with its own Pcg (one-level Schwarz), together on
+  // stream s: refinement sweeps run in lockstep over the systems that still need one, and
+  // each PCG iteration is three launches spanning all of them (a system whose sweep has
+  // converged exits those launches at its first instruction). Every system takes exactly
+  // the iterations, sweeps and values its own solve() would. The caller orders s after the
+  // systems' assembly and its own streams after s.
+  static void solve_batch(hipStream_t s, PcgBatchItem *items, uint32_t n);
+  bool batchable() const { return sz_ && sz_->nd > 0 && sz_->nd1 == 0; }
   // y = (A + C) x in f32 (stride-3 vectors), for parity checks.
   void apply(const SellView &A, const ContactView &C, const float *x, float *y);
   // Average duration (ms) of the PCG's SpMV kernel on (A + C) x over `reps` back-to-back

@@ -169,8 +169,8 @@ template <int MODE> struct SlotGroup {
 // k_pcg_spmv evaluates it, and each free participant's block of H_s p,
 // hp (g.p) g_a + lambda w_a (s_w - n (n . s_w)), stored at its incidence slot.
 template <int MODE>
-__global__ __launch_bounds__(BLOCK) void k_pcg_pre(ContactView C, const float4 *__restrict__ z, const float4 *__restrict__ pold, const double *cur,
-                                                  uint32_t nz, double *scal, uint32_t *flag, uint32_t j) {
+__device__ __forceinline__ void pcg_pre_body(uint32_t bid, const ContactView &C, const float4 *__restrict__ z, const float4 *__restrict__ pold,
+                                             const double *cur, uint32_t nz, double *scal, uint32_t *flag, uint32_t j) {
   __shared__ double lds[2 * BLOCK / WAVE];
   if (MODE != SPMV_PLAIN && flag[0]) return;
   float beta = 0.0f;
@@ -181,7 +181,7 @@ __global__ __launch_bounds__(BLOCK) void k_pcg_pre(ContactView C, const float4 *
     sum_parts_all<BLOCK, 2>(src, n, v, lds);
     const double b = v[0] / scal[8 + ((j - 1) & 1u)];
     const bool conv = v[1] <= scal[5], capped = j >= flag[2];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (bid == 0 && threadIdx.x == 0) {
       flag[1] = j;
       scal[8 + (j & 1u)] = v[0];
       scal[3] = b;
@@ -193,7 +193,7 @@ __global__ __launch_bounds__(BLOCK) void k_pcg_pre(ContactView C, const float4 *
     beta = (float)b;
   }
   if (threadIdx.x >= PRE_STENCILS) return;
-  const uint32_t s = blockIdx.x * PRE_STENCILS + threadIdx.x;
+  const uint32_t s = bid * PRE_STENCILS + threadIdx.x;
   if (s >= C.ns) return;
   float s1 = 0.f, a0 = 0.f, a1 = 0.f, a2 = 0.f;
   for (int b = 0; b < 4; ++b) {
@@ -222,6 +222,19 @@ __global__ __launch_bounds__(BLOCK) void k_pcg_pre(ContactView C, const float4 *
   }
 }
 
+template <int MODE>
+__global__ __launch_bounds__(BLOCK) void k_pcg_pre(ContactView C, const float4 *__restrict__ z, const float4 *__restrict__ pold, const double *cur,
+                                                  uint32_t nz, double *scal, uint32_t *flag, uint32_t j) {
+  pcg_pre_body<MODE>(blockIdx.x, C, z, pold, cur, nz, scal, flag, j);
+}
+
+// The same over several systems (one grid, system i's workgroups at B.off[i]).
+template <int MODE> __global__ __launch_bounds__(BLOCK) void kb_pcg_pre(BatchOf<PreArgs> B, uint32_t j) {
+  const uint32_t i = batch_slot(B.off, B.k, blockIdx.x);
+  const PreArgs &a = B.a[i];
+  pcg_pre_body<MODE>(blockIdx.x - B.off[i], a.C, a.z, a.pold, a.cur, a.nz, a.scal, a.flag, j);
+}
+
 inline unsigned pre_grid(const ContactView &C) { return std::max(1u, div_up(C.ns, PRE_STENCILS)); }
 
 // q = (A + C) p over one SELL slice (64 block rows, one per lane) per single-wavefront
@@ -232,11 +245,11 @@ inline unsigned pre_grid(const ContactView &C) { return std::max(1u, div_up(C.ns
 // and every reader of a neighbour's p evaluates the same expression, so they agree bitwise.
 // part[slice] = p.q over the slice.
 template <int MODE>
-__global__ __launch_bounds__(WAVE) void k_pcg_spmv(SellView A, ContactView C, const float4 *__restrict__ z, const float4 *__restrict__ pold,
-                                                  float4 *__restrict__ pnew, float4 *__restrict__ q, double *part, const double *scal,
-                                                  const uint32_t *flag) {
+__device__ __forceinline__ void pcg_spmv_body(uint32_t slice, const SellView &A, const ContactView &C, const float4 *__restrict__ z,
+                                              const float4 *__restrict__ pold, float4 *__restrict__ pnew, float4 *__restrict__ q, double *part,
+                                              const double *scal, const uint32_t *flag) {
   const uint32_t state = MODE != SPMV_PLAIN ? flag[0] : 0u;
-  const uint32_t lane = threadIdx.x, slice = blockIdx.x, row = slice * 64u + lane;
+  const uint32_t lane = threadIdx.x, row = slice * 64u + lane;
   const bool active = row < A.nb;
   uint32_t base = 0, width = 0, len = 0, ib = 0, ie = 0;
   if (active) {
@@ -278,6 +291,19 @@ __global__ __launch_bounds__(WAVE) void k_pcg_spmv(SellView A, ContactView C, co
   if (MODE == SPMV_PLAIN) return;
   dp = wave_sum(dp);
   if (lane == 0) part[slice] = dp;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(WAVE) void k_pcg_spmv(SellView A, ContactView C, const float4 *__restrict__ z, const float4 *__restrict__ pold,
+                                                  float4 *__restrict__ pnew, float4 *__restrict__ q, double *part, const double *scal,
+                                                  const uint32_t *flag) {
+  pcg_spmv_body<MODE>(blockIdx.x, A, C, z, pold, pnew, q, part, scal, flag);
+}
+
+template <int MODE> __global__ __launch_bounds__(WAVE) void kb_pcg_spmv(BatchOf<SpmvArgs> B) {
+  const uint32_t i = batch_slot(B.off, B.k, blockIdx.x);
+  const SpmvArgs &a = B.a[i];
+  pcg_spmv_body<MODE>(blockIdx.x - B.off[i], a.A, a.C, a.z, a.pold, a.pnew, a.q, a.part, a.scal, a.flag);
 }
 
 // 3x3 block-Jacobi preconditioner step, one block row per thread (vectors of stride 4,
@@ -622,6 +648,161 @@ PcgStats Pcg::solve(const SellMatrix &M, const ContactView &C, const float *b, f
   NGUO_HIP(hipGetLastError());
   st.residual = std::sqrt(rr / bb);
   return st;
+}
+
+void Pcg::solve_batch(hipStream_t s, PcgBatchItem *items, uint32_t n) {
+  NGUO_REQUIRE(n <= MAX_BATCH, NGUO_E_INVALID, "batched PCG: too many systems");
+  for (uint32_t i = 0; i < n; ++i)
+    NGUO_REQUIRE(items[i].pcg->batchable(), NGUO_E_UNSUPPORTED, "batched PCG: every system needs one-level Schwarz");
+  // every Pcg works on the batch stream for the duration
+  std::vector<hipStream_t> own(n);
+  for (uint32_t i = 0; i < n; ++i) { own[i] = items[i].pcg->s_; items[i].pcg->s_ = s; }
+  struct Restore {
+    PcgBatchItem *it; std::vector<hipStream_t> &own; uint32_t n;
+    ~Restore() { for (uint32_t i = 0; i < n; ++i) it[i].pcg->s_ = own[i]; }
+  } restore{items, own, n};
+  std::vector<double> bb(n), rr(n);
+  std::vector<uint8_t> done(n, 0);
+  for (uint32_t i = 0; i < n; ++i) {
+    PcgBatchItem &it = items[i];
+    Pcg &P = *it.pcg;
+    it.stats = PcgStats{};
+    const uint32_t nb = it.M->nb, n3 = 3 * nb;
+    P.ensure(nb, it.C.ns);
+    const SellView A = it.M->view();
+    const unsigned G = div_up(nb, BLOCK), G3 = div_up(n3, BLOCK), GS = div_up(it.C.ns, BLOCK);
+    P.build_precond(*it.M, it.C);
+    hipLaunchKernelGGL(k_fill64, dim3(G3), dim3(BLOCK), 0, s, n3, P.x64_.p, 0.0);
+    if (it.C.ns) hipLaunchKernelGGL((k_contact_sums<double, double>), dim3(GS), dim3(BLOCK), 0, s, it.C, (const double *)P.x64_.p, P.sums64_.p, nullptr);
+    hipLaunchKernelGGL(k_residual64, dim3(G), dim3(BLOCK), 0, s, A, it.C, (const double *)P.sums64_.p, it.b, (const double *)P.x64_.p, P.r64_.p,
+                       P.part_.p);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(BLOCK), 0, s, (const double *)P.part_.p, G, P.scal_.p + 7);
+    NGUO_HIP(hipMemcpyAsync(P.h_scal_ + 7, P.scal_.p + 7, sizeof(double), hipMemcpyDeviceToHost, s));
+  }
+  NGUO_HIP(hipStreamSynchronize(s));
+  for (uint32_t i = 0; i < n; ++i) {
+    bb[i] = rr[i] = items[i].pcg->h_scal_[7];
+    if (!(bb[i] > 0.0)) done[i] = 1;
+  }
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  for (auto &e : ev) NGUO_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  struct EvFree {
+    hipEvent_t *e;
+    ~EvFree() { for (int k = 0; k < 2; ++k) if (e[k]) (void)hipEventDestroy(e[k]); }
+  } evfree{ev};
+  const uint32_t chunk = items[0].pcg->chunk;
+  for (uint32_t sweep = 0; sweep < 12; ++sweep) {
+    // systems that need this sweep
+    std::vector<uint32_t> act;
+    for (uint32_t i = 0; i < n; ++i) {
+      if (done[i]) continue;
+      const PcgBatchItem &it = items[i];
+      const double res = std::sqrt(rr[i] / bb[i]);
+      if (res <= (double)it.tol || it.stats.iters >= it.max_iter) { done[i] = 1; continue; }
+      act.push_back(i);
+    }
+    if (act.empty()) break;
+    BatchOf<PreArgs> bf{}, bn{};
+    BatchOf<SpmvArgs> sf{}, sn0{}, sn1{};
+    BatchOf<SchArgs> c0{}, c1{};
+    const uint32_t k = (uint32_t)act.size();
+    bf.k = bn.k = sf.k = sn0.k = sn1.k = c0.k = c1.k = k;
+    for (uint32_t a = 0; a < k; ++a) {
+      PcgBatchItem &it = items[act[a]];
+      Pcg &P = *it.pcg;
+      const uint32_t nb = it.M->nb, NS = div_up(nb, 64), nd = P.sz_->nd;
+      const double res = std::sqrt(rr[act[a]] / bb[act[a]]);
+      const double eta = std::min(0.5, std::max(0.25 * (double)it.tol / res, 1.0e-5));
+      double *part_spmv = P.part_.p, *part_pc = P.part_.p + std::max<uint32_t>(NS, div_up(nb, BLOCK));
+      P.launch_precond_init(*it.M, part_pc);
+      hipLaunchKernelGGL(k_init_scal, dim3(1), dim3(BLOCK), 0, s, P.scal_.p, P.flag_.p, (const double *)part_pc, nd, eta * eta,
+                         it.max_iter - it.stats.iters);
+      const SellView A = it.M->view();
+      const float4 *z4 = (const float4 *)P.z_.p;
+      float4 *p4 = (float4 *)P.p_.p, *p24 = (float4 *)P.p2_.p;
+      // FIRST (j = 0): p_0 in p_; NEXT with j odd: pold = p_, pnew = p2_; j even: the reverse
+      bf.a[a] = PreArgs{it.C, z4, p4, part_pc, nd, P.scal_.p, P.flag_.p};
+      bn.a[a] = PreArgs{it.C, z4, nullptr, part_pc, nd, P.scal_.p, P.flag_.p};
+      sf.a[a] = SpmvArgs{A, it.C, z4, p4, nullptr, (float4 *)P.q_.p, part_spmv, P.scal_.p, P.flag_.p};
+      sn1.a[a] = SpmvArgs{A, it.C, z4, p4, p24, (float4 *)P.q_.p, part_spmv, P.scal_.p, P.flag_.p};
+      sn0.a[a] = SpmvArgs{A, it.C, z4, p24, p4, (float4 *)P.q_.p, part_spmv, P.scal_.p, P.flag_.p};
+      c0.a[a] = SchArgs{nd, P.sz_->dom_ptr.p, P.sz_->dom_dof.p, P.sz_->ainv.p, P.r_.p, P.z_.p, P.d_.p, P.p_.p, P.q_.p, part_spmv, NS, part_pc,
+                        P.scal_.p, P.flag_.p};
+      c1.a[a] = c0.a[a];
+      c1.a[a].p = P.p2_.p;
+      bf.off[a + 1] = bf.off[a] + (it.C.ns ? pre_grid(it.C) : 0u);
+      bn.off[a + 1] = bn.off[a] + pre_grid(it.C);
+      sf.off[a + 1] = sn0.off[a + 1] = sn1.off[a + 1] = sf.off[a] + NS;
+      c0.off[a + 1] = c1.off[a + 1] = c0.off[a] + nd;
+    }
+    // pold of the NEXT pre launch by parity (pre reads p_{j-1}: j odd -> p_, j even -> p2_)
+    BatchOf<PreArgs> bn1 = bn, bn0 = bn;
+    for (uint32_t a = 0; a < k; ++a) {
+      Pcg &P = *items[act[a]].pcg;
+      bn1.a[a].pold = (const float4 *)P.p_.p;
+      bn0.a[a].pold = (const float4 *)P.p2_.p;
+    }
+    uint32_t j = 0, queued = 0, waited = 0;
+    auto queue_chunk = [&]() {
+      for (uint32_t c = 0; c < chunk; ++c, ++j) {
+        if (j == 0) {
+          if (bf.off[k]) hipLaunchKernelGGL(kb_pcg_pre<SPMV_FIRST>, dim3(bf.off[k]), dim3(BLOCK), 0, s, bf, j);
+          hipLaunchKernelGGL(kb_pcg_spmv<SPMV_FIRST>, dim3(sf.off[k]), dim3(WAVE), 0, s, sf);
+        } else {
+          hipLaunchKernelGGL(kb_pcg_pre<SPMV_NEXT>, dim3(bn.off[k]), dim3(BLOCK), 0, s, (j & 1) ? bn1 : bn0, j);
+          hipLaunchKernelGGL(kb_pcg_spmv<SPMV_NEXT>, dim3(sf.off[k]), dim3(WAVE), 0, s, (j & 1) ? sn1 : sn0);
+        }
+        launch_schwarz_pcg_batch(s, (j & 1) ? c1 : c0, j);
+      }
+      NGUO_HIP(hipGetLastError());
+      const uint32_t sl = queued & 1u;
+      for (uint32_t a = 0; a < k; ++a) {
+        Pcg &P = *items[act[a]].pcg;
+        NGUO_HIP(hipMemcpyAsync(P.h_flag_ + 4 * sl, P.flag_.p, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+      }
+      NGUO_HIP(hipEventRecord(ev[sl], s));
+      ++queued;
+    };
+    uint32_t sl_done = 0;
+    queue_chunk();
+    while (true) {
+      queue_chunk();
+      const uint32_t sl = waited & 1u;
+      NGUO_HIP(hipEventSynchronize(ev[sl]));
+      ++waited;
+      bool all = true;
+      for (uint32_t a = 0; a < k && all; ++a) all = items[act[a]].pcg->h_flag_[4 * sl] != 0;
+      if (all) { sl_done = sl; break; }
+    }
+    for (uint32_t a = 0; a < k; ++a) {
+      PcgBatchItem &it = items[act[a]];
+      Pcg &P = *it.pcg;
+      const uint32_t *fl = P.h_flag_ + 4 * sl_done;
+      const uint32_t nb = it.M->nb, n3 = 3 * nb;
+      const unsigned G = div_up(nb, BLOCK), G3 = div_up(n3, BLOCK), GS = div_up(it.C.ns, BLOCK);
+      it.stats.iters += fl[1];
+      it.stats.sweeps += 1;
+      if (fl[0] == 2) it.stats.breakdown = true;
+      if (it.stats.breakdown && fl[1] == 0) done[act[a]] = 1;
+      hipLaunchKernelGGL(k_accum, dim3(G3), dim3(BLOCK), 0, s, n3, P.x64_.p, (const float *)P.d_.p);
+      if (it.C.ns)
+        hipLaunchKernelGGL((k_contact_sums<double, double>), dim3(GS), dim3(BLOCK), 0, s, it.C, (const double *)P.x64_.p, P.sums64_.p, nullptr);
+      hipLaunchKernelGGL(k_residual64, dim3(G), dim3(BLOCK), 0, s, it.M->view(), it.C, (const double *)P.sums64_.p, it.b, (const double *)P.x64_.p,
+                         P.r64_.p, P.part_.p);
+      hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(BLOCK), 0, s, (const double *)P.part_.p, G, P.scal_.p + 7);
+      NGUO_HIP(hipMemcpyAsync(P.h_scal_ + 7, P.scal_.p + 7, sizeof(double), hipMemcpyDeviceToHost, s));
+    }
+    NGUO_HIP(hipGetLastError());
+    NGUO_HIP(hipStreamSynchronize(s));
+    for (uint32_t a = 0; a < k; ++a) rr[act[a]] = items[act[a]].pcg->h_scal_[7];
+  }
+  for (uint32_t i = 0; i < n; ++i) {
+    PcgBatchItem &it = items[i];
+    const uint32_t n3 = 3 * it.M->nb;
+    hipLaunchKernelGGL(k_to_f32, dim3(div_up(n3, BLOCK)), dim3(BLOCK), 0, s, n3, (const double *)it.pcg->x64_.p, it.x_out);
+    it.stats.residual = bb[i] > 0.0 ? std::sqrt(rr[i] / bb[i]) : 0.0;
+  }
+  NGUO_HIP(hipGetLastError());
 }
 
 void Pcg::apply(const SellView &A, const ContactView &C, const float *x, float *y) {

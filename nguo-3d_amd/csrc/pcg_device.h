@@ -16,6 +16,49 @@
 
 namespace nguo {
 
+// Several systems in one launch: the workgroups of system i are [off[i], off[i + 1]) of the
+// grid, and a[i] holds its arguments (kernel-argument memory, read with scalar loads).
+constexpr uint32_t MAX_BATCH = 8;
+template <class ARGS> struct BatchOf {
+  ARGS a[MAX_BATCH];
+  uint32_t off[MAX_BATCH + 1];
+  uint32_t k;
+};
+__device__ inline uint32_t batch_slot(const uint32_t *off, uint32_t k, uint32_t b) {
+  uint32_t i = 0;
+  while (i + 1 < k && b >= off[i + 1]) ++i;
+  return i;
+}
+
+struct PreArgs {
+  ContactView C;
+  const float4 *z, *pold;
+  const double *cur;
+  uint32_t nz;
+  double *scal;
+  uint32_t *flag;
+};
+struct SpmvArgs {
+  SellView A;
+  ContactView C;
+  const float4 *z, *pold;
+  float4 *pnew, *q;
+  double *part;
+  const double *scal;
+  const uint32_t *flag;
+};
+struct SchArgs {
+  uint32_t nd;
+  const uint32_t *dom_ptr, *dom_dof;
+  const float *ainv;
+  float *r, *z, *dsol, *p;
+  const float *q;
+  const double *pap_part;
+  uint32_t npap;
+  double *nxt, *scal;
+  uint32_t *flag;
+};
+
 // Sum of K values over an NT-thread workgroup, broadcast to every thread. Fixed order.
 template <int NT, int K> __device__ inline void wg_sum_all(double (&v)[K], double *lds /* [NT / WAVE * K] */) {
   constexpr int NW = NT / WAVE;

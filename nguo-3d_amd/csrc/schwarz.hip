@@ -404,15 +404,15 @@ constexpr uint32_t SCH_KPER = (MD + SCH_KG - 1) / SCH_KG;  // rows per k group
 static_assert(SCH_KG * SCH_C4 <= (uint32_t)BLOCK, "k groups x float4 columns must fit the workgroup");
 
 template <bool INIT>
-__global__ __launch_bounds__(BLOCK) void k_schwarz_pcg(uint32_t nd, const uint32_t *__restrict__ dom_ptr, const uint32_t *__restrict__ dom_dof,
-                                                      const float *__restrict__ ainv, const double *__restrict__ r64, float *__restrict__ r,
-                                                      float *__restrict__ z, float *__restrict__ dsol, float *__restrict__ p,
-                                                      const float *__restrict__ q, const double *pap_part, uint32_t npap, double *nxt,
-                                                      double *scal, uint32_t *flag, uint32_t j, float4 *rc) {
+__device__ __forceinline__ void schwarz_pcg_body(uint32_t d, uint32_t nd, const uint32_t *__restrict__ dom_ptr, const uint32_t *__restrict__ dom_dof,
+                                                 const float *__restrict__ ainv, const double *__restrict__ r64, float *__restrict__ r,
+                                                 float *__restrict__ z, float *__restrict__ dsol, float *__restrict__ p,
+                                                 const float *__restrict__ q, const double *pap_part, uint32_t npap, double *nxt, double *scal,
+                                                 uint32_t *flag, uint32_t j, float4 *rc) {
   __shared__ float rs[MD];
   __shared__ float4 zp[SCH_KG][SCH_C4];
   __shared__ double lds[2 * BLOCK / WAVE];
-  const uint32_t d = blockIdx.x, t = threadIdx.x;
+  const uint32_t t = threadIdx.x;
   const uint32_t state = INIT ? 0u : flag[0];
   // loads that need no scalar: the inverse rows of this thread's k group (rows < MD of the
   // domain's block: in bounds whatever m is; rows >= m are masked at the FMA), the domain
@@ -447,12 +447,12 @@ __global__ __launch_bounds__(BLOCK) void k_schwarz_pcg(uint32_t nd, const uint32
     double v[1];
     sum_parts_all<BLOCK, 1>(src, n, v, lds);
     if (!(v[0] > 0.0)) {
-      if (blockIdx.x == 0 && threadIdx.x == 0) flag[0] = 2;
+      if (d == 0 && threadIdx.x == 0) flag[0] = 2;
       return;
     }
     const double a = scal[8 + (j & 1u)] / v[0];
     alpha = (float)a;
-    if (blockIdx.x == 0 && threadIdx.x == 0) { scal[1] = v[0]; scal[2] = a; }
+    if (d == 0 && threadIdx.x == 0) { scal[1] = v[0]; scal[2] = a; }
   }
   float rv = 0.0f;
   if (t < m) {
@@ -499,6 +499,28 @@ __global__ __launch_bounds__(BLOCK) void k_schwarz_pcg(uint32_t nd, const uint32
   }
   wg_sum_all<BLOCK, 2>(v, lds);
   if (t == 0) { nxt[d] = v[0]; nxt[nd + d] = v[1]; }
+}
+
+template <bool INIT>
+__global__ __launch_bounds__(BLOCK) void k_schwarz_pcg(uint32_t nd, const uint32_t *__restrict__ dom_ptr, const uint32_t *__restrict__ dom_dof,
+                                                      const float *__restrict__ ainv, const double *__restrict__ r64, float *__restrict__ r,
+                                                      float *__restrict__ z, float *__restrict__ dsol, float *__restrict__ p,
+                                                      const float *__restrict__ q, const double *pap_part, uint32_t npap, double *nxt,
+                                                      double *scal, uint32_t *flag, uint32_t j, float4 *rc) {
+  schwarz_pcg_body<INIT>(blockIdx.x, nd, dom_ptr, dom_dof, ainv, r64, r, z, dsol, p, q, pap_part, npap, nxt, scal, flag, j, rc);
+}
+
+// Iteration form (not INIT, one level) over several systems.
+__global__ __launch_bounds__(BLOCK) void kb_schwarz_pcg(BatchOf<SchArgs> B, uint32_t j) {
+  const uint32_t i = batch_slot(B.off, B.k, blockIdx.x);
+  const SchArgs &a = B.a[i];
+  schwarz_pcg_body<false>(blockIdx.x - B.off[i], a.nd, a.dom_ptr, a.dom_dof, a.ainv, nullptr, a.r, a.z, a.dsol, a.p, a.q, a.pap_part, a.npap,
+                          a.nxt, a.scal, a.flag, j, nullptr);
+}
+
+void launch_schwarz_pcg_batch(hipStream_t s, const BatchOf<SchArgs> &B, uint32_t j) {
+  if (!B.k || !B.off[B.k]) return;
+  hipLaunchKernelGGL(kb_schwarz_pcg, dim3(B.off[B.k]), dim3(BLOCK), 0, s, B, j);
 }
 
 void launch_schwarz_pcg(hipStream_t s, const Schwarz &sz, bool init, const double *r64, float *r, float *z, float *dsol, float *p,

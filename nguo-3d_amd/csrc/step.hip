@@ -15,8 +15,11 @@
 //      widening search radius (implicit.rs:1834-1943).
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
+#include <exception>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <thread>
@@ -858,6 +861,8 @@ struct ElemSet {
   }
 };
 
+struct SolveGate;
+
 struct nguo_step {
   nguo_ctx *ctx = nullptr;
   hipStream_t s = nullptr;
@@ -907,12 +912,96 @@ struct nguo_step {
   bool adj_valid = false;  // the last step can be differentiated (cleared by set_state / set_rest_scale)
   std::vector<float> trace_toi;
   bool own_stream = false;
+  // batch membership: Newton solves go through the batch's gate (null = solved alone)
+  SolveGate *gate = nullptr;
+  hipEvent_t gate_ev = nullptr;  // this step's assembly, recorded before it joins a batched solve
   ~nguo_step() {
+    if (gate_ev) (void)hipEventDestroy(gate_ev);
     if (own_stream && s) (void)hipStreamSynchronize(s);
     if (hf) (void)hipHostFree(hf);
     if (hu) (void)hipHostFree(hu);
     if (hl) (void)hipHostFree(hl);
     if (own_stream && s) (void)hipStreamDestroy(s);
+  }
+};
+
+// Batched Newton solves of several step objects advancing on their own threads: each thread
+// that reaches its linear solve deposits the system and waits; when every member thread has
+// deposited (or left because its Newton loop ended), the last one to arrive solves all of them
+// together (Pcg::solve_batch, at most MAX_BATCH per call) on the batch stream and releases
+// the others. Stream order: the batch stream waits for each system's assembly event; each
+// member stream waits for the batch's completion event.
+struct SolveGate {
+  std::mutex m;
+  std::condition_variable cv;
+  uint32_t members = 0, arrived = 0;
+  uint64_t gen = 0;
+  hipStream_t s = nullptr;
+  hipEvent_t done = nullptr;
+  std::vector<PcgBatchItem> items;
+  std::vector<nguo_step *> owners;
+  std::vector<PcgStats> results;           // by owner, valid after release
+  std::vector<std::exception_ptr> errors;  // by owner
+  ~SolveGate() {
+    if (done) (void)hipEventDestroy(done);
+    if (s) {
+      (void)hipStreamSynchronize(s);
+      (void)hipStreamDestroy(s);
+    }
+  }
+  void join() {
+    std::lock_guard<std::mutex> lk(m);
+    ++members;
+  }
+  void leave() {
+    std::unique_lock<std::mutex> lk(m);
+    --members;
+    if (arrived > 0 && arrived == members) run();
+  }
+  PcgStats solve(nguo_step *st, const SellMatrix &M, const ContactView &C, const float *b, float tol, uint32_t max_iter, float *x) {
+    if (!st->gate_ev) NGUO_HIP(hipEventCreateWithFlags(&st->gate_ev, hipEventDisableTiming));
+    NGUO_HIP(hipEventRecord(st->gate_ev, st->s));
+    std::unique_lock<std::mutex> lk(m);
+    const uint32_t me = (uint32_t)owners.size();
+    items.push_back(PcgBatchItem{st->pcg.get(), &M, C, b, tol, max_iter, x, PcgStats{}});
+    owners.push_back(st);
+    ++arrived;
+    const uint64_t my = gen;
+    if (arrived == members) run();
+    else cv.wait(lk, [&] { return gen != my; });
+    if (errors[me]) std::rethrow_exception(errors[me]);
+    return results[me];
+  }
+  // With the lock held and every member arrived: solve the deposited systems, release.
+  void run() {
+    const size_t n = items.size();
+    results.assign(n, PcgStats{});
+    errors.assign(n, nullptr);
+    try {
+      if (!s) NGUO_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+      if (!done) NGUO_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+      bool batch = true;
+      for (auto &it : items) batch = batch && it.pcg->batchable();
+      if (batch) {
+        for (auto *st : owners) NGUO_HIP(hipStreamWaitEvent(s, st->gate_ev, 0));
+        for (size_t o = 0; o < n; o += MAX_BATCH) Pcg::solve_batch(s, items.data() + o, (uint32_t)std::min<size_t>(MAX_BATCH, n - o));
+        NGUO_HIP(hipEventRecord(done, s));
+        for (auto *st : owners) NGUO_HIP(hipStreamWaitEvent(st->s, done, 0));
+        for (size_t i = 0; i < n; ++i) results[i] = items[i].stats;
+      } else {
+        for (size_t i = 0; i < n; ++i) {
+          PcgBatchItem &it = items[i];
+          results[i] = it.pcg->solve(*it.M, it.C, it.b, it.tol, it.max_iter, it.x_out);
+        }
+      }
+    } catch (...) {
+      for (auto &e : errors) e = std::current_exception();
+    }
+    items.clear();
+    owners.clear();
+    arrived = 0;
+    ++gen;
+    cv.notify_all();
   }
 };
 
@@ -1275,14 +1364,24 @@ void assemble_and_solve(nguo_step *st, nguo_stats &stats) {
                        (const uint32_t *)st->inc.p, (const float *)st->cgrad.p, st->ns > 0 ? 1 : 0, st->rhs.p);
   NGUO_HIP(hipGetLastError());
   if (st->NF == 0) { stats = nguo_stats{}; return; }
-  PcgStats ps = st->pcg->solve(st->act.M, contact_view(st), st->rhs.p, st->prm.cg_tol, st->prm.cg_max_iter, st->dxf.p);
+  PcgStats ps = st->gate ? st->gate->solve(st, st->act.M, contact_view(st), st->rhs.p, st->prm.cg_tol, st->prm.cg_max_iter, st->dxf.p)
+                         : st->pcg->solve(st->act.M, contact_view(st), st->rhs.p, st->prm.cg_tol, st->prm.cg_max_iter, st->dxf.p);
   stats.iters = ps.iters;
   stats.sweeps = ps.sweeps;
   stats.residual = ps.residual;
 }
 
+// A batch member for the duration of one step's Newton loop (leaves on every exit path).
+struct GateMembership {
+  SolveGate *g;
+  explicit GateMembership(SolveGate *gate) : g(gate) { if (g) g->join(); }
+  void leave() { if (g) { g->leave(); g = nullptr; } }
+  ~GateMembership() { leave(); }
+};
+
 void advance_one(nguo_step *st) {
   hipStream_t s = st->s;
+  GateMembership member(st->gate);
   const uint32_t V = st->V;
   const unsigned GV = div_up(V, BLOCK);
   float h = st->prm.dt;
@@ -1324,6 +1423,7 @@ void advance_one(nguo_step *st) {
     if (final_step) break;
     if (st->prm.target_toi > 0.0f && t >= st->prm.target_toi && taken > st->prm.min_newton_steps) final_step = true;
   }
+  member.leave();
   NGUO_HIP(hipMemsetAsync(st->flag.p, 0, 2 * sizeof(uint32_t), s));
   hipLaunchKernelGGL(k_finite, dim3(GV), dim3(BLOCK), 0, s, V, (const float *)st->x.p, st->flag.p);
   hipLaunchKernelGGL(k_commit, dim3(GV), dim3(BLOCK), 0, s, V, (const float *)st->im.p, st->x.p, st->x0.p, st->v.p, 1.0f / h, st->prm.floor_y,
@@ -1882,6 +1982,7 @@ int nguo_step_bench_spmv(nguo_step *st, uint32_t reps, int iteration_form, doubl
 struct nguo_batch {
   nguo_ctx *ctx = nullptr;
   std::vector<nguo_step *> inst;
+  SolveGate gate;
   ~nguo_batch() {
     for (auto *st : inst) delete st;
   }
@@ -1905,10 +2006,13 @@ int nguo_batch_create(nguo_ctx *ctx, uint32_t n, const nguo_scene_desc *scenes, 
 
 void nguo_batch_destroy(nguo_batch *b) { delete b; }
 
-// Every instance advances on its own stream from its own host thread: the instances' kernels
-// and their host-side convergence reads interleave on the device. Each instance runs exactly
-// the single-scene step (per-instance Newton loop, tolerance, CCD step), so its trajectory is
-// the one nguo_step_advance gives.
+// Every instance advances on its own stream from its own host thread (assembly, contacts,
+// CCD and update interleave on the device). With nguo_batch_set_solve(b, 1) the instances'
+// Newton solves meet in the batch's gate, which runs them as one batched PCG (three launches
+// per iteration for all of them); by default each solves on its own stream. Each instance runs
+// exactly the single-scene step (per-instance Newton loop, tolerance, CCD step) and its solves
+// take the same iterations and values either way, so its trajectory is the one
+// nguo_step_advance gives.
 int nguo_batch_advance(nguo_batch *b, uint32_t nsteps) {
   return guarded([&]() -> int {
     NGUO_REQUIRE(b, NGUO_E_INVALID, "nguo_batch_advance: null handle");
@@ -1936,6 +2040,12 @@ int nguo_batch_advance(nguo_batch *b, uint32_t nsteps) {
       if (code[i] != NGUO_OK) return fail(code[i], "nguo_batch_advance: instance " + std::to_string(i) + ": " + msg[i]);
     return NGUO_OK;
   });
+}
+
+int nguo_batch_set_solve(nguo_batch *b, int batched) {
+  if (!b || (batched != 0 && batched != 1)) return fail(NGUO_E_INVALID, "nguo_batch_set_solve: bad argument");
+  for (auto *st : b->inst) st->gate = batched ? &b->gate : nullptr;
+  return NGUO_OK;
 }
 
 int nguo_batch_instance(nguo_batch *b, uint32_t i, nguo_step **out) {

@@ -30,7 +30,7 @@ EXPORTED = [
     "nguo_lbvh_morton", "nguo_lbvh_sort_pairs", "nguo_lbvh_karras", "nguo_lbvh_aabb",
     "nguo_step_bench_spmv", "nguo_bench_spmv_grid", "nguo_ls_precond_apply",
     "nguo_batch_create", "nguo_batch_destroy", "nguo_batch_advance", "nguo_batch_instance",
-    "nguo_step_set_rest_scale", "nguo_step_adjoint",
+    "nguo_step_set_rest_scale", "nguo_step_adjoint", "nguo_batch_set_solve",
 ]
 
 
@@ -136,6 +136,7 @@ def load():
         lib.nguo_batch_instance.argtypes = [vp, C.c_uint32, P(vp)]
         lib.nguo_step_bench_spmv.argtypes = [vp, C.c_uint32, C.c_int, P(C.c_double), P(C.c_double)]
         lib.nguo_step_set_rest_scale.argtypes = [vp, P(C.c_float), C.c_uint32]
+        lib.nguo_batch_set_solve.argtypes = [vp, C.c_int]
         lib.nguo_step_adjoint.argtypes = [vp, P(C.c_float), C.c_float, C.c_int, P(C.c_float), P(Stats)]
         lib.nguo_bench_spmv_grid.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32, P(C.c_double), P(C.c_double), P(C.c_double)]
     _lib = lib
@@ -419,6 +420,10 @@ class Batch:
 
     def advance(self, steps=1):
         check(load().nguo_batch_advance(self.h, int(steps)))
+
+    def set_solve(self, batched):
+        """True: the instances' Newton solves run as one batched PCG; False: each alone."""
+        check(load().nguo_batch_set_solve(self.h, 1 if batched else 0))
 
     def close(self):
         if self.h:

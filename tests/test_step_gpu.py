@@ -158,11 +158,14 @@ def test_tube_garment_step_parity_from_device_state(ctx):
 
 
 @pytest.mark.gpu
-def test_batch_matches_independent_steppers(ctx):
-    # graded sizes advanced concurrently (nguo_batch_*): every instance takes exactly the
-    # steps a lone nguo_step takes, bit for bit
-    scenes = [nguo_hip.Scene(100, seed=0), nguo_hip.Scene(101, seed=101), nguo_hip.Scene(100, seed=5)]
+@pytest.mark.parametrize("batched,nscenes", [(False, 3), (True, 3), (True, 10)])
+def test_batch_matches_independent_steppers(ctx, batched, nscenes):
+    # graded sizes advanced concurrently (nguo_batch_*), their Newton solves each on its own
+    # stream or as one batched PCG (10 instances: two batched launches of at most 8): every
+    # instance takes exactly the steps a lone nguo_step takes, bit for bit
+    scenes = [nguo_hip.Scene(100, seed=0), nguo_hip.Scene(101, seed=101)] + [nguo_hip.Scene(100, seed=5 + k) for k in range(nscenes - 2)]
     b = nguo_hip.Batch(ctx, scenes)
+    b.set_solve(batched)
     b.advance(4)
     for sc, bst in zip(scenes, b.steppers):
         st = nguo_hip.Stepper(ctx, sc)
