@@ -40,8 +40,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps timed for the CPU baseline (0 = skip)")
     ap.add_argument("--grid", type=int, default=2048, help="side of the synthetic SpMV grid (0 = skip)")
+    ap.add_argument("--sort-n", type=int, default=1 << 26, help="pairs of the synthetic radix-sort measurement (0 = skip)")
     ap.add_argument("--c3-steps", type=int, default=3, help="timed steps of the 8-size graded batch vs sequential (0 = skip)")
     ap.add_argument("--c3-warmup", type=int, default=10)
+    ap.add_argument("--c3-solve", choices=["both", "batched", "concurrent"], default="both",
+                    help="batch solve modes to time (a single mode also skips the sequential runs)")
     ap.add_argument("--c4-iters", type=int, default=20, help="timed gradient-descent iterations of the drape adjoint (0 = skip)")
     ap.add_argument("--c4-hessian", choices=["exact", "newton"], default="newton",
                     help="adjoint Hessian: exact elastic second derivatives, or the Newton iteration's projected blocks")
@@ -109,6 +112,13 @@ def main():
                 "unit": "GB/s", "frac": round(g_ach / HBM_PEAK_GBS, 4), "avg_launch_us": round(g_ms * 1000, 2),
                 "bytes_per_launch": g_bytes}
 
+    sort = None
+    if args.sort_n:
+        s_ms, s_bytes = nh.bench_sort(ctx, args.sort_n, reps=10)
+        s_ach = s_bytes / (s_ms * 1e-3) / 1e9
+        sort = {"workload": f"4-pass LSD radix sort of {args.sort_n} random (u32 key, u32 index) pairs", "avg_sort_ms": round(s_ms, 3),
+                "bytes_per_sort": s_bytes, "achieved": round(s_ach, 1), "unit": "GB/s", "frac": round(s_ach / HBM_PEAK_GBS, 4)}
+
     c3 = None
     if args.c3_steps > 0:
         # graded sizes: circumference factors 0.85 .. 1.20 of the tube garment, each on its
@@ -116,7 +126,7 @@ def main():
         factors = [0.85, 0.90, 0.95, 1.00, 1.05, 1.10, 1.15, 1.20]
         scenes = [nh.Scene(3, seed=3, param=f) for f in factors]
         seq_ms = []
-        for sc in scenes:
+        for sc in (scenes if args.c3_solve == "both" else []):
             s1 = nh.Stepper(ctx, sc)
             s1.advance(args.c3_warmup)
             sync()
@@ -126,7 +136,8 @@ def main():
             seq_ms.append(1000.0 * (time.perf_counter() - t1) / args.c3_steps)
             s1.close()
         b_ms = {}
-        for mode in (True, False):
+        modes = {"both": (True, False), "batched": (True,), "concurrent": (False,)}[args.c3_solve]
+        for mode in modes:
             batch = nh.Batch(ctx, scenes)
             batch.set_solve(mode)
             batch.advance(args.c3_warmup)
@@ -140,9 +151,10 @@ def main():
         best = min(b_ms.values())
         c3 = {"workload": f"8 graded tubes, circumference x{factors[0]}..x{factors[-1]}, {min(free)}..{max(free)} free vertices "
                           f"({sum(free)} total), steps {args.c3_warmup + 1}..{args.c3_warmup + args.c3_steps}",
-              "batched_ms_per_step": round(best, 3), "sequential_ms_per_step": round(sum(seq_ms), 3),
-              "per_size_ms": [round(v, 3) for v in seq_ms], "speedup": round(sum(seq_ms) / best, 3),
-              "batched_pcg_ms_per_step": round(b_ms[True], 3), "concurrent_solves_ms_per_step": round(b_ms[False], 3),
+              "batched_ms_per_step": round(best, 3), "sequential_ms_per_step": round(sum(seq_ms), 3) if seq_ms else None,
+              "per_size_ms": [round(v, 3) for v in seq_ms], "speedup": round(sum(seq_ms) / best, 3) if seq_ms else None,
+              "batched_pcg_ms_per_step": round(b_ms[True], 3) if True in b_ms else None,
+              "concurrent_solves_ms_per_step": round(b_ms[False], 3) if False in b_ms else None,
               "form": "one stream and host thread per instance; Newton solves either as one batched PCG over all instances "
                       "(three launches per iteration) or each on its instance stream"}
 
@@ -239,6 +251,7 @@ def main():
                   "mean_stencils": sum(stencils) / len(stencils), "max_reresid": res},
         "roofline": roofline,
         "spmv_hbm": grid,
+        "sort_hbm": sort,
         "batched_c3": c3,
         "adjoint_c4": c4,
         "cpu_baseline": cpu,

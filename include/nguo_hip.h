@@ -225,6 +225,10 @@ int nguo_batch_set_solve(nguo_batch *batch, int batched);
  * evaluates and stores p = z + beta p_old, reduces the previous partials),
  * *bytes = nnzb*40 + nb*52 + 4. */
 int nguo_step_bench_spmv(nguo_step *st, uint32_t reps, int iteration_form, double *avg_ms, double *bytes);
+/* A full 4-pass LSD radix sort of n random (u32 key, u32 index) pairs on the device (the
+ * LBVH's sort), event-timed per sort; *bytes = 80 n (per pass: 4 B histogram read, 16 B
+ * scatter read + write per pair). */
+int nguo_bench_sort(nguo_ctx *ctx, uint32_t n, uint32_t reps, double *avg_ms, double *bytes);
 /* The same operator on a synthetic nx x ny cloth-grid BSR (13 blocks per interior row). */
 int nguo_bench_spmv_grid(nguo_ctx *ctx, uint32_t nx, uint32_t ny, uint32_t reps, double *avg_ms, double *bytes, double *nnzb);
 

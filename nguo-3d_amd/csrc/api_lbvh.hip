@@ -1,6 +1,7 @@
 // C ABI of the LBVH stages on a triangle set, so each stage can be checked bit for bit:
 // engine-gpu/src/morton.rs:61-230 (GpuMorton::codes), radix.rs:112-135 (sort_pairs),
 // karras.rs:91-262 (GpuKarras::build_tree), bvh_aabb.rs (node AABB propagation).
+#include "device_common.h"
 #include "lbvh.h"
 
 using namespace nguo;
@@ -40,6 +41,54 @@ int nguo_lbvh_sort_pairs(nguo_ctx *ctx, uint32_t n, uint32_t *keys, uint32_t *va
     k.download(keys, n, s);
     v.download(vals, n, s);
     NGUO_HIP(hipStreamSynchronize(s));
+    return NGUO_OK;
+  });
+}
+
+// Random 32-bit keys (splitmix64 of the index and seed) with index payloads, generated on the
+// device; each timed repetition restores the unsorted keys with a device copy (excluded from
+// the timing by events around the sort alone).
+__global__ void k_bench_keys(uint32_t n, uint64_t seed, uint32_t *keys, uint32_t *vals) {
+  uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= n) return;
+  uint64_t z = seed + 0x9e3779b97f4a7c15ull * (uint64_t)(i + 1);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  z ^= z >> 31;
+  keys[i] = (uint32_t)z;
+  vals[i] = i;
+}
+
+int nguo_bench_sort(nguo_ctx *ctx, uint32_t n, uint32_t reps, double *avg_ms, double *bytes) {
+  return guarded([&]() -> int {
+    NGUO_REQUIRE(ctx && avg_ms && bytes && n > 1 && reps > 0, NGUO_E_INVALID, "nguo_bench_sort: bad argument");
+    hipStream_t s = ctx->stream;
+    DevBuf<uint32_t> k0, v0, k, v;
+    k0.reserve(n); v0.reserve(n); k.reserve(n); v.reserve(n);
+    hipLaunchKernelGGL(k_bench_keys, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, s, n, (uint64_t)7, k0.p, v0.p);
+    NGUO_HIP(hipGetLastError());
+    RadixSort rs;
+    hipEvent_t e0, e1;
+    NGUO_HIP(hipEventCreate(&e0));
+    NGUO_HIP(hipEventCreate(&e1));
+    double total = 0.0;
+    for (uint32_t r = 0; r <= reps; ++r) {  // repetition 0 warms up
+      NGUO_HIP(hipMemcpyAsync(k.p, k0.p, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+      NGUO_HIP(hipMemcpyAsync(v.p, v0.p, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+      NGUO_HIP(hipEventRecord(e0, s));
+      rs.sort_u32(n, k.p, v.p, {0, 8, 16, 24}, s);
+      NGUO_HIP(hipEventRecord(e1, s));
+      NGUO_HIP(hipEventSynchronize(e1));
+      float ms = 0.0f;
+      NGUO_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (r) total += ms;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_ms = total / reps;
+    // per 8-bit pass: histogram reads the keys (4 B), the scatter reads key + payload and
+    // writes them (16 B)
+    *bytes = 4.0 * 20.0 * (double)n;
     return NGUO_OK;
   });
 }

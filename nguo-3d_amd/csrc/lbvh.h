@@ -25,7 +25,7 @@ class RadixSort {
   void sort_u64(uint32_t n, uint64_t *keys, uint32_t *vals, const std::vector<int> &shifts, hipStream_t s);
 
  private:
-  DevBuf<uint32_t> k32_, v_, hist_;
+  DevBuf<uint32_t> k32_, v_, hist_, sums_;
   DevBuf<uint64_t> k64_;
 };
 

@@ -7,16 +7,25 @@
 
 namespace nguo {
 
-constexpr uint32_t TILE = 1024;  // items per radix workgroup (4 chunks of 256)
-
 // ---------------------------------------------------------------- radix sort
-template <class K>
+// LSD radix sort, 8-bit digits, stable. A pass over tiles of ITEMS x 256 items:
+//   k_radix_hist     per-tile digit counts, stored digit-major (hist[d * tiles + tile]);
+//   scan             exclusive scan of those counters = each tile's first output slot per
+//                    digit (one workgroup for small inputs, three launches beyond that);
+//   k_radix_scatter  ranks every item among the equal digits before it in the tile (chunks of
+//                    256 in order, 8-ballot wavefront multisplit), stages the tile in LDS in
+//                    (digit, rank) order, then writes it out so that consecutive threads write
+//                    consecutive slots of one digit's run.
+constexpr uint32_t SMALL_ITEMS = 4, LARGE_ITEMS = 16;  // items per thread: tiles of 1024 / 4096
+constexpr uint32_t LARGE_SORT = 1u << 20;              // inputs from this size use the large tile
+
+template <class K, uint32_t ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_radix_hist(uint32_t n, const K *__restrict__ keys, int shift, uint32_t *hist, uint32_t nblocks) {
   __shared__ uint32_t cnt[256];
   cnt[threadIdx.x] = 0;
   __syncthreads();
-  uint32_t base = blockIdx.x * TILE;
-  for (uint32_t c = 0; c < TILE / BLOCK; ++c) {
+  const uint32_t base = blockIdx.x * ITEMS * BLOCK;
+  for (uint32_t c = 0; c < ITEMS; ++c) {
     uint32_t i = base + c * BLOCK + threadIdx.x;
     if (i < n) atomicAdd(&cnt[(uint32_t)(keys[i] >> shift) & 255u], 1u);
   }
@@ -42,60 +51,144 @@ __global__ __launch_bounds__(BLOCK) void k_scan_excl(uint32_t total, uint32_t *a
   for (uint32_t i = lo; i < hi; ++i) { uint32_t v = a[i]; a[i] = acc; acc += v; }
 }
 
-// Stable scatter: rank of an item among equal digits = items of that digit in earlier
-// chunks + in earlier wavefronts of this chunk + earlier lanes (8-ballot multisplit).
-template <class K>
-__global__ __launch_bounds__(BLOCK) void k_radix_scatter(uint32_t n, const K *__restrict__ kin, const uint32_t *__restrict__ vin,
-                                                        K *__restrict__ kout, uint32_t *__restrict__ vout, int shift,
-                                                        const uint32_t *__restrict__ hist, uint32_t nblocks) {
-  __shared__ uint32_t base[256];
-  __shared__ uint32_t wcnt[4][256];
-  const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
-  base[tid] = hist[(size_t)tid * nblocks + blockIdx.x];
-  const unsigned long long lt = (1ull << lane) - 1ull;
-  for (uint32_t c = 0; c < TILE / BLOCK; ++c) {
-    for (int q = 0; q < 4; ++q) wcnt[q][tid] = 0;
-    __syncthreads();
-    uint32_t i = blockIdx.x * TILE + c * BLOCK + tid;
-    bool valid = i < n;
-    K key = valid ? kin[i] : K(0);
-    uint32_t val = valid ? vin[i] : 0u;
-    uint32_t digit = (uint32_t)(key >> shift) & 255u;
-    unsigned long long mask = __ballot(valid);
-    for (int b = 0; b < 8; ++b) {
-      bool bit = (digit >> b) & 1u;
-      unsigned long long bal = __ballot(bit);
-      mask &= bit ? bal : ~bal;
-    }
-    uint32_t rank = (uint32_t)__popcll(mask & lt);
-    if (valid && rank == 0) wcnt[w][digit] = (uint32_t)__popcll(mask);
-    __syncthreads();
-    {
-      uint32_t s = base[tid];
-      for (int q = 0; q < 4; ++q) { uint32_t t = wcnt[q][tid]; wcnt[q][tid] = s; s += t; }
-      base[tid] = s;
-    }
-    __syncthreads();
-    if (valid) {
-      uint32_t pos = wcnt[w][digit] + rank;
-      kout[pos] = key;
-      vout[pos] = val;
-    }
-    __syncthreads();
+// Exclusive scan of one value per thread over the workgroup (fixed order); returns the
+// prefix, *total receives the workgroup sum.
+__device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t *wsum /* [BLOCK / WAVE] */, uint32_t *total) {
+  const uint32_t lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < WAVE; o <<= 1) {
+    uint32_t y = __shfl_up(x, o, WAVE);
+    if (lane >= (uint32_t)o) x += y;
+  }
+  if (lane == WAVE - 1) wsum[w] = x;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+  for (uint32_t q = 0; q < BLOCK / WAVE; ++q) {
+    if (q < w) before += wsum[q];
+    all += wsum[q];
+  }
+  __syncthreads();
+  *total = all;
+  return before + x - v;
+}
+
+// Large scans: each workgroup scans SCAN_TILE counters in place (exclusive, local) and
+// writes its sum; the sums are scanned by one workgroup; each workgroup adds its offset.
+constexpr uint32_t SCAN_TILE = 16 * BLOCK;
+__global__ __launch_bounds__(BLOCK) void k_scan_tiles(uint32_t total, uint32_t *a, uint32_t *sums) {
+  __shared__ uint32_t wsum[BLOCK / WAVE];
+  const uint32_t base = blockIdx.x * SCAN_TILE;
+  uint32_t run = 0;
+  for (uint32_t c = 0; c < SCAN_TILE / BLOCK; ++c) {
+    const uint32_t i = base + c * BLOCK + threadIdx.x;
+    const uint32_t v = i < total ? a[i] : 0u;
+    uint32_t t;
+    const uint32_t e = block_excl_scan(v, wsum, &t);
+    if (i < total) a[i] = run + e;
+    run += t;
+  }
+  if (threadIdx.x == 0) sums[blockIdx.x] = run;
+}
+__global__ __launch_bounds__(BLOCK) void k_scan_add(uint32_t total, uint32_t *a, const uint32_t *sums) {
+  const uint32_t off = sums[blockIdx.x], base = blockIdx.x * SCAN_TILE;
+  for (uint32_t c = 0; c < SCAN_TILE / BLOCK; ++c) {
+    const uint32_t i = base + c * BLOCK + threadIdx.x;
+    if (i < total) a[i] += off;
   }
 }
 
-template <class K>
-static void radix_sort(uint32_t n, K *keys, uint32_t *vals, K *ktmp, uint32_t *vtmp, uint32_t *hist, const std::vector<int> &shifts,
-                       hipStream_t s) {
-  if (n <= 1) return;
-  uint32_t nblocks = div_up(n, TILE);
+template <class K, uint32_t ITEMS>
+__global__ __launch_bounds__(BLOCK) void k_radix_scatter(uint32_t n, const K *__restrict__ kin, const uint32_t *__restrict__ vin,
+                                                        K *__restrict__ kout, uint32_t *__restrict__ vout, int shift,
+                                                        const uint32_t *__restrict__ hist, uint32_t nblocks) {
+  constexpr uint32_t T = ITEMS * BLOCK;
+  __shared__ K sk[T];
+  __shared__ uint32_t sv[T];
+  __shared__ uint32_t gbase[256], lstart[256], dcount[256];
+  __shared__ uint32_t wcnt[4][256];
+  __shared__ uint32_t wsum[BLOCK / WAVE];
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+  const uint32_t tile0 = blockIdx.x * T;
+  gbase[tid] = hist[(size_t)tid * nblocks + blockIdx.x];
+  dcount[tid] = 0;
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  K key[ITEMS];
+  uint32_t val[ITEMS], loc[ITEMS];
+#pragma unroll
+  for (uint32_t c = 0; c < ITEMS; ++c) {
+    const uint32_t i = tile0 + c * BLOCK + tid;
+    const bool valid = i < n;
+    key[c] = valid ? kin[i] : K(0);
+    val[c] = valid ? vin[i] : 0u;
+  }
+  for (uint32_t c = 0; c < ITEMS; ++c) {
+    for (int q = 0; q < 4; ++q) wcnt[q][tid] = 0;
+    __syncthreads();
+    const bool valid = tile0 + c * BLOCK + tid < n;
+    const uint32_t digit = (uint32_t)(key[c] >> shift) & 255u;
+    unsigned long long mask = __ballot(valid);
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (digit >> b) & 1u;
+      const unsigned long long bal = __ballot(bit);
+      mask &= bit ? bal : ~bal;
+    }
+    const uint32_t rank = (uint32_t)__popcll(mask & lt);
+    if (valid && rank == 0) wcnt[w][digit] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    {
+      uint32_t sacc = dcount[tid];
+      for (int q = 0; q < 4; ++q) { uint32_t t = wcnt[q][tid]; wcnt[q][tid] = sacc; sacc += t; }
+      dcount[tid] = sacc;
+    }
+    __syncthreads();
+    loc[c] = valid ? wcnt[w][digit] + rank : 0xffffffffu;
+    __syncthreads();  // every loc read before the next chunk clears wcnt
+  }
+  // tile-local start of every digit's run
+  uint32_t tot;
+  lstart[tid] = block_excl_scan(dcount[tid], wsum, &tot);
+  __syncthreads();
+#pragma unroll
+  for (uint32_t c = 0; c < ITEMS; ++c) {
+    if (loc[c] != 0xffffffffu) {
+      const uint32_t p = lstart[(uint32_t)(key[c] >> shift) & 255u] + loc[c];
+      sk[p] = key[c];
+      sv[p] = val[c];
+    }
+  }
+  __syncthreads();
+  const uint32_t cnt = min(T, n - tile0);
+#pragma unroll
+  for (uint32_t c = 0; c < ITEMS; ++c) {
+    const uint32_t p = c * BLOCK + tid;
+    if (p < cnt) {
+      const K k = sk[p];
+      const uint32_t d = (uint32_t)(k >> shift) & 255u;
+      const uint32_t g = gbase[d] + (p - lstart[d]);
+      kout[g] = k;
+      vout[g] = sv[p];
+    }
+  }
+}
+
+template <class K, uint32_t ITEMS>
+static void radix_passes(uint32_t n, K *keys, uint32_t *vals, K *ktmp, uint32_t *vtmp, uint32_t *hist, uint32_t *sums,
+                         const std::vector<int> &shifts, hipStream_t s) {
+  const uint32_t nblocks = div_up(n, ITEMS * BLOCK), total = 256u * nblocks;
   K *ka = keys, *kb = ktmp;
   uint32_t *va = vals, *vb = vtmp;
   for (int sh : shifts) {
-    hipLaunchKernelGGL((k_radix_hist<K>), dim3(nblocks), dim3(BLOCK), 0, s, n, (const K *)ka, sh, hist, nblocks);
-    hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(BLOCK), 0, s, 256u * nblocks, hist);
-    hipLaunchKernelGGL((k_radix_scatter<K>), dim3(nblocks), dim3(BLOCK), 0, s, n, (const K *)ka, (const uint32_t *)va, kb, vb, sh,
+    hipLaunchKernelGGL((k_radix_hist<K, ITEMS>), dim3(nblocks), dim3(BLOCK), 0, s, n, (const K *)ka, sh, hist, nblocks);
+    if (total <= 16384) {
+      hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(BLOCK), 0, s, total, hist);
+    } else {
+      const uint32_t nt = div_up(total, SCAN_TILE);
+      hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(BLOCK), 0, s, total, hist, sums);
+      hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(BLOCK), 0, s, nt, sums);
+      hipLaunchKernelGGL(k_scan_add, dim3(nt), dim3(BLOCK), 0, s, total, hist, (const uint32_t *)sums);
+    }
+    hipLaunchKernelGGL((k_radix_scatter<K, ITEMS>), dim3(nblocks), dim3(BLOCK), 0, s, n, (const K *)ka, (const uint32_t *)va, kb, vb, sh,
                        (const uint32_t *)hist, nblocks);
     std::swap(ka, kb);
     std::swap(va, vb);
@@ -107,20 +200,30 @@ static void radix_sort(uint32_t n, K *keys, uint32_t *vals, K *ktmp, uint32_t *v
   NGUO_HIP(hipGetLastError());
 }
 
+template <class K>
+static void radix_sort(uint32_t n, K *keys, uint32_t *vals, K *ktmp, uint32_t *vtmp, DevBuf<uint32_t> &hist, DevBuf<uint32_t> &sums,
+                       const std::vector<int> &shifts, hipStream_t s) {
+  if (n <= 1) return;
+  const uint32_t items = n >= LARGE_SORT ? LARGE_ITEMS : SMALL_ITEMS;
+  const uint32_t total = 256u * div_up(n, items * BLOCK);
+  hist.reserve(total);
+  sums.reserve(div_up(total, SCAN_TILE) + 1);
+  if (items == LARGE_ITEMS) radix_passes<K, LARGE_ITEMS>(n, keys, vals, ktmp, vtmp, hist.p, sums.p, shifts, s);
+  else radix_passes<K, SMALL_ITEMS>(n, keys, vals, ktmp, vtmp, hist.p, sums.p, shifts, s);
+}
+
 void RadixSort::sort_u32(uint32_t n, uint32_t *keys, uint32_t *vals, const std::vector<int> &shifts, hipStream_t s) {
   if (n <= 1) return;
   k32_.reserve(n);
   v_.reserve(n);
-  hist_.reserve(256 * (size_t)div_up(n, TILE));
-  radix_sort<uint32_t>(n, keys, vals, k32_.p, v_.p, hist_.p, shifts, s);
+  radix_sort<uint32_t>(n, keys, vals, k32_.p, v_.p, hist_, sums_, shifts, s);
 }
 
 void RadixSort::sort_u64(uint32_t n, uint64_t *keys, uint32_t *vals, const std::vector<int> &shifts, hipStream_t s) {
   if (n <= 1) return;
   k64_.reserve(n);
   v_.reserve(n);
-  hist_.reserve(256 * (size_t)div_up(n, TILE));
-  radix_sort<uint64_t>(n, keys, vals, k64_.p, v_.p, hist_.p, shifts, s);
+  radix_sort<uint64_t>(n, keys, vals, k64_.p, v_.p, hist_, sums_, shifts, s);
 }
 
 // ---------------------------------------------------------------- Morton

@@ -30,7 +30,7 @@ EXPORTED = [
     "nguo_lbvh_morton", "nguo_lbvh_sort_pairs", "nguo_lbvh_karras", "nguo_lbvh_aabb",
     "nguo_step_bench_spmv", "nguo_bench_spmv_grid", "nguo_ls_precond_apply",
     "nguo_batch_create", "nguo_batch_destroy", "nguo_batch_advance", "nguo_batch_instance",
-    "nguo_step_set_rest_scale", "nguo_step_adjoint", "nguo_batch_set_solve",
+    "nguo_step_set_rest_scale", "nguo_step_adjoint", "nguo_batch_set_solve", "nguo_bench_sort",
 ]
 
 
@@ -137,6 +137,7 @@ def load():
         lib.nguo_step_bench_spmv.argtypes = [vp, C.c_uint32, C.c_int, P(C.c_double), P(C.c_double)]
         lib.nguo_step_set_rest_scale.argtypes = [vp, P(C.c_float), C.c_uint32]
         lib.nguo_batch_set_solve.argtypes = [vp, C.c_int]
+        lib.nguo_bench_sort.argtypes = [vp, C.c_uint32, C.c_uint32, P(C.c_double), P(C.c_double)]
         lib.nguo_step_adjoint.argtypes = [vp, P(C.c_float), C.c_float, C.c_int, P(C.c_float), P(Stats)]
         lib.nguo_bench_spmv_grid.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_uint32, P(C.c_double), P(C.c_double), P(C.c_double)]
     _lib = lib
@@ -279,6 +280,13 @@ def bench_spmv_grid(ctx, nx, ny, reps=20):
     ms, by, nnzb = C.c_double(), C.c_double(), C.c_double()
     check(load().nguo_bench_spmv_grid(ctx.h, nx, ny, reps, C.byref(ms), C.byref(by), C.byref(nnzb)))
     return ms.value, by.value, nnzb.value
+
+
+def bench_sort(ctx, n, reps=10):
+    """(average ms, algorithmic bytes) of one 4-pass radix sort of n random pairs."""
+    ms, by = C.c_double(), C.c_double()
+    check(load().nguo_bench_sort(ctx.h, int(n), int(reps), C.byref(ms), C.byref(by)))
+    return ms.value, by.value
 
 
 def drape_panels(scene, n_around=8, n_height=8, axes=None):

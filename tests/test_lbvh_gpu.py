@@ -43,9 +43,15 @@ def test_morton_sort_karras_aabb_bitexact(ctx, cfg):
 @pytest.mark.gpu
 def test_sort_pairs_stability_and_edges(ctx):
     rng = np.random.default_rng(11)
-    for n in (1, 2, 1023, 1024, 1025, 70001):
-        keys = rng.integers(0, 37, n).astype(np.uint32) * np.uint32(0x01010101)
+    # 1 .. 70 001 use the 1024-item tile (70 001: the multi-workgroup counter scan); from 2^20
+    # the 4096-item tile; heavy duplication and full-range keys, ragged last tiles
+    for n, full in ((1, False), (2, False), (1023, False), (1024, False), (1025, False), (70001, False), (70001, True),
+                    (1 << 20, True), (3000001, False), (3000001, True)):
+        if full:
+            keys = rng.integers(0, 1 << 32, n, dtype=np.uint64).astype(np.uint32)
+        else:
+            keys = rng.integers(0, 37, n).astype(np.uint32) * np.uint32(0x01010101)
         vals = np.arange(n, dtype=np.uint32)
         k, v = nguo_hip.lbvh_sort_pairs(ctx, keys, vals)
         order = np.argsort(keys, kind="stable")
-        assert np.array_equal(k, keys[order]) and np.array_equal(v, vals[order])
+        assert np.array_equal(k, keys[order]) and np.array_equal(v, vals[order]), (n, full)
