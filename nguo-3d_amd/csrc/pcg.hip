@@ -484,6 +484,10 @@ __global__ __launch_bounds__(BLOCK) void k_minv(SellView A, ContactView C, const
 }
 
 // ---------------------------------------------------------------- host driver
+// A refinement sweep asks the FP32 iteration for a reduction of its own (recursive) residual
+// by eta = clamp(0.25 tol / res, REFINE_ETA_MIN, 0.5) relative to the sweep's start.
+constexpr double REFINE_ETA_MIN = 1.0e-5;
+
 Pcg::~Pcg() {
   for (auto &e : ev_)
     if (e) (void)hipEventDestroy(e);
@@ -588,7 +592,7 @@ PcgStats Pcg::solve(const SellMatrix &M, const ContactView &C, const float *b, f
   for (uint32_t sweep = 0; sweep < 12; ++sweep) {
     double res = std::sqrt(rr / bb);
     if (res <= (double)tol || st.iters >= max_iter) break;
-    double eta = std::min(0.5, std::max(0.25 * (double)tol / res, 1.0e-5));
+    double eta = std::min(0.5, std::max(0.25 * (double)tol / res, REFINE_ETA_MIN));
     launch_precond_init(M, part_pc);
     hipLaunchKernelGGL(k_init_scal, dim3(1), dim3(BLOCK), 0, s_, scal_.p, flag_.p, (const double *)part_pc, nz, eta * eta, max_iter - st.iters);
     // Chunks of `chunk` iterations are queued one ahead of the one whose state the host
@@ -712,7 +716,7 @@ void Pcg::solve_batch(hipStream_t s, PcgBatchItem *items, uint32_t n) {
       Pcg &P = *it.pcg;
       const uint32_t nb = it.M->nb, NS = div_up(nb, 64), nd = P.sz_->nd;
       const double res = std::sqrt(rr[act[a]] / bb[act[a]]);
-      const double eta = std::min(0.5, std::max(0.25 * (double)it.tol / res, 1.0e-5));
+      const double eta = std::min(0.5, std::max(0.25 * (double)it.tol / res, REFINE_ETA_MIN));
       double *part_spmv = P.part_.p, *part_pc = P.part_.p + std::max<uint32_t>(NS, div_up(nb, BLOCK));
       P.launch_precond_init(*it.M, part_pc);
       hipLaunchKernelGGL(k_init_scal, dim3(1), dim3(BLOCK), 0, s, P.scal_.p, P.flag_.p, (const double *)part_pc, nd, eta * eta,
