@@ -40,6 +40,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps timed for the CPU baseline (0 = skip)")
     ap.add_argument("--grid", type=int, default=2048, help="side of the synthetic SpMV grid (0 = skip)")
+    ap.add_argument("--c0-steps", type=int, default=36, help="steps of the sheet-on-sphere drape timed as a second plain workload (0 = skip)")
     ap.add_argument("--sort-n", type=int, default=1 << 26, help="pairs of the synthetic radix-sort measurement (0 = skip)")
     ap.add_argument("--c3-steps", type=int, default=3, help="timed steps of the 8-size graded batch vs sequential (0 = skip)")
     ap.add_argument("--c3-warmup", type=int, default=10)
@@ -111,6 +112,24 @@ def main():
         grid = {"workload": f"{args.grid}x{args.grid} cloth-grid BSR, {int(g_nnzb)} blocks", "achieved": round(g_ach, 1),
                 "unit": "GB/s", "frac": round(g_ach / HBM_PEAK_GBS, 4), "avg_launch_us": round(g_ms * 1000, 2),
                 "bytes_per_launch": g_bytes}
+
+    c0 = None
+    if args.c0_steps > 0:
+        # the reference's regression scene: 32 x 32 sheet dropped onto a static icosphere
+        sc0 = nh.Scene(0, seed=0)
+        st0 = nh.Stepper(ctx, sc0)
+        sync()
+        t1 = time.perf_counter()
+        it0 = []
+        for _ in range(args.c0_steps):
+            st0.advance(1)
+            it0.append(st0.metrics().max_cg_iters)
+        sync()
+        m0 = st0.metrics()
+        c0 = {"workload": f"32x32 sheet (1024 v, 1922 t) dropped onto a static 2562-vertex icosphere, steps 1..{args.c0_steps}",
+              "ms_per_step": round(1000.0 * (time.perf_counter() - t1) / args.c0_steps, 3), "mean_pcg_iters": sum(it0) / len(it0),
+              "penetration_free": int(m0.penetration_free), "all_finite": int(m0.all_finite)}
+        st0.close()
 
     sort = None
     if args.sort_n:
@@ -252,6 +271,7 @@ def main():
         "roofline": roofline,
         "spmv_hbm": grid,
         "sort_hbm": sort,
+        "plain_c0": c0,
         "batched_c3": c3,
         "adjoint_c4": c4,
         "cpu_baseline": cpu,

@@ -218,6 +218,14 @@ class OracleSolver:
         self.lib.oracle_solver_csr(self.h, up(rp), up(ci), fp(va), fp(rhs))
         return rp, ci, va, rhs
 
+    def kappas(self, x, inv_h2):
+        """Rayleigh contact stiffness of every active stencil at x (the stencils() order)."""
+        x = f32(x)
+        cap = 1 << 20
+        out = np.zeros(cap, np.float32)
+        self.lib.oracle_solver_assemble(self.h, fp(x), fp(x), inv_h2, fp(out), cap)
+        return out
+
     def x_tilde(self):
         out = np.zeros((self.n, 3), np.float32)
         self.lib.oracle_solver_x_tilde(self.h, fp(out))

@@ -61,6 +61,10 @@ def test_stencils_bitexact(ctx, cfg, steps):
     assert np.array_equal(gd, d)  # same FP32 operation order: bitwise equal gaps
     assert np.array_equal(gg, g)
     assert np.all(np.isfinite(gk)) and np.all(gk > 0)
+    # Rayleigh stiffness against the elastic blocks at x (implicit.rs:755-795): the blocks are
+    # summed in the reference's push order on both sides, so the stiffnesses agree bitwise
+    ok = o.kappas(x, 1.0 / (s.params.dt * s.params.dt))[: len(idx)]
+    assert np.array_equal(gk, ok)
     side = o.collider_sides(x)
     assert np.array_equal(st.collider_sides(x), side)
     st.close()
