@@ -82,10 +82,11 @@ def main():
     sync()
     x_w, v_w = st.positions(), st.velocities()  # start state of the timed steps (CPU sample starts here too)
     t0 = time.perf_counter()
-    iters, newton, stencils, res = [], [], [], 0.0
+    iters, newton, stencils, res, mlist = [], [], [], 0.0, []
     for _ in range(args.steps):
         st.advance(1)
         m = st.metrics()
+        mlist.append(m)
         iters.append(m.max_cg_iters)
         newton.append(m.newton_iters)
         stencils.append(m.num_stencils)
@@ -95,6 +96,8 @@ def main():
     elapsed = time.perf_counter() - t0
     ms = 1000.0 * elapsed / args.steps
     x_e, v_e = st.positions(), st.velocities()  # the drape after warmup + timed steps (start of the adjoint loop)
+    inv_ok, inv_checks = nh.check_invariants(mlist, scene.desc.params.cg_tol, scene.desc.params.cg_max_iter)
+    invariants = {"all_ok": inv_ok, "failed": [c[0] for c in inv_checks if not c[1]]}
 
     # the dominant kernel: the SpMV instance of the PCG iteration on the final state's system
     spmv_ms, spmv_bytes = st.bench_spmv(reps=500, iteration_form=True)
@@ -267,7 +270,8 @@ def main():
                    "dt": 1.0 / 60.0, "newton_iters": 4, "cg_tol": 1e-6, "device": name, "cus": cus,
                    "precond": {"schwarz_domain": int(scene.desc.params.schwarz_domain), "coarse_weight": float(scene.desc.params.schwarz_weight)}},
         "stats": {"mean_pcg_iters": sum(iters) / len(iters), "mean_newton": sum(newton) / len(newton),
-                  "mean_stencils": sum(stencils) / len(stencils), "max_reresid": res},
+                  "mean_stencils": sum(stencils) / len(stencils), "max_reresid": res,
+                  "invariants": invariants},
         "roofline": roofline,
         "spmv_hbm": grid,
         "sort_hbm": sort,

@@ -126,6 +126,11 @@ typedef struct nguo_step_metrics {
   int32_t penetration_free; /* 1 = no edge pierces a triangle and all finite */
   int32_t all_finite;
   float last_toi;           /* CCD step fraction of the last Newton iteration */
+  uint32_t num_contacts;    /* free vertices within dhat of an analytic collider (implicit.rs:2212-2230) */
+  float max_sigma;          /* max stretch ratio over edge-spring constraints (implicit.rs:2194-2209);
+                               0: the scene description carries none */
+  uint32_t device_alloc;    /* device allocations made during the step (0 once capacities settle) */
+  uint32_t device_free;
 } nguo_step_metrics;
 
 typedef struct nguo_ctx nguo_ctx;

@@ -18,6 +18,9 @@ namespace nguo {
 //                    consecutive slots of one digit's run.
 constexpr uint32_t SMALL_ITEMS = 4, LARGE_ITEMS = 16;  // items per thread: tiles of 1024 / 4096
 constexpr uint32_t LARGE_SORT = 1u << 20;              // inputs from this size use the large tile
+// Scratch for at least this many items from the first sort on: the per-step sorts (stencil
+// and incidence keys) change size every step.
+constexpr uint32_t SORT_MIN_CAP = 1u << 14;
 
 template <class K, uint32_t ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_radix_hist(uint32_t n, const K *__restrict__ keys, int shift, uint32_t *hist, uint32_t nblocks) {
@@ -206,7 +209,7 @@ static void radix_sort(uint32_t n, K *keys, uint32_t *vals, K *ktmp, uint32_t *v
   if (n <= 1) return;
   const uint32_t items = n >= LARGE_SORT ? LARGE_ITEMS : SMALL_ITEMS;
   const uint32_t total = 256u * div_up(n, items * BLOCK);
-  hist.reserve(total);
+  hist.reserve(std::max<size_t>(total, 256u * div_up(SORT_MIN_CAP, items * BLOCK)));
   sums.reserve(div_up(total, SCAN_TILE) + 1);
   if (items == LARGE_ITEMS) radix_passes<K, LARGE_ITEMS>(n, keys, vals, ktmp, vtmp, hist.p, sums.p, shifts, s);
   else radix_passes<K, SMALL_ITEMS>(n, keys, vals, ktmp, vtmp, hist.p, sums.p, shifts, s);
@@ -214,15 +217,15 @@ static void radix_sort(uint32_t n, K *keys, uint32_t *vals, K *ktmp, uint32_t *v
 
 void RadixSort::sort_u32(uint32_t n, uint32_t *keys, uint32_t *vals, const std::vector<int> &shifts, hipStream_t s) {
   if (n <= 1) return;
-  k32_.reserve(n);
-  v_.reserve(n);
+  k32_.reserve(std::max(n, SORT_MIN_CAP));
+  v_.reserve(std::max(n, SORT_MIN_CAP));
   radix_sort<uint32_t>(n, keys, vals, k32_.p, v_.p, hist_, sums_, shifts, s);
 }
 
 void RadixSort::sort_u64(uint32_t n, uint64_t *keys, uint32_t *vals, const std::vector<int> &shifts, hipStream_t s) {
   if (n <= 1) return;
-  k64_.reserve(n);
-  v_.reserve(n);
+  k64_.reserve(std::max(n, SORT_MIN_CAP));
+  v_.reserve(std::max(n, SORT_MIN_CAP));
   radix_sort<uint64_t>(n, keys, vals, k64_.p, v_.p, hist_, sums_, shifts, s);
 }
 

@@ -2,6 +2,7 @@
 // reference's GpuError / Result<(), String> split, engine-gpu/src/lib.rs:58-66), HIP call
 // checking and an owning device buffer.
 #pragma once
+#include <algorithm>
 #include <cstddef>
 #include <cstdint>
 #include <string>
@@ -52,6 +53,10 @@ template <class F> int guarded(F &&body) {
 }
 
 // Owning device allocation of T[n] (hipMalloc / hipFree). Grows, never shrinks.
+// Device allocations and frees made by DevBuf on the calling thread (the per-step
+// device_alloc / device_free metrics).
+inline thread_local uint64_t g_dev_allocs = 0, g_dev_frees = 0;
+
 template <class T> struct DevBuf {
   T *p = nullptr;
   size_t n = 0;
@@ -60,15 +65,23 @@ template <class T> struct DevBuf {
   DevBuf &operator=(const DevBuf &) = delete;
   ~DevBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      (void)hipFree(p);
+      ++g_dev_frees;
+    }
     p = nullptr;
     n = 0;
   }
+  // Grows to at least `count` elements (contents are not kept). A buffer that grows again
+  // takes 1.5x headroom, so sizes that creep up step by step (contact lists, sort keys)
+  // stop reallocating after a few steps.
   void reserve(size_t count) {
     if (count <= n) return;
+    if (p) count = std::max(count, n + n / 2);
     release();
     size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
     hipError_t e = hipMalloc((void **)&p, bytes);
+    ++g_dev_allocs;
     if (e != hipSuccess) {
       p = nullptr;
       throw HipError{NGUO_E_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e)};

@@ -518,8 +518,9 @@ void Pcg::ensure(uint32_t nb, uint32_t ns) {
   scal_.reserve(16);
   flag_.reserve(4);
   if (ns > cap_ns_) {
-    sums64_.reserve(4 * (size_t)ns);
-    cap_ns_ = ns;
+    const uint32_t c = std::max(ns, 1u << 14);  // contact stencils: sized like the step's contact lists
+    sums64_.reserve(4 * (size_t)c);
+    cap_ns_ = c;
   }
 }
 

@@ -808,6 +808,26 @@ __global__ __launch_bounds__(BLOCK) void k_commit(uint32_t V, const float *im, f
   if (threadIdx.x == 0) part[blockIdx.x] = fmaxf(fmaxf(lds[0], lds[1]), fmaxf(lds[2], lds[3]));
 }
 
+// implicit.rs:2212-2230 count_active_contacts and the analytic-collider part of
+// penetration_free (implicit.rs:2235-2266), per free vertex: out[0] counts the vertices within
+// dhat of a collider, out[1] flags one more than 1e-3 max(dhat, 1e-6) inside one.
+__global__ __launch_bounds__(BLOCK) void k_collider_metrics(uint32_t V, const float *im, const float *x, Params P, uint32_t *out) {
+  uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= V || !(im[i] > 0.0f)) return;
+  const f3 p = ld3(x, i);
+  const float tol = -1.0e-3f * fmaxf(P.dhat, 1.0e-6f);
+  bool near = false, pen = false;
+  for (uint32_t k = 0; k < P.ncoll; ++k) {
+    float d;
+    f3 n;
+    collider_distance(P.coll[k], p, d, n);
+    near = near || d < P.dhat;
+    pen = pen || d < tol;
+  }
+  if (near) atomicAdd(out, 1u);
+  if (pen) atomicOr(out + 1, 1u);
+}
+
 // implicit.rs:618-650 positions_self_intersect: any non-inert, non-adjacent (edge, triangle)
 // with a transverse pierce. Each edge queries both triangle trees (movable edges) or the
 // dynamic one (static edges).
@@ -1381,6 +1401,7 @@ struct GateMembership {
 
 void advance_one(nguo_step *st) {
   hipStream_t s = st->s;
+  const uint64_t allocs0 = g_dev_allocs, frees0 = g_dev_frees;
   GateMembership member(st->gate);
   const uint32_t V = st->V;
   const unsigned GV = div_up(V, BLOCK);
@@ -1432,14 +1453,22 @@ void advance_one(nguo_step *st) {
   uint32_t bad = read_u32(st, st->flag.p);
   st->time += (double)h;
   // penetration_free: finite, analytic colliders respected, no edge pierces a triangle
-  uint32_t hit = 0;
+  uint32_t hit = 0, ncontacts = 0;
+  if (st->P.ncoll && !bad) {
+    NGUO_HIP(hipMemsetAsync(st->flag.p + 2, 0, 2 * sizeof(uint32_t), s));
+    hipLaunchKernelGGL(k_collider_metrics, dim3(GV), dim3(BLOCK), 0, s, V, (const float *)st->im.p, (const float *)st->x.p, st->P, st->flag.p + 2);
+    NGUO_HIP(hipMemcpyAsync(st->hu + 4, st->flag.p + 2, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    NGUO_HIP(hipStreamSynchronize(s));
+    ncontacts = st->hu[4];
+    hit |= st->hu[5];
+  }
   if (st->prm.self_collision && !bad) {
     st->tree_tri_dyn.build(0, st->n_tri_dyn, st->tri_dyn_ids.p, st->tris.p, st->x.p, s);
     Topo T = topo(st);
     hipLaunchKernelGGL(k_pierce, dim3(div_up(st->E, BLOCK)), dim3(BLOCK), 0, s, T, st->tree_tri_dyn.view(), st->tree_tri_stat.view(), st->E,
                        (const float *)st->x.p, st->flag.p + 1, st->err.p);
     NGUO_HIP(hipGetLastError());
-    hit = read_u32(st, st->flag.p + 1);
+    hit |= read_u32(st, st->flag.p + 1);
   }
   nguo_step_metrics &m = st->metrics;
   m.time = st->time;
@@ -1451,6 +1480,10 @@ void advance_one(nguo_step *st) {
   m.all_finite = bad ? 0 : 1;
   m.penetration_free = (!bad && !hit) ? 1 : 0;
   m.last_toi = last_t;
+  m.num_contacts = ncontacts;
+  m.max_sigma = 0.0f;  // no edge-spring constraints in the scene description
+  m.device_alloc = (uint32_t)(g_dev_allocs - allocs0);
+  m.device_free = (uint32_t)(g_dev_frees - frees0);
   st->adj_valid = !bad;
 }
 

@@ -73,6 +73,7 @@ class StepMetrics(C.Structure):
         ("time", C.c_double), ("max_reresid", C.c_double), ("max_cg_iters", C.c_uint32),
         ("newton_iters", C.c_uint32), ("num_stencils", C.c_uint32), ("max_velocity", C.c_float),
         ("penetration_free", C.c_int32), ("all_finite", C.c_int32), ("last_toi", C.c_float),
+        ("num_contacts", C.c_uint32), ("max_sigma", C.c_float), ("device_alloc", C.c_uint32), ("device_free", C.c_uint32),
     ]
 
 
@@ -287,6 +288,33 @@ def bench_sort(ctx, n, reps=10):
     ms, by = C.c_double(), C.c_double()
     check(load().nguo_bench_sort(ctx.h, int(n), int(reps), C.byref(ms), C.byref(by)))
     return ms.value, by.value
+
+
+def check_invariants(steps, cg_tol, cg_max_iter, strain_limit=0.0):
+    """The solver invariants over a run's per-step metrics, as engine-core/src/metrics.rs:84-183
+    check_invariants evaluates them: returns (all_ok, [(name, ok, detail), ...]) in the
+    reference's order. `steps` holds StepMetrics (or objects with the same fields)."""
+    checks = []
+    bad = sum(1 for m in steps if not m.penetration_free)
+    checks.append(("penetration_free", bad == 0, f"{bad} step(s) with penetration/CCD failure"))
+    nonfin = sum(1 for m in steps if not m.all_finite)
+    checks.append(("all_frames_finite", nonfin == 0, f"{nonfin} non-finite step(s)"))
+    if strain_limit <= 0.0:
+        checks.append(("strain_limit_respected", True, "strain limiting disabled (auto-pass)"))
+    else:
+        peak = max([0.0] + [float(m.max_sigma) for m in steps])
+        ceiling = (1.0 + strain_limit) * 1.02
+        checks.append(("strain_limit_respected", peak <= ceiling, f"peak sigma {peak:.4f} vs ceiling {ceiling:.4f}"))
+    worst = max([0.0] + [float(m.max_reresid) for m in steps])
+    tol = float(cg_tol) * 1.5
+    checks.append(("cg_converged", worst <= tol, f"worst reresid {worst:.3e} vs {tol:.3e}"))
+    max_it = max([0] + [int(m.max_cg_iters) for m in steps])
+    checks.append(("cg_iter_under_cap", max_it < cg_max_iter, f"max cg iters {max_it} vs cap {cg_max_iter}"))
+    half = steps[len(steps) // 2:]
+    ta = sum(int(m.device_alloc) for m in half)
+    tf = sum(int(m.device_free) for m in half)
+    checks.append(("zero_steady_state_alloc", ta == 0 and tf == 0, f"steady-state device alloc {ta} free {tf}"))
+    return all(c[1] for c in checks), checks
 
 
 def drape_panels(scene, n_around=8, n_height=8, axes=None):

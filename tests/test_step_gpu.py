@@ -132,11 +132,18 @@ def test_one_step_parity_production_tolerance(ctx, cfg, steps):
 @pytest.mark.gpu
 def test_sheet_drape_invariants_36_steps(ctx):
     s, st = make(ctx, 0)
+    steps = []
     for k in range(36):
         st.advance(1)
         m = st.metrics()
+        steps.append(m)
         assert m.all_finite == 1 and m.penetration_free == 1, k
         assert m.max_reresid <= 1e-6
+    # the reference's acceptance gate (metrics.rs:84-183) over the device metrics, including
+    # no device allocation once the capacities have settled
+    ok, checks = nguo_hip.check_invariants(steps, s.params.cg_tol, s.params.cg_max_iter)
+    assert ok, checks
+    assert sum(m.device_alloc for m in steps) > 0  # the first steps do allocate
     x = st.positions()
     free = s.inv_mass > 0
     assert x[free, 1].min() < 0.0  # draped over the sphere
