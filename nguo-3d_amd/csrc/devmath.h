@@ -245,7 +245,9 @@ __device__ inline bool barrier_of(int kind, double d, double dhat, double kappa,
 }
 
 // contact.rs:378-431 accd_toi in FP64. kind 0: point (x0[0]) vs triangle, 1: edge vs edge.
-__device__ inline double accd_toi(const d3 x0[4], const d3 dx[4], int kind, double offset, double max_t, double reduction, uint32_t max_iter) {
+// *capped (may be null) is set when the iteration limit ended the advancement.
+__device__ inline double accd_toi(const d3 x0[4], const d3 dx[4], int kind, double offset, double max_t, double reduction, uint32_t max_iter,
+                                  bool *capped = nullptr) {
   double u_max = 0;
   for (int i = 0; i < 4; ++i)
     for (int j = i + 1; j < 4; ++j) u_max = fmax(u_max, len(dx[i] - dx[j]));
@@ -260,7 +262,8 @@ __device__ inline double accd_toi(const d3 x0[4], const d3 dx[4], int kind, doub
   if (eps <= 0.0) return max_t;
   double target = eps + offset, eps_sqr = eps * eps, toi = 0;
   d3 moved[4];
-  for (uint32_t it = 0; it < max_iter; ++it) {
+  uint32_t it = 0;
+  for (; it < max_iter; ++it) {
     for (int k = 0; k < 4; ++k) moved[k] = x0[k] + dx[k] * toi;
     double d2 = fmax(sqd(moved), 0.0);
     double dmt = (d2 - target * target) / (sqrt(d2) + target);
@@ -270,6 +273,7 @@ __device__ inline double accd_toi(const d3 x0[4], const d3 dx[4], int kind, doub
     if (tn != toi) toi = tn; else break;
     if (toi > max_t) return max_t;
   }
+  if (capped && it == max_iter) *capped = true;
   return fmax(toi, 0.0);
 }
 

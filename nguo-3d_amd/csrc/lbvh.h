@@ -40,14 +40,20 @@ struct BvhView {
   // [4 (n-1)] traversal nodes: both child boxes and child refs of internal node i in one
   // 64-byte record (lo_l.xyz hi_l.x | hi_l.yz lo_r.xy | lo_r.z hi_r.xyz | left right - -)
   const float4 *trav;
+  // [2 TRAV_PARTS] the tree split into disjoint subtrees that together hold every leaf once:
+  // entry k = (lo.xyz, ref) (hi.xyz, 0), ref ~0u for an unused entry
+  const float4 *front;
 };
+
+// Subtrees a traversal query is split into (one thread each).
+constexpr uint32_t TRAV_PARTS = 8;
 
 class Lbvh {
  public:
   // kind 0: triangles (3 indices per primitive), 1: edges (2 indices). `ids` selects the
   // primitives of this tree (device array of primitive indices into `idx`).
   void build(int kind, uint32_t n, const uint32_t *ids, const uint32_t *idx, const float *pos, hipStream_t s);
-  BvhView view() const { return BvhView{n_, left_.p, right_.p, node_box_.p, leaf_box_.p, prim_.p, (const float4 *)trav_.p}; }
+  BvhView view() const { return BvhView{n_, left_.p, right_.p, node_box_.p, leaf_box_.p, prim_.p, (const float4 *)trav_.p, (const float4 *)front_.p}; }
   uint32_t size() const { return n_; }
   // Stage outputs for parity checks.
   const uint32_t *codes() const { return keys_.p; }   // sorted Morton keys
@@ -56,7 +62,7 @@ class Lbvh {
  private:
   uint32_t n_ = 0;
   DevBuf<uint32_t> keys_, prim_, left_, right_, range_, visit_;
-  DevBuf<float> leaf_box_, node_box_, bounds_, cent_, table_, trav_;
+  DevBuf<float> leaf_box_, node_box_, bounds_, cent_, table_, trav_, front_;
 };
 
 // Kernels exposed for the stage-by-stage C ABI test entry points.
@@ -70,22 +76,24 @@ void aabb_build(uint32_t n, const uint32_t *left, const uint32_t *right, uint32_
 void aabb_from_ranges(uint32_t n, const uint32_t *range, const float *leaf_box, float *node_box, DevBuf<float> &table, hipStream_t s);
 
 // Stack traversal: calls visit(prim) for every leaf whose box passes `overlap(box_lo,
-// box_hi)` (a conservative test). A stack deeper than TRAV_STACK sets *err (reported to the
-// caller as NGUO_E_CAPACITY) instead of silently dropping a subtree.
+// box_hi)` (a conservative test). The first TRAV_LDS stack entries of a thread live in LDS
+// (`lds`: the calling kernel's __shared__ uint32_t[TRAV_LDS * BLOCK], interleaved by thread so
+// that a wavefront's pushes and pops fall in distinct banks); deeper entries go to a private
+// array. A stack deeper than TRAV_STACK sets *err (reported to the caller as
+// NGUO_E_CAPACITY) instead of silently dropping a subtree. Requires BLOCK threads per
+// workgroup.
 constexpr int TRAV_STACK = 96;
+constexpr int TRAV_LDS = 32;
+// Depth-first traversal of the subtree under internal node `root`.
 template <class Overlap, class Visit>
-__device__ inline void bvh_traverse(const BvhView &t, uint32_t *err, Overlap overlap, Visit visit) {
-  if (t.n == 0) return;
-  if (t.n == 1) {
-    const float *b = t.leaf_box;
-    if (overlap(f3{b[0], b[1], b[2]}, f3{b[3], b[4], b[5]})) visit(t.prim[0]);
-    return;
-  }
-  uint32_t stack[TRAV_STACK];
-  int sp = 0;
-  stack[sp++] = 0;  // root
+__device__ inline void bvh_traverse_from(const BvhView &t, uint32_t root, uint32_t *err, uint32_t *lds, Overlap overlap, Visit visit) {
+  uint32_t spill[TRAV_STACK - TRAV_LDS];
+  uint32_t *mine = lds + threadIdx.x;
+  mine[0] = root;
+  int sp = 1;
   while (sp > 0) {
-    const uint32_t node = stack[--sp];
+    --sp;
+    const uint32_t node = sp < TRAV_LDS ? mine[sp * BLOCK] : spill[sp - TRAV_LDS];
     const float4 *rec = t.trav + 4 * (size_t)node;
     const float4 q0 = rec[0], q1 = rec[1], q2 = rec[2], q3 = rec[3];
     const uint32_t ch[2] = {__float_as_uint(q3.x), __float_as_uint(q3.y)};
@@ -96,13 +104,41 @@ __device__ inline void bvh_traverse(const BvhView &t, uint32_t *err, Overlap ove
       if (!overlap(lo[c], hi[c])) continue;
       if (r & LEAF) {
         visit(t.prim[r & ~LEAF]);
+      } else if (sp < TRAV_LDS) {
+        mine[sp * BLOCK] = r;
+        ++sp;
       } else if (sp < TRAV_STACK) {
-        stack[sp++] = r;
+        spill[sp - TRAV_LDS] = r;
+        ++sp;
       } else {
         atomicOr(err, 1u);
       }
     }
   }
+}
+
+template <class Overlap, class Visit>
+__device__ inline void bvh_traverse(const BvhView &t, uint32_t *err, uint32_t *lds, Overlap overlap, Visit visit) {
+  if (t.n == 0) return;
+  if (t.n == 1) {
+    const float *b = t.leaf_box;
+    if (overlap(f3{b[0], b[1], b[2]}, f3{b[3], b[4], b[5]})) visit(t.prim[0]);
+    return;
+  }
+  bvh_traverse_from(t, 0u, err, lds, overlap, visit);
+}
+
+// The part `part` (< TRAV_PARTS) of a traversal split over the tree's frontier subtrees: the
+// parts of one query visit disjoint leaf sets whose union is what bvh_traverse visits.
+template <class Overlap, class Visit>
+__device__ inline void bvh_traverse_part(const BvhView &t, uint32_t part, uint32_t *err, uint32_t *lds, Overlap overlap, Visit visit) {
+  if (t.n == 0) return;
+  const float4 a = t.front[2 * part], b = t.front[2 * part + 1];
+  const uint32_t r = __float_as_uint(a.w);
+  if (r == 0xffffffffu) return;
+  if (!overlap(f3{a.x, a.y, a.z}, f3{b.x, b.y, b.z})) return;
+  if (r & LEAF) visit(t.prim[r & ~LEAF]);
+  else bvh_traverse_from(t, r, err, lds, overlap, visit);
 }
 
 }  // namespace nguo

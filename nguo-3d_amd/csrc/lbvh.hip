@@ -483,6 +483,50 @@ __global__ __launch_bounds__(BLOCK) void k_pack_trav(uint32_t n, const uint32_t 
   o[3] = make_float4(__uint_as_float(ch[0]), __uint_as_float(ch[1]), 0.0f, 0.0f);
 }
 
+// The traversal frontier: starting from the root's children, internal entries are replaced by
+// their two children breadth first while there is room for TRAV_PARTS entries; leaves stay.
+// One thread, at most TRAV_PARTS - 2 record reads.
+__global__ void k_frontier(uint32_t n, const float4 *trav, const float *leaf_box, float4 *front) {
+  if (threadIdx.x != 0) return;
+  uint32_t ref[TRAV_PARTS];
+  float4 lo[TRAV_PARTS], hi[TRAV_PARTS];
+  uint32_t cnt = 0;
+  if (n == 1) {
+    ref[0] = LEAF;
+    lo[0] = make_float4(leaf_box[0], leaf_box[1], leaf_box[2], 0.0f);
+    hi[0] = make_float4(leaf_box[3], leaf_box[4], leaf_box[5], 0.0f);
+    cnt = 1;
+  } else if (n >= 2) {
+    auto children = [&](uint32_t node, uint32_t o0, uint32_t o1) {
+      const float4 *rec = trav + 4 * (size_t)node;
+      const float4 q0 = rec[0], q1 = rec[1], q2 = rec[2], q3 = rec[3];
+      ref[o0] = __float_as_uint(q3.x);
+      ref[o1] = __float_as_uint(q3.y);
+      lo[o0] = make_float4(q0.x, q0.y, q0.z, 0.0f); hi[o0] = make_float4(q0.w, q1.x, q1.y, 0.0f);
+      lo[o1] = make_float4(q1.z, q1.w, q2.x, 0.0f); hi[o1] = make_float4(q2.y, q2.z, q2.w, 0.0f);
+    };
+    children(0u, 0, 1);
+    cnt = 2;
+    bool grew = true;
+    while (grew && cnt < TRAV_PARTS) {
+      grew = false;
+      const uint32_t level = cnt;
+      for (uint32_t e = 0; e < level && cnt < TRAV_PARTS; ++e) {
+        if (ref[e] & LEAF) continue;
+        // entry e becomes the left child, the right child goes to the end
+        children(ref[e], e, cnt);
+        ++cnt;
+        grew = true;
+      }
+    }
+  }
+  for (uint32_t k = 0; k < TRAV_PARTS; ++k) {
+    const bool used = k < cnt;
+    front[2 * k] = used ? make_float4(lo[k].x, lo[k].y, lo[k].z, __uint_as_float(ref[k])) : make_float4(0.f, 0.f, 0.f, __uint_as_float(0xffffffffu));
+    front[2 * k + 1] = used ? hi[k] : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+}
+
 void aabb_build(uint32_t n, const uint32_t *left, const uint32_t *right, uint32_t *parent, uint32_t *visit, const float *leaf_box, float *node_box,
                 hipStream_t s) {
   if (n < 2) return;
@@ -526,6 +570,8 @@ void Lbvh::build(int kind, uint32_t n, const uint32_t *ids, const uint32_t *idx,
     hipLaunchKernelGGL(k_pack_trav, dim3(div_up(n - 1, BLOCK)), dim3(BLOCK), 0, s, n, (const uint32_t *)left_.p, (const uint32_t *)right_.p,
                        (const float *)leaf_box_.p, (const float *)node_box_.p, (float4 *)trav_.p);
   }
+  front_.reserve(8 * TRAV_PARTS);
+  hipLaunchKernelGGL(k_frontier, dim3(1), dim3(WAVE), 0, s, n, (const float4 *)trav_.p, (const float *)leaf_box_.p, (float4 *)front_.p);
   NGUO_HIP(hipGetLastError());
 }
 

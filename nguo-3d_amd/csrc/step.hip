@@ -33,6 +33,7 @@
 namespace nguo {
 
 constexpr uint32_t NONE = 0xffffffffu;
+constexpr uint32_t CCD_DEFER_CAP = 1u << 18;  // CCD pairs one pass can defer (more run inline)
 constexpr uint32_t C_DENSE = 0u, C_ROD_POS = 1u, C_ROD_NEG = 2u, C_HINGE = 3u;
 __host__ __device__ inline uint32_t code(uint32_t type, uint32_t payload) { return (type << 30) | payload; }
 
@@ -333,6 +334,7 @@ __device__ inline bool inside_padded(f3 p, f3 lo, f3 hi, float pad) {
 // implicit.rs:1431-1476 collider_sides: the side of each movable non-collider vertex from
 // its closest collider facet within dhat (ties: lowest facet index); +1 otherwise.
 __global__ __launch_bounds__(BLOCK) void k_sides(Topo T, BvhView coll, const float *__restrict__ x, float dhat, float *side, uint32_t *err) {
+  __shared__ uint32_t tstack[TRAV_LDS * BLOCK];
   uint32_t v = blockIdx.x * BLOCK + threadIdx.x;
   if (v >= T.V) return;
   float s = 1.0f;
@@ -341,7 +343,7 @@ __global__ __launch_bounds__(BLOCK) void k_sides(Topo T, BvhView coll, const flo
     float best = INFINITY;
     uint32_t best_t = NONE;
     float best_side = 1.0f;
-    bvh_traverse(coll, err, [&](f3 lo, f3 hi) { return inside_padded(p, lo, hi, dhat); },
+    bvh_traverse(coll, err, tstack, [&](f3 lo, f3 hi) { return inside_padded(p, lo, hi, dhat); },
                  [&](uint32_t t) {
                    uint32_t t0 = T.tris[3 * (size_t)t], t1 = T.tris[3 * (size_t)t + 1], t2 = T.tris[3 * (size_t)t + 2];
                    f3 a = ld3(x, t0), b = ld3(x, t1), c = ld3(x, t2);
@@ -385,6 +387,7 @@ __device__ inline void emit(const StencilOut &O, uint64_t key, float d, const f3
 // excluded, not both immovable, with (signed for collider facets) distance < dhat.
 __global__ __launch_bounds__(BLOCK) void k_vt_stencils(Topo T, BvhView dyn, BvhView stat, const float *__restrict__ x, float dhat,
                                                       StencilOut O, uint32_t *err) {
+  __shared__ uint32_t tstack[TRAV_LDS * BLOCK];
   uint32_t v = blockIdx.x * BLOCK + threadIdx.x;
   if (v >= T.V) return;
   f3 p = ld3(x, v);
@@ -406,14 +409,17 @@ __global__ __launch_bounds__(BLOCK) void k_vt_stencils(Topo T, BvhView dyn, BvhV
     emit(O, ((uint64_t)v << 32) | t, d, g);
   };
   auto ov = [&](f3 lo, f3 hi) { return inside_padded(p, lo, hi, dhat); };
-  bvh_traverse(dyn, err, ov, leaf);
-  if (mv) bvh_traverse(stat, err, ov, leaf);
+  // parts 0 .. TRAV_PARTS-1: the movable tree's subtrees; the rest: the static tree's
+  const uint32_t part = blockIdx.y;
+  if (part < TRAV_PARTS) bvh_traverse_part(dyn, part, err, tstack, ov, leaf);
+  else if (mv) bvh_traverse_part(stat, part - TRAV_PARTS, err, tstack, ov, leaf);
 }
 
 // implicit.rs:719-751 edge-edge stencils. Query edges are padded, tree edges are not; a
 // movable query pairs only with lower-indexed tree edges, a static query with any.
 __global__ __launch_bounds__(BLOCK) void k_ee_stencils(Topo T, BvhView dyn, const uint32_t *qids, uint32_t nq, int query_dyn,
                                                       const float *__restrict__ x, float pad, float ghat, StencilOut O, uint32_t *err) {
+  __shared__ uint32_t tstack[TRAV_LDS * BLOCK];
   uint32_t q = blockIdx.x * BLOCK + threadIdx.x;
   if (q >= nq) return;
   uint32_t ej = qids[q];
@@ -421,7 +427,7 @@ __global__ __launch_bounds__(BLOCK) void k_ee_stencils(Topo T, BvhView dyn, cons
   f3 pb = ld3(x, b0), qb = ld3(x, b1);
   f3 blo = vmin(pb, qb) - f3{pad, pad, pad}, bhi = vmax(pb, qb) + f3{pad, pad, pad};
   auto ov = [&](f3 lo, f3 hi) { return !(lo.x > bhi.x || lo.y > bhi.y || lo.z > bhi.z || hi.x < blo.x || hi.y < blo.y || hi.z < blo.z); };
-  bvh_traverse(dyn, err, ov, [&](uint32_t ei) {
+  bvh_traverse_part(dyn, blockIdx.y, err, tstack, ov, [&](uint32_t ei) {
     if (ei == ej || (query_dyn && ei > ej)) return;
     uint32_t a0 = T.edges[2 * (size_t)ei], a1 = T.edges[2 * (size_t)ei + 1];
     if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) return;
@@ -706,57 +712,139 @@ __global__ __launch_bounds__(BLOCK) void k_ccd_colliders(uint32_t V, const float
 }
 
 // Vertex-triangle ACCD over the padded-AABB candidates (implicit.rs:1856-1900), FP64 TOI.
+// A pair whose advancement is still running after ACCD_INLINE iterations is handed to
+// k_ccd_deferred (one pair per thread, full iteration limit, same arithmetic from the start,
+// so the same time of impact): a traversal thread never walks a long series of slow pairs.
+constexpr uint32_t ACCD_INLINE = 16;
+
+// Exclusion before the FP64 advancement: at its first iterate accd_toi returns its horizon
+// (no time of impact, so the pair cannot lower the step) when 1.25 u_max < 0.98 d0, with
+// d0 the pair's distance (eps = 0.01 d0). The gap between the two point sets' boxes is a lower
+// bound of d0 and the FP32 u_max an upper bound of the FP64 one once widened; the margins
+// cover the FP32 rounding, so a pair skipped here is one the FP64 test skips too.
+__device__ inline bool accd_surely_clear(const f3 *pa, int na, const f3 *pb, int nb, const f3 dxs[4]) {
+  f3 alo = pa[0], ahi = pa[0], blo = pb[0], bhi = pb[0];
+  for (int k = 1; k < na; ++k) { alo = vmin(alo, pa[k]); ahi = vmax(ahi, pa[k]); }
+  for (int k = 1; k < nb; ++k) { blo = vmin(blo, pb[k]); bhi = vmax(bhi, pb[k]); }
+  const float gx = fmaxf(fmaxf(blo.x - ahi.x, alo.x - bhi.x), 0.0f);
+  const float gy = fmaxf(fmaxf(blo.y - ahi.y, alo.y - bhi.y), 0.0f);
+  const float gz = fmaxf(fmaxf(blo.z - ahi.z, alo.z - bhi.z), 0.0f);
+  const float gap = sqrtf(gx * gx + gy * gy + gz * gz) * (1.0f - 1.0e-5f) - 1.0e-9f;
+  float u = 0.0f;
+  for (int i = 0; i < 4; ++i)
+    for (int j = i + 1; j < 4; ++j) u = fmaxf(u, len(dxs[i] - dxs[j]));
+  return 1.25f * (u * (1.0f + 1.0e-5f) + 1.0e-12f) < 0.98f * gap;
+}
+struct CcdDefer {
+  uint32_t *count;  // pairs appended
+  uint32_t *pairs;  // [2 cap]: (a | kind << 31, b): vertex-triangle (v, t) or edge-edge (lo, hi)
+  uint32_t cap;
+};
+__device__ inline bool defer_pair(const CcdDefer &D, uint32_t a, uint32_t b, uint32_t kind) {
+  const uint32_t k = atomicAdd(D.count, 1u);
+  if (k >= D.cap) return false;
+  D.pairs[2 * (size_t)k] = a | (kind << 31);
+  D.pairs[2 * (size_t)k + 1] = b;
+  return true;
+}
+
 __global__ __launch_bounds__(BLOCK) void k_ccd_vt(Topo T, BvhView dyn, BvhView stat, const float *__restrict__ x, const float *__restrict__ dx,
-                                                 float pad, uint32_t iters, unsigned long long *tmin, uint32_t *err) {
+                                                 float pad, uint32_t iters, CcdDefer D, unsigned long long *tmin, uint32_t *err) {
+  __shared__ uint32_t tstack[TRAV_LDS * BLOCK];
   uint32_t v = blockIdx.x * BLOCK + threadIdx.x;
-  if (v >= T.V) return;
-  f3 p = ld3(x, v);
-  bool mv = movable(T, v);
   double best = 1.0;
-  auto leaf = [&](uint32_t t) {
-    uint32_t t0 = T.tris[3 * (size_t)t], t1 = T.tris[3 * (size_t)t + 1], t2 = T.tris[3 * (size_t)t + 2];
-    f3 a = ld3(x, t0), b = ld3(x, t1), c = ld3(x, t2);
-    if (!inside_padded(p, vmin(vmin(a, b), c), vmax(vmax(a, b), c), pad)) return;
-    bool tm = movable(T, t0) || movable(T, t1) || movable(T, t2);
-    if (!tm && !mv) return;
-    if (excluded(T, v, t0, t1, t2)) return;
-    if (T.static_v && T.static_v[t0] && T.static_v[t1] && T.static_v[t2] && T.side[v] < 0.0f) return;
-    d3 x0[4] = {to_d(p), to_d(a), to_d(b), to_d(c)};
-    d3 d4[4] = {to_d(ld3(dx, v)), to_d(ld3(dx, t0)), to_d(ld3(dx, t1)), to_d(ld3(dx, t2))};
-    double toi = accd_toi(x0, d4, 0, 0.0, 1.25, 0.01, iters);
-    best = fmin(best, toi / 1.25);
-  };
-  auto ov = [&](f3 lo, f3 hi) { return inside_padded(p, lo, hi, pad); };
-  bvh_traverse(dyn, err, ov, leaf);
-  if (mv) bvh_traverse(stat, err, ov, leaf);
-  if (best < 1.0) atomicMin(tmin, (unsigned long long)__double_as_longlong(best));
+  if (v < T.V) {
+    f3 p = ld3(x, v);
+    bool mv = movable(T, v);
+    auto leaf = [&](uint32_t t) {
+      uint32_t t0 = T.tris[3 * (size_t)t], t1 = T.tris[3 * (size_t)t + 1], t2 = T.tris[3 * (size_t)t + 2];
+      f3 a = ld3(x, t0), b = ld3(x, t1), c = ld3(x, t2);
+      if (!inside_padded(p, vmin(vmin(a, b), c), vmax(vmax(a, b), c), pad)) return;
+      bool tm = movable(T, t0) || movable(T, t1) || movable(T, t2);
+      if (!tm && !mv) return;
+      if (excluded(T, v, t0, t1, t2)) return;
+      if (T.static_v && T.static_v[t0] && T.static_v[t1] && T.static_v[t2] && T.side[v] < 0.0f) return;
+      const f3 dv[4] = {ld3(dx, v), ld3(dx, t0), ld3(dx, t1), ld3(dx, t2)};
+      const f3 tri[3] = {a, b, c};
+      if (accd_surely_clear(&p, 1, tri, 3, dv)) return;
+      d3 x0[4] = {to_d(p), to_d(a), to_d(b), to_d(c)};
+      d3 d4[4] = {to_d(dv[0]), to_d(dv[1]), to_d(dv[2]), to_d(dv[3])};
+      bool capped = false;
+      double toi = accd_toi(x0, d4, 0, 0.0, 1.25, 0.01, min(iters, ACCD_INLINE), &capped);
+      if (capped && iters > ACCD_INLINE) {
+        if (defer_pair(D, v, t, 0u)) return;
+        toi = accd_toi(x0, d4, 0, 0.0, 1.25, 0.01, iters);
+      }
+      best = fmin(best, toi / 1.25);
+    };
+    auto ov = [&](f3 lo, f3 hi) { return inside_padded(p, lo, hi, pad); };
+    // parts 0 .. TRAV_PARTS-1: the movable tree's subtrees; the rest: the static tree's
+    const uint32_t part = blockIdx.y;
+    if (part < TRAV_PARTS) bvh_traverse_part(dyn, part, err, tstack, ov, leaf);
+    else if (mv) bvh_traverse_part(stat, part - TRAV_PARTS, err, tstack, ov, leaf);
+  }
+  // one atomic per wavefront: the minimum is order-independent
+  const double wb = wave_min(best);
+  if ((threadIdx.x & (WAVE - 1)) == 0 && wb < 1.0) atomicMin(tmin, (unsigned long long)__double_as_longlong(wb));
 }
 
 // Edge-edge ACCD (implicit.rs:1902-1936).
 __global__ __launch_bounds__(BLOCK) void k_ccd_ee(Topo T, BvhView dyn, const uint32_t *qids, uint32_t nq, int query_dyn, const float *__restrict__ x,
-                                                 const float *__restrict__ dx, float pad, uint32_t iters, unsigned long long *tmin, uint32_t *err) {
+                                                 const float *__restrict__ dx, float pad, uint32_t iters, CcdDefer D, unsigned long long *tmin,
+                                                 uint32_t *err) {
+  __shared__ uint32_t tstack[TRAV_LDS * BLOCK];
   uint32_t q = blockIdx.x * BLOCK + threadIdx.x;
-  if (q >= nq) return;
-  uint32_t ej = qids[q];
-  uint32_t b0 = T.edges[2 * (size_t)ej], b1 = T.edges[2 * (size_t)ej + 1];
-  f3 pb = ld3(x, b0), qb = ld3(x, b1);
-  f3 blo = vmin(pb, qb) - f3{pad, pad, pad}, bhi = vmax(pb, qb) + f3{pad, pad, pad};
-  auto ov = [&](f3 lo, f3 hi) { return !(lo.x > bhi.x || lo.y > bhi.y || lo.z > bhi.z || hi.x < blo.x || hi.y < blo.y || hi.z < blo.z); };
   double best = 1.0;
-  bvh_traverse(dyn, err, ov, [&](uint32_t ei) {
-    if (ei == ej || (query_dyn && ei > ej)) return;
-    uint32_t a0 = T.edges[2 * (size_t)ei], a1 = T.edges[2 * (size_t)ei + 1];
-    if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) return;
-    f3 pa = ld3(x, a0), qa = ld3(x, a1);
-    if (!ov(vmin(pa, qa), vmax(pa, qa))) return;
-    uint32_t lo_e = min(ei, ej), hi_e = max(ei, ej);
-    uint32_t i0 = T.edges[2 * (size_t)lo_e], i1 = T.edges[2 * (size_t)lo_e + 1], i2 = T.edges[2 * (size_t)hi_e], i3 = T.edges[2 * (size_t)hi_e + 1];
-    d3 x0[4] = {to_d(ld3(x, i0)), to_d(ld3(x, i1)), to_d(ld3(x, i2)), to_d(ld3(x, i3))};
-    d3 d4[4] = {to_d(ld3(dx, i0)), to_d(ld3(dx, i1)), to_d(ld3(dx, i2)), to_d(ld3(dx, i3))};
-    double toi = accd_toi(x0, d4, 1, 0.0, 1.25, 0.01, iters);
-    best = fmin(best, toi / 1.25);
-  });
-  if (best < 1.0) atomicMin(tmin, (unsigned long long)__double_as_longlong(best));
+  if (q < nq) {
+    uint32_t ej = qids[q];
+    uint32_t b0 = T.edges[2 * (size_t)ej], b1 = T.edges[2 * (size_t)ej + 1];
+    f3 pb = ld3(x, b0), qb = ld3(x, b1);
+    f3 blo = vmin(pb, qb) - f3{pad, pad, pad}, bhi = vmax(pb, qb) + f3{pad, pad, pad};
+    auto ov = [&](f3 lo, f3 hi) { return !(lo.x > bhi.x || lo.y > bhi.y || lo.z > bhi.z || hi.x < blo.x || hi.y < blo.y || hi.z < blo.z); };
+    bvh_traverse_part(dyn, blockIdx.y, err, tstack, ov, [&](uint32_t ei) {
+      if (ei == ej || (query_dyn && ei > ej)) return;
+      uint32_t a0 = T.edges[2 * (size_t)ei], a1 = T.edges[2 * (size_t)ei + 1];
+      if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) return;
+      f3 pa = ld3(x, a0), qa = ld3(x, a1);
+      if (!ov(vmin(pa, qa), vmax(pa, qa))) return;
+      uint32_t lo_e = min(ei, ej), hi_e = max(ei, ej);
+      uint32_t i0 = T.edges[2 * (size_t)lo_e], i1 = T.edges[2 * (size_t)lo_e + 1], i2 = T.edges[2 * (size_t)hi_e], i3 = T.edges[2 * (size_t)hi_e + 1];
+      const f3 xs[4] = {ld3(x, i0), ld3(x, i1), ld3(x, i2), ld3(x, i3)};
+      const f3 dv[4] = {ld3(dx, i0), ld3(dx, i1), ld3(dx, i2), ld3(dx, i3)};
+      if (accd_surely_clear(xs, 2, xs + 2, 2, dv)) return;
+      d3 x0[4] = {to_d(xs[0]), to_d(xs[1]), to_d(xs[2]), to_d(xs[3])};
+      d3 d4[4] = {to_d(dv[0]), to_d(dv[1]), to_d(dv[2]), to_d(dv[3])};
+      bool capped = false;
+      double toi = accd_toi(x0, d4, 1, 0.0, 1.25, 0.01, min(iters, ACCD_INLINE), &capped);
+      if (capped && iters > ACCD_INLINE) {
+        if (defer_pair(D, lo_e, hi_e, 1u)) return;
+        toi = accd_toi(x0, d4, 1, 0.0, 1.25, 0.01, iters);
+      }
+      best = fmin(best, toi / 1.25);
+    });
+  }
+  // one atomic per wavefront: the minimum is order-independent
+  const double wb = wave_min(best);
+  if ((threadIdx.x & (WAVE - 1)) == 0 && wb < 1.0) atomicMin(tmin, (unsigned long long)__double_as_longlong(wb));
+}
+
+// The deferred pairs of one pass, one per thread, with the full iteration limit.
+__global__ __launch_bounds__(BLOCK) void k_ccd_deferred(Topo T, CcdDefer D, const float *__restrict__ x, const float *__restrict__ dx, uint32_t iters,
+                                                       unsigned long long *tmin) {
+  const uint32_t k = blockIdx.x * BLOCK + threadIdx.x;
+  if (k >= min(*D.count, D.cap)) return;
+  const uint32_t w0 = D.pairs[2 * (size_t)k], b = D.pairs[2 * (size_t)k + 1];
+  const uint32_t kind = w0 >> 31, a = w0 & 0x7fffffffu;
+  uint32_t i[4];
+  if (kind == 0) {
+    i[0] = a; i[1] = T.tris[3 * (size_t)b]; i[2] = T.tris[3 * (size_t)b + 1]; i[3] = T.tris[3 * (size_t)b + 2];
+  } else {
+    i[0] = T.edges[2 * (size_t)a]; i[1] = T.edges[2 * (size_t)a + 1]; i[2] = T.edges[2 * (size_t)b]; i[3] = T.edges[2 * (size_t)b + 1];
+  }
+  d3 x0[4], d4[4];
+  for (int q = 0; q < 4; ++q) { x0[q] = to_d(ld3(x, i[q])); d4[q] = to_d(ld3(dx, i[q])); }
+  const double toi = accd_toi(x0, d4, (int)kind, 0.0, 1.25, 0.01, iters) / 1.25;
+  if (toi < 1.0) atomicMin(tmin, (unsigned long long)__double_as_longlong(toi));
 }
 
 // x += t dx; per-block max |t dx|.
@@ -833,6 +921,7 @@ __global__ __launch_bounds__(BLOCK) void k_collider_metrics(uint32_t V, const fl
 // dynamic one (static edges).
 __global__ __launch_bounds__(BLOCK) void k_pierce(Topo T, BvhView dyn, BvhView stat, uint32_t E, const float *__restrict__ x, uint32_t *hit,
                                                  uint32_t *err) {
+  __shared__ uint32_t tstack[TRAV_LDS * BLOCK];
   uint32_t ei = blockIdx.x * BLOCK + threadIdx.x;
   if (ei >= E) return;
   uint32_t e0 = T.edges[2 * (size_t)ei], e1 = T.edges[2 * (size_t)ei + 1];
@@ -845,8 +934,8 @@ __global__ __launch_bounds__(BLOCK) void k_pierce(Topo T, BvhView dyn, BvhView s
     if (e0 == t0 || e0 == t1 || e0 == t2 || e1 == t0 || e1 == t1 || e1 == t2) return;
     if (edge_triangle_intersect(p, q, ld3(x, t0), ld3(x, t1), ld3(x, t2))) atomicOr(hit, 1u);
   };
-  bvh_traverse(dyn, err, ov, leaf);
-  if (em) bvh_traverse(stat, err, ov, leaf);
+  bvh_traverse(dyn, err, tstack, ov, leaf);
+  if (em) bvh_traverse(stat, err, tstack, ov, leaf);
 }
 
 }  // namespace nguo
@@ -925,6 +1014,7 @@ struct nguo_step {
   unsigned long long *hl = nullptr;
   nguo_step_metrics metrics{};
   DevBuf<float> trace;  // x after every Newton iteration of the last step
+  DevBuf<uint32_t> defer_count, defer_pairs;  // CCD pairs handed to k_ccd_deferred in a pass
   // drape adjoint: start of the last step (its friction reference), adjoint vector and rhs
   uint32_t n_shell_faces = 0;
   DevBuf<float> xprev, lam, adj_b, dgds;
@@ -1264,13 +1354,13 @@ void detect(nguo_step *st) {
     NGUO_HIP(hipMemsetAsync(st->vt_count.p, 0, 2 * sizeof(uint32_t), s));
     StencilOut ovt{st->cap, st->vt_count.p, st->vt_key.p, st->vt_ord.p, st->vt_d.p, st->vt_g.p};
     StencilOut oee{st->cap, st->vt_count.p + 1, st->ee_key.p, st->ee_ord.p, st->ee_d.p, st->ee_g.p};
-    hipLaunchKernelGGL(k_vt_stencils, dim3(div_up(st->V, BLOCK)), dim3(BLOCK), 0, s, T, st->tree_tri_dyn.view(), st->tree_tri_stat.view(),
+    hipLaunchKernelGGL(k_vt_stencils, dim3(div_up(st->V, BLOCK), 2 * TRAV_PARTS), dim3(BLOCK), 0, s, T, st->tree_tri_dyn.view(), st->tree_tri_stat.view(),
                        (const float *)st->x.p, dhat, ovt, st->err.p);
     if (st->n_edge_dyn)
-      hipLaunchKernelGGL(k_ee_stencils, dim3(div_up(st->n_edge_dyn, BLOCK)), dim3(BLOCK), 0, s, T, st->tree_edge_dyn.view(),
+      hipLaunchKernelGGL(k_ee_stencils, dim3(div_up(st->n_edge_dyn, BLOCK), TRAV_PARTS), dim3(BLOCK), 0, s, T, st->tree_edge_dyn.view(),
                          (const uint32_t *)st->edge_dyn_ids.p, st->n_edge_dyn, 1, (const float *)st->x.p, dhat, dhat, oee, st->err.p);
     if (st->n_edge_stat && st->n_edge_dyn)
-      hipLaunchKernelGGL(k_ee_stencils, dim3(div_up(st->n_edge_stat, BLOCK)), dim3(BLOCK), 0, s, T, st->tree_edge_dyn.view(),
+      hipLaunchKernelGGL(k_ee_stencils, dim3(div_up(st->n_edge_stat, BLOCK), TRAV_PARTS), dim3(BLOCK), 0, s, T, st->tree_edge_dyn.view(),
                          (const uint32_t *)st->edge_stat_ids.p, st->n_edge_stat, 0, (const float *)st->x.p, dhat, dhat, oee, st->err.p);
     NGUO_HIP(hipGetLastError());
     NGUO_HIP(hipMemcpyAsync(st->hu, st->vt_count.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -1350,17 +1440,22 @@ float ccd_max_step(nguo_step *st, float max_disp) {
     for (int pass = 0; pass < 64; ++pass) {
       bool full = allowed >= max_disp;
       float search = fixed + 2.0f * allowed;
+      const CcdDefer D{st->defer_count.p, st->defer_pairs.p, CCD_DEFER_CAP};
       double t_self = run_min([&] {
-        hipLaunchKernelGGL(k_ccd_vt, dim3(div_up(st->V, BLOCK)), dim3(BLOCK), 0, s, T, st->tree_tri_dyn.view(), st->tree_tri_stat.view(),
-                           (const float *)st->x.p, (const float *)st->dx.p, search, iters, st->tmin.p, st->err.p);
+        NGUO_HIP(hipMemsetAsync(st->defer_count.p, 0, sizeof(uint32_t), s));
+        hipLaunchKernelGGL(k_ccd_vt, dim3(div_up(st->V, BLOCK), 2 * TRAV_PARTS), dim3(BLOCK), 0, s, T, st->tree_tri_dyn.view(), st->tree_tri_stat.view(),
+                           (const float *)st->x.p, (const float *)st->dx.p, search, iters, D, st->tmin.p, st->err.p);
         if (st->n_edge_dyn)
-          hipLaunchKernelGGL(k_ccd_ee, dim3(div_up(st->n_edge_dyn, BLOCK)), dim3(BLOCK), 0, s, T, st->tree_edge_dyn.view(),
+          hipLaunchKernelGGL(k_ccd_ee, dim3(div_up(st->n_edge_dyn, BLOCK), TRAV_PARTS), dim3(BLOCK), 0, s, T, st->tree_edge_dyn.view(),
                              (const uint32_t *)st->edge_dyn_ids.p, st->n_edge_dyn, 1, (const float *)st->x.p, (const float *)st->dx.p, search, iters,
-                             st->tmin.p, st->err.p);
+                             D, st->tmin.p, st->err.p);
         if (st->n_edge_stat && st->n_edge_dyn)
-          hipLaunchKernelGGL(k_ccd_ee, dim3(div_up(st->n_edge_stat, BLOCK)), dim3(BLOCK), 0, s, T, st->tree_edge_dyn.view(),
+          hipLaunchKernelGGL(k_ccd_ee, dim3(div_up(st->n_edge_stat, BLOCK), TRAV_PARTS), dim3(BLOCK), 0, s, T, st->tree_edge_dyn.view(),
                              (const uint32_t *)st->edge_stat_ids.p, st->n_edge_stat, 0, (const float *)st->x.p, (const float *)st->dx.p, search,
-                             iters, st->tmin.p, st->err.p);
+                             iters, D, st->tmin.p, st->err.p);
+        if (iters > ACCD_INLINE)
+          hipLaunchKernelGGL(k_ccd_deferred, dim3(div_up(CCD_DEFER_CAP, BLOCK)), dim3(BLOCK), 0, s, T, D, (const float *)st->x.p, (const float *)st->dx.p,
+                             iters, st->tmin.p);
       });
       if (full || (float)t_self * max_disp <= allowed) { t = std::min(t, t_self); break; }
       allowed = std::min(allowed * 4.0f, max_disp);
@@ -1650,6 +1745,8 @@ int nguo_step_create(nguo_ctx *ctx, const nguo_scene_desc *sc, nguo_step **out) 
     st->err.zero(1, s);
     st->flag.zero(4, s);
     st->tmin.reserve(1);
+    st->defer_count.zero(1, s);
+    st->defer_pairs.reserve(2 * (size_t)CCD_DEFER_CAP);
     st->vt_count.zero(2, s);
     st->inc_ptr.zero((size_t)st->NF + 2, s);
     upload_elems(st->act, ha, s);
