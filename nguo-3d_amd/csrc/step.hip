@@ -735,6 +735,14 @@ __device__ inline bool accd_surely_clear(const f3 *pa, int na, const f3 *pb, int
     for (int j = i + 1; j < 4; ++j) u = fmaxf(u, len(dxs[i] - dxs[j]));
   return 1.25f * (u * (1.0f + 1.0e-5f) + 1.0e-12f) < 0.98f * gap;
 }
+
+// The same argument bounds the search: every candidate has |dx| <= max_disp, so a pair whose
+// boxes are more than 1.3 (dq + max_disp) apart (dq: the query's largest |dx|) has
+// 1.25 u_max < 0.98 d0 and is skipped; the query's padding is the smaller of the
+// widening pass's and that distance.
+__device__ inline float ccd_query_pad(float pad, float dq, float max_disp) {
+  return fminf(pad, 1.3f * (dq + max_disp) * (1.0f + 1.0e-5f) + 1.0e-9f);
+}
 struct CcdDefer {
   uint32_t *count;  // pairs appended
   uint32_t *pairs;  // [2 cap]: (a | kind << 31, b): vertex-triangle (v, t) or edge-edge (lo, hi)
@@ -749,13 +757,14 @@ __device__ inline bool defer_pair(const CcdDefer &D, uint32_t a, uint32_t b, uin
 }
 
 __global__ __launch_bounds__(BLOCK) void k_ccd_vt(Topo T, BvhView dyn, BvhView stat, const float *__restrict__ x, const float *__restrict__ dx,
-                                                 float pad, uint32_t iters, CcdDefer D, unsigned long long *tmin, uint32_t *err) {
+                                                 float pad0, float max_disp, uint32_t iters, CcdDefer D, unsigned long long *tmin, uint32_t *err) {
   __shared__ uint32_t tstack[TRAV_LDS * BLOCK];
   uint32_t v = blockIdx.x * BLOCK + threadIdx.x;
   double best = 1.0;
   if (v < T.V) {
     f3 p = ld3(x, v);
     bool mv = movable(T, v);
+    const float pad = ccd_query_pad(pad0, len(ld3(dx, v)), max_disp);
     auto leaf = [&](uint32_t t) {
       uint32_t t0 = T.tris[3 * (size_t)t], t1 = T.tris[3 * (size_t)t + 1], t2 = T.tris[3 * (size_t)t + 2];
       f3 a = ld3(x, t0), b = ld3(x, t1), c = ld3(x, t2);
@@ -790,8 +799,8 @@ __global__ __launch_bounds__(BLOCK) void k_ccd_vt(Topo T, BvhView dyn, BvhView s
 
 // Edge-edge ACCD (implicit.rs:1902-1936).
 __global__ __launch_bounds__(BLOCK) void k_ccd_ee(Topo T, BvhView dyn, const uint32_t *qids, uint32_t nq, int query_dyn, const float *__restrict__ x,
-                                                 const float *__restrict__ dx, float pad, uint32_t iters, CcdDefer D, unsigned long long *tmin,
-                                                 uint32_t *err) {
+                                                 const float *__restrict__ dx, float pad0, float max_disp, uint32_t iters, CcdDefer D,
+                                                 unsigned long long *tmin, uint32_t *err) {
   __shared__ uint32_t tstack[TRAV_LDS * BLOCK];
   uint32_t q = blockIdx.x * BLOCK + threadIdx.x;
   double best = 1.0;
@@ -799,6 +808,7 @@ __global__ __launch_bounds__(BLOCK) void k_ccd_ee(Topo T, BvhView dyn, const uin
     uint32_t ej = qids[q];
     uint32_t b0 = T.edges[2 * (size_t)ej], b1 = T.edges[2 * (size_t)ej + 1];
     f3 pb = ld3(x, b0), qb = ld3(x, b1);
+    const float pad = ccd_query_pad(pad0, fmaxf(len(ld3(dx, b0)), len(ld3(dx, b1))), max_disp);
     f3 blo = vmin(pb, qb) - f3{pad, pad, pad}, bhi = vmax(pb, qb) + f3{pad, pad, pad};
     auto ov = [&](f3 lo, f3 hi) { return !(lo.x > bhi.x || lo.y > bhi.y || lo.z > bhi.z || hi.x < blo.x || hi.y < blo.y || hi.z < blo.z); };
     bvh_traverse_part(dyn, blockIdx.y, err, tstack, ov, [&](uint32_t ei) {
@@ -1444,15 +1454,15 @@ float ccd_max_step(nguo_step *st, float max_disp) {
       double t_self = run_min([&] {
         NGUO_HIP(hipMemsetAsync(st->defer_count.p, 0, sizeof(uint32_t), s));
         hipLaunchKernelGGL(k_ccd_vt, dim3(div_up(st->V, BLOCK), 2 * TRAV_PARTS), dim3(BLOCK), 0, s, T, st->tree_tri_dyn.view(), st->tree_tri_stat.view(),
-                           (const float *)st->x.p, (const float *)st->dx.p, search, iters, D, st->tmin.p, st->err.p);
+                           (const float *)st->x.p, (const float *)st->dx.p, search, max_disp, iters, D, st->tmin.p, st->err.p);
         if (st->n_edge_dyn)
           hipLaunchKernelGGL(k_ccd_ee, dim3(div_up(st->n_edge_dyn, BLOCK), TRAV_PARTS), dim3(BLOCK), 0, s, T, st->tree_edge_dyn.view(),
-                             (const uint32_t *)st->edge_dyn_ids.p, st->n_edge_dyn, 1, (const float *)st->x.p, (const float *)st->dx.p, search, iters,
-                             D, st->tmin.p, st->err.p);
+                             (const uint32_t *)st->edge_dyn_ids.p, st->n_edge_dyn, 1, (const float *)st->x.p, (const float *)st->dx.p, search, max_disp,
+                             iters, D, st->tmin.p, st->err.p);
         if (st->n_edge_stat && st->n_edge_dyn)
           hipLaunchKernelGGL(k_ccd_ee, dim3(div_up(st->n_edge_stat, BLOCK), TRAV_PARTS), dim3(BLOCK), 0, s, T, st->tree_edge_dyn.view(),
                              (const uint32_t *)st->edge_stat_ids.p, st->n_edge_stat, 0, (const float *)st->x.p, (const float *)st->dx.p, search,
-                             iters, D, st->tmin.p, st->err.p);
+                             max_disp, iters, D, st->tmin.p, st->err.p);
         if (iters > ACCD_INLINE)
           hipLaunchKernelGGL(k_ccd_deferred, dim3(div_up(CCD_DEFER_CAP, BLOCK)), dim3(BLOCK), 0, s, T, D, (const float *)st->x.p, (const float *)st->dx.p,
                              iters, st->tmin.p);
