@@ -119,6 +119,9 @@ class Pcg {
   // z = M^-1 r with the current preconditioner (after build_precond), for parity checks.
   void apply_precond(const SellMatrix &M, const float *r, float *z);
   uint32_t chunk = 8;  // iterations per host convergence check
+  // SpMV slot-group size (slots whose loads are in flight at once), picked by timing the
+  // instantiated sizes on the live system at the first solve of a system size
+  uint32_t spmv_group() const { return spmv_group_; }
 
  private:
   void ensure(uint32_t nb, uint32_t ns);
@@ -129,6 +132,8 @@ class Pcg {
   hipStream_t s_;
   Schwarz *sz_ = nullptr;
   uint32_t cap_nb_ = 0, cap_ns_ = 0;
+  uint32_t spmv_group_ = 16, tuned_nb_ = 0;     // single-system SpMV
+  uint32_t batch_group_ = 16, batch_tuned_ = 0;  // batched SpMV (tuned on the total slices)
   DevBuf<float> r_, z_, p_, p2_, q_, d_, minv_;
   DevBuf<double> x64_, r64_, part_, scal_, sums64_;
   DevBuf<uint32_t> flag_;

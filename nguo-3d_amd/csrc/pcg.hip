@@ -118,7 +118,7 @@ __global__ __launch_bounds__(BLOCK) void k_contact_sums(ContactView C, const TX 
 // At these sizes a launch costs a few dependent memory round trips, and a workgroup's loads
 // queue on its compute unit's memory pipeline, so the SpMV spreads over one wavefront per
 // workgroup and issues every load of a group of slots before its FMAs.
-constexpr uint32_t SPMV_GROUP = 16;   // slots whose loads are in flight together
+constexpr uint32_t SPMV_GROUP = 16;   // default: slots whose loads are in flight together
 constexpr uint32_t PRE_STENCILS = 64; // stencils per k_pcg_pre workgroup
 enum SpmvMode { SPMV_PLAIN = 0, SPMV_FIRST = 1, SPMV_NEXT = 2 };
 
@@ -130,15 +130,15 @@ template <int MODE> __device__ inline float4 pval4(const float4 *z, const float4
 }
 
 // Loads of a group of slots: the block values and the gathered z / pold of the column.
-template <int MODE> struct SlotGroup {
-  BlockVals v[SPMV_GROUP];
-  float4 zc[SPMV_GROUP], pc[SPMV_GROUP];
+template <int MODE, uint32_t G> struct SlotGroup {
+  BlockVals v[G];
+  float4 zc[G], pc[G];
   __device__ inline void load(const SellView &A, const float4 *z, const float4 *pold, uint32_t base, uint32_t k0, uint32_t len, uint32_t lane) {
-    uint32_t cc[SPMV_GROUP];
+    uint32_t cc[G];
 #pragma unroll
-    for (uint32_t u = 0; u < SPMV_GROUP; ++u) cc[u] = k0 + u < len ? A.col[base + 64u * (k0 + u) + lane] : 0u;
+    for (uint32_t u = 0; u < G; ++u) cc[u] = k0 + u < len ? A.col[base + 64u * (k0 + u) + lane] : 0u;
 #pragma unroll
-    for (uint32_t u = 0; u < SPMV_GROUP; ++u) {
+    for (uint32_t u = 0; u < G; ++u) {
       if (k0 + u < len) {
         v[u] = load_block(A.val, base + 64u * (k0 + u), lane);
         pc[u] = pold[cc[u]];
@@ -149,7 +149,7 @@ template <int MODE> struct SlotGroup {
   // y += sum over the group's slots of B p (p = pold, or z + beta pold), slots ascending
   __device__ inline void accumulate(float &y0, float &y1, float &y2, float beta, uint32_t k0, uint32_t len) const {
 #pragma unroll
-    for (uint32_t u = 0; u < SPMV_GROUP; ++u) {
+    for (uint32_t u = 0; u < G; ++u) {
       if (k0 + u < len) {
         float x0 = pc[u].x, x1 = pc[u].y, x2 = pc[u].z;
         if (MODE == SPMV_NEXT) { x0 = zc[u].x + beta * x0; x1 = zc[u].y + beta * x1; x2 = zc[u].z + beta * x2; }
@@ -244,7 +244,7 @@ inline unsigned pre_grid(const ContactView &C) { return std::max(1u, div_up(C.ns
 // (beta recorded by k_pcg_pre) wherever p is read; the row's owner also stores p_j to pnew,
 // and every reader of a neighbour's p evaluates the same expression, so they agree bitwise.
 // part[slice] = p.q over the slice.
-template <int MODE>
+template <int MODE, uint32_t G>
 __device__ __forceinline__ void pcg_spmv_body(uint32_t slice, const SellView &A, const ContactView &C, const float4 *__restrict__ z,
                                               const float4 *__restrict__ pold, float4 *__restrict__ pnew, float4 *__restrict__ q, double *part,
                                               const double *scal, const uint32_t *flag) {
@@ -258,7 +258,7 @@ __device__ __forceinline__ void pcg_spmv_body(uint32_t slice, const SellView &A,
     len = A.row_len[row];
     if (C.ns) { ib = C.inc_ptr[row]; ie = C.inc_ptr[row + 1]; }
   }
-  SlotGroup<MODE> g;
+  SlotGroup<MODE, G> g;
   g.load(A, z, pold, base, 0, len, lane);
   float4 cv[8];
 #pragma unroll
@@ -270,7 +270,7 @@ __device__ __forceinline__ void pcg_spmv_body(uint32_t slice, const SellView &A,
   if (active) {
     float y0 = 0.0f, y1 = 0.0f, y2 = 0.0f;
     g.accumulate(y0, y1, y2, beta, 0, len);
-    for (uint32_t k0 = SPMV_GROUP; k0 < width; k0 += SPMV_GROUP) {
+    for (uint32_t k0 = G; k0 < width; k0 += G) {
       g.load(A, z, pold, base, k0, len, lane);
       g.accumulate(y0, y1, y2, beta, k0, len);
     }
@@ -293,17 +293,63 @@ __device__ __forceinline__ void pcg_spmv_body(uint32_t slice, const SellView &A,
   if (lane == 0) part[slice] = dp;
 }
 
-template <int MODE>
+template <int MODE, uint32_t G>
 __global__ __launch_bounds__(WAVE) void k_pcg_spmv(SellView A, ContactView C, const float4 *__restrict__ z, const float4 *__restrict__ pold,
                                                   float4 *__restrict__ pnew, float4 *__restrict__ q, double *part, const double *scal,
                                                   const uint32_t *flag) {
-  pcg_spmv_body<MODE>(blockIdx.x, A, C, z, pold, pnew, q, part, scal, flag);
+  pcg_spmv_body<MODE, G>(blockIdx.x, A, C, z, pold, pnew, q, part, scal, flag);
 }
 
-template <int MODE> __global__ __launch_bounds__(WAVE) void kb_pcg_spmv(BatchOf<SpmvArgs> B) {
+template <int MODE, uint32_t G> __global__ __launch_bounds__(WAVE) void kb_pcg_spmv(BatchOf<SpmvArgs> B) {
   const uint32_t i = batch_slot(B.off, B.k, blockIdx.x);
   const SpmvArgs &a = B.a[i];
-  pcg_spmv_body<MODE>(blockIdx.x - B.off[i], a.A, a.C, a.z, a.pold, a.pnew, a.q, a.part, a.scal, a.flag);
+  pcg_spmv_body<MODE, G>(blockIdx.x - B.off[i], a.A, a.C, a.z, a.pold, a.pnew, a.q, a.part, a.scal, a.flag);
+}
+
+// Host-side dispatch over the slot-group sizes the SpMV is instantiated for; the group only
+// changes how many slots' loads are in flight at once (and so the registers and occupancy),
+// not the order of the row's sum, so every size gives bitwise the same q.
+template <int MODE>
+static void launch_spmv(uint32_t G, hipStream_t s, unsigned grid, const SellView &A, const ContactView &C, const float4 *z, const float4 *pold,
+                        float4 *pnew, float4 *q, double *part, const double *scal, const uint32_t *flag) {
+  if (G == 8) hipLaunchKernelGGL((k_pcg_spmv<MODE, 8>), dim3(grid), dim3(WAVE), 0, s, A, C, z, pold, pnew, q, part, scal, flag);
+  else hipLaunchKernelGGL((k_pcg_spmv<MODE, 16>), dim3(grid), dim3(WAVE), 0, s, A, C, z, pold, pnew, q, part, scal, flag);
+}
+template <int MODE> static void launch_spmv_batch(uint32_t G, hipStream_t s, unsigned grid, const BatchOf<SpmvArgs> &B) {
+  if (G == 8) hipLaunchKernelGGL((kb_pcg_spmv<MODE, 8>), dim3(grid), dim3(WAVE), 0, s, B);
+  else hipLaunchKernelGGL((kb_pcg_spmv<MODE, 16>), dim3(grid), dim3(WAVE), 0, s, B);
+}
+constexpr uint32_t SPMV_GROUP_CHOICES[2] = {8, 16};
+
+// Average ms of `reps` launches of f() between two events on s (after two warm-up launches).
+template <class F> static double time_launches(hipStream_t s, uint32_t reps, F f) {
+  f();
+  f();
+  hipEvent_t e0, e1;
+  NGUO_HIP(hipEventCreate(&e0));
+  NGUO_HIP(hipEventCreate(&e1));
+  NGUO_HIP(hipEventRecord(e0, s));
+  for (uint32_t r = 0; r < reps; ++r) f();
+  NGUO_HIP(hipEventRecord(e1, s));
+  NGUO_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  NGUO_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  NGUO_HIP(hipGetLastError());
+  return (double)ms / reps;
+}
+
+// The autotuner: the slot-group size whose launches of `launch(G)` are fastest on the live
+// system (ties keep the larger group).
+template <class L> static uint32_t pick_group(hipStream_t s, L launch) {
+  uint32_t best = SPMV_GROUP;
+  double best_ms = 1e30;
+  for (uint32_t G : SPMV_GROUP_CHOICES) {
+    const double ms = time_launches(s, 10, [&] { launch(G); });
+    if (ms < best_ms * 0.98 || (ms <= best_ms && G > best)) { best_ms = std::min(best_ms, ms); best = G; }
+  }
+  return best;
 }
 
 // 3x3 block-Jacobi preconditioner step, one block row per thread (vectors of stride 4,
@@ -596,6 +642,15 @@ PcgStats Pcg::solve(const SellMatrix &M, const ContactView &C, const float *b, f
     double eta = std::min(0.5, std::max(0.25 * (double)tol / res, REFINE_ETA_MIN));
     launch_precond_init(M, part_pc);
     hipLaunchKernelGGL(k_init_scal, dim3(1), dim3(BLOCK), 0, s_, scal_.p, flag_.p, (const double *)part_pc, nz, eta * eta, max_iter - st.iters);
+    if (tuned_nb_ != nb) {
+      // autotune on the live system: the first iteration's SpMV (idempotent; iteration 0
+      // recomputes its q and partials below)
+      spmv_group_ = pick_group(s_, [&](uint32_t G) {
+        launch_spmv<SPMV_FIRST>(G, s_, NS, A, C, (const float4 *)z_.p, (const float4 *)p_.p, (float4 *)nullptr, (float4 *)q_.p, part_spmv,
+                                (const double *)scal_.p, (const uint32_t *)flag_.p);
+      });
+      tuned_nb_ = nb;
+    }
     // Chunks of `chunk` iterations are queued one ahead of the one whose state the host
     // reads, so the device never idles on the convergence check; iterations queued after
     // convergence exit at their first instruction.
@@ -608,13 +663,13 @@ PcgStats Pcg::solve(const SellMatrix &M, const ContactView &C, const float *b, f
           if (C.ns)
             hipLaunchKernelGGL(k_pcg_pre<SPMV_FIRST>, dim3(GP), dim3(BLOCK), 0, s_, C, (const float4 *)z_.p, (const float4 *)pj,
                                (const double *)part_pc, nz, scal_.p, flag_.p, j);
-          hipLaunchKernelGGL(k_pcg_spmv<SPMV_FIRST>, dim3(NS), dim3(WAVE), 0, s_, A, C, (const float4 *)z_.p, (const float4 *)pj,
-                             (float4 *)nullptr, (float4 *)q_.p, part_spmv, (const double *)scal_.p, (const uint32_t *)flag_.p);
+          launch_spmv<SPMV_FIRST>(spmv_group_, s_, NS, A, C, (const float4 *)z_.p, (const float4 *)pj, (float4 *)nullptr, (float4 *)q_.p, part_spmv,
+                                  (const double *)scal_.p, (const uint32_t *)flag_.p);
         } else {
           hipLaunchKernelGGL(k_pcg_pre<SPMV_NEXT>, dim3(GP), dim3(BLOCK), 0, s_, C, (const float4 *)z_.p, (const float4 *)pprev,
                              (const double *)part_pc, nz, scal_.p, flag_.p, j);
-          hipLaunchKernelGGL(k_pcg_spmv<SPMV_NEXT>, dim3(NS), dim3(WAVE), 0, s_, A, C, (const float4 *)z_.p, (const float4 *)pprev,
-                             (float4 *)pj, (float4 *)q_.p, part_spmv, (const double *)scal_.p, (const uint32_t *)flag_.p);
+          launch_spmv<SPMV_NEXT>(spmv_group_, s_, NS, A, C, (const float4 *)z_.p, (const float4 *)pprev, (float4 *)pj, (float4 *)q_.p, part_spmv,
+                                 (const double *)scal_.p, (const uint32_t *)flag_.p);
         }
         if (sz_) {
           launch_schwarz_pcg(s_, *sz_, false, nullptr, r_.p, z_.p, d_.p, pj, q_.p, part_spmv, NS, part_pc, scal_.p, flag_.p, j);
@@ -747,15 +802,20 @@ void Pcg::solve_batch(hipStream_t s, PcgBatchItem *items, uint32_t n) {
       bn1.a[a].pold = (const float4 *)P.p_.p;
       bn0.a[a].pold = (const float4 *)P.p2_.p;
     }
+    uint32_t &group = items[0].pcg->batch_group_;
+    if (items[0].pcg->batch_tuned_ != sf.off[k]) {
+      group = pick_group(s, [&](uint32_t G) { launch_spmv_batch<SPMV_FIRST>(G, s, sf.off[k], sf); });
+      items[0].pcg->batch_tuned_ = sf.off[k];
+    }
     uint32_t j = 0, queued = 0, waited = 0;
     auto queue_chunk = [&]() {
       for (uint32_t c = 0; c < chunk; ++c, ++j) {
         if (j == 0) {
           if (bf.off[k]) hipLaunchKernelGGL(kb_pcg_pre<SPMV_FIRST>, dim3(bf.off[k]), dim3(BLOCK), 0, s, bf, j);
-          hipLaunchKernelGGL(kb_pcg_spmv<SPMV_FIRST>, dim3(sf.off[k]), dim3(WAVE), 0, s, sf);
+          launch_spmv_batch<SPMV_FIRST>(group, s, sf.off[k], sf);
         } else {
           hipLaunchKernelGGL(kb_pcg_pre<SPMV_NEXT>, dim3(bn.off[k]), dim3(BLOCK), 0, s, (j & 1) ? bn1 : bn0, j);
-          hipLaunchKernelGGL(kb_pcg_spmv<SPMV_NEXT>, dim3(sf.off[k]), dim3(WAVE), 0, s, (j & 1) ? sn1 : sn0);
+          launch_spmv_batch<SPMV_NEXT>(group, s, sf.off[k], (j & 1) ? sn1 : sn0);
         }
         launch_schwarz_pcg_batch(s, (j & 1) ? c1 : c0, j);
       }
@@ -837,27 +897,19 @@ double Pcg::time_spmv(const SellView &A, const ContactView &C, const float *x, u
   hipLaunchKernelGGL(k_pack4, dim3(div_up(A.nb, BLOCK)), dim3(BLOCK), 0, s_, A.nb, x, (float4 *)z_.p);
   if (iteration_form) hipLaunchKernelGGL(k_bench_state, dim3(1), dim3(WAVE), 0, s_, flag_.p, scal_.p);
   if (C.ns) launch_pre_plain(C);  // the contact blocks of p (fixed over the timed launches)
-  auto launch = [&]() {
+  auto launch_g = [&](uint32_t G) {
     if (iteration_form)
-      hipLaunchKernelGGL(k_pcg_spmv<SPMV_NEXT>, dim3(NS), dim3(WAVE), 0, s_, A, C, (const float4 *)z_.p, (const float4 *)p2_.p, (float4 *)p_.p,
-                         (float4 *)q_.p, part_.p, (const double *)scal_.p, (const uint32_t *)flag_.p);
+      launch_spmv<SPMV_NEXT>(G, s_, NS, A, C, (const float4 *)z_.p, (const float4 *)p2_.p, (float4 *)p_.p, (float4 *)q_.p, part_.p,
+                             (const double *)scal_.p, (const uint32_t *)flag_.p);
     else
-      launch_spmv_plain(A, C);
+      launch_spmv<SPMV_PLAIN>(G, s_, NS, A, C, (const float4 *)nullptr, (const float4 *)p2_.p, (float4 *)nullptr, (float4 *)q_.p, (double *)nullptr,
+                              (const double *)nullptr, (const uint32_t *)nullptr);
   };
-  launch();  // warm-up
-  hipEvent_t e0, e1;
-  NGUO_HIP(hipEventCreate(&e0));
-  NGUO_HIP(hipEventCreate(&e1));
-  NGUO_HIP(hipEventRecord(e0, s_));
-  for (uint32_t r = 0; r < reps; ++r) launch();
-  NGUO_HIP(hipEventRecord(e1, s_));
-  NGUO_HIP(hipEventSynchronize(e1));
-  float ms = 0;
-  NGUO_HIP(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  NGUO_HIP(hipGetLastError());
-  return (double)ms / reps;
+  if (tuned_nb_ != A.nb) {  // a system no solve has tuned yet
+    spmv_group_ = pick_group(s_, launch_g);
+    tuned_nb_ = A.nb;
+  }
+  return time_launches(s_, reps, [&] { launch_g(spmv_group_); });
 }
 
 void Pcg::launch_pre_plain(const ContactView &C) {
@@ -866,8 +918,8 @@ void Pcg::launch_pre_plain(const ContactView &C) {
 }
 
 void Pcg::launch_spmv_plain(const SellView &A, const ContactView &C) {
-  hipLaunchKernelGGL(k_pcg_spmv<SPMV_PLAIN>, dim3(div_up(A.nb, 64)), dim3(WAVE), 0, s_, A, C, (const float4 *)nullptr, (const float4 *)p2_.p,
-                     (float4 *)nullptr, (float4 *)q_.p, (double *)nullptr, (const double *)nullptr, (const uint32_t *)nullptr);
+  launch_spmv<SPMV_PLAIN>(spmv_group_, s_, div_up(A.nb, 64), A, C, (const float4 *)nullptr, (const float4 *)p2_.p, (float4 *)nullptr,
+                          (float4 *)q_.p, (double *)nullptr, (const double *)nullptr, (const uint32_t *)nullptr);
 }
 
 }  // namespace nguo
