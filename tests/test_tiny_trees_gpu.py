@@ -14,7 +14,7 @@ import oracle_py as O
 class TinyScene:
     """A scene description built from numpy arrays (kept alive with the descriptor)."""
 
-    def __init__(self, n_cloth_tris, n_static_tris, gap=4e-4, speed=0.5):
+    def __init__(self, n_cloth_tris, n_static_tris, gap=4e-4, speed=0.5, slide=0.0):
         quad = lambda z0, s: np.array([[0, 0, 0], [s, 0, 0], [0, 0, s], [s, 0, s]], np.float32) + np.float32([0, z0, 0])
         cloth = quad(gap, 0.1)[: 3 if n_cloth_tris == 1 else 4] + np.float32([0.02, 0.0, 0.02])
         body = quad(0.0, 0.5)[: 3 if n_static_tris == 1 else 4] - np.float32([0.1, 0.0, 0.1])
@@ -25,6 +25,7 @@ class TinyScene:
         self.static = np.concatenate([np.zeros(nc), np.ones(nb)]).astype(np.uint8)
         vel = np.zeros_like(self.positions)
         vel[:nc, 1] = -speed
+        vel[:nc, 0] = slide
         self.velocities = vel
         tri = lambda k, o: [[o + 0, o + 1, o + 2], [o + 2, o + 1, o + 3]][:k]
         shell = np.array(tri(n_cloth_tris, 0), np.uint32)
@@ -83,4 +84,24 @@ def test_tiny_trees_ccd_limited_step(ctx, nc, ns):
     # two-triangle patches, where the cloth rests across the patch's diagonal edge)
     assert err <= 1e-5
     assert m.penetration_free == 1 and m.all_finite == 1
+    st.close()
+
+
+@pytest.mark.gpu
+def test_sliding_contact_long_advancement(ctx):
+    # sliding fast along the static patch while closing slowly: the advancement moves by
+    # (d - target) / u_max with u_max dominated by the slide, so pairs need many iterations
+    # (the ones past the inline budget go to the per-pair pass); same step as the oracle
+    sc = TinyScene(2, 2, gap=1.5e-3, speed=0.3, slide=3.0)
+    o = O.OracleSolver(sc)
+    st = nguo_hip.Stepper(ctx, sc)
+    o.step(1)
+    st.advance(1)
+    _, tois = st.newton_trace()
+    _, otois = o.newton_trace()
+    err = np.abs(st.positions() - o.positions()).max()
+    print(f"sliding: toi device {np.round(tois, 5)} oracle {np.round(otois, 5)} err={err:.2e}")
+    assert min(tois) < 1.0
+    assert np.allclose(tois, otois, rtol=1e-5, atol=0)
+    assert err <= 1e-5
     st.close()
