@@ -38,13 +38,15 @@ struct SellMatrix {
 // A_d^-1 = L^-T L^-1; the apply is one dense 96 x 96 product per domain.
 constexpr uint32_t SCHWARZ_MAX_DOMAIN = 32;
 constexpr uint32_t SCHWARZ_MAX_DOF = 3 * SCHWARZ_MAX_DOMAIN;
+// floats of one domain's FP32 inverse: the 21 upper 16 x 16 tiles of the symmetric block
+constexpr uint32_t SCHWARZ_INV_FLOATS = (SCHWARZ_MAX_DOF / 16) * (SCHWARZ_MAX_DOF / 16 + 1) / 2 * 256;
 
 struct Schwarz {
   uint32_t nd = 0, domain = SCHWARZ_MAX_DOMAIN;
   std::vector<uint32_t> h_dom_ptr, h_dom_row;
   DevBuf<uint32_t> dom_ptr, dom_row, row_dom, row_pos, fails;
   DevBuf<uint32_t> dom_dof;  // [nd * 96]: stride-4 vector index of dof t of domain d, or ~0u past m
-  DevBuf<float> ainv;  // nd blocks of SCHWARZ_MAX_DOF^2, row-major with stride SCHWARZ_MAX_DOF
+  DevBuf<float> ainv;  // nd blocks of SCHWARZ_INV_FLOATS (upper 16 x 16 tiles, row-major each)
   // Two-level form (engine-core/src/linsolve.rs:522-657 MultilevelSchwarz): the coarse node of
   // a vertex is its domain; the Galerkin coarse operator A1 = C A C^T sums the blocks between
   // two domains; a one-level Schwarz over A1 (coarse domains: BFS over the coarse graph, same
@@ -53,7 +55,7 @@ struct Schwarz {
   uint32_t nd1 = 0;
   std::vector<uint32_t> h_cdom_ptr, h_cdom_node;
   DevBuf<uint32_t> cdom_ptr, cdom_node, node_cdom, node_cpos;
-  DevBuf<float> ainv1;  // nd1 coarse blocks, as ainv
+  DevBuf<float> ainv1;  // nd1 coarse blocks of SCHWARZ_MAX_DOF^2, row-major with stride SCHWARZ_MAX_DOF
   DevBuf<float> rc;     // [4 nd] restricted residual (sum of a domain's rows), per apply
   void build(const SellMatrix &M, uint32_t domain_size, hipStream_t s, float coarse_weight = 0.0f);
 };

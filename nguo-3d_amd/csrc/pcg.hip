@@ -122,11 +122,13 @@ constexpr uint32_t SPMV_GROUP = 16;   // default: slots whose loads are in fligh
 constexpr uint32_t PRE_STENCILS = 64; // stencils per k_pcg_pre workgroup
 enum SpmvMode { SPMV_PLAIN = 0, SPMV_FIRST = 1, SPMV_NEXT = 2 };
 
+__device__ inline float4 pnext(const float4 &zz, const float4 &pp, float beta) {
+  return make_float4(zz.x + beta * pp.x, zz.y + beta * pp.y, zz.z + beta * pp.z, 0.0f);
+}
 template <int MODE> __device__ inline float4 pval4(const float4 *z, const float4 *pold, float beta, size_t i) {
   const float4 pp = pold[i];
   if (MODE != SPMV_NEXT) return pp;
-  const float4 zz = z[i];
-  return make_float4(zz.x + beta * pp.x, zz.y + beta * pp.y, zz.z + beta * pp.z, 0.0f);
+  return pnext(z[i], pp, beta);
 }
 
 // Loads of a group of slots: the block values and the gathered z / pold of the column.
@@ -167,18 +169,48 @@ template <int MODE, uint32_t G> struct SlotGroup {
 // FIRST: p_0 = pold. PLAIN: p = pold, no state. Then, PRE_STENCILS stencils per workgroup:
 // the sums g . p and sum_a w_a p_a over the participants in order, p evaluated exactly as
 // k_pcg_spmv evaluates it, and each free participant's block of H_s p,
-// hp (g.p) g_a + lambda w_a (s_w - n (n . s_w)), stored at its incidence slot.
+// hp (g.p) g_a + lambda w_a (s_w - n (n . s_w)), stored at its incidence slot. The stencil's
+// data and its participants' z / pold entries need no scalar, so they are loaded before the
+// reduction and are in flight while it runs.
 template <int MODE>
 __device__ __forceinline__ void pcg_pre_body(uint32_t bid, const ContactView &C, const float4 *__restrict__ z, const float4 *__restrict__ pold,
                                              const double *cur, uint32_t nz, double *scal, uint32_t *flag, uint32_t j) {
   __shared__ double lds[2 * BLOCK / WAVE];
-  if (MODE != SPMV_PLAIN && flag[0]) return;
+  const uint32_t state = MODE != SPMV_PLAIN ? flag[0] : 0u;
+  const uint32_t s = bid * PRE_STENCILS + threadIdx.x;
+  const bool act = threadIdx.x < PRE_STENCILS && s < C.ns;
+  uint32_t db[4], kp[4];
+  float g[12], w[4], n[3], hp = 0.f, lam = 0.f;
+  float4 zc[4], pc[4];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) { db[b] = NONE; kp[b] = NONE; w[b] = 0.f; }
+  if (act) {
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      db[b] = C.dof[4 * (size_t)s + b];
+      kp[b] = C.pos[4 * (size_t)s + b];
+      w[b] = C.w[4 * (size_t)s + b];
+    }
+#pragma unroll
+    for (int k = 0; k < 12; ++k) g[k] = C.g[12 * (size_t)s + k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) n[k] = C.n[3 * (size_t)s + k];
+    hp = C.coef[2 * (size_t)s];
+    lam = C.coef[2 * (size_t)s + 1];
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+      if (db[b] != NONE) {
+        pc[b] = pold[db[b]];
+        if (MODE == SPMV_NEXT) zc[b] = z[db[b]];
+      }
+  }
+  if (state) return;
   float beta = 0.0f;
   if (MODE == SPMV_NEXT) {
     const double *src[2] = {cur, cur + nz};
-    const uint32_t n[2] = {nz, nz};
+    const uint32_t nn[2] = {nz, nz};
     double v[2];
-    sum_parts_all<BLOCK, 2>(src, n, v, lds);
+    sum_parts_all<BLOCK, 2>(src, nn, v, lds);
     const double b = v[0] / scal[8 + ((j - 1) & 1u)];
     const bool conv = v[1] <= scal[5], capped = j >= flag[2];
     if (bid == 0 && threadIdx.x == 0) {
@@ -192,33 +224,26 @@ __device__ __forceinline__ void pcg_pre_body(uint32_t bid, const ContactView &C,
     if (conv || capped) return;
     beta = (float)b;
   }
-  if (threadIdx.x >= PRE_STENCILS) return;
-  const uint32_t s = bid * PRE_STENCILS + threadIdx.x;
-  if (s >= C.ns) return;
+  if (!act) return;
   float s1 = 0.f, a0 = 0.f, a1 = 0.f, a2 = 0.f;
+#pragma unroll
   for (int b = 0; b < 4; ++b) {
-    const uint32_t db = C.dof[4 * (size_t)s + b];
-    if (db == NONE) continue;
-    const float4 x = pval4<MODE>(z, pold, beta, db);
-    const float *gs = C.g + 12 * (size_t)s + 3 * b;
-    s1 += gs[0] * x.x + gs[1] * x.y + gs[2] * x.z;
-    const float w = C.w[4 * (size_t)s + b];
-    a0 += w * x.x; a1 += w * x.y; a2 += w * x.z;
+    if (db[b] == NONE) continue;
+    const float4 x = MODE == SPMV_NEXT ? pnext(zc[b], pc[b], beta) : pc[b];
+    s1 += g[3 * b] * x.x + g[3 * b + 1] * x.y + g[3 * b + 2] * x.z;
+    a0 += w[b] * x.x; a1 += w[b] * x.y; a2 += w[b] * x.z;
   }
-  const float hp = C.coef[2 * (size_t)s], lam = C.coef[2 * (size_t)s + 1];
   const float t = hp * s1;
-  const float *n = C.n + 3 * (size_t)s;
   const float nd = n[0] * a0 + n[1] * a1 + n[2] * a2;
+#pragma unroll
   for (int a = 0; a < 4; ++a) {
-    const uint32_t k = C.pos[4 * (size_t)s + a];
-    if (k == NONE) continue;
-    const float *ga = C.g + 12 * (size_t)s + 3 * a;
-    float c0 = t * ga[0], c1 = t * ga[1], c2 = t * ga[2];
+    if (kp[a] == NONE) continue;
+    float c0 = t * g[3 * a], c1 = t * g[3 * a + 1], c2 = t * g[3 * a + 2];
     if (lam != 0.0f) {
-      const float pw = lam * C.w[4 * (size_t)s + a];
+      const float pw = lam * w[a];
       c0 += pw * (a0 - n[0] * nd); c1 += pw * (a1 - n[1] * nd); c2 += pw * (a2 - n[2] * nd);
     }
-    C.contrib[k] = make_float4(c0, c1, c2, 0.0f);
+    C.contrib[kp[a]] = make_float4(c0, c1, c2, 0.0f);
   }
 }
 

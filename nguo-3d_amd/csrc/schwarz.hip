@@ -54,7 +54,7 @@ void Schwarz::build(const SellMatrix &M, uint32_t domain_size, hipStream_t s, fl
   dom_row.upload(h_dom_row.empty() ? std::vector<uint32_t>{0} : h_dom_row, s);
   row_dom.upload(rd.empty() ? std::vector<uint32_t>{0} : rd, s);
   row_pos.upload(rp.empty() ? std::vector<uint32_t>{0} : rp, s);
-  ainv.reserve((size_t)std::max(nd, 1u) * MD * MD);
+  ainv.reserve((size_t)std::max(nd, 1u) * SCHWARZ_INV_FLOATS);
   fails.zero(1, s);
   nd1 = 0;
   if (weight > 0.0f && nd > 0) {
@@ -96,9 +96,27 @@ void Schwarz::build(const SellMatrix &M, uint32_t domain_size, hipStream_t s, fl
   }
 }
 
+// The level-0 inverses are stored as the 21 upper 16 x 16 tiles (P <= Q < 6) of the
+// symmetric 96 x 96 block, tile (P, Q) at SCH_TILE * sym_tile(P, Q), row-major; every entry
+// outside the domain's m x m block is zero. 5376 floats per domain instead of 9216.
+constexpr uint32_t SCH_TB = MD / 16;                   // row blocks of 16
+constexpr uint32_t SCH_TILE = 256;                     // floats per tile
+constexpr uint32_t SCH_NT = SCH_TB * (SCH_TB + 1) / 2;  // tiles
+static_assert(MD % 16 == 0, "the tiled inverse needs a whole number of 16-row blocks");
+static_assert(SCHWARZ_INV_FLOATS == SCH_NT * SCH_TILE, "tiled inverse size");
+__host__ __device__ constexpr uint32_t sym_tile(uint32_t P, uint32_t Q) { return P * SCH_TB - P * (P - 1) / 2 + (Q - P); }
+__device__ inline void sym_tile_pq(uint32_t tile, uint32_t &P, uint32_t &Q) {
+  uint32_t rem = tile;
+  P = 0;
+  while (rem >= SCH_TB - P) { rem -= SCH_TB - P; ++P; }
+  Q = P + rem;
+}
+
 // Floor, FP64 Cholesky (diagonal fallback) and explicit FP32 inverse of the dense m x m
-// block in L (row stride m), written to out (row stride MD; columns m..m4-1 zero). All
-// threads of the workgroup call it; the LDS arrays are the caller's.
+// block in L (row stride m), written to out: TILED, as the 21 upper tiles above; otherwise
+// row-major with row stride MD (columns m..m4-1 zero). All threads of the workgroup call it;
+// the LDS arrays are the caller's.
+template <bool TILED>
 __device__ void factor_invert(double *L, double *W, double *diag0, double *colj, double &floor_v, int &failed, uint32_t m, float *out,
                               uint32_t *fails) {
   const uint32_t t = threadIdx.x, ti = t / 16, tj = t % 16;
@@ -139,6 +157,16 @@ __device__ void factor_invert(double *L, double *W, double *diag0, double *colj,
   if (failed) {
     // diagonal fallback: factor sqrt(a_pp) (or 1) -> inverse 1 / a_pp
     if (t == 0) atomicAdd(fails, 1u);
+    if (TILED) {
+      for (uint32_t e = t; e < SCH_NT * SCH_TILE; e += BLOCK) {
+        uint32_t P, Q;
+        sym_tile_pq(e / SCH_TILE, P, Q);
+        const uint32_t p = 16 * P + (e % SCH_TILE) / 16, q = 16 * Q + e % 16;
+        const double v = p < m ? diag0[p] : 0.0;
+        out[e] = (p == q && p < m) ? (float)(v > 1e-14 ? 1.0 / v : 1.0) : 0.0f;
+      }
+      return;
+    }
     for (uint32_t i = t; i < m * m4; i += BLOCK) {
       uint32_t p = i / m4, q = i % m4;
       double v = diag0[p];
@@ -166,28 +194,33 @@ __device__ void factor_invert(double *L, double *W, double *diag0, double *colj,
   // (v_mfma_f64_16x16x4f64; A operand lane a = (row a % 16, k a / 16), B operand lane
   // b = (k b / 16, column b % 16), result lane l register v = (row 4 v + l / 16, column
   // l % 16)). Only tiles with P <= Q are formed (the product is symmetric) and W being lower
-  // triangular, k starts at 16 max(P, Q). Entries outside m x m read as zero, so columns
-  // m..m4-1 of rows < m come out zero (the apply reads float4 groups).
+  // triangular, k starts at 16 max(P, Q). Entries outside m x m read as zero, so they come
+  // out zero: in the row-major form columns m..m4-1 of rows < m (the coarse apply reads no
+  // further), in the tiled form every entry of the 21 tiles (tiles wholly outside run no
+  // k step and store their zero accumulator).
   const uint32_t wave = t / WAVE, lane = t % WAVE;
   auto wget = [&](uint32_t k, uint32_t c) -> double { return (k < m && c < m) ? W[k * m + c] : 0.0; };
-  for (uint32_t tile = wave; tile < 21; tile += BLOCK / WAVE) {
-    // tile -> (P, Q), P <= Q < 6
-    uint32_t P = 0, rem = tile;
-    while (rem >= 6 - P) { rem -= 6 - P; ++P; }
-    const uint32_t Q = P + rem;
-    if (16 * P >= m4 || 16 * Q >= m4) continue;
+  for (uint32_t tile = wave; tile < SCH_NT; tile += BLOCK / WAVE) {
+    uint32_t P, Q;
+    sym_tile_pq(tile, P, Q);
+    if (!TILED && (16 * P >= m4 || 16 * Q >= m4)) continue;
     double4_t acc = {0.0, 0.0, 0.0, 0.0};
-    for (uint32_t kb = 4 * Q; kb < 24 && 4 * kb < m; ++kb) {
+    for (uint32_t kb = 4 * Q; kb < MD / 4 && 4 * kb < m; ++kb) {
       const uint32_t k = 4 * kb + lane / 16;
       const double a = wget(k, 16 * P + lane % 16);
       const double b = wget(k, 16 * Q + lane % 16);
       acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
     }
     for (uint32_t v = 0; v < 4; ++v) {
-      const uint32_t r = 16 * P + 4 * v + lane / 16, c = 16 * Q + lane % 16;
+      const uint32_t i = 4 * v + lane / 16, j = lane % 16;
+      const uint32_t r = 16 * P + i, c = 16 * Q + j;
       const float val = (float)acc[v];
-      if (r < m && c < m4) out[r * MD + c] = val;
-      if (P != Q && c < m && r < m4) out[c * MD + r] = val;
+      if (TILED) {
+        out[SCH_TILE * tile + 16 * i + j] = val;
+      } else {
+        if (r < m && c < m4) out[r * MD + c] = val;
+        if (P != Q && c < m && r < m4) out[c * MD + r] = val;
+      }
     }
   }
 }
@@ -248,7 +281,7 @@ __global__ __launch_bounds__(BLOCK) void k_schwarz_build(SellView A, ContactView
     }
     __syncthreads();
   }
-  factor_invert(L, W, diag0, colj, floor_v, failed, m, ainv + (size_t)d * MD * MD, fails);
+  factor_invert<true>(L, W, diag0, colj, floor_v, failed, m, ainv + (size_t)d * SCHWARZ_INV_FLOATS, fails);
 }
 
 // Coarse block of coarse domain D (linsolve.rs:613-625 coarse_matrix, 230-258 factorize):
@@ -307,7 +340,7 @@ __global__ __launch_bounds__(BLOCK) void k_schwarz_build_coarse(SellView A, Cont
     }
   }
   __syncthreads();
-  factor_invert(L, W, diag0, colj, floor_v, failed, m, ainv1 + (size_t)D * MD * MD, fails);
+  factor_invert<false>(L, W, diag0, colj, floor_v, failed, m, ainv1 + (size_t)D * MD * MD, fails);
 }
 
 // Coarse correction, one workgroup per domain d (member a of coarse domain D): e =
@@ -392,16 +425,18 @@ void Pcg::build_precond(const SellMatrix &M, const ContactView &C) {
 // workgroup reduces pAp (npap SpMV partials) itself, takes rz_j from scal[8 + (j & 1)],
 // stops on breakdown,
 // then applies the PCG update to the domain's rows (r -= alpha q, d += alpha p); INIT:
-// r = (f32) r64, d = 0, p = z. Then z = A_d^-1 r: the domain's FP32 inverse is read as rows
-// of 24 float4 (coalesced 384-byte rows) by 240 threads that split its k range into 10
-// interleaved groups, all of a thread's loads issued before its FMAs; the groups' partial z
-// are summed from LDS in a fixed order. Each row belongs to exactly one domain, so r, d
-// and z have a single writer. Vectors have stride 4 per block row, r64 stride 3.
-// Partials r.z -> nxt[d], r.r -> nxt[nd + d].
-constexpr uint32_t SCH_KG = 10;       // k groups
-constexpr uint32_t SCH_C4 = MD / 4;   // float4 columns of a row
-constexpr uint32_t SCH_KPER = (MD + SCH_KG - 1) / SCH_KG;  // rows per k group
-static_assert(SCH_KG * SCH_C4 <= (uint32_t)BLOCK, "k groups x float4 columns must fit the workgroup");
+// r = (f32) r64, d = 0, p = z. Then z = A_d^-1 r from the tiled inverse: thread t < 168 owns
+// the 4-column quarter t % 4 of rows 2s + (t / 4) % 2 (s < 8) of tile t / 8, eight float4
+// loads issued before anything else (8 adjacent lanes read 128 contiguous bytes); it forms
+// the direct products (row 16P + i of the tile against r's block Q) and, off the diagonal,
+// the transposed ones (column 16Q + j against r's block P). Lanes combine them by shuffles,
+// and row k of z sums its 6 tile partials from LDS in tile order. Each row belongs to exactly
+// one domain, so r, d and z have a single writer. Vectors have stride 4 per block row, r64
+// stride 3. Partials r.z -> nxt[d], r.r -> nxt[nd + d].
+constexpr uint32_t SCH_TPT = 8;                    // threads per tile
+constexpr uint32_t SCH_THREADS = SCH_NT * SCH_TPT;  // 168
+constexpr uint32_t SCH_ROWS = 16 / (SCH_TPT / 4);   // tile rows per thread
+static_assert(SCH_THREADS <= (uint32_t)BLOCK && SCH_TPT == 8, "tile threads must fit the workgroup");
 
 template <bool INIT>
 __device__ __forceinline__ void schwarz_pcg_body(uint32_t d, uint32_t nd, const uint32_t *__restrict__ dom_ptr, const uint32_t *__restrict__ dom_dof,
@@ -409,24 +444,23 @@ __device__ __forceinline__ void schwarz_pcg_body(uint32_t d, uint32_t nd, const 
                                                  float *__restrict__ z, float *__restrict__ dsol, float *__restrict__ p,
                                                  const float *__restrict__ q, const double *pap_part, uint32_t npap, double *nxt, double *scal,
                                                  uint32_t *flag, uint32_t j, float4 *rc) {
-  __shared__ float rs[MD];
-  __shared__ float4 zp[SCH_KG][SCH_C4];
+  __shared__ float4 rs4[MD / 4];
+  __shared__ float zdir[SCH_NT][16];
+  __shared__ float4 ztr[SCH_NT][4];
   __shared__ double lds[2 * BLOCK / WAVE];
+  float *rs = reinterpret_cast<float *>(rs4);
   const uint32_t t = threadIdx.x;
   const uint32_t state = INIT ? 0u : flag[0];
-  // loads that need no scalar: the inverse rows of this thread's k group (rows < MD of the
-  // domain's block: in bounds whatever m is; rows >= m are masked at the FMA), the domain
-  // rows and this thread's r / q / p / d entries
-  const uint32_t c4 = t % SCH_C4, kg = t / SCH_C4;
-  const bool mac = kg < SCH_KG;
-  float4 w[SCH_KPER];
+  // loads that need no scalar: this thread's rows of its tile (every tile entry is written by
+  // the build, zero outside the domain's m x m block), the domain rows and this thread's
+  // r / q / p / d entries
+  const uint32_t tile = t / SCH_TPT, cq = t % 4, h = (t / 4) % 2;
+  const bool mac = t < SCH_THREADS;
+  float4 w[SCH_ROWS];
   {
-    const float4 *a4 = reinterpret_cast<const float4 *>(ainv + (size_t)d * MD * MD) + c4;
+    const float4 *a4 = reinterpret_cast<const float4 *>(ainv + (size_t)d * SCHWARZ_INV_FLOATS) + (size_t)tile * (SCH_TILE / 4) + 4 * h + cq;
 #pragma unroll
-    for (uint32_t i = 0; i < SCH_KPER; ++i) {
-      const uint32_t k = kg + SCH_KG * i;
-      w[i] = mac && k < MD ? a4[(size_t)k * SCH_C4] : make_float4(0.f, 0.f, 0.f, 0.f);
-    }
+    for (uint32_t i = 0; i < SCH_ROWS; ++i) w[i] = mac ? a4[8 * i] : make_float4(0.f, 0.f, 0.f, 0.f);
   }
   const uint32_t base = dom_ptr[d], m = 3 * (dom_ptr[d + 1] - base);
   const uint32_t od = t < MD ? dom_dof[(size_t)d * MD + t] : NONE_S;
@@ -465,6 +499,8 @@ __device__ __forceinline__ void schwarz_pcg_body(uint32_t d, uint32_t nd, const 
     }
     r[o] = rv;
     rs[t] = rv;
+  } else if (t < MD) {
+    rs[t] = 0.0f;
   }
   __syncthreads();
   if (rc && t < 3) {
@@ -473,25 +509,40 @@ __device__ __forceinline__ void schwarz_pcg_body(uint32_t d, uint32_t nd, const 
     for (uint32_t k = t; k < m; k += 3) acc += rs[k];
     reinterpret_cast<float *>(rc)[4 * (size_t)d + t] = acc;
   }
-  const uint32_t m4 = (m + 3) / 4;
-  if (mac && c4 < m4) {
-    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (mac) {
+    uint32_t P, Q;
+    sym_tile_pq(tile, P, Q);
+    const float4 rq = rs4[4 * Q + cq];
+    float dz[SCH_ROWS];
+    float4 tz = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-    for (uint32_t i = 0; i < SCH_KPER; ++i) {
-      const uint32_t k = kg + SCH_KG * i;
-      if (k < m) {
-        const float rk = rs[k];
-        acc.x += w[i].x * rk; acc.y += w[i].y * rk; acc.z += w[i].z * rk; acc.w += w[i].w * rk;
-      }
+    for (uint32_t i = 0; i < SCH_ROWS; ++i) {
+      const float4 a = w[i];
+      dz[i] = a.x * rq.x + a.y * rq.y + a.z * rq.z + a.w * rq.w;
+      const float rp = rs[16 * P + 2 * i + h];
+      tz.x += a.x * rp; tz.y += a.y * rp; tz.z += a.z * rp; tz.w += a.w * rp;
     }
-    zp[kg][c4] = acc;
+    // a + b == b + a, so both lanes of each exchange hold the same sum
+#pragma unroll
+    for (uint32_t i = 0; i < SCH_ROWS; ++i) {
+      dz[i] += __shfl_xor(dz[i], 1);
+      dz[i] += __shfl_xor(dz[i], 2);
+    }
+    tz.x += __shfl_xor(tz.x, 4); tz.y += __shfl_xor(tz.y, 4);
+    tz.z += __shfl_xor(tz.z, 4); tz.w += __shfl_xor(tz.w, 4);
+    if (cq == 0)
+#pragma unroll
+      for (uint32_t i = 0; i < SCH_ROWS; ++i) zdir[tile][2 * i + h] = dz[i];
+    if (h == 0) ztr[tile][cq] = tz;
   }
   __syncthreads();
   double v[2] = {0.0, 0.0};
   if (t < m) {
-    const float *zf = reinterpret_cast<const float *>(&zp[0][0]);
-    float zv = zf[t];
-    for (uint32_t g = 1; g < SCH_KG; ++g) zv += zf[g * MD + t];
+    const uint32_t R = t / 16, k = t % 16;
+    const float *zt = reinterpret_cast<const float *>(&ztr[0][0]);
+    float zv = zdir[sym_tile(R, R)][k];
+    for (uint32_t Q = R + 1; Q < SCH_TB; ++Q) zv += zdir[sym_tile(R, Q)][k];
+    for (uint32_t P = 0; P < R; ++P) zv += zt[16 * sym_tile(P, R) + k];
     z[o] = zv;
     if (INIT) p[o] = zv;
     v[0] = (double)rv * zv;
