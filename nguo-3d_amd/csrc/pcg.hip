@@ -131,21 +131,27 @@ template <int MODE> __device__ inline float4 pval4(const float4 *z, const float4
   return pnext(z[i], pp, beta);
 }
 
-// Loads of a group of slots: the block values and the gathered z / pold of the column.
+// Loads of a group of slots: the block values and the gathered z / pold of the column. Every
+// slot of the group is loaded, with no branch (slots past the slice's width repeat its last
+// slot; padding slots hold a valid column), so all of the group's loads stay in flight
+// together: a load under a branch leaves the compiler unable to count the loads outstanding
+// at the next wait, which then drains all of them, one slot's round trip after another.
+// Needs width >= 1.
 template <int MODE, uint32_t G> struct SlotGroup {
   BlockVals v[G];
   float4 zc[G], pc[G];
-  __device__ inline void load(const SellView &A, const float4 *z, const float4 *pold, uint32_t base, uint32_t k0, uint32_t len, uint32_t lane) {
-    uint32_t cc[G];
+  __device__ inline void load(const SellView &A, const float4 *z, const float4 *pold, uint32_t base, uint32_t k0, uint32_t width, uint32_t lane) {
+    uint32_t grp[G], cc[G];
 #pragma unroll
-    for (uint32_t u = 0; u < G; ++u) cc[u] = k0 + u < len ? A.col[base + 64u * (k0 + u) + lane] : 0u;
+    for (uint32_t u = 0; u < G; ++u) grp[u] = base + 64u * min(k0 + u, width - 1u);
+#pragma unroll
+    for (uint32_t u = 0; u < G; ++u) cc[u] = A.col[grp[u] + lane];
+#pragma unroll
+    for (uint32_t u = 0; u < G; ++u) v[u] = load_block(A.val, grp[u], lane);
 #pragma unroll
     for (uint32_t u = 0; u < G; ++u) {
-      if (k0 + u < len) {
-        v[u] = load_block(A.val, base + 64u * (k0 + u), lane);
-        pc[u] = pold[cc[u]];
-        if (MODE == SPMV_NEXT) zc[u] = z[cc[u]];
-      }
+      pc[u] = pold[cc[u]];
+      if (MODE == SPMV_NEXT) zc[u] = z[cc[u]];
     }
   }
   // y += sum over the group's slots of B p (p = pold, or z + beta pold), slots ascending
@@ -276,19 +282,30 @@ __device__ __forceinline__ void pcg_spmv_body(uint32_t slice, const SellView &A,
   const uint32_t state = MODE != SPMV_PLAIN ? flag[0] : 0u;
   const uint32_t lane = threadIdx.x, row = slice * 64u + lane;
   const bool active = row < A.nb;
-  uint32_t base = 0, width = 0, len = 0, ib = 0, ie = 0;
-  if (active) {
-    base = A.slice_ptr[slice];
-    width = (A.slice_ptr[slice + 1] - base) / 64u;
-    len = A.row_len[row];
-    if (C.ns) { ib = C.inc_ptr[row]; ie = C.inc_ptr[row + 1]; }
+  // lanes past nb (last slice) read row nb - 1 and the slice's padding slots, and add nothing
+  const uint32_t rr = active ? row : A.nb - 1u;
+  const uint32_t base = A.slice_ptr[slice], width = (A.slice_ptr[slice + 1] - base) / 64u;
+  const uint32_t len0 = A.row_len[rr];
+  uint32_t ib = 0, ie = 0;
+  if (C.ns) {
+    ib = C.inc_ptr[rr];
+    ie = C.inc_ptr[rr + 1];
   }
+  const uint32_t len = active ? len0 : 0u;
+  if (!active) ie = ib;
   SlotGroup<MODE, G> g;
-  g.load(A, z, pold, base, 0, len, lane);
+  if (width) g.load(A, z, pold, base, 0, width, lane);
+  float4 prow = make_float4(0.f, 0.f, 0.f, 0.f), zrow = prow;
+  if (MODE != SPMV_PLAIN) {
+    prow = pold[rr];
+    if (MODE == SPMV_NEXT) zrow = z[rr];
+  }
   float4 cv[8];
+  if (C.ns) {
+    // contrib holds at least one entry whenever there are stencils
 #pragma unroll
-  for (uint32_t u = 0; u < 8; ++u)
-    if (ib + u < ie) cv[u] = C.contrib[ib + u];
+    for (uint32_t u = 0; u < 8; ++u) cv[u] = C.contrib[ie > ib ? min(ib + u, ie - 1u) : 0u];
+  }
   if (state) return;
   const float beta = MODE == SPMV_NEXT ? (float)scal[3] : 0.0f;
   double dp = 0.0;
@@ -296,7 +313,7 @@ __device__ __forceinline__ void pcg_spmv_body(uint32_t slice, const SellView &A,
     float y0 = 0.0f, y1 = 0.0f, y2 = 0.0f;
     g.accumulate(y0, y1, y2, beta, 0, len);
     for (uint32_t k0 = G; k0 < width; k0 += G) {
-      g.load(A, z, pold, base, k0, len, lane);
+      g.load(A, z, pold, base, k0, width, lane);
       g.accumulate(y0, y1, y2, beta, k0, len);
     }
 #pragma unroll
@@ -308,7 +325,7 @@ __device__ __forceinline__ void pcg_spmv_body(uint32_t slice, const SellView &A,
     }
     q[row] = make_float4(y0, y1, y2, 0.0f);
     if (MODE != SPMV_PLAIN) {
-      const float4 pr = pval4<MODE>(z, pold, beta, row);
+      const float4 pr = MODE == SPMV_NEXT ? pnext(zrow, prow, beta) : prow;
       if (MODE == SPMV_NEXT) pnew[row] = pr;
       dp = (double)pr.x * y0 + (double)pr.y * y1 + (double)pr.z * y2;
     }
