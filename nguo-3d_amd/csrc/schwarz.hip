@@ -453,26 +453,33 @@ __device__ __forceinline__ void schwarz_pcg_body(uint32_t d, uint32_t nd, const 
   const uint32_t state = INIT ? 0u : flag[0];
   // loads that need no scalar: this thread's rows of its tile (every tile entry is written by
   // the build, zero outside the domain's m x m block), the domain rows and this thread's
-  // r / q / p / d entries
+  // r / q / p / d entries. None sits under a per-thread branch (threads past the tiles read
+  // the last tile, rows past the domain read entry 0, and neither uses what it read), so the
+  // compiler can count the loads in flight and never drains them one by one; the block-wide
+  // test d < nd (always true) keeps them ahead of the state test rather than sunk below it.
   const uint32_t tile = t / SCH_TPT, cq = t % 4, h = (t / 4) % 2;
   const bool mac = t < SCH_THREADS;
   float4 w[SCH_ROWS];
-  {
-    const float4 *a4 = reinterpret_cast<const float4 *>(ainv + (size_t)d * SCHWARZ_INV_FLOATS) + (size_t)tile * (SCH_TILE / 4) + 4 * h + cq;
-#pragma unroll
-    for (uint32_t i = 0; i < SCH_ROWS; ++i) w[i] = mac ? a4[8 * i] : make_float4(0.f, 0.f, 0.f, 0.f);
-  }
-  const uint32_t base = dom_ptr[d], m = 3 * (dom_ptr[d + 1] - base);
-  const uint32_t od = t < MD ? dom_dof[(size_t)d * MD + t] : NONE_S;
-  const size_t o = od;
+  uint32_t od0 = NONE_S;
   float r0 = 0.f, q0 = 0.f, p0 = 0.f, d0 = 0.f;
-  if (od != NONE_S) {
+#pragma unroll
+  for (uint32_t i = 0; i < SCH_ROWS; ++i) w[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  const uint32_t base = dom_ptr[d], m = 3 * (dom_ptr[d + 1] - base);
+  if (d < nd) {
+    const uint32_t tl = mac ? tile : SCH_NT - 1;
+    const float4 *a4 = reinterpret_cast<const float4 *>(ainv + (size_t)d * SCHWARZ_INV_FLOATS) + (size_t)tl * (SCH_TILE / 4) + 4 * h + cq;
+#pragma unroll
+    for (uint32_t i = 0; i < SCH_ROWS; ++i) w[i] = a4[8 * i];
+    od0 = dom_dof[(size_t)d * MD + min(t, MD - 1)];
+    const size_t oc = (t < MD && od0 != NONE_S) ? od0 : 0u;
     if (INIT) {
-      r0 = (float)r64[3 * (o >> 2) + (o & 3)];
+      r0 = (float)r64[3 * (oc >> 2) + (oc & 3)];
     } else {
-      r0 = r[o]; q0 = q[o]; p0 = p[o]; d0 = dsol[o];
+      r0 = r[oc]; q0 = q[oc]; p0 = p[oc]; d0 = dsol[oc];
     }
   }
+  const uint32_t od = t < MD ? od0 : NONE_S;
+  const size_t o = od != NONE_S ? od : 0u;
   if (state) return;
   float alpha = 0.0f;
   if (!INIT) {
