@@ -178,7 +178,7 @@ def main():
               "batched_pcg_ms_per_step": round(b_ms[True], 3) if True in b_ms else None,
               "concurrent_solves_ms_per_step": round(b_ms[False], 3) if False in b_ms else None,
               "form": "one stream and host thread per instance; Newton solves either as one batched PCG over all instances "
-                      "(three launches per iteration) or each on its instance stream"}
+                      "(two launches per iteration) or each on its instance stream"}
 
     c4 = None
     if args.c4_iters > 0:

@@ -217,7 +217,7 @@ int nguo_batch_create(nguo_ctx *ctx, uint32_t n, const nguo_scene_desc *scenes, 
 void nguo_batch_destroy(nguo_batch *batch);
 int nguo_batch_advance(nguo_batch *batch, uint32_t nsteps);
 int nguo_batch_instance(nguo_batch *batch, uint32_t i, nguo_step **out);
-/* batched = 1: the instances' Newton solves run as one batched PCG whose three launches per
+/* batched = 1: the instances' Newton solves run as one batched PCG whose two launches per
  * iteration span all instances (at most 8 per launch); 0 (default): each instance solves on
  * its own stream, concurrently with the others. The results are bitwise the same either way. */
 int nguo_batch_set_solve(nguo_batch *batch, int batched);

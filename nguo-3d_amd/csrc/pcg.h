@@ -68,10 +68,12 @@ struct PcgStats {
 
 // Additive Schwarz preconditioner kernel of the PCG iteration (one workgroup per domain).
 // init: r = (f32) r64, d = 0, z = M^-1 r, p = z. Otherwise iteration j: alpha = rz_j / pAp
-// (npap SpMV partials; rz_j in scal[8 + (j & 1)]), r -= alpha q, d += alpha p, z = M^-1 r.
-// Writes the r.z | r.r partials of the new z to nxt (2 nd).
+// (npap iteration-launch partials; rz_j in scal[8 + (j & 1)]), q += the row's contact blocks
+// contrib[inc_ptr[v] .. inc_ptr[v + 1]) in order (contrib null: no contact term), r -= alpha q,
+// d += alpha p, z = M^-1 r. Writes the r.z | r.r partials of the new z to nxt (2 nd).
 void launch_schwarz_pcg(hipStream_t s, const Schwarz &sz, bool init, const double *r64, float *r, float *z, float *dsol, float *p,
-                        const float *q, const double *pap_part, uint32_t npap, double *nxt, double *scal, uint32_t *flag, uint32_t j);
+                        const float *q, const uint32_t *inc_ptr, const float4 *contrib, const double *pap_part, uint32_t npap, double *nxt,
+                        double *scal, uint32_t *flag, uint32_t j);
 // Iteration form of launch_schwarz_pcg over several one-level systems in one launch.
 void launch_schwarz_pcg_batch(hipStream_t s, const BatchOf<SchArgs> &B, uint32_t j);
 // Two-level form only, right after launch_schwarz_pcg: z += weight C^T S1 (C r) on every
@@ -100,7 +102,7 @@ class Pcg {
   PcgStats solve(const SellMatrix &M, const ContactView &C, const float *b, float tol, uint32_t max_iter, float *x_out);
   // Solves n <= MAX_BATCH systems, each with its own Pcg (one-level Schwarz), together on
   // stream s: refinement sweeps run in lockstep over the systems that still need one, and
-  // each PCG iteration is three launches spanning all of them (a system whose sweep has
+  // each PCG iteration is two launches spanning all of them (a system whose sweep has
   // converged exits those launches at its first instruction). Every system takes exactly
   // the iterations, sweeps and values its own solve() would. The caller orders s after the
   // systems' assembly and its own streams after s.
@@ -109,9 +111,10 @@ class Pcg {
   // y = (A + C) x in f32 (stride-3 vectors), for parity checks.
   void apply(const SellView &A, const ContactView &C, const float *x, float *y);
   // Average duration (ms) of the PCG's SpMV kernel on (A + C) x over `reps` back-to-back
-  // launches, between two HIP events on the solver stream. iteration_form: the instance the
-  // PCG iteration runs (also gathers z, evaluates p = z + beta p_old, stores p, reduces the
-  // previous partials and p.q); otherwise y = (A + C) x alone.
+  // launches, between two HIP events on the solver stream. iteration_form: the iteration
+  // launch (also reduces the preconditioner's partials, gathers z, evaluates p = z + beta
+  // p_old, stores p, evaluates the contact stencils' blocks and the p.q partials); otherwise
+  // y = A x plus the contact blocks (evaluated once, before the timed launches).
   double time_spmv(const SellView &A, const ContactView &C, const float *x, uint32_t reps, bool iteration_form);
   // Block-Jacobi inverse of (A + C)'s diagonal blocks (FP64 inversion, scalar fallback).
   void build_precond(const SellMatrix &M, const ContactView &C);
@@ -129,8 +132,6 @@ class Pcg {
   void ensure(uint32_t nb, uint32_t ns);
   void launch_minv(const SellMatrix &M, const ContactView &C);
   void launch_precond_init(const SellMatrix &M, double *part);
-  void launch_spmv_plain(const SellView &A, const ContactView &C);
-  void launch_pre_plain(const ContactView &C);
   hipStream_t s_;
   Schwarz *sz_ = nullptr;
   uint32_t cap_nb_ = 0, cap_ns_ = 0;

@@ -107,28 +107,27 @@ __global__ __launch_bounds__(BLOCK) void k_contact_sums(ContactView C, const TX 
 // neighbour is one 16-byte load, and a row's three components are one 16-byte store. The
 // fourth component is zero and never enters the arithmetic.
 //
-// One iteration j is three launches:
-//   k_pcg_pre   reduces rz_j and rr_j from the preconditioner's partials (every workgroup,
-//               same order), records the state (workgroup 0) and writes each contact
-//               stencil's blocks of H_s p_j to the incidence slots;
-//   k_pcg_spmv  q = (A + C) p_j, one SELL slice per single-wavefront workgroup, p.q partial
-//               per slice;
-//   preconditioner (k_schwarz_pcg / k_jacobi_pcg) alpha = rz_j / pAp, r -= alpha q,
-//               d += alpha p, z = M^-1 r, partials of r.z and r.r.
+// One iteration j is two launches:
+//   k_pcg_iter  every workgroup reduces rz_j and rr_j from the preconditioner's partials
+//               (same order everywhere; slice workgroup 0 records the state) and forms
+//               p_j = z_j + beta_j p_{j-1} wherever it reads p; the SELL slices' workgroups
+//               (one wavefront, 64 block rows each) write q = A p_j and store p_j, the
+//               stencils' workgroups (64 stencils each) write each contact stencil's blocks
+//               of H_s p_j to its incidence slots; partials of p.q: p.(A p) per slice,
+//               sum_a p_a . (H_s p)_a per stencil workgroup;
+//   preconditioner (k_schwarz_pcg / k_jacobi_pcg) alpha = rz_j / pAp, q += the row's
+//               contact blocks in incidence order, r -= alpha q, d += alpha p, z = M^-1 r,
+//               partials of r.z and r.r.
 // At these sizes a launch costs a few dependent memory round trips, and a workgroup's loads
 // queue on its compute unit's memory pipeline, so the SpMV spreads over one wavefront per
-// workgroup and issues every load of a group of slots before its FMAs.
+// workgroup and issues every load of a group of slots before its FMAs. The plain form (y =
+// (A + C) x, no iteration state) is a stencil launch, then the SpMV adding the blocks.
 constexpr uint32_t SPMV_GROUP = 16;   // default: slots whose loads are in flight together
-constexpr uint32_t PRE_STENCILS = 64; // stencils per k_pcg_pre workgroup
+constexpr uint32_t PRE_STENCILS = 64; // stencils per stencil workgroup (one wavefront)
 enum SpmvMode { SPMV_PLAIN = 0, SPMV_FIRST = 1, SPMV_NEXT = 2 };
 
 __device__ inline float4 pnext(const float4 &zz, const float4 &pp, float beta) {
   return make_float4(zz.x + beta * pp.x, zz.y + beta * pp.y, zz.z + beta * pp.z, 0.0f);
-}
-template <int MODE> __device__ inline float4 pval4(const float4 *z, const float4 *pold, float beta, size_t i) {
-  const float4 pp = pold[i];
-  if (MODE != SPMV_NEXT) return pp;
-  return pnext(z[i], pp, beta);
 }
 
 // Loads of a group of slots: the block values and the gathered z / pold of the column. Every
@@ -169,22 +168,47 @@ template <int MODE, uint32_t G> struct SlotGroup {
   }
 };
 
-// Opens iteration j. NEXT: rz_j, rr_j from the preconditioner's partials cur[0..nz) and
-// cur[nz..2nz), beta_j = rz_j / rz_{j-1} (rz by iteration parity in scal[8 + (j & 1)]);
-// stop on convergence (rr_j <= threshold) or at the cap, workgroup 0 recording the state.
-// FIRST: p_0 = pold. PLAIN: p = pold, no state. Then, PRE_STENCILS stencils per workgroup:
-// the sums g . p and sum_a w_a p_a over the participants in order, p evaluated exactly as
-// k_pcg_spmv evaluates it, and each free participant's block of H_s p,
-// hp (g.p) g_a + lambda w_a (s_w - n (n . s_w)), stored at its incidence slot. The stencil's
-// data and its participants' z / pold entries need no scalar, so they are loaded before the
-// reduction and are in flight while it runs.
+// Opens iteration j in a workgroup of NT threads: rz_j, rr_j from the preconditioner's
+// partials cur[0..nz) and cur[nz..2nz) (summed by the workgroup itself, in the same order as
+// every other), beta_j = rz_j / rz_{j-1} (rz by iteration parity in scal[8 + (j & 1)]). The
+// iteration stops on convergence (rr_j <= threshold) or at the cap; the writer workgroup
+// records the state (a workgroup that reads the new state at its start exits as this test
+// would make it). False when the iteration stops.
+template <int NT>
+__device__ inline bool open_iteration(const double *cur, uint32_t nz, double *scal, uint32_t *flag, uint32_t j, bool writer, double *lds,
+                                      float &beta) {
+  const double *src[2] = {cur, cur + nz};
+  const uint32_t n[2] = {nz, nz};
+  double v[2];
+  sum_parts_all<NT, 2, (NT < BLOCK ? 16u : 4u)>(src, n, v, lds);
+  const double b = v[0] / scal[8 + ((j - 1) & 1u)];
+  const bool conv = v[1] <= scal[5], capped = j >= flag[2];
+  if (writer && threadIdx.x == 0) {
+    flag[1] = j;
+    scal[8 + (j & 1u)] = v[0];
+    scal[3] = b;
+    scal[4] = v[1];
+    if (conv) flag[0] = 1;
+    else if (capped) flag[0] = 3;
+  }
+  beta = (float)b;
+  return !(conv || capped);
+}
+
+// Contact stencils sb * 64 .. sb * 64 + 63, one per lane of a one-wavefront workgroup: the
+// sums g . p and sum_a w_a p_a over the participants in order, p evaluated exactly as the
+// SpMV evaluates it (NEXT: z + beta pold; otherwise pold), and each free participant's block
+// of H_s p, hp (g.p) g_a + lambda w_a (s_w - n (n . s_w)), stored at its incidence slot.
+// Iteration forms also write dpart[sb] = sum over the stencils of sum_a p_a . block_a. The
+// stencil's data and its participants' z / pold entries need no scalar, so they are loaded
+// before the iteration's reduction and are in flight while it runs.
 template <int MODE>
-__device__ __forceinline__ void pcg_pre_body(uint32_t bid, const ContactView &C, const float4 *__restrict__ z, const float4 *__restrict__ pold,
-                                             const double *cur, uint32_t nz, double *scal, uint32_t *flag, uint32_t j) {
-  __shared__ double lds[2 * BLOCK / WAVE];
+__device__ __forceinline__ void pcg_stencil_body(uint32_t sb, const ContactView &C, const float4 *__restrict__ z, const float4 *__restrict__ pold,
+                                                 const double *cur, uint32_t nz, double *scal, uint32_t *flag, uint32_t j, double *dpart) {
+  __shared__ double lds[2];
   const uint32_t state = MODE != SPMV_PLAIN ? flag[0] : 0u;
-  const uint32_t s = bid * PRE_STENCILS + threadIdx.x;
-  const bool act = threadIdx.x < PRE_STENCILS && s < C.ns;
+  const uint32_t s = sb * PRE_STENCILS + threadIdx.x;
+  const bool act = s < C.ns;
   uint32_t db[4], kp[4];
   float g[12], w[4], n[3], hp = 0.f, lam = 0.f;
   float4 zc[4], pc[4];
@@ -212,73 +236,51 @@ __device__ __forceinline__ void pcg_pre_body(uint32_t bid, const ContactView &C,
   }
   if (state) return;
   float beta = 0.0f;
-  if (MODE == SPMV_NEXT) {
-    const double *src[2] = {cur, cur + nz};
-    const uint32_t nn[2] = {nz, nz};
-    double v[2];
-    sum_parts_all<BLOCK, 2>(src, nn, v, lds);
-    const double b = v[0] / scal[8 + ((j - 1) & 1u)];
-    const bool conv = v[1] <= scal[5], capped = j >= flag[2];
-    if (bid == 0 && threadIdx.x == 0) {
-      flag[1] = j;
-      scal[8 + (j & 1u)] = v[0];
-      scal[3] = b;
-      scal[4] = v[1];
-      if (conv) flag[0] = 1;
-      else if (capped) flag[0] = 3;
-    }
-    if (conv || capped) return;
-    beta = (float)b;
-  }
-  if (!act) return;
-  float s1 = 0.f, a0 = 0.f, a1 = 0.f, a2 = 0.f;
+  if (MODE == SPMV_NEXT && !open_iteration<PRE_STENCILS>(cur, nz, scal, flag, j, false, lds, beta)) return;
+  double dp = 0.0;
+  if (act) {
+    float s1 = 0.f, a0 = 0.f, a1 = 0.f, a2 = 0.f;
+    float4 xs[4];
 #pragma unroll
-  for (int b = 0; b < 4; ++b) {
-    if (db[b] == NONE) continue;
-    const float4 x = MODE == SPMV_NEXT ? pnext(zc[b], pc[b], beta) : pc[b];
-    s1 += g[3 * b] * x.x + g[3 * b + 1] * x.y + g[3 * b + 2] * x.z;
-    a0 += w[b] * x.x; a1 += w[b] * x.y; a2 += w[b] * x.z;
-  }
-  const float t = hp * s1;
-  const float nd = n[0] * a0 + n[1] * a1 + n[2] * a2;
-#pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    if (kp[a] == NONE) continue;
-    float c0 = t * g[3 * a], c1 = t * g[3 * a + 1], c2 = t * g[3 * a + 2];
-    if (lam != 0.0f) {
-      const float pw = lam * w[a];
-      c0 += pw * (a0 - n[0] * nd); c1 += pw * (a1 - n[1] * nd); c2 += pw * (a2 - n[2] * nd);
+    for (int b = 0; b < 4; ++b) {
+      xs[b] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (db[b] == NONE) continue;
+      const float4 x = MODE == SPMV_NEXT ? pnext(zc[b], pc[b], beta) : pc[b];
+      xs[b] = x;
+      s1 += g[3 * b] * x.x + g[3 * b + 1] * x.y + g[3 * b + 2] * x.z;
+      a0 += w[b] * x.x; a1 += w[b] * x.y; a2 += w[b] * x.z;
     }
-    C.contrib[kp[a]] = make_float4(c0, c1, c2, 0.0f);
+    const float t = hp * s1;
+    const float nd = n[0] * a0 + n[1] * a1 + n[2] * a2;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      if (kp[a] == NONE) continue;
+      float c0 = t * g[3 * a], c1 = t * g[3 * a + 1], c2 = t * g[3 * a + 2];
+      if (lam != 0.0f) {
+        const float pw = lam * w[a];
+        c0 += pw * (a0 - n[0] * nd); c1 += pw * (a1 - n[1] * nd); c2 += pw * (a2 - n[2] * nd);
+      }
+      C.contrib[kp[a]] = make_float4(c0, c1, c2, 0.0f);
+      dp += (double)xs[a].x * c0 + (double)xs[a].y * c1 + (double)xs[a].z * c2;
+    }
   }
+  if (MODE == SPMV_PLAIN) return;
+  dp = wave_sum(dp);
+  if (threadIdx.x == 0) dpart[sb] = dp;
 }
 
-template <int MODE>
-__global__ __launch_bounds__(BLOCK) void k_pcg_pre(ContactView C, const float4 *__restrict__ z, const float4 *__restrict__ pold, const double *cur,
-                                                  uint32_t nz, double *scal, uint32_t *flag, uint32_t j) {
-  pcg_pre_body<MODE>(blockIdx.x, C, z, pold, cur, nz, scal, flag, j);
-}
-
-// The same over several systems (one grid, system i's workgroups at B.off[i]).
-template <int MODE> __global__ __launch_bounds__(BLOCK) void kb_pcg_pre(BatchOf<PreArgs> B, uint32_t j) {
-  const uint32_t i = batch_slot(B.off, B.k, blockIdx.x);
-  const PreArgs &a = B.a[i];
-  pcg_pre_body<MODE>(blockIdx.x - B.off[i], a.C, a.z, a.pold, a.cur, a.nz, a.scal, a.flag, j);
-}
-
-inline unsigned pre_grid(const ContactView &C) { return std::max(1u, div_up(C.ns, PRE_STENCILS)); }
-
-// q = (A + C) p over one SELL slice (64 block rows, one per lane) per single-wavefront
-// workgroup: SPMV_GROUP slots at a time, column indices, then every value and gather load of
-// the group, then the FMAs in slot order; the row's contact blocks from k_pcg_pre added in
-// incidence order. PLAIN: p = pold, no partial. FIRST: p_0 = pold. NEXT: p_j = z + beta pold
-// (beta recorded by k_pcg_pre) wherever p is read; the row's owner also stores p_j to pnew,
-// and every reader of a neighbour's p evaluates the same expression, so they agree bitwise.
+// A SELL slice (64 block rows, one per lane) of a one-wavefront workgroup: SPMV_GROUP slots at
+// a time, column indices, then every value and gather load of the group, then the FMAs in
+// slot order. PLAIN: q = (A + C) pold, the row's contact blocks (from the stencil launch)
+// added in incidence order. FIRST: q = A p_0 (p_0 = pold). NEXT: q = A p_j, p_j = z + beta
+// pold wherever p is read; the row's owner also stores p_j to pnew, and every reader of a
+// neighbour's p evaluates the same expression, so they agree bitwise. Iteration forms write
 // part[slice] = p.q over the slice.
 template <int MODE, uint32_t G>
 __device__ __forceinline__ void pcg_spmv_body(uint32_t slice, const SellView &A, const ContactView &C, const float4 *__restrict__ z,
                                               const float4 *__restrict__ pold, float4 *__restrict__ pnew, float4 *__restrict__ q, double *part,
-                                              const double *scal, const uint32_t *flag) {
+                                              const double *cur, uint32_t nz, double *scal, uint32_t *flag, uint32_t j) {
+  __shared__ double lds[2];
   const uint32_t state = MODE != SPMV_PLAIN ? flag[0] : 0u;
   const uint32_t lane = threadIdx.x, row = slice * 64u + lane;
   const bool active = row < A.nb;
@@ -287,7 +289,7 @@ __device__ __forceinline__ void pcg_spmv_body(uint32_t slice, const SellView &A,
   const uint32_t base = A.slice_ptr[slice], width = (A.slice_ptr[slice + 1] - base) / 64u;
   const uint32_t len0 = A.row_len[rr];
   uint32_t ib = 0, ie = 0;
-  if (C.ns) {
+  if (MODE == SPMV_PLAIN && C.ns) {
     ib = C.inc_ptr[rr];
     ie = C.inc_ptr[rr + 1];
   }
@@ -301,13 +303,14 @@ __device__ __forceinline__ void pcg_spmv_body(uint32_t slice, const SellView &A,
     if (MODE == SPMV_NEXT) zrow = z[rr];
   }
   float4 cv[8];
-  if (C.ns) {
+  if (MODE == SPMV_PLAIN && C.ns) {
     // contrib holds at least one entry whenever there are stencils
 #pragma unroll
     for (uint32_t u = 0; u < 8; ++u) cv[u] = C.contrib[ie > ib ? min(ib + u, ie - 1u) : 0u];
   }
   if (state) return;
-  const float beta = MODE == SPMV_NEXT ? (float)scal[3] : 0.0f;
+  float beta = 0.0f;
+  if (MODE == SPMV_NEXT && !open_iteration<WAVE>(cur, nz, scal, flag, j, slice == 0, lds, beta)) return;
   double dp = 0.0;
   if (active) {
     float y0 = 0.0f, y1 = 0.0f, y2 = 0.0f;
@@ -316,12 +319,14 @@ __device__ __forceinline__ void pcg_spmv_body(uint32_t slice, const SellView &A,
       g.load(A, z, pold, base, k0, width, lane);
       g.accumulate(y0, y1, y2, beta, k0, len);
     }
+    if (MODE == SPMV_PLAIN) {
 #pragma unroll
-    for (uint32_t u = 0; u < 8; ++u)
-      if (ib + u < ie) { y0 += cv[u].x; y1 += cv[u].y; y2 += cv[u].z; }
-    for (uint32_t k = ib + 8; k < ie; ++k) {
-      const float4 c = C.contrib[k];
-      y0 += c.x; y1 += c.y; y2 += c.z;
+      for (uint32_t u = 0; u < 8; ++u)
+        if (ib + u < ie) { y0 += cv[u].x; y1 += cv[u].y; y2 += cv[u].z; }
+      for (uint32_t k = ib + 8; k < ie; ++k) {
+        const float4 c = C.contrib[k];
+        y0 += c.x; y1 += c.y; y2 += c.z;
+      }
     }
     q[row] = make_float4(y0, y1, y2, 0.0f);
     if (MODE != SPMV_PLAIN) {
@@ -335,31 +340,53 @@ __device__ __forceinline__ void pcg_spmv_body(uint32_t slice, const SellView &A,
   if (lane == 0) part[slice] = dp;
 }
 
-template <int MODE, uint32_t G>
-__global__ __launch_bounds__(WAVE) void k_pcg_spmv(SellView A, ContactView C, const float4 *__restrict__ z, const float4 *__restrict__ pold,
-                                                  float4 *__restrict__ pnew, float4 *__restrict__ q, double *part, const double *scal,
-                                                  const uint32_t *flag) {
-  pcg_spmv_body<MODE, G>(blockIdx.x, A, C, z, pold, pnew, q, part, scal, flag);
+// Workgroup b of a system's iteration launch: a slice (b < slices) or a stencil group.
+template <int MODE, uint32_t G> __device__ __forceinline__ void pcg_iter_body(uint32_t b, const SpmvArgs &a, uint32_t j) {
+  const uint32_t ns = (a.A.nb + 63u) / 64u;
+  if (b < ns)
+    pcg_spmv_body<MODE, G>(b, a.A, a.C, a.z, a.pold, a.pnew, a.q, a.part, a.cur, a.nz, a.scal, a.flag, j);
+  else
+    pcg_stencil_body<MODE>(b - ns, a.C, a.z, a.pold, a.cur, a.nz, a.scal, a.flag, j, a.part + ns);
 }
 
-template <int MODE, uint32_t G> __global__ __launch_bounds__(WAVE) void kb_pcg_spmv(BatchOf<SpmvArgs> B) {
-  const uint32_t i = batch_slot(B.off, B.k, blockIdx.x);
-  const SpmvArgs &a = B.a[i];
-  pcg_spmv_body<MODE, G>(blockIdx.x - B.off[i], a.A, a.C, a.z, a.pold, a.pnew, a.q, a.part, a.scal, a.flag);
+template <int MODE, uint32_t G> __global__ __launch_bounds__(WAVE) void k_pcg_iter(SpmvArgs a, uint32_t j) {
+  pcg_iter_body<MODE, G>(blockIdx.x, a, j);
 }
+
+// The same over several systems (one grid, system i's workgroups at B.off[i]).
+template <int MODE, uint32_t G> __global__ __launch_bounds__(WAVE) void kb_pcg_iter(BatchOf<SpmvArgs> B, uint32_t j) {
+  const uint32_t i = batch_slot(B.off, B.k, blockIdx.x);
+  pcg_iter_body<MODE, G>(blockIdx.x - B.off[i], B.a[i], j);
+}
+
+template <uint32_t G>
+__global__ __launch_bounds__(WAVE) void k_pcg_spmv_plain(SellView A, ContactView C, const float4 *__restrict__ x, float4 *__restrict__ y) {
+  pcg_spmv_body<SPMV_PLAIN, G>(blockIdx.x, A, C, nullptr, x, nullptr, y, nullptr, nullptr, 0u, nullptr, nullptr, 0u);
+}
+__global__ __launch_bounds__(WAVE) void k_pcg_stencils_plain(ContactView C, const float4 *__restrict__ x) {
+  pcg_stencil_body<SPMV_PLAIN>(blockIdx.x, C, nullptr, x, nullptr, 0u, nullptr, nullptr, 0u, nullptr);
+}
+
+inline unsigned stencil_groups(const ContactView &C) { return div_up(C.ns, PRE_STENCILS); }
+inline unsigned iter_grid(const SellView &A, const ContactView &C) { return div_up(A.nb, 64) + stencil_groups(C); }
 
 // Host-side dispatch over the slot-group sizes the SpMV is instantiated for; the group only
 // changes how many slots' loads are in flight at once (and so the registers and occupancy),
 // not the order of the row's sum, so every size gives bitwise the same q.
-template <int MODE>
-static void launch_spmv(uint32_t G, hipStream_t s, unsigned grid, const SellView &A, const ContactView &C, const float4 *z, const float4 *pold,
-                        float4 *pnew, float4 *q, double *part, const double *scal, const uint32_t *flag) {
-  if (G == 8) hipLaunchKernelGGL((k_pcg_spmv<MODE, 8>), dim3(grid), dim3(WAVE), 0, s, A, C, z, pold, pnew, q, part, scal, flag);
-  else hipLaunchKernelGGL((k_pcg_spmv<MODE, 16>), dim3(grid), dim3(WAVE), 0, s, A, C, z, pold, pnew, q, part, scal, flag);
+template <int MODE> static void launch_iter(uint32_t G, hipStream_t s, const SpmvArgs &a, uint32_t j) {
+  const unsigned grid = iter_grid(a.A, a.C);
+  if (G == 8) hipLaunchKernelGGL((k_pcg_iter<MODE, 8>), dim3(grid), dim3(WAVE), 0, s, a, j);
+  else hipLaunchKernelGGL((k_pcg_iter<MODE, 16>), dim3(grid), dim3(WAVE), 0, s, a, j);
 }
-template <int MODE> static void launch_spmv_batch(uint32_t G, hipStream_t s, unsigned grid, const BatchOf<SpmvArgs> &B) {
-  if (G == 8) hipLaunchKernelGGL((kb_pcg_spmv<MODE, 8>), dim3(grid), dim3(WAVE), 0, s, B);
-  else hipLaunchKernelGGL((kb_pcg_spmv<MODE, 16>), dim3(grid), dim3(WAVE), 0, s, B);
+template <int MODE> static void launch_iter_batch(uint32_t G, hipStream_t s, const BatchOf<SpmvArgs> &B, uint32_t j) {
+  if (!B.k || !B.off[B.k]) return;
+  if (G == 8) hipLaunchKernelGGL((kb_pcg_iter<MODE, 8>), dim3(B.off[B.k]), dim3(WAVE), 0, s, B, j);
+  else hipLaunchKernelGGL((kb_pcg_iter<MODE, 16>), dim3(B.off[B.k]), dim3(WAVE), 0, s, B, j);
+}
+static void launch_plain(uint32_t G, hipStream_t s, const SellView &A, const ContactView &C, const float4 *x, float4 *y) {
+  if (C.ns) hipLaunchKernelGGL(k_pcg_stencils_plain, dim3(stencil_groups(C)), dim3(WAVE), 0, s, C, x);
+  if (G == 8) hipLaunchKernelGGL((k_pcg_spmv_plain<8>), dim3(div_up(A.nb, 64)), dim3(WAVE), 0, s, A, C, x, y);
+  else hipLaunchKernelGGL((k_pcg_spmv_plain<16>), dim3(div_up(A.nb, 64)), dim3(WAVE), 0, s, A, C, x, y);
 }
 constexpr uint32_t SPMV_GROUP_CHOICES[2] = {8, 16};
 
@@ -396,14 +423,16 @@ template <class L> static uint32_t pick_group(hipStream_t s, L launch) {
 
 // 3x3 block-Jacobi preconditioner step, one block row per thread (vectors of stride 4,
 // r64 of stride 3). INIT: r = (f32) r64, d = 0, z = Minv r, p = z. Otherwise: pAp from the
-// npap SpMV partials (every workgroup, same order), rz_j from scal[8 + (j & 1)], stop on
-// breakdown, r -= alpha q, d += alpha p, z = Minv r.
+// npap iteration-launch partials (every workgroup, same order), rz_j from scal[8 + (j & 1)],
+// stop on breakdown, q += the row's contact blocks (incidence order; contrib null without
+// contact), r -= alpha q, d += alpha p, z = Minv r.
 // Partials r.z -> nxt[block], r.r -> nxt[gridDim + block].
 template <bool INIT>
 __global__ __launch_bounds__(BLOCK) void k_jacobi_pcg(uint32_t nb, const float *__restrict__ minv, const double *__restrict__ r64,
                                                      float *__restrict__ r, float *__restrict__ z, float *__restrict__ d,
-                                                     float *__restrict__ p, const float *__restrict__ q, const double *pap_part,
-                                                     uint32_t npap, double *nxt, double *scal, uint32_t *flag, uint32_t j) {
+                                                     float *__restrict__ p, const float *__restrict__ q, const uint32_t *inc_ptr,
+                                                     const float4 *contrib, const double *pap_part, uint32_t npap, double *nxt, double *scal,
+                                                     uint32_t *flag, uint32_t j) {
   __shared__ double lds[2 * BLOCK / WAVE];
   float alpha = 0.0f;
   if (!INIT) {
@@ -429,7 +458,13 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_pcg(uint32_t nb, const float *
       r0 = (float)r64[3 * (size_t)i]; r1 = (float)r64[3 * (size_t)i + 1]; r2 = (float)r64[3 * (size_t)i + 2];
       d[o] = 0.f; d[o + 1] = 0.f; d[o + 2] = 0.f;
     } else {
-      r0 = r[o] - alpha * q[o]; r1 = r[o + 1] - alpha * q[o + 1]; r2 = r[o + 2] - alpha * q[o + 2];
+      float q0 = q[o], q1 = q[o + 1], q2 = q[o + 2];
+      if (contrib)
+        for (uint32_t k = inc_ptr[i]; k < inc_ptr[i + 1]; ++k) {
+          const float4 c = contrib[k];
+          q0 += c.x; q1 += c.y; q2 += c.z;
+        }
+      r0 = r[o] - alpha * q0; r1 = r[o + 1] - alpha * q1; r2 = r[o + 2] - alpha * q2;
       d[o] += alpha * p[o]; d[o + 1] += alpha * p[o + 1]; d[o + 2] += alpha * p[o + 2];
     }
     r[o] = r0; r[o + 1] = r1; r[o + 2] = r2;
@@ -585,9 +620,10 @@ Pcg::~Pcg() {
 
 void Pcg::ensure(uint32_t nb, uint32_t ns) {
   {
-    // SpMV partials (one per slice), then the (r.z | r.r) partials of the preconditioner
-    // kernel (per row block or per domain); the FP64 residual reuses the front
-    size_t S = div_up(nb, 64), G = div_up(nb, BLOCK), nz = std::max<size_t>(G, sz_ ? sz_->nd : 0);
+    // iteration-launch partials (one per slice and per stencil group), then the (r.z | r.r)
+    // partials of the preconditioner kernel (per row block or per domain); the FP64 residual
+    // reuses the front
+    size_t S = div_up(nb, 64) + div_up(ns, PRE_STENCILS), G = div_up(nb, BLOCK), nz = std::max<size_t>(G, sz_ ? sz_->nd : 0);
     part_.reserve(std::max(S, G) + 2 * nz + 8);
   }
   if (!h_flag_) {
@@ -630,13 +666,13 @@ __global__ __launch_bounds__(BLOCK) void k_bj_apply(uint32_t nb, const float *mi
 // Opens a sweep: r = (f32) r64, d = 0, z = M^-1 r, p_ = z; (r.z | r.r) partials -> part.
 void Pcg::launch_precond_init(const SellMatrix &M, double *part) {
   if (sz_) {
-    launch_schwarz_pcg(s_, *sz_, true, r64_.p, r_.p, z_.p, d_.p, p_.p, q_.p, nullptr, 0, part, scal_.p, flag_.p, 0);
+    launch_schwarz_pcg(s_, *sz_, true, r64_.p, r_.p, z_.p, d_.p, p_.p, q_.p, nullptr, nullptr, nullptr, 0, part, scal_.p, flag_.p, 0);
     launch_schwarz_coarse(s_, *sz_, true, r_.p, z_.p, p_.p, part, flag_.p);
     return;
   }
   hipLaunchKernelGGL(k_jacobi_pcg<true>, dim3(div_up(M.nb, BLOCK)), dim3(BLOCK), 0, s_, M.nb, (const float *)minv_.p,
-                     (const double *)r64_.p, r_.p, z_.p, d_.p, p_.p, (const float *)q_.p, (const double *)nullptr, 0u, part, scal_.p, flag_.p,
-                     0u);
+                     (const double *)r64_.p, r_.p, z_.p, d_.p, p_.p, (const float *)q_.p, (const uint32_t *)nullptr, (const float4 *)nullptr,
+                     (const double *)nullptr, 0u, part, scal_.p, flag_.p, 0u);
 }
 
 void Pcg::apply_precond(const SellMatrix &M, const float *r, float *z) {
@@ -676,21 +712,24 @@ PcgStats Pcg::solve(const SellMatrix &M, const ContactView &C, const float *b, f
     return st;
   }
   const uint32_t nz = sz_ ? sz_->nd : G;  // preconditioner partials: per domain or per row block
-  const unsigned GP = pre_grid(C);
-  double *part_spmv = part_.p, *part_pc = part_.p + std::max(NS, G);
+  const unsigned NI = iter_grid(A, C);     // iteration-launch partials: slices, then stencil groups
+  double *part_spmv = part_.p, *part_pc = part_.p + std::max(NI, G);
+  const float4 *contrib = C.ns ? (const float4 *)C.contrib : nullptr;
   for (uint32_t sweep = 0; sweep < 12; ++sweep) {
     double res = std::sqrt(rr / bb);
     if (res <= (double)tol || st.iters >= max_iter) break;
     double eta = std::min(0.5, std::max(0.25 * (double)tol / res, REFINE_ETA_MIN));
     launch_precond_init(M, part_pc);
     hipLaunchKernelGGL(k_init_scal, dim3(1), dim3(BLOCK), 0, s_, scal_.p, flag_.p, (const double *)part_pc, nz, eta * eta, max_iter - st.iters);
+    // iteration j: p_j in p_ (even j) or p2_ (odd j); FIRST reads p_0 = z_0 from p_
+    const SpmvArgs first{A, C, (const float4 *)z_.p, (const float4 *)p_.p, nullptr, (float4 *)q_.p, part_spmv, part_pc, nz, scal_.p, flag_.p};
+    SpmvArgs next_odd = first, next_even = first;
+    next_odd.pold = (const float4 *)p_.p; next_odd.pnew = (float4 *)p2_.p;
+    next_even.pold = (const float4 *)p2_.p; next_even.pnew = (float4 *)p_.p;
     if (tuned_nb_ != nb) {
-      // autotune on the live system: the first iteration's SpMV (idempotent; iteration 0
+      // autotune on the live system: the first iteration's launch (idempotent; iteration 0
       // recomputes its q and partials below)
-      spmv_group_ = pick_group(s_, [&](uint32_t G) {
-        launch_spmv<SPMV_FIRST>(G, s_, NS, A, C, (const float4 *)z_.p, (const float4 *)p_.p, (float4 *)nullptr, (float4 *)q_.p, part_spmv,
-                                (const double *)scal_.p, (const uint32_t *)flag_.p);
-      });
+      spmv_group_ = pick_group(s_, [&](uint32_t G) { launch_iter<SPMV_FIRST>(G, s_, first, 0); });
       tuned_nb_ = nb;
     }
     // Chunks of `chunk` iterations are queued one ahead of the one whose state the host
@@ -699,25 +738,15 @@ PcgStats Pcg::solve(const SellMatrix &M, const ContactView &C, const float *b, f
     uint32_t j = 0, queued = 0, waited = 0;
     auto queue_chunk = [&]() {
       for (uint32_t c = 0; c < chunk; ++c, ++j) {
-        // iteration j: p_j in p_ (even j) or p2_ (odd j)
-        float *pj = (j & 1) ? p2_.p : p_.p, *pprev = (j & 1) ? p_.p : p2_.p;
-        if (j == 0) {
-          if (C.ns)
-            hipLaunchKernelGGL(k_pcg_pre<SPMV_FIRST>, dim3(GP), dim3(BLOCK), 0, s_, C, (const float4 *)z_.p, (const float4 *)pj,
-                               (const double *)part_pc, nz, scal_.p, flag_.p, j);
-          launch_spmv<SPMV_FIRST>(spmv_group_, s_, NS, A, C, (const float4 *)z_.p, (const float4 *)pj, (float4 *)nullptr, (float4 *)q_.p, part_spmv,
-                                  (const double *)scal_.p, (const uint32_t *)flag_.p);
-        } else {
-          hipLaunchKernelGGL(k_pcg_pre<SPMV_NEXT>, dim3(GP), dim3(BLOCK), 0, s_, C, (const float4 *)z_.p, (const float4 *)pprev,
-                             (const double *)part_pc, nz, scal_.p, flag_.p, j);
-          launch_spmv<SPMV_NEXT>(spmv_group_, s_, NS, A, C, (const float4 *)z_.p, (const float4 *)pprev, (float4 *)pj, (float4 *)q_.p, part_spmv,
-                                 (const double *)scal_.p, (const uint32_t *)flag_.p);
-        }
+        float *pj = (j & 1) ? p2_.p : p_.p;
+        if (j == 0) launch_iter<SPMV_FIRST>(spmv_group_, s_, first, j);
+        else launch_iter<SPMV_NEXT>(spmv_group_, s_, (j & 1) ? next_odd : next_even, j);
         if (sz_) {
-          launch_schwarz_pcg(s_, *sz_, false, nullptr, r_.p, z_.p, d_.p, pj, q_.p, part_spmv, NS, part_pc, scal_.p, flag_.p, j);
+          launch_schwarz_pcg(s_, *sz_, false, nullptr, r_.p, z_.p, d_.p, pj, q_.p, C.inc_ptr, contrib, part_spmv, NI, part_pc, scal_.p, flag_.p, j);
           launch_schwarz_coarse(s_, *sz_, false, r_.p, z_.p, pj, part_pc, flag_.p);
         } else hipLaunchKernelGGL(k_jacobi_pcg<false>, dim3(G), dim3(BLOCK), 0, s_, nb, (const float *)minv_.p, (const double *)nullptr, r_.p,
-                                z_.p, d_.p, pj, (const float *)q_.p, (const double *)part_spmv, NS, part_pc, scal_.p, flag_.p, j);
+                                z_.p, d_.p, pj, (const float *)q_.p, C.inc_ptr, contrib, (const double *)part_spmv, NI, part_pc, scal_.p,
+                                flag_.p, j);
       }
       NGUO_HIP(hipGetLastError());
       const uint32_t sl = queued & 1u;
@@ -804,61 +833,48 @@ void Pcg::solve_batch(hipStream_t s, PcgBatchItem *items, uint32_t n) {
       act.push_back(i);
     }
     if (act.empty()) break;
-    BatchOf<PreArgs> bf{}, bn{};
     BatchOf<SpmvArgs> sf{}, sn0{}, sn1{};
     BatchOf<SchArgs> c0{}, c1{};
     const uint32_t k = (uint32_t)act.size();
-    bf.k = bn.k = sf.k = sn0.k = sn1.k = c0.k = c1.k = k;
+    sf.k = sn0.k = sn1.k = c0.k = c1.k = k;
     for (uint32_t a = 0; a < k; ++a) {
       PcgBatchItem &it = items[act[a]];
       Pcg &P = *it.pcg;
-      const uint32_t nb = it.M->nb, NS = div_up(nb, 64), nd = P.sz_->nd;
+      const uint32_t nb = it.M->nb, nd = P.sz_->nd;
+      const SellView A = it.M->view();
+      const unsigned NI = iter_grid(A, it.C);
       const double res = std::sqrt(rr[act[a]] / bb[act[a]]);
       const double eta = std::min(0.5, std::max(0.25 * (double)it.tol / res, REFINE_ETA_MIN));
-      double *part_spmv = P.part_.p, *part_pc = P.part_.p + std::max<uint32_t>(NS, div_up(nb, BLOCK));
+      double *part_spmv = P.part_.p, *part_pc = P.part_.p + std::max<uint32_t>(NI, div_up(nb, BLOCK));
       P.launch_precond_init(*it.M, part_pc);
       hipLaunchKernelGGL(k_init_scal, dim3(1), dim3(BLOCK), 0, s, P.scal_.p, P.flag_.p, (const double *)part_pc, nd, eta * eta,
                          it.max_iter - it.stats.iters);
-      const SellView A = it.M->view();
       const float4 *z4 = (const float4 *)P.z_.p;
       float4 *p4 = (float4 *)P.p_.p, *p24 = (float4 *)P.p2_.p;
       // FIRST (j = 0): p_0 in p_; NEXT with j odd: pold = p_, pnew = p2_; j even: the reverse
-      bf.a[a] = PreArgs{it.C, z4, p4, part_pc, nd, P.scal_.p, P.flag_.p};
-      bn.a[a] = PreArgs{it.C, z4, nullptr, part_pc, nd, P.scal_.p, P.flag_.p};
-      sf.a[a] = SpmvArgs{A, it.C, z4, p4, nullptr, (float4 *)P.q_.p, part_spmv, P.scal_.p, P.flag_.p};
-      sn1.a[a] = SpmvArgs{A, it.C, z4, p4, p24, (float4 *)P.q_.p, part_spmv, P.scal_.p, P.flag_.p};
-      sn0.a[a] = SpmvArgs{A, it.C, z4, p24, p4, (float4 *)P.q_.p, part_spmv, P.scal_.p, P.flag_.p};
-      c0.a[a] = SchArgs{nd, P.sz_->dom_ptr.p, P.sz_->dom_dof.p, P.sz_->ainv.p, P.r_.p, P.z_.p, P.d_.p, P.p_.p, P.q_.p, part_spmv, NS, part_pc,
-                        P.scal_.p, P.flag_.p};
+      sf.a[a] = SpmvArgs{A, it.C, z4, p4, nullptr, (float4 *)P.q_.p, part_spmv, part_pc, nd, P.scal_.p, P.flag_.p};
+      sn1.a[a] = sf.a[a];
+      sn1.a[a].pnew = p24;
+      sn0.a[a] = sf.a[a];
+      sn0.a[a].pold = p24;
+      sn0.a[a].pnew = p4;
+      c0.a[a] = SchArgs{nd, P.sz_->dom_ptr.p, P.sz_->dom_dof.p, P.sz_->ainv.p, P.r_.p, P.z_.p, P.d_.p, P.p_.p, P.q_.p, it.C.inc_ptr,
+                        it.C.ns ? (const float4 *)it.C.contrib : nullptr, part_spmv, NI, part_pc, P.scal_.p, P.flag_.p};
       c1.a[a] = c0.a[a];
       c1.a[a].p = P.p2_.p;
-      bf.off[a + 1] = bf.off[a] + (it.C.ns ? pre_grid(it.C) : 0u);
-      bn.off[a + 1] = bn.off[a] + pre_grid(it.C);
-      sf.off[a + 1] = sn0.off[a + 1] = sn1.off[a + 1] = sf.off[a] + NS;
+      sf.off[a + 1] = sn0.off[a + 1] = sn1.off[a + 1] = sf.off[a] + NI;
       c0.off[a + 1] = c1.off[a + 1] = c0.off[a] + nd;
-    }
-    // pold of the NEXT pre launch by parity (pre reads p_{j-1}: j odd -> p_, j even -> p2_)
-    BatchOf<PreArgs> bn1 = bn, bn0 = bn;
-    for (uint32_t a = 0; a < k; ++a) {
-      Pcg &P = *items[act[a]].pcg;
-      bn1.a[a].pold = (const float4 *)P.p_.p;
-      bn0.a[a].pold = (const float4 *)P.p2_.p;
     }
     uint32_t &group = items[0].pcg->batch_group_;
     if (items[0].pcg->batch_tuned_ != sf.off[k]) {
-      group = pick_group(s, [&](uint32_t G) { launch_spmv_batch<SPMV_FIRST>(G, s, sf.off[k], sf); });
+      group = pick_group(s, [&](uint32_t G) { launch_iter_batch<SPMV_FIRST>(G, s, sf, 0); });
       items[0].pcg->batch_tuned_ = sf.off[k];
     }
     uint32_t j = 0, queued = 0, waited = 0;
     auto queue_chunk = [&]() {
       for (uint32_t c = 0; c < chunk; ++c, ++j) {
-        if (j == 0) {
-          if (bf.off[k]) hipLaunchKernelGGL(kb_pcg_pre<SPMV_FIRST>, dim3(bf.off[k]), dim3(BLOCK), 0, s, bf, j);
-          launch_spmv_batch<SPMV_FIRST>(group, s, sf.off[k], sf);
-        } else {
-          hipLaunchKernelGGL(kb_pcg_pre<SPMV_NEXT>, dim3(bn.off[k]), dim3(BLOCK), 0, s, (j & 1) ? bn1 : bn0, j);
-          launch_spmv_batch<SPMV_NEXT>(group, s, sf.off[k], (j & 1) ? sn1 : sn0);
-        }
+        if (j == 0) launch_iter_batch<SPMV_FIRST>(group, s, sf, j);
+        else launch_iter_batch<SPMV_NEXT>(group, s, (j & 1) ? sn1 : sn0, j);
         launch_schwarz_pcg_batch(s, (j & 1) ? c1 : c0, j);
       }
       NGUO_HIP(hipGetLastError());
@@ -917,51 +933,48 @@ void Pcg::apply(const SellView &A, const ContactView &C, const float *x, float *
   if (!A.nb) return;
   const unsigned G = div_up(A.nb, BLOCK);
   hipLaunchKernelGGL(k_pack4, dim3(G), dim3(BLOCK), 0, s_, A.nb, x, (float4 *)p2_.p);
-  if (C.ns) launch_pre_plain(C);
-  launch_spmv_plain(A, C);
+  launch_plain(spmv_group_, s_, A, C, (const float4 *)p2_.p, (float4 *)q_.p);
   hipLaunchKernelGGL(k_unpack4, dim3(G), dim3(BLOCK), 0, s_, A.nb, (const float4 *)q_.p, y);
   NGUO_HIP(hipGetLastError());
 }
 
-// State under which the iteration-form SpMV runs without stopping: running, no cap, beta 1.
-__global__ void k_bench_state(uint32_t *flag, double *scal) {
+// State under which the iteration launch runs without stopping: running, no cap, no
+// convergence threshold, rz_{j-1} = 1 and zero preconditioner partials (beta 0).
+__global__ void k_bench_state(uint32_t *flag, double *scal, double *cur, uint32_t ncur) {
+  for (uint32_t i = threadIdx.x; i < ncur; i += blockDim.x) cur[i] = 0.0;
   if (threadIdx.x == 0) {
     flag[0] = 0u; flag[1] = 0u; flag[2] = 0xffffffffu;
-    scal[3] = 1.0;
+    scal[5] = -1.0;
+    scal[8] = 1.0; scal[9] = 1.0;
   }
 }
 
 double Pcg::time_spmv(const SellView &A, const ContactView &C, const float *x, uint32_t reps, bool iteration_form) {
   ensure(A.nb, C.ns);
   if (!A.nb || !reps) return 0.0;
-  const unsigned NS = div_up(A.nb, 64);
-  hipLaunchKernelGGL(k_pack4, dim3(div_up(A.nb, BLOCK)), dim3(BLOCK), 0, s_, A.nb, x, (float4 *)p2_.p);
-  hipLaunchKernelGGL(k_pack4, dim3(div_up(A.nb, BLOCK)), dim3(BLOCK), 0, s_, A.nb, x, (float4 *)z_.p);
-  if (iteration_form) hipLaunchKernelGGL(k_bench_state, dim3(1), dim3(WAVE), 0, s_, flag_.p, scal_.p);
-  if (C.ns) launch_pre_plain(C);  // the contact blocks of p (fixed over the timed launches)
-  auto launch_g = [&](uint32_t G) {
-    if (iteration_form)
-      launch_spmv<SPMV_NEXT>(G, s_, NS, A, C, (const float4 *)z_.p, (const float4 *)p2_.p, (float4 *)p_.p, (float4 *)q_.p, part_.p,
-                             (const double *)scal_.p, (const uint32_t *)flag_.p);
-    else
-      launch_spmv<SPMV_PLAIN>(G, s_, NS, A, C, (const float4 *)nullptr, (const float4 *)p2_.p, (float4 *)nullptr, (float4 *)q_.p, (double *)nullptr,
-                              (const double *)nullptr, (const uint32_t *)nullptr);
+  const unsigned NS = div_up(A.nb, 64), G = div_up(A.nb, BLOCK), NI = iter_grid(A, C);
+  const uint32_t nz = sz_ ? sz_->nd : G;
+  double *part_spmv = part_.p, *part_pc = part_.p + std::max(NI, G);
+  hipLaunchKernelGGL(k_pack4, dim3(G), dim3(BLOCK), 0, s_, A.nb, x, (float4 *)p2_.p);
+  hipLaunchKernelGGL(k_pack4, dim3(G), dim3(BLOCK), 0, s_, A.nb, x, (float4 *)z_.p);
+  if (iteration_form) hipLaunchKernelGGL(k_bench_state, dim3(1), dim3(BLOCK), 0, s_, flag_.p, scal_.p, part_pc, 2 * nz);
+  // plain form: the contact blocks of x once, outside the timed launches
+  if (!iteration_form && C.ns) hipLaunchKernelGGL(k_pcg_stencils_plain, dim3(stencil_groups(C)), dim3(WAVE), 0, s_, C, (const float4 *)p2_.p);
+  const SpmvArgs it{A, C, (const float4 *)z_.p, (const float4 *)p2_.p, (float4 *)p_.p, (float4 *)q_.p, part_spmv, part_pc, nz, scal_.p, flag_.p};
+  auto launch_g = [&](uint32_t Gs) {
+    if (iteration_form) {
+      launch_iter<SPMV_NEXT>(Gs, s_, it, 1);
+    } else if (Gs == 8) {
+      hipLaunchKernelGGL((k_pcg_spmv_plain<8>), dim3(NS), dim3(WAVE), 0, s_, A, C, (const float4 *)p2_.p, (float4 *)q_.p);
+    } else {
+      hipLaunchKernelGGL((k_pcg_spmv_plain<16>), dim3(NS), dim3(WAVE), 0, s_, A, C, (const float4 *)p2_.p, (float4 *)q_.p);
+    }
   };
   if (tuned_nb_ != A.nb) {  // a system no solve has tuned yet
     spmv_group_ = pick_group(s_, launch_g);
     tuned_nb_ = A.nb;
   }
   return time_launches(s_, reps, [&] { launch_g(spmv_group_); });
-}
-
-void Pcg::launch_pre_plain(const ContactView &C) {
-  hipLaunchKernelGGL(k_pcg_pre<SPMV_PLAIN>, dim3(pre_grid(C)), dim3(BLOCK), 0, s_, C, (const float4 *)nullptr, (const float4 *)p2_.p,
-                     (const double *)nullptr, 0u, (double *)nullptr, (uint32_t *)nullptr, 0u);
-}
-
-void Pcg::launch_spmv_plain(const SellView &A, const ContactView &C) {
-  launch_spmv<SPMV_PLAIN>(spmv_group_, s_, div_up(A.nb, 64), A, C, (const float4 *)nullptr, (const float4 *)p2_.p, (float4 *)nullptr,
-                          (float4 *)q_.p, (double *)nullptr, (const double *)nullptr, (const uint32_t *)nullptr);
 }
 
 }  // namespace nguo

@@ -1,4 +1,4 @@
-// Device helpers of the PCG iteration kernels (k_pcg_spmv and the block-Jacobi / additive
+// Device helpers of the PCG iteration kernels (k_pcg_iter and the block-Jacobi / additive
 // Schwarz preconditioner kernels): workgroup sums with a fixed combination order, and the
 // redundant per-workgroup reduction of the previous launch's partials.
 //
@@ -30,22 +30,18 @@ __device__ inline uint32_t batch_slot(const uint32_t *off, uint32_t k, uint32_t 
   return i;
 }
 
-struct PreArgs {
-  ContactView C;
-  const float4 *z, *pold;
-  const double *cur;
-  uint32_t nz;
-  double *scal;
-  uint32_t *flag;
-};
+// Iteration launch of one system: the SELL slices' workgroups, then its stencils' (64 a
+// workgroup); cur holds the preconditioner's (r.z | r.r) partials, nz each.
 struct SpmvArgs {
   SellView A;
   ContactView C;
   const float4 *z, *pold;
   float4 *pnew, *q;
   double *part;
-  const double *scal;
-  const uint32_t *flag;
+  const double *cur;
+  uint32_t nz;
+  double *scal;
+  uint32_t *flag;
 };
 struct SchArgs {
   uint32_t nd;
@@ -53,6 +49,8 @@ struct SchArgs {
   const float *ainv;
   float *r, *z, *dsol, *p;
   const float *q;
+  const uint32_t *inc_ptr;
+  const float4 *contrib;
   const double *pap_part;
   uint32_t npap;
   double *nxt, *scal;
@@ -77,17 +75,31 @@ template <int NT, int K> __device__ inline void wg_sum_all(double (&v)[K], doubl
   __syncthreads();
 }
 
-// v[k] = sum_{i < n[k]} src[k][i] for K partial arrays, every thread gets the sums. Each
-// thread adds its strided share in index order; the loads of up to 4 strides are issued
-// before the adds, so a launch with up to 4 NT partials per array pays one memory round trip.
-template <int NT, int K>
+// v[k] = sum_{i < n[k]} src[k][i] for K partial arrays (src[k] holds at least one entry),
+// every thread gets the sums. Each thread adds its strided share in index order. The first U
+// strides are loaded with no branch (indices past n[k] re-read entry n[k] - 1 and add
+// nothing) and outside any loop, so they join the loads the caller already has in flight (a
+// loop would make the compiler drain those first); a launch with up to U NT partials per
+// array pays one memory round trip. Further strides follow in a loop.
+template <int NT, int K, uint32_t U = 4>
 __device__ inline void sum_parts_all(const double *const (&src)[K], const uint32_t (&n)[K], double (&v)[K], double *lds) {
-  constexpr uint32_t U = 4;
   uint32_t nmax = 0;
+  double x[K][U];
 #pragma unroll
-  for (int k = 0; k < K; ++k) { v[k] = 0.0; nmax = n[k] > nmax ? n[k] : nmax; }
-  for (uint32_t b = 0; b < nmax; b += U * NT) {
-    double x[K][U];
+  for (int k = 0; k < K; ++k) {
+    nmax = n[k] > nmax ? n[k] : nmax;
+    const uint32_t last = n[k] ? n[k] - 1u : 0u;
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) x[k][u] = src[k][min(u * NT + threadIdx.x, last)];
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    v[k] = 0.0;
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u)
+      if (u * NT + threadIdx.x < n[k]) v[k] += x[k][u];
+  }
+  for (uint32_t b = U * NT; b < nmax; b += U * NT) {
 #pragma unroll
     for (int k = 0; k < K; ++k)
 #pragma unroll

@@ -442,8 +442,9 @@ template <bool INIT>
 __device__ __forceinline__ void schwarz_pcg_body(uint32_t d, uint32_t nd, const uint32_t *__restrict__ dom_ptr, const uint32_t *__restrict__ dom_dof,
                                                  const float *__restrict__ ainv, const double *__restrict__ r64, float *__restrict__ r,
                                                  float *__restrict__ z, float *__restrict__ dsol, float *__restrict__ p,
-                                                 const float *__restrict__ q, const double *pap_part, uint32_t npap, double *nxt, double *scal,
-                                                 uint32_t *flag, uint32_t j, float4 *rc) {
+                                                 const float *__restrict__ q, const uint32_t *__restrict__ inc_ptr,
+                                                 const float4 *__restrict__ contrib, const double *pap_part, uint32_t npap, double *nxt,
+                                                 double *scal, uint32_t *flag, uint32_t j, float4 *rc) {
   __shared__ float4 rs4[MD / 4];
   __shared__ float zdir[SCH_NT][16];
   __shared__ float4 ztr[SCH_NT][4];
@@ -460,7 +461,7 @@ __device__ __forceinline__ void schwarz_pcg_body(uint32_t d, uint32_t nd, const 
   const uint32_t tile = t / SCH_TPT, cq = t % 4, h = (t / 4) % 2;
   const bool mac = t < SCH_THREADS;
   float4 w[SCH_ROWS];
-  uint32_t od0 = NONE_S;
+  uint32_t od0 = NONE_S, ib = 0, ie = 0;
   float r0 = 0.f, q0 = 0.f, p0 = 0.f, d0 = 0.f;
 #pragma unroll
   for (uint32_t i = 0; i < SCH_ROWS; ++i) w[i] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -476,6 +477,7 @@ __device__ __forceinline__ void schwarz_pcg_body(uint32_t d, uint32_t nd, const 
       r0 = (float)r64[3 * (oc >> 2) + (oc & 3)];
     } else {
       r0 = r[oc]; q0 = q[oc]; p0 = p[oc]; d0 = dsol[oc];
+      if (contrib) { ib = inc_ptr[oc >> 2]; ie = inc_ptr[(oc >> 2) + 1]; }
     }
   }
   const uint32_t od = t < MD ? od0 : NONE_S;
@@ -501,7 +503,20 @@ __device__ __forceinline__ void schwarz_pcg_body(uint32_t d, uint32_t nd, const 
       rv = r0;
       dsol[o] = 0.0f;
     } else {
-      rv = r0 - alpha * q0;
+      // the row's contact blocks of H p, in incidence order (the first four loaded together;
+      // contrib holds at least one entry whenever it is given)
+      float qv = q0;
+      if (contrib) {
+        const float *cf = reinterpret_cast<const float *>(contrib) + (o & 3);
+        float cv[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) cv[u] = cf[4 * (size_t)(ie > ib ? min(ib + u, ie - 1u) : 0u)];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u)
+          if (ib + u < ie) qv += cv[u];
+        for (uint32_t k = ib + 4; k < ie; ++k) qv += cf[4 * (size_t)k];
+      }
+      rv = r0 - alpha * qv;
       dsol[o] = d0 + alpha * p0;
     }
     r[o] = rv;
@@ -563,17 +578,18 @@ template <bool INIT>
 __global__ __launch_bounds__(BLOCK) void k_schwarz_pcg(uint32_t nd, const uint32_t *__restrict__ dom_ptr, const uint32_t *__restrict__ dom_dof,
                                                       const float *__restrict__ ainv, const double *__restrict__ r64, float *__restrict__ r,
                                                       float *__restrict__ z, float *__restrict__ dsol, float *__restrict__ p,
-                                                      const float *__restrict__ q, const double *pap_part, uint32_t npap, double *nxt,
+                                                      const float *__restrict__ q, const uint32_t *__restrict__ inc_ptr,
+                                                      const float4 *__restrict__ contrib, const double *pap_part, uint32_t npap, double *nxt,
                                                       double *scal, uint32_t *flag, uint32_t j, float4 *rc) {
-  schwarz_pcg_body<INIT>(blockIdx.x, nd, dom_ptr, dom_dof, ainv, r64, r, z, dsol, p, q, pap_part, npap, nxt, scal, flag, j, rc);
+  schwarz_pcg_body<INIT>(blockIdx.x, nd, dom_ptr, dom_dof, ainv, r64, r, z, dsol, p, q, inc_ptr, contrib, pap_part, npap, nxt, scal, flag, j, rc);
 }
 
 // Iteration form (not INIT, one level) over several systems.
 __global__ __launch_bounds__(BLOCK) void kb_schwarz_pcg(BatchOf<SchArgs> B, uint32_t j) {
   const uint32_t i = batch_slot(B.off, B.k, blockIdx.x);
   const SchArgs &a = B.a[i];
-  schwarz_pcg_body<false>(blockIdx.x - B.off[i], a.nd, a.dom_ptr, a.dom_dof, a.ainv, nullptr, a.r, a.z, a.dsol, a.p, a.q, a.pap_part, a.npap,
-                          a.nxt, a.scal, a.flag, j, nullptr);
+  schwarz_pcg_body<false>(blockIdx.x - B.off[i], a.nd, a.dom_ptr, a.dom_dof, a.ainv, nullptr, a.r, a.z, a.dsol, a.p, a.q, a.inc_ptr, a.contrib,
+                          a.pap_part, a.npap, a.nxt, a.scal, a.flag, j, nullptr);
 }
 
 void launch_schwarz_pcg_batch(hipStream_t s, const BatchOf<SchArgs> &B, uint32_t j) {
@@ -582,15 +598,16 @@ void launch_schwarz_pcg_batch(hipStream_t s, const BatchOf<SchArgs> &B, uint32_t
 }
 
 void launch_schwarz_pcg(hipStream_t s, const Schwarz &sz, bool init, const double *r64, float *r, float *z, float *dsol, float *p,
-                        const float *q, const double *pap_part, uint32_t npap, double *nxt, double *scal, uint32_t *flag, uint32_t j) {
+                        const float *q, const uint32_t *inc_ptr, const float4 *contrib, const double *pap_part, uint32_t npap, double *nxt,
+                        double *scal, uint32_t *flag, uint32_t j) {
   if (!sz.nd) return;
   if (init)
     hipLaunchKernelGGL(k_schwarz_pcg<true>, dim3(sz.nd), dim3(BLOCK), 0, s, sz.nd, (const uint32_t *)sz.dom_ptr.p, (const uint32_t *)sz.dom_dof.p,
-                       (const float *)sz.ainv.p, r64, r, z, dsol, p, q, pap_part, npap, nxt, scal, flag, j,
-                       sz.nd1 ? (float4 *)sz.rc.p : (float4 *)nullptr);
+                       (const float *)sz.ainv.p, r64, r, z, dsol, p, q, (const uint32_t *)nullptr, (const float4 *)nullptr, pap_part, npap, nxt,
+                       scal, flag, j, sz.nd1 ? (float4 *)sz.rc.p : (float4 *)nullptr);
   else
     hipLaunchKernelGGL(k_schwarz_pcg<false>, dim3(sz.nd), dim3(BLOCK), 0, s, sz.nd, (const uint32_t *)sz.dom_ptr.p, (const uint32_t *)sz.dom_dof.p,
-                       (const float *)sz.ainv.p, r64, r, z, dsol, p, q, pap_part, npap, nxt, scal, flag, j,
+                       (const float *)sz.ainv.p, r64, r, z, dsol, p, q, inc_ptr, contrib, pap_part, npap, nxt, scal, flag, j,
                        sz.nd1 ? (float4 *)sz.rc.p : (float4 *)nullptr);
 }
 

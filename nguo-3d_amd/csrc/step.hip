@@ -2113,7 +2113,10 @@ int nguo_step_bench_spmv(nguo_step *st, uint32_t reps, int iteration_form, doubl
     float *xin = st->egrad.p;
     NGUO_HIP(hipMemcpyAsync(xin, st->x.p, 3 * (size_t)st->NF * sizeof(float), hipMemcpyDeviceToDevice, s));
     *avg_ms = st->pcg->time_spmv(st->act.M.view(), C, xin, reps, iteration_form != 0);
-    *bytes = st->act.M.spmv_bytes(iteration_form != 0);
+    // the iteration launch also reads each contact stencil's participants, gradients,
+    // weights, coefficients, normal and incidence slots (116 B; the blocks it writes are not
+    // counted)
+    *bytes = st->act.M.spmv_bytes(iteration_form != 0) + (iteration_form ? 116.0 * (double)C.ns : 0.0);
     return NGUO_OK;
   });
 }
@@ -2148,7 +2151,7 @@ void nguo_batch_destroy(nguo_batch *b) { delete b; }
 
 // Every instance advances on its own stream from its own host thread (assembly, contacts,
 // CCD and update interleave on the device). With nguo_batch_set_solve(b, 1) the instances'
-// Newton solves meet in the batch's gate, which runs them as one batched PCG (three launches
+// Newton solves meet in the batch's gate, which runs them as one batched PCG (two launches
 // per iteration for all of them); by default each solves on its own stream. Each instance runs
 // exactly the single-scene step (per-instance Newton loop, tolerance, CCD step) and its solves
 // take the same iterations and values either way, so its trajectory is the one
