@@ -103,7 +103,7 @@ def main():
     spmv_ms, spmv_bytes = st.bench_spmv(reps=500, iteration_form=True)
     achieved = spmv_bytes / (spmv_ms * 1e-3) / 1e9
     plain_ms, plain_bytes = st.bench_spmv(reps=500, iteration_form=False)
-    roofline = {"kernel": "k_pcg_spmv<SPMV_NEXT> (SELL-64 3x3 BSR + matrix-free contact, p = z + beta p fused)", "bound": "hbm",
+    roofline = {"kernel": "k_pcg_iter<SPMV_NEXT> (SELL-64 3x3 BSR SpMV + contact stencil blocks, p = z + beta p and the rz/rr reduction fused)", "bound": "hbm",
                 "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": args.traffic_bytes, "bytes_per_launch": spmv_bytes, "avg_launch_us": round(spmv_ms * 1000, 3),
                 "plain_spmv": {"avg_launch_us": round(plain_ms * 1000, 3), "bytes_per_launch": plain_bytes,

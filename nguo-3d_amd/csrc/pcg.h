@@ -47,6 +47,9 @@ struct Schwarz {
   DevBuf<uint32_t> dom_ptr, dom_row, row_dom, row_pos, fails;
   DevBuf<uint32_t> dom_dof;  // [nd * 96]: stride-4 vector index of dof t of domain d, or ~0u past m
   DevBuf<float> ainv;  // nd blocks of SCHWARZ_INV_FLOATS (upper 16 x 16 tiles, row-major each)
+  // [2 nd SCHWARZ_MAX_DOMAIN], written with the inverses: the contact incidence range
+  // (inc_ptr[row], inc_ptr[row + 1]) of row p of domain d at 2 (d SCHWARZ_MAX_DOMAIN + p)
+  DevBuf<uint32_t> inc_rng;
   // Two-level form (engine-core/src/linsolve.rs:522-657 MultilevelSchwarz): the coarse node of
   // a vertex is its domain; the Galerkin coarse operator A1 = C A C^T sums the blocks between
   // two domains; a one-level Schwarz over A1 (coarse domains: BFS over the coarse graph, same
@@ -69,11 +72,12 @@ struct PcgStats {
 // Additive Schwarz preconditioner kernel of the PCG iteration (one workgroup per domain).
 // init: r = (f32) r64, d = 0, z = M^-1 r, p = z. Otherwise iteration j: alpha = rz_j / pAp
 // (npap iteration-launch partials; rz_j in scal[8 + (j & 1)]), q += the row's contact blocks
-// contrib[inc_ptr[v] .. inc_ptr[v + 1]) in order (contrib null: no contact term), r -= alpha q,
-// d += alpha p, z = M^-1 r. Writes the r.z | r.r partials of the new z to nxt (2 nd).
+// contrib over the row's incidence range recorded by the build (in order; contrib null: no
+// contact term), r -= alpha q, d += alpha p, z = M^-1 r. Writes the r.z | r.r partials of the
+// new z to nxt (2 nd).
 void launch_schwarz_pcg(hipStream_t s, const Schwarz &sz, bool init, const double *r64, float *r, float *z, float *dsol, float *p,
-                        const float *q, const uint32_t *inc_ptr, const float4 *contrib, const double *pap_part, uint32_t npap, double *nxt,
-                        double *scal, uint32_t *flag, uint32_t j);
+                        const float *q, const float4 *contrib, const double *pap_part, uint32_t npap, double *nxt, double *scal, uint32_t *flag,
+                        uint32_t j);
 // Iteration form of launch_schwarz_pcg over several one-level systems in one launch.
 void launch_schwarz_pcg_batch(hipStream_t s, const BatchOf<SchArgs> &B, uint32_t j);
 // Two-level form only, right after launch_schwarz_pcg: z += weight C^T S1 (C r) on every

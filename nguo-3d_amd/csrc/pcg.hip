@@ -666,7 +666,7 @@ __global__ __launch_bounds__(BLOCK) void k_bj_apply(uint32_t nb, const float *mi
 // Opens a sweep: r = (f32) r64, d = 0, z = M^-1 r, p_ = z; (r.z | r.r) partials -> part.
 void Pcg::launch_precond_init(const SellMatrix &M, double *part) {
   if (sz_) {
-    launch_schwarz_pcg(s_, *sz_, true, r64_.p, r_.p, z_.p, d_.p, p_.p, q_.p, nullptr, nullptr, nullptr, 0, part, scal_.p, flag_.p, 0);
+    launch_schwarz_pcg(s_, *sz_, true, r64_.p, r_.p, z_.p, d_.p, p_.p, q_.p, nullptr, nullptr, 0, part, scal_.p, flag_.p, 0);
     launch_schwarz_coarse(s_, *sz_, true, r_.p, z_.p, p_.p, part, flag_.p);
     return;
   }
@@ -742,7 +742,7 @@ PcgStats Pcg::solve(const SellMatrix &M, const ContactView &C, const float *b, f
         if (j == 0) launch_iter<SPMV_FIRST>(spmv_group_, s_, first, j);
         else launch_iter<SPMV_NEXT>(spmv_group_, s_, (j & 1) ? next_odd : next_even, j);
         if (sz_) {
-          launch_schwarz_pcg(s_, *sz_, false, nullptr, r_.p, z_.p, d_.p, pj, q_.p, C.inc_ptr, contrib, part_spmv, NI, part_pc, scal_.p, flag_.p, j);
+          launch_schwarz_pcg(s_, *sz_, false, nullptr, r_.p, z_.p, d_.p, pj, q_.p, contrib, part_spmv, NI, part_pc, scal_.p, flag_.p, j);
           launch_schwarz_coarse(s_, *sz_, false, r_.p, z_.p, pj, part_pc, flag_.p);
         } else hipLaunchKernelGGL(k_jacobi_pcg<false>, dim3(G), dim3(BLOCK), 0, s_, nb, (const float *)minv_.p, (const double *)nullptr, r_.p,
                                 z_.p, d_.p, pj, (const float *)q_.p, C.inc_ptr, contrib, (const double *)part_spmv, NI, part_pc, scal_.p,
@@ -858,7 +858,7 @@ void Pcg::solve_batch(hipStream_t s, PcgBatchItem *items, uint32_t n) {
       sn0.a[a] = sf.a[a];
       sn0.a[a].pold = p24;
       sn0.a[a].pnew = p4;
-      c0.a[a] = SchArgs{nd, P.sz_->dom_ptr.p, P.sz_->dom_dof.p, P.sz_->ainv.p, P.r_.p, P.z_.p, P.d_.p, P.p_.p, P.q_.p, it.C.inc_ptr,
+      c0.a[a] = SchArgs{nd, P.sz_->dom_ptr.p, P.sz_->dom_dof.p, P.sz_->ainv.p, P.r_.p, P.z_.p, P.d_.p, P.p_.p, P.q_.p, P.sz_->inc_rng.p,
                         it.C.ns ? (const float4 *)it.C.contrib : nullptr, part_spmv, NI, part_pc, P.scal_.p, P.flag_.p};
       c1.a[a] = c0.a[a];
       c1.a[a].p = P.p2_.p;

@@ -49,7 +49,7 @@ struct SchArgs {
   const float *ainv;
   float *r, *z, *dsol, *p;
   const float *q;
-  const uint32_t *inc_ptr;
+  const uint32_t *inc_rng;
   const float4 *contrib;
   const double *pap_part;
   uint32_t npap;
@@ -80,40 +80,50 @@ template <int NT, int K> __device__ inline void wg_sum_all(double (&v)[K], doubl
 // strides are loaded with no branch (indices past n[k] re-read entry n[k] - 1 and add
 // nothing) and outside any loop, so they join the loads the caller already has in flight (a
 // loop would make the compiler drain those first); a launch with up to U NT partials per
-// array pays one memory round trip. Further strides follow in a loop.
-template <int NT, int K, uint32_t U = 4>
-__device__ inline void sum_parts_all(const double *const (&src)[K], const uint32_t (&n)[K], double (&v)[K], double *lds) {
-  uint32_t nmax = 0;
+// array pays one memory round trip. Further strides follow in a loop. PartSums splits the
+// two so a kernel can issue the first strides' loads early and sum them later.
+template <int NT, int K, uint32_t U = 4> struct PartSums {
   double x[K][U];
+  __device__ inline void load(const double *const (&src)[K], const uint32_t (&n)[K]) {
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
-    nmax = n[k] > nmax ? n[k] : nmax;
-    const uint32_t last = n[k] ? n[k] - 1u : 0u;
+    for (int k = 0; k < K; ++k) {
+      const uint32_t last = n[k] ? n[k] - 1u : 0u;
 #pragma unroll
-    for (uint32_t u = 0; u < U; ++u) x[k][u] = src[k][min(u * NT + threadIdx.x, last)];
+      for (uint32_t u = 0; u < U; ++u) x[k][u] = src[k][min(u * NT + threadIdx.x, last)];
+    }
   }
+  __device__ inline void sum(const double *const (&src)[K], const uint32_t (&n)[K], double (&v)[K], double *lds) {
+    uint32_t nmax = 0;
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
-    v[k] = 0.0;
-#pragma unroll
-    for (uint32_t u = 0; u < U; ++u)
-      if (u * NT + threadIdx.x < n[k]) v[k] += x[k][u];
-  }
-  for (uint32_t b = U * NT; b < nmax; b += U * NT) {
-#pragma unroll
-    for (int k = 0; k < K; ++k)
-#pragma unroll
-      for (uint32_t u = 0; u < U; ++u) {
-        const uint32_t i = b + u * NT + threadIdx.x;
-        x[k][u] = i < n[k] ? src[k][i] : 0.0;
-      }
-#pragma unroll
-    for (int k = 0; k < K; ++k)
+    for (int k = 0; k < K; ++k) {
+      nmax = n[k] > nmax ? n[k] : nmax;
+      v[k] = 0.0;
 #pragma unroll
       for (uint32_t u = 0; u < U; ++u)
-        if (b + u * NT + threadIdx.x < n[k]) v[k] += x[k][u];
+        if (u * NT + threadIdx.x < n[k]) v[k] += x[k][u];
+    }
+    for (uint32_t b = U * NT; b < nmax; b += U * NT) {
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+          const uint32_t i = b + u * NT + threadIdx.x;
+          x[k][u] = i < n[k] ? src[k][i] : 0.0;
+        }
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u)
+          if (b + u * NT + threadIdx.x < n[k]) v[k] += x[k][u];
+    }
+    wg_sum_all<NT, K>(v, lds);
   }
-  wg_sum_all<NT, K>(v, lds);
+};
+template <int NT, int K, uint32_t U = 4>
+__device__ inline void sum_parts_all(const double *const (&src)[K], const uint32_t (&n)[K], double (&v)[K], double *lds) {
+  PartSums<NT, K, U> ps;
+  ps.load(src, n);
+  ps.sum(src, n, v, lds);
 }
 
 }  // namespace nguo

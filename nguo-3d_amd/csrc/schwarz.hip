@@ -55,6 +55,7 @@ void Schwarz::build(const SellMatrix &M, uint32_t domain_size, hipStream_t s, fl
   row_dom.upload(rd.empty() ? std::vector<uint32_t>{0} : rd, s);
   row_pos.upload(rp.empty() ? std::vector<uint32_t>{0} : rp, s);
   ainv.reserve((size_t)std::max(nd, 1u) * SCHWARZ_INV_FLOATS);
+  inc_rng.zero(2 * (size_t)std::max(nd, 1u) * SCHWARZ_MAX_DOMAIN, s);
   fails.zero(1, s);
   nd1 = 0;
   if (weight > 0.0f && nd > 0) {
@@ -229,7 +230,8 @@ __device__ void factor_invert(double *L, double *W, double *diag0, double *colj,
 // diagonal fallback, explicit inverse A^-1 = L^-T L^-1 written as FP32.
 __global__ __launch_bounds__(BLOCK) void k_schwarz_build(SellView A, ContactView C, const uint32_t *__restrict__ dom_ptr,
                                                         const uint32_t *__restrict__ dom_row, const uint32_t *__restrict__ row_dom,
-                                                        const uint32_t *__restrict__ row_pos, float *__restrict__ ainv, uint32_t *fails) {
+                                                        const uint32_t *__restrict__ row_pos, float *__restrict__ ainv, uint32_t *inc_rng,
+                                                        uint32_t *fails) {
   __shared__ double L[MD * MD];
   __shared__ double W[MD * MD];
   __shared__ double diag0[MD];
@@ -256,10 +258,13 @@ __global__ __launch_bounds__(BLOCK) void k_schwarz_build(SellView A, ContactView
   }
   __syncthreads();
   // contact: hp g_a g_b^T + lambda w_a w_b (I - n n^T) for partners inside the domain (one
-  // thread per row: the row's incidences are added in list order)
+  // thread per row: the row's incidences are added in list order); the row's incidence range
+  // is kept for the apply
   if (C.ns) {
     for (uint32_t p = t; p < nv; p += BLOCK) {
       const uint32_t row = dom_row[base + p];
+      inc_rng[2 * ((size_t)d * SCHWARZ_MAX_DOMAIN + p)] = C.inc_ptr[row];
+      inc_rng[2 * ((size_t)d * SCHWARZ_MAX_DOMAIN + p) + 1] = C.inc_ptr[row + 1];
       for (uint32_t k = C.inc_ptr[row]; k < C.inc_ptr[row + 1]; ++k) {
         uint32_t e = C.inc[k], s = e >> 2, a = e & 3u;
         double hp = C.coef[2 * s], lam = C.coef[2 * s + 1];
@@ -409,7 +414,7 @@ void Pcg::build_precond(const SellMatrix &M, const ContactView &C) {
   if (sz_) {
     if (sz_->nd) hipLaunchKernelGGL(k_schwarz_build, dim3(sz_->nd), dim3(BLOCK), 0, s_, M.view(), C, (const uint32_t *)sz_->dom_ptr.p,
                                     (const uint32_t *)sz_->dom_row.p, (const uint32_t *)sz_->row_dom.p, (const uint32_t *)sz_->row_pos.p,
-                                    sz_->ainv.p, sz_->fails.p);
+                                    sz_->ainv.p, sz_->inc_rng.p, sz_->fails.p);
     if (sz_->nd1)
       hipLaunchKernelGGL(k_schwarz_build_coarse, dim3(sz_->nd1), dim3(BLOCK), 0, s_, M.view(), C, (const uint32_t *)sz_->dom_ptr.p,
                          (const uint32_t *)sz_->dom_row.p, (const uint32_t *)sz_->row_dom.p, (const uint32_t *)sz_->cdom_ptr.p,
@@ -442,7 +447,7 @@ template <bool INIT>
 __device__ __forceinline__ void schwarz_pcg_body(uint32_t d, uint32_t nd, const uint32_t *__restrict__ dom_ptr, const uint32_t *__restrict__ dom_dof,
                                                  const float *__restrict__ ainv, const double *__restrict__ r64, float *__restrict__ r,
                                                  float *__restrict__ z, float *__restrict__ dsol, float *__restrict__ p,
-                                                 const float *__restrict__ q, const uint32_t *__restrict__ inc_ptr,
+                                                 const float *__restrict__ q, const uint32_t *__restrict__ inc_rng,
                                                  const float4 *__restrict__ contrib, const double *pap_part, uint32_t npap, double *nxt,
                                                  double *scal, uint32_t *flag, uint32_t j, float4 *rc) {
   __shared__ float4 rs4[MD / 4];
@@ -453,31 +458,46 @@ __device__ __forceinline__ void schwarz_pcg_body(uint32_t d, uint32_t nd, const 
   const uint32_t t = threadIdx.x;
   const uint32_t state = INIT ? 0u : flag[0];
   // loads that need no scalar: this thread's rows of its tile (every tile entry is written by
-  // the build, zero outside the domain's m x m block), the domain rows and this thread's
-  // r / q / p / d entries. None sits under a per-thread branch (threads past the tiles read
-  // the last tile, rows past the domain read entry 0, and neither uses what it read), so the
-  // compiler can count the loads in flight and never drains them one by one; the block-wide
-  // test d < nd (always true) keeps them ahead of the state test rather than sunk below it.
+  // the build, zero outside the domain's m x m block), the iteration launch's partials, the
+  // domain rows and their incidence ranges; then this thread's r / q / p / d entries and the
+  // first four contact blocks of its row. None sits under a per-thread branch (threads past
+  // the tiles read the last tile, rows past the domain read entry 0, missing blocks re-read a
+  // present one, and none uses what it read), so the compiler can count the loads in flight
+  // and never drains them one by one; the block-wide test d < nd (always true) keeps them
+  // ahead of the state test rather than sunk below it. Two memory round trips in all.
   const uint32_t tile = t / SCH_TPT, cq = t % 4, h = (t / 4) % 2;
   const bool mac = t < SCH_THREADS;
   float4 w[SCH_ROWS];
   uint32_t od0 = NONE_S, ib = 0, ie = 0;
-  float r0 = 0.f, q0 = 0.f, p0 = 0.f, d0 = 0.f;
+  float r0 = 0.f, q0 = 0.f, p0 = 0.f, d0 = 0.f, cv[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (uint32_t i = 0; i < SCH_ROWS; ++i) w[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  const double *src[1] = {pap_part};
+  const uint32_t npp[1] = {npap};
+  PartSums<BLOCK, 1> ps;
   const uint32_t base = dom_ptr[d], m = 3 * (dom_ptr[d + 1] - base);
   if (d < nd) {
     const uint32_t tl = mac ? tile : SCH_NT - 1;
     const float4 *a4 = reinterpret_cast<const float4 *>(ainv + (size_t)d * SCHWARZ_INV_FLOATS) + (size_t)tl * (SCH_TILE / 4) + 4 * h + cq;
 #pragma unroll
     for (uint32_t i = 0; i < SCH_ROWS; ++i) w[i] = a4[8 * i];
+    if (!INIT) ps.load(src, npp);
     od0 = dom_dof[(size_t)d * MD + min(t, MD - 1)];
+    uint2 rg = make_uint2(0u, 0u);
+    if (!INIT) rg = reinterpret_cast<const uint2 *>(inc_rng)[(size_t)d * SCHWARZ_MAX_DOMAIN + min(t / 3, SCHWARZ_MAX_DOMAIN - 1)];
+    // every load above is issued before anything waits on the domain rows
+    __builtin_amdgcn_sched_barrier(0);
     const size_t oc = (t < MD && od0 != NONE_S) ? od0 : 0u;
     if (INIT) {
       r0 = (float)r64[3 * (oc >> 2) + (oc & 3)];
     } else {
+      ib = contrib ? rg.x : 0u;
+      ie = contrib ? rg.y : 0u;
       r0 = r[oc]; q0 = q[oc]; p0 = p[oc]; d0 = dsol[oc];
-      if (contrib) { ib = inc_ptr[oc >> 2]; ie = inc_ptr[(oc >> 2) + 1]; }
+      // contrib holds at least one entry whenever it is given; otherwise r stands in
+      const float *cf = contrib ? reinterpret_cast<const float *>(contrib) + (oc & 3) : r;
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u) cv[u] = cf[4 * (size_t)(ie > ib ? min(ib + u, ie - 1u) : 0u)];
     }
   }
   const uint32_t od = t < MD ? od0 : NONE_S;
@@ -485,10 +505,8 @@ __device__ __forceinline__ void schwarz_pcg_body(uint32_t d, uint32_t nd, const 
   if (state) return;
   float alpha = 0.0f;
   if (!INIT) {
-    const double *src[1] = {pap_part};
-    const uint32_t n[1] = {npap};
     double v[1];
-    sum_parts_all<BLOCK, 1>(src, n, v, lds);
+    ps.sum(src, npp, v, lds);
     if (!(v[0] > 0.0)) {
       if (d == 0 && threadIdx.x == 0) flag[0] = 2;
       return;
@@ -503,19 +521,12 @@ __device__ __forceinline__ void schwarz_pcg_body(uint32_t d, uint32_t nd, const 
       rv = r0;
       dsol[o] = 0.0f;
     } else {
-      // the row's contact blocks of H p, in incidence order (the first four loaded together;
-      // contrib holds at least one entry whenever it is given)
+      // the row's contact blocks of H p, in incidence order
       float qv = q0;
-      if (contrib) {
-        const float *cf = reinterpret_cast<const float *>(contrib) + (o & 3);
-        float cv[4];
 #pragma unroll
-        for (uint32_t u = 0; u < 4; ++u) cv[u] = cf[4 * (size_t)(ie > ib ? min(ib + u, ie - 1u) : 0u)];
-#pragma unroll
-        for (uint32_t u = 0; u < 4; ++u)
-          if (ib + u < ie) qv += cv[u];
-        for (uint32_t k = ib + 4; k < ie; ++k) qv += cf[4 * (size_t)k];
-      }
+      for (uint32_t u = 0; u < 4; ++u)
+        if (ib + u < ie) qv += cv[u];
+      for (uint32_t k = ib + 4; k < ie; ++k) qv += reinterpret_cast<const float *>(contrib)[4 * (size_t)k + (o & 3)];
       rv = r0 - alpha * qv;
       dsol[o] = d0 + alpha * p0;
     }
@@ -578,17 +589,17 @@ template <bool INIT>
 __global__ __launch_bounds__(BLOCK) void k_schwarz_pcg(uint32_t nd, const uint32_t *__restrict__ dom_ptr, const uint32_t *__restrict__ dom_dof,
                                                       const float *__restrict__ ainv, const double *__restrict__ r64, float *__restrict__ r,
                                                       float *__restrict__ z, float *__restrict__ dsol, float *__restrict__ p,
-                                                      const float *__restrict__ q, const uint32_t *__restrict__ inc_ptr,
+                                                      const float *__restrict__ q, const uint32_t *__restrict__ inc_rng,
                                                       const float4 *__restrict__ contrib, const double *pap_part, uint32_t npap, double *nxt,
                                                       double *scal, uint32_t *flag, uint32_t j, float4 *rc) {
-  schwarz_pcg_body<INIT>(blockIdx.x, nd, dom_ptr, dom_dof, ainv, r64, r, z, dsol, p, q, inc_ptr, contrib, pap_part, npap, nxt, scal, flag, j, rc);
+  schwarz_pcg_body<INIT>(blockIdx.x, nd, dom_ptr, dom_dof, ainv, r64, r, z, dsol, p, q, inc_rng, contrib, pap_part, npap, nxt, scal, flag, j, rc);
 }
 
 // Iteration form (not INIT, one level) over several systems.
 __global__ __launch_bounds__(BLOCK) void kb_schwarz_pcg(BatchOf<SchArgs> B, uint32_t j) {
   const uint32_t i = batch_slot(B.off, B.k, blockIdx.x);
   const SchArgs &a = B.a[i];
-  schwarz_pcg_body<false>(blockIdx.x - B.off[i], a.nd, a.dom_ptr, a.dom_dof, a.ainv, nullptr, a.r, a.z, a.dsol, a.p, a.q, a.inc_ptr, a.contrib,
+  schwarz_pcg_body<false>(blockIdx.x - B.off[i], a.nd, a.dom_ptr, a.dom_dof, a.ainv, nullptr, a.r, a.z, a.dsol, a.p, a.q, a.inc_rng, a.contrib,
                           a.pap_part, a.npap, a.nxt, a.scal, a.flag, j, nullptr);
 }
 
@@ -598,8 +609,8 @@ void launch_schwarz_pcg_batch(hipStream_t s, const BatchOf<SchArgs> &B, uint32_t
 }
 
 void launch_schwarz_pcg(hipStream_t s, const Schwarz &sz, bool init, const double *r64, float *r, float *z, float *dsol, float *p,
-                        const float *q, const uint32_t *inc_ptr, const float4 *contrib, const double *pap_part, uint32_t npap, double *nxt,
-                        double *scal, uint32_t *flag, uint32_t j) {
+                        const float *q, const float4 *contrib, const double *pap_part, uint32_t npap, double *nxt, double *scal, uint32_t *flag,
+                        uint32_t j) {
   if (!sz.nd) return;
   if (init)
     hipLaunchKernelGGL(k_schwarz_pcg<true>, dim3(sz.nd), dim3(BLOCK), 0, s, sz.nd, (const uint32_t *)sz.dom_ptr.p, (const uint32_t *)sz.dom_dof.p,
@@ -607,7 +618,7 @@ void launch_schwarz_pcg(hipStream_t s, const Schwarz &sz, bool init, const doubl
                        scal, flag, j, sz.nd1 ? (float4 *)sz.rc.p : (float4 *)nullptr);
   else
     hipLaunchKernelGGL(k_schwarz_pcg<false>, dim3(sz.nd), dim3(BLOCK), 0, s, sz.nd, (const uint32_t *)sz.dom_ptr.p, (const uint32_t *)sz.dom_dof.p,
-                       (const float *)sz.ainv.p, r64, r, z, dsol, p, q, inc_ptr, contrib, pap_part, npap, nxt, scal, flag, j,
+                       (const float *)sz.ainv.p, r64, r, z, dsol, p, q, (const uint32_t *)sz.inc_rng.p, contrib, pap_part, npap, nxt, scal, flag, j,
                        sz.nd1 ? (float4 *)sz.rc.p : (float4 *)nullptr);
 }
 
