@@ -108,6 +108,13 @@ def main():
                 "traffic": args.traffic_bytes, "bytes_per_launch": spmv_bytes, "avg_launch_us": round(spmv_ms * 1000, 3),
                 "plain_spmv": {"avg_launch_us": round(plain_ms * 1000, 3), "bytes_per_launch": plain_bytes,
                                "achieved": round(plain_bytes / (plain_ms * 1e-3) / 1e9, 1)}}
+    # the other launch of every PCG iteration: the Schwarz apply with the PCG update
+    pc_ms, pc_bytes = st.bench_precond(reps=500)
+    if pc_ms > 0:
+        pc_ach = pc_bytes / (pc_ms * 1e-3) / 1e9
+        roofline["schwarz_apply"] = {"kernel": "k_schwarz_pcg<false> (FP32 tiled inverses, contact blocks, r/d update, r.z and r.r partials)",
+                                     "avg_launch_us": round(pc_ms * 1000, 3), "bytes_per_launch": pc_bytes, "achieved": round(pc_ach, 1),
+                                     "frac": round(pc_ach / HBM_PEAK_GBS, 4)}
     grid = None
     if args.grid:
         g_ms, g_bytes, g_nnzb = nh.bench_spmv_grid(ctx, args.grid, args.grid, reps=20)
@@ -269,7 +276,12 @@ def main():
         "config": {"workload": "tube garment 19950 v / 39600 t + 3x200-segment rods on a 12-capsule body (50688 t), 60-step drape config",
                    "dt": 1.0 / 60.0, "newton_iters": 4, "cg_tol": 1e-6, "device": name, "cus": cus,
                    "precond": {"schwarz_domain": int(scene.desc.params.schwarz_domain), "coarse_weight": float(scene.desc.params.schwarz_weight)}},
-        "stats": {"mean_pcg_iters": sum(iters) / len(iters), "mean_newton": sum(newton) / len(newton),
+        # mean_pcg_iters: the largest Newton solve of a step; total: all solves of a step (the
+        # step's time over it bounds the cost of one PCG iteration from above)
+        "stats": {"mean_pcg_iters": sum(iters) / len(iters),
+                  "mean_total_pcg_iters": sum(m.sum_cg_iters for m in mlist) / len(mlist),
+                  "us_per_pcg_iter_max": round(1000.0 * ms / max(1.0, sum(m.sum_cg_iters for m in mlist) / len(mlist)), 2),
+                  "mean_newton": sum(newton) / len(newton),
                   "mean_stencils": sum(stencils) / len(stencils), "max_reresid": res,
                   "invariants": invariants},
         "roofline": roofline,

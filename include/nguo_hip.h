@@ -131,6 +131,7 @@ typedef struct nguo_step_metrics {
                                0: the scene description carries none */
   uint32_t device_alloc;    /* device allocations made during the step (0 once capacities settle) */
   uint32_t device_free;
+  uint32_t sum_cg_iters;    /* inner PCG iterations summed over the step's Newton solves */
 } nguo_step_metrics;
 
 typedef struct nguo_ctx nguo_ctx;
@@ -225,11 +226,18 @@ int nguo_batch_set_solve(nguo_batch *batch, int batched);
 /* ---- measurement (no reference counterpart: the reference has no GPU timestamps,
  * core-engine/docs/WGPU_PIN.md:142-144). Average duration of the PCG operator y = (A+C) x on
  * the current Newton system, timed with HIP events on the library stream over `reps`
- * back-to-back launches. iteration_form = 0: y = (A+C) x alone, *bytes = nnzb*40 + nb*28 + 4;
- * iteration_form = 1: the kernel instance the PCG iteration launches (also gathers z,
- * evaluates and stores p = z + beta p_old, reduces the previous partials),
- * *bytes = nnzb*40 + nb*52 + 4. */
+ * back-to-back launches. iteration_form = 0: y = A x plus the contact blocks (evaluated once
+ * before the timed launches), *bytes = nnzb*40 + nb*28 + 4; iteration_form = 1: the launch
+ * the PCG iteration makes (also reduces the preconditioner's partials, gathers z, evaluates
+ * and stores p = z + beta p_old, evaluates the contact stencils' blocks),
+ * *bytes = nnzb*40 + nb*52 + 4 + 116 per contact stencil. */
 int nguo_step_bench_spmv(nguo_step *st, uint32_t reps, int iteration_form, double *avg_ms, double *bytes);
+/* The preconditioner's launch of the PCG iteration on the current Newton system (one-level
+ * Schwarz: alpha, the row's contact blocks added to q, r and d updated, z = A_d^-1 r per
+ * domain), event-timed the same way after building the preconditioner for that system;
+ * *bytes = the inverses' upper triangles (2 m (m + 1) per domain of m dof) + 32 per dof
+ * + with contact 8 per vertex and 12 per incidence. 0 ms for a block-Jacobi scene. */
+int nguo_step_bench_precond(nguo_step *st, uint32_t reps, double *avg_ms, double *bytes);
 /* A full 4-pass LSD radix sort of n random (u32 key, u32 index) pairs on the device (the
  * LBVH's sort), event-timed per sort; *bytes = 80 n (per pass: 4 B histogram read, 16 B
  * scatter read + write per pair). */

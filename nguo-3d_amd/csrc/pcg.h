@@ -120,6 +120,12 @@ class Pcg {
   // p_old, stores p, evaluates the contact stencils' blocks and the p.q partials); otherwise
   // y = A x plus the contact blocks (evaluated once, before the timed launches).
   double time_spmv(const SellView &A, const ContactView &C, const float *x, uint32_t reps, bool iteration_form);
+  // Average duration (ms) of the preconditioner's iteration launch (the Schwarz apply with the
+  // PCG update and the row's contact blocks) over `reps` back-to-back launches on (M, C), after
+  // building the preconditioner for it; bytes receives the launch's algorithmic bytes (the
+  // inverses' upper triangles, 32 B per dof of vectors and domain rows, the contact ranges and
+  // blocks). 0 without a Schwarz preconditioner.
+  double time_precond(const SellMatrix &M, const ContactView &C, uint32_t reps, double *bytes);
   // Block-Jacobi inverse of (A + C)'s diagonal blocks (FP64 inversion, scalar fallback).
   void build_precond(const SellMatrix &M, const ContactView &C);
   // Use `sz` (built over the same matrix pattern) instead of 3x3 block Jacobi; null = Jacobi.

@@ -977,4 +977,44 @@ double Pcg::time_spmv(const SellView &A, const ContactView &C, const float *x, u
   return time_launches(s_, reps, [&] { launch_g(spmv_group_); });
 }
 
+// State under which the preconditioner's iteration launch runs without stopping: running,
+// rz_j = 1 and pAp partials of 1 (alpha = 1 / npap).
+__global__ void k_bench_precond_state(uint32_t *flag, double *scal, double *pap, uint32_t npap) {
+  for (uint32_t i = threadIdx.x; i < npap; i += blockDim.x) pap[i] = 1.0;
+  if (threadIdx.x == 0) {
+    flag[0] = 0u; flag[1] = 0u; flag[2] = 0xffffffffu;
+    scal[8] = 1.0; scal[9] = 1.0;
+  }
+}
+
+double Pcg::time_precond(const SellMatrix &M, const ContactView &C, uint32_t reps, double *bytes) {
+  *bytes = 0.0;
+  if (!sz_ || !M.nb || !reps) return 0.0;
+  ensure(M.nb, C.ns);
+  build_precond(M, C);
+  if (!sz_->nd) return 0.0;
+  const SellView A = M.view();
+  const unsigned G = div_up(M.nb, BLOCK), NI = iter_grid(A, C);
+  double *part_spmv = part_.p, *part_pc = part_.p + std::max(NI, G);
+  hipLaunchKernelGGL(k_bench_precond_state, dim3(1), dim3(BLOCK), 0, s_, flag_.p, scal_.p, part_spmv, NI);
+  const float4 *contrib = C.ns ? (const float4 *)C.contrib : nullptr;
+  const double ms = time_launches(s_, reps, [&] {
+    launch_schwarz_pcg(s_, *sz_, false, nullptr, r_.p, z_.p, d_.p, p_.p, q_.p, contrib, part_spmv, NI, part_pc, scal_.p, flag_.p, 1);
+  });
+  double b = 0.0;
+  for (uint32_t d = 0; d < sz_->nd; ++d) {
+    const double m = 3.0 * (sz_->h_dom_ptr[d + 1] - sz_->h_dom_ptr[d]);
+    b += 2.0 * m * (m + 1.0);  // upper triangle of the FP32 inverse
+  }
+  b += 32.0 * 3.0 * M.nb;  // r, d (read + write), q, p (read), z (write), domain rows
+  if (C.ns) {
+    uint32_t ninc = 0;
+    NGUO_HIP(hipMemcpyAsync(&ninc, C.inc_ptr + M.nb, sizeof(uint32_t), hipMemcpyDeviceToHost, s_));
+    NGUO_HIP(hipStreamSynchronize(s_));
+    b += 8.0 * M.nb + 12.0 * ninc;  // incidence ranges, the blocks' three components
+  }
+  *bytes = b;
+  return ms;
+}
+
 }  // namespace nguo

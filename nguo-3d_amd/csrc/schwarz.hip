@@ -460,7 +460,7 @@ __device__ __forceinline__ void schwarz_pcg_body(uint32_t d, uint32_t nd, const 
   // loads that need no scalar: this thread's rows of its tile (every tile entry is written by
   // the build, zero outside the domain's m x m block), the iteration launch's partials, the
   // domain rows and their incidence ranges; then this thread's r / q / p / d entries and the
-  // first four contact blocks of its row. None sits under a per-thread branch (threads past
+  // first eight contact blocks of its row. None sits under a per-thread branch (threads past
   // the tiles read the last tile, rows past the domain read entry 0, missing blocks re-read a
   // present one, and none uses what it read), so the compiler can count the loads in flight
   // and never drains them one by one; the block-wide test d < nd (always true) keeps them
@@ -469,7 +469,10 @@ __device__ __forceinline__ void schwarz_pcg_body(uint32_t d, uint32_t nd, const 
   const bool mac = t < SCH_THREADS;
   float4 w[SCH_ROWS];
   uint32_t od0 = NONE_S, ib = 0, ie = 0;
-  float r0 = 0.f, q0 = 0.f, p0 = 0.f, d0 = 0.f, cv[4] = {0.f, 0.f, 0.f, 0.f};
+  constexpr uint32_t NCB = 8;  // contact blocks of a row loaded together
+  float r0 = 0.f, q0 = 0.f, p0 = 0.f, d0 = 0.f, cv[NCB];
+#pragma unroll
+  for (uint32_t u = 0; u < NCB; ++u) cv[u] = 0.f;
 #pragma unroll
   for (uint32_t i = 0; i < SCH_ROWS; ++i) w[i] = make_float4(0.f, 0.f, 0.f, 0.f);
   const double *src[1] = {pap_part};
@@ -497,7 +500,7 @@ __device__ __forceinline__ void schwarz_pcg_body(uint32_t d, uint32_t nd, const 
       // contrib holds at least one entry whenever it is given; otherwise r stands in
       const float *cf = contrib ? reinterpret_cast<const float *>(contrib) + (oc & 3) : r;
 #pragma unroll
-      for (uint32_t u = 0; u < 4; ++u) cv[u] = cf[4 * (size_t)(ie > ib ? min(ib + u, ie - 1u) : 0u)];
+      for (uint32_t u = 0; u < NCB; ++u) cv[u] = cf[4 * (size_t)(ie > ib ? min(ib + u, ie - 1u) : 0u)];
     }
   }
   const uint32_t od = t < MD ? od0 : NONE_S;
@@ -521,12 +524,20 @@ __device__ __forceinline__ void schwarz_pcg_body(uint32_t d, uint32_t nd, const 
       rv = r0;
       dsol[o] = 0.0f;
     } else {
-      // the row's contact blocks of H p, in incidence order
+      // the row's contact blocks of H p, in incidence order; past the first NCB, NCB more per
+      // memory round trip (clamped to the row's last block, so no load waits on a branch)
       float qv = q0;
 #pragma unroll
-      for (uint32_t u = 0; u < 4; ++u)
+      for (uint32_t u = 0; u < NCB; ++u)
         if (ib + u < ie) qv += cv[u];
-      for (uint32_t k = ib + 4; k < ie; ++k) qv += reinterpret_cast<const float *>(contrib)[4 * (size_t)k + (o & 3)];
+      for (uint32_t k0 = ib + NCB; k0 < ie; k0 += NCB) {
+        const float *cf = reinterpret_cast<const float *>(contrib) + (o & 3);
+#pragma unroll
+        for (uint32_t u = 0; u < NCB; ++u) cv[u] = cf[4 * (size_t)min(k0 + u, ie - 1u)];
+#pragma unroll
+        for (uint32_t u = 0; u < NCB; ++u)
+          if (k0 + u < ie) qv += cv[u];
+      }
       rv = r0 - alpha * qv;
       dsol[o] = d0 + alpha * p0;
     }

@@ -1520,7 +1520,7 @@ void advance_one(nguo_step *st) {
   st->trace_toi.clear();
   bool final_step = false;
   double max_res = 0;
-  uint32_t max_it = 0;
+  uint32_t max_it = 0, sum_it = 0;
   float last_t = 1.0f;
   for (uint32_t it = 0; it < cap_it; ++it) {
     taken++;
@@ -1528,6 +1528,7 @@ void advance_one(nguo_step *st) {
     assemble_and_solve(st, ls);
     max_res = std::max(max_res, ls.residual);
     max_it = std::max(max_it, ls.iters);
+    sum_it += ls.iters;
     hipLaunchKernelGGL(k_scatter_dx, dim3(GV), dim3(BLOCK), 0, s, V, (const uint32_t *)st->vdof.p, (const float *)st->dxf.p, st->dx.p, st->part.p);
     float max_norm = reduce_max(st, GV);
     if (max_norm > 0.0f) {
@@ -1579,6 +1580,7 @@ void advance_one(nguo_step *st) {
   m.time = st->time;
   m.max_reresid = max_res;
   m.max_cg_iters = max_it;
+  m.sum_cg_iters = sum_it;
   m.newton_iters = taken;
   m.num_stencils = st->ns;
   m.max_velocity = vmax;
@@ -2117,6 +2119,20 @@ int nguo_step_bench_spmv(nguo_step *st, uint32_t reps, int iteration_form, doubl
     // weights, coefficients, normal and incidence slots (116 B; the blocks it writes are not
     // counted)
     *bytes = st->act.M.spmv_bytes(iteration_form != 0) + (iteration_form ? 116.0 * (double)C.ns : 0.0);
+    return NGUO_OK;
+  });
+}
+
+int nguo_step_bench_precond(nguo_step *st, uint32_t reps, double *avg_ms, double *bytes) {
+  return guarded([&]() -> int {
+    NGUO_REQUIRE(st && avg_ms && bytes && reps > 0, NGUO_E_INVALID, "nguo_step_bench_precond: bad argument");
+    hipStream_t s = st->s;
+    if (st->NF == 0) { *avg_ms = 0; *bytes = 0; return NGUO_OK; }
+    // the system of the current state (elastic blocks + contact term)
+    run_elements(st->act, st->P, st->x.p, s);
+    gather(st->act, st->P, 1, st->im.p, st->x.p, st->x0.p, s);
+    contacts(st);
+    *avg_ms = st->pcg->time_precond(st->act.M, contact_view(st), reps, bytes);
     return NGUO_OK;
   });
 }

@@ -28,7 +28,7 @@ EXPORTED = [
     "nguo_scene_generate", "nguo_scene_view", "nguo_scene_free",
     "nguo_step_get_newton_trace", "nguo_step_stencils", "nguo_step_collider_sides",
     "nguo_lbvh_morton", "nguo_lbvh_sort_pairs", "nguo_lbvh_karras", "nguo_lbvh_aabb",
-    "nguo_step_bench_spmv", "nguo_bench_spmv_grid", "nguo_ls_precond_apply",
+    "nguo_step_bench_spmv", "nguo_step_bench_precond", "nguo_bench_spmv_grid", "nguo_ls_precond_apply",
     "nguo_batch_create", "nguo_batch_destroy", "nguo_batch_advance", "nguo_batch_instance",
     "nguo_step_set_rest_scale", "nguo_step_adjoint", "nguo_batch_set_solve", "nguo_bench_sort",
 ]
@@ -74,6 +74,7 @@ class StepMetrics(C.Structure):
         ("newton_iters", C.c_uint32), ("num_stencils", C.c_uint32), ("max_velocity", C.c_float),
         ("penetration_free", C.c_int32), ("all_finite", C.c_int32), ("last_toi", C.c_float),
         ("num_contacts", C.c_uint32), ("max_sigma", C.c_float), ("device_alloc", C.c_uint32), ("device_free", C.c_uint32),
+        ("sum_cg_iters", C.c_uint32),
     ]
 
 
@@ -136,6 +137,7 @@ def load():
         lib.nguo_batch_advance.argtypes = [vp, C.c_uint32]
         lib.nguo_batch_instance.argtypes = [vp, C.c_uint32, P(vp)]
         lib.nguo_step_bench_spmv.argtypes = [vp, C.c_uint32, C.c_int, P(C.c_double), P(C.c_double)]
+        lib.nguo_step_bench_precond.argtypes = [vp, C.c_uint32, P(C.c_double), P(C.c_double)]
         lib.nguo_step_set_rest_scale.argtypes = [vp, P(C.c_float), C.c_uint32]
         lib.nguo_batch_set_solve.argtypes = [vp, C.c_int]
         lib.nguo_bench_sort.argtypes = [vp, C.c_uint32, C.c_uint32, P(C.c_double), P(C.c_double)]
@@ -412,6 +414,13 @@ class Stepper:
         iteration form is the kernel instance the PCG iteration launches."""
         ms, by = C.c_double(), C.c_double()
         check(load().nguo_step_bench_spmv(self.h, int(reps), int(bool(iteration_form)), C.byref(ms), C.byref(by)))
+        return ms.value, by.value
+
+    def bench_precond(self, reps=200):
+        """(average ms per launch, algorithmic bytes per launch) of the preconditioner's
+        launch in the PCG iteration (Schwarz apply with the PCG update)."""
+        ms, by = C.c_double(), C.c_double()
+        check(load().nguo_step_bench_precond(self.h, int(reps), C.byref(ms), C.byref(by)))
         return ms.value, by.value
 
     def set_rest_scale(self, face_scale):
