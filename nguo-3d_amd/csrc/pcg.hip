@@ -320,12 +320,16 @@ __device__ __forceinline__ void pcg_spmv_body(uint32_t slice, const SellView &A,
       g.accumulate(y0, y1, y2, beta, k0, len);
     }
     if (MODE == SPMV_PLAIN) {
+      // past the first 8 blocks, 8 more per memory round trip (clamped to the row's last)
 #pragma unroll
       for (uint32_t u = 0; u < 8; ++u)
         if (ib + u < ie) { y0 += cv[u].x; y1 += cv[u].y; y2 += cv[u].z; }
-      for (uint32_t k = ib + 8; k < ie; ++k) {
-        const float4 c = C.contrib[k];
-        y0 += c.x; y1 += c.y; y2 += c.z;
+      for (uint32_t k0 = ib + 8; k0 < ie; k0 += 8) {
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u) cv[u] = C.contrib[min(k0 + u, ie - 1u)];
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u)
+          if (k0 + u < ie) { y0 += cv[u].x; y1 += cv[u].y; y2 += cv[u].z; }
       }
     }
     q[row] = make_float4(y0, y1, y2, 0.0f);

@@ -756,6 +756,33 @@ __device__ inline bool defer_pair(const CcdDefer &D, uint32_t a, uint32_t b, uin
   return true;
 }
 
+// The query vertex's exclusion list (its 1-ring), up to EXC_REG entries held in registers for
+// the leaf tests; a longer list has its tail read from memory.
+constexpr uint32_t EXC_REG = 12;
+struct ExcList {
+  uint32_t r[EXC_REG];
+  uint32_t b, e;
+  __device__ inline void load(const Topo &T, uint32_t v) {
+    b = T.ex_ptr[v];
+    e = T.ex_ptr[v + 1];
+#pragma unroll
+    for (uint32_t u = 0; u < EXC_REG; ++u) r[u] = b + u < e ? T.ex[b + u] : 0xffffffffu;
+  }
+  __device__ inline bool has(const Topo &T, uint32_t t0, uint32_t t1, uint32_t t2) const {
+    bool hit = false;
+#pragma unroll
+    for (uint32_t u = 0; u < EXC_REG; ++u) hit |= r[u] == t0 || r[u] == t1 || r[u] == t2;
+    for (uint32_t k = b + EXC_REG; k < e && !hit; ++k) {
+      const uint32_t x = T.ex[k];
+      hit = x == t0 || x == t1 || x == t2;
+    }
+    return hit;
+  }
+};
+
+// Leaves load their positions, motions and flags in one memory round trip (after the
+// triangle's vertex ids) and test the filters from registers: the filters are predicates, so
+// their order changes nothing but the latency.
 __global__ __launch_bounds__(BLOCK) void k_ccd_vt(Topo T, BvhView dyn, BvhView stat, const float *__restrict__ x, const float *__restrict__ dx,
                                                  float pad0, float max_disp, uint32_t iters, CcdDefer D, unsigned long long *tmin, uint32_t *err) {
   __shared__ uint32_t tstack[TRAV_LDS * BLOCK];
@@ -763,17 +790,24 @@ __global__ __launch_bounds__(BLOCK) void k_ccd_vt(Topo T, BvhView dyn, BvhView s
   double best = 1.0;
   if (v < T.V) {
     f3 p = ld3(x, v);
+    const f3 dvv = ld3(dx, v);
     bool mv = movable(T, v);
-    const float pad = ccd_query_pad(pad0, len(ld3(dx, v)), max_disp);
+    const bool below = T.side ? T.side[v] < 0.0f : false;
+    ExcList exl;
+    exl.load(T, v);
+    const float pad = ccd_query_pad(pad0, len(dvv), max_disp);
     auto leaf = [&](uint32_t t) {
       uint32_t t0 = T.tris[3 * (size_t)t], t1 = T.tris[3 * (size_t)t + 1], t2 = T.tris[3 * (size_t)t + 2];
       f3 a = ld3(x, t0), b = ld3(x, t1), c = ld3(x, t2);
+      const f3 da = ld3(dx, t0), db = ld3(dx, t1), dc = ld3(dx, t2);
+      const float m0 = T.im[t0], m1 = T.im[t1], m2 = T.im[t2];
+      const bool st_all = T.static_v ? (T.static_v[t0] && T.static_v[t1] && T.static_v[t2]) : false;
       if (!inside_padded(p, vmin(vmin(a, b), c), vmax(vmax(a, b), c), pad)) return;
-      bool tm = movable(T, t0) || movable(T, t1) || movable(T, t2);
+      bool tm = m0 > 0.0f || m1 > 0.0f || m2 > 0.0f;
       if (!tm && !mv) return;
-      if (excluded(T, v, t0, t1, t2)) return;
-      if (T.static_v && T.static_v[t0] && T.static_v[t1] && T.static_v[t2] && T.side[v] < 0.0f) return;
-      const f3 dv[4] = {ld3(dx, v), ld3(dx, t0), ld3(dx, t1), ld3(dx, t2)};
+      if (exl.has(T, t0, t1, t2)) return;
+      if (st_all && below) return;
+      const f3 dv[4] = {dvv, da, db, dc};
       const f3 tri[3] = {a, b, c};
       if (accd_surely_clear(&p, 1, tri, 3, dv)) return;
       d3 x0[4] = {to_d(p), to_d(a), to_d(b), to_d(c)};
@@ -808,19 +842,24 @@ __global__ __launch_bounds__(BLOCK) void k_ccd_ee(Topo T, BvhView dyn, const uin
     uint32_t ej = qids[q];
     uint32_t b0 = T.edges[2 * (size_t)ej], b1 = T.edges[2 * (size_t)ej + 1];
     f3 pb = ld3(x, b0), qb = ld3(x, b1);
-    const float pad = ccd_query_pad(pad0, fmaxf(len(ld3(dx, b0)), len(ld3(dx, b1))), max_disp);
+    const f3 db0 = ld3(dx, b0), db1 = ld3(dx, b1);
+    const float pad = ccd_query_pad(pad0, fmaxf(len(db0), len(db1)), max_disp);
     f3 blo = vmin(pb, qb) - f3{pad, pad, pad}, bhi = vmax(pb, qb) + f3{pad, pad, pad};
     auto ov = [&](f3 lo, f3 hi) { return !(lo.x > bhi.x || lo.y > bhi.y || lo.z > bhi.z || hi.x < blo.x || hi.y < blo.y || hi.z < blo.z); };
+    // a leaf loads its edge's positions and motions in one memory round trip after the vertex
+    // ids, and reuses the query edge's from registers
     bvh_traverse_part(dyn, blockIdx.y, err, tstack, ov, [&](uint32_t ei) {
       if (ei == ej || (query_dyn && ei > ej)) return;
       uint32_t a0 = T.edges[2 * (size_t)ei], a1 = T.edges[2 * (size_t)ei + 1];
       if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) return;
       f3 pa = ld3(x, a0), qa = ld3(x, a1);
+      const f3 da0 = ld3(dx, a0), da1 = ld3(dx, a1);
       if (!ov(vmin(pa, qa), vmax(pa, qa))) return;
-      uint32_t lo_e = min(ei, ej), hi_e = max(ei, ej);
-      uint32_t i0 = T.edges[2 * (size_t)lo_e], i1 = T.edges[2 * (size_t)lo_e + 1], i2 = T.edges[2 * (size_t)hi_e], i3 = T.edges[2 * (size_t)hi_e + 1];
-      const f3 xs[4] = {ld3(x, i0), ld3(x, i1), ld3(x, i2), ld3(x, i3)};
-      const f3 dv[4] = {ld3(dx, i0), ld3(dx, i1), ld3(dx, i2), ld3(dx, i3)};
+      // the pair's vertices in edge-index order: the lower edge's two, then the higher's
+      const bool a_lo = ei < ej;
+      const uint32_t lo_e = min(ei, ej), hi_e = max(ei, ej);
+      const f3 xs[4] = {a_lo ? pa : pb, a_lo ? qa : qb, a_lo ? pb : pa, a_lo ? qb : qa};
+      const f3 dv[4] = {a_lo ? da0 : db0, a_lo ? da1 : db1, a_lo ? db0 : da0, a_lo ? db1 : da1};
       if (accd_surely_clear(xs, 2, xs + 2, 2, dv)) return;
       d3 x0[4] = {to_d(xs[0]), to_d(xs[1]), to_d(xs[2]), to_d(xs[3])};
       d3 d4[4] = {to_d(dv[0]), to_d(dv[1]), to_d(dv[2]), to_d(dv[3])};
