@@ -51,9 +51,9 @@ def main():
                     help="adjoint Hessian: exact elastic second derivatives, or the Newton iteration's projected blocks")
     ap.add_argument("--schwarz-domain", type=int, default=None, help="override the scene's Schwarz domain size")
     ap.add_argument("--schwarz-weight", type=float, default=None, help="override the scene's two-level coarse weight (0 = one level)")
-    # 14.16 MB read + 0.61 MB written per launch of the iteration-form SpMV on this workload,
-    # from rocprofv3 passes over the L2-to-fabric request counters (128 B / 64 B requests)
-    ap.add_argument("--traffic-bytes", type=float, default=14.77e6,
+    # 14.44 MB read + 0.68 MB written per iteration launch on this workload in contact, from
+    # rocprofv3 passes over the L2-to-fabric request counters (128 B / 64 B requests)
+    ap.add_argument("--traffic-bytes", type=float, default=15.12e6,
                     help="HBM bytes per launch of the roofline kernel, measured with rocprofv3 counter passes")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="after the bench, copy a buffer of this many MiB once (counter calibration for --pmc passes)")
