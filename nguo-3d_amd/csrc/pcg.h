@@ -133,7 +133,7 @@ class Pcg {
   Schwarz *schwarz() const { return sz_; }
   // z = M^-1 r with the current preconditioner (after build_precond), for parity checks.
   void apply_precond(const SellMatrix &M, const float *r, float *z);
-  uint32_t chunk = 8;  // iterations per host convergence check
+  uint32_t chunk = 8;  // iterations per host convergence check, at least (grows with the sweep)
   // SpMV slot-group size (slots whose loads are in flight at once), picked by timing the
   // instantiated sizes on the live system at the first solve of a system size
   uint32_t spmv_group() const { return spmv_group_; }

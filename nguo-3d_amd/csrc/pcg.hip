@@ -612,8 +612,17 @@ __global__ __launch_bounds__(BLOCK) void k_minv(SellView A, ContactView C, const
 
 // ---------------------------------------------------------------- host driver
 // A refinement sweep asks the FP32 iteration for a reduction of its own (recursive) residual
-// by eta = clamp(0.25 tol / res, REFINE_ETA_MIN, 0.5) relative to the sweep's start.
-constexpr double REFINE_ETA_MIN = 1.0e-5;
+// by eta = clamp(0.25 tol / res, REFINE_ETA_MIN, 0.5) relative to the sweep's start. The floor
+// sits at the FP32 unit roundoff: a smaller reduction of the recursive residual would no
+// longer improve the FP32 iterate. At a 1e-6 tolerance the first sweep thus aims at 2.5e-7
+// and most solves take one sweep (a floor of 1e-5 forced a second, restarted Krylov sweep:
+// 7% more iterations on the same drape states).
+constexpr double REFINE_ETA_MIN = 1.0e-7;
+// Iterations queued per host convergence check: at least Pcg::chunk, growing with the sweep
+// to a sixteenth of the iterations done (at most CHUNK_MAX), so the flag copies thin out on
+// long solves while the launches queued past convergence stay a small share.
+constexpr uint32_t CHUNK_MAX = 32;
+inline uint32_t chunk_len(uint32_t chunk, uint32_t j) { return std::min(CHUNK_MAX, std::max(chunk, j / 16)); }
 
 Pcg::~Pcg() {
   for (auto &e : ev_)
@@ -736,12 +745,12 @@ PcgStats Pcg::solve(const SellMatrix &M, const ContactView &C, const float *b, f
       spmv_group_ = pick_group(s_, [&](uint32_t G) { launch_iter<SPMV_FIRST>(G, s_, first, 0); });
       tuned_nb_ = nb;
     }
-    // Chunks of `chunk` iterations are queued one ahead of the one whose state the host
+    // Chunks of chunk_len iterations are queued one ahead of the one whose state the host
     // reads, so the device never idles on the convergence check; iterations queued after
     // convergence exit at their first instruction.
     uint32_t j = 0, queued = 0, waited = 0;
     auto queue_chunk = [&]() {
-      for (uint32_t c = 0; c < chunk; ++c, ++j) {
+      for (uint32_t c = 0, n = chunk_len(chunk, j); c < n; ++c, ++j) {
         float *pj = (j & 1) ? p2_.p : p_.p;
         if (j == 0) launch_iter<SPMV_FIRST>(spmv_group_, s_, first, j);
         else launch_iter<SPMV_NEXT>(spmv_group_, s_, (j & 1) ? next_odd : next_even, j);
@@ -876,7 +885,7 @@ void Pcg::solve_batch(hipStream_t s, PcgBatchItem *items, uint32_t n) {
     }
     uint32_t j = 0, queued = 0, waited = 0;
     auto queue_chunk = [&]() {
-      for (uint32_t c = 0; c < chunk; ++c, ++j) {
+      for (uint32_t c = 0, n = chunk_len(chunk, j); c < n; ++c, ++j) {
         if (j == 0) launch_iter_batch<SPMV_FIRST>(group, s, sf, j);
         else launch_iter_batch<SPMV_NEXT>(group, s, (j & 1) ? sn1 : sn0, j);
         launch_schwarz_pcg_batch(s, (j & 1) ? c1 : c0, j);
