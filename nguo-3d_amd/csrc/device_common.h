@@ -15,6 +15,45 @@ template <class T> __device__ inline T wave_sum(T v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
   return v;
 }
+
+// Lane exchanges inside a row of 16 lanes (DPP, no LDS traffic). Controls used here: quad
+// permutations 0xB1 (lane ^ 1) and 0x4E (lane ^ 2), 0x141 the half-row mirror (lane i of
+// each 8 with lane 7 - i), 0x140 the row mirror (lane i with 15 - i). All lanes must be active.
+constexpr int DPP_XOR1 = 0xB1, DPP_XOR2 = 0x4E, DPP_HALF_MIRROR = 0x141, DPP_MIRROR = 0x140;
+template <int CTRL> __device__ inline float dpp_f(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL> __device__ inline double dpp_d(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ inline double readlane_d(double v, int lane) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane), __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+// Sum over the 64 lanes of a wavefront, every lane gets it (all lanes active). Four
+// symmetric exchanges inside each row of 16 (lane ^ 1, lane ^ 2, half-row mirror, row
+// mirror) leave the row's sum in all of its lanes: both partners of an exchange add the same
+// two values, and a + b == b + a bitwise. The four row sums are then read from lanes 0, 16,
+// 32 and 48 and added in a fixed order.
+__device__ inline double wave_sum(double v) {
+  v += dpp_d<DPP_XOR1>(v);
+  v += dpp_d<DPP_XOR2>(v);
+  v += dpp_d<DPP_HALF_MIRROR>(v);
+  v += dpp_d<DPP_MIRROR>(v);
+  return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
+}
+__device__ inline float wave_sum(float v) {
+  v += dpp_f<DPP_XOR1>(v);
+  v += dpp_f<DPP_XOR2>(v);
+  v += dpp_f<DPP_HALF_MIRROR>(v);
+  v += dpp_f<DPP_MIRROR>(v);
+  const float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+  const float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
+  const float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
+  const float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
+  return (r0 + r1) + (r2 + r3);
+}
 template <class T> __device__ inline T wave_max(T v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) { T w = __shfl_xor(v, o, WAVE); v = v > w ? v : w; }

@@ -465,7 +465,9 @@ __device__ __forceinline__ void schwarz_pcg_body(uint32_t d, uint32_t nd, const 
   // present one, and none uses what it read), so the compiler can count the loads in flight
   // and never drains them one by one; the block-wide test d < nd (always true) keeps them
   // ahead of the state test rather than sunk below it. Two memory round trips in all.
-  const uint32_t tile = t / SCH_TPT, cq = t % 4, h = (t / 4) % 2;
+  // lane l of a tile's 8: rows of parity h = l / 4, column quarter cq = l (h = 0) or 7 - l
+  // (h = 1), so the half-row mirror pairs the two row parities of one quarter
+  const uint32_t tile = t / SCH_TPT, h = (t / 4) % 2, cq = h ? 3 - t % 4 : t % 4;
   const bool mac = t < SCH_THREADS;
   float4 w[SCH_ROWS];
   uint32_t od0 = NONE_S, ib = 0, ie = 0;
@@ -566,14 +568,16 @@ __device__ __forceinline__ void schwarz_pcg_body(uint32_t d, uint32_t nd, const 
       const float rp = rs[16 * P + 2 * i + h];
       tz.x += a.x * rp; tz.y += a.y * rp; tz.z += a.z * rp; tz.w += a.w * rp;
     }
-    // a + b == b + a, so both lanes of each exchange hold the same sum
+    // DPP exchanges: the four quarters of a row inside each quad, the two row parities of a
+    // quarter by the half-row mirror; a + b == b + a, so both lanes of each exchange hold the
+    // same sum
 #pragma unroll
     for (uint32_t i = 0; i < SCH_ROWS; ++i) {
-      dz[i] += __shfl_xor(dz[i], 1);
-      dz[i] += __shfl_xor(dz[i], 2);
+      dz[i] += dpp_f<DPP_XOR1>(dz[i]);
+      dz[i] += dpp_f<DPP_XOR2>(dz[i]);
     }
-    tz.x += __shfl_xor(tz.x, 4); tz.y += __shfl_xor(tz.y, 4);
-    tz.z += __shfl_xor(tz.z, 4); tz.w += __shfl_xor(tz.w, 4);
+    tz.x += dpp_f<DPP_HALF_MIRROR>(tz.x); tz.y += dpp_f<DPP_HALF_MIRROR>(tz.y);
+    tz.z += dpp_f<DPP_HALF_MIRROR>(tz.z); tz.w += dpp_f<DPP_HALF_MIRROR>(tz.w);
     if (cq == 0)
 #pragma unroll
       for (uint32_t i = 0; i < SCH_ROWS; ++i) zdir[tile][2 * i + h] = dz[i];
