@@ -63,6 +63,7 @@ template <int NT, int K> __device__ inline void wg_sum_all(double (&v)[K], doubl
   const int w = threadIdx.x / WAVE;
 #pragma unroll
   for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
+  if (NW == 1) return;  // one wavefront: its sum is already in every lane
   if ((threadIdx.x & (WAVE - 1)) == 0)
     for (int k = 0; k < K; ++k) lds[w * K + k] = v[k];
   __syncthreads();

@@ -736,12 +736,20 @@ __device__ inline bool accd_surely_clear(const f3 *pa, int na, const f3 *pb, int
   return 1.25f * (u * (1.0f + 1.0e-5f) + 1.0e-12f) < 0.98f * gap;
 }
 
-// The same argument bounds the search: every candidate has |dx| <= max_disp, so a pair whose
-// boxes are more than 1.3 (dq + max_disp) apart (dq: the query's largest |dx|) has
-// 1.25 u_max < 0.98 d0 and is skipped; the query's padding is the smaller of the
-// widening pass's and that distance.
-__device__ inline float ccd_query_pad(float pad, float dq, float max_disp) {
-  return fminf(pad, 1.3f * (dq + max_disp) * (1.0f + 1.0e-5f) + 1.0e-9f);
+// The same argument bounds the search (dq: the query's largest |dx|, every vertex has
+// |dx| <= max_disp). u_max, the largest |dx_i - dx_j| over the pair's four vertices, is at
+// most 2 max_disp, so a pair whose boxes are more than 1.3 (2 max_disp) apart returns the
+// horizon at the first iterate. A tighter bound holds for the pair's distance: every point
+// of one primitive moves relative to every point of the other by at most W = dq + max_disp,
+// so with boxes more than 1.3 W apart the distance stays above 0.0385 d0 for t <= 1.25; the
+// advancement then moves by at least 0.0185 of the step per iteration and returns the
+// horizon within 68 iterations, never a time of impact. With an iteration limit of at least
+// CCD_TIGHT_ITERS the query's padding is capped at 1.3 W, otherwise at 1.3 (2 max_disp); the
+// padding is the smaller of the widening pass's and that cap.
+constexpr uint32_t CCD_TIGHT_ITERS = 80;
+__device__ inline float ccd_query_pad(float pad, float dq, float max_disp, uint32_t iters) {
+  const float w = iters >= CCD_TIGHT_ITERS ? dq + max_disp : 2.0f * max_disp;
+  return fminf(pad, 1.3f * w * (1.0f + 1.0e-5f) + 1.0e-9f);
 }
 struct CcdDefer {
   uint32_t *count;  // pairs appended
@@ -795,7 +803,7 @@ __global__ __launch_bounds__(BLOCK) void k_ccd_vt(Topo T, BvhView dyn, BvhView s
     const bool below = T.side ? T.side[v] < 0.0f : false;
     ExcList exl;
     exl.load(T, v);
-    const float pad = ccd_query_pad(pad0, len(dvv), max_disp);
+    const float pad = ccd_query_pad(pad0, len(dvv), max_disp, iters);
     auto leaf = [&](uint32_t t) {
       uint32_t t0 = T.tris[3 * (size_t)t], t1 = T.tris[3 * (size_t)t + 1], t2 = T.tris[3 * (size_t)t + 2];
       f3 a = ld3(x, t0), b = ld3(x, t1), c = ld3(x, t2);
@@ -843,7 +851,7 @@ __global__ __launch_bounds__(BLOCK) void k_ccd_ee(Topo T, BvhView dyn, const uin
     uint32_t b0 = T.edges[2 * (size_t)ej], b1 = T.edges[2 * (size_t)ej + 1];
     f3 pb = ld3(x, b0), qb = ld3(x, b1);
     const f3 db0 = ld3(dx, b0), db1 = ld3(dx, b1);
-    const float pad = ccd_query_pad(pad0, fmaxf(len(db0), len(db1)), max_disp);
+    const float pad = ccd_query_pad(pad0, fmaxf(len(db0), len(db1)), max_disp, iters);
     f3 blo = vmin(pb, qb) - f3{pad, pad, pad}, bhi = vmax(pb, qb) + f3{pad, pad, pad};
     auto ov = [&](f3 lo, f3 hi) { return !(lo.x > bhi.x || lo.y > bhi.y || lo.z > bhi.z || hi.x < blo.x || hi.y < blo.y || hi.z < blo.z); };
     // a leaf loads its edge's positions and motions in one memory round trip after the vertex
