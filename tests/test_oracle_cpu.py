@@ -89,6 +89,46 @@ def test_accd_known_answers(fp64):
     assert lib.oracle_accd_toi(fp64, 0, O.dp(x), O.dp(dx), 0.0, c["max_t"], c["reduction"], c["iters"]) == c["toi_exact"]
 
 
+@pytest.mark.parametrize("kind", [0, 1])
+def test_accd_horizon_beyond_the_query_cap(kind):
+    # The device caps each CCD query's search padding at 1.3 W, W = |dx| of the query plus
+    # the largest |dx|: every point of one primitive moves relative to the other's by at most
+    # W, so the distance stays above 0.0385 d0 up to t = 1.25 and the FP64 advancement reaches
+    # its horizon within 68 iterations. Random pairs whose boxes are 1.3 W .. 2 W apart, with
+    # the candidate deforming by up to twice the largest |dx|, all return the horizon with a
+    # limit of 68; head-on pairs at 1.1 W do not, which shows the check can fail. Beyond
+    # 1.3 x twice the largest |dx| the horizon comes at the first iterate (the cap used with an
+    # iteration limit below 80).
+    rng = np.random.default_rng(17 + kind)
+    nq = 1 if kind == 0 else 2
+
+    def pair(gap_factor, head_on, iters=68, wide=False):
+        maxd = rng.uniform(0.01, 0.2)
+        dq = rng.uniform(0.0, 1.0) * maxd
+        cand = rng.uniform(0.0, 1.0, size=(4 - nq, 3)) * rng.uniform(0.05, 1.0)
+        w = 2.0 * maxd if wide else dq + maxd
+        q = rng.uniform(0.0, 1.0, size=(nq, 3)) * 0.2
+        q[:, 0] += cand[:, 0].max() - q[:, 0].min() + gap_factor * w
+        q[:, 1:] += cand[:, 1:].mean(axis=0) - q[:, 1:].mean(axis=0)
+        unit = lambda n: (lambda v: v / np.linalg.norm(v, axis=1, keepdims=True))(rng.normal(size=(n, 3)))
+        if head_on:
+            dxq = np.tile([[-dq, 0.0, 0.0]], (nq, 1))
+            dxc = np.tile([[maxd, 0.0, 0.0]], (4 - nq, 1))
+        else:
+            dxq = unit(nq) * dq * rng.uniform(0.0, 1.0, size=(nq, 1))
+            dxc = unit(4 - nq) * maxd * rng.uniform(0.5, 1.0, size=(4 - nq, 1))
+        x = np.concatenate([q, cand]).ravel()
+        dx = np.concatenate([dxq, dxc]).ravel()
+        return lib.oracle_accd_toi(1, kind, O.dp(x), O.dp(dx), 0.0, 1.25, 0.01, iters)
+
+    for _ in range(400):
+        assert pair(rng.uniform(1.3001, 2.0), False) == 1.25
+    assert min(pair(1.1, True) for _ in range(20)) < 1.25
+    # below the limit the cap is 1.3 x twice the largest |dx|: the horizon at the first iterate
+    for _ in range(200):
+        assert pair(rng.uniform(1.3001, 2.0), False, iters=1, wide=True) == 1.25
+
+
 def test_lame_known_answers():
     import ctypes as C
     for c in GOLD["lame"]["cases"]:
