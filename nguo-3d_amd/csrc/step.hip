@@ -326,6 +326,30 @@ __device__ inline bool excluded(const Topo &T, uint32_t v, uint32_t t0, uint32_t
   }
   return false;
 }
+// The query vertex's exclusion list (its 1-ring), up to EXC_REG entries held in registers for
+// the leaf tests; a longer list has its tail read from memory.
+constexpr uint32_t EXC_REG = 12;
+struct ExcList {
+  uint32_t r[EXC_REG];
+  uint32_t b, e;
+  __device__ inline void load(const Topo &T, uint32_t v) {
+    b = T.ex_ptr[v];
+    e = T.ex_ptr[v + 1];
+#pragma unroll
+    for (uint32_t u = 0; u < EXC_REG; ++u) r[u] = b + u < e ? T.ex[b + u] : 0xffffffffu;
+  }
+  __device__ inline bool has(const Topo &T, uint32_t t0, uint32_t t1, uint32_t t2) const {
+    bool hit = false;
+#pragma unroll
+    for (uint32_t u = 0; u < EXC_REG; ++u) hit |= r[u] == t0 || r[u] == t1 || r[u] == t2;
+    for (uint32_t k = b + EXC_REG; k < e && !hit; ++k) {
+      const uint32_t x = T.ex[k];
+      hit = x == t0 || x == t1 || x == t2;
+    }
+    return hit;
+  }
+};
+
 __device__ inline bool inside_padded(f3 p, f3 lo, f3 hi, float pad) {
   f3 l = lo - f3{pad, pad, pad}, h = hi + f3{pad, pad, pad};
   return !(p.x < l.x || p.y < l.y || p.z < l.z || p.x > h.x || p.y > h.y || p.z > h.z);
@@ -392,16 +416,23 @@ __global__ __launch_bounds__(BLOCK) void k_vt_stencils(Topo T, BvhView dyn, BvhV
   if (v >= T.V) return;
   f3 p = ld3(x, v);
   bool mv = movable(T, v);
+  const bool below = T.side ? T.side[v] < 0.0f : false;
+  ExcList exl;
+  exl.load(T, v);
+  // a leaf loads its positions and flags in one memory round trip after the triangle's
+  // vertex ids and tests the filters (predicates, in any order) from registers
   auto leaf = [&](uint32_t t) {
     uint32_t t0 = T.tris[3 * (size_t)t], t1 = T.tris[3 * (size_t)t + 1], t2 = T.tris[3 * (size_t)t + 2];
     f3 a = ld3(x, t0), b = ld3(x, t1), c = ld3(x, t2);
+    const float m0 = T.im[t0], m1 = T.im[t1], m2 = T.im[t2];
+    const bool st_all = T.static_v ? (T.static_v[t0] && T.static_v[t1] && T.static_v[t2]) : false;
     if (!inside_padded(p, vmin(vmin(a, b), c), vmax(vmax(a, b), c), dhat)) return;
-    bool tm = movable(T, t0) || movable(T, t1) || movable(T, t2);
+    bool tm = m0 > 0.0f || m1 > 0.0f || m2 > 0.0f;
     if (!tm && !mv) return;
-    if (excluded(T, v, t0, t1, t2)) return;
+    if (exl.has(T, t0, t1, t2)) return;
     f3 g[4];
     float d = point_tri_distance_grad(p, a, b, c, g);
-    if (T.static_v && T.static_v[t0] && T.static_v[t1] && T.static_v[t2] && T.side[v] < 0.0f) {
+    if (st_all && below) {
       d = -d;
       for (int k = 0; k < 4; ++k) g[k] = -g[k];
     }
@@ -427,16 +458,18 @@ __global__ __launch_bounds__(BLOCK) void k_ee_stencils(Topo T, BvhView dyn, cons
   f3 pb = ld3(x, b0), qb = ld3(x, b1);
   f3 blo = vmin(pb, qb) - f3{pad, pad, pad}, bhi = vmax(pb, qb) + f3{pad, pad, pad};
   auto ov = [&](f3 lo, f3 hi) { return !(lo.x > bhi.x || lo.y > bhi.y || lo.z > bhi.z || hi.x < blo.x || hi.y < blo.y || hi.z < blo.z); };
+  // a leaf reuses the query edge's positions from registers (the pair's vertices in
+  // edge-index order: the lower edge's two, then the higher's)
   bvh_traverse_part(dyn, blockIdx.y, err, tstack, ov, [&](uint32_t ei) {
     if (ei == ej || (query_dyn && ei > ej)) return;
     uint32_t a0 = T.edges[2 * (size_t)ei], a1 = T.edges[2 * (size_t)ei + 1];
     if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) return;
     f3 pa = ld3(x, a0), qa = ld3(x, a1);
     if (!ov(vmin(pa, qa), vmax(pa, qa))) return;
-    uint32_t lo_e = min(ei, ej), hi_e = max(ei, ej);
-    uint32_t i0 = T.edges[2 * (size_t)lo_e], i1 = T.edges[2 * (size_t)lo_e + 1], i2 = T.edges[2 * (size_t)hi_e], i3 = T.edges[2 * (size_t)hi_e + 1];
+    const bool a_lo = ei < ej;
+    const uint32_t lo_e = min(ei, ej), hi_e = max(ei, ej);
     f3 g[4];
-    float d = segment_segment_distance_grad(ld3(x, i0), ld3(x, i1), ld3(x, i2), ld3(x, i3), g);
+    float d = segment_segment_distance_grad(a_lo ? pa : pb, a_lo ? qa : qb, a_lo ? pb : pa, a_lo ? qb : qa, g);
     if (d - 0.0f >= ghat) return;
     emit(O, ((uint64_t)lo_e << 32) | hi_e, d - 0.0f, g);
   });
@@ -763,30 +796,6 @@ __device__ inline bool defer_pair(const CcdDefer &D, uint32_t a, uint32_t b, uin
   D.pairs[2 * (size_t)k + 1] = b;
   return true;
 }
-
-// The query vertex's exclusion list (its 1-ring), up to EXC_REG entries held in registers for
-// the leaf tests; a longer list has its tail read from memory.
-constexpr uint32_t EXC_REG = 12;
-struct ExcList {
-  uint32_t r[EXC_REG];
-  uint32_t b, e;
-  __device__ inline void load(const Topo &T, uint32_t v) {
-    b = T.ex_ptr[v];
-    e = T.ex_ptr[v + 1];
-#pragma unroll
-    for (uint32_t u = 0; u < EXC_REG; ++u) r[u] = b + u < e ? T.ex[b + u] : 0xffffffffu;
-  }
-  __device__ inline bool has(const Topo &T, uint32_t t0, uint32_t t1, uint32_t t2) const {
-    bool hit = false;
-#pragma unroll
-    for (uint32_t u = 0; u < EXC_REG; ++u) hit |= r[u] == t0 || r[u] == t1 || r[u] == t2;
-    for (uint32_t k = b + EXC_REG; k < e && !hit; ++k) {
-      const uint32_t x = T.ex[k];
-      hit = x == t0 || x == t1 || x == t2;
-    }
-    return hit;
-  }
-};
 
 // Leaves load their positions, motions and flags in one memory round trip (after the
 // triangle's vertex ids) and test the filters from registers: the filters are predicates, so
