@@ -37,12 +37,14 @@ struct BvhView {
   const float *node_box;  // [6 (n-1)] lo.xyz hi.xyz
   const float *leaf_box;  // [6 n] in sorted order
   const uint32_t *prim;   // [n] primitive id of sorted leaf slot
-  // [4 (n-1)] traversal nodes: both child boxes and child refs of internal node i in one
-  // 64-byte record (lo_l.xyz hi_l.x | hi_l.yz lo_r.xy | lo_r.z hi_r.xyz | left right - -)
+  // [4 (n-1)] traversal nodes: both child boxes, child refs and the last sorted leaf slot
+  // under each child of internal node i in one 64-byte record
+  // (lo_l.xyz hi_l.x | hi_l.yz lo_r.xy | lo_r.z hi_r.xyz | left right last_l last_r)
   const float4 *trav;
   // [2 TRAV_PARTS] the tree split into disjoint subtrees that together hold every leaf once:
-  // entry k = (lo.xyz, ref) (hi.xyz, 0), ref ~0u for an unused entry
+  // entry k = (lo.xyz, ref) (hi.xyz, last leaf slot), ref ~0u for an unused entry
   const float4 *front;
+  const uint32_t *slot;   // [n] sorted leaf slot of element k of the build's primitive list
 };
 
 // Subtrees a traversal query is split into (one thread each).
@@ -53,7 +55,9 @@ class Lbvh {
   // kind 0: triangles (3 indices per primitive), 1: edges (2 indices). `ids` selects the
   // primitives of this tree (device array of primitive indices into `idx`).
   void build(int kind, uint32_t n, const uint32_t *ids, const uint32_t *idx, const float *pos, hipStream_t s);
-  BvhView view() const { return BvhView{n_, left_.p, right_.p, node_box_.p, leaf_box_.p, prim_.p, (const float4 *)trav_.p, (const float4 *)front_.p}; }
+  BvhView view() const {
+    return BvhView{n_, left_.p, right_.p, node_box_.p, leaf_box_.p, prim_.p, (const float4 *)trav_.p, (const float4 *)front_.p, slot_.p};
+  }
   uint32_t size() const { return n_; }
   // Stage outputs for parity checks.
   const uint32_t *codes() const { return keys_.p; }   // sorted Morton keys
@@ -61,7 +65,7 @@ class Lbvh {
 
  private:
   uint32_t n_ = 0;
-  DevBuf<uint32_t> keys_, prim_, left_, right_, range_, visit_;
+  DevBuf<uint32_t> keys_, prim_, left_, right_, range_, visit_, slot_;
   DevBuf<float> leaf_box_, node_box_, bounds_, cent_, table_, trav_, front_;
 };
 
@@ -81,12 +85,15 @@ void aabb_from_ranges(uint32_t n, const uint32_t *range, const float *leaf_box, 
 // that a wavefront's pushes and pops fall in distinct banks); deeper entries go to a private
 // array. A stack deeper than TRAV_STACK sets *err (reported to the caller as
 // NGUO_E_CAPACITY) instead of silently dropping a subtree. Requires BLOCK threads per
-// workgroup.
+// workgroup. Only leaves in sorted slots >= min_slot are visited, and subtrees whose last
+// slot lies below it are not entered (a query of a tree against itself visits each pair
+// once by passing its own slot + 1).
 constexpr int TRAV_STACK = 96;
 constexpr int TRAV_LDS = 32;
 // Depth-first traversal of the subtree under internal node `root`.
 template <class Overlap, class Visit>
-__device__ inline void bvh_traverse_from(const BvhView &t, uint32_t root, uint32_t *err, uint32_t *lds, Overlap overlap, Visit visit) {
+__device__ inline void bvh_traverse_from(const BvhView &t, uint32_t root, uint32_t *err, uint32_t *lds, Overlap overlap, Visit visit,
+                                         uint32_t min_slot = 0) {
   uint32_t spill[TRAV_STACK - TRAV_LDS];
   uint32_t *mine = lds + threadIdx.x;
   mine[0] = root;
@@ -97,11 +104,12 @@ __device__ inline void bvh_traverse_from(const BvhView &t, uint32_t root, uint32
     const float4 *rec = t.trav + 4 * (size_t)node;
     const float4 q0 = rec[0], q1 = rec[1], q2 = rec[2], q3 = rec[3];
     const uint32_t ch[2] = {__float_as_uint(q3.x), __float_as_uint(q3.y)};
+    const uint32_t last[2] = {__float_as_uint(q3.z), __float_as_uint(q3.w)};
     const f3 lo[2] = {f3{q0.x, q0.y, q0.z}, f3{q1.z, q1.w, q2.x}};
     const f3 hi[2] = {f3{q0.w, q1.x, q1.y}, f3{q2.y, q2.z, q2.w}};
     for (int c = 0; c < 2; ++c) {
       const uint32_t r = ch[c];
-      if (!overlap(lo[c], hi[c])) continue;
+      if (last[c] < min_slot || !overlap(lo[c], hi[c])) continue;
       if (r & LEAF) {
         visit(t.prim[r & ~LEAF]);
       } else if (sp < TRAV_LDS) {
@@ -119,6 +127,7 @@ __device__ inline void bvh_traverse_from(const BvhView &t, uint32_t root, uint32
 
 template <class Overlap, class Visit>
 __device__ inline void bvh_traverse(const BvhView &t, uint32_t *err, uint32_t *lds, Overlap overlap, Visit visit) {
+  // whole-tree form (no slot bound)
   if (t.n == 0) return;
   if (t.n == 1) {
     const float *b = t.leaf_box;
@@ -131,14 +140,15 @@ __device__ inline void bvh_traverse(const BvhView &t, uint32_t *err, uint32_t *l
 // The part `part` (< TRAV_PARTS) of a traversal split over the tree's frontier subtrees: the
 // parts of one query visit disjoint leaf sets whose union is what bvh_traverse visits.
 template <class Overlap, class Visit>
-__device__ inline void bvh_traverse_part(const BvhView &t, uint32_t part, uint32_t *err, uint32_t *lds, Overlap overlap, Visit visit) {
+__device__ inline void bvh_traverse_part(const BvhView &t, uint32_t part, uint32_t *err, uint32_t *lds, Overlap overlap, Visit visit,
+                                         uint32_t min_slot = 0) {
   if (t.n == 0) return;
   const float4 a = t.front[2 * part], b = t.front[2 * part + 1];
   const uint32_t r = __float_as_uint(a.w);
-  if (r == 0xffffffffu) return;
+  if (r == 0xffffffffu || __float_as_uint(b.w) < min_slot) return;
   if (!overlap(f3{a.x, a.y, a.z}, f3{b.x, b.y, b.z})) return;
   if (r & LEAF) visit(t.prim[r & ~LEAF]);
-  else bvh_traverse_from(t, r, err, lds, overlap, visit);
+  else bvh_traverse_from(t, r, err, lds, overlap, visit, min_slot);
 }
 
 }  // namespace nguo

@@ -466,21 +466,30 @@ void aabb_from_ranges(uint32_t n, const uint32_t *range, const float *leaf_box, 
 }
 
 // Traversal records: child boxes and refs of each internal node in 64 bytes.
-__global__ __launch_bounds__(BLOCK) void k_pack_trav(uint32_t n, const uint32_t *left, const uint32_t *right, const float *leaf_box,
-                                                    const float *node_box, float4 *trav) {
+__global__ __launch_bounds__(BLOCK) void k_pack_trav(uint32_t n, const uint32_t *left, const uint32_t *right, const uint32_t *range,
+                                                    const float *leaf_box, const float *node_box, float4 *trav) {
   uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
   if (i + 1 >= n) return;
   const uint32_t ch[2] = {left[i], right[i]};
   float b[2][6];
+  uint32_t last[2];
   for (int c = 0; c < 2; ++c) {
-    const float *src = (ch[c] & LEAF) ? leaf_box + 6 * (size_t)(ch[c] & ~LEAF) : node_box + 6 * (size_t)ch[c];
+    const bool leaf = ch[c] & LEAF;
+    const float *src = leaf ? leaf_box + 6 * (size_t)(ch[c] & ~LEAF) : node_box + 6 * (size_t)ch[c];
     for (int k = 0; k < 6; ++k) b[c][k] = src[k];
+    last[c] = leaf ? ch[c] & ~LEAF : range[2 * (size_t)ch[c] + 1];
   }
   float4 *o = trav + 4 * (size_t)i;
   o[0] = make_float4(b[0][0], b[0][1], b[0][2], b[0][3]);
   o[1] = make_float4(b[0][4], b[0][5], b[1][0], b[1][1]);
   o[2] = make_float4(b[1][2], b[1][3], b[1][4], b[1][5]);
-  o[3] = make_float4(__uint_as_float(ch[0]), __uint_as_float(ch[1]), 0.0f, 0.0f);
+  o[3] = make_float4(__uint_as_float(ch[0]), __uint_as_float(ch[1]), __uint_as_float(last[0]), __uint_as_float(last[1]));
+}
+
+// slot[k] = the sorted leaf slot holding element k of the build's primitive list.
+__global__ __launch_bounds__(BLOCK) void k_slot_of(uint32_t n, const uint32_t *order, uint32_t *slot) {
+  const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i < n) slot[order[i]] = i;
 }
 
 // The traversal frontier: starting from the root's children, internal entries are replaced by
@@ -489,12 +498,12 @@ __global__ __launch_bounds__(BLOCK) void k_pack_trav(uint32_t n, const uint32_t 
 __global__ void k_frontier(uint32_t n, const float4 *trav, const float *leaf_box, float4 *front) {
   if (threadIdx.x != 0) return;
   uint32_t ref[TRAV_PARTS];
-  float4 lo[TRAV_PARTS], hi[TRAV_PARTS];
+  float4 lo[TRAV_PARTS], hi[TRAV_PARTS];  // hi.w: the entry's last leaf slot
   uint32_t cnt = 0;
   if (n == 1) {
     ref[0] = LEAF;
     lo[0] = make_float4(leaf_box[0], leaf_box[1], leaf_box[2], 0.0f);
-    hi[0] = make_float4(leaf_box[3], leaf_box[4], leaf_box[5], 0.0f);
+    hi[0] = make_float4(leaf_box[3], leaf_box[4], leaf_box[5], __uint_as_float(0u));
     cnt = 1;
   } else if (n >= 2) {
     auto children = [&](uint32_t node, uint32_t o0, uint32_t o1) {
@@ -502,8 +511,8 @@ __global__ void k_frontier(uint32_t n, const float4 *trav, const float *leaf_box
       const float4 q0 = rec[0], q1 = rec[1], q2 = rec[2], q3 = rec[3];
       ref[o0] = __float_as_uint(q3.x);
       ref[o1] = __float_as_uint(q3.y);
-      lo[o0] = make_float4(q0.x, q0.y, q0.z, 0.0f); hi[o0] = make_float4(q0.w, q1.x, q1.y, 0.0f);
-      lo[o1] = make_float4(q1.z, q1.w, q2.x, 0.0f); hi[o1] = make_float4(q2.y, q2.z, q2.w, 0.0f);
+      lo[o0] = make_float4(q0.x, q0.y, q0.z, 0.0f); hi[o0] = make_float4(q0.w, q1.x, q1.y, q3.z);
+      lo[o1] = make_float4(q1.z, q1.w, q2.x, 0.0f); hi[o1] = make_float4(q2.y, q2.z, q2.w, q3.w);
     };
     children(0u, 0, 1);
     cnt = 2;
@@ -523,7 +532,7 @@ __global__ void k_frontier(uint32_t n, const float4 *trav, const float *leaf_box
   for (uint32_t k = 0; k < TRAV_PARTS; ++k) {
     const bool used = k < cnt;
     front[2 * k] = used ? make_float4(lo[k].x, lo[k].y, lo[k].z, __uint_as_float(ref[k])) : make_float4(0.f, 0.f, 0.f, __uint_as_float(0xffffffffu));
-    front[2 * k + 1] = used ? hi[k] : make_float4(0.f, 0.f, 0.f, 0.f);
+    front[2 * k + 1] = used ? hi[k] : make_float4(0.f, 0.f, 0.f, __uint_as_float(0u));
   }
 }
 
@@ -561,6 +570,8 @@ void Lbvh::build(int kind, uint32_t n, const uint32_t *ids, const uint32_t *idx,
   hipLaunchKernelGGL(k_morton, dim3(g), dim3(BLOCK), 0, s, n, (const float *)cent_.p, (const float *)bounds_.p, keys_.p, order.p);
   sorter.sort_u32(n, keys_.p, order.p, {0, 8, 16, 24}, s);
   hipLaunchKernelGGL(k_gather_prim, dim3(g), dim3(BLOCK), 0, s, n, (const uint32_t *)order.p, ids, prim_.p);
+  slot_.reserve(n);
+  hipLaunchKernelGGL(k_slot_of, dim3(g), dim3(BLOCK), 0, s, n, (const uint32_t *)order.p, slot_.p);
   hipLaunchKernelGGL(k_leaf_boxes, dim3(g), dim3(BLOCK), 0, s, kind, n, (const uint32_t *)prim_.p, idx, pos, leaf_box_.p);
   if (n >= 2) {
     hipLaunchKernelGGL(k_karras, dim3(div_up(n - 1, BLOCK)), dim3(BLOCK), 0, s, n, (const uint32_t *)keys_.p, left_.p, right_.p,
@@ -568,7 +579,7 @@ void Lbvh::build(int kind, uint32_t n, const uint32_t *ids, const uint32_t *idx,
     aabb_from_ranges(n, range_.p, leaf_box_.p, node_box_.p, table_, s);
     trav_.reserve(16 * (size_t)n);
     hipLaunchKernelGGL(k_pack_trav, dim3(div_up(n - 1, BLOCK)), dim3(BLOCK), 0, s, n, (const uint32_t *)left_.p, (const uint32_t *)right_.p,
-                       (const float *)leaf_box_.p, (const float *)node_box_.p, (float4 *)trav_.p);
+                       (const uint32_t *)range_.p, (const float *)leaf_box_.p, (const float *)node_box_.p, (float4 *)trav_.p);
   }
   front_.reserve(8 * TRAV_PARTS);
   hipLaunchKernelGGL(k_frontier, dim3(1), dim3(WAVE), 0, s, n, (const float4 *)trav_.p, (const float *)leaf_box_.p, (float4 *)front_.p);

@@ -446,8 +446,9 @@ __global__ __launch_bounds__(BLOCK) void k_vt_stencils(Topo T, BvhView dyn, BvhV
   else if (mv) bvh_traverse_part(stat, part - TRAV_PARTS, err, tstack, ov, leaf);
 }
 
-// implicit.rs:719-751 edge-edge stencils. Query edges are padded, tree edges are not; a
-// movable query pairs only with lower-indexed tree edges, a static query with any.
+// implicit.rs:719-751 edge-edge stencils. Query edges are padded, tree edges are not (the
+// overlap is symmetric); each movable pair is met once, by the query whose sorted leaf slot
+// is the lower, a static query meets every movable edge. Keys are (lower, higher edge id).
 __global__ __launch_bounds__(BLOCK) void k_ee_stencils(Topo T, BvhView dyn, const uint32_t *qids, uint32_t nq, int query_dyn,
                                                       const float *__restrict__ x, float pad, float ghat, StencilOut O, uint32_t *err) {
   __shared__ uint32_t tstack[TRAV_LDS * BLOCK];
@@ -459,9 +460,11 @@ __global__ __launch_bounds__(BLOCK) void k_ee_stencils(Topo T, BvhView dyn, cons
   f3 blo = vmin(pb, qb) - f3{pad, pad, pad}, bhi = vmax(pb, qb) + f3{pad, pad, pad};
   auto ov = [&](f3 lo, f3 hi) { return !(lo.x > bhi.x || lo.y > bhi.y || lo.z > bhi.z || hi.x < blo.x || hi.y < blo.y || hi.z < blo.z); };
   // a leaf reuses the query edge's positions from registers (the pair's vertices in
-  // edge-index order: the lower edge's two, then the higher's)
+  // edge-index order: the lower edge's two, then the higher's); a movable query meets each
+  // other movable edge once: only the leaves after its own sorted slot
+  const uint32_t min_slot = query_dyn ? dyn.slot[q] + 1 : 0u;
   bvh_traverse_part(dyn, blockIdx.y, err, tstack, ov, [&](uint32_t ei) {
-    if (ei == ej || (query_dyn && ei > ej)) return;
+    if (ei == ej) return;
     uint32_t a0 = T.edges[2 * (size_t)ei], a1 = T.edges[2 * (size_t)ei + 1];
     if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) return;
     f3 pa = ld3(x, a0), qa = ld3(x, a1);
@@ -472,7 +475,7 @@ __global__ __launch_bounds__(BLOCK) void k_ee_stencils(Topo T, BvhView dyn, cons
     float d = segment_segment_distance_grad(a_lo ? pa : pb, a_lo ? qa : qb, a_lo ? pb : pa, a_lo ? qb : qa, g);
     if (d - 0.0f >= ghat) return;
     emit(O, ((uint64_t)lo_e << 32) | hi_e, d - 0.0f, g);
-  });
+  }, min_slot);
 }
 
 // Gathers the sorted stencils (V-T block then E-E block) into one array.
@@ -864,9 +867,12 @@ __global__ __launch_bounds__(BLOCK) void k_ccd_ee(Topo T, BvhView dyn, const uin
     f3 blo = vmin(pb, qb) - f3{pad, pad, pad}, bhi = vmax(pb, qb) + f3{pad, pad, pad};
     auto ov = [&](f3 lo, f3 hi) { return !(lo.x > bhi.x || lo.y > bhi.y || lo.z > bhi.z || hi.x < blo.x || hi.y < blo.y || hi.z < blo.z); };
     // a leaf loads its edge's positions and motions in one memory round trip after the vertex
-    // ids, and reuses the query edge's from registers
+    // ids, and reuses the query edge's from registers. A movable query meets each other
+    // movable edge once: only the leaves after its own sorted slot (whole subtrees before it
+    // are skipped).
+    const uint32_t min_slot = query_dyn ? dyn.slot[q] + 1 : 0u;
     bvh_traverse_part(dyn, blockIdx.y, err, tstack, ov, [&](uint32_t ei) {
-      if (ei == ej || (query_dyn && ei > ej)) return;
+      if (ei == ej) return;
       uint32_t a0 = T.edges[2 * (size_t)ei], a1 = T.edges[2 * (size_t)ei + 1];
       if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) return;
       f3 pa = ld3(x, a0), qa = ld3(x, a1);
@@ -887,7 +893,7 @@ __global__ __launch_bounds__(BLOCK) void k_ccd_ee(Topo T, BvhView dyn, const uin
         toi = accd_toi(x0, d4, 1, 0.0, 1.25, 0.01, iters);
       }
       best = fmin(best, toi / 1.25);
-    });
+    }, min_slot);
   }
   // one atomic per wavefront: the minimum is order-independent
   const double wb = wave_min(best);
