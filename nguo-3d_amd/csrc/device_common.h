@@ -89,6 +89,7 @@ struct SellView {
   const uint32_t *row_len;        // [nb] blocks in each row
   const uint32_t *col;            // [slots]
   const float *val;               // [slots * 9], plane layout (see val_index)
+  uint32_t ell_w;                 // nonzero: every slice has this width (slice_ptr[s] = 64 s ell_w)
 };
 
 __host__ __device__ inline size_t sell_val_index(uint32_t slot, int e) {

@@ -13,6 +13,9 @@ namespace nguo {
 // Pattern + values of a SELL-64 3x3-block matrix on the device, with its host pattern.
 struct SellMatrix {
   uint32_t nb = 0, nslices = 0, nslots = 0, nnzb = 0;
+  // nonzero: all slices padded to this width (when that costs at most ELL_MAX_PAD more
+  // slots), so a slice's first slot is 64 s ell_w without reading slice_ptr
+  uint32_t ell_w = 0;
   std::vector<uint32_t> h_slice_ptr, h_row_len, h_col;
   DevBuf<uint32_t> slice_ptr, row_len, col, diag_slot;
   DevBuf<float> val;
@@ -21,7 +24,7 @@ struct SellMatrix {
   uint32_t slot(uint32_t row, uint32_t k) const { return h_slice_ptr[row / 64] + 64u * k + (row & 63u); }
   // slot of block (row, col) from the host pattern, or 0xffffffff.
   uint32_t find(uint32_t row, uint32_t colj) const;
-  SellView view() const { return SellView{nb, slice_ptr.p, row_len.p, col.p, val.p}; }
+  SellView view() const { return SellView{nb, slice_ptr.p, row_len.p, col.p, val.p, ell_w}; }
   // Algorithmic bytes of one y = A x (f32 values, u32 column index per 3x3 block; row
   // length, x and y per block row). The PCG iteration's instance also reads z and writes p
   // (24 more bytes per block row).

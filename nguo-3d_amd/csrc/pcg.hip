@@ -18,12 +18,18 @@ namespace nguo {
 constexpr uint32_t NONE = 0xffffffffu;
 
 // ---------------------------------------------------------------- pattern
+// Padding (in slots) accepted to give every slice the same width.
+constexpr double ELL_MAX_PAD = 1.15;
+
 void SellMatrix::build_pattern(uint32_t nblocks, const std::vector<std::vector<uint32_t>> &cols, hipStream_t s) {
   nb = nblocks;
   nslices = (nb + 63) / 64;
   h_slice_ptr.assign(nslices + 1, 0);
   h_row_len.assign(nb, 0);
   nnzb = 0;
+  std::vector<uint32_t> width(nslices, 0);
+  uint64_t sum_w = 0;
+  uint32_t max_w = 0;
   for (uint32_t sl = 0; sl < nslices; ++sl) {
     uint32_t mx = 0;
     for (uint32_t r = sl * 64; r < std::min(nb, sl * 64 + 64); ++r) {
@@ -31,8 +37,12 @@ void SellMatrix::build_pattern(uint32_t nblocks, const std::vector<std::vector<u
       mx = std::max(mx, h_row_len[r]);
       nnzb += h_row_len[r];
     }
-    h_slice_ptr[sl + 1] = h_slice_ptr[sl] + 64u * mx;
+    width[sl] = mx;
+    sum_w += mx;
+    max_w = std::max(max_w, mx);
   }
+  ell_w = (max_w > 0 && (double)max_w * nslices <= ELL_MAX_PAD * (double)sum_w) ? max_w : 0u;
+  for (uint32_t sl = 0; sl < nslices; ++sl) h_slice_ptr[sl + 1] = h_slice_ptr[sl] + 64u * (ell_w ? ell_w : width[sl]);
   nslots = h_slice_ptr[nslices];
   h_col.assign(nslots, 0);
   std::vector<uint32_t> dslot(nb, NONE);
@@ -286,7 +296,8 @@ __device__ __forceinline__ void pcg_spmv_body(uint32_t slice, const SellView &A,
   const bool active = row < A.nb;
   // lanes past nb (last slice) read row nb - 1 and the slice's padding slots, and add nothing
   const uint32_t rr = active ? row : A.nb - 1u;
-  const uint32_t base = A.slice_ptr[slice], width = (A.slice_ptr[slice + 1] - base) / 64u;
+  const uint32_t base = A.ell_w ? 64u * A.ell_w * slice : A.slice_ptr[slice];
+  const uint32_t width = A.ell_w ? A.ell_w : (A.slice_ptr[slice + 1] - base) / 64u;
   const uint32_t len0 = A.row_len[rr];
   uint32_t ib = 0, ie = 0;
   if (MODE == SPMV_PLAIN && C.ns) {
