@@ -118,8 +118,7 @@ __device__ inline void sym_tile_pq(uint32_t tile, uint32_t &P, uint32_t &Q) {
 // row-major with row stride MD (columns m..m4-1 zero). All threads of the workgroup call it;
 // the LDS arrays are the caller's.
 template <bool TILED>
-__device__ void factor_invert(double *L, double *W, double *diag0, double *colj, double &floor_v, int &failed, uint32_t m, float *out,
-                              uint32_t *fails) {
+__device__ void factor_invert(double *L, double *diag0, double *colj, double &floor_v, int &failed, uint32_t m, float *out, uint32_t *fails) {
   const uint32_t t = threadIdx.x, ti = t / 16, tj = t % 16;
   // floor 1e-6 tr/m + 1e-8 (linsolve.rs:232-238)
   if (t == 0) {
@@ -175,22 +174,33 @@ __device__ void factor_invert(double *L, double *W, double *diag0, double *colj,
     }
     return;
   }
-  // W = L^-1 (lower) by forward substitution on all columns at once: row i is scaled by
-  // 1 / l_ii, then its multiples leave the rows below (entry (r, c) accumulates
-  // -l_rk w_kc for k = c .. r-1 in order, then the division, as a column-by-column
-  // substitution would).
-  for (uint32_t e = t; e < m * m; e += BLOCK) W[e] = (e / m == e % m) ? 1.0 : 0.0;
-  __syncthreads();
-  for (uint32_t i = 0; i < m; ++i) {
-    const double lii = L[i * m + i];
-    for (uint32_t c = t; c <= i; c += BLOCK) W[i * m + c] = W[i * m + c] / lii;
+  // W = L^-1 in place of L's lower triangle, a column at a time from the last (the unblocked
+  // lower-triangular inverse): w_jj = 1 / l_jj and, below the diagonal, w_ij = -w_jj
+  // sum_{k = j+1 .. i} w_ik l_kj with the trailing block already inverted and the original
+  // column j copied to colj. A row's sum is split over the 16 lanes of a DPP row (lane q takes
+  // k = j + 1 + q, q + 16, ...) and combined by the row exchanges, a fixed order. L's upper
+  // triangle (A's entries) is left as it is and never read as W. One m x m array in LDS
+  // instead of two lets two workgroups share a compute unit.
+  for (int jj = (int)m - 1; jj >= 0; --jj) {
+    const uint32_t j = (uint32_t)jj;
+    for (uint32_t i = j + 1 + t; i < m; i += BLOCK) colj[i] = L[i * m + j];
     __syncthreads();
-    for (uint32_t r = i + 1 + ti; r < m; r += 16) {
-      const double lri = L[r * m + i];
-      for (uint32_t c = tj; c <= i; c += 16) W[r * m + c] -= lri * W[i * m + c];
+    const double wjj = 1.0 / L[j * m + j];
+    for (uint32_t i0 = j + 1; i0 < m; i0 += 16) {
+      const uint32_t i = i0 + ti;
+      double acc = 0.0;
+      if (i < m)
+        for (uint32_t k = j + 1 + tj; k <= i; k += 16) acc += L[i * m + k] * colj[k];
+      acc += dpp_d<DPP_XOR1>(acc);
+      acc += dpp_d<DPP_XOR2>(acc);
+      acc += dpp_d<DPP_HALF_MIRROR>(acc);
+      acc += dpp_d<DPP_MIRROR>(acc);
+      if (i < m && tj == 0) L[i * m + j] = -wjj * acc;
     }
     __syncthreads();
+    if (t == 0) L[j * m + j] = wjj;
   }
+  __syncthreads();
   // A^-1 = W^T W on the FP64 matrix cores: 16 x 16 output tiles, k in steps of 4
   // (v_mfma_f64_16x16x4f64; A operand lane a = (row a % 16, k a / 16), B operand lane
   // b = (k b / 16, column b % 16), result lane l register v = (row 4 v + l / 16, column
@@ -200,7 +210,7 @@ __device__ void factor_invert(double *L, double *W, double *diag0, double *colj,
   // further), in the tiled form every entry of the 21 tiles (tiles wholly outside run no
   // k step and store their zero accumulator).
   const uint32_t wave = t / WAVE, lane = t % WAVE;
-  auto wget = [&](uint32_t k, uint32_t c) -> double { return (k < m && c < m) ? W[k * m + c] : 0.0; };
+  auto wget = [&](uint32_t k, uint32_t c) -> double { return (k < m && c <= k) ? L[k * m + c] : 0.0; };
   for (uint32_t tile = wave; tile < SCH_NT; tile += BLOCK / WAVE) {
     uint32_t P, Q;
     sym_tile_pq(tile, P, Q);
@@ -233,7 +243,6 @@ __global__ __launch_bounds__(BLOCK) void k_schwarz_build(SellView A, ContactView
                                                         const uint32_t *__restrict__ row_pos, float *__restrict__ ainv, uint32_t *inc_rng,
                                                         uint32_t *fails) {
   __shared__ double L[MD * MD];
-  __shared__ double W[MD * MD];
   __shared__ double diag0[MD];
   __shared__ double colj[MD];
   __shared__ double floor_v;
@@ -286,7 +295,7 @@ __global__ __launch_bounds__(BLOCK) void k_schwarz_build(SellView A, ContactView
     }
     __syncthreads();
   }
-  factor_invert<true>(L, W, diag0, colj, floor_v, failed, m, ainv + (size_t)d * SCHWARZ_INV_FLOATS, fails);
+  factor_invert<true>(L, diag0, colj, floor_v, failed, m, ainv + (size_t)d * SCHWARZ_INV_FLOATS, fails);
 }
 
 // Coarse block of coarse domain D (linsolve.rs:613-625 coarse_matrix, 230-258 factorize):
@@ -300,7 +309,6 @@ __global__ __launch_bounds__(BLOCK) void k_schwarz_build_coarse(SellView A, Cont
                                                                const uint32_t *__restrict__ node_cdom, const uint32_t *__restrict__ node_cpos,
                                                                float *__restrict__ ainv1, uint32_t *fails) {
   __shared__ double L[MD * MD];
-  __shared__ double W[MD * MD];
   __shared__ double diag0[MD];
   __shared__ double colj[MD];
   __shared__ double floor_v;
@@ -345,7 +353,7 @@ __global__ __launch_bounds__(BLOCK) void k_schwarz_build_coarse(SellView A, Cont
     }
   }
   __syncthreads();
-  factor_invert<false>(L, W, diag0, colj, floor_v, failed, m, ainv1 + (size_t)D * MD * MD, fails);
+  factor_invert<false>(L, diag0, colj, floor_v, failed, m, ainv1 + (size_t)D * MD * MD, fails);
 }
 
 // Coarse correction, one workgroup per domain d (member a of coarse domain D): e =

@@ -55,6 +55,52 @@ def test_spmv_matches_oracle(ctx):
     ls.close()
 
 
+def hub_laplacian(n, stride=5, w=0.05):
+    # a 1-D Laplacian plus dof 0 coupled to every stride-th dof (weight -w, diagonally
+    # dominant): the first block row is far longer than the others, so the slices of the
+    # block matrix keep widths of their own instead of one common width
+    rows = [dict() for _ in range(n)]
+    for i in range(n):
+        rows[i][i] = 2.0
+        if i > 0:
+            rows[i][i - 1] = -1.0
+        if i + 1 < n:
+            rows[i][i + 1] = -1.0
+    for j in range(stride, n, stride):
+        rows[0][j] = rows[0].get(j, 0.0) - w
+        rows[j][0] = rows[j].get(0, 0.0) - w
+        rows[j][j] += w
+        rows[0][0] += w
+    rp, ci, va = [0], [], []
+    for r in rows:
+        for j in sorted(r):
+            ci.append(j)
+            va.append(r[j])
+        rp.append(len(ci))
+    return np.array(rp, np.uint32), np.array(ci, np.uint32), np.array(va, np.float32)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("stride", [5, 1000])
+def test_spmv_and_pcg_irregular_rows(ctx, stride):
+    # stride 5: one block row of ~180 blocks among rows of 2-3 (per-slice widths); stride
+    # 1000: no hub coupling, every slice the same width (one common width)
+    n = 3 * 300
+    rp, ci, va = hub_laplacian(n, stride)
+    ls = nguo_hip.LinearSolver(ctx)
+    rng = np.random.default_rng(23)
+    x = rng.standard_normal(n).astype(np.float32)
+    y = ls.spmv(rp, ci, va, x)
+    y64 = np.array([np.dot(va[rp[r]:rp[r + 1]].astype(np.float64), x[ci[rp[r]:rp[r + 1]]].astype(np.float64)) for r in range(n)])
+    assert np.abs(y - y64).max() <= 1e-6 * np.abs(va).max() * np.abs(x).max() * 16
+    xs = np.linspace(-1, 1, n).astype(np.float32)
+    b = O.spmv(rp, ci, va, xs)
+    xsol, st = ls.solve(rp, ci, va, b, 1e-6, 100000)
+    assert st.residual <= 1e-6
+    assert host_residual(rp, ci, va, b, xsol) <= 2e-6
+    ls.close()
+
+
 @pytest.mark.gpu
 def test_pcg_manufactured_laplacian(ctx):
     n = 3 * 200
