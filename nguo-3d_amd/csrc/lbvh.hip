@@ -467,10 +467,10 @@ void aabb_from_ranges(uint32_t n, const uint32_t *range, const float *leaf_box, 
 
 // Traversal records: child boxes and refs of each internal node in 64 bytes.
 __global__ __launch_bounds__(BLOCK) void k_pack_trav(uint32_t n, const uint32_t *left, const uint32_t *right, const uint32_t *range,
-                                                    const float *leaf_box, const float *node_box, float4 *trav) {
+                                                    const uint32_t *prim, const float *leaf_box, const float *node_box, float4 *trav) {
   uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
   if (i + 1 >= n) return;
-  const uint32_t ch[2] = {left[i], right[i]};
+  uint32_t ch[2] = {left[i], right[i]};
   float b[2][6];
   uint32_t last[2];
   for (int c = 0; c < 2; ++c) {
@@ -478,6 +478,8 @@ __global__ __launch_bounds__(BLOCK) void k_pack_trav(uint32_t n, const uint32_t 
     const float *src = leaf ? leaf_box + 6 * (size_t)(ch[c] & ~LEAF) : node_box + 6 * (size_t)ch[c];
     for (int k = 0; k < 6; ++k) b[c][k] = src[k];
     last[c] = leaf ? ch[c] & ~LEAF : range[2 * (size_t)ch[c] + 1];
+    // a leaf child's ref names its primitive, so a traversal visits it without a lookup
+    if (leaf) ch[c] = LEAF | prim[ch[c] & ~LEAF];
   }
   float4 *o = trav + 4 * (size_t)i;
   o[0] = make_float4(b[0][0], b[0][1], b[0][2], b[0][3]);
@@ -495,13 +497,13 @@ __global__ __launch_bounds__(BLOCK) void k_slot_of(uint32_t n, const uint32_t *o
 // The traversal frontier: starting from the root's children, internal entries are replaced by
 // their two children breadth first while there is room for TRAV_PARTS entries; leaves stay.
 // One thread, at most TRAV_PARTS - 2 record reads.
-__global__ void k_frontier(uint32_t n, const float4 *trav, const float *leaf_box, float4 *front) {
+__global__ void k_frontier(uint32_t n, const float4 *trav, const uint32_t *prim, const float *leaf_box, float4 *front) {
   if (threadIdx.x != 0) return;
   uint32_t ref[TRAV_PARTS];
   float4 lo[TRAV_PARTS], hi[TRAV_PARTS];  // hi.w: the entry's last leaf slot
   uint32_t cnt = 0;
   if (n == 1) {
-    ref[0] = LEAF;
+    ref[0] = LEAF | prim[0];
     lo[0] = make_float4(leaf_box[0], leaf_box[1], leaf_box[2], 0.0f);
     hi[0] = make_float4(leaf_box[3], leaf_box[4], leaf_box[5], __uint_as_float(0u));
     cnt = 1;
@@ -579,10 +581,12 @@ void Lbvh::build(int kind, uint32_t n, const uint32_t *ids, const uint32_t *idx,
     aabb_from_ranges(n, range_.p, leaf_box_.p, node_box_.p, table_, s);
     trav_.reserve(16 * (size_t)n);
     hipLaunchKernelGGL(k_pack_trav, dim3(div_up(n - 1, BLOCK)), dim3(BLOCK), 0, s, n, (const uint32_t *)left_.p, (const uint32_t *)right_.p,
-                       (const uint32_t *)range_.p, (const float *)leaf_box_.p, (const float *)node_box_.p, (float4 *)trav_.p);
+                       (const uint32_t *)range_.p, (const uint32_t *)prim_.p, (const float *)leaf_box_.p, (const float *)node_box_.p,
+                       (float4 *)trav_.p);
   }
   front_.reserve(8 * TRAV_PARTS);
-  hipLaunchKernelGGL(k_frontier, dim3(1), dim3(WAVE), 0, s, n, (const float4 *)trav_.p, (const float *)leaf_box_.p, (float4 *)front_.p);
+  hipLaunchKernelGGL(k_frontier, dim3(1), dim3(WAVE), 0, s, n, (const float4 *)trav_.p, (const uint32_t *)prim_.p, (const float *)leaf_box_.p,
+                     (float4 *)front_.p);
   NGUO_HIP(hipGetLastError());
 }
 
