@@ -1,7 +1,7 @@
 """ctypes binding of libnguo_hip.so (the C ABI in include/nguo_hip.h).
 
 This is the Python-side stand-in for the reference's Rust FFI crate: the same entry points a
-Rust `extern "C"` block would bind (see INTEGRATION.md), used by the tests, the benchmark
+Rust `extern "C"` block would bind, used by the tests, the benchmark
 and the smoke check. It loads the in-tree library and fails loudly when it is missing; there
 is no fallback path.
 """
