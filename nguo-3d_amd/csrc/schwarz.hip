@@ -113,21 +113,25 @@ __device__ inline void sym_tile_pq(uint32_t tile, uint32_t &P, uint32_t &Q) {
   Q = P + rem;
 }
 
-// Floor, FP64 Cholesky (diagonal fallback) and explicit FP32 inverse of the dense m x m
-// block in L (row stride m), written to out: TILED, as the 21 upper tiles above; otherwise
-// row-major with row stride MD (columns m..m4-1 zero). All threads of the workgroup call it;
-// the LDS arrays are the caller's.
+// The build keeps a block's lower triangle packed by rows: entry (i, j), j <= i, at tri(i) + j.
+// The factorisation and the inverse read nothing above the diagonal.
+__host__ __device__ constexpr uint32_t tri(uint32_t i) { return i * (i + 1) / 2; }
+
+// Floor, FP64 Cholesky (diagonal fallback) and explicit FP32 inverse of the symmetric m x m
+// block whose lower triangle is packed in L, written to out: TILED, as the 21 upper tiles
+// above; otherwise row-major with row stride MD (columns m..m4-1 zero). All threads of the
+// workgroup call it; the LDS arrays are the caller's.
 template <bool TILED>
 __device__ void factor_invert(double *L, double *diag0, double *colj, double &floor_v, int &failed, uint32_t m, float *out, uint32_t *fails) {
   const uint32_t t = threadIdx.x, ti = t / 16, tj = t % 16;
   // floor 1e-6 tr/m + 1e-8 (linsolve.rs:232-238)
   if (t == 0) {
     double tr = 0.0;
-    for (uint32_t k = 0; k < m; ++k) tr += L[k * m + k];
+    for (uint32_t k = 0; k < m; ++k) tr += L[tri(k) + k];
     floor_v = 1e-6 * (tr / (double)m) + 1e-8;
   }
   __syncthreads();
-  for (uint32_t k = t; k < m; k += BLOCK) { L[k * m + k] += floor_v; diag0[k] = L[k * m + k]; }
+  for (uint32_t k = t; k < m; k += BLOCK) { L[tri(k) + k] += floor_v; diag0[k] = L[tri(k) + k]; }
   __syncthreads();
   // Cholesky, right-looking: after column j is scaled, its outer product leaves the trailing
   // lower triangle. Entry (i, jj) receives its updates for k = 0, 1, ... in order, so each
@@ -136,21 +140,21 @@ __device__ void factor_invert(double *L, double *diag0, double *colj, double &fl
   // colj, so the update reads it without strided LDS accesses.
   for (uint32_t j = 0; j < m; ++j) {
     if (t == 0) {
-      const double s = L[j * m + j];
+      const double s = L[tri(j) + j];
       if (s <= 1e-14) failed = 1;
-      else L[j * m + j] = sqrt(s);
+      else L[tri(j) + j] = sqrt(s);
     }
     __syncthreads();
     if (failed) break;
-    const double ljj = L[j * m + j];
+    const double ljj = L[tri(j) + j];
     for (uint32_t i = j + 1 + t; i < m; i += BLOCK) {
-      const double v = L[i * m + j] / ljj;
-      L[i * m + j] = v;
+      const double v = L[tri(i) + j] / ljj;
+      L[tri(i) + j] = v;
       colj[i] = v;
     }
     __syncthreads();
     for (uint32_t i = j + 1 + ti; i < m; i += 16)
-      for (uint32_t jj = j + 1 + tj; jj <= i; jj += 16) L[i * m + jj] -= colj[i] * colj[jj];
+      for (uint32_t jj = j + 1 + tj; jj <= i; jj += 16) L[tri(i) + jj] -= colj[i] * colj[jj];
     __syncthreads();
   }
   const uint32_t m4 = (m + 3) & ~3u;
@@ -179,26 +183,26 @@ __device__ void factor_invert(double *L, double *diag0, double *colj, double &fl
   // sum_{k = j+1 .. i} w_ik l_kj with the trailing block already inverted and the original
   // column j copied to colj. A row's sum is split over the 16 lanes of a DPP row (lane q takes
   // k = j + 1 + q, q + 16, ...) and combined by the row exchanges, a fixed order. L's upper
-  // triangle (A's entries) is left as it is and never read as W. One m x m array in LDS
-  // instead of two lets two workgroups share a compute unit.
+  // triangle holds W. One packed triangle in LDS instead of two m x m arrays lets four
+  // workgroups share a compute unit.
   for (int jj = (int)m - 1; jj >= 0; --jj) {
     const uint32_t j = (uint32_t)jj;
-    for (uint32_t i = j + 1 + t; i < m; i += BLOCK) colj[i] = L[i * m + j];
+    for (uint32_t i = j + 1 + t; i < m; i += BLOCK) colj[i] = L[tri(i) + j];
     __syncthreads();
-    const double wjj = 1.0 / L[j * m + j];
+    const double wjj = 1.0 / L[tri(j) + j];
     for (uint32_t i0 = j + 1; i0 < m; i0 += 16) {
       const uint32_t i = i0 + ti;
       double acc = 0.0;
       if (i < m)
-        for (uint32_t k = j + 1 + tj; k <= i; k += 16) acc += L[i * m + k] * colj[k];
+        for (uint32_t k = j + 1 + tj; k <= i; k += 16) acc += L[tri(i) + k] * colj[k];
       acc += dpp_d<DPP_XOR1>(acc);
       acc += dpp_d<DPP_XOR2>(acc);
       acc += dpp_d<DPP_HALF_MIRROR>(acc);
       acc += dpp_d<DPP_MIRROR>(acc);
-      if (i < m && tj == 0) L[i * m + j] = -wjj * acc;
+      if (i < m && tj == 0) L[tri(i) + j] = -wjj * acc;
     }
     __syncthreads();
-    if (t == 0) L[j * m + j] = wjj;
+    if (t == 0) L[tri(j) + j] = wjj;
   }
   __syncthreads();
   // A^-1 = W^T W on the FP64 matrix cores: 16 x 16 output tiles, k in steps of 4
@@ -210,7 +214,7 @@ __device__ void factor_invert(double *L, double *diag0, double *colj, double &fl
   // further), in the tiled form every entry of the 21 tiles (tiles wholly outside run no
   // k step and store their zero accumulator).
   const uint32_t wave = t / WAVE, lane = t % WAVE;
-  auto wget = [&](uint32_t k, uint32_t c) -> double { return (k < m && c <= k) ? L[k * m + c] : 0.0; };
+  auto wget = [&](uint32_t k, uint32_t c) -> double { return (k < m && c <= k) ? L[tri(k) + c] : 0.0; };
   for (uint32_t tile = wave; tile < SCH_NT; tile += BLOCK / WAVE) {
     uint32_t P, Q;
     sym_tile_pq(tile, P, Q);
@@ -242,14 +246,14 @@ __global__ __launch_bounds__(BLOCK) void k_schwarz_build(SellView A, ContactView
                                                         const uint32_t *__restrict__ dom_row, const uint32_t *__restrict__ row_dom,
                                                         const uint32_t *__restrict__ row_pos, float *__restrict__ ainv, uint32_t *inc_rng,
                                                         uint32_t *fails) {
-  __shared__ double L[MD * MD];
+  __shared__ double L[MD * (MD + 1) / 2];
   __shared__ double diag0[MD];
   __shared__ double colj[MD];
   __shared__ double floor_v;
   __shared__ int failed;
   const uint32_t d = blockIdx.x, t = threadIdx.x;
   const uint32_t base = dom_ptr[d], nv = dom_ptr[d + 1] - base, m = 3 * nv;
-  for (uint32_t i = t; i < m * m; i += BLOCK) L[i] = 0.0;
+  for (uint32_t i = t; i < tri(m); i += BLOCK) L[i] = 0.0;
   if (t == 0) failed = 0;
   __syncthreads();
   // elastic blocks of the domain rows: 8 threads per row, one writer per block
@@ -262,7 +266,8 @@ __global__ __launch_bounds__(BLOCK) void k_schwarz_build(SellView A, ContactView
       if (row_dom[c] != d) continue;
       const uint32_t q = row_pos[c];
       for (int r = 0; r < 3; ++r)
-        for (int cc = 0; cc < 3; ++cc) L[(3 * p + r) * m + 3 * q + cc] = (double)A.val[sell_val_index(sl, 3 * r + cc)];
+        for (int cc = 0; cc < 3; ++cc)
+          if (3 * q + cc <= 3 * p + r) L[tri(3 * p + r) + 3 * q + cc] = (double)A.val[sell_val_index(sl, 3 * r + cc)];
     }
   }
   __syncthreads();
@@ -288,8 +293,9 @@ __global__ __launch_bounds__(BLOCK) void k_schwarz_build(SellView A, ContactView
           double wb = C.w[4 * (size_t)s + b];
           for (int r = 0; r < 3; ++r)
             for (int cc = 0; cc < 3; ++cc)
-              L[(3 * p + r) * m + 3 * q + cc] +=
-                  hp * (double)ga[r] * (double)gb[cc] + lam * wa * wb * ((r == cc ? 1.0 : 0.0) - (double)n[r] * (double)n[cc]);
+              if (3 * q + cc <= 3 * p + r)
+                L[tri(3 * p + r) + 3 * q + cc] +=
+                    hp * (double)ga[r] * (double)gb[cc] + lam * wa * wb * ((r == cc ? 1.0 : 0.0) - (double)n[r] * (double)n[cc]);
         }
       }
     }
@@ -308,19 +314,19 @@ __global__ __launch_bounds__(BLOCK) void k_schwarz_build_coarse(SellView A, Cont
                                                                const uint32_t *__restrict__ cdom_ptr, const uint32_t *__restrict__ cdom_node,
                                                                const uint32_t *__restrict__ node_cdom, const uint32_t *__restrict__ node_cpos,
                                                                float *__restrict__ ainv1, uint32_t *fails) {
-  __shared__ double L[MD * MD];
+  __shared__ double L[MD * (MD + 1) / 2];
   __shared__ double diag0[MD];
   __shared__ double colj[MD];
   __shared__ double floor_v;
   __shared__ int failed;
   const uint32_t D = blockIdx.x, t = threadIdx.x;
   const uint32_t cb = cdom_ptr[D], n1 = cdom_ptr[D + 1] - cb, m = 3 * n1;
-  for (uint32_t i = t; i < m * m; i += BLOCK) L[i] = 0.0;
+  for (uint32_t i = t; i < tri(m); i += BLOCK) L[i] = 0.0;
   if (t == 0) failed = 0;
   __syncthreads();
   if (t < m) {
     const uint32_t a = t / 3, r = t % 3, G = cdom_node[cb + a];
-    double *lrow = L + (size_t)t * m;
+    double *lrow = L + tri(t);  // row t of the packed lower triangle: columns 0 .. t
     for (uint32_t k = dom_ptr[G]; k < dom_ptr[G + 1]; ++k) {
       const uint32_t row = dom_row[k];
       const uint32_t sbase = A.slice_ptr[row >> 6], lane = row & 63u, len = A.row_len[row];
@@ -329,7 +335,8 @@ __global__ __launch_bounds__(BLOCK) void k_schwarz_build_coarse(SellView A, Cont
         const uint32_t H = row_dom[A.col[sl]];
         if (node_cdom[H] != D) continue;
         const uint32_t b = node_cpos[H];
-        for (int c = 0; c < 3; ++c) lrow[3 * b + c] += (double)A.val[sell_val_index(sl, 3 * r + c)];
+        for (int c = 0; c < 3; ++c)
+          if (3 * b + c <= t) lrow[3 * b + c] += (double)A.val[sell_val_index(sl, 3 * r + c)];
       }
       if (C.ns) {
         for (uint32_t kk = C.inc_ptr[row]; kk < C.inc_ptr[row + 1]; ++kk) {
@@ -345,8 +352,9 @@ __global__ __launch_bounds__(BLOCK) void k_schwarz_build_coarse(SellView A, Cont
             const uint32_t b = node_cpos[H];
             const double wb = C.w[4 * (size_t)s + bi];
             for (int c = 0; c < 3; ++c)
-              lrow[3 * b + c] += hp * gar * (double)C.g[12 * (size_t)s + 3 * bi + c] +
-                                 lam * wa * wb * ((r == (uint32_t)c ? 1.0 : 0.0) - nr * (double)C.n[3 * (size_t)s + c]);
+              if (3 * b + c <= t)
+                lrow[3 * b + c] += hp * gar * (double)C.g[12 * (size_t)s + 3 * bi + c] +
+                                   lam * wa * wb * ((r == (uint32_t)c ? 1.0 : 0.0) - nr * (double)C.n[3 * (size_t)s + c]);
           }
         }
       }
