@@ -15,6 +15,8 @@
 namespace nguo {
 
 constexpr uint32_t LEAF = 0x80000000u;
+constexpr uint32_t TRAV_WIDTH = 4;            // entries of a traversal record
+constexpr uint32_t TRAV_NONE = 0xffffffffu;   // unused record entry (primitive ids are below 2^31)
 
 // Stable ascending sort of (key, value) pairs on the device.
 class RadixSort {
@@ -37,10 +39,10 @@ struct BvhView {
   const float *node_box;  // [6 (n-1)] lo.xyz hi.xyz
   const float *leaf_box;  // [6 n] in sorted order
   const uint32_t *prim;   // [n] primitive id of sorted leaf slot
-  // [4 (n-1)] traversal nodes: both child boxes, child refs and the last sorted leaf slot
-  // under each child of internal node i in one 64-byte record
-  // (lo_l.xyz hi_l.x | hi_l.yz lo_r.xy | lo_r.z hi_r.xyz | left right last_l last_r); a leaf
-  // child's ref is LEAF | its primitive id (not its slot, as in left / right)
+  // [2 TRAV_WIDTH (n-1)] traversal records: internal node i's up to TRAV_WIDTH grandchildren
+  // (a leaf child stands for itself) in one 128-byte record, entry e as (lo.xyz, hi.x)
+  // (hi.yz, ref, last sorted leaf slot under it); ref is LEAF | the primitive id for a leaf
+  // (not its slot, as in left / right), else the binary node id; TRAV_NONE for an unused entry
   const float4 *trav;
   // [2 TRAV_PARTS] the tree split into disjoint subtrees that together hold every leaf once:
   // entry k = (lo.xyz, ref) (hi.xyz, last leaf slot), ref as in trav, ~0u for an unused entry
@@ -89,7 +91,7 @@ void aabb_from_ranges(uint32_t n, const uint32_t *range, const float *leaf_box, 
 // workgroup. Only leaves in sorted slots >= min_slot are visited, and subtrees whose last
 // slot lies below it are not entered (a query of a tree against itself visits each pair
 // once by passing its own slot + 1).
-constexpr int TRAV_STACK = 96;
+constexpr int TRAV_STACK = 128;
 constexpr int TRAV_LDS = 32;
 // Depth-first traversal of the subtree under internal node `root`.
 template <class Overlap, class Visit>
@@ -99,32 +101,45 @@ __device__ inline void bvh_traverse_from(const BvhView &t, uint32_t root, uint32
   uint32_t *mine = lds + threadIdx.x;
   mine[0] = root;
   int sp = 1;
-  // Leaves are visited where they are met. (Holding them for a wavefront-wide visit phase, the
-  // "while-while" form, made the garment drape's step 20% slower, its CCD and contact queries
-  // about twice as slow: lanes that hold a leaf wait for the deepest descent of the wavefront.)
+  // Each pop loads one record and tests up to four boxes: the grandchildren of the popped
+  // binary node. Leaves are visited where they are met. (Holding them for a wavefront-wide
+  // visit phase, the "while-while" form, made the garment drape's step 20% slower, its CCD and
+  // contact queries about twice as slow: lanes that hold a leaf wait for the deepest descent
+  // of the wavefront.)
   while (sp > 0) {
     --sp;
     const uint32_t node = sp < TRAV_LDS ? mine[sp * BLOCK] : spill[sp - TRAV_LDS];
-    const float4 *rec = t.trav + 4 * (size_t)node;
-    const float4 q0 = rec[0], q1 = rec[1], q2 = rec[2], q3 = rec[3];
-    const uint32_t ch[2] = {__float_as_uint(q3.x), __float_as_uint(q3.y)};
-    const uint32_t last[2] = {__float_as_uint(q3.z), __float_as_uint(q3.w)};
-    const f3 lo[2] = {f3{q0.x, q0.y, q0.z}, f3{q1.z, q1.w, q2.x}};
-    const f3 hi[2] = {f3{q0.w, q1.x, q1.y}, f3{q2.y, q2.z, q2.w}};
-    for (int c = 0; c < 2; ++c) {
-      const uint32_t r = ch[c];
-      if (last[c] < min_slot || !overlap(lo[c], hi[c])) continue;
-      if (r & LEAF) {
-        visit(r & ~LEAF);
+    const float4 *rec = t.trav + 2 * TRAV_WIDTH * (size_t)node;
+    float4 q[2 * TRAV_WIDTH];
+#pragma unroll
+    for (uint32_t k = 0; k < 2 * TRAV_WIDTH; ++k) q[k] = rec[k];
+    // box tests and pushes unrolled (the record stays in registers); the leaves met are
+    // visited after, from a mask, through one copy of the visit code
+    uint32_t ref[TRAV_WIDTH], leaves = 0;
+#pragma unroll
+    for (uint32_t c = 0; c < TRAV_WIDTH; ++c) {
+      ref[c] = __float_as_uint(q[2 * c + 1].z);
+      const uint32_t last = __float_as_uint(q[2 * c + 1].w);
+      if (ref[c] == TRAV_NONE || last < min_slot) continue;
+      if (!overlap(f3{q[2 * c].x, q[2 * c].y, q[2 * c].z}, f3{q[2 * c].w, q[2 * c + 1].x, q[2 * c + 1].y})) continue;
+      if (ref[c] & LEAF) {
+        leaves |= 1u << c;
       } else if (sp < TRAV_LDS) {
-        mine[sp * BLOCK] = r;
+        mine[sp * BLOCK] = ref[c];
         ++sp;
       } else if (sp < TRAV_STACK) {
-        spill[sp - TRAV_LDS] = r;
+        spill[sp - TRAV_LDS] = ref[c];
         ++sp;
       } else {
         atomicOr(err, 1u);
       }
+    }
+    static_assert(TRAV_WIDTH == 4, "the leaf selection below names four entries");
+    while (leaves) {
+      const uint32_t c = __builtin_ctz(leaves);
+      leaves &= leaves - 1;
+      const uint32_t r = c == 0 ? ref[0] : c == 1 ? ref[1] : c == 2 ? ref[2] : ref[3];
+      visit(r & ~LEAF);
     }
   }
 }

@@ -465,27 +465,43 @@ void aabb_from_ranges(uint32_t n, const uint32_t *range, const float *leaf_box, 
   NGUO_HIP(hipGetLastError());
 }
 
-// Traversal records: child boxes and refs of each internal node in 64 bytes.
+// Traversal records: internal node i's up to four grandchildren (a child that is a leaf
+// stands for itself) in 128 bytes, entry e as (lo.xyz, hi.x) (hi.yz, ref, last leaf slot).
+// ref is LEAF | the primitive id for a leaf, else the binary node id, whose own record holds
+// its grandchildren; an unused entry has ref TRAV_NONE. A traversal then tests four boxes per
+// dependent record load, and its chain of loads is half as long as over the binary nodes.
+__device__ inline void trav_entry(uint32_t c, const uint32_t *range, const uint32_t *prim, const float *leaf_box, const float *node_box,
+                                  float4 &e0, float4 &e1) {
+  const bool leaf = c & LEAF;
+  const float *b = leaf ? leaf_box + 6 * (size_t)(c & ~LEAF) : node_box + 6 * (size_t)c;
+  const uint32_t last = leaf ? c & ~LEAF : range[2 * (size_t)c + 1];
+  const uint32_t ref = leaf ? (LEAF | prim[c & ~LEAF]) : c;
+  e0 = make_float4(b[0], b[1], b[2], b[3]);
+  e1 = make_float4(b[4], b[5], __uint_as_float(ref), __uint_as_float(last));
+}
 __global__ __launch_bounds__(BLOCK) void k_pack_trav(uint32_t n, const uint32_t *left, const uint32_t *right, const uint32_t *range,
                                                     const uint32_t *prim, const float *leaf_box, const float *node_box, float4 *trav) {
   uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
   if (i + 1 >= n) return;
-  uint32_t ch[2] = {left[i], right[i]};
-  float b[2][6];
-  uint32_t last[2];
+  float4 e[2 * TRAV_WIDTH];
+  uint32_t k = 0;
+  const uint32_t ch[2] = {left[i], right[i]};
   for (int c = 0; c < 2; ++c) {
-    const bool leaf = ch[c] & LEAF;
-    const float *src = leaf ? leaf_box + 6 * (size_t)(ch[c] & ~LEAF) : node_box + 6 * (size_t)ch[c];
-    for (int k = 0; k < 6; ++k) b[c][k] = src[k];
-    last[c] = leaf ? ch[c] & ~LEAF : range[2 * (size_t)ch[c] + 1];
-    // a leaf child's ref names its primitive, so a traversal visits it without a lookup
-    if (leaf) ch[c] = LEAF | prim[ch[c] & ~LEAF];
+    if (ch[c] & LEAF) {
+      trav_entry(ch[c], range, prim, leaf_box, node_box, e[2 * k], e[2 * k + 1]);
+      ++k;
+    } else {
+      trav_entry(left[ch[c]], range, prim, leaf_box, node_box, e[2 * k], e[2 * k + 1]);
+      trav_entry(right[ch[c]], range, prim, leaf_box, node_box, e[2 * k + 2], e[2 * k + 3]);
+      k += 2;
+    }
   }
-  float4 *o = trav + 4 * (size_t)i;
-  o[0] = make_float4(b[0][0], b[0][1], b[0][2], b[0][3]);
-  o[1] = make_float4(b[0][4], b[0][5], b[1][0], b[1][1]);
-  o[2] = make_float4(b[1][2], b[1][3], b[1][4], b[1][5]);
-  o[3] = make_float4(__uint_as_float(ch[0]), __uint_as_float(ch[1]), __uint_as_float(last[0]), __uint_as_float(last[1]));
+  for (; k < TRAV_WIDTH; ++k) {
+    e[2 * k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    e[2 * k + 1] = make_float4(0.f, 0.f, __uint_as_float(TRAV_NONE), __uint_as_float(0u));
+  }
+  float4 *o = trav + 2 * TRAV_WIDTH * (size_t)i;
+  for (uint32_t q = 0; q < 2 * TRAV_WIDTH; ++q) o[q] = e[q];
 }
 
 // slot[k] = the sorted leaf slot holding element k of the build's primitive list.
@@ -496,8 +512,9 @@ __global__ __launch_bounds__(BLOCK) void k_slot_of(uint32_t n, const uint32_t *o
 
 // The traversal frontier: starting from the root's children, internal entries are replaced by
 // their two children breadth first while there is room for TRAV_PARTS entries; leaves stay.
-// One thread, at most TRAV_PARTS - 2 record reads.
-__global__ void k_frontier(uint32_t n, const float4 *trav, const uint32_t *prim, const float *leaf_box, float4 *front) {
+// One thread, at most TRAV_PARTS - 2 node expansions.
+__global__ void k_frontier(uint32_t n, const uint32_t *left, const uint32_t *right, const uint32_t *range, const uint32_t *prim, const float *leaf_box,
+                           const float *node_box, float4 *front) {
   if (threadIdx.x != 0) return;
   uint32_t ref[TRAV_PARTS];
   float4 lo[TRAV_PARTS], hi[TRAV_PARTS];  // hi.w: the entry's last leaf slot
@@ -508,13 +525,16 @@ __global__ void k_frontier(uint32_t n, const float4 *trav, const uint32_t *prim,
     hi[0] = make_float4(leaf_box[3], leaf_box[4], leaf_box[5], __uint_as_float(0u));
     cnt = 1;
   } else if (n >= 2) {
+    auto entry = [&](uint32_t c, uint32_t o) {
+      float4 e0, e1;
+      trav_entry(c, range, prim, leaf_box, node_box, e0, e1);
+      ref[o] = __float_as_uint(e1.z);
+      lo[o] = make_float4(e0.x, e0.y, e0.z, 0.0f);
+      hi[o] = make_float4(e0.w, e1.x, e1.y, e1.w);
+    };
     auto children = [&](uint32_t node, uint32_t o0, uint32_t o1) {
-      const float4 *rec = trav + 4 * (size_t)node;
-      const float4 q0 = rec[0], q1 = rec[1], q2 = rec[2], q3 = rec[3];
-      ref[o0] = __float_as_uint(q3.x);
-      ref[o1] = __float_as_uint(q3.y);
-      lo[o0] = make_float4(q0.x, q0.y, q0.z, 0.0f); hi[o0] = make_float4(q0.w, q1.x, q1.y, q3.z);
-      lo[o1] = make_float4(q1.z, q1.w, q2.x, 0.0f); hi[o1] = make_float4(q2.y, q2.z, q2.w, q3.w);
+      entry(left[node], o0);
+      entry(right[node], o1);
     };
     children(0u, 0, 1);
     cnt = 2;
@@ -579,14 +599,14 @@ void Lbvh::build(int kind, uint32_t n, const uint32_t *ids, const uint32_t *idx,
     hipLaunchKernelGGL(k_karras, dim3(div_up(n - 1, BLOCK)), dim3(BLOCK), 0, s, n, (const uint32_t *)keys_.p, left_.p, right_.p,
                        (uint32_t *)nullptr, range_.p);
     aabb_from_ranges(n, range_.p, leaf_box_.p, node_box_.p, table_, s);
-    trav_.reserve(16 * (size_t)n);
+    trav_.reserve(8 * TRAV_WIDTH * (size_t)n);
     hipLaunchKernelGGL(k_pack_trav, dim3(div_up(n - 1, BLOCK)), dim3(BLOCK), 0, s, n, (const uint32_t *)left_.p, (const uint32_t *)right_.p,
                        (const uint32_t *)range_.p, (const uint32_t *)prim_.p, (const float *)leaf_box_.p, (const float *)node_box_.p,
                        (float4 *)trav_.p);
   }
   front_.reserve(8 * TRAV_PARTS);
-  hipLaunchKernelGGL(k_frontier, dim3(1), dim3(WAVE), 0, s, n, (const float4 *)trav_.p, (const uint32_t *)prim_.p, (const float *)leaf_box_.p,
-                     (float4 *)front_.p);
+  hipLaunchKernelGGL(k_frontier, dim3(1), dim3(WAVE), 0, s, n, (const uint32_t *)left_.p, (const uint32_t *)right_.p, (const uint32_t *)range_.p,
+                     (const uint32_t *)prim_.p, (const float *)leaf_box_.p, (const float *)node_box_.p, (float4 *)front_.p);
   NGUO_HIP(hipGetLastError());
 }
 
