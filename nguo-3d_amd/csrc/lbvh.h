@@ -15,7 +15,9 @@
 namespace nguo {
 
 constexpr uint32_t LEAF = 0x80000000u;
-constexpr uint32_t TRAV_WIDTH = 4;            // entries of a traversal record
+// entries of a traversal record: a node's grandchildren. (Eight, its great-grandchildren in
+// 256 bytes, made the garment drape's step 1% slower than four.)
+constexpr uint32_t TRAV_WIDTH = 4;
 constexpr uint32_t TRAV_NONE = 0xffffffffu;   // unused record entry (primitive ids are below 2^31)
 
 // Stable ascending sort of (key, value) pairs on the device.
