@@ -447,9 +447,10 @@ __global__ __launch_bounds__(BLOCK) void k_vt_stencils(Topo T, BvhView dyn, BvhV
 }
 
 // implicit.rs:719-751 edge-edge stencils. Query edges are padded, tree edges are not (the
-// overlap is symmetric); each movable pair is met once, by the query whose sorted leaf slot
-// is the lower, a static query meets every movable edge. Keys are (lower, higher edge id).
-__global__ __launch_bounds__(BLOCK) void k_ee_stencils(Topo T, BvhView dyn, const uint32_t *qids, uint32_t nq, int query_dyn,
+// overlap is symmetric). self_query: the queries are the tree's own edges and each pair is met
+// once, by the query whose sorted leaf slot is the lower; otherwise (movable edges against
+// the static edges' tree) every leaf is a partner. Keys are (lower, higher edge id).
+__global__ __launch_bounds__(BLOCK) void k_ee_stencils(Topo T, BvhView tree, const uint32_t *qids, uint32_t nq, int self_query,
                                                       const float *__restrict__ x, float pad, float ghat, StencilOut O, uint32_t *err) {
   __shared__ uint32_t tstack[TRAV_LDS * BLOCK];
   uint32_t q = blockIdx.x * BLOCK + threadIdx.x;
@@ -462,8 +463,8 @@ __global__ __launch_bounds__(BLOCK) void k_ee_stencils(Topo T, BvhView dyn, cons
   // a leaf reuses the query edge's positions from registers (the pair's vertices in
   // edge-index order: the lower edge's two, then the higher's); a movable query meets each
   // other movable edge once: only the leaves after its own sorted slot
-  const uint32_t min_slot = query_dyn ? dyn.slot[q] + 1 : 0u;
-  bvh_traverse_part(dyn, blockIdx.y, err, tstack, ov, [&](uint32_t ei) {
+  const uint32_t min_slot = self_query ? tree.slot[q] + 1 : 0u;
+  bvh_traverse_part(tree, blockIdx.y, err, tstack, ov, [&](uint32_t ei) {
     if (ei == ej) return;
     uint32_t a0 = T.edges[2 * (size_t)ei], a1 = T.edges[2 * (size_t)ei + 1];
     if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) return;
@@ -852,7 +853,7 @@ __global__ __launch_bounds__(BLOCK) void k_ccd_vt(Topo T, BvhView dyn, BvhView s
 }
 
 // Edge-edge ACCD (implicit.rs:1902-1936).
-__global__ __launch_bounds__(BLOCK) void k_ccd_ee(Topo T, BvhView dyn, const uint32_t *qids, uint32_t nq, int query_dyn, const float *__restrict__ x,
+__global__ __launch_bounds__(BLOCK) void k_ccd_ee(Topo T, BvhView tree, const uint32_t *qids, uint32_t nq, int self_query, const float *__restrict__ x,
                                                  const float *__restrict__ dx, float pad0, float max_disp, uint32_t iters, CcdDefer D,
                                                  unsigned long long *tmin, uint32_t *err) {
   __shared__ uint32_t tstack[TRAV_LDS * BLOCK];
@@ -867,11 +868,11 @@ __global__ __launch_bounds__(BLOCK) void k_ccd_ee(Topo T, BvhView dyn, const uin
     f3 blo = vmin(pb, qb) - f3{pad, pad, pad}, bhi = vmax(pb, qb) + f3{pad, pad, pad};
     auto ov = [&](f3 lo, f3 hi) { return !(lo.x > bhi.x || lo.y > bhi.y || lo.z > bhi.z || hi.x < blo.x || hi.y < blo.y || hi.z < blo.z); };
     // a leaf loads its edge's positions and motions in one memory round trip after the vertex
-    // ids, and reuses the query edge's from registers. A movable query meets each other
-    // movable edge once: only the leaves after its own sorted slot (whole subtrees before it
-    // are skipped).
-    const uint32_t min_slot = query_dyn ? dyn.slot[q] + 1 : 0u;
-    bvh_traverse_part(dyn, blockIdx.y, err, tstack, ov, [&](uint32_t ei) {
+    // ids, and reuses the query edge's from registers. A query of the tree's own edges meets
+    // each other edge once: only the leaves after its own sorted slot (whole subtrees before
+    // it are skipped).
+    const uint32_t min_slot = self_query ? tree.slot[q] + 1 : 0u;
+    bvh_traverse_part(tree, blockIdx.y, err, tstack, ov, [&](uint32_t ei) {
       if (ei == ej) return;
       uint32_t a0 = T.edges[2 * (size_t)ei], a1 = T.edges[2 * (size_t)ei + 1];
       if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) return;
@@ -1064,7 +1065,7 @@ struct nguo_step {
   // trees and their primitive id lists
   DevBuf<uint32_t> tri_dyn_ids, tri_stat_ids, tri_coll_ids, edge_dyn_ids, edge_stat_ids;
   uint32_t n_tri_dyn = 0, n_tri_stat = 0, n_tri_coll = 0, n_edge_dyn = 0, n_edge_stat = 0;
-  Lbvh tree_tri_dyn, tree_tri_stat, tree_tri_coll, tree_edge_dyn;
+  Lbvh tree_tri_dyn, tree_tri_stat, tree_tri_coll, tree_edge_dyn, tree_edge_stat;
   // elements
   ElemSet act, stat;
   // contact buffers
@@ -1431,9 +1432,10 @@ void detect(nguo_step *st) {
     if (st->n_edge_dyn)
       hipLaunchKernelGGL(k_ee_stencils, dim3(div_up(st->n_edge_dyn, BLOCK), TRAV_PARTS), dim3(BLOCK), 0, s, T, st->tree_edge_dyn.view(),
                          (const uint32_t *)st->edge_dyn_ids.p, st->n_edge_dyn, 1, (const float *)st->x.p, dhat, dhat, oee, st->err.p);
+    // movable edges against the static edges' tree (built once: its positions never change)
     if (st->n_edge_stat && st->n_edge_dyn)
-      hipLaunchKernelGGL(k_ee_stencils, dim3(div_up(st->n_edge_stat, BLOCK), TRAV_PARTS), dim3(BLOCK), 0, s, T, st->tree_edge_dyn.view(),
-                         (const uint32_t *)st->edge_stat_ids.p, st->n_edge_stat, 0, (const float *)st->x.p, dhat, dhat, oee, st->err.p);
+      hipLaunchKernelGGL(k_ee_stencils, dim3(div_up(st->n_edge_dyn, BLOCK), TRAV_PARTS), dim3(BLOCK), 0, s, T, st->tree_edge_stat.view(),
+                         (const uint32_t *)st->edge_dyn_ids.p, st->n_edge_dyn, 0, (const float *)st->x.p, dhat, dhat, oee, st->err.p);
     NGUO_HIP(hipGetLastError());
     NGUO_HIP(hipMemcpyAsync(st->hu, st->vt_count.p, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     NGUO_HIP(hipMemcpyAsync(st->hu + 2, st->err.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -1521,9 +1523,12 @@ float ccd_max_step(nguo_step *st, float max_disp) {
           hipLaunchKernelGGL(k_ccd_ee, dim3(div_up(st->n_edge_dyn, BLOCK), TRAV_PARTS), dim3(BLOCK), 0, s, T, st->tree_edge_dyn.view(),
                              (const uint32_t *)st->edge_dyn_ids.p, st->n_edge_dyn, 1, (const float *)st->x.p, (const float *)st->dx.p, search, max_disp,
                              iters, D, st->tmin.p, st->err.p);
+        // movable edges against the static edges' tree: a movable query's padding covers
+        // every pair the static side's padding would (its own motion is part of the cap), and
+        // a pair beyond the static side's cap returns the horizon, so the minimum is the same
         if (st->n_edge_stat && st->n_edge_dyn)
-          hipLaunchKernelGGL(k_ccd_ee, dim3(div_up(st->n_edge_stat, BLOCK), TRAV_PARTS), dim3(BLOCK), 0, s, T, st->tree_edge_dyn.view(),
-                             (const uint32_t *)st->edge_stat_ids.p, st->n_edge_stat, 0, (const float *)st->x.p, (const float *)st->dx.p, search,
+          hipLaunchKernelGGL(k_ccd_ee, dim3(div_up(st->n_edge_dyn, BLOCK), TRAV_PARTS), dim3(BLOCK), 0, s, T, st->tree_edge_stat.view(),
+                             (const uint32_t *)st->edge_dyn_ids.p, st->n_edge_dyn, 0, (const float *)st->x.p, (const float *)st->dx.p, search,
                              max_disp, iters, D, st->tmin.p, st->err.p);
         if (iters > ACCD_INLINE)
           hipLaunchKernelGGL(k_ccd_deferred, dim3(div_up(CCD_DEFER_CAP, BLOCK)), dim3(BLOCK), 0, s, T, D, (const float *)st->x.p, (const float *)st->dx.p,
@@ -1535,6 +1540,15 @@ float ccd_max_step(nguo_step *st, float max_disp) {
     NGUO_REQUIRE(read_u32(st, st->err.p) == 0, NGUO_E_CAPACITY, "LBVH traversal stack overflow");
   }
   return (float)std::min(std::max(t, 0.0), 1.0);
+}
+
+// Trees over the pinned primitives (triangles and edges with every vertex pinned) and the
+// collider's triangles: their positions do not change while stepping, so they are built from
+// the current positions at creation and when a state is set, not per Newton iteration.
+void build_static_trees(nguo_step *st, hipStream_t s) {
+  st->tree_tri_stat.build(0, st->n_tri_stat, st->tri_stat_ids.p, st->tris.p, st->x.p, s);
+  st->tree_edge_stat.build(1, st->n_edge_stat, st->edge_stat_ids.p, st->edges.p, st->x.p, s);
+  if (st->has_collider) st->tree_tri_coll.build(0, st->n_tri_coll, st->tri_coll_ids.p, st->tris.p, st->x.p, s);
 }
 
 void assemble_and_solve(nguo_step *st, nguo_stats &stats) {
@@ -1968,8 +1982,7 @@ int nguo_step_create(nguo_ctx *ctx, const nguo_scene_desc *sc, nguo_step **out) 
     upl(st->tri_coll_ids, coll_ids, st->n_tri_coll);
     upl(st->edge_dyn_ids, edyn, st->n_edge_dyn);
     upl(st->edge_stat_ids, estat, st->n_edge_stat);
-    st->tree_tri_stat.build(0, st->n_tri_stat, st->tri_stat_ids.p, st->tris.p, st->x.p, s);
-    if (st->has_collider) st->tree_tri_coll.build(0, st->n_tri_coll, st->tri_coll_ids.p, st->tris.p, st->x.p, s);
+    build_static_trees(st.get(), s);
     ensure_contact_cap(st.get(), std::max<uint32_t>(4 * V, 1u << 16));
     st->pcg.reset(new Pcg(s));
     NGUO_REQUIRE(st->prm.precond == NGUO_PRECOND_BLOCK || st->prm.precond == NGUO_PRECOND_SCHWARZ, NGUO_E_INVALID, "unknown preconditioner");
@@ -2026,6 +2039,7 @@ int nguo_step_set_state(nguo_step *st, const float *positions, const float *velo
     st->x0.upload(positions, 3 * (size_t)st->V, st->s);
     st->x.upload(positions, 3 * (size_t)st->V, st->s);
     if (velocities) st->v.upload(velocities, 3 * (size_t)st->V, st->s);
+    build_static_trees(st, st->s);  // the state names the pinned vertices' positions too
     st->adj_valid = false;
     NGUO_HIP(hipStreamSynchronize(st->s));
     return NGUO_OK;
