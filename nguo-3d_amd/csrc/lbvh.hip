@@ -36,24 +36,6 @@ __global__ __launch_bounds__(BLOCK) void k_radix_hist(uint32_t n, const K *__res
   hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
 }
 
-// Exclusive scan of `total` counters by one workgroup (contiguous segment per thread).
-__global__ __launch_bounds__(BLOCK) void k_scan_excl(uint32_t total, uint32_t *a) {
-  __shared__ uint32_t part[BLOCK];
-  uint32_t seg = (total + BLOCK - 1) / BLOCK;
-  uint32_t lo = threadIdx.x * seg, hi = min(total, lo + seg);
-  uint32_t s = 0;
-  for (uint32_t i = lo; i < hi; ++i) s += a[i];
-  part[threadIdx.x] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t acc = 0;
-    for (int t = 0; t < BLOCK; ++t) { uint32_t v = part[t]; part[t] = acc; acc += v; }
-  }
-  __syncthreads();
-  uint32_t acc = part[threadIdx.x];
-  for (uint32_t i = lo; i < hi; ++i) { uint32_t v = a[i]; a[i] = acc; acc += v; }
-}
-
 // Exclusive scan of one value per thread over the workgroup (fixed order); returns the
 // prefix, *total receives the workgroup sum.
 __device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t *wsum /* [BLOCK / WAVE] */, uint32_t *total) {
@@ -74,6 +56,30 @@ __device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t *wsum /* [BLOCK 
   __syncthreads();
   *total = all;
   return before + x - v;
+}
+
+// In-place exclusive scan of up to SCAN_EXCL_MAX counters by one workgroup: staged in LDS
+// with coalesced loads, each thread sums a contiguous segment, the segment sums are scanned
+// across the workgroup by wavefront shuffles, and each thread writes its segment's prefixes.
+constexpr uint32_t SCAN_EXCL_MAX = 16384;
+__global__ __launch_bounds__(BLOCK) void k_scan_excl(uint32_t total, uint32_t *a) {
+  __shared__ uint32_t buf[SCAN_EXCL_MAX];
+  __shared__ uint32_t wsum[BLOCK / WAVE];
+  for (uint32_t i = threadIdx.x; i < total; i += BLOCK) buf[i] = a[i];
+  __syncthreads();
+  const uint32_t seg = (total + BLOCK - 1) / BLOCK;
+  const uint32_t lo = min(total, threadIdx.x * seg), hi = min(total, lo + seg);
+  uint32_t s = 0;
+  for (uint32_t i = lo; i < hi; ++i) s += buf[i];
+  uint32_t all;
+  uint32_t acc = block_excl_scan(s, wsum, &all);
+  for (uint32_t i = lo; i < hi; ++i) {
+    const uint32_t v = buf[i];
+    buf[i] = acc;
+    acc += v;
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < total; i += BLOCK) a[i] = buf[i];
 }
 
 // Large scans: each workgroup scans SCAN_TILE counters in place (exclusive, local) and
@@ -183,10 +189,11 @@ static void radix_passes(uint32_t n, K *keys, uint32_t *vals, K *ktmp, uint32_t 
   uint32_t *va = vals, *vb = vtmp;
   for (int sh : shifts) {
     hipLaunchKernelGGL((k_radix_hist<K, ITEMS>), dim3(nblocks), dim3(BLOCK), 0, s, n, (const K *)ka, sh, hist, nblocks);
-    if (total <= 16384) {
+    if (total <= SCAN_EXCL_MAX) {
       hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(BLOCK), 0, s, total, hist);
     } else {
       const uint32_t nt = div_up(total, SCAN_TILE);
+      NGUO_REQUIRE(nt <= SCAN_EXCL_MAX, NGUO_E_CAPACITY, "radix sort: too many tiles for the tile-sum scan");
       hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(BLOCK), 0, s, total, hist, sums);
       hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(BLOCK), 0, s, nt, sums);
       hipLaunchKernelGGL(k_scan_add, dim3(nt), dim3(BLOCK), 0, s, total, hist, (const uint32_t *)sums);
