@@ -153,7 +153,8 @@ class Pcg {
   DevBuf<float> r_, z_, p_, p2_, q_, d_, minv_;
   DevBuf<double> x64_, r64_, part_, scal_, sums64_;
   DevBuf<uint32_t> flag_;
-  uint32_t *h_flag_ = nullptr;  // two slots of 4: state, iterations, cap
+  uint32_t *h_flag_ = nullptr;  // two slots of 4: state, iterations, cap (coherent pinned memory)
+  uint32_t *d_flag_ = nullptr;  // the device's address of h_flag_
   hipEvent_t ev_[2] = {nullptr, nullptr};
   double *h_scal_ = nullptr;
 };

@@ -186,7 +186,7 @@ template <int MODE, uint32_t G> struct SlotGroup {
 // would make it). False when the iteration stops.
 template <int NT>
 __device__ inline bool open_iteration(const double *cur, uint32_t nz, double *scal, uint32_t *flag, uint32_t j, bool writer, double *lds,
-                                      float &beta) {
+                                      float &beta, uint32_t *hflag = nullptr) {
   const double *src[2] = {cur, cur + nz};
   const uint32_t n[2] = {nz, nz};
   double v[2];
@@ -200,6 +200,11 @@ __device__ inline bool open_iteration(const double *cur, uint32_t nz, double *sc
     scal[4] = v[1];
     if (conv) flag[0] = 1;
     else if (capped) flag[0] = 3;
+    if (hflag) {
+      hflag[0] = conv ? 1u : (capped ? 3u : 0u);
+      hflag[1] = j;
+      hflag[2] = flag[2];
+    }
   }
   beta = (float)b;
   return !(conv || capped);
@@ -289,7 +294,8 @@ __device__ __forceinline__ void pcg_stencil_body(uint32_t sb, const ContactView 
 template <int MODE, uint32_t G>
 __device__ __forceinline__ void pcg_spmv_body(uint32_t slice, const SellView &A, const ContactView &C, const float4 *__restrict__ z,
                                               const float4 *__restrict__ pold, float4 *__restrict__ pnew, float4 *__restrict__ q, double *part,
-                                              const double *cur, uint32_t nz, double *scal, uint32_t *flag, uint32_t j) {
+                                              const double *cur, uint32_t nz, double *scal, uint32_t *flag, uint32_t j,
+                                              uint32_t *hflag = nullptr) {
   __shared__ double lds[2];
   const uint32_t state = MODE != SPMV_PLAIN ? flag[0] : 0u;
   const uint32_t lane = threadIdx.x, row = slice * 64u + lane;
@@ -319,9 +325,15 @@ __device__ __forceinline__ void pcg_spmv_body(uint32_t slice, const SellView &A,
 #pragma unroll
     for (uint32_t u = 0; u < 8; ++u) cv[u] = C.contrib[ie > ib ? min(ib + u, ie - 1u) : 0u];
   }
-  if (state) return;
+  // the first slice's workgroup publishes the flag to the host slot (hflag) on every path
+  const bool publish = hflag && slice == 0 && lane == 0;
+  if (state) {
+    if (publish) { hflag[0] = state; hflag[1] = flag[1]; hflag[2] = flag[2]; }
+    return;
+  }
+  if (MODE == SPMV_FIRST && publish) { hflag[0] = 0u; hflag[1] = flag[1]; hflag[2] = flag[2]; }
   float beta = 0.0f;
-  if (MODE == SPMV_NEXT && !open_iteration<WAVE>(cur, nz, scal, flag, j, slice == 0, lds, beta)) return;
+  if (MODE == SPMV_NEXT && !open_iteration<WAVE>(cur, nz, scal, flag, j, slice == 0, lds, beta, hflag)) return;
   double dp = 0.0;
   if (active) {
     float y0 = 0.0f, y1 = 0.0f, y2 = 0.0f;
@@ -359,7 +371,7 @@ __device__ __forceinline__ void pcg_spmv_body(uint32_t slice, const SellView &A,
 template <int MODE, uint32_t G> __device__ __forceinline__ void pcg_iter_body(uint32_t b, const SpmvArgs &a, uint32_t j) {
   const uint32_t ns = (a.A.nb + 63u) / 64u;
   if (b < ns)
-    pcg_spmv_body<MODE, G>(b, a.A, a.C, a.z, a.pold, a.pnew, a.q, a.part, a.cur, a.nz, a.scal, a.flag, j);
+    pcg_spmv_body<MODE, G>(b, a.A, a.C, a.z, a.pold, a.pnew, a.q, a.part, a.cur, a.nz, a.scal, a.flag, j, a.hflag);
   else
     pcg_stencil_body<MODE>(b - ns, a.C, a.z, a.pold, a.cur, a.nz, a.scal, a.flag, j, a.part + ns);
 }
@@ -630,7 +642,7 @@ __global__ __launch_bounds__(BLOCK) void k_minv(SellView A, ContactView C, const
 // 7% more iterations on the same drape states).
 constexpr double REFINE_ETA_MIN = 1.0e-7;
 // Iterations queued per host convergence check: at least Pcg::chunk, growing with the sweep
-// to a sixteenth of the iterations done (at most CHUNK_MAX), so the flag copies thin out on
+// to a sixteenth of the iterations done (at most CHUNK_MAX), so the host checks thin out on
 // long solves while the launches queued past convergence stay a small share.
 constexpr uint32_t CHUNK_MAX = 32;
 inline uint32_t chunk_len(uint32_t chunk, uint32_t j) { return std::min(CHUNK_MAX, std::max(chunk, j / 16)); }
@@ -651,7 +663,9 @@ void Pcg::ensure(uint32_t nb, uint32_t ns) {
     part_.reserve(std::max(S, G) + 2 * nz + 8);
   }
   if (!h_flag_) {
-    NGUO_HIP(hipHostMalloc((void **)&h_flag_, 8 * sizeof(uint32_t), hipHostMallocDefault));
+    // coherent and mapped: the iteration launch stores a chunk's flag here directly
+    NGUO_HIP(hipHostMalloc((void **)&h_flag_, 8 * sizeof(uint32_t), hipHostMallocMapped | hipHostMallocCoherent));
+    NGUO_HIP(hipHostGetDevicePointer((void **)&d_flag_, h_flag_, 0));
     for (auto &e : ev_) NGUO_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     NGUO_HIP(hipHostMalloc((void **)&h_scal_, 8 * sizeof(double), hipHostMallocDefault));
   }
@@ -761,10 +775,14 @@ PcgStats Pcg::solve(const SellMatrix &M, const ContactView &C, const float *b, f
     // convergence exit at their first instruction.
     uint32_t j = 0, queued = 0, waited = 0;
     auto queue_chunk = [&]() {
+      const uint32_t sl = queued & 1u;
       for (uint32_t c = 0, n = chunk_len(chunk, j); c < n; ++c, ++j) {
         float *pj = (j & 1) ? p2_.p : p_.p;
-        if (j == 0) launch_iter<SPMV_FIRST>(spmv_group_, s_, first, j);
-        else launch_iter<SPMV_NEXT>(spmv_group_, s_, (j & 1) ? next_odd : next_even, j);
+        // the chunk's last iteration launch publishes the flag to host slot sl
+        SpmvArgs a = j == 0 ? first : ((j & 1) ? next_odd : next_even);
+        a.hflag = c + 1 == n ? d_flag_ + 4 * sl : nullptr;
+        if (j == 0) launch_iter<SPMV_FIRST>(spmv_group_, s_, a, j);
+        else launch_iter<SPMV_NEXT>(spmv_group_, s_, a, j);
         if (sz_) {
           launch_schwarz_pcg(s_, *sz_, false, nullptr, r_.p, z_.p, d_.p, pj, q_.p, contrib, part_spmv, NI, part_pc, scal_.p, flag_.p, j);
           launch_schwarz_coarse(s_, *sz_, false, r_.p, z_.p, pj, part_pc, flag_.p);
@@ -773,19 +791,21 @@ PcgStats Pcg::solve(const SellMatrix &M, const ContactView &C, const float *b, f
                                 flag_.p, j);
       }
       NGUO_HIP(hipGetLastError());
-      const uint32_t sl = queued & 1u;
-      NGUO_HIP(hipMemcpyAsync(h_flag_ + 4 * sl, flag_.p, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, s_));
       NGUO_HIP(hipEventRecord(ev_[sl], s_));
       ++queued;
     };
     const uint32_t *fl = nullptr;
     queue_chunk();
+    const uint32_t sweep_cap = max_iter - st.iters;
     while (true) {
       queue_chunk();
       const uint32_t sl = waited & 1u;
       NGUO_HIP(hipEventSynchronize(ev_[sl]));
       ++waited;
       if (h_flag_[4 * sl]) { fl = h_flag_ + 4 * sl; break; }
+      // the cap sets a state within the chunk that reaches it: a flag still clear two chunks
+      // later means the host slot is not being written
+      NGUO_REQUIRE(j <= sweep_cap + 2 * CHUNK_MAX + 2, NGUO_E_DEVICE, "PCG: the convergence flag did not reach the host");
     }
     st.iters += fl[1];
     st.sweeps += 1;

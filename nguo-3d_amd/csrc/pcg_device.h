@@ -42,6 +42,9 @@ struct SpmvArgs {
   uint32_t nz;
   double *scal;
   uint32_t *flag;
+  // host-visible copy of flag[0..2] (state, iterations, cap), written by the first slice's
+  // workgroup; set on the last launch of a queued chunk, null otherwise
+  uint32_t *hflag = nullptr;
 };
 struct SchArgs {
   uint32_t nd;
