@@ -173,6 +173,8 @@ void nguo_step_destroy(nguo_step *st);
 int nguo_step_advance(nguo_step *st, uint32_t nsteps);
 int nguo_step_get_positions(nguo_step *st, float *out /* [3*n_verts] */);
 int nguo_step_get_velocities(nguo_step *st, float *out /* [3*n_verts] */);
+/* positions of every vertex (pinned ones included: the trees over the pinned primitives are
+ * rebuilt from them); velocities may be null (kept) */
 int nguo_step_set_state(nguo_step *st, const float *positions, const float *velocities);
 int nguo_step_get_metrics(nguo_step *st, nguo_step_metrics *out);
 /* Positions after Newton iteration k of the last advanced step (the oracle's per-iteration
