@@ -117,6 +117,35 @@ def test_pcg_manufactured_laplacian(ctx):
 
 
 @pytest.mark.gpu
+def test_pcg_stops_at_the_iteration_cap(ctx):
+    # the 1-D Laplacian of 600 unknowns needs far more than these iterations: each solve ends
+    # on its cap, and the iteration count the solver reports is exactly the cap. Caps inside
+    # a queued chunk, at a chunk's start and at a chunk boundary (host convergence checks
+    # every 8 iterations at first). PCG minimises the A-norm of the error over a growing
+    # Krylov space, so that norm falls with every cap.
+    n = 3 * 200
+    rp, ci, va = laplacian_1d(n)
+    xs = np.linspace(-1, 1, n).astype(np.float32)
+    b = O.spmv(rp, ci, va, xs)
+    ls = nguo_hip.LinearSolver(ctx)
+
+    def a_norm_err(x):
+        e = x.astype(np.float64) - xs.astype(np.float64)
+        ae = np.array([np.dot(va[rp[r]:rp[r + 1]].astype(np.float64), e[ci[rp[r]:rp[r + 1]]]) for r in range(n)])
+        return float(np.sqrt(max(np.dot(e, ae), 0.0)))
+
+    errs = [a_norm_err(np.zeros(n, np.float32))]
+    for cap in (5, 8, 13, 16):
+        x, st = ls.solve(rp, ci, va, b, 1e-9, cap)
+        assert st.iters == cap, (cap, st.iters)
+        assert np.all(np.isfinite(x))
+        assert host_residual(rp, ci, va, b, x) > 1e-9  # stopped by the cap, not converged
+        errs.append(a_norm_err(x))
+    assert all(b2 < a for a, b2 in zip(errs, errs[1:])), errs
+    ls.close()
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("cfg,steps", [(0, 0), (0, 14)])
 def test_pcg_matches_oracle_cg(ctx, cfg, steps):
     rp, ci, va, rhs = newton_system(cfg, steps)
