@@ -7,6 +7,7 @@
 // radix.wgsl (serial per-block scatter, n <= 65 536), scan.rs, karras.rs:24-88 + karras.wgsl,
 // bvh_aabb.rs:28-55 + bvh_aabb.wgsl (host-driven level passes), bvh_traverse.rs:77-260.
 #pragma once
+#include <type_traits>
 #include <vector>
 
 #include "devmath.h"
@@ -18,6 +19,7 @@ constexpr uint32_t LEAF = 0x80000000u;
 // entries of a traversal record: a node's grandchildren. (Eight, its great-grandchildren in
 // 256 bytes, made the garment drape's step 1% slower than four.)
 constexpr uint32_t TRAV_WIDTH = 4;
+constexpr uint32_t TRAV_ENTRY = 3;  // float4 per record entry: box and refs, then vertex ids
 constexpr uint32_t TRAV_NONE = 0xffffffffu;   // unused record entry (primitive ids are below 2^31)
 
 // Stable ascending sort of (key, value) pairs on the device.
@@ -41,13 +43,16 @@ struct BvhView {
   const float *node_box;  // [6 (n-1)] lo.xyz hi.xyz
   const float *leaf_box;  // [6 n] in sorted order
   const uint32_t *prim;   // [n] primitive id of sorted leaf slot
-  // [2 TRAV_WIDTH (n-1)] traversal records: internal node i's up to TRAV_WIDTH grandchildren
-  // (a leaf child stands for itself) in one 128-byte record, entry e as (lo.xyz, hi.x)
-  // (hi.yz, ref, last sorted leaf slot under it); ref is LEAF | the primitive id for a leaf
-  // (not its slot, as in left / right), else the binary node id; TRAV_NONE for an unused entry
+  // [TRAV_ENTRY TRAV_WIDTH (n-1)] traversal records: internal node i's up to TRAV_WIDTH
+  // grandchildren (a leaf child stands for itself) in one 192-byte record, entry e as
+  // (lo.xyz, hi.x) (hi.yz, ref, last sorted leaf slot under it) (vertex ids of a leaf's
+  // primitive: three for a triangle, two and 0 for an edge; 0 for a node); ref is LEAF | the
+  // primitive id for a leaf (not its slot, as in left / right), else the binary node id;
+  // TRAV_NONE for an unused entry
   const float4 *trav;
-  // [2 TRAV_PARTS] the tree split into disjoint subtrees that together hold every leaf once:
-  // entry k = (lo.xyz, ref) (hi.xyz, last leaf slot), ref as in trav, ~0u for an unused entry
+  // [TRAV_ENTRY TRAV_PARTS] the tree split into disjoint subtrees that together hold every
+  // leaf once: entry k = (lo.xyz, ref) (hi.xyz, last leaf slot) (vertex ids as in trav), ref
+  // as in trav, ~0u for an unused entry
   const float4 *front;
   const uint32_t *slot;   // [n] sorted leaf slot of element k of the build's primitive list
 };
@@ -84,8 +89,9 @@ void aabb_build(uint32_t n, const uint32_t *left, const uint32_t *right, uint32_
 // [range[2i], range[2i+1]] (every Karras tree); `table` is scratch.
 void aabb_from_ranges(uint32_t n, const uint32_t *range, const float *leaf_box, float *node_box, DevBuf<float> &table, hipStream_t s);
 
-// Stack traversal: calls visit(prim) for every leaf whose box passes `overlap(box_lo,
-// box_hi)` (a conservative test). The first TRAV_LDS stack entries of a thread live in LDS
+// Stack traversal: calls visit(prim) -- or visit(prim, vid) when the visitor takes the
+// primitive's vertex ids (uint3, from the record: the leaf then needs no index load) -- for
+// every leaf whose box passes `overlap(box_lo, box_hi)` (a conservative test). The first TRAV_LDS stack entries of a thread live in LDS
 // (`lds`: the calling kernel's __shared__ uint32_t[TRAV_LDS * BLOCK], interleaved by thread so
 // that a wavefront's pushes and pops fall in distinct banks); deeper entries go to a private
 // array. A stack deeper than TRAV_STACK sets *err (reported to the caller as
@@ -95,6 +101,12 @@ void aabb_from_ranges(uint32_t n, const uint32_t *range, const float *leaf_box, 
 // once by passing its own slot + 1).
 constexpr int TRAV_STACK = 128;
 constexpr int TRAV_LDS = 32;
+template <class Visit>
+__device__ __forceinline__ void trav_visit(Visit &visit, uint32_t prim, uint3 vid) {
+  if constexpr (std::is_invocable_v<Visit &, uint32_t, uint3>) visit(prim, vid);
+  else visit(prim);
+}
+
 // Depth-first traversal of the subtree under internal node `root`.
 template <class Overlap, class Visit>
 __device__ inline void bvh_traverse_from(const BvhView &t, uint32_t root, uint32_t *err, uint32_t *lds, Overlap overlap, Visit visit,
@@ -111,19 +123,20 @@ __device__ inline void bvh_traverse_from(const BvhView &t, uint32_t root, uint32
   while (sp > 0) {
     --sp;
     const uint32_t node = sp < TRAV_LDS ? mine[sp * BLOCK] : spill[sp - TRAV_LDS];
-    const float4 *rec = t.trav + 2 * TRAV_WIDTH * (size_t)node;
-    float4 q[2 * TRAV_WIDTH];
+    const float4 *rec = t.trav + TRAV_ENTRY * TRAV_WIDTH * (size_t)node;
+    float4 q[TRAV_ENTRY * TRAV_WIDTH];
 #pragma unroll
-    for (uint32_t k = 0; k < 2 * TRAV_WIDTH; ++k) q[k] = rec[k];
+    for (uint32_t k = 0; k < TRAV_ENTRY * TRAV_WIDTH; ++k) q[k] = rec[k];
     // box tests and pushes unrolled (the record stays in registers); the leaves met are
     // visited after, from a mask, through one copy of the visit code
     uint32_t ref[TRAV_WIDTH], leaves = 0;
 #pragma unroll
     for (uint32_t c = 0; c < TRAV_WIDTH; ++c) {
-      ref[c] = __float_as_uint(q[2 * c + 1].z);
-      const uint32_t last = __float_as_uint(q[2 * c + 1].w);
+      const float4 e0 = q[TRAV_ENTRY * c], e1 = q[TRAV_ENTRY * c + 1];
+      ref[c] = __float_as_uint(e1.z);
+      const uint32_t last = __float_as_uint(e1.w);
       if (ref[c] == TRAV_NONE || last < min_slot) continue;
-      if (!overlap(f3{q[2 * c].x, q[2 * c].y, q[2 * c].z}, f3{q[2 * c].w, q[2 * c + 1].x, q[2 * c + 1].y})) continue;
+      if (!overlap(f3{e0.x, e0.y, e0.z}, f3{e0.w, e1.x, e1.y})) continue;
       if (ref[c] & LEAF) {
         leaves |= 1u << c;
       } else if (sp < TRAV_LDS) {
@@ -141,7 +154,8 @@ __device__ inline void bvh_traverse_from(const BvhView &t, uint32_t root, uint32
       const uint32_t c = __builtin_ctz(leaves);
       leaves &= leaves - 1;
       const uint32_t r = c == 0 ? ref[0] : c == 1 ? ref[1] : c == 2 ? ref[2] : ref[3];
-      visit(r & ~LEAF);
+      const float4 v = c == 0 ? q[2] : c == 1 ? q[TRAV_ENTRY + 2] : c == 2 ? q[2 * TRAV_ENTRY + 2] : q[3 * TRAV_ENTRY + 2];
+      trav_visit(visit, r & ~LEAF, make_uint3(__float_as_uint(v.x), __float_as_uint(v.y), __float_as_uint(v.z)));
     }
   }
 }
@@ -151,8 +165,10 @@ __device__ inline void bvh_traverse(const BvhView &t, uint32_t *err, uint32_t *l
   // whole-tree form (no slot bound)
   if (t.n == 0) return;
   if (t.n == 1) {
-    const float *b = t.leaf_box;
-    if (overlap(f3{b[0], b[1], b[2]}, f3{b[3], b[4], b[5]})) visit(t.prim[0]);
+    // the frontier's first entry is the one leaf
+    const float4 a = t.front[0], b = t.front[1], v = t.front[2];
+    if (overlap(f3{a.x, a.y, a.z}, f3{b.x, b.y, b.z}))
+      trav_visit(visit, __float_as_uint(a.w) & ~LEAF, make_uint3(__float_as_uint(v.x), __float_as_uint(v.y), __float_as_uint(v.z)));
     return;
   }
   bvh_traverse_from(t, 0u, err, lds, overlap, visit);
@@ -164,11 +180,14 @@ template <class Overlap, class Visit>
 __device__ inline void bvh_traverse_part(const BvhView &t, uint32_t part, uint32_t *err, uint32_t *lds, Overlap overlap, Visit visit,
                                          uint32_t min_slot = 0) {
   if (t.n == 0) return;
-  const float4 a = t.front[2 * part], b = t.front[2 * part + 1];
+  const float4 a = t.front[TRAV_ENTRY * part], b = t.front[TRAV_ENTRY * part + 1];
   const uint32_t r = __float_as_uint(a.w);
   if (r == 0xffffffffu || __float_as_uint(b.w) < min_slot) return;
   if (!overlap(f3{a.x, a.y, a.z}, f3{b.x, b.y, b.z})) return;
-  if (r & LEAF) visit(r & ~LEAF);
+  if (r & LEAF) {
+    const float4 v = t.front[TRAV_ENTRY * part + 2];
+    trav_visit(visit, r & ~LEAF, make_uint3(__float_as_uint(v.x), __float_as_uint(v.y), __float_as_uint(v.z)));
+  }
   else bvh_traverse_from(t, r, err, lds, overlap, visit, min_slot);
 }
 

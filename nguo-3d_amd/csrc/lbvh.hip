@@ -473,42 +473,55 @@ void aabb_from_ranges(uint32_t n, const uint32_t *range, const float *leaf_box, 
 }
 
 // Traversal records: internal node i's up to four grandchildren (a child that is a leaf
-// stands for itself) in 128 bytes, entry e as (lo.xyz, hi.x) (hi.yz, ref, last leaf slot).
-// ref is LEAF | the primitive id for a leaf, else the binary node id, whose own record holds
-// its grandchildren; an unused entry has ref TRAV_NONE. A traversal then tests four boxes per
-// dependent record load, and its chain of loads is half as long as over the binary nodes.
-__device__ inline void trav_entry(uint32_t c, const uint32_t *range, const uint32_t *prim, const float *leaf_box, const float *node_box,
-                                  float4 &e0, float4 &e1) {
+// stands for itself) in 192 bytes, entry e as (lo.xyz, hi.x) (hi.yz, ref, last leaf slot)
+// (a leaf's vertex ids). ref is LEAF | the primitive id for a leaf, else the binary node id,
+// whose own record holds its grandchildren; an unused entry has ref TRAV_NONE. A traversal
+// then tests four boxes per dependent record load, so its chain of loads is half as long as
+// over the binary nodes, and a leaf it meets loads its positions at once.
+struct TravSrc {
+  int kind;  // 0: triangles (3 vertex ids per primitive), 1: edges (2)
+  const uint32_t *idx, *range, *prim;
+  const float *leaf_box, *node_box;
+};
+__device__ inline void trav_entry(uint32_t c, const TravSrc &S, float4 &e0, float4 &e1, float4 &e2) {
   const bool leaf = c & LEAF;
-  const float *b = leaf ? leaf_box + 6 * (size_t)(c & ~LEAF) : node_box + 6 * (size_t)c;
-  const uint32_t last = leaf ? c & ~LEAF : range[2 * (size_t)c + 1];
-  const uint32_t ref = leaf ? (LEAF | prim[c & ~LEAF]) : c;
+  const float *b = leaf ? S.leaf_box + 6 * (size_t)(c & ~LEAF) : S.node_box + 6 * (size_t)c;
+  const uint32_t last = leaf ? c & ~LEAF : S.range[2 * (size_t)c + 1];
+  const uint32_t p = leaf ? S.prim[c & ~LEAF] : 0u;
+  const uint32_t ref = leaf ? (LEAF | p) : c;
+  uint32_t v[3] = {0u, 0u, 0u};
+  if (leaf) {
+    const uint32_t k = S.kind == 0 ? 3u : 2u;
+    for (uint32_t i = 0; i < k; ++i) v[i] = S.idx[k * (size_t)p + i];
+  }
   e0 = make_float4(b[0], b[1], b[2], b[3]);
   e1 = make_float4(b[4], b[5], __uint_as_float(ref), __uint_as_float(last));
+  e2 = make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]), 0.0f);
 }
-__global__ __launch_bounds__(BLOCK) void k_pack_trav(uint32_t n, const uint32_t *left, const uint32_t *right, const uint32_t *range,
-                                                    const uint32_t *prim, const float *leaf_box, const float *node_box, float4 *trav) {
+__global__ __launch_bounds__(BLOCK) void k_pack_trav(uint32_t n, const uint32_t *left, const uint32_t *right, TravSrc S, float4 *trav) {
   uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
   if (i + 1 >= n) return;
-  float4 e[2 * TRAV_WIDTH];
+  float4 e[TRAV_ENTRY * TRAV_WIDTH];
   uint32_t k = 0;
   const uint32_t ch[2] = {left[i], right[i]};
   for (int c = 0; c < 2; ++c) {
     if (ch[c] & LEAF) {
-      trav_entry(ch[c], range, prim, leaf_box, node_box, e[2 * k], e[2 * k + 1]);
+      trav_entry(ch[c], S, e[3 * k], e[3 * k + 1], e[3 * k + 2]);
       ++k;
     } else {
-      trav_entry(left[ch[c]], range, prim, leaf_box, node_box, e[2 * k], e[2 * k + 1]);
-      trav_entry(right[ch[c]], range, prim, leaf_box, node_box, e[2 * k + 2], e[2 * k + 3]);
+      trav_entry(left[ch[c]], S, e[3 * k], e[3 * k + 1], e[3 * k + 2]);
+      trav_entry(right[ch[c]], S, e[3 * k + 3], e[3 * k + 4], e[3 * k + 5]);
       k += 2;
     }
   }
   for (; k < TRAV_WIDTH; ++k) {
-    e[2 * k] = make_float4(0.f, 0.f, 0.f, 0.f);
-    e[2 * k + 1] = make_float4(0.f, 0.f, __uint_as_float(TRAV_NONE), __uint_as_float(0u));
+    e[3 * k] = make_float4(0.f, 0.f, 0.f, 0.f);
+    e[3 * k + 1] = make_float4(0.f, 0.f, __uint_as_float(TRAV_NONE), __uint_as_float(0u));
+    e[3 * k + 2] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
-  float4 *o = trav + 2 * TRAV_WIDTH * (size_t)i;
-  for (uint32_t q = 0; q < 2 * TRAV_WIDTH; ++q) o[q] = e[q];
+  static_assert(TRAV_ENTRY == 3, "three float4 per entry");
+  float4 *o = trav + TRAV_ENTRY * TRAV_WIDTH * (size_t)i;
+  for (uint32_t q = 0; q < TRAV_ENTRY * TRAV_WIDTH; ++q) o[q] = e[q];
 }
 
 // slot[k] = the sorted leaf slot holding element k of the build's primitive list.
@@ -520,25 +533,23 @@ __global__ __launch_bounds__(BLOCK) void k_slot_of(uint32_t n, const uint32_t *o
 // The traversal frontier: starting from the root's children, internal entries are replaced by
 // their two children breadth first while there is room for TRAV_PARTS entries; leaves stay.
 // One thread, at most TRAV_PARTS - 2 node expansions.
-__global__ void k_frontier(uint32_t n, const uint32_t *left, const uint32_t *right, const uint32_t *range, const uint32_t *prim, const float *leaf_box,
-                           const float *node_box, float4 *front) {
+__global__ void k_frontier(uint32_t n, const uint32_t *left, const uint32_t *right, TravSrc S, float4 *front) {
   if (threadIdx.x != 0) return;
   uint32_t ref[TRAV_PARTS];
-  float4 lo[TRAV_PARTS], hi[TRAV_PARTS];  // hi.w: the entry's last leaf slot
+  float4 lo[TRAV_PARTS], hi[TRAV_PARTS], vid[TRAV_PARTS];  // hi.w: the entry's last leaf slot
   uint32_t cnt = 0;
+  auto entry = [&](uint32_t c, uint32_t o) {
+    float4 e0, e1, e2;
+    trav_entry(c, S, e0, e1, e2);
+    ref[o] = __float_as_uint(e1.z);
+    lo[o] = make_float4(e0.x, e0.y, e0.z, 0.0f);
+    hi[o] = make_float4(e0.w, e1.x, e1.y, e1.w);
+    vid[o] = e2;
+  };
   if (n == 1) {
-    ref[0] = LEAF | prim[0];
-    lo[0] = make_float4(leaf_box[0], leaf_box[1], leaf_box[2], 0.0f);
-    hi[0] = make_float4(leaf_box[3], leaf_box[4], leaf_box[5], __uint_as_float(0u));
+    entry(LEAF | 0u, 0);
     cnt = 1;
   } else if (n >= 2) {
-    auto entry = [&](uint32_t c, uint32_t o) {
-      float4 e0, e1;
-      trav_entry(c, range, prim, leaf_box, node_box, e0, e1);
-      ref[o] = __float_as_uint(e1.z);
-      lo[o] = make_float4(e0.x, e0.y, e0.z, 0.0f);
-      hi[o] = make_float4(e0.w, e1.x, e1.y, e1.w);
-    };
     auto children = [&](uint32_t node, uint32_t o0, uint32_t o1) {
       entry(left[node], o0);
       entry(right[node], o1);
@@ -560,8 +571,9 @@ __global__ void k_frontier(uint32_t n, const uint32_t *left, const uint32_t *rig
   }
   for (uint32_t k = 0; k < TRAV_PARTS; ++k) {
     const bool used = k < cnt;
-    front[2 * k] = used ? make_float4(lo[k].x, lo[k].y, lo[k].z, __uint_as_float(ref[k])) : make_float4(0.f, 0.f, 0.f, __uint_as_float(0xffffffffu));
-    front[2 * k + 1] = used ? hi[k] : make_float4(0.f, 0.f, 0.f, __uint_as_float(0u));
+    front[TRAV_ENTRY * k] = used ? make_float4(lo[k].x, lo[k].y, lo[k].z, __uint_as_float(ref[k])) : make_float4(0.f, 0.f, 0.f, __uint_as_float(0xffffffffu));
+    front[TRAV_ENTRY * k + 1] = used ? hi[k] : make_float4(0.f, 0.f, 0.f, __uint_as_float(0u));
+    front[TRAV_ENTRY * k + 2] = used ? vid[k] : make_float4(0.f, 0.f, 0.f, 0.f);
   }
 }
 
@@ -602,18 +614,17 @@ void Lbvh::build(int kind, uint32_t n, const uint32_t *ids, const uint32_t *idx,
   slot_.reserve(n);
   hipLaunchKernelGGL(k_slot_of, dim3(g), dim3(BLOCK), 0, s, n, (const uint32_t *)order.p, slot_.p);
   hipLaunchKernelGGL(k_leaf_boxes, dim3(g), dim3(BLOCK), 0, s, kind, n, (const uint32_t *)prim_.p, idx, pos, leaf_box_.p);
+  const TravSrc src{kind, idx, range_.p, prim_.p, leaf_box_.p, node_box_.p};
   if (n >= 2) {
     hipLaunchKernelGGL(k_karras, dim3(div_up(n - 1, BLOCK)), dim3(BLOCK), 0, s, n, (const uint32_t *)keys_.p, left_.p, right_.p,
                        (uint32_t *)nullptr, range_.p);
     aabb_from_ranges(n, range_.p, leaf_box_.p, node_box_.p, table_, s);
-    trav_.reserve(8 * TRAV_WIDTH * (size_t)n);
-    hipLaunchKernelGGL(k_pack_trav, dim3(div_up(n - 1, BLOCK)), dim3(BLOCK), 0, s, n, (const uint32_t *)left_.p, (const uint32_t *)right_.p,
-                       (const uint32_t *)range_.p, (const uint32_t *)prim_.p, (const float *)leaf_box_.p, (const float *)node_box_.p,
+    trav_.reserve(4 * TRAV_ENTRY * TRAV_WIDTH * (size_t)n);
+    hipLaunchKernelGGL(k_pack_trav, dim3(div_up(n - 1, BLOCK)), dim3(BLOCK), 0, s, n, (const uint32_t *)left_.p, (const uint32_t *)right_.p, src,
                        (float4 *)trav_.p);
   }
-  front_.reserve(8 * TRAV_PARTS);
-  hipLaunchKernelGGL(k_frontier, dim3(1), dim3(WAVE), 0, s, n, (const uint32_t *)left_.p, (const uint32_t *)right_.p, (const uint32_t *)range_.p,
-                     (const uint32_t *)prim_.p, (const float *)leaf_box_.p, (const float *)node_box_.p, (float4 *)front_.p);
+  front_.reserve(4 * TRAV_ENTRY * TRAV_PARTS);
+  hipLaunchKernelGGL(k_frontier, dim3(1), dim3(WAVE), 0, s, n, (const uint32_t *)left_.p, (const uint32_t *)right_.p, src, (float4 *)front_.p);
   NGUO_HIP(hipGetLastError());
 }
 

@@ -419,10 +419,11 @@ __global__ __launch_bounds__(BLOCK) void k_vt_stencils(Topo T, BvhView dyn, BvhV
   const bool below = T.side ? T.side[v] < 0.0f : false;
   ExcList exl;
   exl.load(T, v);
-  // a leaf loads its positions and flags in one memory round trip after the triangle's
-  // vertex ids and tests the filters (predicates, in any order) from registers
-  auto leaf = [&](uint32_t t) {
-    uint32_t t0 = T.tris[3 * (size_t)t], t1 = T.tris[3 * (size_t)t + 1], t2 = T.tris[3 * (size_t)t + 2];
+  // a leaf loads its positions and flags in one memory round trip (the triangle's vertex ids
+  // come with the traversal record) and tests the filters (predicates, in any order) from
+  // registers
+  auto leaf = [&](uint32_t t, uint3 vid) {
+    const uint32_t t0 = vid.x, t1 = vid.y, t2 = vid.z;
     f3 a = ld3(x, t0), b = ld3(x, t1), c = ld3(x, t2);
     const float m0 = T.im[t0], m1 = T.im[t1], m2 = T.im[t2];
     const bool st_all = T.static_v ? (T.static_v[t0] && T.static_v[t1] && T.static_v[t2]) : false;
@@ -464,9 +465,9 @@ __global__ __launch_bounds__(BLOCK) void k_ee_stencils(Topo T, BvhView tree, con
   // edge-index order: the lower edge's two, then the higher's); a movable query meets each
   // other movable edge once: only the leaves after its own sorted slot
   const uint32_t min_slot = self_query ? tree.slot[q] + 1 : 0u;
-  bvh_traverse_part(tree, blockIdx.y, err, tstack, ov, [&](uint32_t ei) {
+  bvh_traverse_part(tree, blockIdx.y, err, tstack, ov, [&](uint32_t ei, uint3 vid) {
     if (ei == ej) return;
-    uint32_t a0 = T.edges[2 * (size_t)ei], a1 = T.edges[2 * (size_t)ei + 1];
+    const uint32_t a0 = vid.x, a1 = vid.y;
     if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) return;
     f3 pa = ld3(x, a0), qa = ld3(x, a1);
     if (!ov(vmin(pa, qa), vmax(pa, qa))) return;
@@ -801,9 +802,9 @@ __device__ inline bool defer_pair(const CcdDefer &D, uint32_t a, uint32_t b, uin
   return true;
 }
 
-// Leaves load their positions, motions and flags in one memory round trip (after the
-// triangle's vertex ids) and test the filters from registers: the filters are predicates, so
-// their order changes nothing but the latency.
+// Leaves load their positions, motions and flags in one memory round trip (the triangle's
+// vertex ids come with the traversal record) and test the filters from registers: the
+// filters are predicates, so their order changes nothing but the latency.
 __global__ __launch_bounds__(BLOCK) void k_ccd_vt(Topo T, BvhView dyn, BvhView stat, const float *__restrict__ x, const float *__restrict__ dx,
                                                  float pad0, float max_disp, uint32_t iters, CcdDefer D, unsigned long long *tmin, uint32_t *err) {
   __shared__ uint32_t tstack[TRAV_LDS * BLOCK];
@@ -817,8 +818,8 @@ __global__ __launch_bounds__(BLOCK) void k_ccd_vt(Topo T, BvhView dyn, BvhView s
     ExcList exl;
     exl.load(T, v);
     const float pad = ccd_query_pad(pad0, len(dvv), max_disp, iters);
-    auto leaf = [&](uint32_t t) {
-      uint32_t t0 = T.tris[3 * (size_t)t], t1 = T.tris[3 * (size_t)t + 1], t2 = T.tris[3 * (size_t)t + 2];
+    auto leaf = [&](uint32_t t, uint3 vid) {
+      const uint32_t t0 = vid.x, t1 = vid.y, t2 = vid.z;
       f3 a = ld3(x, t0), b = ld3(x, t1), c = ld3(x, t2);
       const f3 da = ld3(dx, t0), db = ld3(dx, t1), dc = ld3(dx, t2);
       const float m0 = T.im[t0], m1 = T.im[t1], m2 = T.im[t2];
@@ -867,14 +868,14 @@ __global__ __launch_bounds__(BLOCK) void k_ccd_ee(Topo T, BvhView tree, const ui
     const float pad = ccd_query_pad(pad0, fmaxf(len(db0), len(db1)), max_disp, iters);
     f3 blo = vmin(pb, qb) - f3{pad, pad, pad}, bhi = vmax(pb, qb) + f3{pad, pad, pad};
     auto ov = [&](f3 lo, f3 hi) { return !(lo.x > bhi.x || lo.y > bhi.y || lo.z > bhi.z || hi.x < blo.x || hi.y < blo.y || hi.z < blo.z); };
-    // a leaf loads its edge's positions and motions in one memory round trip after the vertex
-    // ids, and reuses the query edge's from registers. A query of the tree's own edges meets
+    // a leaf loads its edge's positions and motions in one memory round trip (the vertex ids
+    // come with the traversal record), and reuses the query edge's from registers. A query of the tree's own edges meets
     // each other edge once: only the leaves after its own sorted slot (whole subtrees before
     // it are skipped).
     const uint32_t min_slot = self_query ? tree.slot[q] + 1 : 0u;
-    bvh_traverse_part(tree, blockIdx.y, err, tstack, ov, [&](uint32_t ei) {
+    bvh_traverse_part(tree, blockIdx.y, err, tstack, ov, [&](uint32_t ei, uint3 vid) {
       if (ei == ej) return;
-      uint32_t a0 = T.edges[2 * (size_t)ei], a1 = T.edges[2 * (size_t)ei + 1];
+      const uint32_t a0 = vid.x, a1 = vid.y;
       if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) return;
       f3 pa = ld3(x, a0), qa = ld3(x, a1);
       const f3 da0 = ld3(dx, a0), da1 = ld3(dx, a1);
