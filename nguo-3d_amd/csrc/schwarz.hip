@@ -182,9 +182,10 @@ __device__ void factor_invert(double *L, double *diag0, double *colj, double &fl
   // lower-triangular inverse): w_jj = 1 / l_jj and, below the diagonal, w_ij = -w_jj
   // sum_{k = j+1 .. i} w_ik l_kj with the trailing block already inverted and the original
   // column j copied to colj. A row's sum is split over the 16 lanes of a DPP row (lane q takes
-  // k = j + 1 + q, q + 16, ...) and combined by the row exchanges, a fixed order. L's upper
-  // triangle holds W. One packed triangle in LDS instead of two m x m arrays lets four
-  // workgroups share a compute unit.
+  // k = j + 1 + q, q + 16, ...) and combined by the row exchanges, a fixed order. The packed
+  // lower triangle then holds W: one triangle in LDS instead of two m x m arrays lets four
+  // workgroups share a compute unit. (Running this and the Cholesky in one wavefront, with
+  // wavefront fences instead of workgroup barriers, took twice as long.)
   for (int jj = (int)m - 1; jj >= 0; --jj) {
     const uint32_t j = (uint32_t)jj;
     for (uint32_t i = j + 1 + t; i < m; i += BLOCK) colj[i] = L[tri(i) + j];
