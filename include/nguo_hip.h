@@ -254,6 +254,19 @@ int nguo_lbvh_sort_pairs(nguo_ctx *ctx, uint32_t n, uint32_t *keys, uint32_t *va
 int nguo_lbvh_karras(nguo_ctx *ctx, uint32_t n, const uint32_t *codes, uint32_t *left, uint32_t *right);
 int nguo_lbvh_aabb(nguo_ctx *ctx, uint32_t n, const uint32_t *left, const uint32_t *right, const float *leaf_box, float *node_box);
 
+/* The sort of nguo_lbvh_sort_pairs on DEVICE buffers, allocating nothing: no hipMalloc,
+ * hipFree or allocating library call. No counterpart in the reference; it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation). keys_dev / vals_dev (n u32 each, device memory of the
+ * context's device) are sorted in place, stably by key, bitwise as nguo_lbvh_sort_pairs
+ * sorts them. The intermediate buffers live in one file-scope device array per GPU (9.4 MB,
+ * held until the process ends), so n is at most NGUO_NOALLOC_SORT_MAX (NGUO_E_CAPACITY
+ * beyond). Every call on a GPU shares that GPU's array: calls that overlap on one GPU (from
+ * several threads or contexts) race and must be serialised by the caller. The call queues on
+ * the context's stream and returns after the sort has finished. */
+#define NGUO_NOALLOC_SORT_MAX (1u << 20)
+int nguo_lbvh_sort_pairs_noalloc(nguo_ctx *ctx, uint32_t n, uint32_t *keys_dev, uint32_t *vals_dev);
+
 /* ---- scene generators for the benchmark workloads (host only, no device needed).
  * config: 0 sheet on icosphere, 1 tube garment on a capsule body with seam rods,
  * 2 two tangled concentric tubes, 3 graded tube (param = circumference factor),

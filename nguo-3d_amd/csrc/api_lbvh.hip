@@ -1,12 +1,70 @@
 // C ABI of the LBVH stages on a triangle set, so each stage can be checked bit for bit:
 // engine-gpu/src/morton.rs:61-230 (GpuMorton::codes), radix.rs:112-135 (sort_pairs),
 // karras.rs:91-262 (GpuKarras::build_tree), bvh_aabb.rs (node AABB propagation).
+#include <mutex>
+
 #include "device_common.h"
 #include "lbvh.h"
 
 using namespace nguo;
 
+// ---- the allocation-free sort's scratch: one file-scope device array, NGUO_NOALLOC_SORT_MAX
+// pairs' worth (the key and value ping-pong halves, the digit counters of the small-tile
+// passes, which need the most, and the scan's tile sums: 9.4 MB). Every GPU the library runs
+// on holds its own copy until the process ends. In host code `scratch` names a host-side
+// stand-in, never the device memory: the copy's device address comes from
+// hipGetSymbolAddress, per device, and only that address reaches a kernel.
+namespace {
+constexpr size_t NOALLOC_WORDS =
+    2 * (size_t)NGUO_NOALLOC_SORT_MAX + 256u * ((size_t)NGUO_NOALLOC_SORT_MAX + SMALL_ITEMS * BLOCK - 1) / (SMALL_ITEMS * BLOCK) +
+    (256u * (((size_t)NGUO_NOALLOC_SORT_MAX + SMALL_ITEMS * BLOCK - 1) / (SMALL_ITEMS * BLOCK)) + SCAN_TILE - 1) / SCAN_TILE + 1;
+static __device__ __align__(256) uint32_t scratch[NOALLOC_WORDS];
+constexpr int NOALLOC_DEVICES = 64;
+std::mutex g_scratch_mu;
+uint32_t *g_scratch_ptr[NOALLOC_DEVICES] = {};
+const std::vector<int> kSortShifts{0, 8, 16, 24};  // built once, at load
+
+// This device's copy of `scratch` (the device must be current). Looked up once per device.
+uint32_t *scratch_for(int device) {
+  NGUO_REQUIRE(device >= 0 && device < NOALLOC_DEVICES, NGUO_E_UNSUPPORTED, "allocation-free sort: device index beyond 63");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    NGUO_REQUIRE(e == hipSuccess && ptr, NGUO_E_DEVICE,
+                 std::string("hipGetSymbolAddress of the sort scratch failed: ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = (uint32_t *)ptr;
+  }
+  return g_scratch_ptr[device];
+}
+}  // namespace
+
+namespace nguo {
+void noalloc_scratch_init(int device) { (void)scratch_for(device); }
+}  // namespace nguo
+
 extern "C" {
+
+int nguo_lbvh_sort_pairs_noalloc(nguo_ctx *ctx, uint32_t n, uint32_t *keys_dev, uint32_t *vals_dev) {
+  return guarded([&]() -> int {
+    NGUO_REQUIRE(ctx, NGUO_E_INVALID, "nguo_lbvh_sort_pairs_noalloc: null context");
+    NGUO_REQUIRE(n <= NGUO_NOALLOC_SORT_MAX, NGUO_E_CAPACITY,
+                 "nguo_lbvh_sort_pairs_noalloc: " + std::to_string(n) + " pairs exceed the scratch maximum of " +
+                     std::to_string(NGUO_NOALLOC_SORT_MAX));
+    if (n <= 1) return NGUO_OK;
+    NGUO_REQUIRE(keys_dev && vals_dev, NGUO_E_INVALID, "nguo_lbvh_sort_pairs_noalloc: null device buffer");
+    static_assert(NOALLOC_WORDS >= radix_scratch_words(NGUO_NOALLOC_SORT_MAX), "scratch below the maximum input's need");
+    NGUO_REQUIRE(radix_scratch_words(n) <= NOALLOC_WORDS, NGUO_E_CAPACITY, "nguo_lbvh_sort_pairs_noalloc: scratch too small");
+    NGUO_HIP(hipSetDevice(ctx->device));
+    uint32_t *buf = scratch_for(ctx->device);
+    // every scratch word a pass reads it has written earlier in the same call (counters by the
+    // histogram launch, tile sums by the scan, the ping-pong halves by the scatter), so the
+    // last call's contents need no clearing
+    radix_sort_u32_scratch(n, keys_dev, vals_dev, buf, kSortShifts, ctx->stream);
+    NGUO_HIP(hipStreamSynchronize(ctx->stream));
+    return NGUO_OK;
+  });
+}
 
 int nguo_lbvh_morton(nguo_ctx *ctx, uint32_t nv, const float *pos, uint32_t nt, const uint32_t *tris, uint32_t *codes) {
   return guarded([&]() -> int {

@@ -105,6 +105,13 @@ int nguo_ctx_create(int device, nguo_ctx **out) {
     NGUO_HIP(hipSetDevice(device));
     NGUO_HIP(hipGetDeviceProperties(&c->prop, device));
     NGUO_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    try {
+      noalloc_scratch_init(device);  // this device's copy of the allocation-free sort's scratch
+    } catch (...) {
+      (void)hipStreamDestroy(c->stream);
+      delete c;
+      throw;
+    }
     *out = c;
     return NGUO_OK;
   });

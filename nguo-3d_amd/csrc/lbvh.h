@@ -22,6 +22,24 @@ constexpr uint32_t TRAV_WIDTH = 4;
 constexpr uint32_t TRAV_ENTRY = 3;  // float4 per record entry: box and refs, then vertex ids
 constexpr uint32_t TRAV_NONE = 0xffffffffu;   // unused record entry (primitive ids are below 2^31)
 
+// Radix sort tiles: items per thread (tiles of 1024 / 4096 items), the input size from which
+// the large tile is used, and the counters one workgroup of the large scan covers.
+constexpr uint32_t SMALL_ITEMS = 4, LARGE_ITEMS = 16;
+constexpr uint32_t LARGE_SORT = 1u << 20;
+constexpr uint32_t SCAN_TILE = 16 * BLOCK;
+
+constexpr uint32_t radix_items(uint32_t n) { return n >= LARGE_SORT ? LARGE_ITEMS : SMALL_ITEMS; }
+// digit counters of one pass: 256 per tile
+constexpr size_t radix_counters(uint32_t n) { return 256u * (((size_t)n + radix_items(n) * BLOCK - 1) / (radix_items(n) * BLOCK)); }
+// 32-bit words of scratch radix_sort_u32_scratch needs for n items: the key and value
+// ping-pong halves, the digit counters and the large scan's tile sums.
+constexpr size_t radix_scratch_words(uint32_t n) {
+  return 2 * (size_t)n + radix_counters(n) + (radix_counters(n) + SCAN_TILE - 1) / SCAN_TILE + 1;
+}
+// The sort of RadixSort::sort_u32 with every intermediate buffer carved from `scratch`
+// (radix_scratch_words(n) words); allocates nothing.
+void radix_sort_u32_scratch(uint32_t n, uint32_t *keys, uint32_t *vals, uint32_t *scratch, const std::vector<int> &shifts, hipStream_t s);
+
 // Stable ascending sort of (key, value) pairs on the device.
 class RadixSort {
  public:

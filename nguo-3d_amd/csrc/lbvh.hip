@@ -16,8 +16,6 @@ namespace nguo {
 //                    256 in order, 8-ballot wavefront multisplit), stages the tile in LDS in
 //                    (digit, rank) order, then writes it out so that consecutive threads write
 //                    consecutive slots of one digit's run.
-constexpr uint32_t SMALL_ITEMS = 4, LARGE_ITEMS = 16;  // items per thread: tiles of 1024 / 4096
-constexpr uint32_t LARGE_SORT = 1u << 20;              // inputs from this size use the large tile
 // Scratch for at least this many items from the first sort on: the per-step sorts (stencil
 // and incidence keys) change size every step.
 constexpr uint32_t SORT_MIN_CAP = 1u << 14;
@@ -84,7 +82,6 @@ __global__ __launch_bounds__(BLOCK) void k_scan_excl(uint32_t total, uint32_t *a
 
 // Large scans: each workgroup scans SCAN_TILE counters in place (exclusive, local) and
 // writes its sum; the sums are scanned by one workgroup; each workgroup adds its offset.
-constexpr uint32_t SCAN_TILE = 16 * BLOCK;
 __global__ __launch_bounds__(BLOCK) void k_scan_tiles(uint32_t total, uint32_t *a, uint32_t *sums) {
   __shared__ uint32_t wsum[BLOCK / WAVE];
   const uint32_t base = blockIdx.x * SCAN_TILE;
@@ -220,6 +217,13 @@ static void radix_sort(uint32_t n, K *keys, uint32_t *vals, K *ktmp, uint32_t *v
   sums.reserve(div_up(total, SCAN_TILE) + 1);
   if (items == LARGE_ITEMS) radix_passes<K, LARGE_ITEMS>(n, keys, vals, ktmp, vtmp, hist.p, sums.p, shifts, s);
   else radix_passes<K, SMALL_ITEMS>(n, keys, vals, ktmp, vtmp, hist.p, sums.p, shifts, s);
+}
+
+void radix_sort_u32_scratch(uint32_t n, uint32_t *keys, uint32_t *vals, uint32_t *scratch, const std::vector<int> &shifts, hipStream_t s) {
+  if (n <= 1) return;
+  uint32_t *ktmp = scratch, *vtmp = scratch + n, *hist = vtmp + n, *sums = hist + radix_counters(n);
+  if (radix_items(n) == LARGE_ITEMS) radix_passes<uint32_t, LARGE_ITEMS>(n, keys, vals, ktmp, vtmp, hist, sums, shifts, s);
+  else radix_passes<uint32_t, SMALL_ITEMS>(n, keys, vals, ktmp, vtmp, hist, sums, shifts, s);
 }
 
 void RadixSort::sort_u32(uint32_t n, uint32_t *keys, uint32_t *vals, const std::vector<int> &shifts, hipStream_t s) {

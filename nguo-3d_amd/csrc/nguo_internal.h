@@ -104,6 +104,10 @@ template <class T> struct DevBuf {
 
 inline unsigned div_up(size_t a, unsigned b) { return (unsigned)((a + b - 1) / b); }
 
+// Looks up this device's copy of the allocation-free sort's scratch array (once per device,
+// with that device current; nguo_ctx_create calls it).
+void noalloc_scratch_init(int device);
+
 }  // namespace nguo
 
 // Device context: one HIP device and the (default-priority) stream all work is queued on.

@@ -31,6 +31,7 @@ EXPORTED = [
     "nguo_step_bench_spmv", "nguo_step_bench_precond", "nguo_bench_spmv_grid", "nguo_ls_precond_apply",
     "nguo_batch_create", "nguo_batch_destroy", "nguo_batch_advance", "nguo_batch_instance",
     "nguo_step_set_rest_scale", "nguo_step_adjoint", "nguo_batch_set_solve", "nguo_bench_sort",
+    "nguo_lbvh_sort_pairs_noalloc",
 ]
 
 
@@ -129,6 +130,7 @@ def load():
         lib.nguo_step_collider_sides.argtypes = [vp, P(C.c_float), P(C.c_float)]
         lib.nguo_lbvh_morton.argtypes = [vp, C.c_uint32, P(C.c_float), C.c_uint32, P(C.c_uint32), P(C.c_uint32)]
         lib.nguo_lbvh_sort_pairs.argtypes = [vp, C.c_uint32, P(C.c_uint32), P(C.c_uint32)]
+        lib.nguo_lbvh_sort_pairs_noalloc.argtypes = [vp, C.c_uint32, vp, vp]
         lib.nguo_lbvh_karras.argtypes = [vp, C.c_uint32, P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
         lib.nguo_lbvh_aabb.argtypes = [vp, C.c_uint32, P(C.c_uint32), P(C.c_uint32), P(C.c_float), P(C.c_float)]
         lib.nguo_batch_create.argtypes = [vp, C.c_uint32, P(SceneDesc), P(vp)]
@@ -258,6 +260,15 @@ def lbvh_sort_pairs(ctx, keys, vals):
     vals = np.ascontiguousarray(vals, dtype=np.uint32).copy()
     check(load().nguo_lbvh_sort_pairs(ctx.h, len(keys), uptr(keys), uptr(vals)))
     return keys, vals
+
+
+NOALLOC_SORT_MAX = 1 << 20  # NGUO_NOALLOC_SORT_MAX
+
+
+def lbvh_sort_pairs_noalloc(ctx, n, keys_dev, vals_dev):
+    """Sorts n (u32 key, u32 value) pairs in place in device memory (addresses as ints),
+    allocating nothing (nguo_lbvh_sort_pairs_noalloc)."""
+    check(load().nguo_lbvh_sort_pairs_noalloc(ctx.h, int(n), C.c_void_p(keys_dev), C.c_void_p(vals_dev)))
 
 
 def lbvh_karras(ctx, codes):
