@@ -179,7 +179,11 @@ int nguo_step_set_state(nguo_step *st, const float *positions, const float *velo
 int nguo_step_get_metrics(nguo_step *st, nguo_step_metrics *out);
 /* Positions after Newton iteration k of the last advanced step (the oracle's per-iteration
  * parity point); *count receives the number of iterations taken, toi their CCD fraction.
- * out == NULL only queries the count. */
+ * out == NULL only queries the count (and toi of iteration k). Positions are kept only for steps advanced with
+ * tracing on (nguo_step_set_trace(st, 1); off by default: the copy is diagnostics and stays
+ * out of the production step); asking for them otherwise is NGUO_E_INVALID. No reference
+ * counterpart: the reference logs per-iteration scalars only (implicit.rs:2183-2190). */
+int nguo_step_set_trace(nguo_step *st, int enable);
 int nguo_step_get_newton_trace(nguo_step *st, uint32_t k, float *out, uint32_t *count, float *toi);
 /* The active contact stencils at positions x, as implicit.rs:678-753
  * self_collision_stencils returns them (vertex-triangle sorted by (v, t), then edge-edge
@@ -187,7 +191,10 @@ int nguo_step_get_newton_trace(nguo_step *st, uint32_t k, float *out, uint32_t *
  * Rayleigh stiffness kappa [cap] (implicit.rs:755-795) against the elastic Hessian at x.
  * *count receives the number of stencils; at most cap are written. State is unchanged. */
 int nguo_step_stencils(nguo_step *st, const float *x, uint32_t cap, uint32_t *idx, float *d, float *g, float *kappa, uint32_t *count);
-/* implicit.rs:1431-1476 collider_sides at positions x: -1 inside the static body, +1 else. */
+/* implicit.rs:1431-1476 collider_sides at positions x: -1 inside the static body, +1 else.
+ * Both this and nguo_step_stencils evaluate at any x that keeps the state's pinned positions
+ * (the trees over pinned primitives are built from the state); an x that moves a pinned
+ * vertex is NGUO_E_INVALID (set the state first). */
 int nguo_step_collider_sides(nguo_step *st, const float *x, float *side);
 
 /* ---- drape adjoint (the differentiation form, C4). No reference counterpart: the reference

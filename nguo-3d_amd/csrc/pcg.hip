@@ -805,7 +805,9 @@ PcgStats Pcg::solve(const SellMatrix &M, const ContactView &C, const float *b, f
       if (h_flag_[4 * sl]) { fl = h_flag_ + 4 * sl; break; }
       // the cap sets a state within the chunk that reaches it: a flag still clear two chunks
       // later means the host slot is not being written
-      NGUO_REQUIRE(j <= sweep_cap + 2 * CHUNK_MAX + 2, NGUO_E_DEVICE, "PCG: the convergence flag did not reach the host");
+      // (in 64 bits: a cap near UINT32_MAX, "no cap", must not wrap the bound)
+      NGUO_REQUIRE((uint64_t)j <= (uint64_t)sweep_cap + 2ull * CHUNK_MAX + 2ull, NGUO_E_DEVICE,
+                   "PCG: the convergence flag did not reach the host");
     }
     st.iters += fl[1];
     st.sweeps += 1;

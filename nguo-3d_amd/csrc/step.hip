@@ -1054,6 +1054,7 @@ struct nguo_step {
   uint32_t V = 0, NF = 0, F = 0, E = 0;
   std::vector<uint32_t> h_vdof, h_free, h_tris, h_edges;
   std::vector<float> h_im;
+  std::vector<float> h_pin_x;  // positions of the pinned vertices (xyz per pinned vertex, in vertex order)
   std::vector<uint8_t> h_static;
   bool has_collider = false;
   float char_len = 1.0f;
@@ -1087,7 +1088,9 @@ struct nguo_step {
   uint32_t *hu = nullptr;
   unsigned long long *hl = nullptr;
   nguo_step_metrics metrics{};
-  DevBuf<float> trace;  // x after every Newton iteration of the last step
+  DevBuf<float> trace;  // x after every Newton iteration of the last step (when trace_on)
+  bool trace_on = false;
+  uint32_t trace_count = 0;  // iterates held in trace
   DevBuf<uint32_t> defer_count, defer_pairs;  // CCD pairs handed to k_ccd_deferred in a pass
   // drape adjoint: start of the last step (its friction reference), adjoint vector and rhs
   uint32_t n_shell_faces = 0;
@@ -1391,6 +1394,23 @@ ContactView contact_view(nguo_step *st) {
   return C;
 }
 
+// The trees over the pinned primitives are built from the state's pinned positions, so entry
+// points that evaluate at caller-given positions require x to keep them.
+void record_pinned(nguo_step *st, const float *x) {
+  st->h_pin_x.clear();
+  for (uint32_t i = 0; i < st->V; ++i)
+    if (!(st->h_im[i] > 0.0f)) st->h_pin_x.insert(st->h_pin_x.end(), x + 3 * (size_t)i, x + 3 * (size_t)i + 3);
+}
+void require_pinned(const nguo_step *st, const float *x, const char *who) {
+  size_t k = 0;
+  for (uint32_t i = 0; i < st->V; ++i) {
+    if (st->h_im[i] > 0.0f) continue;
+    for (int c = 0; c < 3; ++c, ++k)
+      NGUO_REQUIRE(x[3 * (size_t)i + c] == st->h_pin_x[k], NGUO_E_INVALID,
+                   std::string(who) + ": x moves pinned vertex " + std::to_string(i) + " (set the state first)");
+  }
+}
+
 void ensure_contact_cap(nguo_step *st, uint32_t want) {
   if (want <= st->cap) return;
   uint32_t c = std::max<uint32_t>(want, 1u << 14);
@@ -1595,6 +1615,7 @@ void advance_one(nguo_step *st) {
                      f3{st->prm.gravity[0], st->prm.gravity[1], st->prm.gravity[2]}, st->xt.p, st->x.p);
   uint32_t cap_it = std::max(st->prm.newton_iters, 1u), taken = 0;
   st->trace_toi.clear();
+  st->trace_count = 0;
   bool final_step = false;
   double max_res = 0;
   uint32_t max_it = 0, sum_it = 0;
@@ -1618,10 +1639,13 @@ void advance_one(nguo_step *st) {
     float t = ccd_max_step(st, max_norm);
     last_t = t;
     hipLaunchKernelGGL(k_apply, dim3(GV), dim3(BLOCK), 0, s, V, st->x.p, (const float *)st->dx.p, t, st->part.p);
-    st->trace.reserve(3 * (size_t)V * cap_it);
-    NGUO_HIP(hipMemcpyAsync(st->trace.p + 3 * (size_t)V * it, st->x.p, 3 * (size_t)V * sizeof(float), hipMemcpyDeviceToDevice, s));
+    if (st->trace_on) {  // per-iterate positions for parity checks (off in the production step)
+      st->trace.reserve(3 * (size_t)V * cap_it);
+      NGUO_HIP(hipMemcpyAsync(st->trace.p + 3 * (size_t)V * it, st->x.p, 3 * (size_t)V * sizeof(float), hipMemcpyDeviceToDevice, s));
+    }
     st->trace_toi.resize(it + 1);
     st->trace_toi[it] = t;
+    if (st->trace_on) st->trace_count = it + 1;
     float max_step = reduce_max(st, GV);
     if (max_step < 1.0e-7f) break;
     if (final_step) break;
@@ -1708,6 +1732,7 @@ int nguo_step_create(nguo_ctx *ctx, const nguo_scene_desc *sc, nguo_step **out) 
     for (uint32_t i = 0; i < V; ++i)
       if (st->h_im[i] > 0.0f) { st->h_vdof[i] = (uint32_t)st->h_free.size(); st->h_free.push_back(i); }
     st->NF = (uint32_t)st->h_free.size();
+    record_pinned(st.get(), sc->positions);
     st->F = sc->n_tris;
     st->h_tris.assign(sc->tris, sc->tris + 3 * (size_t)sc->n_tris);
     // Params
@@ -2041,10 +2066,17 @@ int nguo_step_set_state(nguo_step *st, const float *positions, const float *velo
     st->x.upload(positions, 3 * (size_t)st->V, st->s);
     if (velocities) st->v.upload(velocities, 3 * (size_t)st->V, st->s);
     build_static_trees(st, st->s);  // the state names the pinned vertices' positions too
+    record_pinned(st, positions);
     st->adj_valid = false;
     NGUO_HIP(hipStreamSynchronize(st->s));
     return NGUO_OK;
   });
+}
+
+int nguo_step_set_trace(nguo_step *st, int enable) {
+  if (!st) return fail(NGUO_E_INVALID, "nguo_step_set_trace: null handle");
+  st->trace_on = enable != 0;
+  return NGUO_OK;
 }
 
 int nguo_step_get_newton_trace(nguo_step *st, uint32_t k, float *out, uint32_t *count, float *toi) {
@@ -2052,8 +2084,10 @@ int nguo_step_get_newton_trace(nguo_step *st, uint32_t k, float *out, uint32_t *
     NGUO_REQUIRE(st, NGUO_E_INVALID, "nguo_step_get_newton_trace: null handle");
     uint32_t n = (uint32_t)st->trace_toi.size();
     if (count) *count = n;
+    if (toi && k < n) *toi = st->trace_toi[k];
     if (!out) return NGUO_OK;
     NGUO_REQUIRE(k < n, NGUO_E_INVALID, "nguo_step_get_newton_trace: iteration index out of range");
+    NGUO_REQUIRE(k < st->trace_count, NGUO_E_INVALID, "nguo_step_get_newton_trace: positions were not traced (nguo_step_set_trace)");
     NGUO_HIP(hipMemcpyAsync(out, st->trace.p + 3 * (size_t)st->V * k, 3 * (size_t)st->V * sizeof(float), hipMemcpyDeviceToHost, st->s));
     NGUO_HIP(hipStreamSynchronize(st->s));
     if (toi) *toi = st->trace_toi[k];
@@ -2064,6 +2098,7 @@ int nguo_step_get_newton_trace(nguo_step *st, uint32_t k, float *out, uint32_t *
 int nguo_step_stencils(nguo_step *st, const float *x, uint32_t cap, uint32_t *idx, float *d, float *g, float *kappa, uint32_t *count) {
   return guarded([&]() -> int {
     NGUO_REQUIRE(st && x && count, NGUO_E_INVALID, "nguo_step_stencils: null argument");
+    require_pinned(st, x, "nguo_step_stencils");
     hipStream_t s = st->s;
     st->x.upload(x, 3 * (size_t)st->V, s);
     run_elements(st->act, st->P, st->x.p, s);
@@ -2086,6 +2121,7 @@ int nguo_step_stencils(nguo_step *st, const float *x, uint32_t cap, uint32_t *id
 int nguo_step_collider_sides(nguo_step *st, const float *x, float *side) {
   return guarded([&]() -> int {
     NGUO_REQUIRE(st && x && side, NGUO_E_INVALID, "nguo_step_collider_sides: null argument");
+    require_pinned(st, x, "nguo_step_collider_sides");
     hipStream_t s = st->s;
     if (!st->has_collider) {
       std::fill(side, side + st->V, 1.0f);

@@ -31,7 +31,7 @@ EXPORTED = [
     "nguo_step_bench_spmv", "nguo_step_bench_precond", "nguo_bench_spmv_grid", "nguo_ls_precond_apply",
     "nguo_batch_create", "nguo_batch_destroy", "nguo_batch_advance", "nguo_batch_instance",
     "nguo_step_set_rest_scale", "nguo_step_adjoint", "nguo_batch_set_solve", "nguo_bench_sort",
-    "nguo_lbvh_sort_pairs_noalloc",
+    "nguo_step_set_trace", "nguo_lbvh_sort_pairs_noalloc",
 ]
 
 
@@ -125,6 +125,7 @@ def load():
         lib.nguo_step_set_state.argtypes = [vp, P(C.c_float), P(C.c_float)]
         lib.nguo_step_get_metrics.argtypes = [vp, P(StepMetrics)]
         lib.nguo_step_get_newton_trace.argtypes = [vp, C.c_uint32, P(C.c_float), P(C.c_uint32), P(C.c_float)]
+        lib.nguo_step_set_trace.argtypes = [vp, C.c_int]
         lib.nguo_step_stencils.argtypes = [vp, P(C.c_float), C.c_uint32, P(C.c_uint32), P(C.c_float), P(C.c_float),
                                            P(C.c_float), P(C.c_uint32)]
         lib.nguo_step_collider_sides.argtypes = [vp, P(C.c_float), P(C.c_float)]
@@ -355,11 +356,17 @@ def drape_panels(scene, n_around=8, n_height=8, axes=None):
 class Stepper:
     """The whole implicit step (GpuImplicitSolver / ImplicitSolver::step) on the device."""
 
-    def __init__(self, ctx, scene):
+    def __init__(self, ctx, scene, trace=False):
         self.h = C.c_void_p()
         self.n = scene.n_verts
         self.nfaces = scene.desc.n_shell_faces
         check(load().nguo_step_create(ctx.h, C.byref(scene.desc), C.byref(self.h)))
+        if trace:
+            self.set_trace(True)
+
+    def set_trace(self, on):
+        """Keep the positions after every Newton iteration of the following steps."""
+        check(load().nguo_step_set_trace(self.h, 1 if on else 0))
 
     @classmethod
     def from_handle(cls, handle, scene):
@@ -394,16 +401,20 @@ class Stepper:
         check(load().nguo_step_get_metrics(self.h, C.byref(m)))
         return m
 
-    def newton_trace(self):
-        """Positions after every Newton iteration of the last step, and their CCD fractions."""
+    def newton_trace(self, positions=True):
+        """Positions after every Newton iteration of the last step (traced steps only), and
+        their CCD fractions; positions=False returns ([], fractions) for any step."""
         cnt = C.c_uint32()
         check(load().nguo_step_get_newton_trace(self.h, 0, None, C.byref(cnt), None))
         xs, tois = [], []
         for k in range(cnt.value):
-            out = np.zeros((self.n, 3), dtype=np.float32)
             t = C.c_float()
-            check(load().nguo_step_get_newton_trace(self.h, k, fptr(out), None, C.byref(t)))
-            xs.append(out)
+            if positions:
+                out = np.zeros((self.n, 3), dtype=np.float32)
+                check(load().nguo_step_get_newton_trace(self.h, k, fptr(out), None, C.byref(t)))
+                xs.append(out)
+            else:
+                check(load().nguo_step_get_newton_trace(self.h, k, None, None, C.byref(t)))
             tois.append(t.value)
         return xs, tois
 

@@ -14,6 +14,8 @@ struct Handle {
   int fp64;
   std::unique_ptr<Solver<float>> s32;
   std::unique_ptr<Solver<double>> s64;
+  std::unique_ptr<Solver<float>::Frozen> fz32;
+  std::unique_ptr<Solver<double>::Frozen> fz64;
   Csr csr;
   std::vector<float> rhs;
 };
@@ -212,6 +214,114 @@ void oracle_solver_gradient_d(void *p, const double *x, const double *xt, double
   if (h->fp64) go(*h->s64, 0.0); else go(*h->s32, 0.0f);
 }
 
+// ---------------------------------------------------------------- step energy
+}  // extern "C"
+namespace {
+template <class T> std::vector<V3<T>> load_d(const double *x, uint32_t n) {
+  std::vector<V3<T>> v(n);
+  for (uint32_t i = 0; i < n; ++i) v[i] = V3<T>(T(x[3 * i]), T(x[3 * i + 1]), T(x[3 * i + 2]));
+  return v;
+}
+template <class T> typename Solver<T>::Frozen *frozen_of(Handle *h);
+template <> Solver<float>::Frozen *frozen_of<float>(Handle *h) { return h->fz32.get(); }
+template <> Solver<double>::Frozen *frozen_of<double>(Handle *h) { return h->fz64.get(); }
+}  // namespace
+extern "C" {
+
+// Holds the contact quantities of an assembly at x (Solver::Frozen) for the calls below.
+void oracle_solver_freeze(void *p, const double *x, const double *xt, double inv_h2) {
+  auto *h = (Handle *)p;
+  uint32_t n = oracle_solver_nverts(p);
+  if (h->fp64) h->fz64.reset(new Solver<double>::Frozen(h->s64->freeze(load_d<double>(x, n), load_d<double>(xt, n), inv_h2)));
+  else h->fz32.reset(new Solver<float>::Frozen(h->s32->freeze(load_d<float>(x, n), load_d<float>(xt, n), (float)inv_h2)));
+}
+
+// Step energy at x (terms[8]: inertia, shells, hinges, rods, rod bends, barriers, friction,
+// analytic colliders). use_frozen: with the held quantities; else with those of x itself.
+double oracle_solver_energy(void *p, const double *x, const double *xt, double inv_h2, int use_frozen, double *terms) {
+  auto *h = (Handle *)p;
+  uint32_t n = oracle_solver_nverts(p);
+  auto go = [&](auto &s, auto tag) -> double {
+    using T = decltype(tag);
+    auto xv = load_d<T>(x, n), tv = load_d<T>(xt, n);
+    T out[Solver<T>::E_TERMS];
+    T e;
+    if (use_frozen) {
+      auto *f = frozen_of<T>(h);
+      if (!f) return std::nan("");
+      e = s.energy(xv, tv, T(inv_h2), *f, out);
+    } else {
+      e = s.energy(xv, tv, T(inv_h2), s.freeze(xv, tv, T(inv_h2)), out);
+    }
+    if (terms)
+      for (int k = 0; k < Solver<T>::E_TERMS; ++k) terms[k] = (double)out[k];
+    return (double)e;
+  };
+  return h->fp64 ? go(*h->s64, 0.0) : go(*h->s32, 0.0f);
+}
+
+}  // extern "C"
+namespace {
+// Assembly switches of the derivative checks: 1 = the frozen-direction gradient model of the
+// held quantities, 2 = unprojected shell and rod-segment blocks, 4 = no strand bending terms.
+template <class S> struct Switches {
+  S &s;
+  Switches(S &solver, const typename S::Frozen *f, int flags) : s(solver) {
+    s.gn = (flags & 1) ? f : nullptr;
+    s.project_elastic = !(flags & 2);
+    s.with_rod_hinges = !(flags & 4);
+  }
+  ~Switches() { s.gn = nullptr; s.project_elastic = true; s.with_rod_hinges = true; s.keep_blocks = nullptr; }
+};
+}  // namespace
+extern "C" {
+
+// Assembled gradient at x under the switches.
+int oracle_solver_model_gradient(void *p, const double *x, const double *xt, double inv_h2, int flags, double *grad) {
+  auto *h = (Handle *)p;
+  uint32_t n = oracle_solver_nverts(p);
+  auto go = [&](auto &s, auto tag) -> int {
+    using T = decltype(tag);
+    auto *f = frozen_of<T>(h);
+    if ((flags & 1) && !f) return -1;
+    Switches<Solver<T>> sw(s, f, flags);
+    std::vector<float> rhs;
+    std::vector<T> gr;
+    s.assemble(load_d<T>(x, n), load_d<T>(xt, n), T(inv_h2), rhs, nullptr, nullptr, &gr);
+    for (size_t i = 0; i < gr.size(); ++i) grad[i] = (double)gr[i];
+    return 0;
+  };
+  return h->fp64 ? go(*h->s64, 0.0) : go(*h->s32, 0.0f);
+}
+
+// hu = H u with H the assembled Hessian at x under the switches (summed blocks, before the
+// Dirichlet filter and the f32 rounding of the CSR); rows and columns of pinned vertices are
+// left out (their entries of hu are 0).
+int oracle_solver_hess_vec(void *p, const double *x, const double *xt, double inv_h2, int flags, const double *u, double *hu) {
+  auto *h = (Handle *)p;
+  uint32_t n = oracle_solver_nverts(p);
+  auto go = [&](auto &s, auto tag) -> int {
+    using T = decltype(tag);
+    auto *f = frozen_of<T>(h);
+    if ((flags & 1) && !f) return -1;
+    Switches<Solver<T>> sw(s, f, flags);
+    typename Solver<T>::Blocks blk;
+    s.keep_blocks = &blk;
+    std::vector<float> rhs;
+    s.assemble(load_d<T>(x, n), load_d<T>(xt, n), T(inv_h2), rhs);
+    for (uint32_t i = 0; i < 3 * n; ++i) hu[i] = 0.0;
+    for (auto &kv : blk.at) {
+      const uint32_t r = (uint32_t)(kv.first >> 32), c = (uint32_t)(kv.first & 0xffffffffu);
+      if (s.inv_mass[r] <= 0.0f || s.inv_mass[c] <= 0.0f) continue;
+      const auto &v = blk.val[kv.second];
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) hu[3 * r + a] += (double)v[3 * a + b] * u[3 * c + b];
+    }
+    return 0;
+  };
+  return h->fp64 ? go(*h->s64, 0.0) : go(*h->s32, 0.0f);
+}
+
 double oracle_solver_ccd(void *p, const float *x, const float *dx) {
   auto *h = (Handle *)p;
   uint32_t n = oracle_solver_nverts(p);
@@ -253,6 +363,19 @@ void oracle_solver_x_tilde(void *p, float *out) {
     std::vector<V3<T>> xt(s.pos.size());
     for (size_t i = 0; i < s.pos.size(); ++i) xt[i] = s.inv_mass[i] > 0.0f ? s.pos[i] + s.vel[i] * hh + g * (hh * hh) : s.pos[i];
     store(xt, out);
+  };
+  if (h->fp64) go(*h->s64, 0.0); else go(*h->s32, 0.0f);
+}
+
+void oracle_solver_x_tilde_d(void *p, double *out) {
+  auto *h = (Handle *)p;
+  auto go = [&](auto &s, auto tag) {
+    using T = decltype(tag);
+    T hh = T(s.prm.dt);
+    V3<T> g(T(s.prm.gravity[0]), T(s.prm.gravity[1]), T(s.prm.gravity[2]));
+    std::vector<V3<T>> xt(s.pos.size());
+    for (size_t i = 0; i < s.pos.size(); ++i) xt[i] = s.inv_mass[i] > 0.0f ? s.pos[i] + s.vel[i] * hh + g * (hh * hh) : s.pos[i];
+    store_d(xt, out);
   };
   if (h->fp64) go(*h->s64, 0.0); else go(*h->s32, 0.0f);
 }

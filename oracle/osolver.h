@@ -38,6 +38,8 @@ template <class T> struct Stencil {
   T d;
   V3<T> g[4];
   T ghat;
+  uint8_t ee = 0;      // 0: vertex-triangle (idx = v, t0, t1, t2), 1: edge-edge (ea0, ea1, eb0, eb1)
+  uint8_t inside = 0;  // vertex-triangle against a collider facet with the vertex inside (signed gap)
 };
 
 struct CgStats {
@@ -413,6 +415,31 @@ template <class T> class Solver {
   std::vector<T> face_scale;
   std::vector<V> last_xt;  // inertial target of the last step
   bool project_elastic = true;  // PSD-project shell / rod-segment Hessians (false: exact)
+  bool with_rod_hinges = true;  // false: leave the strand bending terms out of the assembly
+
+  // What one Newton iteration holds fixed when it assembles at a point x_ref: the stencil set,
+  // each stencil's Rayleigh stiffness (implicit.rs:755-795, recomputed per assembly but a
+  // constant of it), the friction normal force, contact normal and proximity weights
+  // (contact.rs:613-706, lagged), and for the analytic colliders the same per (vertex,
+  // collider). The step energy at x is the energy with these taken at x itself; its gradient
+  // at x is the assembled gradient. hinge_g / fric_lam / col_lam additionally fix the search
+  // directions of the Gauss-Newton terms: the assembled Hessian is the Jacobian of the
+  // gradient with those directions held (the frozen-direction gradient model below).
+  struct Frozen {
+    std::vector<Stencil<T>> st;
+    std::vector<T> kappa;
+    std::vector<int> kind;                 // barrier kind per stencil
+    std::vector<uint8_t> fric;             // friction term present at x_ref
+    std::vector<T> fric_c;                 // mu * normal force magnitude (-mu b'(d))
+    std::vector<V> fric_n;
+    std::vector<std::array<T, 4>> fric_w;
+    std::vector<T> fric_lam;               // lambda = mu c / max(eps, |P slip|) at x_ref
+    std::vector<std::array<V, 4>> hinge_g;  // dtheta/dx at x_ref
+    std::vector<uint8_t> col_fric;         // per (vertex, collider), vertex-major
+    std::vector<V> col_n;
+    std::vector<T> col_c, col_lam;
+  };
+  const Frozen *gn = nullptr;  // non-null: assemble the frozen-direction gradient model
   ShellElem<T> scaled(const ShellElem<T> &el) const {
     if (face_scale.empty()) return el;
     const T sc = face_scale[el.face];
@@ -675,9 +702,10 @@ template <class T> class Solver {
       Stencil<T> s;
       s.idx[0] = pr.first; s.idx[1] = t[0]; s.idx[2] = t[1]; s.idx[3] = t[2];
       T d;
-      if (is_collider_tri(s.idx))
+      if (is_collider_tri(s.idx)) {
+        s.inside = side[s.idx[0]] < T(0) ? 1 : 0;
         d = signed_point_tri_distance_grad(x[s.idx[0]], x[s.idx[1]], x[s.idx[2]], x[s.idx[3]], side[s.idx[0]] < T(0), s.g);
-      else
+      } else
         d = point_tri_distance_grad(x[s.idx[0]], x[s.idx[1]], x[s.idx[2]], x[s.idx[3]], s.g);
       if (d >= dh) continue;
       s.d = d; s.ghat = dh;
@@ -688,6 +716,7 @@ template <class T> class Solver {
       const Edge &ea = edges[pr.first], &eb = edges[pr.second];
       Stencil<T> s;
       s.idx[0] = ea[0]; s.idx[1] = ea[1]; s.idx[2] = eb[0]; s.idx[3] = eb[1];
+      s.ee = 1;
       T d = segment_segment_distance_grad(x[s.idx[0]], x[s.idx[1]], x[s.idx[2]], x[s.idx[3]], s.g);
       T pair_ghat = T(0.5) * (dh + dh);
       if (d - T(0) >= pair_ghat) continue;
@@ -715,6 +744,8 @@ template <class T> class Solver {
       return it == at.end() ? nullptr : &val[it->second];
     }
   };
+
+  Blocks *keep_blocks = nullptr;  // non-null: the next assembly leaves its summed blocks here
 
   // implicit.rs:755-795 self_contact_rayleigh.
   T self_contact_rayleigh(const Blocks &blk, const Stencil<T> &s) const {
@@ -747,7 +778,9 @@ template <class T> class Solver {
     std::vector<T> grad(3 * n, T(0));
     Blocks blk;
     auto push = [&](uint32_t r, uint32_t c, T v) { blk.push(r, c, v); };
-    std::vector<Stencil<T>> sc = prm.self_collision ? stencils(x) : std::vector<Stencil<T>>{};
+    std::vector<Stencil<T>> sc = gn ? gn->st : (prm.self_collision ? stencils(x) : std::vector<Stencil<T>>{});
+    if (gn)
+      for (auto &s0 : sc) s0.d = stencil_gap(x, s0);  // the gap moves, its direction stays
     // inertia
     for (uint32_t i = 0; i < n; ++i) {
       float im = inv_mass[i];
@@ -776,7 +809,7 @@ template <class T> class Solver {
     }
     // rod bending (implicit.rs:2426-2452)
     for (auto &h : rod_hinges) {
-      if (h.stiff <= T(0)) continue;
+      if (h.stiff <= T(0) || !with_rod_hinges) continue;
       uint32_t v[3] = {h.j, h.i, h.k};
       V g[3];
       T hh[81];
@@ -818,11 +851,13 @@ template <class T> class Solver {
     // hinge bending (implicit.rs:2697-2740)
     T kb = T(prm.bend_stiffness);
     if (kb > T(0)) {
-      for (auto &hg : hinges) {
+      for (size_t hi = 0; hi < hinges.size(); ++hi) {
+        const auto &hg = hinges[hi];
         V p[4] = {x[hg.v[0]], x[hg.v[1]], x[hg.v[2]], x[hg.v[3]]};
         T theta = dihedral_angle(p);
         V g[4];
-        dihedral_grad(p, g);
+        if (gn) for (int a = 0; a < 4; ++a) g[a] = gn->hinge_g[hi][a];
+        else dihedral_grad(p, g);
         T kwt = kb * hg.w;
         if (kwt <= T(0)) continue;
         T gval = theta - hg.rest;
@@ -848,10 +883,13 @@ template <class T> class Solver {
         T mu_i = T(prm.friction);
         V gsum{};
         T hsum[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        for (auto &c : colliders) {
+        for (size_t ci = 0; ci < colliders.size(); ++ci) {
+          const auto &c = colliders[ci];
+          const size_t fi = i * colliders.size() + ci;
           T d;
           V nn;
           collider_distance(c, x[i], d, nn);
+          if (gn) nn = gn->col_n[fi];
           T k = kappa;  // contact_dynamic = 0 -> dynamic_stiffness returns the base kappa
           T bo[3];
           if (!barrier_of(prm.barrier_kind, d, dh, k, bo)) continue;
@@ -860,7 +898,14 @@ template <class T> class Solver {
           T hp = std::fmax(bo[2], T(0));
           for (int r = 0; r < 3; ++r)
             for (int cc = 0; cc < 3; ++cc) hsum[3 * r + cc] += hp * nn[r] * nn[cc];
-          if (mu_i > T(0)) {
+          if (gn) {
+            if (gn->col_fric[fi]) {
+              const V u = x[i] - pos[i], n0 = gn->col_n[fi];
+              gsum += (u - dot(n0, u) * n0) * gn->col_lam[fi];
+              for (int r = 0; r < 3; ++r)
+                for (int cc = 0; cc < 3; ++cc) hsum[3 * r + cc] += gn->col_lam[fi] * ((r == cc ? T(1) : T(0)) - n0[r] * n0[cc]);
+            }
+          } else if (mu_i > T(0)) {
             V fg;
             T fh[9];
             if (friction(f_c, x[i] - pos[i], nn, mu_i, T(prm.friction_eps), fg, fh)) {
@@ -880,10 +925,11 @@ template <class T> class Solver {
       Blocks snap = blk;
       size_t si = 0;
       for (auto &s : sc) {
-        T k = kappa_in ? (*kappa_in)[si] : self_contact_rayleigh(snap, s);
+        T k = gn ? gn->kappa[si] : (kappa_in ? (*kappa_in)[si] : self_contact_rayleigh(snap, s));
+        const size_t fi = si;
         ++si;
         if (kappa_out) kappa_out->push_back(k);
-        int kind = s.d < T(0) ? BARRIER_CUBIC : prm.barrier_kind;
+        int kind = gn ? gn->kind[fi] : (s.d < T(0) ? BARRIER_CUBIC : prm.barrier_kind);
         T bo[3];
         if (!barrier_of(kind, s.d, s.ghat, k, bo)) continue;
         T bg = bo[1], hp = std::fmax(bo[2], T(0));
@@ -897,7 +943,28 @@ template <class T> class Solver {
               }
         }
         T mu_s = T(prm.friction);
-        if (mu_s > T(0)) {
+        if (gn) {
+          if (gn->fric[fi]) {
+            const V n0 = gn->fric_n[fi];
+            const auto &w = gn->fric_w[fi];
+            V slip{};
+            for (int a = 0; a < 4; ++a) slip += (x[s.idx[a]] - pos[s.idx[a]]) * w[a];
+            const T lam = gn->fric_lam[fi];
+            const V fg = (slip - dot(n0, slip) * n0) * lam;
+            for (int a = 0; a < 4; ++a) {
+              for (int c = 0; c < 3; ++c) grad[3 * s.idx[a] + c] += fg[c] * w[a];
+              for (int b = 0; b < 4; ++b) {
+                T wab = w[a] * w[b];
+                if (wab == T(0)) continue;
+                for (int r = 0; r < 3; ++r)
+                  for (int cc = 0; cc < 3; ++cc) {
+                    T vh = wab * lam * ((r == cc ? T(1) : T(0)) - n0[r] * n0[cc]);
+                    if (vh != T(0)) push(3 * s.idx[a] + r, 3 * s.idx[b] + cc, vh);
+                  }
+              }
+            }
+          }
+        } else if (mu_s > T(0)) {
           V dxs[4];
           for (int a = 0; a < 4; ++a) dxs[a] = x[s.idx[a]] - pos[s.idx[a]];
           V fg[4];
@@ -921,6 +988,7 @@ template <class T> class Solver {
     }
     if (stencils_out) *stencils_out = sc;
     if (grad_out) *grad_out = grad;
+    if (keep_blocks) *keep_blocks = blk;
     // rhs = -grad, pinned rows zero; Dirichlet finish (csr.rs:385-416)
     rhs_out.assign(3 * n, 0.0f);
     for (uint32_t i = 0; i < 3 * n; ++i) rhs_out[i] = (float)(-grad[i]);
@@ -966,6 +1034,155 @@ template <class T> class Solver {
       if (l < T(1.0e-9)) { d = -T(c.radius); nrm = V(0, 1, 0); }
       else { d = l - T(c.radius); nrm = v / l; }
     }
+  }
+
+  // The gap of stencil s at positions x (its distance function, signed for a collider facet
+  // with the vertex inside), without the gradient.
+  T stencil_gap(const std::vector<V> &x, const Stencil<T> &s) const {
+    V g[4];
+    const V a = x[s.idx[0]], b = x[s.idx[1]], c = x[s.idx[2]], d = x[s.idx[3]];
+    if (s.ee) return segment_segment_distance_grad(a, b, c, d, g);
+    if (s.inside) return signed_point_tri_distance_grad(a, b, c, d, true, g);
+    return point_tri_distance_grad(a, b, c, d, g);
+  }
+
+  // The quantities an assembly at x_ref holds fixed (see Frozen).
+  Frozen freeze(const std::vector<V> &x, const std::vector<V> &xt, T inv_h2) const {
+    Frozen f;
+    std::vector<float> rhs;
+    (void)assemble(x, xt, inv_h2, rhs, &f.st, &f.kappa);
+    const T eps = std::fmax(T(prm.friction_eps), T(1.0e-12)), mu = T(prm.friction);
+    for (size_t i = 0; i < f.st.size(); ++i) {
+      const Stencil<T> &s = f.st[i];
+      const int kind = s.d < T(0) ? BARRIER_CUBIC : prm.barrier_kind;
+      f.kind.push_back(kind);
+      T bo[3] = {0, 0, 0};
+      const bool act = barrier_of(kind, s.d, s.ghat, f.kappa[i], bo);
+      V dxs[4], fg[4];
+      T w[4] = {0, 0, 0, 0}, fh[9];
+      for (int a = 0; a < 4; ++a) dxs[a] = x[s.idx[a]] - pos[s.idx[a]];
+      const bool fr = act && mu > T(0) && stencil_friction(s.g, dxs, bo[1], mu, eps, fg, w, fh);
+      V n{};
+      T lam = 0;
+      if (fr) {
+        n = s.g[0] / length(s.g[0]);
+        V slip{};
+        for (int a = 0; a < 4; ++a) slip += dxs[a] * w[a];
+        lam = mu * (-bo[1]) / std::fmax(length(slip - dot(n, slip) * n), eps);
+      }
+      f.fric.push_back(fr ? 1 : 0);
+      f.fric_c.push_back(fr ? -mu * bo[1] : T(0));
+      f.fric_n.push_back(n);
+      f.fric_w.push_back({w[0], w[1], w[2], w[3]});
+      f.fric_lam.push_back(lam);
+    }
+    for (auto &hg : hinges) {
+      V p[4] = {x[hg.v[0]], x[hg.v[1]], x[hg.v[2]], x[hg.v[3]]}, g[4];
+      dihedral_grad(p, g);
+      f.hinge_g.push_back({g[0], g[1], g[2], g[3]});
+    }
+    const size_t nc = colliders.size();
+    f.col_fric.assign(x.size() * nc, 0);
+    f.col_n.assign(x.size() * nc, V{});
+    f.col_c.assign(x.size() * nc, T(0));
+    f.col_lam.assign(x.size() * nc, T(0));
+    for (uint32_t i = 0; i < x.size(); ++i) {
+      if (inv_mass[i] <= 0.0f) continue;
+      for (size_t ci = 0; ci < nc; ++ci) {
+        T d, bo[3];
+        V nn, fg;
+        T fh[9];
+        collider_distance(colliders[ci], x[i], d, nn);
+        const size_t fi = i * nc + ci;
+        f.col_n[fi] = nn;
+        if (!barrier_of(prm.barrier_kind, d, dhat(), T(prm.contact_kappa), bo)) continue;
+        const V u = x[i] - pos[i];
+        if (mu > T(0) && friction(nn * bo[1], u, nn, mu, eps, fg, fh)) {
+          f.col_fric[fi] = 1;
+          f.col_c[fi] = -mu * bo[1];
+          f.col_lam[fi] = mu * (-bo[1]) / std::fmax(length(u - dot(nn, u) * nn), eps);
+        }
+      }
+    }
+    return f;
+  }
+
+  // Terms of the step energy (the incremental potential each Newton iteration decreases).
+  enum EnergyTerm { E_INERTIA, E_SHELL, E_HINGE, E_ROD, E_ROD_BEND, E_BARRIER, E_FRICTION, E_COLLIDER, E_TERMS };
+
+  // The smoothed Coulomb potential whose gradient is contact.rs:613-643 friction with the
+  // normal force, normal and weights held: c f0(|P u|), f0(s) = s^2 / (2 eps) up to eps, then
+  // s - eps / 2 (c = mu times the normal force).
+  static T friction_potential(T c, V n, V u, T eps) {
+    const T s = length(u - dot(n, u) * n);
+    return c * (s <= eps ? s * s / (T(2) * eps) : s - T(0.5) * eps);
+  }
+
+  // Step energy at x towards the inertial target xt (friction against the step's start pos),
+  // with the contact quantities of `f`; out[E_TERMS] receives the terms. Every term is the one
+  // whose gradient the assembly adds: inertia 1/2 m/h^2 |x - xt|^2 (implicit.rs:2392-2405),
+  // rod stretch 1/2 k (l - rest)^2 (implicit.rs:3287-3308, the rod tests' energy), strand
+  // bending 1/2 k (theta - rest)^2 (rod_bend.rs:1-20), shells area psi(sigma)
+  // (shell_model.rs:112-127), hinges 1/2 k_b w (theta - rest)^2 (bending.rs:105-123), the
+  // barriers (contact.rs:580-591, 1014-1025) and the friction potentials.
+  T energy(const std::vector<V> &x, const std::vector<V> &xt, T inv_h2, const Frozen &f, T out[E_TERMS]) const {
+    for (int k = 0; k < E_TERMS; ++k) out[k] = T(0);
+    const uint32_t n = (uint32_t)x.size();
+    for (uint32_t i = 0; i < n; ++i) {
+      float im = inv_mass[i];
+      if (im <= 0.0f) continue;
+      out[E_INERTIA] += T(0.5) * (T(1) / T(im)) * inv_h2 * length_sq(x[i] - xt[i]);
+    }
+    for (auto &rd : rods) {
+      const T e = length(x[rd.a] - x[rd.b]) - rd.rest;
+      out[E_ROD] += T(0.5) * rd.k * e * e;
+    }
+    for (auto &h : rod_hinges) {
+      if (h.stiff <= T(0) || !with_rod_hinges) continue;
+      const T w = strand_angle(x[h.j], x[h.i], x[h.k]) - h.rest;
+      out[E_ROD_BEND] += T(0.5) * h.stiff * w * w;
+    }
+    if (!shells.empty() && (prm.shell_mu > 0.0f || prm.shell_lambda > 0.0f))
+      for (auto &el0 : shells) {
+        const ShellElem<T> el = scaled(el0);
+        V p[3] = {x[el.v[0]], x[el.v[1]], x[el.v[2]]};
+        out[E_SHELL] += shell_energy(p, el, prm.shell_model, T(prm.shell_mu), T(prm.shell_lambda));
+      }
+    const T kb = T(prm.bend_stiffness);
+    if (kb > T(0))
+      for (auto &hg : hinges) {
+        const T kwt = kb * hg.w;
+        if (kwt <= T(0)) continue;
+        V p[4] = {x[hg.v[0]], x[hg.v[1]], x[hg.v[2]], x[hg.v[3]]};
+        const T e = dihedral_angle(p) - hg.rest;
+        out[E_HINGE] += T(0.5) * kwt * e * e;
+      }
+    const T eps = std::fmax(T(prm.friction_eps), T(1.0e-12));
+    const size_t nc = colliders.size();
+    for (uint32_t i = 0; i < n && nc; ++i) {
+      if (inv_mass[i] <= 0.0f) continue;
+      for (size_t ci = 0; ci < nc; ++ci) {
+        T d, bo[3];
+        V nn;
+        collider_distance(colliders[ci], x[i], d, nn);
+        if (barrier_of(prm.barrier_kind, d, dhat(), T(prm.contact_kappa), bo)) out[E_COLLIDER] += bo[0];
+        const size_t fi = i * nc + ci;
+        if (f.col_fric[fi]) out[E_COLLIDER] += friction_potential(f.col_c[fi], f.col_n[fi], x[i] - pos[i], eps);
+      }
+    }
+    for (size_t si = 0; si < f.st.size(); ++si) {
+      const Stencil<T> &s = f.st[si];
+      T bo[3];
+      if (barrier_of(f.kind[si], stencil_gap(x, s), s.ghat, f.kappa[si], bo)) out[E_BARRIER] += bo[0];
+      if (f.fric[si]) {
+        V slip{};
+        for (int a = 0; a < 4; ++a) slip += (x[s.idx[a]] - pos[s.idx[a]]) * f.fric_w[si][a];
+        out[E_FRICTION] += friction_potential(f.fric_c[si], f.fric_n[si], slip, eps);
+      }
+    }
+    T tot = 0;
+    for (int k = 0; k < E_TERMS; ++k) tot += out[k];
+    return tot;
   }
 
   // ---------------------------------------------------------------- CCD

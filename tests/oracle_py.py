@@ -89,6 +89,14 @@ def load():
         lib.oracle_strand.restype = C.c_double
         lib.oracle_strand.argtypes = [C.c_int, D, C.c_double, D, D]
         lib.oracle_rod_stretch.argtypes = [C.c_int, D, C.c_double, C.c_double, D, D]
+        lib.oracle_solver_freeze.argtypes = [vp, D, D, C.c_double]
+        lib.oracle_solver_energy.restype = C.c_double
+        lib.oracle_solver_energy.argtypes = [vp, D, D, C.c_double, C.c_int, D]
+        lib.oracle_solver_model_gradient.restype = C.c_int
+        lib.oracle_solver_model_gradient.argtypes = [vp, D, D, C.c_double, C.c_int, D]
+        lib.oracle_solver_hess_vec.restype = C.c_int
+        lib.oracle_solver_hess_vec.argtypes = [vp, D, D, C.c_double, C.c_int, D, D]
+        lib.oracle_solver_x_tilde_d.argtypes = [vp, D]
         lib.oracle_morton_codes.argtypes = [C.c_uint32, F, C.c_uint32, U, U]
         lib.oracle_sort_pairs.argtypes = [C.c_uint32, U, U]
         lib.oracle_karras.argtypes = [C.c_uint32, U, U, U]
@@ -230,6 +238,35 @@ class OracleSolver:
         out = np.zeros((self.n, 3), np.float32)
         self.lib.oracle_solver_x_tilde(self.h, fp(out))
         return out
+
+    def x_tilde_d(self):
+        out = np.zeros((self.n, 3), np.float64)
+        self.lib.oracle_solver_x_tilde_d(self.h, dp(out))
+        return out
+
+    # Assembly switches of the derivative checks (oracle_solver_model_gradient / hess_vec).
+    GN_MODEL, UNPROJECTED, NO_ROD_BENDS = 1, 2, 4
+    TERMS = ("inertia", "shells", "hinges", "rods", "rod_bends", "barriers", "friction", "colliders")
+
+    def freeze(self, x, xt, inv_h2):
+        """Hold the contact quantities of an assembly at x for energy / model_gradient."""
+        self.lib.oracle_solver_freeze(self.h, dp(d64(x)), dp(d64(xt)), inv_h2)
+
+    def energy(self, x, xt, inv_h2, frozen=False):
+        """(total, terms[8]) of the step energy at x; frozen: with the held contact quantities."""
+        terms = np.zeros(8)
+        e = self.lib.oracle_solver_energy(self.h, dp(d64(x)), dp(d64(xt)), inv_h2, 1 if frozen else 0, dp(terms))
+        return e, terms
+
+    def model_gradient(self, x, xt, inv_h2, flags=0):
+        g = np.zeros(3 * self.n)
+        assert self.lib.oracle_solver_model_gradient(self.h, dp(d64(x)), dp(d64(xt)), inv_h2, flags, dp(g)) == 0
+        return g.reshape(-1, 3)
+
+    def hess_vec(self, x, xt, inv_h2, u, flags=0):
+        hu = np.zeros(3 * self.n)
+        assert self.lib.oracle_solver_hess_vec(self.h, dp(d64(x)), dp(d64(xt)), inv_h2, flags, dp(d64(u).reshape(-1)), dp(hu)) == 0
+        return hu.reshape(-1, 3)
 
     def ccd(self, x, dx):
         x, dx = f32(x), f32(dx)
