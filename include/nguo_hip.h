@@ -197,6 +197,19 @@ int nguo_step_stencils(nguo_step *st, const float *x, uint32_t cap, uint32_t *id
  * vertex is NGUO_E_INVALID (set the state first). */
 int nguo_step_collider_sides(nguo_step *st, const float *x, float *side);
 
+/* Step energy at positions x (the incremental potential the Newton iteration of the last
+ * advanced step decreases: its inertial target and friction reference, contact quantities at
+ * x), evaluated in FP64 from the FP32 positions. terms (may be NULL) receives the
+ * NGUO_ENERGY_TERMS parts: inertia 1/2 m/h^2 |x - x~|^2 (implicit.rs:2392-2405), shells
+ * (shell_model.rs:112-127), hinges (bending.rs:105-123), rod stretch (implicit.rs:3287-3308),
+ * strand bends (rod_bend.rs:1-20), stencil barriers (contact.rs:580-591, 1014-1025), stencil
+ * friction potentials (contact.rs:613-706 integrated with the normal force held), analytic
+ * colliders (barrier + friction, contact.rs:1082-1125); total (may be NULL) their sum. x
+ * must keep the state's pinned positions. No reference counterpart: the reference logs only
+ * the kinetic energy (implicit.rs:2183-2190, state.rs:84-92). State is unchanged. */
+#define NGUO_ENERGY_TERMS 8
+int nguo_step_energy(nguo_step *st, const float *x, double *terms, double *total);
+
 /* ---- drape adjoint (the differentiation form, C4). No reference counterpart: the reference
  * steps a fixed rest shape (engine-core/src/shell_model.rs:309-349 build_shells) and has no
  * gradient. Parameters are per shell face rest-shape scales s_f: face f's rest shape is

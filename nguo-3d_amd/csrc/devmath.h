@@ -101,14 +101,15 @@ template <class T> __device__ inline T point_triangle_distance(v3<T> p, v3<T> a,
   return len(p - closest_point_bary(p, a, b, c, w));
 }
 
-// contact.rs:836-858 point_tri_distance_grad.
-__device__ inline float point_tri_distance_grad(f3 p, f3 a, f3 b, f3 c, f3 g[4]) {
-  float w[3];
-  f3 q = closest_point_bary(p, a, b, c, w);
-  f3 e = p - q;
-  float d = len(e);
-  if (d < 1.0e-12f) { for (int i = 0; i < 4; ++i) g[i] = f3{0, 0, 0}; return d; }
-  f3 n = e / d;
+// contact.rs:836-858 point_tri_distance_grad (FP32 on the step's path; FP64 in the energy
+// evaluation).
+template <class T> __device__ inline T point_tri_distance_grad(v3<T> p, v3<T> a, v3<T> b, v3<T> c, v3<T> g[4]) {
+  T w[3];
+  v3<T> q = closest_point_bary(p, a, b, c, w);
+  v3<T> e = p - q;
+  T d = len(e);
+  if (d < T(1.0e-12)) { for (int i = 0; i < 4; ++i) g[i] = v3<T>{0, 0, 0}; return d; }
+  v3<T> n = e / d;
   g[0] = n; g[1] = (-w[0]) * n; g[2] = (-w[1]) * n; g[3] = (-w[2]) * n;
   return d;
 }
@@ -122,62 +123,62 @@ template <class T> __device__ inline void point_edge_coeff(v3<T> p, v3<T> e0, v3
 }
 
 // contact.rs:238-310 edge_edge_witness.
-__device__ inline void edge_edge_witness(f3 ea0, f3 ea1, f3 eb0, f3 eb1, float out[4]) {
-  f3 cog = 0.25f * (ea0 + ea1 + eb0 + eb1);
-  f3 a0 = ea0 - cog, a1 = ea1 - cog, b0v = eb0 - cog, b1v = eb1 - cog;
-  auto dist2 = [&](const float c[4]) {
-    f3 x0 = c[0] * a0 + c[1] * a1;
-    f3 x1 = c[2] * b0v + c[3] * b1v;
-    f3 d = x1 - x0;
+template <class T> __device__ inline void edge_edge_witness(v3<T> ea0, v3<T> ea1, v3<T> eb0, v3<T> eb1, T out[4]) {
+  v3<T> cog = T(0.25) * (ea0 + ea1 + eb0 + eb1);
+  v3<T> a0 = ea0 - cog, a1 = ea1 - cog, b0v = eb0 - cog, b1v = eb1 - cog;
+  auto dist2 = [&](const T c[4]) {
+    v3<T> x0 = c[0] * a0 + c[1] * a1;
+    v3<T> x1 = c[2] * b0v + c[3] * b1v;
+    v3<T> d = x1 - x0;
     return dot(d, d);
   };
-  float pe[2];
+  T pe[2];
   point_edge_coeff(ea0, eb0, eb1, pe);
-  float cls[4] = {1.0f, 0.0f, pe[0], pe[1]};
-  f3 r0 = ea1 - ea0, r1 = eb1 - eb0;
-  float a00 = dot(r0, r0), a01 = -dot(r0, r1), a11 = dot(r1, r1);
-  f3 rhs = eb0 - ea0;
-  float b0 = dot(r0, rhs), b1 = -dot(r1, rhs);
-  float det = a00 * a11 - a01 * a01;
-  if (det != 0.0f) {
-    float sp = (a11 * b0 - a01 * b1) / det;
-    float tp = (a00 * b1 - a01 * b0) / det;
-    float direct[4] = {1.0f - sp, sp, 1.0f - tp, tp};
+  T cls[4] = {T(1), T(0), pe[0], pe[1]};
+  v3<T> r0 = ea1 - ea0, r1 = eb1 - eb0;
+  T a00 = dot(r0, r0), a01 = -dot(r0, r1), a11 = dot(r1, r1);
+  v3<T> rhs = eb0 - ea0;
+  T b0 = dot(r0, rhs), b1 = -dot(r1, rhs);
+  T det = a00 * a11 - a01 * a01;
+  if (det != T(0)) {
+    T sp = (a11 * b0 - a01 * b1) / det;
+    T tp = (a00 * b1 - a01 * b0) / det;
+    T direct[4] = {T(1) - sp, sp, T(1) - tp, tp};
     if (dist2(direct) < dist2(cls))
       for (int i = 0; i < 4; ++i) cls[i] = direct[i];
   }
   bool inrange = true;
-  for (int i = 0; i < 4; ++i) inrange = inrange && (cls[i] >= 0.0f && cls[i] <= 1.0f);
+  for (int i = 0; i < 4; ++i) inrange = inrange && (cls[i] >= T(0) && cls[i] <= T(1));
   if (inrange) { for (int i = 0; i < 4; ++i) out[i] = cls[i]; return; }
-  auto snap = [](float c[2]) {
-    if (c[0] < 0.0f) { c[0] = 0.0f; c[1] = 1.0f; }
-    else if (c[0] > 1.0f) { c[0] = 1.0f; c[1] = 0.0f; }
+  auto snap = [](T c[2]) {
+    if (c[0] < T(0)) { c[0] = T(0); c[1] = T(1); }
+    else if (c[0] > T(1)) { c[0] = T(1); c[1] = T(0); }
   };
-  float c1[2], c2[2], c3[2], c4[2];
+  T c1[2], c2[2], c3[2], c4[2];
   point_edge_coeff(ea0, eb0, eb1, c1); snap(c1);
   point_edge_coeff(ea1, eb0, eb1, c2); snap(c2);
   point_edge_coeff(eb0, ea0, ea1, c3); snap(c3);
   point_edge_coeff(eb1, ea0, ea1, c4); snap(c4);
-  float types[4][4] = {{1.0f, 0.0f, c1[0], c1[1]}, {0.0f, 1.0f, c2[0], c2[1]}, {c3[0], c3[1], 1.0f, 0.0f}, {c4[0], c4[1], 0.0f, 1.0f}};
+  T types[4][4] = {{T(1), T(0), c1[0], c1[1]}, {T(0), T(1), c2[0], c2[1]}, {c3[0], c3[1], T(1), T(0)}, {c4[0], c4[1], T(0), T(1)}};
   int best = 0;
-  float best_d = INFINITY;
+  T best_d = T(INFINITY);
   for (int k = 0; k < 4; ++k) {
-    float d = dist2(types[k]);
+    T d = dist2(types[k]);
     if (d < best_d) { best_d = d; best = k; }
   }
   for (int i = 0; i < 4; ++i) out[i] = types[best][i];
 }
 
 // contact.rs:489-512 segment_segment_distance_grad.
-__device__ inline float segment_segment_distance_grad(f3 p1, f3 q1, f3 p2, f3 q2, f3 g[4]) {
-  float w[4];
+template <class T> __device__ inline T segment_segment_distance_grad(v3<T> p1, v3<T> q1, v3<T> p2, v3<T> q2, v3<T> g[4]) {
+  T w[4];
   edge_edge_witness(p1, q1, p2, q2, w);
-  f3 ca = w[0] * p1 + w[1] * q1;
-  f3 cb = w[2] * p2 + w[3] * q2;
-  f3 e = ca - cb;
-  float d = len(e);
-  if (d < 1.0e-12f) { for (int i = 0; i < 4; ++i) g[i] = f3{0, 0, 0}; return d; }
-  f3 n = e / d;
+  v3<T> ca = w[0] * p1 + w[1] * q1;
+  v3<T> cb = w[2] * p2 + w[3] * q2;
+  v3<T> e = ca - cb;
+  T d = len(e);
+  if (d < T(1.0e-12)) { for (int i = 0; i < 4; ++i) g[i] = v3<T>{0, 0, 0}; return d; }
+  v3<T> n = e / d;
   g[0] = w[0] * n; g[1] = w[1] * n; g[2] = (-w[2]) * n; g[3] = (-w[3]) * n;
   return d;
 }
@@ -438,15 +439,16 @@ __device__ inline void shell_element(const f3 p[3], Mat2 dm, float area, int mod
     }
 }
 
-// bending.rs:34-48 dihedral_angle.
-__device__ inline float dihedral_angle(const f3 p[4]) {
-  f3 n1 = cross(p[1] - p[0], p[2] - p[0]);
-  f3 n2 = cross(p[1] - p[2], p[3] - p[2]);
-  f3 e = normalize_or_zero(p[2] - p[1]);
-  f3 n1n = normalize_or_zero(n1), n2n = normalize_or_zero(n2);
-  float sn = dot(cross(n1n, n2n), e);
-  float cs = clampv(dot(n1n, n2n), -1.0f, 1.0f);
-  return atan2f(sn, cs);
+// bending.rs:34-48 dihedral_angle (FP32 for the step; FP64 for the energy evaluation).
+template <class T> __device__ inline T dihedral_angle(const v3<T> p[4]) {
+  v3<T> n1 = cross(p[1] - p[0], p[2] - p[0]);
+  v3<T> n2 = cross(p[1] - p[2], p[3] - p[2]);
+  v3<T> e = normalize_or_zero(p[2] - p[1]);
+  v3<T> n1n = normalize_or_zero(n1), n2n = normalize_or_zero(n2);
+  T sn = dot(cross(n1n, n2n), e);
+  T cs = clampv(dot(n1n, n2n), T(-1), T(1));
+  if constexpr (sizeof(T) == 4) return atan2f(sn, cs);
+  else return atan2(sn, cs);
 }
 
 // bending.rs:78-98 dihedral_grad_analytic (central differences on a degenerate hinge).

@@ -1017,10 +1017,207 @@ __global__ __launch_bounds__(BLOCK) void k_pierce(Topo T, BvhView dyn, BvhView s
 // ================================================================ host side
 using namespace nguo;
 
+// ---------------------------------------------------------------- step energy (diagnostic)
+// The step energy, the incremental potential each Newton iteration decreases, at the current
+// positions: inertia, shells, hinges, rod stretch, strand bending, the barriers and friction
+// potentials of the active stencils and of the analytic colliders. Each term is the one
+// whose gradient the assembly adds (implicit.rs:2392-3103): inertia 1/2 m/h^2 |x - x~|^2,
+// shells area psi(sigma) (shell_model.rs:112-127), hinges 1/2 k_b (theta - theta0)^2
+// (bending.rs:105-123), rods 1/2 k (l - l0)^2 (implicit.rs:3287-3308), strand bends
+// 1/2 k (theta - theta0)^2 (rod_bend.rs:1-20), barriers (contact.rs:580-591, 1014-1025) and the
+// smoothed Coulomb potential c f0(|P u|) whose gradient is contact.rs:613-706. Everything is
+// evaluated in FP64 from the FP32 positions (gaps from the stencils' own witness weights,
+// Rayleigh stiffnesses from the assembled blocks, shells from FP64 rest frames), so the value
+// is that of the FP64 scheme at these positions. It is a diagnostic, run on request only.
+enum : int { EN_INERTIA, EN_SHELL, EN_HINGE, EN_ROD, EN_ROD_BEND, EN_BARRIER, EN_FRICTION, EN_COLLIDER, EN_TERMS };
+
+// Sums each term over the workgroup's wavefronts into acc (all lanes active).
+__device__ inline void energy_commit(const double e[EN_TERMS], double *acc) {
+#pragma unroll
+  for (int k = 0; k < EN_TERMS; ++k) {
+    const double v = wave_sum(e[k]);
+    if ((threadIdx.x & (WAVE - 1)) == 0 && v != 0.0) atomicAdd(acc + k, v);
+  }
+}
+
+// c f0(|P u|), f0(s) = s^2 / (2 eps) up to eps, then s - eps / 2: the potential whose gradient
+// is contact.rs:613-643 friction with its normal force (c = mu times it) and normal held.
+__device__ inline double friction_potential(double c, d3 n, d3 u, double eps) {
+  const double sl = len(u - n * dot(n, u));
+  return c * (sl <= eps ? sl * sl / (2.0 * eps) : sl - 0.5 * eps);
+}
+
+// A stencil's gap and distance gradient in FP64 (the FP64 scheme's own closest points),
+// signed like its FP32 gap d32 (negative for a vertex inside the static body).
+__device__ inline double stencil_geom64(bool vt, const uint32_t id[4], const float *__restrict__ x, float d32, d3 g[4]) {
+  const d3 p0 = to_d(ld3(x, id[0])), p1 = to_d(ld3(x, id[1])), p2 = to_d(ld3(x, id[2])), p3 = to_d(ld3(x, id[3]));
+  double d = vt ? point_tri_distance_grad(p0, p1, p2, p3, g) : segment_segment_distance_grad(p0, p1, p2, p3, g);
+  if (d32 < 0.0f) {
+    d = -d;
+    for (int a = 0; a < 4; ++a) g[a] = -g[a];
+  }
+  return d;
+}
+
+// Inertia and analytic colliders, one thread per vertex.
+__global__ __launch_bounds__(BLOCK) void k_energy_verts(uint32_t V, const float *__restrict__ im, const float *__restrict__ x,
+                                                       const float *__restrict__ xt, const float *__restrict__ xs, double inv_h2, Params P,
+                                                       double *acc) {
+  double e[EN_TERMS] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i < V && im[i] > 0.0f) {
+    const d3 xi = to_d(ld3(x, i)), dv = xi - to_d(ld3(xt, i));
+    e[EN_INERTIA] = 0.5 * (1.0 / (double)im[i]) * inv_h2 * dot(dv, dv);
+    const double eps = fmax((double)P.feps, 1.0e-12);
+    for (uint32_t k = 0; k < P.ncoll; ++k) {
+      const nguo_collider &c = P.coll[k];
+      double dd;
+      d3 n;
+      if (c.kind == NGUO_COLLIDER_HALFSPACE) {
+        n = d3{c.b[0], c.b[1], c.b[2]};
+        dd = dot(xi - d3{c.a[0], c.a[1], c.a[2]}, n);
+      } else {
+        const d3 v = xi - d3{c.a[0], c.a[1], c.a[2]};
+        const double l = len(v);
+        if (l < 1.0e-9) { dd = -(double)c.radius; n = d3{0, 1, 0}; }
+        else { dd = l - (double)c.radius; n = v / l; }
+      }
+      double bo[3];
+      if (!barrier_of(P.barrier_kind, dd, (double)P.dhat, (double)P.kappa, bo)) continue;
+      e[EN_COLLIDER] += bo[0];
+      const double cf = -(double)P.mu * bo[1];
+      if (P.mu > 0.0f && cf > 0.0) e[EN_COLLIDER] += friction_potential(cf, n, xi - to_d(ld3(xs, i)), eps);
+    }
+  }
+  energy_commit(e, acc);
+}
+
+// Shells, hinges, rod segments and strand bends of one element set, one thread per element.
+__global__ __launch_bounds__(BLOCK) void k_energy_elems(ElemDev E, const double *__restrict__ dm64, const double *__restrict__ area64, Params P,
+                                                       const float *__restrict__ x, double *acc) {
+  double e[EN_TERMS] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  const uint32_t b1 = E.ns, b2 = b1 + E.nh, b3 = b2 + E.nr, b4 = b3 + E.nrh;
+  if (i < b1) {
+    const uint32_t *v = E.sh_v + 3 * (size_t)i;
+    const d3 p0 = to_d(ld3(x, v[0])), e1 = to_d(ld3(x, v[1])) - p0, e2 = to_d(ld3(x, v[2])) - p0;
+    const double sc = (double)E.sh_scale[i];
+    const double *dm = dm64 + 4 * (size_t)i;
+    const d3 f0 = e1 * (dm[0] / sc) + e2 * (dm[1] / sc), f1 = e1 * (dm[2] / sc) + e2 * (dm[3] / sc);
+    // eigenvalues of F^T F - I, then sigma - 1 = l / (sigma + 1) and J - 1 without cancellation
+    const double c00 = dot(f0, f0) - 1.0, c11 = dot(f1, f1) - 1.0, c01 = dot(f0, f1);
+    const double mm = 0.5 * (c00 + c11), hd = 0.5 * (c00 - c11), r = sqrt(hd * hd + c01 * c01);
+    const double l1 = mm + r, l2 = mm - r;
+    const double s1 = sqrt(fmax(1.0 + l1, 0.0)), s2 = sqrt(fmax(1.0 + l2, 0.0));
+    const double a1 = l1 / (s1 + 1.0), a2 = l2 / (s2 + 1.0);
+    const double jm = (l1 + l2 + l1 * l2) / (s1 * s2 + 1.0);
+    const double mu = P.shell_mu, lam = P.shell_lambda;
+    const double dev = P.shell_model == NGUO_SHELL_STVK ? mu * (a1 * a1 * a1 * a1 + a2 * a2 * a2 * a2) / 4.0 : 0.5 * mu * (a1 * a1 + a2 * a2);
+    e[EN_SHELL] = area64[i] * sc * sc * (dev + 0.5 * lam * jm * jm);
+  } else if (i < b2) {
+    const uint32_t h = i - b1;
+    const uint32_t *v = E.hg_v + 4 * (size_t)h;
+    const d3 p[4] = {to_d(ld3(x, v[0])), to_d(ld3(x, v[1])), to_d(ld3(x, v[2])), to_d(ld3(x, v[3]))};
+    const double w = dihedral_angle(p) - (double)E.hg_rest[h];
+    e[EN_HINGE] = 0.5 * (double)P.kb * w * w;  // hinge weight 1
+  } else if (i < b3) {
+    const uint32_t r = i - b2;
+    const double l = len(to_d(ld3(x, E.rd_v[2 * (size_t)r])) - to_d(ld3(x, E.rd_v[2 * (size_t)r + 1]))) - (double)E.rd_rest[r];
+    e[EN_ROD] = 0.5 * (double)E.rd_k[r] * l * l;
+  } else if (i < b4) {
+    const uint32_t r = i - b3;
+    const float stiff = E.rh_stiff[r];
+    if (stiff > 0.0f) {
+      const uint32_t *v = E.rh_v + 3 * (size_t)r;
+      const d3 c = to_d(ld3(x, v[1])), e0 = to_d(ld3(x, v[0])) - c, e1 = to_d(ld3(x, v[2])) - c;
+      const double w = atan2(len(cross(e0, e1)), dot(e0, e1)) - (double)E.rh_rest[r];
+      e[EN_ROD_BEND] = 0.5 * (double)stiff * w * w;
+    }
+  }
+  energy_commit(e, acc);
+}
+
+// Barrier and friction potentials of the active stencils, one thread per stencil: the
+// stiffness and the friction quantities as k_stencil_eval forms them (normal from the first
+// participant's distance gradient, proximity weights, the contact_kappa fallback), all from the
+// FP64 gap and gradient.
+__global__ __launch_bounds__(BLOCK) void k_energy_contacts(uint32_t ns, const uint32_t *__restrict__ sidx, const float *__restrict__ sd,
+                                                          float ghat_vt, float ghat_ee, uint32_t nvt, Topo T,
+                                                          SellView K, SellView S, Params P, const float *__restrict__ x,
+                                                          const float *__restrict__ xs, double *acc) {
+  double e[EN_TERMS] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const uint32_t s = blockIdx.x * BLOCK + threadIdx.x;
+  if (s < ns) {
+    uint32_t id[4];
+    for (int a = 0; a < 4; ++a) id[a] = sidx[4 * (size_t)s + a];
+    const float d32 = sd[s];
+    const double ghat = s < nvt ? ghat_vt : ghat_ee;
+    d3 g[4];
+    const double d = stencil_geom64(s < nvt, id, x, d32, g);
+    double k = P.kappa, w[4] = {0, 0, 0, 0};
+    d3 n{0, 0, 0};
+    const bool has_n = !(len(g[0]) < 1.0e-12);
+    if (has_n) {
+      n = g[0] / len(g[0]);
+      double mass[4], m_max = 0.0;
+      for (int a = 0; a < 4; ++a) {
+        w[a] = dot(g[a], n);
+        const float imv = T.im[id[a]];
+        mass[a] = imv > 0.0f ? 1.0 / (double)imv : 0.0;
+        m_max = fmax(m_max, mass[a]);
+      }
+      const double gap = fabs(d);
+      double kk = 0.0;
+      if (gap > 0.0) {
+        double q[12], ss = 0.0;
+        for (int a = 0; a < 4; ++a) {
+          const d3 v = n * (w[a] * d);
+          q[3 * a] = v.x; q[3 * a + 1] = v.y; q[3 * a + 2] = v.z;
+        }
+        for (int c = 0; c < 12; ++c) ss += q[c] * q[c];
+        const double nq = sqrt(ss);
+        if (nq > 0.0) {
+          for (int c = 0; c < 12; ++c) q[c] /= nq;
+          for (int ii = 0; ii < 4; ++ii) {
+            const double m_ii = mass[ii] > 0.0 ? mass[ii] : m_max;
+            for (int jj = 0; jj < 4; ++jj) {
+              float blk[9];
+              lookup_block(T, K, S, id[ii], id[jj], blk);
+              for (int r = 0; r < 3; ++r) {
+                double row = 0.0;
+                for (int c = 0; c < 3; ++c) {
+                  double v = (double)blk[3 * r + c];
+                  if (ii == jj && r == c) v += m_ii / (gap * gap);
+                  row += v * q[3 * jj + c];
+                }
+                kk += q[3 * ii + r] * row;
+              }
+            }
+          }
+        }
+      }
+      k = kk > 0.0 ? kk : (double)P.kappa;
+    }
+    const int kind = d32 < 0.0f ? 1 : P.barrier_kind;
+    double bo[3];
+    if (barrier_of(kind, d, ghat, k, bo)) {
+      e[EN_BARRIER] = bo[0];
+      const double cf = -(double)P.mu * bo[1];
+      if (P.mu > 0.0f && has_n && cf > 0.0) {
+        d3 slip{0, 0, 0};
+        for (int a = 0; a < 4; ++a) slip = slip + (to_d(ld3(x, id[a])) - to_d(ld3(xs, id[a]))) * w[a];
+        e[EN_FRICTION] = friction_potential(cf, n, slip, fmax((double)P.feps, 1.0e-12));
+      }
+    }
+  }
+  energy_commit(e, acc);
+}
+
 struct ElemHost {
   std::vector<uint32_t> sh_v, hg_v, rd_v, rh_v;
   std::vector<uint32_t> sh_face;  // scene shell face index of each shell
   std::vector<float> sh_dm, sh_area, hg_rest, rd_rest, rd_k, rh_rest, rh_stiff;
+  std::vector<double> sh_dm64, sh_area64;  // the rest frames in FP64 (energy evaluation)
   uint32_t ns() const { return (uint32_t)sh_area.size(); }
   uint32_t nh() const { return (uint32_t)hg_rest.size(); }
   uint32_t nr() const { return (uint32_t)rd_rest.size(); }
@@ -1032,6 +1229,7 @@ struct ElemSet {
   DevBuf<uint32_t> sh_v, hg_v, rd_v, rh_v;
   DevBuf<float> sh_dm, sh_area, hg_rest, rd_rest, rd_k, rh_rest, rh_stiff;
   DevBuf<float> sh_scale;
+  DevBuf<double> sh_dm64, sh_area64;
   DevBuf<uint32_t> sh_face;
   std::vector<uint32_t> h_sh_face;
   std::vector<float> h_sh_scale;  // host copy (kept alive for the asynchronous upload)
@@ -1088,6 +1286,7 @@ struct nguo_step {
   uint32_t *hu = nullptr;
   unsigned long long *hl = nullptr;
   nguo_step_metrics metrics{};
+  DevBuf<double> eacc;  // step energy terms (nguo_step_energy)
   DevBuf<float> trace;  // x after every Newton iteration of the last step (when trace_on)
   bool trace_on = false;
   uint32_t trace_count = 0;  // iterates held in trace
@@ -1206,19 +1405,22 @@ std::vector<int> digit_shifts(uint64_t maxval_lo, uint64_t maxval_hi) {
 }
 
 // shell_model.rs:309-349 build_shells (rest frame, Dm^-1, area) on the host.
-bool shell_rest(const float *p0, const float *p1, const float *p2, float dm_out[4], float &area) {
-  f3 a{p0[0], p0[1], p0[2]}, b{p1[0], p1[1], p1[2]}, c{p2[0], p2[1], p2[2]};
-  f3 e1 = b - a, e2 = c - a;
-  f3 n = cross(e1, e2);
-  area = 0.5f * std::sqrt(dot(n, n));
-  if (area <= 1.0e-12f) return false;
-  f3 ax = e1 * (1.0f / std::sqrt(dot(e1, e1)));
-  f3 t = e2 - dot(e2, ax) * ax;
-  f3 ay = t * (1.0f / std::sqrt(dot(t, t)));
-  float m00 = dot(e1, ax), m10 = dot(e1, ay), m01 = dot(e2, ax), m11 = dot(e2, ay);
-  float det = m00 * m11 - m10 * m01;
-  if (std::fabs(det) < 1e-20f) return false;
-  float inv_det = 1.0f / det;
+// shell_model.rs:309-349 the rest frame of one face: Dm^-1 (column-major) and area, in FP32
+// for the step and in FP64 for the energy evaluation.
+template <class T> bool shell_rest(const float *p0, const float *p1, const float *p2, T dm_out[4], T &area) {
+  using V = v3<T>;
+  V a{T(p0[0]), T(p0[1]), T(p0[2])}, b{T(p1[0]), T(p1[1]), T(p1[2])}, c{T(p2[0]), T(p2[1]), T(p2[2])};
+  V e1 = b - a, e2 = c - a;
+  V n = cross(e1, e2);
+  area = T(0.5) * std::sqrt(dot(n, n));
+  if (area <= T(1.0e-12)) return false;
+  V ax = e1 * (T(1) / std::sqrt(dot(e1, e1)));
+  V t = e2 - dot(e2, ax) * ax;
+  V ay = t * (T(1) / std::sqrt(dot(t, t)));
+  T m00 = dot(e1, ax), m10 = dot(e1, ay), m01 = dot(e2, ax), m11 = dot(e2, ay);
+  T det = m00 * m11 - m10 * m01;
+  if (std::fabs(det) < T(1e-20)) return false;
+  T inv_det = T(1) / det;
   // inverse, column-major (xa = (m11, -m10) / det, ya = (-m01, m00) / det)
   dm_out[0] = m11 * inv_det;
   dm_out[1] = -m10 * inv_det;
@@ -1332,6 +1534,7 @@ void upload_elems(ElemSet &S, const ElemHost &H, hipStream_t s) {
     (void)sizeof(T);
   };
   up(S.sh_v, H.sh_v); up(S.sh_dm, H.sh_dm); up(S.sh_area, H.sh_area);
+  up(S.sh_dm64, H.sh_dm64); up(S.sh_area64, H.sh_area64);
   S.h_sh_face = H.sh_face;
   S.h_sh_scale.assign(H.ns(), 1.0f);
   up(S.sh_scale, S.h_sh_scale);
@@ -1776,13 +1979,17 @@ int nguo_step_create(nguo_ctx *ctx, const nguo_scene_desc *sc, nguo_step **out) 
       for (uint32_t f = 0; f < sc->n_shell_faces; ++f) {
         const uint32_t *t = sc->shell_faces + 3 * f;
         float dm[4], area;
+        double dm64[4], area64;
         if (!shell_rest(X + 3 * t[0], X + 3 * t[1], X + 3 * t[2], dm, area)) continue;
+        (void)shell_rest(X + 3 * t[0], X + 3 * t[1], X + 3 * t[2], dm64, area64);
         int c = classify(t, 3);
         NGUO_REQUIRE(c != 2, NGUO_E_UNSUPPORTED, "a shell face mixes pinned and free vertices");
         ElemHost &H = c == 0 ? ha : hs;
         H.sh_v.insert(H.sh_v.end(), t, t + 3);
         H.sh_dm.insert(H.sh_dm.end(), dm, dm + 4);
         H.sh_area.push_back(area);
+        H.sh_dm64.insert(H.sh_dm64.end(), dm64, dm64 + 4);
+        H.sh_area64.push_back(area64);
         H.sh_face.push_back(f);
       }
     }
@@ -2211,6 +2418,55 @@ int nguo_step_adjoint(nguo_step *st, const float *dLdx, float tol, int hessian, 
       stats->sweeps = ps.sweeps;
       stats->residual = ps.residual;
     }
+    return NGUO_OK;
+  });
+}
+
+int nguo_step_energy(nguo_step *st, const float *x, double *terms, double *total) {
+  return guarded([&]() -> int {
+    NGUO_REQUIRE(st && x, NGUO_E_INVALID, "nguo_step_energy: null argument");
+    NGUO_REQUIRE(st->adj_valid, NGUO_E_INVALID,
+                 "nguo_step_energy: no step to evaluate (advance first; set_state and set_rest_scale clear the last step)");
+    require_pinned(st, x, "nguo_step_energy");
+    hipStream_t s = st->s;
+    const size_t n3 = 3 * (size_t)st->V;
+    // the committed end state waits in dx; the step's friction reference (its start) goes to
+    // x0 for the contact evaluation, x takes the positions asked for
+    st->dx.reserve(n3);
+    NGUO_HIP(hipMemcpyAsync(st->dx.p, st->x0.p, n3 * sizeof(float), hipMemcpyDeviceToDevice, s));
+    NGUO_HIP(hipMemcpyAsync(st->x0.p, st->xprev.p, n3 * sizeof(float), hipMemcpyDeviceToDevice, s));
+    st->x.upload(x, n3, s);
+    run_elements(st->act, st->P, st->x.p, s);
+    gather(st->act, st->P, 1, st->im.p, st->x.p, st->x0.p, s);
+    contacts(st);
+    st->eacc.zero(EN_TERMS, s);
+    const double h = (double)st->prm.dt;
+    hipLaunchKernelGGL(k_energy_verts, dim3(div_up(st->V, BLOCK)), dim3(BLOCK), 0, s, st->V, (const float *)st->im.p, (const float *)st->x.p,
+                       (const float *)st->xt.p, (const float *)st->x0.p, 1.0 / (h * h), st->P, st->eacc.p);
+    for (ElemSet *es : {&st->act, &st->stat}) {
+      const uint32_t ne = es->ns + es->nh + es->nr + es->nrh;
+      if (ne)
+        hipLaunchKernelGGL(k_energy_elems, dim3(div_up(ne, BLOCK)), dim3(BLOCK), 0, s, es->dev(), (const double *)es->sh_dm64.p,
+                           (const double *)es->sh_area64.p, st->P, (const float *)st->x.p, st->eacc.p);
+    }
+    if (st->ns) {
+      const float dhat = st->prm.contact_dhat;
+      hipLaunchKernelGGL(k_energy_contacts, dim3(div_up(st->ns, BLOCK)), dim3(BLOCK), 0, s, st->ns, (const uint32_t *)st->sidx.p,
+                         (const float *)st->sd.p, dhat, 0.5f * (dhat + dhat), st->nvt, topo(st), st->act.M.view(),
+                         st->stat.M.view(), st->P, (const float *)st->x.p, (const float *)st->x0.p, st->eacc.p);
+    }
+    NGUO_HIP(hipGetLastError());
+    double e[EN_TERMS];
+    st->eacc.download(e, EN_TERMS, s);
+    NGUO_HIP(hipMemcpyAsync(st->x0.p, st->dx.p, n3 * sizeof(float), hipMemcpyDeviceToDevice, s));
+    NGUO_HIP(hipMemcpyAsync(st->x.p, st->dx.p, n3 * sizeof(float), hipMemcpyDeviceToDevice, s));
+    NGUO_HIP(hipStreamSynchronize(s));
+    double tot = 0.0;
+    for (int k = 0; k < EN_TERMS; ++k) {
+      tot += e[k];
+      if (terms) terms[k] = e[k];
+    }
+    if (total) *total = tot;
     return NGUO_OK;
   });
 }

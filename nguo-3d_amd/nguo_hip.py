@@ -31,7 +31,7 @@ EXPORTED = [
     "nguo_step_bench_spmv", "nguo_step_bench_precond", "nguo_bench_spmv_grid", "nguo_ls_precond_apply",
     "nguo_batch_create", "nguo_batch_destroy", "nguo_batch_advance", "nguo_batch_instance",
     "nguo_step_set_rest_scale", "nguo_step_adjoint", "nguo_batch_set_solve", "nguo_bench_sort",
-    "nguo_step_set_trace", "nguo_lbvh_sort_pairs_noalloc",
+    "nguo_step_set_trace", "nguo_lbvh_sort_pairs_noalloc", "nguo_step_energy",
 ]
 
 
@@ -126,6 +126,7 @@ def load():
         lib.nguo_step_get_metrics.argtypes = [vp, P(StepMetrics)]
         lib.nguo_step_get_newton_trace.argtypes = [vp, C.c_uint32, P(C.c_float), P(C.c_uint32), P(C.c_float)]
         lib.nguo_step_set_trace.argtypes = [vp, C.c_int]
+        lib.nguo_step_energy.argtypes = [vp, P(C.c_float), P(C.c_double), P(C.c_double)]
         lib.nguo_step_stencils.argtypes = [vp, P(C.c_float), C.c_uint32, P(C.c_uint32), P(C.c_float), P(C.c_float),
                                            P(C.c_float), P(C.c_uint32)]
         lib.nguo_step_collider_sides.argtypes = [vp, P(C.c_float), P(C.c_float)]
@@ -458,6 +459,16 @@ class Stepper:
         st = Stats()
         check(load().nguo_step_adjoint(self.h, fptr(g), float(tol), 1 if exact_elastic else 0, fptr(out), C.byref(st)))
         return out, st
+
+    ENERGY_TERMS = ("inertia", "shells", "hinges", "rods", "rod_bends", "barriers", "friction", "colliders")
+
+    def energy(self, x):
+        """(total, terms[8]) of the step energy of the last advanced step at positions x."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        terms = np.zeros(8, np.float64)
+        tot = C.c_double()
+        check(load().nguo_step_energy(self.h, fptr(x), terms.ctypes.data_as(C.POINTER(C.c_double)), C.byref(tot)))
+        return tot.value, terms
 
     def collider_sides(self, x):
         x = np.ascontiguousarray(x, dtype=np.float32)

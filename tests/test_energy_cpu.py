@@ -12,43 +12,16 @@ force, normals and weights), two checks pin the whole assembled system at x_ref:
      checked element by element elsewhere.
 Each check is shown to fail on a planted mistake: the gradient of the same state with one
 term dropped (friction, hinge bending) in the oracle."""
-import ctypes as C
-
 import numpy as np
 import pytest
 
-from conftest import nguo_hip
 import oracle_py as O
+from cases import scene
 
 H = 1e-8       # central-difference step (m)
 GRAD_TOL = 1e-6  # relative to |g| |u| (the bound of g.u): measured <= 6e-10 on these states
 HESS_TOL = 1e-6  # relative 2-norm: measured <= 3e-8
 FLAGS_H = O.OracleSolver.GN_MODEL | O.OracleSolver.UNPROJECTED | O.OracleSolver.NO_ROD_BENDS
-
-
-def collider_scene():
-    """The 10 x 10 sheet falling on an analytic sphere in place of the mesh icosphere (the
-    analytic-collider branch, implicit.rs:2954-3021), self contact off."""
-    s = nguo_hip.Scene(100, seed=0)
-    st = s.static.astype(bool)
-    ctr = s.positions[st].astype(np.float64).mean(axis=0)
-    r = float(np.linalg.norm(s.positions[st] - ctr, axis=1).mean())
-    arr = (nguo_hip.Collider * 1)()
-    arr[0].kind = 1
-    arr[0].a[:] = [float(c) for c in ctr]
-    arr[0].radius = r
-    s._colliders = arr  # keeps the array alive with the scene
-    s.desc.colliders = C.cast(arr, C.POINTER(nguo_hip.Collider))
-    s.desc.n_colliders = 1
-    s.desc.params.self_collision = 0
-    return s
-
-
-def scene(cfg, **params):
-    s = collider_scene() if cfg == "collider" else nguo_hip.Scene(cfg, seed=cfg)
-    for k, v in params.items():
-        setattr(s.desc.params, k, v)
-    return s
 
 
 def newton_point(cfg, steps, **params):

@@ -6,6 +6,7 @@ import pytest
 
 from conftest import nguo_hip
 import oracle_py as O
+from cases import scene
 
 POS_TOL = 1e-5  # |x_gpu - x_oracle|_inf / max(1, |x|_inf) after each Newton iteration / step
 
@@ -38,9 +39,7 @@ def test_free_fall_closed_form(ctx):
 
 
 def oracle_state(cfg, steps, cg_tol=None):
-    s = nguo_hip.Scene(cfg, seed=cfg)
-    if cg_tol is not None:
-        s.desc.params.cg_tol = cg_tol
+    s = scene(cfg) if cg_tol is None else scene(cfg, cg_tol=cg_tol)
     o = O.OracleSolver(s)
     if steps:
         o.step(steps)
@@ -85,13 +84,11 @@ def test_stencils_bitexact(ctx, cfg, steps):
 # that stencil stiffnesses ~650x apart. Such a state records its FP64 gap as an expected
 # failure with the measured figure, instead of widening the bar.
 @pytest.mark.gpu
-@pytest.mark.parametrize("cfg,steps", [(100, 0), (100, 12), (100, 30), (0, 12), (0, 20), (0, 33), (101, 10), (101, 25)])
+@pytest.mark.parametrize("cfg,steps", [(100, 0), (100, 12), (100, 30), (0, 12), (0, 20), (0, 33), (101, 10), (101, 25), ("collider", 14)])
 def test_one_step_parity(ctx, cfg, steps):
     s, o = oracle_state(cfg, steps, cg_tol=1e-10)
     x, v = o.positions(), o.velocities()
-    s64 = nguo_hip.Scene(cfg, seed=cfg)
-    s64.desc.params.cg_tol = 1e-10
-    o64 = O.OracleSolver(s64, fp64=True)
+    o64 = O.OracleSolver(scene(cfg, cg_tol=1e-10), fp64=True)
     o64.set_state(x, v)
     st = nguo_hip.Stepper(ctx, s, trace=True)
     st.set_state(x, v)
@@ -110,8 +107,13 @@ def test_one_step_parity(ctx, cfg, steps):
         worst64 = max(worst64, e64)
     m = st.metrics()
     # the pierce metric agrees with the oracle's own test on the same positions (the
-    # 101 tube's seam rod reaches a piercing state under the reference scheme too)
-    assert m.all_finite == 1 and m.penetration_free == int(not o.self_intersects(st.positions()))
+    # 101 tube's seam rod reaches a piercing state under the reference scheme too); without
+    # self contact the metric is the analytic colliders' side test, compared with the mirror's
+    if s.params.self_collision:
+        want = int(not o.self_intersects(st.positions()))
+    else:
+        want = int(o.metrics()["penetration_free"])
+    assert m.all_finite == 1 and m.penetration_free == want
     assert m.max_reresid <= 1e-9
     st.close()
     if worst64 > POS_TOL:
