@@ -184,6 +184,15 @@ int nguo_step_get_metrics(nguo_step *st, nguo_step_metrics *out);
  * out of the production step); asking for them otherwise is NGUO_E_INVALID. No reference
  * counterpart: the reference logs per-iteration scalars only (implicit.rs:2183-2190). */
 int nguo_step_set_trace(nguo_step *st, int enable);
+/* The Newton solves' arithmetic. NGUO_PCG_FP32_REFINED (default): FP32 PCG (vectors, matrix,
+ * Schwarz inverses) with FP64 iterative refinement on the FP64 residual. NGUO_PCG_FP64: PCG
+ * with FP64 vectors, scalars and Schwarz inverses over the same FP32 matrix values and contact
+ * data; needs the one-level Schwarz preconditioner (NGUO_E_UNSUPPORTED otherwise). Both stop at
+ * the FP64 residual cg_tol. No reference counterpart (the reference's CPU CG is f64, its wgpu
+ * PCG f32, engine-core/src/linsolve.rs:350-411, engine-gpu/src/pcg.rs:488-673). */
+#define NGUO_PCG_FP32_REFINED 0
+#define NGUO_PCG_FP64 1
+int nguo_step_set_precision(nguo_step *st, int precision);
 int nguo_step_get_newton_trace(nguo_step *st, uint32_t k, float *out, uint32_t *count, float *toi);
 /* The active contact stencils at positions x, as implicit.rs:678-753
  * self_collision_stencils returns them (vertex-triangle sorted by (v, t), then edge-edge

@@ -44,12 +44,18 @@ constexpr uint32_t SCHWARZ_MAX_DOF = 3 * SCHWARZ_MAX_DOMAIN;
 // floats of one domain's FP32 inverse: the 21 upper 16 x 16 tiles of the symmetric block
 constexpr uint32_t SCHWARZ_INV_FLOATS = (SCHWARZ_MAX_DOF / 16) * (SCHWARZ_MAX_DOF / 16 + 1) / 2 * 256;
 
+// A domain's tiled inverse: the upper 16 x 16 tiles (P <= Q < 6) of the symmetric 96 x 96
+// block, tile (P, Q) at SCH_TILE64 sym_tile64(P, Q), each row-major.
+constexpr uint32_t SCH_TILE64 = 256;
+__host__ __device__ constexpr uint32_t sym_tile64(uint32_t P, uint32_t Q) { return P * 6u - P * (P - 1u) / 2u + (Q - P); }
+
 struct Schwarz {
   uint32_t nd = 0, domain = SCHWARZ_MAX_DOMAIN;
   std::vector<uint32_t> h_dom_ptr, h_dom_row;
   DevBuf<uint32_t> dom_ptr, dom_row, row_dom, row_pos, fails;
   DevBuf<uint32_t> dom_dof;  // [nd * 96]: stride-4 vector index of dof t of domain d, or ~0u past m
   DevBuf<float> ainv;  // nd blocks of SCHWARZ_INV_FLOATS (upper 16 x 16 tiles, row-major each)
+  DevBuf<double> ainv64;  // the same tiles in FP64 (the FP64 PCG)
   // [2 nd SCHWARZ_MAX_DOMAIN], written with the inverses: the contact incidence range
   // (inc_ptr[row], inc_ptr[row + 1]) of row p of domain d at 2 (d SCHWARZ_MAX_DOMAIN + p)
   DevBuf<uint32_t> inc_rng;
@@ -64,6 +70,18 @@ struct Schwarz {
   DevBuf<float> ainv1;  // nd1 coarse blocks of SCHWARZ_MAX_DOF^2, row-major with stride SCHWARZ_MAX_DOF
   DevBuf<float> rc;     // [4 nd] restricted residual (sum of a domain's rows), per apply
   void build(const SellMatrix &M, uint32_t domain_size, hipStream_t s, float coarse_weight = 0.0f);
+  // Domains for a system with contact stencils (ns; their free-vertex rows, (hp, lambda),
+  // distance gradients and proximity weights on the host): the strongest couplings kept inside
+  // domains (see the definition). Domain size and coarse weight stay those of build().
+  void build_contact(const SellMatrix &M, uint32_t ns, const uint32_t *dof, const float *coef, const float *g, const float *w, hipStream_t s);
+  // Back to build()'s elastic-pattern domains (after build_contact).
+  void restore_elastic(const SellMatrix &M, hipStream_t s);
+  bool contact_built = false;
+  std::vector<uint32_t> h_row_dom, h_row_pos, h_dom_dof, h_node_cdom, h_node_cpos;  // upload sources
+  std::vector<uint32_t> elastic_ptr, elastic_row;  // build()'s domains
+
+ private:
+  void upload(const SellMatrix &M, hipStream_t s);
 };
 
 struct PcgStats {
@@ -157,6 +175,39 @@ class Pcg {
   uint32_t *d_flag_ = nullptr;  // the device's address of h_flag_
   hipEvent_t ev_[2] = {nullptr, nullptr};
   double *h_scal_ = nullptr;
+};
+
+// The FP64 PCG's preconditioner build: Schwarz::build's domains, blocks and factorisation
+// with the inverses kept in FP64 (sz.ainv64).
+void build_schwarz64(hipStream_t s, const SellMatrix &M, const ContactView &C, Schwarz &sz);
+
+// Preconditioned CG in FP64 on the same system: the matrix keeps its FP32 values (read into
+// FP64 sums), the contact term is applied matrix-free from its FP32 stencil data in FP64,
+// vectors, scalars and the one-level Schwarz inverses are FP64. At the drape's contact states
+// the barrier curvature reaches ~1e11 against a mass term of ~0.04, so the system's condition
+// number is ~1e12: FP32 vectors and inverses cannot hold both scales (the FP32 iteration
+// stalls near 1e-4 and leans on refinement sweeps), while FP64 converges in the iteration
+// count of exact arithmetic. Each iteration is four launches: the stencils' blocks of H p,
+// q = (A + C) p with the p.q partials, the Schwarz launch (alpha, r -= alpha q, x += alpha p,
+// z = A_d^-1 r, the r.z | r.r partials), and p = z + beta p with the convergence state.
+class Pcg64 {
+ public:
+  explicit Pcg64(hipStream_t s) : s_(s) {}
+  ~Pcg64();
+  // Solve (A + C) x = b to ||b - (A+C) x|| <= tol ||b|| (FP64 residual) with the one-level
+  // Schwarz `sz` (built over M's pattern); b, x_out device f32 [3 nb].
+  PcgStats solve(const SellMatrix &M, const ContactView &C, Schwarz &sz, const float *b, float tol, uint32_t max_iter, float *x_out);
+  uint32_t chunk = 16;  // iterations per host convergence check
+
+ private:
+  void ensure(uint32_t nb, uint32_t ns, uint32_t nd);
+  hipStream_t s_;
+  uint32_t cap_nb_ = 0, cap_ns_ = 0, cap_nd_ = 0;
+  DevBuf<double> r_, z_, p_, q_, contrib_, x64_, r64_, part_, nxt_, scal_, sums64_;
+  DevBuf<uint32_t> flag_;
+  uint32_t *h_flag_ = nullptr;  // two slots of 4 (pinned)
+  double *h_scal_ = nullptr;
+  hipEvent_t ev_[2] = {nullptr, nullptr};
 };
 
 }  // namespace nguo

@@ -1063,4 +1063,290 @@ double Pcg::time_precond(const SellMatrix &M, const ContactView &C, uint32_t rep
   return ms;
 }
 
+// ---------------------------------------------------------------- FP64 PCG
+// Vectors r, z, p, q of stride 4 doubles per block row (x, y, z, 0; one 32-byte load per
+// gathered neighbour), the solution accumulated in the FP64 iterate of stride 3. scal: 1 pAp,
+// 2 alpha, 4 rr, 5 threshold (eta^2 rr0), 8 / 9 rz by iteration parity. flag: 0 state
+// (0 running, 1 converged, 2 breakdown, 3 cap), 1 iterations done, 2 cap.
+
+// Stencil s (one thread each): the sums g . p and sum_a w_a p_a over its free participants,
+// then each free participant's block of H_s p, hp (g . p) g_a + lambda w_a (s_w - n (n . s_w)),
+// stored at its incidence slot.
+__global__ __launch_bounds__(BLOCK) void k64_stencils(ContactView C, const double4 *__restrict__ p, double4 *__restrict__ contrib,
+                                                     const uint32_t *flag) {
+  if (flag[0]) return;
+  const uint32_t s = blockIdx.x * BLOCK + threadIdx.x;
+  if (s >= C.ns) return;
+  uint32_t db[4], kp[4];
+  double g[12], w[4], x[4][3];
+  for (int b = 0; b < 4; ++b) {
+    db[b] = C.dof[4 * (size_t)s + b];
+    kp[b] = C.pos[4 * (size_t)s + b];
+    w[b] = C.w[4 * (size_t)s + b];
+  }
+  for (int k = 0; k < 12; ++k) g[k] = C.g[12 * (size_t)s + k];
+  const double n0 = C.n[3 * (size_t)s], n1 = C.n[3 * (size_t)s + 1], n2 = C.n[3 * (size_t)s + 2];
+  const double hp = C.coef[2 * (size_t)s], lam = C.coef[2 * (size_t)s + 1];
+  double s1 = 0.0, a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  for (int b = 0; b < 4; ++b) {
+    x[b][0] = x[b][1] = x[b][2] = 0.0;
+    if (db[b] == NONE) continue;
+    const double4 v = p[db[b]];
+    x[b][0] = v.x; x[b][1] = v.y; x[b][2] = v.z;
+    s1 += g[3 * b] * v.x + g[3 * b + 1] * v.y + g[3 * b + 2] * v.z;
+    a0 += w[b] * v.x; a1 += w[b] * v.y; a2 += w[b] * v.z;
+  }
+  const double t = hp * s1, nd = n0 * a0 + n1 * a1 + n2 * a2;
+  for (int a = 0; a < 4; ++a) {
+    if (kp[a] == NONE) continue;
+    double c0 = t * g[3 * a], c1 = t * g[3 * a + 1], c2 = t * g[3 * a + 2];
+    if (lam != 0.0) {
+      const double pw = lam * w[a];
+      c0 += pw * (a0 - n0 * nd); c1 += pw * (a1 - n1 * nd); c2 += pw * (a2 - n2 * nd);
+    }
+    contrib[kp[a]] = make_double4(c0, c1, c2, 0.0);
+  }
+}
+
+// q = (A + C) p, one block row per thread (FP32 block values, FP64 sums, the row's contact
+// blocks added in incidence order); part[workgroup] = p . q over its rows.
+__global__ __launch_bounds__(BLOCK) void k64_spmv(SellView A, ContactView C, const double4 *__restrict__ p, const double4 *__restrict__ contrib,
+                                                 double4 *__restrict__ q, double *part, const uint32_t *flag) {
+  __shared__ double lds[4];
+  if (flag[0]) return;
+  const uint32_t row = blockIdx.x * BLOCK + threadIdx.x;
+  double dp = 0.0;
+  if (row < A.nb) {
+    const uint32_t base = A.ell_w ? 64u * A.ell_w * (row >> 6) : A.slice_ptr[row >> 6], lane = row & 63u, len = A.row_len[row];
+    double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+    for (uint32_t k = 0; k < len; ++k) {
+      const uint32_t c = A.col[base + 64u * k + lane];
+      const BlockVals v = load_block(A.val, base + 64u * k, lane);
+      const double4 x = p[c];
+      y0 += (double)v.a.x * x.x + (double)v.a.y * x.y + (double)v.a.z * x.z;
+      y1 += (double)v.a.w * x.x + (double)v.b.x * x.y + (double)v.b.y * x.z;
+      y2 += (double)v.b.z * x.x + (double)v.b.w * x.y + (double)v.c * x.z;
+    }
+    if (C.ns)
+      for (uint32_t k = C.inc_ptr[row]; k < C.inc_ptr[row + 1]; ++k) {
+        const double4 cv = contrib[k];
+        y0 += cv.x; y1 += cv.y; y2 += cv.z;
+      }
+    q[row] = make_double4(y0, y1, y2, 0.0);
+    const double4 pr = p[row];
+    dp = pr.x * y0 + pr.y * y1 + pr.z * y2;
+  }
+  const double t = block_sum(dp, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// Schwarz launch of the FP64 PCG, one workgroup per domain. INIT: r = r64 (stride 3), z =
+// A_d^-1 r, p = z. Otherwise: alpha = rz / pAp (npq SpMV partials, reduced by every workgroup
+// in the same order; breakdown stops the solve), r -= alpha q, x += alpha p, z = A_d^-1 r.
+// The inverse is read from the 21 upper 16 x 16 FP64 tiles: row t's entry k comes from tile
+// (t / 16, k / 16) directly, or transposed from tile (k / 16, t / 16) below the diagonal.
+// Partials r.z -> nxt[d], r.r -> nxt[nd + d].
+template <bool INIT>
+__global__ __launch_bounds__(BLOCK) void k64_schwarz(uint32_t nd, const uint32_t *__restrict__ dom_ptr, const uint32_t *__restrict__ dom_dof,
+                                                    const double *__restrict__ ainv, const double *__restrict__ r64, double *__restrict__ r,
+                                                    double *__restrict__ z, double *__restrict__ p, const double *__restrict__ q,
+                                                    double *__restrict__ x64, const double *pq_part, uint32_t npq, double *nxt, double *scal,
+                                                    uint32_t *flag, uint32_t j) {
+  __shared__ double rs[SCHWARZ_MAX_DOF];
+  __shared__ double lds[2 * BLOCK / WAVE];
+  const uint32_t d = blockIdx.x, t = threadIdx.x;
+  if (!INIT && flag[0]) return;
+  double alpha = 0.0;
+  if (!INIT) {
+    const double *src[1] = {pq_part};
+    const uint32_t n[1] = {npq};
+    double v[1];
+    sum_parts_all<BLOCK, 1>(src, n, v, lds);
+    if (!(v[0] > 0.0)) {
+      if (d == 0 && t == 0) flag[0] = 2;
+      return;
+    }
+    alpha = scal[8 + (j & 1u)] / v[0];
+    if (d == 0 && t == 0) { scal[1] = v[0]; scal[2] = alpha; }
+  }
+  const uint32_t m = 3 * (dom_ptr[d + 1] - dom_ptr[d]);
+  const uint32_t od = t < m ? dom_dof[(size_t)d * SCHWARZ_MAX_DOF + t] : NONE;
+  const size_t i3 = od != NONE ? 3 * (size_t)(od >> 2) + (od & 3u) : 0;
+  double rv = 0.0;
+  if (t < m) {
+    if (INIT) {
+      rv = r64[i3];
+    } else {
+      rv = r[od] - alpha * q[od];
+      x64[i3] += alpha * p[od];
+    }
+    r[od] = rv;
+  }
+  if (t < SCHWARZ_MAX_DOF) rs[t] = rv;
+  __syncthreads();
+  double v[2] = {0.0, 0.0};
+  if (t < m) {
+    const double *a = ainv + (size_t)d * SCHWARZ_INV_FLOATS;
+    const uint32_t P = t / 16, i = t % 16;
+    double zv = 0.0;
+    for (uint32_t k = 0; k < m; ++k) {
+      const uint32_t Q = k / 16, jj = k % 16;
+      zv += (P <= Q ? a[SCH_TILE64 * sym_tile64(P, Q) + 16 * i + jj] : a[SCH_TILE64 * sym_tile64(Q, P) + 16 * jj + i]) * rs[k];
+    }
+    z[od] = zv;
+    if (INIT) p[od] = zv;
+    v[0] = rv * zv;
+    v[1] = rv * rv;
+  }
+  wg_sum_all<BLOCK, 2>(v, lds);
+  if (t == 0) { nxt[d] = v[0]; nxt[nd + d] = v[1]; }
+}
+
+// Opens the solve: rz0 and rr0 from the Schwarz partials, threshold eta^2 rr0, state.
+__global__ __launch_bounds__(BLOCK) void k64_init(const double *nxt, uint32_t nd, double eta2, uint32_t cap, double *scal, uint32_t *flag) {
+  __shared__ double lds[2 * BLOCK / WAVE];
+  const double *src[2] = {nxt, nxt + nd};
+  const uint32_t n[2] = {nd, nd};
+  double v[2];
+  sum_parts_all<BLOCK, 2>(src, n, v, lds);
+  if (threadIdx.x == 0) {
+    scal[8] = v[0]; scal[4] = v[1]; scal[5] = eta2 * v[1];
+    flag[0] = v[1] == 0.0 ? 1u : (cap == 0 ? 3u : 0u);
+    flag[1] = 0;
+    flag[2] = cap;
+  }
+}
+
+// Closes iteration j: rz_{j+1}, rr_{j+1} from the Schwarz partials (every workgroup, same
+// order); converged (rr <= threshold) or capped stops the solve, else p = z + beta p.
+__global__ __launch_bounds__(BLOCK) void k64_pnext(uint32_t n4, const double *__restrict__ z, double *__restrict__ p, const double *nxt,
+                                                  uint32_t nd, double *scal, uint32_t *flag, uint32_t j) {
+  __shared__ double lds[2 * BLOCK / WAVE];
+  if (flag[0]) return;
+  const double *src[2] = {nxt, nxt + nd};
+  const uint32_t n[2] = {nd, nd};
+  double v[2];
+  sum_parts_all<BLOCK, 2>(src, n, v, lds);
+  const bool conv = v[1] <= scal[5], capped = j + 1 >= flag[2];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    flag[1] = j + 1;
+    scal[4] = v[1];
+    scal[8 + ((j + 1) & 1u)] = v[0];
+    if (conv) flag[0] = 1;
+    else if (capped) flag[0] = 3;
+  }
+  if (conv || capped) return;
+  const double beta = v[0] / scal[8 + (j & 1u)];
+  const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i < n4) p[i] = z[i] + beta * p[i];
+}
+
+Pcg64::~Pcg64() {
+  for (auto &e : ev_)
+    if (e) (void)hipEventDestroy(e);
+  if (h_flag_) (void)hipHostFree(h_flag_);
+  if (h_scal_) (void)hipHostFree(h_scal_);
+}
+
+void Pcg64::ensure(uint32_t nb, uint32_t ns, uint32_t nd) {
+  if (!h_flag_) {
+    NGUO_HIP(hipHostMalloc((void **)&h_flag_, 8 * sizeof(uint32_t), hipHostMallocDefault));
+    NGUO_HIP(hipHostMalloc((void **)&h_scal_, 8 * sizeof(double), hipHostMallocDefault));
+    for (auto &e : ev_) NGUO_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  if (nb > cap_nb_) {
+    const size_t n4 = 4 * (size_t)nb, n3 = 3 * (size_t)nb;
+    r_.zero(n4, s_); z_.zero(n4, s_); p_.zero(n4, s_); q_.zero(n4, s_);  // fourth components stay zero
+    x64_.reserve(n3); r64_.reserve(n3);
+    cap_nb_ = nb;
+  }
+  part_.reserve(std::max<size_t>(div_up(nb, BLOCK), 1) + 8);
+  if (nd > cap_nd_) { nxt_.reserve(2 * (size_t)nd + 2); cap_nd_ = nd; }
+  if (ns > cap_ns_) {
+    const uint32_t c = std::max(ns, 1u << 14);
+    contrib_.reserve(16 * (size_t)c);
+    sums64_.reserve(4 * (size_t)c);
+    cap_ns_ = c;
+  }
+  scal_.reserve(16);
+  flag_.reserve(4);
+}
+
+PcgStats Pcg64::solve(const SellMatrix &M, const ContactView &C, Schwarz &sz, const float *b, float tol, uint32_t max_iter, float *x_out) {
+  PcgStats st;
+  const uint32_t nb = M.nb, n3 = 3 * nb, n4 = 4 * nb, nd = sz.nd;
+  NGUO_REQUIRE(nd > 0 && sz.nd1 == 0, NGUO_E_UNSUPPORTED, "FP64 PCG: needs the one-level Schwarz preconditioner");
+  ensure(nb, C.ns, nd);
+  const SellView A = M.view();
+  const unsigned G = div_up(nb, BLOCK), G3 = div_up(n3, BLOCK), G4 = div_up(n4, BLOCK), GS = div_up(C.ns, BLOCK);
+  build_schwarz64(s_, M, C, sz);
+  // ||b||^2 through the FP64 residual at x = 0
+  hipLaunchKernelGGL(k_fill64, dim3(G3), dim3(BLOCK), 0, s_, n3, x64_.p, 0.0);
+  if (C.ns) hipLaunchKernelGGL((k_contact_sums<double, double>), dim3(GS), dim3(BLOCK), 0, s_, C, (const double *)x64_.p, sums64_.p, nullptr);
+  hipLaunchKernelGGL(k_residual64, dim3(G), dim3(BLOCK), 0, s_, A, C, (const double *)sums64_.p, b, (const double *)x64_.p, r64_.p, part_.p);
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(BLOCK), 0, s_, (const double *)part_.p, G, scal_.p + 7);
+  NGUO_HIP(hipMemcpyAsync(h_scal_, scal_.p + 7, sizeof(double), hipMemcpyDeviceToHost, s_));
+  NGUO_HIP(hipStreamSynchronize(s_));
+  const double bb = h_scal_[0];
+  double rr = bb;
+  double4 *r4 = (double4 *)r_.p, *p4 = (double4 *)p_.p, *q4 = (double4 *)q_.p;
+  (void)r4;
+  if (bb > 0.0) {
+    for (uint32_t sweep = 0; sweep < 8; ++sweep) {
+      const double res = std::sqrt(rr / bb);
+      if (res <= (double)tol || st.iters >= max_iter) break;
+      // one sweep reaches the tolerance unless the recursive residual drifts from the true one
+      const double eta = std::min(0.5, 0.5 * (double)tol / res);
+      hipLaunchKernelGGL(k64_schwarz<true>, dim3(nd), dim3(BLOCK), 0, s_, nd, (const uint32_t *)sz.dom_ptr.p, (const uint32_t *)sz.dom_dof.p,
+                         (const double *)sz.ainv64.p, (const double *)r64_.p, r_.p, z_.p, p_.p, (const double *)q_.p, x64_.p,
+                         (const double *)part_.p, G, nxt_.p, scal_.p, flag_.p, 0u);
+      hipLaunchKernelGGL(k64_init, dim3(1), dim3(BLOCK), 0, s_, (const double *)nxt_.p, nd, eta * eta, max_iter - st.iters, scal_.p, flag_.p);
+      uint32_t j = 0, queued = 0, waited = 0, done_it = 0, state = 0;
+      auto queue_chunk = [&]() {
+        const uint32_t sl = queued & 1u;
+        for (uint32_t c = 0; c < chunk; ++c, ++j) {
+          if (C.ns) hipLaunchKernelGGL(k64_stencils, dim3(GS), dim3(BLOCK), 0, s_, C, (const double4 *)p4, (double4 *)contrib_.p, (const uint32_t *)flag_.p);
+          hipLaunchKernelGGL(k64_spmv, dim3(G), dim3(BLOCK), 0, s_, A, C, (const double4 *)p4, (const double4 *)contrib_.p, q4, part_.p,
+                             (const uint32_t *)flag_.p);
+          hipLaunchKernelGGL(k64_schwarz<false>, dim3(nd), dim3(BLOCK), 0, s_, nd, (const uint32_t *)sz.dom_ptr.p, (const uint32_t *)sz.dom_dof.p,
+                             (const double *)sz.ainv64.p, (const double *)nullptr, r_.p, z_.p, p_.p, (const double *)q_.p, x64_.p,
+                             (const double *)part_.p, G, nxt_.p, scal_.p, flag_.p, j);
+          hipLaunchKernelGGL(k64_pnext, dim3(G4), dim3(BLOCK), 0, s_, n4, (const double *)z_.p, p_.p, (const double *)nxt_.p, nd, scal_.p, flag_.p, j);
+        }
+        NGUO_HIP(hipGetLastError());
+        NGUO_HIP(hipMemcpyAsync(h_flag_ + 4 * sl, flag_.p, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, s_));
+        NGUO_HIP(hipEventRecord(ev_[sl], s_));
+        ++queued;
+      };
+      queue_chunk();
+      const uint64_t sweep_cap = (uint64_t)max_iter - st.iters;
+      while (true) {
+        queue_chunk();
+        const uint32_t sl = waited & 1u;
+        NGUO_HIP(hipEventSynchronize(ev_[sl]));
+        ++waited;
+        state = h_flag_[4 * sl];
+        done_it = h_flag_[4 * sl + 1];
+        if (state) break;
+        NGUO_REQUIRE((uint64_t)j <= sweep_cap + 2ull * chunk + 2ull, NGUO_E_DEVICE, "FP64 PCG: the convergence state did not reach the host");
+      }
+      NGUO_HIP(hipStreamSynchronize(s_));  // the chunk queued past convergence exits at once
+      st.iters += done_it;
+      st.sweeps += 1;
+      if (state == 2) st.breakdown = true;
+      if (C.ns) hipLaunchKernelGGL((k_contact_sums<double, double>), dim3(GS), dim3(BLOCK), 0, s_, C, (const double *)x64_.p, sums64_.p, nullptr);
+      hipLaunchKernelGGL(k_residual64, dim3(G), dim3(BLOCK), 0, s_, A, C, (const double *)sums64_.p, b, (const double *)x64_.p, r64_.p, part_.p);
+      hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(BLOCK), 0, s_, (const double *)part_.p, G, scal_.p + 7);
+      NGUO_HIP(hipMemcpyAsync(h_scal_, scal_.p + 7, sizeof(double), hipMemcpyDeviceToHost, s_));
+      NGUO_HIP(hipStreamSynchronize(s_));
+      rr = h_scal_[0];
+      if (st.breakdown && done_it == 0) break;
+    }
+  }
+  hipLaunchKernelGGL(k_to_f32, dim3(G3), dim3(BLOCK), 0, s_, n3, (const double *)x64_.p, x_out);
+  NGUO_HIP(hipGetLastError());
+  st.residual = bb > 0.0 ? std::sqrt(rr / bb) : 0.0;
+  return st;
+}
+
 }  // namespace nguo

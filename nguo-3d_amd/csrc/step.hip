@@ -1280,6 +1280,10 @@ struct nguo_step {
   uint32_t ns = 0, nvt = 0, nee = 0;
   RadixSort sorter;
   std::unique_ptr<Pcg> pcg;
+  std::unique_ptr<Pcg64> pcg64;  // the FP64 PCG (precision NGUO_PCG_FP64)
+  int precision = NGUO_PCG_FP32_REFINED;
+  std::vector<uint32_t> h_sdof;  // contact stencil data read back for the Schwarz domains
+  std::vector<float> h_scoef, h_sg, h_sw;
   std::unique_ptr<Schwarz> sz;
   // host scratch (pinned)
   float *hf = nullptr;
@@ -1775,6 +1779,44 @@ void build_static_trees(nguo_step *st, hipStream_t s) {
   if (st->has_collider) st->tree_tri_coll.build(0, st->n_tri_coll, st->tri_coll_ids.p, st->tris.p, st->x.p, s);
 }
 
+// True when the step's Newton solves use the FP64 PCG (its Schwarz preconditioner is one-level).
+bool uses_fp64(const nguo_step *st) { return st->precision == NGUO_PCG_FP64 && st->sz && st->sz->nd1 == 0; }
+
+// The Newton system (act.M + contacts) x = b with the step's solver: the FP64 PCG, or the FP32
+// PCG with FP64 refinement (through the batch's gate when the step is a batch member).
+// The Schwarz domains of the system about to be solved: with contact stencils, built around
+// their couplings (Schwarz::build_contact, from the stencils' rows and coefficients read back);
+// without, the elastic-pattern domains of the step object.
+void prepare_schwarz(nguo_step *st) {
+  if (!st->sz) return;
+  hipStream_t s = st->s;
+  if (st->ns == 0) {
+    st->sz->restore_elastic(st->act.M, s);
+    return;
+  }
+  const size_t ns = st->ns;
+  st->h_sdof.resize(4 * ns);
+  st->h_scoef.resize(2 * ns);
+  st->h_sg.resize(12 * ns);
+  st->h_sw.resize(4 * ns);
+  st->cdof.download(st->h_sdof.data(), 4 * ns, s);
+  st->ccoef.download(st->h_scoef.data(), 2 * ns, s);
+  st->cg.download(st->h_sg.data(), 12 * ns, s);
+  st->cw.download(st->h_sw.data(), 4 * ns, s);
+  NGUO_HIP(hipStreamSynchronize(s));
+  st->sz->build_contact(st->act.M, st->ns, st->h_sdof.data(), st->h_scoef.data(), st->h_sg.data(), st->h_sw.data(), s);
+}
+
+PcgStats solve_system(nguo_step *st, const float *b, float tol, float *x) {
+  prepare_schwarz(st);
+  if (uses_fp64(st)) {
+    if (!st->pcg64) st->pcg64.reset(new Pcg64(st->s));
+    return st->pcg64->solve(st->act.M, contact_view(st), *st->sz, b, tol, st->prm.cg_max_iter, x);
+  }
+  if (st->gate) return st->gate->solve(st, st->act.M, contact_view(st), b, tol, st->prm.cg_max_iter, x);
+  return st->pcg->solve(st->act.M, contact_view(st), b, tol, st->prm.cg_max_iter, x);
+}
+
 void assemble_and_solve(nguo_step *st, nguo_stats &stats) {
   hipStream_t s = st->s;
   run_elements(st->act, st->P, st->x.p, s);
@@ -1789,8 +1831,7 @@ void assemble_and_solve(nguo_step *st, nguo_stats &stats) {
                        (const uint32_t *)st->inc.p, (const float *)st->cgrad.p, st->ns > 0 ? 1 : 0, st->rhs.p);
   NGUO_HIP(hipGetLastError());
   if (st->NF == 0) { stats = nguo_stats{}; return; }
-  PcgStats ps = st->gate ? st->gate->solve(st, st->act.M, contact_view(st), st->rhs.p, st->prm.cg_tol, st->prm.cg_max_iter, st->dxf.p)
-                         : st->pcg->solve(st->act.M, contact_view(st), st->rhs.p, st->prm.cg_tol, st->prm.cg_max_iter, st->dxf.p);
+  PcgStats ps = solve_system(st, st->rhs.p, st->prm.cg_tol, st->dxf.p);
   stats.iters = ps.iters;
   stats.sweeps = ps.sweeps;
   stats.residual = ps.residual;
@@ -1807,7 +1848,7 @@ struct GateMembership {
 void advance_one(nguo_step *st) {
   hipStream_t s = st->s;
   const uint64_t allocs0 = g_dev_allocs, frees0 = g_dev_frees;
-  GateMembership member(st->gate);
+  GateMembership member(uses_fp64(st) ? nullptr : st->gate);  // FP64 solves run on the instance's own stream
   const uint32_t V = st->V;
   const unsigned GV = div_up(V, BLOCK);
   float h = st->prm.dt;
@@ -2280,6 +2321,15 @@ int nguo_step_set_state(nguo_step *st, const float *positions, const float *velo
   });
 }
 
+int nguo_step_set_precision(nguo_step *st, int precision) {
+  if (!st) return fail(NGUO_E_INVALID, "nguo_step_set_precision: null handle");
+  if (precision != NGUO_PCG_FP32_REFINED && precision != NGUO_PCG_FP64) return fail(NGUO_E_INVALID, "nguo_step_set_precision: unknown precision");
+  if (precision == NGUO_PCG_FP64 && !(st->sz && st->sz->nd1 == 0))
+    return fail(NGUO_E_UNSUPPORTED, "nguo_step_set_precision: the FP64 PCG needs the one-level Schwarz preconditioner");
+  st->precision = precision;
+  return NGUO_OK;
+}
+
 int nguo_step_set_trace(nguo_step *st, int enable) {
   if (!st) return fail(NGUO_E_INVALID, "nguo_step_set_trace: null handle");
   st->trace_on = enable != 0;
@@ -2402,7 +2452,14 @@ int nguo_step_adjoint(nguo_step *st, const float *dLdx, float tol, int hessian, 
       for (int c = 0; c < 3; ++c) hb[3 * (size_t)k + c] = dLdx[3 * (size_t)st->h_free[k] + c];
     st->adj_b.upload(hb, s);
     st->lam.reserve(3 * (size_t)st->NF);
-    PcgStats ps = st->pcg->solve(st->act.M, contact_view(st), st->adj_b.p, tol, st->prm.cg_max_iter, st->lam.p);
+    PcgStats ps;
+    prepare_schwarz(st);
+    if (uses_fp64(st)) {
+      if (!st->pcg64) st->pcg64.reset(new Pcg64(s));
+      ps = st->pcg64->solve(st->act.M, contact_view(st), *st->sz, st->adj_b.p, tol, st->prm.cg_max_iter, st->lam.p);
+    } else {
+      ps = st->pcg->solve(st->act.M, contact_view(st), st->adj_b.p, tol, st->prm.cg_max_iter, st->lam.p);
+    }
     NGUO_HIP(hipMemcpyAsync(st->x0.p, st->x.p, 3 * (size_t)st->V * sizeof(float), hipMemcpyDeviceToDevice, s));
     NGUO_REQUIRE(!ps.breakdown, NGUO_E_UNSUPPORTED,
                  "nguo_step_adjoint: the adjoint Hessian is not positive definite at x* (PCG breakdown); use the Newton Hessian");

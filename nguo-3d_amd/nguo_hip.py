@@ -31,7 +31,7 @@ EXPORTED = [
     "nguo_step_bench_spmv", "nguo_step_bench_precond", "nguo_bench_spmv_grid", "nguo_ls_precond_apply",
     "nguo_batch_create", "nguo_batch_destroy", "nguo_batch_advance", "nguo_batch_instance",
     "nguo_step_set_rest_scale", "nguo_step_adjoint", "nguo_batch_set_solve", "nguo_bench_sort",
-    "nguo_step_set_trace", "nguo_lbvh_sort_pairs_noalloc", "nguo_step_energy",
+    "nguo_step_set_trace", "nguo_lbvh_sort_pairs_noalloc", "nguo_step_energy", "nguo_step_set_precision",
 ]
 
 
@@ -126,6 +126,7 @@ def load():
         lib.nguo_step_get_metrics.argtypes = [vp, P(StepMetrics)]
         lib.nguo_step_get_newton_trace.argtypes = [vp, C.c_uint32, P(C.c_float), P(C.c_uint32), P(C.c_float)]
         lib.nguo_step_set_trace.argtypes = [vp, C.c_int]
+        lib.nguo_step_set_precision.argtypes = [vp, C.c_int]
         lib.nguo_step_energy.argtypes = [vp, P(C.c_float), P(C.c_double), P(C.c_double)]
         lib.nguo_step_stencils.argtypes = [vp, P(C.c_float), C.c_uint32, P(C.c_uint32), P(C.c_float), P(C.c_float),
                                            P(C.c_float), P(C.c_uint32)]
@@ -364,6 +365,12 @@ class Stepper:
         check(load().nguo_step_create(ctx.h, C.byref(scene.desc), C.byref(self.h)))
         if trace:
             self.set_trace(True)
+
+    PCG_FP32_REFINED, PCG_FP64 = 0, 1
+
+    def set_precision(self, precision):
+        """Newton solves in FP32 PCG + FP64 refinement (0) or FP64 PCG (1)."""
+        check(load().nguo_step_set_precision(self.h, int(precision)))
 
     def set_trace(self, on):
         """Keep the positions after every Newton iteration of the following steps."""

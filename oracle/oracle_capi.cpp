@@ -276,6 +276,32 @@ template <class S> struct Switches {
 }  // namespace
 extern "C" {
 
+// The held contact quantities (oracle_solver_freeze) per stencil, for building the contact
+// operator outside: idx [4], gap, distance gradient [12], Rayleigh stiffness, barrier kind,
+// ghat, friction flag, lambda, normal [3], weights [4]. Returns the stencil count (arrays of
+// at least cap entries; nothing written beyond cap).
+uint32_t oracle_solver_frozen_stencils(void *p, uint32_t cap, uint32_t *idx, double *d, double *g12, double *kappa, int32_t *kind,
+                                       double *ghat, int32_t *fric, double *lam, double *n3, double *w4) {
+  auto *h = (Handle *)p;
+  auto go = [&](auto *f) -> uint32_t {
+    if (!f) return 0;
+    const uint32_t ns = (uint32_t)f->st.size();
+    for (uint32_t i = 0; i < ns && i < cap; ++i) {
+      const auto &s = f->st[i];
+      for (int a = 0; a < 4; ++a) {
+        idx[4 * i + a] = s.idx[a];
+        for (int c = 0; c < 3; ++c) g12[12 * i + 3 * a + c] = (double)s.g[a][c];
+        w4[4 * i + a] = (double)f->fric_w[i][a];
+      }
+      d[i] = (double)s.d; kappa[i] = (double)f->kappa[i]; kind[i] = f->kind[i]; ghat[i] = (double)s.ghat;
+      fric[i] = f->fric[i]; lam[i] = (double)f->fric_lam[i];
+      for (int c = 0; c < 3; ++c) n3[3 * i + c] = (double)f->fric_n[i][c];
+    }
+    return ns;
+  };
+  return h->fp64 ? go(h->fz64.get()) : go(h->fz32.get());
+}
+
 // Assembled gradient at x under the switches.
 int oracle_solver_model_gradient(void *p, const double *x, const double *xt, double inv_h2, int flags, double *grad) {
   auto *h = (Handle *)p;

@@ -97,6 +97,8 @@ def load():
         lib.oracle_solver_hess_vec.restype = C.c_int
         lib.oracle_solver_hess_vec.argtypes = [vp, D, D, C.c_double, C.c_int, D, D]
         lib.oracle_solver_x_tilde_d.argtypes = [vp, D]
+        lib.oracle_solver_frozen_stencils.restype = C.c_uint32
+        lib.oracle_solver_frozen_stencils.argtypes = [vp, C.c_uint32, U, D, D, D, C.POINTER(C.c_int32), D, C.POINTER(C.c_int32), D, D, D]
         lib.oracle_morton_codes.argtypes = [C.c_uint32, F, C.c_uint32, U, U]
         lib.oracle_sort_pairs.argtypes = [C.c_uint32, U, U]
         lib.oracle_karras.argtypes = [C.c_uint32, U, U, U]
@@ -257,6 +259,18 @@ class OracleSolver:
         terms = np.zeros(8)
         e = self.lib.oracle_solver_energy(self.h, dp(d64(x)), dp(d64(xt)), inv_h2, 1 if frozen else 0, dp(terms))
         return e, terms
+
+    def frozen_stencils(self):
+        """The held contact quantities per stencil (after freeze): dict of arrays."""
+        ns = self.lib.oracle_solver_frozen_stencils(self.h, 0, None, None, None, None, None, None, None, None, None, None)
+        m = max(ns, 1)
+        out = dict(idx=np.zeros((m, 4), np.uint32), d=np.zeros(m), g=np.zeros((m, 12)), kappa=np.zeros(m), kind=np.zeros(m, np.int32),
+                   ghat=np.zeros(m), fric=np.zeros(m, np.int32), lam=np.zeros(m), n=np.zeros((m, 3)), w=np.zeros((m, 4)))
+        I32 = C.POINTER(C.c_int32)
+        self.lib.oracle_solver_frozen_stencils(self.h, ns, up(out["idx"]), dp(out["d"]), dp(out["g"]), dp(out["kappa"]),
+                                               out["kind"].ctypes.data_as(I32), dp(out["ghat"]), out["fric"].ctypes.data_as(I32),
+                                               dp(out["lam"]), dp(out["n"]), dp(out["w"]))
+        return {k: v[:ns] for k, v in out.items()}
 
     def model_gradient(self, x, xt, inv_h2, flags=0):
         g = np.zeros(3 * self.n)

@@ -205,3 +205,31 @@ def test_batch_matches_independent_steppers(ctx, batched, nscenes):
         assert mb.newton_iters == ms.newton_iters and mb.max_cg_iters == ms.max_cg_iters
         st.close()
     b.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cfg,steps", [(0, 20), (100, 30)])
+def test_fp64_pcg_step_parity(ctx, cfg, steps):
+    # the FP64 PCG (nguo_step_set_precision) on sheet states in contact: every Newton iterate
+    # against the FP32 mirror and the FP64 twin at 1e-10, as the default solver is held
+    s, o = oracle_state(cfg, steps, cg_tol=1e-10)
+    x, v = o.positions(), o.velocities()
+    o64 = O.OracleSolver(scene(cfg, cg_tol=1e-10), fp64=True)
+    o64.set_state(x, v)
+    st = nguo_hip.Stepper(ctx, s, trace=True)
+    st.set_precision(nguo_hip.Stepper.PCG_FP64)
+    st.set_state(x, v)
+    o.step(1)
+    o64.step(1)
+    st.advance(1)
+    ox, _ = o.newton_trace()
+    dx, _ = o64.newton_trace()
+    gx, _ = st.newton_trace()
+    assert len(gx) == len(ox) == len(dx)
+    for k in range(len(ox)):
+        e32, e64, spread = rel_err(gx[k], ox[k]), rel_err(gx[k], dx[k]), rel_err(ox[k], dx[k])
+        print(f"newton {k}: |gpu-oracle32| {e32:.2e} |gpu-oracle64| {e64:.2e} |oracle32-oracle64| {spread:.2e}")
+        assert e64 <= POS_TOL and e32 <= max(POS_TOL, 1.5 * spread), (k, e32, e64, spread)
+    m = st.metrics()
+    assert m.all_finite == 1 and m.max_reresid <= 1e-9 and m.num_stencils > 0
+    st.close()
