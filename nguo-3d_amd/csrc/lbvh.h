@@ -47,6 +47,8 @@ class RadixSort {
   // digit positions to sort by (least significant first), e.g. {0, 8, 16, 24} for u32 keys.
   void sort_u32(uint32_t n, uint32_t *keys, uint32_t *vals, const std::vector<int> &shifts, hipStream_t s);
   void sort_u64(uint32_t n, uint64_t *keys, uint32_t *vals, const std::vector<int> &shifts, hipStream_t s);
+  // Sizes the scratch for sorts of up to n u64-keyed pairs (no allocation on later sorts).
+  void reserve_u64(uint32_t n);
 
  private:
   DevBuf<uint32_t> k32_, v_, hist_, sums_;

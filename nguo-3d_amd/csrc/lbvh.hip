@@ -233,6 +233,15 @@ void RadixSort::sort_u32(uint32_t n, uint32_t *keys, uint32_t *vals, const std::
   radix_sort<uint32_t>(n, keys, vals, k32_.p, v_.p, hist_, sums_, shifts, s);
 }
 
+void RadixSort::reserve_u64(uint32_t n) {
+  n = std::max(n, SORT_MIN_CAP);
+  k64_.reserve(n);
+  v_.reserve(n);
+  // the small-tile passes use the most counters
+  hist_.reserve(256u * (size_t)div_up(n, SMALL_ITEMS * BLOCK));
+  sums_.reserve(div_up(256u * (size_t)div_up(n, SMALL_ITEMS * BLOCK), SCAN_TILE) + 1);
+}
+
 void RadixSort::sort_u64(uint32_t n, uint64_t *keys, uint32_t *vals, const std::vector<int> &shifts, hipStream_t s) {
   if (n <= 1) return;
   k64_.reserve(std::max(n, SORT_MIN_CAP));

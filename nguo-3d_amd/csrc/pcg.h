@@ -76,6 +76,10 @@ struct Schwarz {
   void build_contact(const SellMatrix &M, uint32_t ns, const uint32_t *dof, const float *coef, const float *g, const float *w, hipStream_t s);
   // Back to build()'s elastic-pattern domains (after build_contact).
   void restore_elastic(const SellMatrix &M, hipStream_t s);
+  // Sizes the per-domain device arrays for up to nd_cap domains (contact-built domain sets
+  // vary in count from system to system). Reallocates without keeping contents: call before
+  // build().
+  void reserve(uint32_t nd_cap);
   bool contact_built = false;
   std::vector<uint32_t> h_row_dom, h_row_pos, h_dom_dof, h_node_cdom, h_node_cpos;  // upload sources
   std::vector<uint32_t> elastic_ptr, elastic_row;  // build()'s domains
@@ -133,6 +137,9 @@ class Pcg {
   // systems' assembly and its own streams after s.
   static void solve_batch(hipStream_t s, PcgBatchItem *items, uint32_t n);
   bool batchable() const { return sz_ && sz_->nd > 0 && sz_->nd1 == 0; }
+  // Sizes the solver's buffers for nb block rows, up to ns_cap contact stencils and nz_cap
+  // preconditioner partials (no allocation on later solves within these).
+  void reserve(uint32_t nb, uint32_t ns_cap, uint32_t nz_cap);
   // y = (A + C) x in f32 (stride-3 vectors), for parity checks.
   void apply(const SellView &A, const ContactView &C, const float *x, float *y);
   // Average duration (ms) of the PCG's SpMV kernel on (A + C) x over `reps` back-to-back

@@ -686,6 +686,12 @@ void Pcg::ensure(uint32_t nb, uint32_t ns) {
   }
 }
 
+void Pcg::reserve(uint32_t nb, uint32_t ns_cap, uint32_t nz_cap) {
+  ensure(nb, ns_cap);
+  const size_t S = div_up(nb, 64) + div_up(ns_cap, PRE_STENCILS), G = div_up(nb, BLOCK);
+  part_.reserve(std::max(S, G) + 2 * std::max<size_t>(G, nz_cap) + 8);
+}
+
 void Pcg::launch_minv(const SellMatrix &M, const ContactView &C) {
   hipLaunchKernelGGL(k_minv, dim3(div_up(M.nb, BLOCK)), dim3(BLOCK), 0, s_, M.view(), C, M.diag_slot.p, minv_.p);
 }

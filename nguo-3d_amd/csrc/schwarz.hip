@@ -165,6 +165,15 @@ void Schwarz::build_contact(const SellMatrix &M, uint32_t ns, const uint32_t *do
   contact_built = true;
 }
 
+void Schwarz::reserve(uint32_t nd_cap) {
+  nd_cap = std::max(nd_cap, 1u);
+  ainv.reserve((size_t)nd_cap * SCHWARZ_INV_FLOATS);
+  dom_dof.reserve((size_t)nd_cap * MD);
+  dom_ptr.reserve((size_t)nd_cap + 1);
+  inc_rng.reserve(2 * (size_t)nd_cap * SCHWARZ_MAX_DOMAIN);
+  if (weight > 0.0f) rc.reserve(4 * (size_t)nd_cap);
+}
+
 // Device copies of the domains in h_dom_ptr / h_dom_row (and the coarse level over them).
 void Schwarz::upload(const SellMatrix &M, hipStream_t s) {
   const uint32_t nv = M.nb;

@@ -2266,9 +2266,18 @@ int nguo_step_create(nguo_ctx *ctx, const nguo_scene_desc *sc, nguo_step **out) 
       st->sz.reset(new Schwarz());
       NGUO_REQUIRE(st->prm.schwarz_weight >= 0.0f && std::isfinite(st->prm.schwarz_weight), NGUO_E_INVALID,
                    "Schwarz coarse weight must be finite and >= 0");
+      // contact-built domain sets vary in count: room for twice a full packing of the free
+      // vertices, reserved before build() uploads anything (reserving reallocates)
+      const uint32_t nd_cap = 2 * div_up(st->NF, st->prm.schwarz_domain) + 64;
+      st->sz->reserve(nd_cap);
       st->sz->build(st->act.M, st->prm.schwarz_domain, s, st->prm.schwarz_weight);
       st->pcg->set_schwarz(st->sz.get());
     }
+    // capacities from the mesh, so that the growing contact of a drape does not reallocate:
+    // the step's sorts (incidence keys: 4 per stencil) and the solver's per-stencil and
+    // per-domain buffers
+    st->sorter.reserve_u64(4 * st->cap);
+    if (st->NF) st->pcg->reserve(st->NF, 2 * st->cap, st->sz ? 2 * div_up(st->NF, st->prm.schwarz_domain) + 64 : 0);
     NGUO_HIP(hipHostMalloc((void **)&st->hf, 16 * sizeof(float), hipHostMallocDefault));
     NGUO_HIP(hipHostMalloc((void **)&st->hu, 16 * sizeof(uint32_t), hipHostMallocDefault));
     NGUO_HIP(hipHostMalloc((void **)&st->hl, 4 * sizeof(unsigned long long), hipHostMallocDefault));
