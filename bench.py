@@ -57,7 +57,10 @@ def main():
                     help="adjoint Hessian: exact elastic second derivatives, or the Newton iteration's projected blocks")
     ap.add_argument("--schwarz-domain", type=int, default=None, help="override the scene's Schwarz domain size")
     ap.add_argument("--schwarz-weight", type=float, default=None, help="override the scene's two-level coarse weight (0 = one level)")
-    ap.add_argument("--traffic-bytes", type=float, default=None,
+    # 17.94 MB read + 1.22 MB written per launch of the roofline kernel (the Schwarz apply) on
+    # this workload in contact, from rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by a
+    # 256 MiB add and fill of known traffic
+    ap.add_argument("--traffic-bytes", type=float, default=19.15e6,
                     help="HBM bytes per launch of the roofline kernel, measured with rocprofv3 counter passes")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="after the bench, copy a buffer of this many MiB once (counter calibration for --pmc passes)")
