@@ -356,8 +356,10 @@ __device__ inline bool inside_padded(f3 p, f3 lo, f3 hi, float pad) {
 }
 
 // implicit.rs:1431-1476 collider_sides: the side of each movable non-collider vertex from
-// its closest collider facet within dhat (ties: lowest facet index); +1 otherwise.
-__global__ __launch_bounds__(BLOCK) void k_sides(Topo T, BvhView coll, const float *__restrict__ x, float dhat, float *side, uint32_t *err) {
+// its closest collider facet within dhat (ties: lowest facet index); +1 otherwise. A vertex
+// found inside sets below_any[0] (may be null).
+__global__ __launch_bounds__(BLOCK) void k_sides(Topo T, BvhView coll, const float *__restrict__ x, float dhat, float *side, uint32_t *err,
+                                                uint32_t *below_any) {
   __shared__ uint32_t tstack[TRAV_LDS * BLOCK];
   uint32_t v = blockIdx.x * BLOCK + threadIdx.x;
   if (v >= T.V) return;
@@ -386,7 +388,40 @@ __global__ __launch_bounds__(BLOCK) void k_sides(Topo T, BvhView coll, const flo
     if (best < dhat * dhat) s = best_side;
   }
   side[v] = s;
+  if (below_any && s < 0.0f) atomicOr(below_any, 1u);
 }
+
+// The vertex-triangle broad phase's grid cell (broadphase.rs:209-241 grid_cell and
+// mean_tri_extent): max(dhat, mean over all triangles of the largest side of the triangle's
+// AABB, 1e-6), the mean summed in FP32 in triangle order as the reference sums it. Only a
+// vertex inside a collider within dhat needs it: its signed gap to a collider facet is
+// negative whatever the distance, so the reference's grid cells alone decide which facets it
+// meets. cell[0] is the flag k_sides raised; without it the kernel returns at once. cell[1]
+// receives the cell's FP32 bits. One block: the extents of 4096 triangles at a time go to LDS
+// in parallel, then thread 0 adds them in order.
+constexpr uint32_t CELL_BLOCK = 256, CELL_CHUNK = 4096;
+__global__ __launch_bounds__(CELL_BLOCK) void k_vt_cell(const uint32_t *__restrict__ tris, uint32_t F, const float *__restrict__ x, float dhat,
+                                                       uint32_t *cell) {
+  if (cell[0] == 0) return;
+  __shared__ float ext[CELL_CHUNK];
+  float sum = 0.0f;
+  for (uint32_t base = 0; base < F; base += CELL_CHUNK) {
+    const uint32_t n = min(CELL_CHUNK, F - base);
+    for (uint32_t k = threadIdx.x; k < n; k += CELL_BLOCK) {
+      const size_t t = base + k;
+      f3 a = ld3(x, tris[3 * t]), b = ld3(x, tris[3 * t + 1]), c = ld3(x, tris[3 * t + 2]);
+      f3 d = vmax(vmax(a, b), c) - vmin(vmin(a, b), c);
+      ext[k] = fmaxf(fmaxf(d.x, d.y), d.z);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (uint32_t k = 0; k < n; ++k) sum += ext[k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) cell[1] = __float_as_uint(fmaxf(fmaxf(dhat, F ? sum / (float)F : 0.0f), 1e-6f));
+}
+
+__device__ inline int cell_key(float v, float inv) { return (int)floorf(v * inv); }
 
 struct StencilOut {
   uint32_t cap;
@@ -408,15 +443,30 @@ __device__ inline void emit(const StencilOut &O, uint64_t key, float d, const f3
 }
 
 // implicit.rs:683-717 vertex-triangle stencils: every (v, t) with v in AABB(t) + dhat, not
-// excluded, not both immovable, with (signed for collider facets) distance < dhat.
+// excluded, not both immovable, with (signed for collider facets) distance < dhat. A vertex
+// inside a collider (side -1) meets a collider facet when its grid cell lies in the cells of
+// the facet's padded AABB (broadphase.rs:299-325; the cell from k_vt_cell in cell[1]), as the
+// reference's hash grid offers it; any other pair's gap below dhat puts v inside the padded
+// AABB already, so the box test is the same filter there.
 __global__ __launch_bounds__(BLOCK) void k_vt_stencils(Topo T, BvhView dyn, BvhView stat, const float *__restrict__ x, float dhat,
-                                                      StencilOut O, uint32_t *err) {
+                                                      const uint32_t *__restrict__ cell, StencilOut O, uint32_t *err) {
   __shared__ uint32_t tstack[TRAV_LDS * BLOCK];
   uint32_t v = blockIdx.x * BLOCK + threadIdx.x;
   if (v >= T.V) return;
   f3 p = ld3(x, v);
   bool mv = movable(T, v);
-  const bool below = T.side ? T.side[v] < 0.0f : false;
+  const bool below = T.side && cell ? T.side[v] < 0.0f : false;
+  // an inside vertex's traversal pad covers every facet whose padded cells can hold it: a
+  // facet AABB within dhat + one cell of p, with a second cell of margin for the rounding of
+  // the keys
+  float inv = 0.0f, tpad = dhat;
+  int kx = 0, ky = 0, kz = 0;
+  if (below) {
+    const float c = __uint_as_float(cell[1]);
+    inv = 1.0f / c;
+    tpad = dhat + 2.0f * c;
+    kx = cell_key(p.x, inv); ky = cell_key(p.y, inv); kz = cell_key(p.z, inv);
+  }
   ExcList exl;
   exl.load(T, v);
   // a leaf loads its positions and flags in one memory round trip (the triangle's vertex ids
@@ -427,7 +477,13 @@ __global__ __launch_bounds__(BLOCK) void k_vt_stencils(Topo T, BvhView dyn, BvhV
     f3 a = ld3(x, t0), b = ld3(x, t1), c = ld3(x, t2);
     const float m0 = T.im[t0], m1 = T.im[t1], m2 = T.im[t2];
     const bool st_all = T.static_v ? (T.static_v[t0] && T.static_v[t1] && T.static_v[t2]) : false;
-    if (!inside_padded(p, vmin(vmin(a, b), c), vmax(vmax(a, b), c), dhat)) return;
+    if (st_all && below) {
+      const f3 lo = vmin(vmin(a, b), c) - f3{dhat, dhat, dhat}, hi = vmax(vmax(a, b), c) + f3{dhat, dhat, dhat};
+      if (kx < cell_key(lo.x, inv) || kx > cell_key(hi.x, inv) || ky < cell_key(lo.y, inv) || ky > cell_key(hi.y, inv) ||
+          kz < cell_key(lo.z, inv) || kz > cell_key(hi.z, inv))
+        return;
+    } else if (!inside_padded(p, vmin(vmin(a, b), c), vmax(vmax(a, b), c), dhat))
+      return;
     bool tm = m0 > 0.0f || m1 > 0.0f || m2 > 0.0f;
     if (!tm && !mv) return;
     if (exl.has(T, t0, t1, t2)) return;
@@ -440,7 +496,7 @@ __global__ __launch_bounds__(BLOCK) void k_vt_stencils(Topo T, BvhView dyn, BvhV
     if (d >= dhat) return;
     emit(O, ((uint64_t)v << 32) | t, d, g);
   };
-  auto ov = [&](f3 lo, f3 hi) { return inside_padded(p, lo, hi, dhat); };
+  auto ov = [&](f3 lo, f3 hi) { return inside_padded(p, lo, hi, tpad); };
   // parts 0 .. TRAV_PARTS-1: the movable tree's subtrees; the rest: the static tree's
   const uint32_t part = blockIdx.y;
   if (part < TRAV_PARTS) bvh_traverse_part(dyn, part, err, tstack, ov, leaf);
@@ -1259,6 +1315,7 @@ struct nguo_step {
   double time = 0.0;
   // device state
   DevBuf<float> x, v, x0, xt, dx, dxf, im, rhs, egrad, side, pn, part;
+  DevBuf<uint32_t> vt_cell;  // [0] some vertex is inside a collider, [1] the broad phase's grid cell (FP32 bits)
   DevBuf<uint32_t> vdof, sdof, free_list, tris, edges, ex_ptr, ex, err, flag;
   DevBuf<uint8_t> static_v;
   DevBuf<unsigned long long> tmin;
@@ -1648,15 +1705,19 @@ void detect(nguo_step *st) {
   // dynamic trees at x
   st->tree_tri_dyn.build(0, st->n_tri_dyn, st->tri_dyn_ids.p, st->tris.p, st->x.p, s);
   st->tree_edge_dyn.build(1, st->n_edge_dyn, st->edge_dyn_ids.p, st->edges.p, st->x.p, s);
-  if (st->has_collider)
+  if (st->has_collider) {
+    NGUO_HIP(hipMemsetAsync(st->vt_cell.p, 0, sizeof(uint32_t), s));
     hipLaunchKernelGGL(k_sides, dim3(div_up(st->V, BLOCK)), dim3(BLOCK), 0, s, T, st->tree_tri_coll.view(), (const float *)st->x.p, dhat,
-                       st->side.p, st->err.p);
+                       st->side.p, st->err.p, st->vt_cell.p);
+    hipLaunchKernelGGL(k_vt_cell, dim3(1), dim3(CELL_BLOCK), 0, s, (const uint32_t *)st->tris.p, st->F, (const float *)st->x.p, dhat,
+                       st->vt_cell.p);
+  }
   for (int attempt = 0; attempt < 4; ++attempt) {
     NGUO_HIP(hipMemsetAsync(st->vt_count.p, 0, 2 * sizeof(uint32_t), s));
     StencilOut ovt{st->cap, st->vt_count.p, st->vt_key.p, st->vt_ord.p, st->vt_d.p, st->vt_g.p};
     StencilOut oee{st->cap, st->vt_count.p + 1, st->ee_key.p, st->ee_ord.p, st->ee_d.p, st->ee_g.p};
     hipLaunchKernelGGL(k_vt_stencils, dim3(div_up(st->V, BLOCK), 2 * TRAV_PARTS), dim3(BLOCK), 0, s, T, st->tree_tri_dyn.view(), st->tree_tri_stat.view(),
-                       (const float *)st->x.p, dhat, ovt, st->err.p);
+                       (const float *)st->x.p, dhat, st->has_collider ? (const uint32_t *)st->vt_cell.p : nullptr, ovt, st->err.p);
     if (st->n_edge_dyn)
       hipLaunchKernelGGL(k_ee_stencils, dim3(div_up(st->n_edge_dyn, BLOCK), TRAV_PARTS), dim3(BLOCK), 0, s, T, st->tree_edge_dyn.view(),
                          (const uint32_t *)st->edge_dyn_ids.p, st->n_edge_dyn, 1, (const float *)st->x.p, dhat, dhat, oee, st->err.p);
@@ -2236,6 +2297,7 @@ int nguo_step_create(nguo_ctx *ctx, const nguo_scene_desc *sc, nguo_step **out) 
     }
     st->pn.upload(pn, s);
     st->side.reserve(V);
+    st->vt_cell.reserve(2);
     {
       std::vector<float> ones(V, 1.0f);
       st->side.upload(ones, s);
@@ -2395,7 +2457,7 @@ int nguo_step_collider_sides(nguo_step *st, const float *x, float *side) {
     }
     st->x.upload(x, 3 * (size_t)st->V, s);
     hipLaunchKernelGGL(k_sides, dim3(div_up(st->V, BLOCK)), dim3(BLOCK), 0, s, topo(st), st->tree_tri_coll.view(), (const float *)st->x.p,
-                       st->prm.contact_dhat, st->side.p, st->err.p);
+                       st->prm.contact_dhat, st->side.p, st->err.p, nullptr);
     NGUO_HIP(hipGetLastError());
     st->side.download(side, st->V, s);
     NGUO_HIP(hipMemcpyAsync(st->x.p, st->x0.p, 3 * (size_t)st->V * sizeof(float), hipMemcpyDeviceToDevice, s));

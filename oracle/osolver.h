@@ -569,8 +569,23 @@ template <class T> class Solver {
   T dhat() const { return T(prm.contact_dhat); }
 
   // ---------------------------------------------------------------- candidate sets
-  // All (v, t) with t over `tl`, v inside AABB(t) padded by `pad`, not excluded, and not
-  // both immovable (broadphase.rs:269-329), each pair once, in ascending (t, v) order.
+  // Grid cell of a point: exact integer coordinates (no hashing, so no two cells share a list).
+  struct Cell {
+    int64_t a, b, c;
+    bool operator==(const Cell &o) const { return a == o.a && b == o.b && c == o.c; }
+  };
+  struct CellHash {
+    size_t operator()(const Cell &k) const {
+      return (size_t)((uint64_t)(k.a * 73856093) ^ (uint64_t)(k.b * 19349663) ^ (uint64_t)(k.c * 83492791));
+    }
+  };
+
+  // All (v, t) with t over `tl` whose vertex v lies in a grid cell that AABB(t) padded by
+  // `pad` touches (cell = max(pad, mean triangle extent, 1e-6), keys floor(coordinate / cell)),
+  // not excluded, and not both immovable (broadphase.rs:269-329), each pair once, in
+  // ascending (t, v) order. The test is on cells, as the reference's, not on the padded box:
+  // an unsigned gap below `pad` lies inside the box anyway, but a signed (negative) gap to a
+  // collider facet does not, and such a facet is a candidate exactly when its cells are.
   template <class F> void for_each_vt(const std::vector<V> &x, const std::vector<Tri> &tl, T pad, F visit) const {
     if (x.empty() || tl.empty()) return;
     // grid over vertices with cell = max(pad, mean tri extent)
@@ -583,12 +598,9 @@ template <class T> class Solver {
     T cell = std::fmax(std::fmax(pad, ext_sum / T(tl.size())), T(1e-6));
     T inv = T(1) / cell;
     auto key = [&](T v) { return (int64_t)std::floor(v * inv); };
-    auto hk = [](int64_t a, int64_t b, int64_t c) {
-      return (uint64_t)(a * 73856093) ^ (uint64_t)(b * 19349663) ^ (uint64_t)(c * 83492791);
-    };
-    std::unordered_map<uint64_t, std::vector<uint32_t>> grid;
+    std::unordered_map<Cell, std::vector<uint32_t>, CellHash> grid;
     grid.reserve(x.size() * 2);
-    for (uint32_t i = 0; i < x.size(); ++i) grid[hk(key(x[i].x), key(x[i].y), key(x[i].z))].push_back(i);
+    for (uint32_t i = 0; i < x.size(); ++i) grid[Cell{key(x[i].x), key(x[i].y), key(x[i].z)}].push_back(i);
     std::vector<uint32_t> cand;
     for (uint32_t ti = 0; ti < tl.size(); ++ti) {
       const Tri &t = tl[ti];
@@ -599,11 +611,9 @@ template <class T> class Solver {
       for (int64_t cx = key(lo.x); cx <= key(hi.x); ++cx)
         for (int64_t cy = key(lo.y); cy <= key(hi.y); ++cy)
           for (int64_t cz = key(lo.z); cz <= key(hi.z); ++cz) {
-            auto it = grid.find(hk(cx, cy, cz));
+            auto it = grid.find(Cell{cx, cy, cz});
             if (it == grid.end()) continue;
             for (uint32_t v : it->second) {
-              V p = x[v];
-              if (p.x < lo.x || p.y < lo.y || p.z < lo.z || p.x > hi.x || p.y > hi.y || p.z > hi.z) continue;
               if (!tm && !movable(v)) continue;
               if (excluded(v, t)) continue;
               cand.push_back(v);

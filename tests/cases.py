@@ -31,3 +31,44 @@ def scene(cfg, **params):
     for k, v in params.items():
         setattr(s.desc.params, k, v)
     return s
+
+
+def inside_state(steps=30, shallow=0.4, deep=2.5):
+    """The 10 x 10 sheet resting on the icosphere after `steps` oracle steps, with two free
+    vertices moved inside the body: one `shallow` * dhat behind the centroid of a facet it
+    touches (within dhat of the surface, so its side is -1 and its gaps to collider facets are
+    signed), one `deep` * dhat behind another facet (deeper than dhat: side +1, unsigned).
+    Returns (scene, x, v, shallow vertex, deep vertex, their facets)."""
+    import oracle_py as O
+
+    s = nguo_hip.Scene(100, seed=0)
+    o = O.OracleSolver(s)
+    o.step(steps)
+    x, v = o.positions().copy(), o.velocities()
+    static = s.static.astype(bool)
+    tris = s.tris.astype(np.int64)
+    facets = np.nonzero(static[tris].all(axis=1))[0]
+    free = np.nonzero(s.inv_mass > 0)[0]
+    # each free vertex's nearest collider facet, by its centroid distance
+    cen = x[tris[facets]].astype(np.float64).mean(axis=1)
+    dist = np.linalg.norm(x[free].astype(np.float64)[:, None, :] - cen[None, :, :], axis=2)
+    near = dist.argmin(axis=1)
+    order = np.argsort(dist[np.arange(len(free)), near])
+    centre = s.positions[static].astype(np.float64).mean(axis=0)
+    dhat = float(s.params.contact_dhat)
+    picked = []
+    for k in order:
+        vert, tri = int(free[k]), tuple(int(a) for a in tris[facets[near[k]]])
+        if any(vert == p[0] or tri == p[1] or set(tri) & set(p[1]) for p in picked):
+            continue
+        picked.append((vert, tri))
+        if len(picked) == 2:
+            break
+    for (vert, tri), depth in zip(picked, (shallow, deep)):
+        a, b, c = (x[k].astype(np.float64) for k in tri)
+        n = np.cross(b - a, c - a)
+        n /= np.linalg.norm(n)
+        if n @ ((a + b + c) / 3 - centre) < 0:
+            n = -n
+        x[vert] = ((a + b + c) / 3 - depth * dhat * n).astype(np.float32)
+    return s, x, v, picked[0][0], picked[1][0], picked[0][1], picked[1][1]

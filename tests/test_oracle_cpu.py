@@ -479,47 +479,6 @@ def test_free_fall_is_exact_inertial_target():
     assert np.abs(x1 - want).max() < 1e-7
 
 
-def brute_vt_stencils(o, x, dhat, scene):
-    """All vertex-triangle pairs by brute force with the oracle's element distance."""
-    tris = scene.tris
-    im = scene.inv_mass
-    n = len(x)
-    nbr = [set([i]) for i in range(n)]
-    for t in tris:
-        for a in t:
-            nbr[a] |= set(int(b) for b in t)
-    static = scene.static.astype(bool)
-    out = []
-    for v in range(n):
-        for ti, t in enumerate(tris):
-            if not (im[v] > 0 or (im[t] > 0).any()):
-                continue
-            if nbr[v] & set(int(b) for b in t):
-                continue
-            d = lib.oracle_point_tri_distance(0, O.dp(O.d64(np.concatenate([x[v], x[t[0]], x[t[1]], x[t[2]]]))))
-            if static[t].all():
-                pass  # signed: sign only matters inside; this scene keeps every vertex outside
-            if d < dhat:
-                out.append((v, int(t[0]), int(t[1]), int(t[2])))
-    return sorted(out)
-
-
-def test_vt_stencils_match_brute_force_tiny():
-    s = nguo_hip.Scene(100, seed=0)
-    o = O.OracleSolver(s)
-    o.step(30)  # sheet on the sphere
-    x = o.positions()
-    idx, d, g = o.stencils(x)
-    vt = [tuple(int(a) for a in r) for r, dd in zip(idx, d) if True]
-    dhat = s.params.contact_dhat
-    brute = brute_vt_stencils(o, x, dhat, s)
-    vt_part = [r for r in vt if r in set(brute)]
-    assert len(brute) > 0
-    assert sorted(vt_part) == brute
-    side = o.collider_sides(x)
-    assert side is not None and np.all(side == 1)  # nothing starts inside the sphere
-
-
 def test_oracle_drape_invariants_small():
     s = nguo_hip.Scene(100, seed=0)
     o = O.OracleSolver(s)

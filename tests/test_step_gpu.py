@@ -6,7 +6,7 @@ import pytest
 
 from conftest import nguo_hip
 import oracle_py as O
-from cases import scene
+from cases import inside_state, scene
 
 POS_TOL = 1e-5  # |x_gpu - x_oracle|_inf / max(1, |x|_inf) after each Newton iteration / step
 
@@ -50,11 +50,27 @@ def oracle_state(cfg, steps, cg_tol=None):
 @pytest.mark.parametrize("cfg,steps", [(100, 30), (0, 16), (0, 30), (101, 25)])
 def test_stencils_bitexact(ctx, cfg, steps):
     s, o = oracle_state(cfg, steps)
-    x = o.positions()
+    check_stencils(ctx, s, o, o.positions(), o.velocities())
+
+
+@pytest.mark.gpu
+def test_stencils_bitexact_inside_collider(ctx):
+    # a vertex 0.4 dhat behind a collider facet (side -1) meets every collider facet of its
+    # grid cells at a negative signed gap; one 2.5 dhat behind meets none (side +1)
+    s, x, v, vs, vd, _, _ = inside_state()
+    o = O.OracleSolver(s)
+    side = o.collider_sides(x)
+    assert side[vs] == -1 and side[vd] == 1
+    idx, d = check_stencils(ctx, s, o, x, v)
+    static = s.static.astype(bool)
+    assert sum(1 for r, dd in zip(idx, d) if r[0] == vs and static[r[1:]].all() and dd < 0) >= 2
+
+
+def check_stencils(ctx, s, o, x, v):
     idx, d, g = o.stencils(x)
     assert len(idx) > 0, "the state must be in contact"
     st = nguo_hip.Stepper(ctx, s)
-    st.set_state(x, o.velocities())
+    st.set_state(x, v)
     gidx, gd, gg, gk = st.stencils(x)
     assert np.array_equal(gidx, idx)
     assert np.array_equal(gd, d)  # same FP32 operation order: bitwise equal gaps
@@ -67,6 +83,7 @@ def test_stencils_bitexact(ctx, cfg, steps):
     side = o.collider_sides(x)
     assert np.array_equal(st.collider_sides(x), side)
     st.close()
+    return idx, d
 
 
 # Both sides solve each Newton system to 1e-10 relative residual here, so the comparison
