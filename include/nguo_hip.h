@@ -81,6 +81,10 @@ typedef struct nguo_params {
   uint32_t schwarz_domain;/* vertices per Schwarz domain (1..32) */
   float schwarz_weight;   /* coarse weight of the two-level Schwarz form; 0 = one level
                              (engine-core/src/linsolve.rs:522-657, GpuPrecond::SchwarzMl) */
+  float stretch_k;        /* edge-spring stiffness k over every unique triangle edge, rest
+                             lengths from the scene positions (ImplicitParams::stiffness over
+                             SimState::stretch, implicit.rs:2487-2507, spring_block
+                             :3240-3261); 0 = no springs */
 } nguo_params;
 
 /* A scene in the solver's vertex space (engine-api/src/scene.rs:89-152 `lower` output plus
@@ -213,10 +217,11 @@ int nguo_step_collider_sides(nguo_step *st, const float *x, float *side);
  * (shell_model.rs:112-127), hinges (bending.rs:105-123), rod stretch (implicit.rs:3287-3308),
  * strand bends (rod_bend.rs:1-20), stencil barriers (contact.rs:580-591, 1014-1025), stencil
  * friction potentials (contact.rs:613-706 integrated with the normal force held), analytic
- * colliders (barrier + friction, contact.rs:1082-1125); total (may be NULL) their sum. x
+ * colliders (barrier + friction, contact.rs:1082-1125), edge springs 1/2 k (l - l0)^2
+ * (implicit.rs:2487-2507); total (may be NULL) their sum. x
  * must keep the state's pinned positions. No reference counterpart: the reference logs only
  * the kinetic energy (implicit.rs:2183-2190, state.rs:84-92). State is unchanged. */
-#define NGUO_ENERGY_TERMS 8
+#define NGUO_ENERGY_TERMS 9
 int nguo_step_energy(nguo_step *st, const float *x, double *terms, double *total);
 
 /* ---- drape adjoint (the differentiation form, C4). No reference counterpart: the reference

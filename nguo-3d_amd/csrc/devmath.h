@@ -504,6 +504,29 @@ __device__ inline void rod_stretch_block(f3 d, float rest, float k, float h[9], 
   ga = nrm * (k * (l - rest));
 }
 
+// implicit.rs:3240-3261 spring_block: an edge spring's 3x3 block and the gradient on a for
+// d = x_a - x_b; the tangential coefficient 1 - rest/l is clamped at 0 (project = false: the
+// exact second derivative).
+__device__ inline void spring_block(f3 d, float rest, float k, float h[9], f3 &ga, bool project = true) {
+  float l = len(d);
+  if (l < 1.0e-9f) {
+    for (int i = 0; i < 9; ++i) h[i] = 0.0f;
+    h[0] = k; h[4] = k; h[8] = k;
+    ga = f3{0, 0, 0};
+    return;
+  }
+  f3 nrm = d / l;
+  float coeff_t = 1.0f - rest / l;
+  if (project) coeff_t = fmaxf(coeff_t, 0.0f);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      float nnt = comp(nrm, i) * comp(nrm, j);
+      float iden = i == j ? 1.0f : 0.0f;
+      h[3 * i + j] = k * (nnt + coeff_t * (iden - nnt));
+    }
+  ga = nrm * (k * (l - rest));
+}
+
 // rod_bend.rs:81-175 strand_force_hessian (unit stiffness).
 __device__ inline void strand_force_hessian(f3 x0, f3 x1, f3 x2, float rest_angle, f3 grad[3], float *hess) {
   f3 e0 = x0 - x1, e1 = x2 - x1;

@@ -222,6 +222,7 @@ nguo_params default_params() {
   p.precond = NGUO_PRECOND_SCHWARZ;
   p.schwarz_domain = 32;
   p.schwarz_weight = 0.0f;
+  p.stretch_k = 0.0f;
   return p;
 }
 

@@ -1,10 +1,10 @@
 // The resident implicit step: engine-core/src/implicit.rs:1951-2191 ImplicitSolver::step
 // (and the wgpu GpuImplicitSolver it generalises, engine-gpu/src/gpu_implicit.rs:94-659)
 // on one MI355X. Per Newton iteration, all on the device:
-//   1. element kernels (shells, hinges, rods, rod bends) write local blocks;
+//   1. element kernels (shells, hinges, rods, rod bends, edge springs) write local blocks;
 //   2. a fixed-order gather sums them into the SELL-64 3x3 BSR over the free vertices
-//      (push order of implicit.rs:2392-2740: inertia, rods, rod bends, shells, hinges,
-//      analytic colliders) - no float atomics, bitwise deterministic;
+//      (push order of implicit.rs:2392-2740: inertia, rods, rod bends, edge springs, shells,
+//      hinges, analytic colliders) - no float atomics, bitwise deterministic;
 //   3. LBVH queries produce the active vertex-triangle and edge-edge stencils, sorted by
 //      (v, t) / (ei, ej) exactly as implicit.rs:678-753 orders them;
 //   4. per stencil: Rayleigh stiffness from the assembled elastic blocks, barrier (FP64),
@@ -39,21 +39,24 @@ __host__ __device__ inline uint32_t code(uint32_t type, uint32_t payload) { retu
 
 // ---------------------------------------------------------------- element sets
 struct ElemDev {
-  uint32_t ns, nh, nr, nrh;
+  uint32_t ns, nh, nr, nrh, nsp;
   const uint32_t *sh_v;  const float *sh_dm; const float *sh_area;
   const float *sh_scale;  // rest-shape scale per shell (Dm^-1 / s, area s^2)
   const uint32_t *hg_v;  const float *hg_rest;
   const uint32_t *rd_v;  const float *rd_rest; const float *rd_k;
   const uint32_t *rh_v;  const float *rh_rest; const float *rh_stiff;
+  const uint32_t *sp_v;  const float *sp_rest;  // edge springs (stiffness Params::ksp)
+  const double *sp_rest64;  // their rest lengths in FP64 (energy evaluation)
   float *out_dense;  // 81 floats per rod bend, then per shell (9 blocks of 9)
   float *out_dg;     // 9 floats per rod bend, then per shell
-  float *out_rod;    // 12 per rod: hk (9) + ga (3)
+  float *out_rod;    // 12 per rod, then per edge spring: hk (9) + ga (3)
   float *out_hinge;  // 14 per hinge: kwt, kwt*(theta-rest), dtheta/dx (12)
 };
 
 struct Params {
   int shell_model;
   float shell_mu, shell_lambda, eig_eps, kb;
+  float ksp;  // edge-spring stiffness
   float inv_h2;
   float dhat, kappa, mu, feps;
   int barrier_kind;
@@ -139,13 +142,21 @@ __global__ __launch_bounds__(BLOCK) void k_hinges(ElemDev E, Params P, const flo
   for (int a = 0; a < 4; ++a) { o[2 + 3 * a] = g[a].x; o[3 + 3 * a] = g[a].y; o[4 + 3 * a] = g[a].z; }
 }
 
-__global__ __launch_bounds__(BLOCK) void k_rods(ElemDev E, int project, const float *__restrict__ x) {
+// Rod segments (implicit.rs:2407-2424), then the edge springs (implicit.rs:2487-2507), one
+// thread per two-vertex element.
+__global__ __launch_bounds__(BLOCK) void k_rods(ElemDev E, int project, float ksp, const float *__restrict__ x) {
   uint32_t e = blockIdx.x * BLOCK + threadIdx.x;
-  if (e >= E.nr) return;
-  uint32_t a = E.rd_v[2 * (size_t)e], b = E.rd_v[2 * (size_t)e + 1];
+  if (e >= E.nr + E.nsp) return;
   float h[9];
   f3 ga;
-  rod_stretch_block(ld3(x, a) - ld3(x, b), E.rd_rest[e], E.rd_k[e], h, ga, project != 0);
+  if (e < E.nr) {
+    uint32_t a = E.rd_v[2 * (size_t)e], b = E.rd_v[2 * (size_t)e + 1];
+    rod_stretch_block(ld3(x, a) - ld3(x, b), E.rd_rest[e], E.rd_k[e], h, ga, project != 0);
+  } else {
+    const uint32_t sp = e - E.nr;
+    uint32_t a = E.sp_v[2 * (size_t)sp], b = E.sp_v[2 * (size_t)sp + 1];
+    spring_block(ld3(x, a) - ld3(x, b), E.sp_rest[sp], ksp, h, ga, project != 0);
+  }
   float *o = E.out_rod + 12 * (size_t)e;
   for (int i = 0; i < 9; ++i) o[i] = h[i];
   o[9] = ga.x; o[10] = ga.y; o[11] = ga.z;
@@ -1079,13 +1090,13 @@ using namespace nguo;
 // potentials of the active stencils and of the analytic colliders. Each term is the one
 // whose gradient the assembly adds (implicit.rs:2392-3103): inertia 1/2 m/h^2 |x - x~|^2,
 // shells area psi(sigma) (shell_model.rs:112-127), hinges 1/2 k_b (theta - theta0)^2
-// (bending.rs:105-123), rods 1/2 k (l - l0)^2 (implicit.rs:3287-3308), strand bends
+// (bending.rs:105-123), rods and edge springs 1/2 k (l - l0)^2 (implicit.rs:3287-3308, :1-10), strand bends
 // 1/2 k (theta - theta0)^2 (rod_bend.rs:1-20), barriers (contact.rs:580-591, 1014-1025) and the
 // smoothed Coulomb potential c f0(|P u|) whose gradient is contact.rs:613-706. Everything is
 // evaluated in FP64 from the FP32 positions (gaps from the stencils' own witness weights,
 // Rayleigh stiffnesses from the assembled blocks, shells from FP64 rest frames), so the value
 // is that of the FP64 scheme at these positions. It is a diagnostic, run on request only.
-enum : int { EN_INERTIA, EN_SHELL, EN_HINGE, EN_ROD, EN_ROD_BEND, EN_BARRIER, EN_FRICTION, EN_COLLIDER, EN_TERMS };
+enum : int { EN_INERTIA, EN_SHELL, EN_HINGE, EN_ROD, EN_ROD_BEND, EN_BARRIER, EN_FRICTION, EN_COLLIDER, EN_SPRING, EN_TERMS };
 
 // Sums each term over the workgroup's wavefronts into acc (all lanes active).
 __device__ inline void energy_commit(const double e[EN_TERMS], double *acc) {
@@ -1119,7 +1130,7 @@ __device__ inline double stencil_geom64(bool vt, const uint32_t id[4], const flo
 __global__ __launch_bounds__(BLOCK) void k_energy_verts(uint32_t V, const float *__restrict__ im, const float *__restrict__ x,
                                                        const float *__restrict__ xt, const float *__restrict__ xs, double inv_h2, Params P,
                                                        double *acc) {
-  double e[EN_TERMS] = {0, 0, 0, 0, 0, 0, 0, 0};
+  double e[EN_TERMS] = {};
   const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
   if (i < V && im[i] > 0.0f) {
     const d3 xi = to_d(ld3(x, i)), dv = xi - to_d(ld3(xt, i));
@@ -1148,12 +1159,13 @@ __global__ __launch_bounds__(BLOCK) void k_energy_verts(uint32_t V, const float 
   energy_commit(e, acc);
 }
 
-// Shells, hinges, rod segments and strand bends of one element set, one thread per element.
+// Shells, hinges, rod segments, strand bends and edge springs of one element set, one thread
+// per element.
 __global__ __launch_bounds__(BLOCK) void k_energy_elems(ElemDev E, const double *__restrict__ dm64, const double *__restrict__ area64, Params P,
                                                        const float *__restrict__ x, double *acc) {
-  double e[EN_TERMS] = {0, 0, 0, 0, 0, 0, 0, 0};
+  double e[EN_TERMS] = {};
   const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-  const uint32_t b1 = E.ns, b2 = b1 + E.nh, b3 = b2 + E.nr, b4 = b3 + E.nrh;
+  const uint32_t b1 = E.ns, b2 = b1 + E.nh, b3 = b2 + E.nr, b4 = b3 + E.nrh, b5 = b4 + E.nsp;
   if (i < b1) {
     const uint32_t *v = E.sh_v + 3 * (size_t)i;
     const d3 p0 = to_d(ld3(x, v[0])), e1 = to_d(ld3(x, v[1])) - p0, e2 = to_d(ld3(x, v[2])) - p0;
@@ -1189,6 +1201,10 @@ __global__ __launch_bounds__(BLOCK) void k_energy_elems(ElemDev E, const double 
       const double w = atan2(len(cross(e0, e1)), dot(e0, e1)) - (double)E.rh_rest[r];
       e[EN_ROD_BEND] = 0.5 * (double)stiff * w * w;
     }
+  } else if (i < b5) {
+    const uint32_t sp = i - b4;
+    const double l = len(to_d(ld3(x, E.sp_v[2 * (size_t)sp])) - to_d(ld3(x, E.sp_v[2 * (size_t)sp + 1]))) - E.sp_rest64[sp];
+    e[EN_SPRING] = 0.5 * (double)P.ksp * l * l;
   }
   energy_commit(e, acc);
 }
@@ -1201,7 +1217,7 @@ __global__ __launch_bounds__(BLOCK) void k_energy_contacts(uint32_t ns, const ui
                                                           float ghat_vt, float ghat_ee, uint32_t nvt, Topo T,
                                                           SellView K, SellView S, Params P, const float *__restrict__ x,
                                                           const float *__restrict__ xs, double *acc) {
-  double e[EN_TERMS] = {0, 0, 0, 0, 0, 0, 0, 0};
+  double e[EN_TERMS] = {};
   const uint32_t s = blockIdx.x * BLOCK + threadIdx.x;
   if (s < ns) {
     uint32_t id[4];
@@ -1270,22 +1286,23 @@ __global__ __launch_bounds__(BLOCK) void k_energy_contacts(uint32_t ns, const ui
 }
 
 struct ElemHost {
-  std::vector<uint32_t> sh_v, hg_v, rd_v, rh_v;
+  std::vector<uint32_t> sh_v, hg_v, rd_v, rh_v, sp_v;
   std::vector<uint32_t> sh_face;  // scene shell face index of each shell
-  std::vector<float> sh_dm, sh_area, hg_rest, rd_rest, rd_k, rh_rest, rh_stiff;
-  std::vector<double> sh_dm64, sh_area64;  // the rest frames in FP64 (energy evaluation)
+  std::vector<float> sh_dm, sh_area, hg_rest, rd_rest, rd_k, rh_rest, rh_stiff, sp_rest;
+  std::vector<double> sh_dm64, sh_area64, sp_rest64;  // rest frames and lengths in FP64 (energy evaluation)
   uint32_t ns() const { return (uint32_t)sh_area.size(); }
   uint32_t nh() const { return (uint32_t)hg_rest.size(); }
   uint32_t nr() const { return (uint32_t)rd_rest.size(); }
   uint32_t nrh() const { return (uint32_t)rh_rest.size(); }
+  uint32_t nsp() const { return (uint32_t)sp_rest.size(); }
 };
 
 struct ElemSet {
-  uint32_t ns = 0, nh = 0, nr = 0, nrh = 0;
-  DevBuf<uint32_t> sh_v, hg_v, rd_v, rh_v;
-  DevBuf<float> sh_dm, sh_area, hg_rest, rd_rest, rd_k, rh_rest, rh_stiff;
+  uint32_t ns = 0, nh = 0, nr = 0, nrh = 0, nsp = 0;
+  DevBuf<uint32_t> sh_v, hg_v, rd_v, rh_v, sp_v;
+  DevBuf<float> sh_dm, sh_area, hg_rest, rd_rest, rd_k, rh_rest, rh_stiff, sp_rest;
   DevBuf<float> sh_scale;
-  DevBuf<double> sh_dm64, sh_area64;
+  DevBuf<double> sh_dm64, sh_area64, sp_rest64;
   DevBuf<uint32_t> sh_face;
   std::vector<uint32_t> h_sh_face;
   std::vector<float> h_sh_scale;  // host copy (kept alive for the asynchronous upload)
@@ -1293,8 +1310,8 @@ struct ElemSet {
   SellMatrix M;
   DevBuf<uint32_t> slot_row, hc_ptr, hc, gc_ptr, gc, row_vert;
   ElemDev dev() const {
-    return ElemDev{ns, nh, nr, nrh, sh_v.p, sh_dm.p, sh_area.p, sh_scale.p, hg_v.p, hg_rest.p, rd_v.p, rd_rest.p, rd_k.p,
-                   rh_v.p, rh_rest.p, rh_stiff.p, out_dense.p, out_dg.p, out_rod.p, out_hinge.p};
+    return ElemDev{ns, nh, nr, nrh, nsp, sh_v.p, sh_dm.p, sh_area.p, sh_scale.p, hg_v.p, hg_rest.p, rd_v.p, rd_rest.p, rd_k.p,
+                   rh_v.p, rh_rest.p, rh_stiff.p, sp_v.p, sp_rest.p, sp_rest64.p, out_dense.p, out_dg.p, out_rod.p, out_hinge.p};
   }
 };
 
@@ -1516,6 +1533,7 @@ void build_gather(ElemSet &S, const ElemHost &H, uint32_t nrows, const std::vect
   };
   for (uint32_t e = 0; e < H.nr(); ++e) couple(&H.rd_v[2 * e], 2);
   for (uint32_t e = 0; e < H.nrh(); ++e) couple(&H.rh_v[3 * e], 3);
+  for (uint32_t e = 0; e < H.nsp(); ++e) couple(&H.sp_v[2 * e], 2);
   for (uint32_t e = 0; e < H.ns(); ++e) couple(&H.sh_v[3 * e], 3);
   for (uint32_t e = 0; e < H.nh(); ++e) couple(&H.hg_v[4 * e], 4);
   for (auto &c : cols) { std::sort(c.begin(), c.end()); c.erase(std::unique(c.begin(), c.end()), c.end()); }
@@ -1528,9 +1546,9 @@ void build_gather(ElemSet &S, const ElemHost &H, uint32_t nrows, const std::vect
   std::vector<std::pair<uint32_t, uint32_t>> hlist;
   std::vector<std::pair<uint32_t, uint32_t>> glist;
   const uint32_t nrh = H.nrh();
-  for (uint32_t e = 0; e < H.nr(); ++e) {
-    uint32_t a = H.rd_v[2 * e], b = H.rd_v[2 * e + 1];
-    uint32_t ra = vmap[a], rb = vmap[b];
+  // a two-vertex element's blocks [[H, -H], [-H, H]] and gradient (+g, -g), out_rod entry e
+  auto pair = [&](const uint32_t *v, uint32_t e) {
+    uint32_t ra = vmap[v[0]], rb = vmap[v[1]];
     if (ra != NONE) hlist.push_back({S.M.find(ra, ra), code(C_ROD_POS, e)});
     if (rb != NONE) hlist.push_back({S.M.find(rb, rb), code(C_ROD_POS, e)});
     if (ra != NONE && rb != NONE) {
@@ -1539,7 +1557,8 @@ void build_gather(ElemSet &S, const ElemHost &H, uint32_t nrows, const std::vect
     }
     if (ra != NONE) glist.push_back({ra, code(C_ROD_POS, e)});
     if (rb != NONE) glist.push_back({rb, code(C_ROD_NEG, e)});
-  }
+  };
+  for (uint32_t e = 0; e < H.nr(); ++e) pair(&H.rd_v[2 * e], e);
   auto dense = [&](const uint32_t *v, uint32_t elem_base) {
     for (int a = 0; a < 3; ++a) {
       uint32_t ra = vmap[v[a]];
@@ -1552,6 +1571,7 @@ void build_gather(ElemSet &S, const ElemHost &H, uint32_t nrows, const std::vect
       if (vmap[v[a]] != NONE) glist.push_back({vmap[v[a]], code(C_DENSE, elem_base * 3 + a)});
   };
   for (uint32_t e = 0; e < nrh; ++e) dense(&H.rh_v[3 * e], e);
+  for (uint32_t e = 0; e < H.nsp(); ++e) pair(&H.sp_v[2 * e], H.nr() + e);
   for (uint32_t e = 0; e < H.ns(); ++e) dense(&H.sh_v[3 * e], nrh + e);
   for (uint32_t e = 0; e < H.nh(); ++e) {
     const uint32_t *v = &H.hg_v[4 * e];
@@ -1587,7 +1607,7 @@ void build_gather(ElemSet &S, const ElemHost &H, uint32_t nrows, const std::vect
 }
 
 void upload_elems(ElemSet &S, const ElemHost &H, hipStream_t s) {
-  S.ns = H.ns(); S.nh = H.nh(); S.nr = H.nr(); S.nrh = H.nrh();
+  S.ns = H.ns(); S.nh = H.nh(); S.nr = H.nr(); S.nrh = H.nrh(); S.nsp = H.nsp();
   auto up = [&](auto &buf, const auto &vec) {
     using T = typename std::decay_t<decltype(vec)>::value_type;
     if (vec.empty()) buf.reserve(1);
@@ -1603,15 +1623,16 @@ void upload_elems(ElemSet &S, const ElemHost &H, hipStream_t s) {
   up(S.hg_v, H.hg_v); up(S.hg_rest, H.hg_rest);
   up(S.rd_v, H.rd_v); up(S.rd_rest, H.rd_rest); up(S.rd_k, H.rd_k);
   up(S.rh_v, H.rh_v); up(S.rh_rest, H.rh_rest); up(S.rh_stiff, H.rh_stiff);
+  up(S.sp_v, H.sp_v); up(S.sp_rest, H.sp_rest); up(S.sp_rest64, H.sp_rest64);
   S.out_dense.reserve(81 * ((size_t)S.nrh + S.ns) + 1);
   S.out_dg.reserve(9 * ((size_t)S.nrh + S.ns) + 1);
-  S.out_rod.reserve(12 * (size_t)S.nr + 1);
+  S.out_rod.reserve(12 * ((size_t)S.nr + S.nsp) + 1);
   S.out_hinge.reserve(14 * (size_t)S.nh + 1);
 }
 
 void run_elements(ElemSet &S, const Params &P, const float *x, hipStream_t s) {
   ElemDev E = S.dev();
-  if (S.nr) hipLaunchKernelGGL(k_rods, dim3(div_up(S.nr, BLOCK)), dim3(BLOCK), 0, s, E, P.project, x);
+  if (S.nr + S.nsp) hipLaunchKernelGGL(k_rods, dim3(div_up(S.nr + S.nsp, BLOCK)), dim3(BLOCK), 0, s, E, P.project, P.ksp, x);
   if (S.nrh) hipLaunchKernelGGL(k_rod_hinges, dim3(div_up(S.nrh, BLOCK)), dim3(BLOCK), 0, s, E, x);
   if (S.ns) hipLaunchKernelGGL(k_shells, dim3(div_up(S.ns, BLOCK)), dim3(BLOCK), 0, s, E, P, x);
   if (S.nh) hipLaunchKernelGGL(k_hinges, dim3(div_up(S.nh, BLOCK)), dim3(BLOCK), 0, s, E, P, x);
@@ -2019,6 +2040,8 @@ int nguo_step_create(nguo_ctx *ctx, const nguo_scene_desc *sc, nguo_step **out) 
     st->V = V;
     st->prm = sc->params;
     NGUO_REQUIRE(sc->n_colliders <= 4, NGUO_E_UNSUPPORTED, "at most 4 analytic colliders");
+    NGUO_REQUIRE(sc->params.stretch_k >= 0.0f && std::isfinite(sc->params.stretch_k), NGUO_E_INVALID,
+                 "edge-spring stiffness must be finite and >= 0");
     // validation (implicit.rs:826-838 style)
     for (uint32_t i = 0; i < 3 * sc->n_tris; ++i) NGUO_REQUIRE(sc->tris[i] < V, NGUO_E_INVALID, "triangle index out of range");
     for (uint32_t i = 0; i < 3 * sc->n_shell_faces; ++i) NGUO_REQUIRE(sc->shell_faces[i] < V, NGUO_E_INVALID, "shell face index out of range");
@@ -2047,6 +2070,7 @@ int nguo_step_create(nguo_ctx *ctx, const nguo_scene_desc *sc, nguo_step **out) 
     P.shell_lambda = st->prm.shell_lambda;
     P.eig_eps = st->prm.eiganalysis_eps;
     P.kb = st->prm.bend_stiffness;
+    P.ksp = st->prm.stretch_k;
     P.dhat = st->prm.contact_dhat;
     P.kappa = st->prm.contact_kappa;
     P.mu = st->prm.friction;
@@ -2141,13 +2165,37 @@ int nguo_step_create(nguo_ctx *ctx, const nguo_scene_desc *sc, nguo_step **out) 
       H.rh_rest.push_back(sc->rod_hinge_rest[h]);
       H.rh_stiff.push_back(sc->rod_hinge_stiff[h]);
     }
+    if (st->prm.stretch_k > 0.0f) {
+      // state.rs:39-65 SimState::from_mesh: one spring per unique triangle edge, in the
+      // sorted order of geometry.rs:37-45, rest length |x_a - x_b| of the scene positions
+      std::vector<std::pair<uint32_t, uint32_t>> ue;
+      ue.reserve(3 * (size_t)st->F);
+      for (uint32_t f = 0; f < st->F; ++f) {
+        const uint32_t *t = &st->h_tris[3 * f];
+        for (int k = 0; k < 3; ++k) ue.push_back({std::min(t[k], t[(k + 1) % 3]), std::max(t[k], t[(k + 1) % 3])});
+      }
+      std::sort(ue.begin(), ue.end());
+      ue.erase(std::unique(ue.begin(), ue.end()), ue.end());
+      for (auto &e : ue) {
+        const uint32_t v[2] = {e.first, e.second};
+        int c = classify(v, 2);
+        NGUO_REQUIRE(c != 2, NGUO_E_UNSUPPORTED, "an edge spring joins a pinned and a free vertex");
+        ElemHost &H = c == 0 ? ha : hs;
+        H.sp_v.insert(H.sp_v.end(), v, v + 2);
+        const float dx = X[3 * v[0]] - X[3 * v[1]], dy = X[3 * v[0] + 1] - X[3 * v[1] + 1], dz = X[3 * v[0] + 2] - X[3 * v[1] + 2];
+        H.sp_rest.push_back(std::sqrt(dx * dx + dy * dy + dz * dz));
+        const double ex = (double)X[3 * v[0]] - X[3 * v[1]], ey = (double)X[3 * v[0] + 1] - X[3 * v[1] + 1],
+                     ez = (double)X[3 * v[0] + 2] - X[3 * v[1] + 2];
+        H.sp_rest64.push_back(std::sqrt(ex * ex + ey * ey + ez * ez));
+      }
+    }
     // static rows: pinned vertices touched by static elements
     std::vector<uint32_t> sdof(V, NONE), srow_vert;
     auto touch = [&](const std::vector<uint32_t> &vv) {
       for (uint32_t v : vv)
         if (sdof[v] == NONE) { sdof[v] = (uint32_t)srow_vert.size(); srow_vert.push_back(v); }
     };
-    touch(hs.rd_v); touch(hs.rh_v); touch(hs.sh_v); touch(hs.hg_v);
+    touch(hs.rd_v); touch(hs.rh_v); touch(hs.sp_v); touch(hs.sh_v); touch(hs.hg_v);
     // device uploads
     st->x.upload(X, 3 * (size_t)V, s);
     st->x0.upload(X, 3 * (size_t)V, s);
@@ -2572,7 +2620,7 @@ int nguo_step_energy(nguo_step *st, const float *x, double *terms, double *total
     hipLaunchKernelGGL(k_energy_verts, dim3(div_up(st->V, BLOCK)), dim3(BLOCK), 0, s, st->V, (const float *)st->im.p, (const float *)st->x.p,
                        (const float *)st->xt.p, (const float *)st->x0.p, 1.0 / (h * h), st->P, st->eacc.p);
     for (ElemSet *es : {&st->act, &st->stat}) {
-      const uint32_t ne = es->ns + es->nh + es->nr + es->nrh;
+      const uint32_t ne = es->ns + es->nh + es->nr + es->nrh + es->nsp;
       if (ne)
         hipLaunchKernelGGL(k_energy_elems, dim3(div_up(ne, BLOCK)), dim3(BLOCK), 0, s, es->dev(), (const double *)es->sh_dm64.p,
                            (const double *)es->sh_area64.p, st->P, (const float *)st->x.p, st->eacc.p);

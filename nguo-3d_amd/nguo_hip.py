@@ -49,6 +49,7 @@ class Params(C.Structure):
         ("target_toi", C.c_float), ("min_newton_steps", C.c_uint32), ("ccd_max_iter", C.c_uint32),
         ("cg_tol", C.c_float), ("cg_max_iter", C.c_uint32), ("floor_y", C.c_float),
         ("precond", C.c_int32), ("schwarz_domain", C.c_uint32), ("schwarz_weight", C.c_float),
+        ("stretch_k", C.c_float),
     ]
 
 
@@ -467,12 +468,12 @@ class Stepper:
         check(load().nguo_step_adjoint(self.h, fptr(g), float(tol), 1 if exact_elastic else 0, fptr(out), C.byref(st)))
         return out, st
 
-    ENERGY_TERMS = ("inertia", "shells", "hinges", "rods", "rod_bends", "barriers", "friction", "colliders")
+    ENERGY_TERMS = ("inertia", "shells", "hinges", "rods", "rod_bends", "barriers", "friction", "colliders", "springs")
 
     def energy(self, x):
-        """(total, terms[8]) of the step energy of the last advanced step at positions x."""
+        """(total, terms[9]) of the step energy of the last advanced step at positions x."""
         x = np.ascontiguousarray(x, dtype=np.float32)
-        terms = np.zeros(8, np.float64)
+        terms = np.zeros(len(self.ENERGY_TERMS), np.float64)
         tot = C.c_double()
         check(load().nguo_step_energy(self.h, fptr(x), terms.ctypes.data_as(C.POINTER(C.c_double)), C.byref(tot)))
         return tot.value, terms

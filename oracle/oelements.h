@@ -345,6 +345,29 @@ template <class T> inline void rod_stretch_block(V3<T> d, T rest, T k, T h[9], V
   ga = nrm * (k * (l - rest));
 }
 
+// implicit.rs:3240-3261 spring_block: the edge spring's 3x3 block (row-major) and gradient on
+// a for d = x_a - x_b; the tangential coefficient 1 - rest/l is clamped at 0 (SPD projection)
+// unless project is false (the exact second derivative).
+template <class T> inline void spring_block(V3<T> d, T rest, T k, T h[9], V3<T> &ga, bool project = true) {
+  T l = length(d);
+  if (l < T(1.0e-9)) {
+    for (int i = 0; i < 9; ++i) h[i] = 0;
+    h[0] = k; h[4] = k; h[8] = k;
+    ga = V3<T>{};
+    return;
+  }
+  V3<T> nrm = d / l;
+  T coeff_t = T(1) - rest / l;
+  if (project) coeff_t = std::fmax(coeff_t, T(0));
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      T nnt = nrm[i] * nrm[j];
+      T iden = i == j ? T(1) : T(0);
+      h[3 * i + j] = k * (nnt + coeff_t * (iden - nnt));
+    }
+  ga = nrm * (k * (l - rest));
+}
+
 // rod_bend.rs:43-57 gradient_theta.
 template <class T> inline void strand_gradient_theta(V3<T> e0, V3<T> e1, V3<T> out[2]) {
   V3<T> cr = cross(e0, e1);

@@ -605,6 +605,15 @@ void oracle_rod_stretch(int fp64, const double *d3, double rest, double k, doubl
     for (int c = 0; c < 3; ++c) ga3[c] = (double)ga[c];
   })
 }
+void oracle_spring(int fp64, const double *d3, double rest, double k, int project, double *h9, double *ga3) {
+  PREC_SWITCH(fp64, {
+    T h[9];
+    V3<T> ga;
+    spring_block(v3<T>(d3), T(rest), T(k), h, ga, project != 0);
+    for (int i = 0; i < 9; ++i) h9[i] = (double)h[i];
+    for (int c = 0; c < 3; ++c) ga3[c] = (double)ga[c];
+  })
+}
 
 // ---------------------------------------------------------------- LBVH stages
 void oracle_morton_codes(uint32_t nv, const float *pos, uint32_t nt, const uint32_t *tris, uint32_t *codes) {

@@ -449,6 +449,8 @@ template <class T> class Solver {
     return e;
   }
   struct Rod { uint32_t a, b; T rest, k; };
+  struct Spring { uint32_t a, b; T rest; };
+  std::vector<Spring> springs;  // every unique triangle edge when stretch_k > 0
   struct RodHinge { uint32_t j, i, k; T rest, stiff; };
   std::vector<Rod> rods;
   std::vector<RodHinge> rod_hinges;
@@ -499,6 +501,10 @@ template <class T> class Solver {
       std::sort(es.begin(), es.end());
       es.erase(std::unique(es.begin(), es.end()), es.end());
       edges = es;
+      // state.rs:39-65 SimState::from_mesh: a stretch constraint per unique edge (sorted, as
+      // geometry.rs:37-45 derive_unique_edges), rest length from the scene positions
+      if (s.params.stretch_k > 0.0f)
+        for (auto &e : es) springs.push_back({e[0], e[1], length(pos[e[0]] - pos[e[1]])});
     }
     // implicit.rs:1492-1499 set_shell_faces (min_area 1e-12).
     std::vector<Tri> sf(s.n_shell_faces);
@@ -834,6 +840,21 @@ template <class T> class Solver {
               if (val != T(0)) push(3 * v[a] + r, 3 * v[b] + cc, val);
             }
     }
+    // edge springs (implicit.rs:2487-2507)
+    for (auto &sp : springs) {
+      T hk[9];
+      V ga;
+      spring_block(x[sp.a] - x[sp.b], sp.rest, T(prm.stretch_k), hk, ga, project_elastic);
+      for (int c = 0; c < 3; ++c) { grad[3 * sp.a + c] += ga[c]; grad[3 * sp.b + c] -= ga[c]; }
+      for (int r = 0; r < 3; ++r)
+        for (int cc = 0; cc < 3; ++cc) {
+          T v = hk[3 * r + cc];
+          push(3 * sp.a + r, 3 * sp.a + cc, v);
+          push(3 * sp.b + r, 3 * sp.b + cc, v);
+          push(3 * sp.a + r, 3 * sp.b + cc, -v);
+          push(3 * sp.b + r, 3 * sp.a + cc, -v);
+        }
+    }
     // isotropic shells (implicit.rs:2608-2652)
     if (!shells.empty() && (prm.shell_mu > 0.0f || prm.shell_lambda > 0.0f)) {
       T smu = T(prm.shell_mu), slam = T(prm.shell_lambda);
@@ -1118,7 +1139,7 @@ template <class T> class Solver {
   }
 
   // Terms of the step energy (the incremental potential each Newton iteration decreases).
-  enum EnergyTerm { E_INERTIA, E_SHELL, E_HINGE, E_ROD, E_ROD_BEND, E_BARRIER, E_FRICTION, E_COLLIDER, E_TERMS };
+  enum EnergyTerm { E_INERTIA, E_SHELL, E_HINGE, E_ROD, E_ROD_BEND, E_BARRIER, E_FRICTION, E_COLLIDER, E_SPRING, E_TERMS };
 
   // The smoothed Coulomb potential whose gradient is contact.rs:613-643 friction with the
   // normal force, normal and weights held: c f0(|P u|), f0(s) = s^2 / (2 eps) up to eps, then
@@ -1131,7 +1152,8 @@ template <class T> class Solver {
   // Step energy at x towards the inertial target xt (friction against the step's start pos),
   // with the contact quantities of `f`; out[E_TERMS] receives the terms. Every term is the one
   // whose gradient the assembly adds: inertia 1/2 m/h^2 |x - xt|^2 (implicit.rs:2392-2405),
-  // rod stretch 1/2 k (l - rest)^2 (implicit.rs:3287-3308, the rod tests' energy), strand
+  // rod stretch and edge springs 1/2 k (l - rest)^2 (implicit.rs:3287-3308, the rod tests'
+  // energy; implicit.rs:1-10 the mass-spring potential), strand
   // bending 1/2 k (theta - rest)^2 (rod_bend.rs:1-20), shells area psi(sigma)
   // (shell_model.rs:112-127), hinges 1/2 k_b w (theta - rest)^2 (bending.rs:105-123), the
   // barriers (contact.rs:580-591, 1014-1025) and the friction potentials.
@@ -1146,6 +1168,10 @@ template <class T> class Solver {
     for (auto &rd : rods) {
       const T e = length(x[rd.a] - x[rd.b]) - rd.rest;
       out[E_ROD] += T(0.5) * rd.k * e * e;
+    }
+    for (auto &sp : springs) {
+      const T e = length(x[sp.a] - x[sp.b]) - sp.rest;
+      out[E_SPRING] += T(0.5) * T(prm.stretch_k) * e * e;
     }
     for (auto &h : rod_hinges) {
       if (h.stiff <= T(0) || !with_rod_hinges) continue;

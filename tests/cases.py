@@ -1,5 +1,6 @@
-"""Scenes shared by the CPU and GPU tests beyond the generated configs: the analytic-collider
-variant of the small sheet, and parameter overrides."""
+"""Scenes and checks shared by the CPU and GPU tests beyond the generated configs: the
+analytic-collider variant of the small sheet, parameter overrides, a sheet state with vertices
+moved inside the body, and the device-vs-oracle stencil comparison."""
 import ctypes as C
 
 import numpy as np
@@ -25,9 +26,21 @@ def collider_scene():
     return s
 
 
+def spring_scene():
+    """The 10 x 10 sheet on the icosphere with edge springs (k = 800 over every unique
+    triangle edge, implicit.rs:2487-2507) in place of the shells: the mass-spring element set
+    of the reference's scene files (scene.rs:165-229 build_implicit)."""
+    s = nguo_hip.Scene(100, seed=0)
+    p = s.desc.params
+    p.stretch_k = 800.0
+    p.shell_mu = 0.0
+    p.shell_lambda = 0.0
+    return s
+
+
 def scene(cfg, **params):
-    """Generated config `cfg` (seed = cfg) or "collider", with params overridden."""
-    s = collider_scene() if cfg == "collider" else nguo_hip.Scene(cfg, seed=cfg)
+    """Generated config `cfg` (seed = cfg), "collider" or "springs", with params overridden."""
+    s = collider_scene() if cfg == "collider" else spring_scene() if cfg == "springs" else nguo_hip.Scene(cfg, seed=cfg)
     for k, v in params.items():
         setattr(s.desc.params, k, v)
     return s
@@ -72,3 +85,25 @@ def inside_state(steps=30, shallow=0.4, deep=2.5):
             n = -n
         x[vert] = ((a + b + c) / 3 - depth * dhat * n).astype(np.float32)
     return s, x, v, picked[0][0], picked[1][0], picked[0][1], picked[1][1]
+
+
+def check_stencils(ctx, s, o, x, v):
+    """The device's sorted stencil list, gaps, gradients, Rayleigh stiffnesses and collider
+    sides at x against the oracle's, bitwise. Returns the oracle's (idx, d)."""
+    idx, d, g = o.stencils(x)
+    assert len(idx) > 0, "the state must be in contact"
+    st = nguo_hip.Stepper(ctx, s)
+    st.set_state(x, v)
+    gidx, gd, gg, gk = st.stencils(x)
+    assert np.array_equal(gidx, idx)
+    assert np.array_equal(gd, d)  # same FP32 operation order: bitwise equal gaps
+    assert np.array_equal(gg, g)
+    assert np.all(np.isfinite(gk)) and np.all(gk > 0)
+    # Rayleigh stiffness against the elastic blocks at x (implicit.rs:755-795): the blocks are
+    # summed in the reference's push order on both sides, so the stiffnesses agree bitwise
+    ok = o.kappas(x, 1.0 / (s.params.dt * s.params.dt))[: len(idx)]
+    assert np.array_equal(gk, ok)
+    side = o.collider_sides(x)
+    assert np.array_equal(st.collider_sides(x), side)
+    st.close()
+    return idx, d

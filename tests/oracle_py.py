@@ -89,6 +89,7 @@ def load():
         lib.oracle_strand.restype = C.c_double
         lib.oracle_strand.argtypes = [C.c_int, D, C.c_double, D, D]
         lib.oracle_rod_stretch.argtypes = [C.c_int, D, C.c_double, C.c_double, D, D]
+        lib.oracle_spring.argtypes = [C.c_int, D, C.c_double, C.c_double, C.c_int, D, D]
         lib.oracle_solver_freeze.argtypes = [vp, D, D, C.c_double]
         lib.oracle_solver_energy.restype = C.c_double
         lib.oracle_solver_energy.argtypes = [vp, D, D, C.c_double, C.c_int, D]
@@ -248,15 +249,15 @@ class OracleSolver:
 
     # Assembly switches of the derivative checks (oracle_solver_model_gradient / hess_vec).
     GN_MODEL, UNPROJECTED, NO_ROD_BENDS = 1, 2, 4
-    TERMS = ("inertia", "shells", "hinges", "rods", "rod_bends", "barriers", "friction", "colliders")
+    TERMS = ("inertia", "shells", "hinges", "rods", "rod_bends", "barriers", "friction", "colliders", "springs")
 
     def freeze(self, x, xt, inv_h2):
         """Hold the contact quantities of an assembly at x for energy / model_gradient."""
         self.lib.oracle_solver_freeze(self.h, dp(d64(x)), dp(d64(xt)), inv_h2)
 
     def energy(self, x, xt, inv_h2, frozen=False):
-        """(total, terms[8]) of the step energy at x; frozen: with the held contact quantities."""
-        terms = np.zeros(8)
+        """(total, terms[9]) of the step energy at x; frozen: with the held contact quantities."""
+        terms = np.zeros(len(self.TERMS))
         e = self.lib.oracle_solver_energy(self.h, dp(d64(x)), dp(d64(xt)), inv_h2, 1 if frozen else 0, dp(terms))
         return e, terms
 

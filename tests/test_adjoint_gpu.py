@@ -82,6 +82,46 @@ def test_adjoint_matches_fp64_reference_in_contact(ctx, exact):
 
 
 @pytest.mark.gpu
+def test_adjoint_matches_fp64_reference_on_tube_garment(ctx):
+    # The 150 x 133 tube garment with its 64 panels, one step from a device drape state (6
+    # steps in, in contact with the body): the device adjoint against the oracle's FP64
+    # adjoint at the device's end state, with the same dL/dx.
+    s = nguo_hip.Scene(1, seed=1)
+    s.desc.params.cg_tol = 1e-10
+    panel = nguo_hip.drape_panels(s)
+    assert panel.max() == 63
+    theta, _, _ = D.draws()
+    st = nguo_hip.Stepper(ctx, s)
+    st.advance(6)
+    xs, vs = st.positions(), st.velocities()
+    o = O.OracleSolver(s, fp64=True)
+    for obj in (o, st):
+        obj.set_rest_scale(theta[panel].astype(np.float32) if obj is st else theta[panel])
+        obj.set_state(xs, vs)
+    o.step(1)
+    st.advance(1)
+    assert st.metrics().num_stencils > 0, "the state must be in contact"
+    # both adjoints are taken at the device's end state, so the step's own FP32-vs-FP64 spread
+    # (CCD-fraction sensitive on this garment) does not enter the comparison
+    x = st.positions()
+    print(f"end state |gpu - oracle64| {np.abs(x - o.positions()).max():.2e}")
+    rng = np.random.default_rng(12)
+    free = s.inv_mass > 0
+    dLdx = np.zeros((s.n_verts, 3))
+    dLdx[free] = rng.normal(size=(free.sum(), 3)) / free.sum()
+    g_dev, stats = st.adjoint(dLdx, tol=1e-7)
+    assert stats.residual <= 1e-7
+    g_ref, _, res = o.adjoint(dLdx, s.desc.n_shell_faces, 1e-12, xstar=x.astype(np.float64))
+    assert res < 1e-11
+    pd = np.bincount(panel, weights=g_dev.astype(np.float64), minlength=64)
+    pr = np.bincount(panel, weights=g_ref, minlength=64)
+    rel = np.linalg.norm(pd - pr) / np.linalg.norm(pr)
+    print(f"panels rel {rel:.3e}, faces rel {np.linalg.norm(g_dev - g_ref) / np.linalg.norm(g_ref):.3e}, iters {stats.iters}")
+    assert rel <= REF_REL
+    st.close()
+
+
+@pytest.mark.gpu
 def test_adjoint_matches_central_fd(ctx):
     s = D.sheet_scene()
     theta, theta_t, pick = D.draws()

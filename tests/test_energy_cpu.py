@@ -1,6 +1,7 @@
 """The oracle's step energy and what it pins (CPU). The step energy is the incremental
 potential a Newton iteration of engine-core/src/implicit.rs:1951-2191 decreases: inertia,
-shells, hinges, rod stretch, strand bending, barriers, friction and analytic colliders, each the
+shells, hinges, rod stretch, strand bending, edge springs, barriers, friction and analytic
+colliders, each the
 term whose gradient the assembly adds (implicit.rs:2280-3155). With the quantities an
 assembly holds fixed taken at a point x_ref (stencil set, Rayleigh stiffnesses, friction normal
 force, normals and weights), two checks pin the whole assembled system at x_ref:
@@ -61,7 +62,7 @@ def fd_energy(o, xr, xt, inv_h2, u):
     return (ep - em) / (2 * H)
 
 
-CASES = [(100, 12), (101, 10), (101, 25), ("collider", 14)]
+CASES = [(100, 12), (101, 10), (101, 25), ("collider", 14), ("springs", 14)]
 
 
 @pytest.mark.parametrize("cfg,steps", CASES)
@@ -75,6 +76,8 @@ def test_assembled_gradient_is_the_energy_gradient(cfg, steps):
     print({k: f"{v:.3e}" for k, v in names.items()})
     if cfg == "collider":
         assert names["colliders"] > 0 and names["barriers"] == 0
+    elif cfg == "springs":
+        assert names["springs"] > 0 and names["shells"] == 0 and names["barriers"] > 0 and names["hinges"] > 0
     else:
         assert names["barriers"] > 0 and names["friction"] > 0 and names["shells"] > 0 and names["hinges"] > 0
     g = o.model_gradient(xr, xt, inv_h2)
@@ -101,11 +104,11 @@ def test_assembled_hessian_is_the_jacobian_of_the_frozen_gradient(cfg, steps):
         assert rel <= HESS_TOL
 
 
-@pytest.mark.parametrize("dropped", [{"friction": 0.0}, {"bend_stiffness": 0.0}])
-def test_checks_fail_on_a_dropped_term(dropped):
+@pytest.mark.parametrize("cfg,steps,dropped", [(101, 10, {"friction": 0.0}), (101, 10, {"bend_stiffness": 0.0}),
+                                               ("springs", 14, {"stretch_k": 0.0})])
+def test_checks_fail_on_a_dropped_term(cfg, steps, dropped):
     # the same state assembled with one term missing: the energy check (and the Hessian check
     # against the full gradient) must reject it
-    cfg, steps = 101, 10
     o, s, xr, xt, inv_h2, (x, v) = newton_point(cfg, steps)
     o.freeze(xr, xt, inv_h2)
     bad = O.OracleSolver(scene(cfg, **dropped), fp64=True)

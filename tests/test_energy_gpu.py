@@ -26,7 +26,7 @@ from cases import scene
 
 EVAL_TOL = 1e-7  # same positions and target: measured <= 1e-8 per term
 E_TOL = 1e-5
-CASES = [(100, 12), (100, 30), (0, 12), (0, 20), (101, 10), (101, 25), ("collider", 14)]
+CASES = [(100, 12), (100, 30), (0, 12), (0, 20), (101, 10), (101, 25), ("collider", 14), ("springs", 14)]
 
 
 def rel(a, b):

@@ -353,6 +353,30 @@ def test_rod_stretch_exact_when_stretched():
     assert np.allclose(h.reshape(3, 3), hf, rtol=1e-3, atol=1e-3)
 
 
+def test_spring_block_exact_when_stretched_and_projected_when_compressed():
+    # implicit.rs:3240-3261: gradient k (l - L) d/l of 1/2 k (l - L)^2; stretched, the block
+    # is the exact Hessian; compressed, the tangential term (1 - L/l < 0) is clamped, leaving
+    # k n n^T, and the unprojected form keeps it (the exact Hessian again)
+    rest, k = 1.0, 8e2
+    energy = lambda q: 0.5 * k * (np.linalg.norm(q) - rest) ** 2
+    for d in (np.array([0.3, 0.4, 1.2]), np.array([0.2, -0.3, 0.5])):
+        exact = np.stack([fd_grad(lambda q, i=i: fd_grad(energy, q, 1e-5)[i], d, 1e-5) for i in range(3)])
+        for project in (1, 0):
+            h, ga = np.zeros(9), np.zeros(3)
+            lib.oracle_spring(1, O.dp(d), rest, k, project, O.dp(h), O.dp(ga))
+            assert np.allclose(ga, fd_grad(energy, d), atol=1e-6)
+            l = np.linalg.norm(d)
+            if l >= rest or not project:
+                assert np.allclose(h.reshape(3, 3), exact, rtol=1e-3, atol=1e-3)
+            else:
+                n = d / l
+                assert np.allclose(h.reshape(3, 3), k * np.outer(n, n), rtol=1e-12, atol=1e-9)
+    # a degenerate edge: isotropic k, no force
+    h, ga = np.zeros(9), np.zeros(3)
+    lib.oracle_spring(0, O.dp(np.zeros(3)), rest, k, 1, O.dp(h), O.dp(ga))
+    assert np.array_equal(h.reshape(3, 3), k * np.eye(3)) and not ga.any()
+
+
 # ------------------------------------------------------------------ linear solve
 def test_cg_manufactured_laplacian_and_zero_rhs():
     n = 3 * GOLD["pcg_manufactured"]["n"]

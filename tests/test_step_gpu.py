@@ -6,7 +6,7 @@ import pytest
 
 from conftest import nguo_hip
 import oracle_py as O
-from cases import inside_state, scene
+from cases import check_stencils, inside_state, scene
 
 POS_TOL = 1e-5  # |x_gpu - x_oracle|_inf / max(1, |x|_inf) after each Newton iteration / step
 
@@ -66,26 +66,6 @@ def test_stencils_bitexact_inside_collider(ctx):
     assert sum(1 for r, dd in zip(idx, d) if r[0] == vs and static[r[1:]].all() and dd < 0) >= 2
 
 
-def check_stencils(ctx, s, o, x, v):
-    idx, d, g = o.stencils(x)
-    assert len(idx) > 0, "the state must be in contact"
-    st = nguo_hip.Stepper(ctx, s)
-    st.set_state(x, v)
-    gidx, gd, gg, gk = st.stencils(x)
-    assert np.array_equal(gidx, idx)
-    assert np.array_equal(gd, d)  # same FP32 operation order: bitwise equal gaps
-    assert np.array_equal(gg, g)
-    assert np.all(np.isfinite(gk)) and np.all(gk > 0)
-    # Rayleigh stiffness against the elastic blocks at x (implicit.rs:755-795): the blocks are
-    # summed in the reference's push order on both sides, so the stiffnesses agree bitwise
-    ok = o.kappas(x, 1.0 / (s.params.dt * s.params.dt))[: len(idx)]
-    assert np.array_equal(gk, ok)
-    side = o.collider_sides(x)
-    assert np.array_equal(st.collider_sides(x), side)
-    st.close()
-    return idx, d
-
-
 # Both sides solve each Newton system to 1e-10 relative residual here, so the comparison
 # measures the implementation (assembly, contact, CCD) and not the solver tolerance: at the
 # production 1e-6 two correct solvers already differ by up to cond(H) * 1e-6 in dx.
@@ -101,7 +81,7 @@ def check_stencils(ctx, s, o, x, v):
 # that stencil stiffnesses ~650x apart. Such a state records its FP64 gap as an expected
 # failure with the measured figure, instead of widening the bar.
 @pytest.mark.gpu
-@pytest.mark.parametrize("cfg,steps", [(100, 0), (100, 12), (100, 30), (0, 12), (0, 20), (0, 33), (101, 10), (101, 25), ("collider", 14)])
+@pytest.mark.parametrize("cfg,steps", [(100, 0), (100, 12), (100, 30), (0, 12), (0, 20), (0, 33), (101, 10), (101, 25), ("collider", 14), ("springs", 14)])
 def test_one_step_parity(ctx, cfg, steps):
     s, o = oracle_state(cfg, steps, cg_tol=1e-10)
     x, v = o.positions(), o.velocities()
