@@ -191,6 +191,35 @@ class Scene:
             self._buf = None
 
 
+class ArrayScene:
+    """A scene built from numpy arrays (the caller's own lowering), with the attributes of
+    Scene: positions [n, 3], inv_mass [n], tris [m, 3], static [n] (or None), colliders (a
+    list of Collider) and params (a Params). The arrays are kept alive with the descriptor."""
+
+    def __init__(self, positions, inv_mass, tris, params, static=None, colliders=()):
+        self.positions = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
+        self.n_verts = len(self.positions)
+        self.inv_mass = np.ascontiguousarray(inv_mass, dtype=np.float32).reshape(-1)
+        self.tris = np.ascontiguousarray(tris, dtype=np.uint32).reshape(-1, 3)
+        self.static = None if static is None else np.ascontiguousarray(static, dtype=np.uint8).reshape(-1)
+        self._vel = np.zeros_like(self.positions)
+        self._coll = (Collider * max(1, len(colliders)))(*colliders)
+        d = SceneDesc()
+        d.n_verts = self.n_verts
+        d.positions = self.positions.ctypes.data_as(C.POINTER(C.c_float))
+        d.velocities = self._vel.ctypes.data_as(C.POINTER(C.c_float))
+        d.inv_mass = self.inv_mass.ctypes.data_as(C.POINTER(C.c_float))
+        d.n_tris = len(self.tris)
+        d.tris = self.tris.ctypes.data_as(C.POINTER(C.c_uint32))
+        if self.static is not None:
+            d.static_collider = self.static.ctypes.data_as(C.POINTER(C.c_uint8))
+        d.n_colliders = len(colliders)
+        d.colliders = C.cast(self._coll, C.POINTER(Collider))
+        d.params = params
+        self.desc = d
+        self.params = d.params
+
+
 class Context:
     def __init__(self, device=0):
         lib = load()
