@@ -327,6 +327,7 @@ struct Topo {
   const float *side;        // null = no mesh collider
   const float *pn;          // [3V] pseudonormals
   const uint32_t *vdof, *sdof;
+  const uint32_t *xdof;     // vertex -> row of the mixed-element table; null = no such element
 };
 
 __device__ inline bool movable(const Topo &T, uint32_t v) { return T.im[v] > 0.0f; }
@@ -573,24 +574,33 @@ __global__ __launch_bounds__(BLOCK) void k_stencil_gather(uint32_t nvt, uint32_t
   for (int k = 0; k < 12; ++k) sg[12 * (size_t)s + k] = gg[12 * (size_t)src + k];
 }
 
-// Block (va, vb) of the elastic Hessian gathered so far (the reference's fixed_hess_in):
-// from the free system for two free vertices, from the static table for two pinned ones.
-__device__ inline void lookup_block(const Topo &T, const SellView &K, const SellView &S, uint32_t va, uint32_t vb, float out[9]) {
-  for (int e = 0; e < 9; ++e) out[e] = 0.0f;
-  const SellView *M = nullptr;
-  uint32_t r, c;
-  if (T.vdof[va] != NONE && T.vdof[vb] != NONE) { M = &K; r = T.vdof[va]; c = T.vdof[vb]; }
-  else if (T.sdof && T.sdof[va] != NONE && T.sdof[vb] != NONE) { M = &S; r = T.sdof[va]; c = T.sdof[vb]; }
-  if (!M || M->nb == 0) return;
-  uint32_t base = M->slice_ptr[r >> 6], lane = r & 63u, lo = 0, hi = M->row_len[r];
+// Adds block (r, c) of M to out when the pattern holds it.
+__device__ inline void sell_add_block(const SellView &M, uint32_t r, uint32_t c, float out[9]) {
+  if (M.nb == 0) return;
+  uint32_t base = M.slice_ptr[r >> 6], lane = r & 63u, lo = 0, hi = M.row_len[r];
   while (lo < hi) {
     uint32_t mid = (lo + hi) >> 1;
-    if (M->col[base + 64u * mid + lane] < c) lo = mid + 1; else hi = mid;
+    if (M.col[base + 64u * mid + lane] < c) lo = mid + 1; else hi = mid;
   }
-  if (lo < M->row_len[r] && M->col[base + 64u * lo + lane] == c) {
+  if (lo < M.row_len[r] && M.col[base + 64u * lo + lane] == c) {
     uint32_t sl = base + 64u * lo + lane;
-    for (int e = 0; e < 9; ++e) out[e] = M->val[sell_val_index(sl, e)];
+    for (int e = 0; e < 9; ++e) out[e] += M.val[sell_val_index(sl, e)];
   }
+}
+
+// Block (va, vb) of the elastic Hessian gathered so far, pinned rows and columns included (the
+// reference's fixed_hess_in, implicit.rs:3175-3228): from the free system for two free
+// vertices; otherwise from the static table (elements on pinned vertices only) plus the
+// mixed-element table (elements joining pinned and free vertices).
+__device__ inline void lookup_block(const Topo &T, const SellView &K, const SellView &S, const SellView &X, uint32_t va, uint32_t vb,
+                                    float out[9]) {
+  for (int e = 0; e < 9; ++e) out[e] = 0.0f;
+  if (T.vdof[va] != NONE && T.vdof[vb] != NONE) {
+    sell_add_block(K, T.vdof[va], T.vdof[vb], out);
+    return;
+  }
+  if (T.sdof && T.sdof[va] != NONE && T.sdof[vb] != NONE) sell_add_block(S, T.sdof[va], T.sdof[vb], out);
+  if (T.xdof && T.xdof[va] != NONE) sell_add_block(X, T.xdof[va], vb, out);
 }
 
 struct ContactOut {
@@ -602,7 +612,7 @@ struct ContactOut {
 // implicit.rs:3026-3103 per-stencil barrier + friction, implicit.rs:755-795 Rayleigh stiffness.
 __global__ __launch_bounds__(BLOCK) void k_stencil_eval(uint32_t ns, const uint32_t *__restrict__ sidx, const float *__restrict__ sd,
                                                        const float *__restrict__ sg, float ghat_vt, float ghat_ee, uint32_t nvt, Topo T,
-                                                       SellView K, SellView S, Params P, const float *__restrict__ x, const float *__restrict__ xs,
+                                                       SellView K, SellView S, SellView X, Params P, const float *__restrict__ x, const float *__restrict__ xs,
                                                        ContactOut C) {
   uint32_t s = blockIdx.x * BLOCK + threadIdx.x;
   if (s >= ns) return;
@@ -647,7 +657,7 @@ __global__ __launch_bounds__(BLOCK) void k_stencil_eval(uint32_t ns, const uint3
           float m_ii = mass[ii] > 0.0f ? mass[ii] : m_max;
           for (int jj = 0; jj < 4; ++jj) {
             float blk[9];
-            lookup_block(T, K, S, id[ii], id[jj], blk);
+            lookup_block(T, K, S, X, id[ii], id[jj], blk);
             for (int r = 0; r < 3; ++r) {
               float row = 0.0f;
               for (int c = 0; c < 3; ++c) {
@@ -1215,7 +1225,7 @@ __global__ __launch_bounds__(BLOCK) void k_energy_elems(ElemDev E, const double 
 // FP64 gap and gradient.
 __global__ __launch_bounds__(BLOCK) void k_energy_contacts(uint32_t ns, const uint32_t *__restrict__ sidx, const float *__restrict__ sd,
                                                           float ghat_vt, float ghat_ee, uint32_t nvt, Topo T,
-                                                          SellView K, SellView S, Params P, const float *__restrict__ x,
+                                                          SellView K, SellView S, SellView X, Params P, const float *__restrict__ x,
                                                           const float *__restrict__ xs, double *acc) {
   double e[EN_TERMS] = {};
   const uint32_t s = blockIdx.x * BLOCK + threadIdx.x;
@@ -1254,7 +1264,7 @@ __global__ __launch_bounds__(BLOCK) void k_energy_contacts(uint32_t ns, const ui
             const double m_ii = mass[ii] > 0.0 ? mass[ii] : m_max;
             for (int jj = 0; jj < 4; ++jj) {
               float blk[9];
-              lookup_block(T, K, S, id[ii], id[jj], blk);
+              lookup_block(T, K, S, X, id[ii], id[jj], blk);
               for (int r = 0; r < 3; ++r) {
                 double row = 0.0;
                 for (int c = 0; c < 3; ++c) {
@@ -1340,8 +1350,9 @@ struct nguo_step {
   DevBuf<uint32_t> tri_dyn_ids, tri_stat_ids, tri_coll_ids, edge_dyn_ids, edge_stat_ids;
   uint32_t n_tri_dyn = 0, n_tri_stat = 0, n_tri_coll = 0, n_edge_dyn = 0, n_edge_stat = 0;
   Lbvh tree_tri_dyn, tree_tri_stat, tree_tri_coll, tree_edge_dyn, tree_edge_stat;
-  // elements
-  ElemSet act, stat;
+  // elements: active (system), static (all pinned) and mixed (pinned and free: contact blocks)
+  ElemSet act, stat, mix;
+  DevBuf<uint32_t> xdof;  // vertex -> row of the mixed-element table
   // contact buffers
   uint32_t cap = 0;
   DevBuf<uint32_t> vt_count, vt_ord, ee_ord;
@@ -1519,16 +1530,26 @@ float dihedral_host(const float *p0, const float *p1, const float *p2, const flo
 }
 
 // Builds the pattern, slot->row map and the fixed-order contribution lists of one element
-// set over `rows` (vertex -> row map, NONE = not in this system).
+// set over `rows` (vertex -> row map, NONE = not in this system). Columns use the same map,
+// or, with `cmap`, their own (the mixed-element table: global vertex columns); with `pinned`,
+// only blocks with a pinned row or column vertex are kept.
 void build_gather(ElemSet &S, const ElemHost &H, uint32_t nrows, const std::vector<uint32_t> &vmap, const std::vector<uint32_t> &row_vert,
-                  bool with_grad, hipStream_t s) {
+                  bool with_grad, hipStream_t s, const std::vector<uint32_t> *cmap = nullptr, const std::vector<uint8_t> *pinned = nullptr) {
+  // (row, column) of block (va, vb), or NONE
+  auto at = [&](uint32_t va, uint32_t vb) -> std::pair<uint32_t, uint32_t> {
+    const uint32_t r = vmap[va], c = cmap ? (*cmap)[vb] : vmap[vb];
+    if (r == NONE || c == NONE) return {NONE, NONE};
+    if (pinned && !(*pinned)[va] && !(*pinned)[vb]) return {NONE, NONE};
+    return {r, c};
+  };
   std::vector<std::vector<uint32_t>> cols(nrows);
-  for (uint32_t r = 0; r < nrows; ++r) cols[r].push_back(r);
+  if (!cmap)
+    for (uint32_t r = 0; r < nrows; ++r) cols[r].push_back(r);
   auto couple = [&](const uint32_t *v, int k) {
     for (int a = 0; a < k; ++a)
       for (int b = 0; b < k; ++b) {
-        uint32_t ra = vmap[v[a]], rb = vmap[v[b]];
-        if (ra != NONE && rb != NONE) cols[ra].push_back(rb);
+        auto rc = at(v[a], v[b]);
+        if (rc.first != NONE) cols[rc.first].push_back(rc.second);
       }
   };
   for (uint32_t e = 0; e < H.nr(); ++e) couple(&H.rd_v[2 * e], 2);
@@ -1547,26 +1568,23 @@ void build_gather(ElemSet &S, const ElemHost &H, uint32_t nrows, const std::vect
   std::vector<std::pair<uint32_t, uint32_t>> glist;
   const uint32_t nrh = H.nrh();
   // a two-vertex element's blocks [[H, -H], [-H, H]] and gradient (+g, -g), out_rod entry e
+  auto block = [&](uint32_t va, uint32_t vb, uint32_t c) {
+    auto rc = at(va, vb);
+    if (rc.first != NONE) hlist.push_back({S.M.find(rc.first, rc.second), c});
+  };
   auto pair = [&](const uint32_t *v, uint32_t e) {
+    block(v[0], v[0], code(C_ROD_POS, e));
+    block(v[1], v[1], code(C_ROD_POS, e));
+    block(v[0], v[1], code(C_ROD_NEG, e));
+    block(v[1], v[0], code(C_ROD_NEG, e));
     uint32_t ra = vmap[v[0]], rb = vmap[v[1]];
-    if (ra != NONE) hlist.push_back({S.M.find(ra, ra), code(C_ROD_POS, e)});
-    if (rb != NONE) hlist.push_back({S.M.find(rb, rb), code(C_ROD_POS, e)});
-    if (ra != NONE && rb != NONE) {
-      hlist.push_back({S.M.find(ra, rb), code(C_ROD_NEG, e)});
-      hlist.push_back({S.M.find(rb, ra), code(C_ROD_NEG, e)});
-    }
     if (ra != NONE) glist.push_back({ra, code(C_ROD_POS, e)});
     if (rb != NONE) glist.push_back({rb, code(C_ROD_NEG, e)});
   };
   for (uint32_t e = 0; e < H.nr(); ++e) pair(&H.rd_v[2 * e], e);
   auto dense = [&](const uint32_t *v, uint32_t elem_base) {
-    for (int a = 0; a < 3; ++a) {
-      uint32_t ra = vmap[v[a]];
-      for (int b = 0; b < 3; ++b) {
-        uint32_t rb = vmap[v[b]];
-        if (ra != NONE && rb != NONE) hlist.push_back({S.M.find(ra, rb), code(C_DENSE, elem_base * 9 + 3 * a + b)});
-      }
-    }
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) block(v[a], v[b], code(C_DENSE, elem_base * 9 + 3 * a + b));
     for (int a = 0; a < 3; ++a)
       if (vmap[v[a]] != NONE) glist.push_back({vmap[v[a]], code(C_DENSE, elem_base * 3 + a)});
   };
@@ -1575,13 +1593,8 @@ void build_gather(ElemSet &S, const ElemHost &H, uint32_t nrows, const std::vect
   for (uint32_t e = 0; e < H.ns(); ++e) dense(&H.sh_v[3 * e], nrh + e);
   for (uint32_t e = 0; e < H.nh(); ++e) {
     const uint32_t *v = &H.hg_v[4 * e];
-    for (int a = 0; a < 4; ++a) {
-      uint32_t ra = vmap[v[a]];
-      for (int b = 0; b < 4; ++b) {
-        uint32_t rb = vmap[v[b]];
-        if (ra != NONE && rb != NONE) hlist.push_back({S.M.find(ra, rb), code(C_HINGE, e * 16 + 4 * a + b)});
-      }
-    }
+    for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b) block(v[a], v[b], code(C_HINGE, e * 16 + 4 * a + b));
     for (int a = 0; a < 4; ++a)
       if (vmap[v[a]] != NONE) glist.push_back({vmap[v[a]], code(C_HINGE, e * 4 + a)});
   }
@@ -1647,6 +1660,18 @@ void gather(ElemSet &S, const Params &P, int active, const float *im, const floa
   NGUO_HIP(hipGetLastError());
 }
 
+// The elastic blocks at x: the free system (inertia, elements, analytic colliders) and, when
+// elements join pinned and free vertices, the mixed-element table's blocks on pinned rows and
+// columns (read by the contact stiffness).
+void assemble_elastic(nguo_step *st, const Params &P, hipStream_t s) {
+  run_elements(st->act, P, st->x.p, s);
+  gather(st->act, P, 1, st->im.p, st->x.p, st->x0.p, s);
+  if (st->mix.M.nb) {
+    run_elements(st->mix, P, st->x.p, s);
+    gather(st->mix, P, 0, st->im.p, st->x.p, st->x0.p, s);
+  }
+}
+
 Topo topo(nguo_step *st) {
   Topo T{};
   T.V = st->V;
@@ -1660,6 +1685,7 @@ Topo topo(nguo_step *st) {
   T.pn = st->pn.p;
   T.vdof = st->vdof.p;
   T.sdof = st->sdof.p;
+  T.xdof = st->mix.M.nb ? st->xdof.p : nullptr;
   return T;
 }
 
@@ -1776,7 +1802,7 @@ void contacts(nguo_step *st) {
   float dhat = st->prm.contact_dhat;
   ContactOut CO{st->cdof.p, st->cg.p, st->cw.p, st->ccoef.p, st->cn.p, st->cgrad.p, st->ckappa.p};
   hipLaunchKernelGGL(k_stencil_eval, dim3(div_up(st->ns, BLOCK)), dim3(BLOCK), 0, s, st->ns, (const uint32_t *)st->sidx.p, (const float *)st->sd.p,
-                     (const float *)st->sg.p, dhat, 0.5f * (dhat + dhat), st->nvt, T, st->act.M.view(), st->stat.M.view(), st->P,
+                     (const float *)st->sg.p, dhat, 0.5f * (dhat + dhat), st->nvt, T, st->act.M.view(), st->stat.M.view(), st->mix.M.view(), st->P,
                      (const float *)st->x.p, (const float *)st->x0.p, CO);
   uint32_t m = 4 * st->ns;
   hipLaunchKernelGGL(k_inc_keys, dim3(div_up(m, BLOCK)), dim3(BLOCK), 0, s, st->ns, (const uint32_t *)st->cdof.p, st->NF, st->inc_key.p, st->inc_val.p);
@@ -1901,8 +1927,7 @@ PcgStats solve_system(nguo_step *st, const float *b, float tol, float *x) {
 
 void assemble_and_solve(nguo_step *st, nguo_stats &stats) {
   hipStream_t s = st->s;
-  run_elements(st->act, st->P, st->x.p, s);
-  gather(st->act, st->P, 1, st->im.p, st->x.p, st->x0.p, s);
+  assemble_elastic(st, st->P, s);
   if (st->NF)
     hipLaunchKernelGGL(k_gather_grad, dim3(div_up(st->NF, BLOCK)), dim3(BLOCK), 0, s, st->NF, (const uint32_t *)st->free_list.p,
                        (const uint32_t *)st->act.gc_ptr.p, (const uint32_t *)st->act.gc.p, st->act.dev(), (const float *)st->im.p, st->P,
@@ -2089,15 +2114,22 @@ int nguo_step_create(nguo_ctx *ctx, const nguo_scene_desc *sc, nguo_step **out) 
     }
     const float *X = sc->positions;
     auto pinned = [&](uint32_t v) { return !(st->h_im[v] > 0.0f); };
-    // elements, split into active (all free) and static (all pinned); mixed ones are not
-    // representable in the free-vertex system's gather
-    ElemHost ha, hs;
+    // elements, split into active (any free vertex: their free-free blocks and free rows build
+    // the system) and static (all pinned: blocks gathered once); an element joining pinned and
+    // free vertices is also listed in the mixed table, whose blocks on a pinned row or column
+    // serve the contact stiffness (the reference's unfiltered blocks, implicit.rs:3175-3228)
+    ElemHost ha, hs, hm;
     auto classify = [&](const uint32_t *v, int k) -> int {
       int np = 0;
       for (int a = 0; a < k; ++a) np += pinned(v[a]) ? 1 : 0;
       if (np == 0) return 0;
       if (np == k) return 1;
       return 2;
+    };
+    auto homes = [&](int c) -> std::vector<ElemHost *> {
+      if (c == 0) return {&ha};
+      if (c == 1) return {&hs};
+      return {&ha, &hm};
     };
     if (!sc->shell_faces || !(st->prm.shell_mu > 0.0f || st->prm.shell_lambda > 0.0f)) {
       // shells contribute nothing (implicit.rs:2608-2611)
@@ -2108,15 +2140,14 @@ int nguo_step_create(nguo_ctx *ctx, const nguo_scene_desc *sc, nguo_step **out) 
         double dm64[4], area64;
         if (!shell_rest(X + 3 * t[0], X + 3 * t[1], X + 3 * t[2], dm, area)) continue;
         (void)shell_rest(X + 3 * t[0], X + 3 * t[1], X + 3 * t[2], dm64, area64);
-        int c = classify(t, 3);
-        NGUO_REQUIRE(c != 2, NGUO_E_UNSUPPORTED, "a shell face mixes pinned and free vertices");
-        ElemHost &H = c == 0 ? ha : hs;
-        H.sh_v.insert(H.sh_v.end(), t, t + 3);
-        H.sh_dm.insert(H.sh_dm.end(), dm, dm + 4);
-        H.sh_area.push_back(area);
-        H.sh_dm64.insert(H.sh_dm64.end(), dm64, dm64 + 4);
-        H.sh_area64.push_back(area64);
-        H.sh_face.push_back(f);
+        for (ElemHost *H : homes(classify(t, 3))) {
+          H->sh_v.insert(H->sh_v.end(), t, t + 3);
+          H->sh_dm.insert(H->sh_dm.end(), dm, dm + 4);
+          H->sh_area.push_back(area);
+          H->sh_dm64.insert(H->sh_dm64.end(), dm64, dm64 + 4);
+          H->sh_area64.push_back(area64);
+          H->sh_face.push_back(f);
+        }
       }
     }
     if (st->prm.bend_stiffness > 0.0f) {
@@ -2137,33 +2168,31 @@ int nguo_step_create(nguo_ctx *ctx, const nguo_scene_desc *sc, nguo_step **out) 
         while (j < ek.size() && ek[j].first == ek[i].first) ++j;
         if (j - i == 2) {
           uint32_t v[4] = {ek[i].second, (uint32_t)(ek[i].first >> 32), (uint32_t)ek[i].first, ek[i + 1].second};
-          int c = classify(v, 4);
-          NGUO_REQUIRE(c != 2, NGUO_E_UNSUPPORTED, "a bending hinge mixes pinned and free vertices");
-          ElemHost &H = c == 0 ? ha : hs;
-          H.hg_v.insert(H.hg_v.end(), v, v + 4);
-          H.hg_rest.push_back(dihedral_host(X + 3 * v[0], X + 3 * v[1], X + 3 * v[2], X + 3 * v[3]));
+          const float rest = dihedral_host(X + 3 * v[0], X + 3 * v[1], X + 3 * v[2], X + 3 * v[3]);
+          for (ElemHost *H : homes(classify(v, 4))) {
+            H->hg_v.insert(H->hg_v.end(), v, v + 4);
+            H->hg_rest.push_back(rest);
+          }
         }
         i = j;
       }
     }
     for (uint32_t r = 0; r < sc->n_rods; ++r) {
       const uint32_t *v = sc->rod_verts + 2 * r;
-      int c = classify(v, 2);
-      NGUO_REQUIRE(c != 2, NGUO_E_UNSUPPORTED, "a rod segment mixes pinned and free vertices");
-      ElemHost &H = c == 0 ? ha : hs;
-      H.rd_v.insert(H.rd_v.end(), v, v + 2);
-      H.rd_rest.push_back(sc->rod_rest[r]);
-      H.rd_k.push_back(sc->rod_k[r]);
+      for (ElemHost *H : homes(classify(v, 2))) {
+        H->rd_v.insert(H->rd_v.end(), v, v + 2);
+        H->rd_rest.push_back(sc->rod_rest[r]);
+        H->rd_k.push_back(sc->rod_k[r]);
+      }
     }
     for (uint32_t h = 0; h < sc->n_rod_hinges; ++h) {
       if (!(sc->rod_hinge_stiff[h] > 0.0f)) continue;
       const uint32_t *v = sc->rod_hinge_verts + 3 * h;
-      int c = classify(v, 3);
-      NGUO_REQUIRE(c != 2, NGUO_E_UNSUPPORTED, "a rod bend mixes pinned and free vertices");
-      ElemHost &H = c == 0 ? ha : hs;
-      H.rh_v.insert(H.rh_v.end(), v, v + 3);
-      H.rh_rest.push_back(sc->rod_hinge_rest[h]);
-      H.rh_stiff.push_back(sc->rod_hinge_stiff[h]);
+      for (ElemHost *H : homes(classify(v, 3))) {
+        H->rh_v.insert(H->rh_v.end(), v, v + 3);
+        H->rh_rest.push_back(sc->rod_hinge_rest[h]);
+        H->rh_stiff.push_back(sc->rod_hinge_stiff[h]);
+      }
     }
     if (st->prm.stretch_k > 0.0f) {
       // state.rs:39-65 SimState::from_mesh: one spring per unique triangle edge, in the
@@ -2178,15 +2207,14 @@ int nguo_step_create(nguo_ctx *ctx, const nguo_scene_desc *sc, nguo_step **out) 
       ue.erase(std::unique(ue.begin(), ue.end()), ue.end());
       for (auto &e : ue) {
         const uint32_t v[2] = {e.first, e.second};
-        int c = classify(v, 2);
-        NGUO_REQUIRE(c != 2, NGUO_E_UNSUPPORTED, "an edge spring joins a pinned and a free vertex");
-        ElemHost &H = c == 0 ? ha : hs;
-        H.sp_v.insert(H.sp_v.end(), v, v + 2);
         const float dx = X[3 * v[0]] - X[3 * v[1]], dy = X[3 * v[0] + 1] - X[3 * v[1] + 1], dz = X[3 * v[0] + 2] - X[3 * v[1] + 2];
-        H.sp_rest.push_back(std::sqrt(dx * dx + dy * dy + dz * dz));
         const double ex = (double)X[3 * v[0]] - X[3 * v[1]], ey = (double)X[3 * v[0] + 1] - X[3 * v[1] + 1],
                      ez = (double)X[3 * v[0] + 2] - X[3 * v[1] + 2];
-        H.sp_rest64.push_back(std::sqrt(ex * ex + ey * ey + ez * ez));
+        for (ElemHost *H : homes(classify(v, 2))) {
+          H->sp_v.insert(H->sp_v.end(), v, v + 2);
+          H->sp_rest.push_back(std::sqrt(dx * dx + dy * dy + dz * dz));
+          H->sp_rest64.push_back(std::sqrt(ex * ex + ey * ey + ez * ez));
+        }
       }
     }
     // static rows: pinned vertices touched by static elements
@@ -2222,8 +2250,21 @@ int nguo_step_create(nguo_ctx *ctx, const nguo_scene_desc *sc, nguo_step **out) 
     st->inc_ptr.zero((size_t)st->NF + 2, s);
     upload_elems(st->act, ha, s);
     upload_elems(st->stat, hs, s);
+    upload_elems(st->mix, hm, s);
     build_gather(st->act, ha, st->NF, st->h_vdof, st->h_free, true, s);
     build_gather(st->stat, hs, (uint32_t)srow_vert.size(), sdof, srow_vert, false, s);
+    {
+      // the mixed-element table: a row per vertex of a mixed element, global vertex columns,
+      // blocks with a pinned row or column vertex only
+      std::vector<uint32_t> xdof(V, NONE), xrow_vert, ident(V);
+      std::vector<uint8_t> pin(V);
+      for (uint32_t v = 0; v < V; ++v) { ident[v] = v; pin[v] = pinned(v) ? 1 : 0; }
+      for (const auto *vv : {&hm.rd_v, &hm.rh_v, &hm.sp_v, &hm.sh_v, &hm.hg_v})
+        for (uint32_t v : *vv)
+          if (xdof[v] == NONE) { xdof[v] = (uint32_t)xrow_vert.size(); xrow_vert.push_back(v); }
+      build_gather(st->mix, hm, (uint32_t)xrow_vert.size(), xdof, xrow_vert, false, s, &ident, &pin);
+      st->xdof.upload(xdof, s);
+    }
     // static element blocks once (the body never moves)
     if (st->stat.M.nb) {
       run_elements(st->stat, P, st->x.p, s);
@@ -2477,8 +2518,7 @@ int nguo_step_stencils(nguo_step *st, const float *x, uint32_t cap, uint32_t *id
     require_pinned(st, x, "nguo_step_stencils");
     hipStream_t s = st->s;
     st->x.upload(x, 3 * (size_t)st->V, s);
-    run_elements(st->act, st->P, st->x.p, s);
-    gather(st->act, st->P, 1, st->im.p, st->x.p, st->x0.p, s);
+    assemble_elastic(st, st->P, s);
     contacts(st);
     *count = st->ns;
     uint32_t m = std::min(cap, st->ns);
@@ -2522,7 +2562,7 @@ int nguo_step_set_rest_scale(nguo_step *st, const float *face_scale, uint32_t nf
       NGUO_REQUIRE(std::isfinite(face_scale[f]) && face_scale[f] > 0.0f, NGUO_E_INVALID, "nguo_step_set_rest_scale: scales must be finite and > 0");
     hipStream_t s = st->s;
     NGUO_HIP(hipStreamSynchronize(s));  // the host copies below may still feed an earlier upload
-    for (ElemSet *S : {&st->act, &st->stat}) {
+    for (ElemSet *S : {&st->act, &st->stat, &st->mix}) {
       if (!S->ns) continue;
       for (uint32_t e = 0; e < S->ns; ++e) S->h_sh_scale[e] = face_scale[S->h_sh_face[e]];
       S->sh_scale.upload(S->h_sh_scale, s);
@@ -2555,14 +2595,12 @@ int nguo_step_adjoint(nguo_step *st, const float *dLdx, float tol, int hessian, 
     // H at x* (= x = x0 after the commit) with the last step's inertial target (xt) and
     // friction reference (its start, xprev)
     NGUO_HIP(hipMemcpyAsync(st->x0.p, st->xprev.p, 3 * (size_t)st->V * sizeof(float), hipMemcpyDeviceToDevice, s));
-    run_elements(st->act, st->P, st->x.p, s);
-    gather(st->act, st->P, 1, st->im.p, st->x.p, st->x0.p, s);
+    assemble_elastic(st, st->P, s);
     contacts(st);  // contact stiffnesses from the step's (projected) elastic blocks
     if (hessian == NGUO_ADJ_EXACT_ELASTIC) {
       Params PA = st->P;
       PA.project = 0;
-      run_elements(st->act, PA, st->x.p, s);
-      gather(st->act, PA, 1, st->im.p, st->x.p, st->x0.p, s);
+      assemble_elastic(st, PA, s);
     }
     // rhs: dL/dx on the free dofs
     std::vector<float> &hb = st->h_adj_b;
@@ -2612,8 +2650,7 @@ int nguo_step_energy(nguo_step *st, const float *x, double *terms, double *total
     NGUO_HIP(hipMemcpyAsync(st->dx.p, st->x0.p, n3 * sizeof(float), hipMemcpyDeviceToDevice, s));
     NGUO_HIP(hipMemcpyAsync(st->x0.p, st->xprev.p, n3 * sizeof(float), hipMemcpyDeviceToDevice, s));
     st->x.upload(x, n3, s);
-    run_elements(st->act, st->P, st->x.p, s);
-    gather(st->act, st->P, 1, st->im.p, st->x.p, st->x0.p, s);
+    assemble_elastic(st, st->P, s);
     contacts(st);
     st->eacc.zero(EN_TERMS, s);
     const double h = (double)st->prm.dt;
@@ -2629,7 +2666,7 @@ int nguo_step_energy(nguo_step *st, const float *x, double *terms, double *total
       const float dhat = st->prm.contact_dhat;
       hipLaunchKernelGGL(k_energy_contacts, dim3(div_up(st->ns, BLOCK)), dim3(BLOCK), 0, s, st->ns, (const uint32_t *)st->sidx.p,
                          (const float *)st->sd.p, dhat, 0.5f * (dhat + dhat), st->nvt, topo(st), st->act.M.view(),
-                         st->stat.M.view(), st->P, (const float *)st->x.p, (const float *)st->x0.p, st->eacc.p);
+                         st->stat.M.view(), st->mix.M.view(), st->P, (const float *)st->x.p, (const float *)st->x0.p, st->eacc.p);
     }
     NGUO_HIP(hipGetLastError());
     double e[EN_TERMS];
@@ -2653,8 +2690,7 @@ int nguo_step_bench_spmv(nguo_step *st, uint32_t reps, int iteration_form, doubl
     hipStream_t s = st->s;
     if (st->NF == 0) { *avg_ms = 0; *bytes = 0; return NGUO_OK; }
     // the system of the current state (elastic blocks + contact term) and a probe vector
-    run_elements(st->act, st->P, st->x.p, s);
-    gather(st->act, st->P, 1, st->im.p, st->x.p, st->x0.p, s);
+    assemble_elastic(st, st->P, s);
     contacts(st);
     ContactView C = contact_view(st);
     float *xin = st->egrad.p;
@@ -2674,8 +2710,7 @@ int nguo_step_bench_precond(nguo_step *st, uint32_t reps, double *avg_ms, double
     hipStream_t s = st->s;
     if (st->NF == 0) { *avg_ms = 0; *bytes = 0; return NGUO_OK; }
     // the system of the current state (elastic blocks + contact term)
-    run_elements(st->act, st->P, st->x.p, s);
-    gather(st->act, st->P, 1, st->im.p, st->x.p, st->x0.p, s);
+    assemble_elastic(st, st->P, s);
     contacts(st);
     *avg_ms = st->pcg->time_precond(st->act.M, contact_view(st), reps, bytes);
     return NGUO_OK;

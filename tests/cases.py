@@ -107,3 +107,33 @@ def check_stencils(ctx, s, o, x, v):
     assert np.array_equal(st.collider_sides(x), side)
     st.close()
     return idx, d
+
+
+def pinned_state(steps=14, npin=3):
+    """The spring sheet resting on the icosphere (`steps` oracle steps of spring_scene), rebuilt
+    from that state with the `npin` free vertices nearest the body pinned: springs and hinges
+    then join pinned and free vertices, and contact stencils include pinned cloth vertices.
+    Returns (scene, x, v, pinned vertex ids)."""
+    import oracle_py as O
+
+    s0 = spring_scene()
+    o = O.OracleSolver(s0)
+    o.step(steps)
+    x, v = o.positions(), o.velocities()
+    static = s0.static.astype(bool)
+    idx, d, _ = o.stencils(x)
+    free = s0.inv_mass > 0
+    near = []
+    for r, dd in sorted(zip(idx.tolist(), d.tolist()), key=lambda t: t[1]):
+        for a in r:
+            if free[a] and a not in near:
+                near.append(a)
+    pins = np.array(near[:npin], np.int64)
+    assert len(pins) == npin, "the sheet must touch the body"
+    im = s0.inv_mass.copy()
+    im[pins] = 0.0
+    v = v.copy()
+    v[pins] = 0.0
+    cols = [s0.desc.colliders[k] for k in range(s0.desc.n_colliders)]
+    s = nguo_hip.ArrayScene(x, im, s0.tris, s0.desc.params, static=static.astype(np.uint8), colliders=cols)
+    return s, x, v, pins

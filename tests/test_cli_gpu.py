@@ -90,8 +90,25 @@ def test_verify_reports_the_invariants():
 
 
 @pytest.mark.gpu
-def test_pinned_cloth_is_reported_unsupported():
-    # the hanging cloth pins two corners of the sheet: its springs and hinges join pinned and
-    # free vertices, which this build does not assemble; the run must say so and exit 1
-    r = mguso("bench", os.path.join(SCENES, "hanging_cloth.json"), "--steps", "1")
-    assert r.returncode == 1 and r.stdout == "" and "pinned" in r.stderr
+def test_hanging_cloth_matches_the_oracle(tmp_path):
+    # the hanging cloth pins two corners of the sheet (its springs and hinges join pinned and
+    # free vertices); its "is-static" key is not the Object field `is_static`, so it is ignored
+    text = json.load(open(os.path.join(SCENES, "hanging_cloth.json")))
+    text["config"]["cg-tol"] = 1e-10
+    path = str(tmp_path / "hanging_tight.json")
+    json.dump(text, open(path, "w"))
+    out_dir = tmp_path / "frames"
+    frames = 10
+    r = mguso("run", path, "--solver", "implicit", "--frames", str(frames), "--out", str(out_dir))
+    assert r.returncode == 0, r.stderr
+    scene, cfg = S.lower(path)
+    assert (scene.inv_mass == 0).sum() == 2 and scene.static is None
+    o = O.OracleSolver(scene)
+    o.step(frames * S.steps_per_frame(cfg))
+    x, _ = S.read_obj(os.path.join(out_dir, f"frame_{frames - 1:04d}.obj"))
+    ox = o.positions()
+    err = np.abs(x - ox).max() / max(1.0, np.abs(ox).max())
+    print(f"hanging cloth frame {frames - 1}: |mguso - oracle| {err:.2e}")
+    assert err <= 1e-5
+    pins = scene.inv_mass == 0
+    assert np.array_equal(x[pins], scene.positions[pins])

@@ -6,7 +6,7 @@ import pytest
 
 from conftest import nguo_hip
 import oracle_py as O
-from cases import check_stencils, inside_state, scene
+from cases import check_stencils, inside_state, pinned_state, scene
 
 POS_TOL = 1e-5  # |x_gpu - x_oracle|_inf / max(1, |x|_inf) after each Newton iteration / step
 
@@ -66,6 +66,17 @@ def test_stencils_bitexact_inside_collider(ctx):
     assert sum(1 for r, dd in zip(idx, d) if r[0] == vs and static[r[1:]].all() and dd < 0) >= 2
 
 
+@pytest.mark.gpu
+def test_stencils_bitexact_pinned_cloth(ctx):
+    # stencils with pinned cloth participants: their Rayleigh stiffness reads the elastic
+    # blocks between pinned and free vertices (the mixed-element table)
+    s, x, v, pins = pinned_state()
+    o = O.OracleSolver(s)
+    o.set_state(x, v)
+    idx, d = check_stencils(ctx, s, o, x, v)
+    assert sum(1 for r in idx if np.isin(r, pins).any()) >= 3
+
+
 # Both sides solve each Newton system to 1e-10 relative residual here, so the comparison
 # measures the implementation (assembly, contact, CCD) and not the solver tolerance: at the
 # production 1e-6 two correct solvers already differ by up to cond(H) * 1e-6 in dx.
@@ -81,11 +92,18 @@ def test_stencils_bitexact_inside_collider(ctx):
 # that stencil stiffnesses ~650x apart. Such a state records its FP64 gap as an expected
 # failure with the measured figure, instead of widening the bar.
 @pytest.mark.gpu
-@pytest.mark.parametrize("cfg,steps", [(100, 0), (100, 12), (100, 30), (0, 12), (0, 20), (0, 33), (101, 10), (101, 25), ("collider", 14), ("springs", 14)])
+@pytest.mark.parametrize("cfg,steps", [(100, 0), (100, 12), (100, 30), (0, 12), (0, 20), (0, 33), (101, 10), (101, 25), ("collider", 14),
+                                       ("springs", 14), ("pinned", 0)])
 def test_one_step_parity(ctx, cfg, steps):
-    s, o = oracle_state(cfg, steps, cg_tol=1e-10)
-    x, v = o.positions(), o.velocities()
-    o64 = O.OracleSolver(scene(cfg, cg_tol=1e-10), fp64=True)
+    if cfg == "pinned":  # cloth vertices pinned in contact: elements join pinned and free vertices
+        s, x, v, _ = pinned_state()
+        s.desc.params.cg_tol = 1e-10
+        o, o64 = O.OracleSolver(s), O.OracleSolver(s, fp64=True)
+        o.set_state(x, v)
+    else:
+        s, o = oracle_state(cfg, steps, cg_tol=1e-10)
+        x, v = o.positions(), o.velocities()
+        o64 = O.OracleSolver(scene(cfg, cg_tol=1e-10), fp64=True)
     o64.set_state(x, v)
     st = nguo_hip.Stepper(ctx, s, trace=True)
     st.set_state(x, v)
