@@ -181,6 +181,14 @@ int nguo_step_get_velocities(nguo_step *st, float *out /* [3*n_verts] */);
  * rebuilt from them); velocities may be null (kept) */
 int nguo_step_set_state(nguo_step *st, const float *positions, const float *velocities);
 int nguo_step_get_metrics(nguo_step *st, nguo_step_metrics *out);
+/* implicit.rs:1368-1398 set_kinematic_collider_positions: moves the static mesh collider to the
+ * collider rows of x ([3*n_verts]; the other rows are not read), between steps, once per step.
+ * Its positions and velocities (move / dt), pseudonormals, trees and element blocks follow, and
+ * the move enters the next step's friction slip at contacts with it (implicit.rs:430-444
+ * kinematic_dx, cleared by the step). Like the reference it teleports the body: motion beyond
+ * the barrier's reach within one step is not resolved. NGUO_E_INVALID (nothing changed) without
+ * a collider or with a non-finite collider position, where the reference returns false. */
+int nguo_step_set_kinematic_collider_positions(nguo_step *st, const float *x);
 /* Positions after Newton iteration k of the last advanced step (the oracle's per-iteration
  * parity point); *count receives the number of iterations taken, toi their CCD fraction.
  * out == NULL only queries the count (and toi of iteration k). Positions are kept only for steps advanced with

@@ -613,6 +613,7 @@ struct ContactOut {
 __global__ __launch_bounds__(BLOCK) void k_stencil_eval(uint32_t ns, const uint32_t *__restrict__ sidx, const float *__restrict__ sd,
                                                        const float *__restrict__ sg, float ghat_vt, float ghat_ee, uint32_t nvt, Topo T,
                                                        SellView K, SellView S, SellView X, Params P, const float *__restrict__ x, const float *__restrict__ xs,
+                                                       const float *__restrict__ kdx,
                                                        ContactOut C) {
   uint32_t s = blockIdx.x * BLOCK + threadIdx.x;
   if (s >= ns) return;
@@ -687,9 +688,13 @@ __global__ __launch_bounds__(BLOCK) void k_stencil_eval(uint32_t ns, const uint3
   if (act && P.mu > 0.0f && !(nlen < 1.0e-12f)) {
     // contact.rs:658-706 stencil_friction
     f3 slip{0, 0, 0};
+    // implicit.rs:3068-3080: a kinematic body vertex's slip includes the move it was given
+    // before this step (kdx, null when the body did not move)
     for (int a = 0; a < 4; ++a) {
       wv[a] = dot(g[a], nrm);
-      slip = slip + (ld3(x, id[a]) - ld3(xs, id[a])) * wv[a];
+      f3 dxa = ld3(x, id[a]) - ld3(xs, id[a]);
+      if (kdx) dxa = dxa + ld3(kdx, id[a]);
+      slip = slip + dxa * wv[a];
     }
     f3 f_c = nrm * bg;
     lam = friction(f_c, slip, nrm, P.mu, P.feps, fg);
@@ -1226,7 +1231,7 @@ __global__ __launch_bounds__(BLOCK) void k_energy_elems(ElemDev E, const double 
 __global__ __launch_bounds__(BLOCK) void k_energy_contacts(uint32_t ns, const uint32_t *__restrict__ sidx, const float *__restrict__ sd,
                                                           float ghat_vt, float ghat_ee, uint32_t nvt, Topo T,
                                                           SellView K, SellView S, SellView X, Params P, const float *__restrict__ x,
-                                                          const float *__restrict__ xs, double *acc) {
+                                                          const float *__restrict__ xs, const float *__restrict__ kdx, double *acc) {
   double e[EN_TERMS] = {};
   const uint32_t s = blockIdx.x * BLOCK + threadIdx.x;
   if (s < ns) {
@@ -1287,7 +1292,11 @@ __global__ __launch_bounds__(BLOCK) void k_energy_contacts(uint32_t ns, const ui
       const double cf = -(double)P.mu * bo[1];
       if (P.mu > 0.0f && has_n && cf > 0.0) {
         d3 slip{0, 0, 0};
-        for (int a = 0; a < 4; ++a) slip = slip + (to_d(ld3(x, id[a])) - to_d(ld3(xs, id[a]))) * w[a];
+        for (int a = 0; a < 4; ++a) {
+          d3 dxa = to_d(ld3(x, id[a])) - to_d(ld3(xs, id[a]));
+          if (kdx) dxa = dxa + to_d(ld3(kdx, id[a]));
+          slip = slip + dxa * w[a];
+        }
         e[EN_FRICTION] = friction_potential(cf, n, slip, fmax((double)P.feps, 1.0e-12));
       }
     }
@@ -1353,6 +1362,12 @@ struct nguo_step {
   // elements: active (system), static (all pinned) and mixed (pinned and free: contact blocks)
   ElemSet act, stat, mix;
   DevBuf<uint32_t> xdof;  // vertex -> row of the mixed-element table
+  // kinematic collider (implicit.rs:1324-1398): the move given to its vertices since the last
+  // step, read by the friction slip and cleared by the step; the collider triangles' ids
+  DevBuf<float> kdx;
+  std::vector<float> h_kdx;
+  bool kdx_on = false;
+  std::vector<uint32_t> h_coll_ids;
   // contact buffers
   uint32_t cap = 0;
   DevBuf<uint32_t> vt_count, vt_ord, ee_ord;
@@ -1803,7 +1818,7 @@ void contacts(nguo_step *st) {
   ContactOut CO{st->cdof.p, st->cg.p, st->cw.p, st->ccoef.p, st->cn.p, st->cgrad.p, st->ckappa.p};
   hipLaunchKernelGGL(k_stencil_eval, dim3(div_up(st->ns, BLOCK)), dim3(BLOCK), 0, s, st->ns, (const uint32_t *)st->sidx.p, (const float *)st->sd.p,
                      (const float *)st->sg.p, dhat, 0.5f * (dhat + dhat), st->nvt, T, st->act.M.view(), st->stat.M.view(), st->mix.M.view(), st->P,
-                     (const float *)st->x.p, (const float *)st->x0.p, CO);
+                     (const float *)st->x.p, (const float *)st->x0.p, st->kdx_on ? (const float *)st->kdx.p : nullptr, CO);
   uint32_t m = 4 * st->ns;
   hipLaunchKernelGGL(k_inc_keys, dim3(div_up(m, BLOCK)), dim3(BLOCK), 0, s, st->ns, (const uint32_t *)st->cdof.p, st->NF, st->inc_key.p, st->inc_val.p);
   st->sorter.sort_u64(m, st->inc_key.p, st->inc_val.p, digit_shifts(m, st->NF + 1), s);
@@ -1881,6 +1896,41 @@ float ccd_max_step(nguo_step *st, float max_disp) {
 // Trees over the pinned primitives (triangles and edges with every vertex pinned) and the
 // collider's triangles: their positions do not change while stepping, so they are built from
 // the current positions at creation and when a state is set, not per Newton iteration.
+// contact.rs:943-981 pseudonormals of the collider triangles at positions X (3V floats):
+// vertices welded on exact bits (-0 folded), face normals summed with their corner angles.
+std::vector<float> collider_pseudonormals(const nguo_step *st, const float *X) {
+  const uint32_t V = st->V;
+  std::vector<float> pn(3 * (size_t)V, 0.0f);
+  std::vector<std::pair<std::array<uint32_t, 3>, uint32_t>> keys(V);
+  auto bits = [](float c) { uint32_t u; float z = c == 0.0f ? 0.0f : c; std::memcpy(&u, &z, 4); return u; };
+  for (uint32_t i = 0; i < V; ++i) keys[i] = {{bits(X[3 * i]), bits(X[3 * i + 1]), bits(X[3 * i + 2])}, i};
+  std::sort(keys.begin(), keys.end());
+  std::vector<uint32_t> of(V);
+  uint32_t nw = 0;
+  for (uint32_t i = 0; i < V; ++i) {
+    if (i > 0 && keys[i].first != keys[i - 1].first) nw++;
+    of[keys[i].second] = nw;
+  }
+  std::vector<f3> acc(nw + 1, f3{0, 0, 0});
+  for (uint32_t f : st->h_coll_ids) {
+    const uint32_t *t = &st->h_tris[3 * f];
+    f3 p[3];
+    for (int k = 0; k < 3; ++k) p[k] = f3{X[3 * t[k]], X[3 * t[k] + 1], X[3 * t[k] + 2]};
+    f3 face = normalize_or_zero(cross(p[1] - p[0], p[2] - p[0]));
+    if (face.x == 0.0f && face.y == 0.0f && face.z == 0.0f) continue;
+    for (int k = 0; k < 3; ++k) {
+      f3 u = normalize_or_zero(p[(k + 1) % 3] - p[k]);
+      f3 w = normalize_or_zero(p[(k + 2) % 3] - p[k]);
+      acc[of[t[k]]] = acc[of[t[k]]] + face * std::acos(clampv(dot(u, w), -1.0f, 1.0f));
+    }
+  }
+  for (uint32_t i = 0; i < V; ++i) {
+    f3 n = normalize_or_zero(acc[of[i]]);
+    pn[3 * i] = n.x; pn[3 * i + 1] = n.y; pn[3 * i + 2] = n.z;
+  }
+  return pn;
+}
+
 void build_static_trees(nguo_step *st, hipStream_t s) {
   st->tree_tri_stat.build(0, st->n_tri_stat, st->tri_stat_ids.p, st->tris.p, st->x.p, s);
   st->tree_edge_stat.build(1, st->n_edge_stat, st->edge_stat_ids.p, st->edges.p, st->x.p, s);
@@ -2010,6 +2060,11 @@ void advance_one(nguo_step *st) {
   float vmax = reduce_max(st, GV);
   uint32_t bad = read_u32(st, st->flag.p);
   st->time += (double)h;
+  if (st->kdx_on) {  // implicit.rs:2065-2070: the collider's move described this step only
+    NGUO_HIP(hipMemsetAsync(st->kdx.p, 0, 3 * (size_t)V * sizeof(float), s));
+    std::fill(st->h_kdx.begin(), st->h_kdx.end(), 0.0f);
+    st->kdx_on = false;
+  }
   // penetration_free: finite, analytic colliders respected, no edge pierces a triangle
   uint32_t hit = 0, ncontacts = 0;
   if (st->P.ncoll && !bad) {
@@ -2349,37 +2404,11 @@ int nguo_step_create(nguo_ctx *ctx, const nguo_scene_desc *sc, nguo_step **out) 
       }
     }
     st->has_collider = !coll_ids.empty();
+    st->h_coll_ids = coll_ids;
     std::vector<float> pn(3 * (size_t)V, 0.0f);
     if (st->has_collider) {
       st->h_static.assign(sc->static_collider, sc->static_collider + V);
-      // contact.rs:943-981 pseudonormals: welded on exact bits (-0 folded), angle-weighted
-      std::vector<std::pair<std::array<uint32_t, 3>, uint32_t>> keys(V);
-      auto bits = [](float c) { uint32_t u; float z = c == 0.0f ? 0.0f : c; std::memcpy(&u, &z, 4); return u; };
-      for (uint32_t i = 0; i < V; ++i) keys[i] = {{bits(X[3 * i]), bits(X[3 * i + 1]), bits(X[3 * i + 2])}, i};
-      std::sort(keys.begin(), keys.end());
-      std::vector<uint32_t> of(V);
-      uint32_t nw = 0;
-      for (uint32_t i = 0; i < V; ++i) {
-        if (i > 0 && keys[i].first != keys[i - 1].first) nw++;
-        of[keys[i].second] = nw;
-      }
-      std::vector<f3> acc(nw + 1, f3{0, 0, 0});
-      for (uint32_t f : coll_ids) {
-        const uint32_t *t = &st->h_tris[3 * f];
-        f3 p[3];
-        for (int k = 0; k < 3; ++k) p[k] = f3{X[3 * t[k]], X[3 * t[k] + 1], X[3 * t[k] + 2]};
-        f3 face = normalize_or_zero(cross(p[1] - p[0], p[2] - p[0]));
-        if (face.x == 0.0f && face.y == 0.0f && face.z == 0.0f) continue;
-        for (int k = 0; k < 3; ++k) {
-          f3 u = normalize_or_zero(p[(k + 1) % 3] - p[k]);
-          f3 w = normalize_or_zero(p[(k + 2) % 3] - p[k]);
-          acc[of[t[k]]] = acc[of[t[k]]] + face * std::acos(clampv(dot(u, w), -1.0f, 1.0f));
-        }
-      }
-      for (uint32_t i = 0; i < V; ++i) {
-        f3 n = normalize_or_zero(acc[of[i]]);
-        pn[3 * i] = n.x; pn[3 * i + 1] = n.y; pn[3 * i + 2] = n.z;
-      }
+      pn = collider_pseudonormals(st.get(), X);
       st->static_v.upload(st->h_static, s);
     } else {
       st->static_v.zero(V, s);
@@ -2477,6 +2506,54 @@ int nguo_step_set_state(nguo_step *st, const float *positions, const float *velo
     record_pinned(st, positions);
     st->adj_valid = false;
     NGUO_HIP(hipStreamSynchronize(st->s));
+    return NGUO_OK;
+  });
+}
+
+int nguo_step_set_kinematic_collider_positions(nguo_step *st, const float *x) {
+  return guarded([&]() -> int {
+    NGUO_REQUIRE(st && x, NGUO_E_INVALID, "nguo_step_set_kinematic_collider_positions: null argument");
+    NGUO_REQUIRE(st->has_collider, NGUO_E_INVALID, "nguo_step_set_kinematic_collider_positions: the scene has no static mesh collider");
+    const uint32_t V = st->V;
+    for (uint32_t i = 0; i < V; ++i) {
+      if (!st->h_static[i]) continue;
+      NGUO_REQUIRE(std::isfinite(x[3 * i]) && std::isfinite(x[3 * i + 1]) && std::isfinite(x[3 * i + 2]), NGUO_E_INVALID,
+                   "nguo_step_set_kinematic_collider_positions: non-finite position on collider vertex " + std::to_string(i));
+      NGUO_REQUIRE(!(st->h_im[i] > 0.0f), NGUO_E_UNSUPPORTED,
+                   "nguo_step_set_kinematic_collider_positions: collider vertex " + std::to_string(i) + " is not pinned");
+    }
+    hipStream_t s = st->s;
+    std::vector<float> pos(3 * (size_t)V), vel(3 * (size_t)V);
+    st->x.download(pos.data(), 3 * (size_t)V, s);
+    st->v.download(vel.data(), 3 * (size_t)V, s);
+    NGUO_HIP(hipStreamSynchronize(s));
+    if (st->h_kdx.empty()) st->h_kdx.assign(3 * (size_t)V, 0.0f);
+    st->kdx.reserve(3 * (size_t)V);
+    const float inv_h = 1.0f / st->prm.dt;
+    for (uint32_t i = 0; i < V; ++i) {
+      if (!st->h_static[i]) continue;
+      for (int c = 0; c < 3; ++c) {
+        const size_t k = 3 * (size_t)i + c;
+        const float d = x[k] - pos[k];
+        st->h_kdx[k] += d;
+        pos[k] = x[k];
+        vel[k] = d * inv_h;
+      }
+    }
+    st->x.upload(pos, s);
+    st->x0.upload(pos, s);
+    st->v.upload(vel, s);
+    st->kdx.upload(st->h_kdx, s);
+    st->kdx_on = true;
+    record_pinned(st, pos.data());
+    st->pn.upload(collider_pseudonormals(st, pos.data()), s);
+    build_static_trees(st, s);
+    if (st->stat.M.nb) {  // the body's own element blocks at its new pose
+      run_elements(st->stat, st->P, st->x.p, s);
+      gather(st->stat, st->P, 0, st->im.p, st->x.p, st->x0.p, s);
+    }
+    st->adj_valid = false;
+    NGUO_HIP(hipStreamSynchronize(s));
     return NGUO_OK;
   });
 }
@@ -2666,7 +2743,8 @@ int nguo_step_energy(nguo_step *st, const float *x, double *terms, double *total
       const float dhat = st->prm.contact_dhat;
       hipLaunchKernelGGL(k_energy_contacts, dim3(div_up(st->ns, BLOCK)), dim3(BLOCK), 0, s, st->ns, (const uint32_t *)st->sidx.p,
                          (const float *)st->sd.p, dhat, 0.5f * (dhat + dhat), st->nvt, topo(st), st->act.M.view(),
-                         st->stat.M.view(), st->mix.M.view(), st->P, (const float *)st->x.p, (const float *)st->x0.p, st->eacc.p);
+                         st->stat.M.view(), st->mix.M.view(), st->P, (const float *)st->x.p, (const float *)st->x0.p,
+                         st->kdx_on ? (const float *)st->kdx.p : nullptr, st->eacc.p);
     }
     NGUO_HIP(hipGetLastError());
     double e[EN_TERMS];

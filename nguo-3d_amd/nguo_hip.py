@@ -32,6 +32,7 @@ EXPORTED = [
     "nguo_batch_create", "nguo_batch_destroy", "nguo_batch_advance", "nguo_batch_instance",
     "nguo_step_set_rest_scale", "nguo_step_adjoint", "nguo_batch_set_solve", "nguo_bench_sort",
     "nguo_step_set_trace", "nguo_lbvh_sort_pairs_noalloc", "nguo_step_energy", "nguo_step_set_precision",
+    "nguo_step_set_kinematic_collider_positions",
 ]
 
 
@@ -125,6 +126,7 @@ def load():
         lib.nguo_step_get_velocities.argtypes = [vp, P(C.c_float)]
         lib.nguo_step_set_state.argtypes = [vp, P(C.c_float), P(C.c_float)]
         lib.nguo_step_get_metrics.argtypes = [vp, P(StepMetrics)]
+        lib.nguo_step_set_kinematic_collider_positions.argtypes = [vp, P(C.c_float)]
         lib.nguo_step_get_newton_trace.argtypes = [vp, C.c_uint32, P(C.c_float), P(C.c_uint32), P(C.c_float)]
         lib.nguo_step_set_trace.argtypes = [vp, C.c_int]
         lib.nguo_step_set_precision.argtypes = [vp, C.c_int]
@@ -506,6 +508,12 @@ class Stepper:
         tot = C.c_double()
         check(load().nguo_step_energy(self.h, fptr(x), terms.ctypes.data_as(C.POINTER(C.c_double)), C.byref(tot)))
         return tot.value, terms
+
+    def set_kinematic(self, x):
+        """Move the static mesh collider to x's collider rows before the next step
+        (ImplicitSolver::set_kinematic_collider_positions)."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        check(load().nguo_step_set_kinematic_collider_positions(self.h, fptr(x)))
 
     def collider_sides(self, x):
         x = np.ascontiguousarray(x, dtype=np.float32)

@@ -145,6 +145,14 @@ int64_t oracle_solver_stencils(void *p, const float *x, uint32_t cap, uint32_t *
   return h->fp64 ? go(*h->s64, 0.0) : go(*h->s32, 0.0f);
 }
 
+// implicit.rs:1368-1398 set_kinematic_collider_positions (x: all vertices, f32); 1 = applied.
+int oracle_solver_set_kinematic(void *p, const float *x) {
+  auto *h = (Handle *)p;
+  uint32_t n = oracle_solver_nverts(p);
+  return h->fp64 ? (h->s64->set_kinematic_collider_positions(load<double>(x, n)) ? 1 : 0)
+                 : (h->s32->set_kinematic_collider_positions(load<float>(x, n)) ? 1 : 0);
+}
+
 // Collider side (+1 / -1) per vertex at x; returns 0 when the scene has no mesh collider.
 int oracle_solver_collider_sides(void *p, const float *x, float *side) {
   auto *h = (Handle *)p;

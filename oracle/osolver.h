@@ -414,6 +414,10 @@ template <class T> class Solver {
   // i.e. Dm^-1 / s and area s^2 (the parameters of the drape adjoint).
   std::vector<T> face_scale;
   std::vector<V> last_xt;  // inertial target of the last step
+  // implicit.rs:430-444 kinematic_dx: the static collider's displacement set since the last
+  // step (the friction slip override), consumed and cleared by step(); empty = never moved
+  std::vector<V> kinematic_dx;
+  V body_dx(uint32_t i) const { return kinematic_dx.empty() ? V{} : kinematic_dx[i]; }
   bool project_elastic = true;  // PSD-project shell / rod-segment Hessians (false: exact)
   bool with_rod_hinges = true;  // false: leave the strand bending terms out of the assembly
 
@@ -560,6 +564,27 @@ template <class T> class Solver {
     static_vert.assign(mask, mask + pos.size());
     collider_tris = marked;
     collider_pn = pseudonormals<T>(pos, collider_tris);
+  }
+
+  // implicit.rs:1368-1398 set_kinematic_collider_positions: teleport the static collider's
+  // vertices to x (only its rows are read), set their velocities to the move over dt,
+  // accumulate the move into kinematic_dx and re-bake the pseudonormals. false (and nothing
+  // changed) without a collider or with a non-finite position on a collider row.
+  bool set_kinematic_collider_positions(const std::vector<V> &x) {
+    if (static_vert.empty() || x.size() != pos.size()) return false;
+    for (size_t i = 0; i < x.size(); ++i)
+      if (static_vert[i] && !finite3(x[i])) return false;
+    if (kinematic_dx.empty()) kinematic_dx.assign(x.size(), V{});
+    const T inv_h = T(1) / T(prm.dt);
+    for (size_t i = 0; i < x.size(); ++i) {
+      if (!static_vert[i]) continue;
+      const V d = x[i] - pos[i];
+      kinematic_dx[i] += d;
+      pos[i] = x[i];
+      vel[i] = d * inv_h;
+    }
+    collider_pn = pseudonormals<T>(pos, collider_tris);
+    return true;
   }
 
   bool movable(uint32_t v) const { return inv_mass[v] > 0.0f; }
@@ -979,7 +1004,7 @@ template <class T> class Solver {
             const V n0 = gn->fric_n[fi];
             const auto &w = gn->fric_w[fi];
             V slip{};
-            for (int a = 0; a < 4; ++a) slip += (x[s.idx[a]] - pos[s.idx[a]]) * w[a];
+            for (int a = 0; a < 4; ++a) slip += (x[s.idx[a]] - pos[s.idx[a]] + body_dx(s.idx[a])) * w[a];
             const T lam = gn->fric_lam[fi];
             const V fg = (slip - dot(n0, slip) * n0) * lam;
             for (int a = 0; a < 4; ++a) {
@@ -996,8 +1021,9 @@ template <class T> class Solver {
             }
           }
         } else if (mu_s > T(0)) {
+          // implicit.rs:3068-3080: the slip of a kinematic body vertex includes its move
           V dxs[4];
-          for (int a = 0; a < 4; ++a) dxs[a] = x[s.idx[a]] - pos[s.idx[a]];
+          for (int a = 0; a < 4; ++a) dxs[a] = x[s.idx[a]] - pos[s.idx[a]] + body_dx(s.idx[a]);
           V fg[4];
           T w[4], fh[9];
           if (stencil_friction(s.g, dxs, bg, mu_s, T(prm.friction_eps), fg, w, fh)) {
@@ -1091,7 +1117,7 @@ template <class T> class Solver {
       const bool act = barrier_of(kind, s.d, s.ghat, f.kappa[i], bo);
       V dxs[4], fg[4];
       T w[4] = {0, 0, 0, 0}, fh[9];
-      for (int a = 0; a < 4; ++a) dxs[a] = x[s.idx[a]] - pos[s.idx[a]];
+      for (int a = 0; a < 4; ++a) dxs[a] = x[s.idx[a]] - pos[s.idx[a]] + body_dx(s.idx[a]);
       const bool fr = act && mu > T(0) && stencil_friction(s.g, dxs, bo[1], mu, eps, fg, w, fh);
       V n{};
       T lam = 0;
@@ -1212,7 +1238,7 @@ template <class T> class Solver {
       if (barrier_of(f.kind[si], stencil_gap(x, s), s.ghat, f.kappa[si], bo)) out[E_BARRIER] += bo[0];
       if (f.fric[si]) {
         V slip{};
-        for (int a = 0; a < 4; ++a) slip += (x[s.idx[a]] - pos[s.idx[a]]) * f.fric_w[si][a];
+        for (int a = 0; a < 4; ++a) slip += (x[s.idx[a]] - pos[s.idx[a]] + body_dx(s.idx[a])) * f.fric_w[si][a];
         out[E_FRICTION] += friction_potential(f.fric_c[si], f.fric_n[si], slip, eps);
       }
     }
@@ -1482,6 +1508,8 @@ template <class T> class Solver {
       }
     }
     time += (double)h;
+    // implicit.rs:2065-2070: the collider's move described this step only
+    for (auto &d : kinematic_dx) d = V{};
     last_newton = taken;
     last_all_finite = fin;
     // implicit.rs:2235-2266 penetration_free: analytic colliders, floor, edge-triangle pierce.

@@ -43,6 +43,8 @@ def load():
         lib.oracle_solver_newton_trace.argtypes = [vp, C.c_uint32, F, D]
         lib.oracle_solver_stencils.restype = C.c_int64
         lib.oracle_solver_stencils.argtypes = [vp, F, C.c_uint32, U, F, F]
+        lib.oracle_solver_set_kinematic.restype = C.c_int
+        lib.oracle_solver_set_kinematic.argtypes = [vp, F]
         lib.oracle_solver_collider_sides.restype = C.c_int
         lib.oracle_solver_collider_sides.argtypes = [vp, F, F]
         lib.oracle_solver_assemble.restype = C.c_int64
@@ -211,6 +213,10 @@ class OracleSolver:
         g = np.zeros((max(cnt, 1), 12), np.float32)
         self.lib.oracle_solver_stencils(self.h, fp(x), cnt, up(idx), fp(d), fp(g))
         return idx[:cnt], d[:cnt], g[:cnt]
+
+    def set_kinematic(self, x):
+        """Teleport the static collider to x's collider rows (True = applied)."""
+        return bool(self.lib.oracle_solver_set_kinematic(self.h, fp(f32(x))))
 
     def collider_sides(self, x):
         x = f32(x)
