@@ -131,8 +131,8 @@ typedef struct nguo_step_metrics {
   int32_t all_finite;
   float last_toi;           /* CCD step fraction of the last Newton iteration */
   uint32_t num_contacts;    /* free vertices within dhat of an analytic collider (implicit.rs:2212-2230) */
-  float max_sigma;          /* max stretch ratio over edge-spring constraints (implicit.rs:2194-2209);
-                               0: the scene description carries none */
+  float max_sigma;          /* max |x_a - x_b| / rest over the triangle mesh's unique edges, rest from
+                               the scene positions (implicit.rs:2193-2209, state.rs:39-65); 0: no triangles */
   uint32_t device_alloc;    /* device allocations made during the step (0 once capacities settle) */
   uint32_t device_free;
   uint32_t sum_cg_iters;    /* inner PCG iterations summed over the step's Newton solves */

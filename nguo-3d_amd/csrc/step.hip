@@ -1072,6 +1072,23 @@ __global__ __launch_bounds__(BLOCK) void k_collider_metrics(uint32_t V, const fl
   if (pen) atomicOr(out + 1, 1u);
 }
 
+// implicit.rs:2193-2209 max_stretch_ratio: the largest |x_a - x_b| / rest over the mesh's
+// unique edges (state.rs:39-65), 1 for an edge with rest <= 1e-9; a NaN ratio drops out as
+// f32::max drops it. Ratios are non-negative, so their bit patterns order as the values.
+__global__ __launch_bounds__(BLOCK) void k_max_sigma(uint32_t n, const uint32_t *__restrict__ ev, const float *__restrict__ rest,
+                                                    const float *__restrict__ x, uint32_t *out) {
+  const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  float r = 0.0f;
+  if (i < n) {
+    const f3 d = ld3(x, ev[2 * (size_t)i]) - ld3(x, ev[2 * (size_t)i + 1]);
+    const float l = sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
+    r = rest[i] > 1.0e-9f ? l / rest[i] : 1.0f;
+    if (!(r >= 0.0f)) r = 0.0f;
+  }
+  r = wave_max(r);
+  if ((threadIdx.x & 63) == 0 && r > 0.0f) atomicMax(out, __float_as_uint(r));
+}
+
 // implicit.rs:618-650 positions_self_intersect: any non-inert, non-adjacent (edge, triangle)
 // with a transverse pierce. Each edge queries both triangle trees (movable edges) or the
 // dynamic one (static edges).
@@ -1391,6 +1408,9 @@ struct nguo_step {
   unsigned long long *hl = nullptr;
   nguo_step_metrics metrics{};
   DevBuf<double> eacc;  // step energy terms (nguo_step_energy)
+  DevBuf<uint32_t> sig_v;  // the mesh's unique edges and their rest lengths (max_sigma)
+  DevBuf<float> sig_rest;
+  uint32_t n_sig = 0;
   DevBuf<float> trace;  // x after every Newton iteration of the last step (when trace_on)
   bool trace_on = false;
   uint32_t trace_count = 0;  // iterates held in trace
@@ -2095,7 +2115,14 @@ void advance_one(nguo_step *st) {
   m.penetration_free = (!bad && !hit) ? 1 : 0;
   m.last_toi = last_t;
   m.num_contacts = ncontacts;
-  m.max_sigma = 0.0f;  // no edge-spring constraints in the scene description
+  m.max_sigma = 0.0f;
+  if (st->n_sig) {
+    NGUO_HIP(hipMemsetAsync(st->flag.p + 3, 0, sizeof(uint32_t), s));
+    hipLaunchKernelGGL(k_max_sigma, dim3(div_up(st->n_sig, BLOCK)), dim3(BLOCK), 0, s, st->n_sig, (const uint32_t *)st->sig_v.p,
+                       (const float *)st->sig_rest.p, (const float *)st->x.p, st->flag.p + 3);
+    uint32_t bits = read_u32(st, st->flag.p + 3);
+    std::memcpy(&m.max_sigma, &bits, sizeof bits);
+  }
   m.device_alloc = (uint32_t)(g_dev_allocs - allocs0);
   m.device_free = (uint32_t)(g_dev_frees - frees0);
   st->adj_valid = !bad;
@@ -2249,28 +2276,40 @@ int nguo_step_create(nguo_ctx *ctx, const nguo_scene_desc *sc, nguo_step **out) 
         H->rh_stiff.push_back(sc->rod_hinge_stiff[h]);
       }
     }
-    if (st->prm.stretch_k > 0.0f) {
-      // state.rs:39-65 SimState::from_mesh: one spring per unique triangle edge, in the
-      // sorted order of geometry.rs:37-45, rest length |x_a - x_b| of the scene positions
-      std::vector<std::pair<uint32_t, uint32_t>> ue;
-      ue.reserve(3 * (size_t)st->F);
-      for (uint32_t f = 0; f < st->F; ++f) {
-        const uint32_t *t = &st->h_tris[3 * f];
-        for (int k = 0; k < 3; ++k) ue.push_back({std::min(t[k], t[(k + 1) % 3]), std::max(t[k], t[(k + 1) % 3])});
+    // state.rs:39-65 SimState::from_mesh: one stretch constraint per unique triangle edge, in
+    // the sorted order of geometry.rs:37-45, rest length |x_a - x_b| of the scene positions.
+    // They carry the max_sigma channel always and the edge springs when stretch_k > 0.
+    std::vector<std::pair<uint32_t, uint32_t>> ue;
+    ue.reserve(3 * (size_t)st->F);
+    for (uint32_t f = 0; f < st->F; ++f) {
+      const uint32_t *t = &st->h_tris[3 * f];
+      for (int k = 0; k < 3; ++k) ue.push_back({std::min(t[k], t[(k + 1) % 3]), std::max(t[k], t[(k + 1) % 3])});
+    }
+    std::sort(ue.begin(), ue.end());
+    ue.erase(std::unique(ue.begin(), ue.end()), ue.end());
+    std::vector<uint32_t> sig_v;
+    std::vector<float> sig_rest;
+    sig_v.reserve(2 * ue.size());
+    sig_rest.reserve(ue.size());
+    for (auto &e : ue) {
+      const uint32_t v[2] = {e.first, e.second};
+      const float dx = X[3 * v[0]] - X[3 * v[1]], dy = X[3 * v[0] + 1] - X[3 * v[1] + 1], dz = X[3 * v[0] + 2] - X[3 * v[1] + 2];
+      const float rest = std::sqrt(dx * dx + dy * dy + dz * dz);
+      sig_v.insert(sig_v.end(), v, v + 2);
+      sig_rest.push_back(rest);
+      if (!(st->prm.stretch_k > 0.0f)) continue;
+      const double ex = (double)X[3 * v[0]] - X[3 * v[1]], ey = (double)X[3 * v[0] + 1] - X[3 * v[1] + 1],
+                   ez = (double)X[3 * v[0] + 2] - X[3 * v[1] + 2];
+      for (ElemHost *H : homes(classify(v, 2))) {
+        H->sp_v.insert(H->sp_v.end(), v, v + 2);
+        H->sp_rest.push_back(rest);
+        H->sp_rest64.push_back(std::sqrt(ex * ex + ey * ey + ez * ez));
       }
-      std::sort(ue.begin(), ue.end());
-      ue.erase(std::unique(ue.begin(), ue.end()), ue.end());
-      for (auto &e : ue) {
-        const uint32_t v[2] = {e.first, e.second};
-        const float dx = X[3 * v[0]] - X[3 * v[1]], dy = X[3 * v[0] + 1] - X[3 * v[1] + 1], dz = X[3 * v[0] + 2] - X[3 * v[1] + 2];
-        const double ex = (double)X[3 * v[0]] - X[3 * v[1]], ey = (double)X[3 * v[0] + 1] - X[3 * v[1] + 1],
-                     ez = (double)X[3 * v[0] + 2] - X[3 * v[1] + 2];
-        for (ElemHost *H : homes(classify(v, 2))) {
-          H->sp_v.insert(H->sp_v.end(), v, v + 2);
-          H->sp_rest.push_back(std::sqrt(dx * dx + dy * dy + dz * dz));
-          H->sp_rest64.push_back(std::sqrt(ex * ex + ey * ey + ez * ez));
-        }
-      }
+    }
+    st->n_sig = (uint32_t)ue.size();
+    if (st->n_sig) {
+      st->sig_v.upload(sig_v.data(), sig_v.size(), s);
+      st->sig_rest.upload(sig_rest.data(), sig_rest.size(), s);
     }
     // static rows: pinned vertices touched by static elements
     std::vector<uint32_t> sdof(V, NONE), srow_vert;

@@ -137,3 +137,23 @@ def pinned_state(steps=14, npin=3):
     cols = [s0.desc.colliders[k] for k in range(s0.desc.n_colliders)]
     s = nguo_hip.ArrayScene(x, im, s0.tris, s0.desc.params, static=static.astype(np.uint8), colliders=cols)
     return s, x, v, pins
+
+
+def max_stretch_ratio(x, rest_pos, tris):
+    """implicit.rs:2193-2209 max_stretch_ratio over the mesh's unique edges (state.rs:39-65),
+    in FP32 with glam's length order ((x*x + y*y) + z*z): the max_sigma channel."""
+    t = np.asarray(tris, dtype=np.int64)
+    e = np.sort(np.concatenate([t[:, [0, 1]], t[:, [1, 2]], t[:, [2, 0]]]), axis=1)
+    e = np.unique(e, axis=0)
+    if len(e) == 0:
+        return np.float32(0.0)
+
+    def length(p):
+        d = p[e[:, 0]].astype(np.float32) - p[e[:, 1]].astype(np.float32)
+        return np.sqrt((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2])
+
+    rest, cur = length(rest_pos), length(x)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        r = np.where(rest > np.float32(1e-9), cur / rest, np.float32(1.0)).astype(np.float32)
+    r = r[~np.isnan(r)]
+    return np.float32(r.max()) if len(r) else np.float32(0.0)

@@ -6,7 +6,7 @@ import pytest
 
 from conftest import nguo_hip
 import oracle_py as O
-from cases import check_stencils, inside_state, pinned_state, scene
+from cases import check_stencils, inside_state, max_stretch_ratio, pinned_state, scene
 
 POS_TOL = 1e-5  # |x_gpu - x_oracle|_inf / max(1, |x|_inf) after each Newton iteration / step
 
@@ -172,6 +172,8 @@ def test_sheet_drape_invariants_36_steps(ctx):
         steps.append(m)
         assert m.all_finite == 1 and m.penetration_free == 1, k
         assert m.max_reresid <= 1e-6
+        # max_sigma over the mesh edges, the sphere's included (their ratio stays 1)
+        assert m.max_sigma == max_stretch_ratio(st.positions(), s.positions, s.tris), (k, m.max_sigma)
     # the reference's acceptance gate (metrics.rs:84-183) over the device metrics, including
     # no device allocation once the capacities have settled
     ok, checks = nguo_hip.check_invariants(steps, s.params.cg_tol, s.params.cg_max_iter)
