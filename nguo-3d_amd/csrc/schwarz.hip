@@ -261,13 +261,53 @@ __device__ inline void sym_tile_pq(uint32_t tile, uint32_t &P, uint32_t &Q) {
 // The factorisation and the inverse read nothing above the diagonal.
 __host__ __device__ constexpr uint32_t tri(uint32_t i) { return i * (i + 1) / 2; }
 
-// Floor, FP64 Cholesky (diagonal fallback) and explicit FP32 inverse of the symmetric m x m
-// block whose lower triangle is packed in L, written to out: TILED, as the 21 upper tiles
-// above; otherwise row-major with row stride MD (columns m..m4-1 zero). All threads of the
-// workgroup call it; the LDS arrays are the caller's.
+// Lower 16 x 16 tiles of the symmetric 96 x 96 block during the factorisation: tile (I, J),
+// J <= I, has index I (I + 1) / 2 + J; wavefront w of the workgroup holds tiles w, w + 4, ...
+// (at most SW_K per wavefront) in registers, in the matrix cores' result layout (lane l,
+// register v: row 4 v + l / 16, column l % 16). Diagonal tiles are held whole.
+constexpr uint32_t SW_K = (SCH_NT + 3) / 4;
+__device__ inline void lower_tile_ij(uint32_t tile, uint32_t &I, uint32_t &J) {
+  I = 0;
+  while (tile > I) { tile -= I + 1; ++I; }
+  J = tile;
+}
+__device__ inline double pick4(const double4_t &a, uint32_t v) { return v == 0 ? a[0] : v == 1 ? a[1] : v == 2 ? a[2] : a[3]; }
+// x[k], k < 4 lane-dependent, as two levels of selects (no divergent branches)
+__device__ inline double sel4(uint32_t k, double x0, double x1, double x2, double x3) {
+  const double lo = (k & 1u) ? x1 : x0, hi = (k & 1u) ? x3 : x2;
+  return (k & 2u) ? hi : lo;
+}
+__device__ inline void put4(double4_t &a, uint32_t v, double x) {
+  if (v == 0) a[0] = x;
+  else if (v == 1) a[1] = x;
+  else if (v == 2) a[2] = x;
+  else a[3] = x;
+}
+
+// Floor, FP64 inverse (diagonal fallback) of the symmetric m x m block whose lower triangle
+// is packed in L (entry (i, j), j <= i, at tri(i) + j), written to out: TILED, as the 21 upper
+// tiles above; otherwise row-major with row stride MD (columns m..m4-1 zero). All threads of
+// the workgroup call it; L and diag0 are the caller's LDS.
+//
+// The inverse is the symmetric sweep (Gauss-Jordan elimination on an SPD matrix) over pivot
+// blocks of four indices p0 .. p0 + 3: with D = S^-1 of the current 4 x 4 pivot block S,
+//   A_ij <- A_ij - (A_iP D) A_Pj   (i, j outside P),  A_iP <- A_iP D,  A_PP <- -D,
+// and after every index has been swept the block holds -A^-1. The Schur update of a 16 x 16
+// tile is one rank-4 product on the FP64 matrix cores (v_mfma_f64_16x16x4f64: A operand lane
+// a = (row a % 16, k a / 16), B operand lane b = (k b / 16, column b % 16)), with the tiles
+// kept in registers across all sweeps; the four pivot columns are published to LDS by the
+// tiles that hold them (double-buffered, one workgroup barrier per sweep). Every thread
+// factors the 4 x 4 pivot block itself (Cholesky, then D = W^T W with W = L^-1). The pivots
+// of those Cholesky factorisations are the Schur-complement pivots a_jj - sum_k l_jk^2 of the
+// full Cholesky in order, so the reference's failure test (pivot <= 1e-14, linsolve.rs:80-99)
+// and its diagonal fallback apply unchanged. Per sweep a wavefront loads the operands of all
+// its tiles, issues the rank-4 products back to back, then rewrites the pivot rows and
+// columns; lane-dependent choices are selects, not branches. The kernels that call this fit
+// 168 VGPRs (three workgroups per compute unit: the 651 domains of a 20k-vertex garment run
+// in one round).
 template <bool TILED, class TOut = float>
-__device__ void factor_invert(double *L, double *diag0, double *colj, double &floor_v, int &failed, uint32_t m, TOut *out, uint32_t *fails) {
-  const uint32_t t = threadIdx.x, ti = t / 16, tj = t % 16;
+__device__ void factor_invert(double *L, double *diag0, double &floor_v, uint32_t m, TOut *out, uint32_t *fails) {
+  const uint32_t t = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(t / WAVE), lane = t % WAVE, r16 = lane % 16, q4 = lane / 16;
   // floor 1e-6 tr/m + 1e-8 (linsolve.rs:232-238)
   if (t == 0) {
     double tr = 0.0;
@@ -277,32 +317,133 @@ __device__ void factor_invert(double *L, double *diag0, double *colj, double &fl
   __syncthreads();
   for (uint32_t k = t; k < m; k += BLOCK) { L[tri(k) + k] += floor_v; diag0[k] = L[tri(k) + k]; }
   __syncthreads();
-  // Cholesky, right-looking: after column j is scaled, its outer product leaves the trailing
-  // lower triangle. Entry (i, jj) receives its updates for k = 0, 1, ... in order, so each
-  // entry sees the same operations in the same order as the left-looking form
-  // s = a_ij - sum_{k<j} l_ik l_jk of linsolve.rs:80-99. The scaled column is also kept in
-  // colj, so the update reads it without strided LDS accesses.
-  for (uint32_t j = 0; j < m; ++j) {
-    if (t == 0) {
-      const double s = L[tri(j) + j];
-      if (s <= 1e-14) failed = 1;
-      else L[tri(j) + j] = sqrt(s);
+  // registers <- tiles; indices m .. 16 nt - 1 padded as identity
+  const uint32_t nt = (m + 15) / 16;
+  double4_t acc[SW_K];
+  uint32_t TI[SW_K], TJ[SW_K];
+  bool act[SW_K];
+#pragma unroll
+  for (uint32_t k = 0; k < SW_K; ++k) {
+    const uint32_t tile = wave + 4 * k;
+    lower_tile_ij(tile < SCH_NT ? tile : 0, TI[k], TJ[k]);
+    act[k] = tile < SCH_NT && TI[k] < nt;
+    acc[k] = double4_t{0.0, 0.0, 0.0, 0.0};
+    if (!act[k]) continue;
+#pragma unroll
+    for (uint32_t v = 0; v < 4; ++v) {
+      const uint32_t i = 16 * TI[k] + 4 * v + q4, j = 16 * TJ[k] + r16;
+      acc[k][v] = (i < m && j < m) ? (i >= j ? L[tri(i) + j] : L[tri(j) + i]) : (i == j ? 1.0 : 0.0);
     }
-    __syncthreads();
-    if (failed) break;
-    const double ljj = L[tri(j) + j];
-    for (uint32_t i = j + 1 + t; i < m; i += BLOCK) {
-      const double v = L[tri(i) + j] / ljj;
-      L[tri(i) + j] = v;
-      colj[i] = v;
+  }
+  __syncthreads();
+  // L's storage becomes the two pivot-column buffers: row i of buffer b at cbuf[b][4 i .. 4 i + 3]
+  double *cbuf = L;
+  auto publish = [&](uint32_t p0, double *cb) {
+    const uint32_t Pt = p0 / 16, c0 = p0 % 16;
+#pragma unroll
+    for (uint32_t k = 0; k < SW_K; ++k) {
+      if (!act[k]) continue;
+      if (TJ[k] == Pt) {  // tiles (I, Pt): column p0 + (r16 - c0) of rows 16 I + 4 v + q4
+        if (r16 >= c0 && r16 < c0 + 4)
+#pragma unroll
+          for (uint32_t v = 0; v < 4; ++v) cb[4 * (16 * TI[k] + 4 * v + q4) + (r16 - c0)] = acc[k][v];
+      } else if (TI[k] == Pt) {  // tiles (Pt, J), J < Pt: row p0 + q4 is column p0 + q4 of row 16 J + r16
+        cb[4 * (16 * TJ[k] + r16) + q4] = pick4(acc[k], c0 / 4);
+      }
     }
-    __syncthreads();
-    for (uint32_t i = j + 1 + ti; i < m; i += 16)
-      for (uint32_t jj = j + 1 + tj; jj <= i; jj += 16) L[tri(i) + jj] -= colj[i] * colj[jj];
+  };
+  publish(0, cbuf);
+  __syncthreads();
+  bool bad = false;
+  uint32_t buf = 0;
+  for (uint32_t p0 = 0; p0 < m; p0 += 4) {
+    const double *cb = cbuf + buf * (4 * MD);
+    const uint32_t Pt = p0 / 16, c0 = p0 % 16;
+    // S = A_PP (row p0 + a at cb[4 (p0 + a) + b]); Cholesky with the pivot test, W = L^-1, D = W^T W.
+    // 1 / l_jj = rsqrt(pivot): the hardware estimate and two Newton steps (full FP64 precision).
+    const double *S = cb + 4 * p0;
+    auto rsqrt64 = [](double x) {
+      double y = __builtin_amdgcn_rsq(x);
+      y = y * fma(-0.5 * x, y * y, 1.5);
+      return y * fma(-0.5 * x, y * y, 1.5);
+    };
+    const double d0 = S[0];
+    if (d0 <= 1e-14) { bad = true; break; }
+    const double w00 = rsqrt64(d0), l10 = S[4] * w00, l20 = S[8] * w00, l30 = S[12] * w00;
+    const double d1 = fma(-l10, l10, S[5]);
+    if (d1 <= 1e-14) { bad = true; break; }
+    const double w11 = rsqrt64(d1), l21 = fma(-l20, l10, S[9]) * w11, l31 = fma(-l30, l10, S[13]) * w11;
+    const double d2 = fma(-l21, l21, fma(-l20, l20, S[10]));
+    if (d2 <= 1e-14) { bad = true; break; }
+    const double w22 = rsqrt64(d2), l32 = fma(-l31, l21, fma(-l30, l20, S[14])) * w22;
+    const double d3 = fma(-l32, l32, fma(-l31, l31, fma(-l30, l30, S[15])));
+    if (d3 <= 1e-14) { bad = true; break; }
+    const double w33 = rsqrt64(d3);
+    const double w10 = -w11 * (l10 * w00);
+    const double w21 = -w22 * (l21 * w11), w20 = -w22 * fma(l21, w10, l20 * w00);
+    const double w32 = -w33 * (l32 * w22), w31 = -w33 * fma(l32, w21, l31 * w11), w30 = -w33 * fma(l32, w20, fma(l31, w10, l30 * w00));
+    // D = W^T W (symmetric)
+    const double D00 = fma(w30, w30, fma(w20, w20, fma(w10, w10, w00 * w00)));
+    const double D11 = fma(w31, w31, fma(w21, w21, w11 * w11));
+    const double D22 = fma(w32, w32, w22 * w22);
+    const double D33 = w33 * w33;
+    const double D01 = fma(w31, w30, fma(w21, w20, w11 * w10));
+    const double D02 = fma(w32, w30, w22 * w20);
+    const double D03 = w33 * w30;
+    const double D12 = fma(w32, w31, w22 * w21);
+    const double D13 = w33 * w31;
+    const double D23 = w33 * w32;
+    // column q4 of D for the A operand and the pivot rows, column r16 - c0 for the pivot columns
+    const uint32_t qc = (r16 - c0) & 3u;
+    const double dk[4] = {sel4(q4, D00, D01, D02, D03), sel4(q4, D01, D11, D12, D13), sel4(q4, D02, D12, D22, D23), sel4(q4, D03, D13, D23, D33)};
+    const double dc[4] = {sel4(qc, D00, D01, D02, D03), sel4(qc, D01, D11, D12, D13), sel4(qc, D02, D12, D22, D23), sel4(qc, D03, D13, D23, D33)};
+    auto row_times = [&](uint32_t row, const double *dd) {
+      const double *rr = cb + 4 * row;
+      return fma(rr[3], dd[3], fma(rr[2], dd[2], fma(rr[1], dd[1], rr[0] * dd[0])));
+    };
+    // operands of every tile first (the LDS reads of all tiles in flight together), then the
+    // rank-4 products back to back, then the pivot rows and columns
+    double av[SW_K], bv[SW_K];
+#pragma unroll
+    for (uint32_t k = 0; k < SW_K; ++k) {
+      if (!act[k]) continue;
+      // A_IJ -= (A_IP D) (A_JP)^T, pivot rows and columns left out
+      const uint32_t i = 16 * TI[k] + r16, j = 16 * TJ[k] + r16;
+      const double a = -row_times(i, dk), b = cb[4 * j + q4];
+      av[k] = i - p0 >= 4u ? a : 0.0;
+      bv[k] = j - p0 >= 4u ? b : 0.0;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < SW_K; ++k)
+      if (act[k]) acc[k] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[k], bv[k], acc[k], 0, 0, 0);
+#pragma unroll
+    for (uint32_t k = 0; k < SW_K; ++k) {
+      if (!act[k]) continue;
+      const uint32_t I = TI[k], J = TJ[k];
+      if (J == Pt) {
+        // pivot columns p0 + qc (lanes r16 = c0 + qc): A_iP D for rows outside P, -D on the pivot block
+        double x[4];
+#pragma unroll
+        for (uint32_t v = 0; v < 4; ++v) {
+          const uint32_t row = 16 * I + 4 * v + q4, pr = (row - p0) & 3u;
+          const double piv = -sel4(pr, dc[0], dc[1], dc[2], dc[3]);
+          x[v] = row - p0 < 4u ? piv : row_times(row, dc);
+        }
+        if (r16 - c0 < 4u) acc[k] = double4_t{x[0], x[1], x[2], x[3]};
+      }
+      if (I == Pt) {
+        // pivot rows p0 + q4 (register c0 / 4) of tiles (Pt, J), columns outside P
+        const uint32_t j = 16 * J + r16;
+        const double y = row_times(j, dk);
+        if (j - p0 >= 4u) put4(acc[k], c0 / 4, y);
+      }
+    }
+    buf ^= 1u;
+    if (p0 + 4 < m) publish(p0 + 4, cbuf + buf * (4 * MD));
     __syncthreads();
   }
   const uint32_t m4 = (m + 3) & ~3u;
-  if (failed) {
+  if (bad) {
     // diagonal fallback: factor sqrt(a_pp) (or 1) -> inverse 1 / a_pp
     if (t == 0) atomicAdd(fails, 1u);
     if (TILED) {
@@ -322,64 +463,21 @@ __device__ void factor_invert(double *L, double *diag0, double *colj, double &fl
     }
     return;
   }
-  // W = L^-1 in place of L's lower triangle, a column at a time from the last (the unblocked
-  // lower-triangular inverse): w_jj = 1 / l_jj and, below the diagonal, w_ij = -w_jj
-  // sum_{k = j+1 .. i} w_ik l_kj with the trailing block already inverted and the original
-  // column j copied to colj. A row's sum is split over the 16 lanes of a DPP row (lane q takes
-  // k = j + 1 + q, q + 16, ...) and combined by the row exchanges, a fixed order. The packed
-  // lower triangle then holds W: one triangle in LDS instead of two m x m arrays lets four
-  // workgroups share a compute unit. (Running this and the Cholesky in one wavefront, with
-  // wavefront fences instead of workgroup barriers, took twice as long.)
-  for (int jj = (int)m - 1; jj >= 0; --jj) {
-    const uint32_t j = (uint32_t)jj;
-    for (uint32_t i = j + 1 + t; i < m; i += BLOCK) colj[i] = L[tri(i) + j];
-    __syncthreads();
-    const double wjj = 1.0 / L[tri(j) + j];
-    for (uint32_t i0 = j + 1; i0 < m; i0 += 16) {
-      const uint32_t i = i0 + ti;
-      double acc = 0.0;
-      if (i < m)
-        for (uint32_t k = j + 1 + tj; k <= i; k += 16) acc += L[tri(i) + k] * colj[k];
-      acc += dpp_d<DPP_XOR1>(acc);
-      acc += dpp_d<DPP_XOR2>(acc);
-      acc += dpp_d<DPP_HALF_MIRROR>(acc);
-      acc += dpp_d<DPP_MIRROR>(acc);
-      if (i < m && tj == 0) L[tri(i) + j] = -wjj * acc;
-    }
-    __syncthreads();
-    if (t == 0) L[tri(j) + j] = wjj;
-  }
-  __syncthreads();
-  // A^-1 = W^T W on the FP64 matrix cores: 16 x 16 output tiles, k in steps of 4
-  // (v_mfma_f64_16x16x4f64; A operand lane a = (row a % 16, k a / 16), B operand lane
-  // b = (k b / 16, column b % 16), result lane l register v = (row 4 v + l / 16, column
-  // l % 16)). Only tiles with P <= Q are formed (the product is symmetric) and W being lower
-  // triangular, k starts at 16 max(P, Q). Entries outside m x m read as zero, so they come
-  // out zero: in the row-major form columns m..m4-1 of rows < m (the coarse apply reads no
-  // further), in the tiled form every entry of the 21 tiles (tiles wholly outside run no
-  // k step and store their zero accumulator).
-  const uint32_t wave = t / WAVE, lane = t % WAVE;
-  auto wget = [&](uint32_t k, uint32_t c) -> double { return (k < m && c <= k) ? L[tri(k) + c] : 0.0; };
-  for (uint32_t tile = wave; tile < SCH_NT; tile += BLOCK / WAVE) {
-    uint32_t P, Q;
-    sym_tile_pq(tile, P, Q);
-    if (!TILED && (16 * P >= m4 || 16 * Q >= m4)) continue;
-    double4_t acc = {0.0, 0.0, 0.0, 0.0};
-    for (uint32_t kb = 4 * Q; kb < MD / 4 && 4 * kb < m; ++kb) {
-      const uint32_t k = 4 * kb + lane / 16;
-      const double a = wget(k, 16 * P + lane % 16);
-      const double b = wget(k, 16 * Q + lane % 16);
-      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
-    }
+  // the registers hold -A^-1: lower tile (I, J) is the transpose of upper tile (J, I)
+#pragma unroll
+  for (uint32_t k = 0; k < SW_K; ++k) {
+    const uint32_t tile = wave + 4 * k;
+    if (tile >= SCH_NT) continue;
+    const uint32_t I = TI[k], J = TJ[k];
+#pragma unroll
     for (uint32_t v = 0; v < 4; ++v) {
-      const uint32_t i = 4 * v + lane / 16, j = lane % 16;
-      const uint32_t r = 16 * P + i, c = 16 * Q + j;
-      const TOut val = (TOut)acc[v];
+      const uint32_t i = 16 * I + 4 * v + q4, j = 16 * J + r16;
+      const TOut val = (act[k] && i < m && j < m) ? (TOut)(-acc[k][v]) : TOut(0);
       if (TILED) {
-        out[SCH_TILE * tile + 16 * i + j] = val;
-      } else {
-        if (r < m && c < m4) out[r * MD + c] = val;
-        if (P != Q && c < m && r < m4) out[c * MD + r] = val;
+        out[SCH_TILE * sym_tile(J, I) + 16 * r16 + 4 * v + q4] = val;
+      } else if (act[k]) {
+        if (i < m && j < m4) out[i * MD + j] = val;
+        if (I != J && j < m && i < m4) out[j * MD + i] = val;
       }
     }
   }
@@ -389,19 +487,16 @@ __device__ void factor_invert(double *L, double *diag0, double *colj, double &fl
 // diagonal fallback, explicit inverse A^-1 = L^-T L^-1 written as FP32 (the FP32 PCG) or FP64
 // (the FP64 PCG).
 template <class TOut>
-__global__ __launch_bounds__(BLOCK) void k_schwarz_build(SellView A, ContactView C, const uint32_t *__restrict__ dom_ptr,
+__global__ __launch_bounds__(BLOCK, 3) void k_schwarz_build(SellView A, ContactView C, const uint32_t *__restrict__ dom_ptr,
                                                         const uint32_t *__restrict__ dom_row, const uint32_t *__restrict__ row_dom,
                                                         const uint32_t *__restrict__ row_pos, TOut *__restrict__ ainv, uint32_t *inc_rng,
                                                         uint32_t *fails) {
   __shared__ double L[MD * (MD + 1) / 2];
   __shared__ double diag0[MD];
-  __shared__ double colj[MD];
   __shared__ double floor_v;
-  __shared__ int failed;
   const uint32_t d = blockIdx.x, t = threadIdx.x;
   const uint32_t base = dom_ptr[d], nv = dom_ptr[d + 1] - base, m = 3 * nv;
   for (uint32_t i = t; i < tri(m); i += BLOCK) L[i] = 0.0;
-  if (t == 0) failed = 0;
   __syncthreads();
   // elastic blocks of the domain rows: 8 threads per row, one writer per block
   for (uint32_t p = t / 8; p < nv; p += BLOCK / 8) {
@@ -448,7 +543,7 @@ __global__ __launch_bounds__(BLOCK) void k_schwarz_build(SellView A, ContactView
     }
     __syncthreads();
   }
-  factor_invert<true, TOut>(L, diag0, colj, floor_v, failed, m, ainv + (size_t)d * SCHWARZ_INV_FLOATS, fails);
+  factor_invert<true, TOut>(L, diag0, floor_v, m, ainv + (size_t)d * SCHWARZ_INV_FLOATS, fails);
 }
 
 // Coarse block of coarse domain D (linsolve.rs:613-625 coarse_matrix, 230-258 factorize):
@@ -456,20 +551,17 @@ __global__ __launch_bounds__(BLOCK) void k_schwarz_build(SellView A, ContactView
 // domain G_b, elastic and contact, G_a and G_b members a, b of D. Thread 3a + r owns coarse
 // row 3a + r and walks the rows of G_a in domain order (no write conflicts, fixed order);
 // then the same floor, FP64 factorisation and FP32 inverse as level 0.
-__global__ __launch_bounds__(BLOCK) void k_schwarz_build_coarse(SellView A, ContactView C, const uint32_t *__restrict__ dom_ptr,
+__global__ __launch_bounds__(BLOCK, 3) void k_schwarz_build_coarse(SellView A, ContactView C, const uint32_t *__restrict__ dom_ptr,
                                                                const uint32_t *__restrict__ dom_row, const uint32_t *__restrict__ row_dom,
                                                                const uint32_t *__restrict__ cdom_ptr, const uint32_t *__restrict__ cdom_node,
                                                                const uint32_t *__restrict__ node_cdom, const uint32_t *__restrict__ node_cpos,
                                                                float *__restrict__ ainv1, uint32_t *fails) {
   __shared__ double L[MD * (MD + 1) / 2];
   __shared__ double diag0[MD];
-  __shared__ double colj[MD];
   __shared__ double floor_v;
-  __shared__ int failed;
   const uint32_t D = blockIdx.x, t = threadIdx.x;
   const uint32_t cb = cdom_ptr[D], n1 = cdom_ptr[D + 1] - cb, m = 3 * n1;
   for (uint32_t i = t; i < tri(m); i += BLOCK) L[i] = 0.0;
-  if (t == 0) failed = 0;
   __syncthreads();
   if (t < m) {
     const uint32_t a = t / 3, r = t % 3, G = cdom_node[cb + a];
@@ -508,7 +600,7 @@ __global__ __launch_bounds__(BLOCK) void k_schwarz_build_coarse(SellView A, Cont
     }
   }
   __syncthreads();
-  factor_invert<false>(L, diag0, colj, floor_v, failed, m, ainv1 + (size_t)D * MD * MD, fails);
+  factor_invert<false>(L, diag0, floor_v, m, ainv1 + (size_t)D * MD * MD, fails);
 }
 
 // Coarse correction, one workgroup per domain d (member a of coarse domain D): e =

@@ -183,6 +183,43 @@ def test_schwarz_apply_matches_oracle(ctx, cfg, steps, domain):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("domain", [32, 5])
+def test_schwarz_apply_fallback_on_indefinite_domain(ctx, domain):
+    # a domain whose block is not positive definite fails the Cholesky pivot test
+    # (pivot <= 1e-14, linsolve.rs:80-99) and falls back to the diagonal with the floor
+    # 1e-6 tr/m + 1e-8 added: 1 / a_pp, or 1 where a_pp <= 1e-14 (linsolve.rs:232-250). Two
+    # vertices are decoupled from their neighbours and given a negative diagonal, so their
+    # pivots fail whatever the elimination order; the other domains are unaffected.
+    rp, ci, va, rhs = newton_system(0, 14)
+    va = va.copy()
+    bad = {100: -1.0e6, 700: -5.0}
+    for row in range(len(rhs)):
+        for k in range(rp[row], rp[row + 1]):
+            vr, vc = row // 3, ci[k] // 3
+            if vr in bad or vc in bad:
+                va[k] = bad[vr] if (row == ci[k]) else 0.0
+    r = np.random.default_rng(23).standard_normal(len(rhs)).astype(np.float32)
+    doms, z_ref = O.schwarz(rp, ci, va, domain, r)
+    for v0 in bad:
+        dom = next(d for d in doms if v0 in d)
+        rows = np.concatenate([np.arange(3 * u, 3 * u + 3) for u in dom])
+        diag = np.array([float(va[k]) for row in rows for k in range(rp[row], rp[row + 1]) if ci[k] == row])
+        floor = 1e-6 * (diag.sum() / len(rows)) + 1e-8
+        dinv = np.where(diag + floor > 1e-14, 1.0 / (diag + floor), 1.0)
+        # the oracle took the fallback on that domain
+        assert np.abs(z_ref[rows] - r[rows] * dinv).max() <= 1e-9 * np.abs(z_ref[rows]).max()
+    ls = nguo_hip.LinearSolver(ctx, precond=1, domain=domain)
+    z = ls.precond_apply(rp, ci, va, r)
+    err = np.linalg.norm(z - z_ref) / np.linalg.norm(z_ref)
+    assert err < 2e-4, err
+    for v0 in bad:
+        dom = next(d for d in doms if v0 in d)
+        rows = np.concatenate([np.arange(3 * u, 3 * u + 3) for u in dom])
+        assert np.abs(z[rows] - z_ref[rows]).max() <= 1e-6 * np.abs(z_ref[rows]).max()
+    ls.close()
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("cfg,steps", [(0, 0), (0, 14), (101, 0)])
 def test_schwarz_pcg_matches_oracle(ctx, cfg, steps):
     rp, ci, va, rhs = newton_system(cfg, steps)
