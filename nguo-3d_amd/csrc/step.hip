@@ -14,7 +14,10 @@
 //   6. the ACCD line search (FP64 time of impact) over LBVH candidates with the reference's
 //      widening search radius (implicit.rs:1834-1943).
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <condition_variable>
 #include <cstring>
 #include <exception>
@@ -1353,6 +1356,42 @@ struct ElemSet {
 
 struct SolveGate;
 
+// Wall time of the step's phases, for diagnostics: with NGUO_PHASE_TIMES=1 in the environment
+// at step creation, every phase boundary synchronises the stream and each step prints one line
+// of milliseconds per phase to stderr. Off (no synchronisation, nothing printed) otherwise.
+enum Phase { PH_ASSEMBLE, PH_CONTACTS, PH_DOMAINS, PH_SOLVE, PH_CCD, PH_UPDATE, PH_POST, PH_N };
+struct PhaseClock {
+  bool on = false;
+  double ms[PH_N] = {};
+  std::chrono::steady_clock::time_point t;
+  void start(hipStream_t s) {
+    if (!on) return;
+    (void)hipStreamSynchronize(s);
+    for (double &m : ms) m = 0.0;
+    t = std::chrono::steady_clock::now();
+  }
+  void mark(int ph, hipStream_t s) {
+    if (!on) return;
+    (void)hipStreamSynchronize(s);
+    const auto now = std::chrono::steady_clock::now();
+    ms[ph] += std::chrono::duration<double, std::milli>(now - t).count();
+    t = now;
+  }
+  void report(double time) const {
+    if (!on) return;
+    static const char *names[PH_N] = {"assemble", "contacts", "domains", "solve", "ccd", "update", "post"};
+    double tot = 0.0;
+    std::string line;
+    char buf[64];
+    for (int i = 0; i < PH_N; ++i) {
+      tot += ms[i];
+      std::snprintf(buf, sizeof buf, " %s %.3f", names[i], ms[i]);
+      line += buf;
+    }
+    std::fprintf(stderr, "nguo phase ms t=%.4f:%s total %.3f\n", time, line.c_str(), tot);
+  }
+};
+
 struct nguo_step {
   nguo_ctx *ctx = nullptr;
   hipStream_t s = nullptr;
@@ -1422,6 +1461,7 @@ struct nguo_step {
   bool adj_valid = false;  // the last step can be differentiated (cleared by set_state / set_rest_scale)
   std::vector<float> trace_toi;
   bool own_stream = false;
+  PhaseClock clock;
   // batch membership: Newton solves go through the batch's gate (null = solved alone)
   SolveGate *gate = nullptr;
   hipEvent_t gate_ev = nullptr;  // this step's assembly, recorded before it joins a batched solve
@@ -1987,6 +2027,7 @@ void prepare_schwarz(nguo_step *st) {
 
 PcgStats solve_system(nguo_step *st, const float *b, float tol, float *x) {
   prepare_schwarz(st);
+  st->clock.mark(PH_DOMAINS, st->s);
   if (uses_fp64(st)) {
     if (!st->pcg64) st->pcg64.reset(new Pcg64(st->s));
     return st->pcg64->solve(st->act.M, contact_view(st), *st->sz, b, tol, st->prm.cg_max_iter, x);
@@ -2002,13 +2043,16 @@ void assemble_and_solve(nguo_step *st, nguo_stats &stats) {
     hipLaunchKernelGGL(k_gather_grad, dim3(div_up(st->NF, BLOCK)), dim3(BLOCK), 0, s, st->NF, (const uint32_t *)st->free_list.p,
                        (const uint32_t *)st->act.gc_ptr.p, (const uint32_t *)st->act.gc.p, st->act.dev(), (const float *)st->im.p, st->P,
                        (const float *)st->x.p, (const float *)st->xt.p, (const float *)st->x0.p, st->egrad.p);
+  st->clock.mark(PH_ASSEMBLE, s);
   contacts(st);
   if (st->NF)
     hipLaunchKernelGGL(k_rhs, dim3(div_up(st->NF, BLOCK)), dim3(BLOCK), 0, s, st->NF, (const float *)st->egrad.p, (const uint32_t *)st->inc_ptr.p,
                        (const uint32_t *)st->inc.p, (const float *)st->cgrad.p, st->ns > 0 ? 1 : 0, st->rhs.p);
   NGUO_HIP(hipGetLastError());
+  st->clock.mark(PH_CONTACTS, s);
   if (st->NF == 0) { stats = nguo_stats{}; return; }
   PcgStats ps = solve_system(st, st->rhs.p, st->prm.cg_tol, st->dxf.p);
+  st->clock.mark(PH_SOLVE, s);
   stats.iters = ps.iters;
   stats.sweeps = ps.sweeps;
   stats.residual = ps.residual;
@@ -2031,6 +2075,7 @@ void advance_one(nguo_step *st) {
   float h = st->prm.dt;
   st->P.inv_h2 = 1.0f / (h * h);
   st->adj_valid = false;
+  st->clock.start(s);
   NGUO_HIP(hipMemcpyAsync(st->xprev.p, st->x0.p, 3 * (size_t)V * sizeof(float), hipMemcpyDeviceToDevice, s));
   hipLaunchKernelGGL(k_predict, dim3(GV), dim3(BLOCK), 0, s, V, (const float *)st->im.p, (const float *)st->x0.p, (const float *)st->v.p, h,
                      f3{st->prm.gravity[0], st->prm.gravity[1], st->prm.gravity[2]}, st->xt.p, st->x.p);
@@ -2057,7 +2102,9 @@ void advance_one(nguo_step *st) {
         max_norm = max_norm * rescale;
       }
     }
+    st->clock.mark(PH_UPDATE, s);
     float t = ccd_max_step(st, max_norm);
+    st->clock.mark(PH_CCD, s);
     last_t = t;
     hipLaunchKernelGGL(k_apply, dim3(GV), dim3(BLOCK), 0, s, V, st->x.p, (const float *)st->dx.p, t, st->part.p);
     if (st->trace_on) {  // per-iterate positions for parity checks (off in the production step)
@@ -2068,6 +2115,7 @@ void advance_one(nguo_step *st) {
     st->trace_toi[it] = t;
     if (st->trace_on) st->trace_count = it + 1;
     float max_step = reduce_max(st, GV);
+    st->clock.mark(PH_UPDATE, s);
     if (max_step < 1.0e-7f) break;
     if (final_step) break;
     if (st->prm.target_toi > 0.0f && t >= st->prm.target_toi && taken > st->prm.min_newton_steps) final_step = true;
@@ -2126,6 +2174,8 @@ void advance_one(nguo_step *st) {
   m.device_alloc = (uint32_t)(g_dev_allocs - allocs0);
   m.device_free = (uint32_t)(g_dev_frees - frees0);
   st->adj_valid = !bad;
+  st->clock.mark(PH_POST, s);
+  st->clock.report(st->time);
 }
 
 }  // namespace
@@ -2138,6 +2188,7 @@ int nguo_step_create(nguo_ctx *ctx, const nguo_scene_desc *sc, nguo_step **out) 
     NGUO_REQUIRE(sc->n_verts > 0 && sc->positions && sc->inv_mass, NGUO_E_INVALID, "nguo_step_create: empty scene");
     NGUO_HIP(hipSetDevice(ctx->device));
     std::unique_ptr<nguo_step> st(new nguo_step());
+    if (const char *e = std::getenv("NGUO_PHASE_TIMES")) st->clock.on = e[0] == '1';
     st->ctx = ctx;
     // each step object owns a stream (default priority), so several can advance concurrently
     NGUO_HIP(hipStreamCreateWithFlags(&st->s, hipStreamNonBlocking));
