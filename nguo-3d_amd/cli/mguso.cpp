@@ -8,10 +8,12 @@
 //   mguso run <scene.json> [--frames N] [--out DIR] [--solver xpbd|implicit] [--cpu] [--precond block|schwarz-ml]
 //   mguso bench <scene.json> [--steps N] [--solver implicit] [--cpu] [--precond block|schwarz-ml]
 //   mguso verify <scene.json> [--steps N]
+//   mguso gauge-scene <scene-dir> [--steps N] [--compare-frame N]
 //
 // The implicit solver runs on the GPU (its linear solve included). The XPBD preview tier, the
-// CPU linear solvers (--cpu) and the reference-capture commands (golden, gauge-scene) are not
-// part of this build and exit 1 saying so.
+// CPU linear solvers (--cpu) and the capture-fitting command (golden) are not part of this
+// build and exit 1 saying so. gauge-scene prints its report as text lines, as the reference's
+// does (main.rs:264-450); loading and reporting need no device.
 #include <sys/stat.h>
 
 #include <cctype>
@@ -28,6 +30,7 @@
 #include <vector>
 
 #include "../../include/nguo_hip.h"
+#include "capture.h"
 #include "json.h"
 #include "scene.h"
 
@@ -85,8 +88,8 @@ void check(int rc, const std::string &what) {
 // ---------------------------------------------------------------- arguments
 struct Args {
   std::string command, scene, out;
-  bool has_steps = false, has_frames = false, has_precond = false, cpu = false;
-  uint32_t steps = 200, frames = 0;
+  bool has_steps = false, has_frames = false, has_precond = false, cpu = false, has_compare = false;
+  uint32_t steps = 200, frames = 0, compare_frame = 0;
   std::string solver, precond = "block";
 };
 
@@ -96,7 +99,8 @@ const char *USAGE =
     "  gpu-info  Print the device and run a no-op kernel probe\n"
     "  run       Simulate a scene (JSON scene file)\n"
     "  bench     Benchmark solver throughput (steps/sec) on a scene\n"
-    "  verify    Run a scene with the implicit solver and check the solver invariants\n\n"
+    "  verify    Run a scene with the implicit solver and check the solver invariants\n"
+    "  gauge-scene  Inspect a scene from the reference capture; optionally step it or compare a frame\n\n"
     "Options:\n"
     "  --log-format <text|json>  [default: text]\n"
     "  -v, --verbose             more log output\n"
@@ -142,6 +146,9 @@ Args parse_args(int argc, char **argv, bool &done) {
       uint32_t n = parse_u32(arg, value(i, arg, inline_v, has_inline));
       if (arg == "--steps") { a.steps = n; a.has_steps = true; }
       else { a.frames = n; a.has_frames = true; }
+    } else if (arg == "--compare-frame") {
+      a.compare_frame = parse_u32(arg, value(i, arg, inline_v, has_inline));
+      a.has_compare = true;
     } else if (arg == "--out") {
       a.out = value(i, arg, inline_v, has_inline);
     } else if (arg == "--solver") {
@@ -172,7 +179,10 @@ Args parse_args(int argc, char **argv, bool &done) {
   if (a.command == "gpu-info" && !a.scene.empty()) throw Usage("unexpected argument '" + a.scene + "' found");
   // per-command flags (main.rs:92-187)
   const bool is_run = a.command == "run", is_bench = a.command == "bench", is_verify = a.command == "verify";
-  if (a.has_steps && !(is_bench || is_verify)) throw Usage("unexpected argument '--steps' found");
+  const bool is_gauge = a.command == "gauge-scene";
+  if (a.has_steps && !(is_bench || is_verify || is_gauge)) throw Usage("unexpected argument '--steps' found");
+  if (a.has_compare && !is_gauge) throw Usage("unexpected argument '--compare-frame' found");
+  if (is_gauge && !a.has_steps) a.steps = 0;  // main.rs:103-105: 0 = load and report only
   if (a.has_frames && !is_run) throw Usage("unexpected argument '--frames' found");
   if (!a.out.empty() && !is_run) throw Usage("unexpected argument '--out' found");
   if ((!a.solver.empty() || a.cpu || a.has_precond) && !(is_run || is_bench))
@@ -478,6 +488,123 @@ int cmd_verify(const Args &a) {
   return all_ok ? 0 : 2;
 }
 
+// main.rs:264-450 GaugeScene: the driver report, then either --compare-frame N (step until the
+// clock reaches frame N / fps and compare with checkpoints/vert_N.bin, against the null model of
+// frame 0 held still) or --steps N (step N times and summarise).
+int cmd_gauge_scene(const Args &a) {
+  DriverReport rep;
+  CaptureScene S;
+  try {
+    S = lower_capture(a.scene, rep);
+  } catch (const CaptureError &e) {
+    throw Failure("loading gauge scene " + a.scene + ": " + e.what());
+  }
+  S.bind();
+  std::printf("scene: %s\n", a.scene.c_str());
+  std::printf("  verts %zu  tris %zu  tets %zu  rods %zu (bend hinges %zu)\n", rep.n_verts, rep.n_tris, rep.n_tets, rep.n_rods, rep.n_rod_hinges);
+  std::printf("  model split: baraff-witkin %zu  isotropic-shell %zu\n", rep.n_baraff_witkin, rep.n_shell);
+  std::printf("  solid-surface faces %zu  strain-limited faces %zu  pins %zu (%zu kinematic)  pulls %zu  colliders %zu (%zu animated)\n",
+              rep.n_solid_surface, rep.n_strain_limited, rep.n_pins_applied, rep.n_pins_kinematic, rep.n_pulls_applied, rep.n_colliders,
+              rep.n_colliders_animated);
+  const nguo_params &p = S.params;
+  std::printf("  solver: dt %s  gravity [%s, %s, %s]  shell mu %s lambda %s  bend %s  dhat %s  friction %s  barrier %s  cg_tol %s  newton cap %u\n",
+              num_f32(p.dt).c_str(), num_f32(p.gravity[0]).c_str(), num_f32(p.gravity[1]).c_str(), num_f32(p.gravity[2]).c_str(),
+              num_f32(p.shell_mu).c_str(), num_f32(p.shell_lambda).c_str(), num_f32(p.bend_stiffness).c_str(), num_f32(p.contact_dhat).c_str(),
+              num_f32(p.friction).c_str(), p.barrier_kind == NGUO_BARRIER_CUBIC ? "cubic" : "log", num_f32(p.cg_tol).c_str(), p.newton_iters);
+  if (rep.unsupported.empty()) {
+    std::printf("  every declared feature of this scene is modeled\n");
+  } else {
+    std::printf("  NOT reproduced by this build:\n");
+    for (auto &u : rep.unsupported) std::printf("    - %s\n", u.c_str());
+  }
+  std::fflush(stdout);
+  if (!a.has_compare && a.steps == 0) return 0;
+  if (S.has_tets) throw Failure("stepping a scene with tets: volumetric solids are not on the device step");
+  const uint32_t V = S.n_verts();
+  if (a.has_compare) {
+    if (!S.has_toml) throw Failure("param.toml is required to map a frame to a time");
+    const double target_t = (double)a.compare_frame / (double)S.toml.fps;
+    std::vector<float> reference;
+    std::vector<float> frame0;
+    try {
+      reference = load_checkpoint(a.scene, a.compare_frame);
+      frame0 = load_checkpoint(a.scene, 0);
+    } catch (const CaptureError &e) {
+      throw Failure(std::string("checkpoints/vert_") + std::to_string(a.compare_frame) + ".bin: " + e.what());
+    }
+    // step until the clock reaches the frame's time (dt need not divide the frame interval)
+    std::vector<float> ours = S.positions;
+    double time = 0.0;
+    uint64_t n_steps = 0;
+    if (time < target_t) {
+      Session s;
+      s.open_ctx();
+      check(nguo_step_create(s.ctx, &S.desc, &s.st), "building the implicit solver");
+      while (time < target_t) {
+        s.advance(1);
+        ++n_steps;
+        time = s.metrics().time;
+        if (!s.metrics().all_finite) throw Failure("non-finite state at step " + std::to_string(n_steps));
+      }
+      ours = s.positions(V);
+    }
+    GateReport gate, null;
+    try {
+      gate = compare_frames(ours, reference);
+      null = compare_frames(frame0, reference);
+    } catch (const CaptureError &e) {
+      throw Failure(e.what());
+    }
+    auto e4 = [](double v) { char b[32]; std::snprintf(b, sizeof b, "%.4e", v); return std::string(b); };
+    std::printf("  frame %u: target t %.6f, reached %.6f in %llu steps\n", a.compare_frame, target_t, time, (unsigned long long)n_steps);
+    std::printf("  divergence vs reference: max_rel %s  rms %s  centroid_rel %s  (diagonal %.4f)\n", e4(gate.max_rel).c_str(), e4(gate.rms).c_str(),
+                e4(gate.centroid_rel).c_str(), gate.diagonal);
+    std::printf("  gates (max_rel < 8e-2, rms < 2e-2, centroid_rel < 6e-3): %s\n", gate.passes() ? "PASS" : "FAIL");
+    std::printf("  null model (predict frame 0, no simulation): max_rel %s  rms %s  gates %s\n", e4(null.max_rel).c_str(), e4(null.rms).c_str(),
+                null.passes() ? "PASS  <-- the gates cannot distinguish a solver from no solver here" : "FAIL");
+    auto skill = [](float o, float n) { return n > 0.0f ? 1.0 - (double)o / (double)n : NAN; };
+    std::printf("  SKILL vs null (1 = exact, 0 = no better than not simulating): max %.3f  rms %.3f\n", skill(gate.max_rel, null.max_rel),
+                skill(gate.rms, null.rms));
+    // displacement fields from frame 0: magnitude ratio and direction cosine
+    double mo = 0, mr = 0, dot = 0, la = 0, lb = 0;
+    for (uint32_t i = 0; i < V; ++i) {
+      float od[3], rd[3];
+      for (int k = 0; k < 3; ++k) { od[k] = ours[3 * i + k] - frame0[3 * i + k]; rd[k] = reference[3 * i + k] - frame0[3 * i + k]; }
+      const float oo = od[0] * od[0] + od[1] * od[1] + od[2] * od[2], rr = rd[0] * rd[0] + rd[1] * rd[1] + rd[2] * rd[2];
+      mo += std::sqrt((double)oo);
+      mr += std::sqrt((double)rr);
+      dot += (double)(od[0] * rd[0] + od[1] * rd[1] + od[2] * rd[2]);
+      la += (double)oo;
+      lb += (double)rr;
+    }
+    mo /= std::max<uint32_t>(V, 1);
+    mr /= std::max<uint32_t>(V, 1);
+    la = std::sqrt(la);
+    lb = std::sqrt(lb);
+    const double cosv = (la > 0 && lb > 0) ? dot / (la * lb) : NAN;
+    std::printf("  displacement field: ours %.6f  reference %.6f  ratio %.3f  cosine %.3f\n", mo, mr, mr > 0 ? mo / mr : NAN, cosv);
+    return 0;
+  }
+  Session s;
+  s.open_ctx();
+  check(nguo_step_create(s.ctx, &S.desc, &s.st), "building the implicit solver");
+  uint32_t ni_max = 0, cg_max = 0, nc = 0;
+  for (uint32_t i = 0; i < a.steps; ++i) {
+    s.advance(1);
+    const nguo_step_metrics m = s.metrics();
+    if (!m.all_finite) throw Failure("non-finite state at step " + std::to_string(i));
+    ni_max = std::max(ni_max, m.newton_iters);
+    cg_max = std::max(cg_max, m.max_cg_iters);
+    nc = std::max(nc, m.num_contacts);
+  }
+  const std::vector<float> x = s.positions(V);
+  float min_y = INFINITY;
+  for (uint32_t i = 0; i < V; ++i) min_y = std::min(min_y, x[3 * i + 1]);
+  std::printf("  max per step: newton %u  cg %u  contacts %u\n", ni_max, cg_max, nc);
+  std::printf("  stepped %u step(s) to t = %.4f: all finite, min y = %.6f\n", a.steps, s.metrics().time, min_y);
+  return 0;
+}
+
 }  // namespace
 }  // namespace mguso
 
@@ -497,7 +624,8 @@ int main(int argc, char **argv) {
     if (a.command == "bench") return cmd_bench(a);
     if (a.command == "run") return cmd_run(a);
     if (a.command == "verify") return cmd_verify(a);
-    throw Failure("`" + a.command + "` drives the reference's capture corpus, which is not distributed with this source and not part of this build");
+    if (a.command == "gauge-scene") return cmd_gauge_scene(a);
+    throw Failure("`" + a.command + "` fits solver parameters to the reference's capture corpus, which is not distributed with this source and not part of this build");
   } catch (const std::exception &e) {
     log(ERROR, a.command == "verify" ? "verify failed" : "command failed", {{"error", e.what()}});
     return 1;

@@ -198,23 +198,42 @@ class ArrayScene:
     Scene: positions [n, 3], inv_mass [n], tris [m, 3], static [n] (or None), colliders (a
     list of Collider) and params (a Params). The arrays are kept alive with the descriptor."""
 
-    def __init__(self, positions, inv_mass, tris, params, static=None, colliders=()):
+    def __init__(self, positions, inv_mass, tris, params, static=None, colliders=(), velocities=None, shell_faces=None,
+                 rods=None, rod_hinges=None):
+        """rods: (verts [r, 2], rest [r], k [r]); rod_hinges: (verts [h, 3] as (j, i, k), rest [h], stiff [h])."""
         self.positions = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
         self.n_verts = len(self.positions)
         self.inv_mass = np.ascontiguousarray(inv_mass, dtype=np.float32).reshape(-1)
         self.tris = np.ascontiguousarray(tris, dtype=np.uint32).reshape(-1, 3)
         self.static = None if static is None else np.ascontiguousarray(static, dtype=np.uint8).reshape(-1)
-        self._vel = np.zeros_like(self.positions)
+        self._vel = np.zeros_like(self.positions) if velocities is None else np.ascontiguousarray(velocities, dtype=np.float32).reshape(-1, 3)
         self._coll = (Collider * max(1, len(colliders)))(*colliders)
+        u32p, f32p = C.POINTER(C.c_uint32), C.POINTER(C.c_float)
         d = SceneDesc()
         d.n_verts = self.n_verts
-        d.positions = self.positions.ctypes.data_as(C.POINTER(C.c_float))
-        d.velocities = self._vel.ctypes.data_as(C.POINTER(C.c_float))
-        d.inv_mass = self.inv_mass.ctypes.data_as(C.POINTER(C.c_float))
+        d.positions = self.positions.ctypes.data_as(f32p)
+        d.velocities = self._vel.ctypes.data_as(f32p)
+        d.inv_mass = self.inv_mass.ctypes.data_as(f32p)
         d.n_tris = len(self.tris)
-        d.tris = self.tris.ctypes.data_as(C.POINTER(C.c_uint32))
+        d.tris = self.tris.ctypes.data_as(u32p)
         if self.static is not None:
             d.static_collider = self.static.ctypes.data_as(C.POINTER(C.c_uint8))
+        if shell_faces is not None and len(shell_faces):
+            self._shell = np.ascontiguousarray(shell_faces, dtype=np.uint32).reshape(-1, 3)
+            d.n_shell_faces = len(self._shell)
+            d.shell_faces = self._shell.ctypes.data_as(u32p)
+        if rods is not None and len(rods[0]):
+            self._rods = [np.ascontiguousarray(rods[0], dtype=np.uint32).reshape(-1, 2),
+                          np.ascontiguousarray(rods[1], dtype=np.float32).reshape(-1), np.ascontiguousarray(rods[2], dtype=np.float32).reshape(-1)]
+            d.n_rods = len(self._rods[0])
+            d.rod_verts, d.rod_rest, d.rod_k = (self._rods[0].ctypes.data_as(u32p), self._rods[1].ctypes.data_as(f32p),
+                                                self._rods[2].ctypes.data_as(f32p))
+        if rod_hinges is not None and len(rod_hinges[0]):
+            self._rh = [np.ascontiguousarray(rod_hinges[0], dtype=np.uint32).reshape(-1, 3),
+                        np.ascontiguousarray(rod_hinges[1], dtype=np.float32).reshape(-1), np.ascontiguousarray(rod_hinges[2], dtype=np.float32).reshape(-1)]
+            d.n_rod_hinges = len(self._rh[0])
+            d.rod_hinge_verts, d.rod_hinge_rest, d.rod_hinge_stiff = (self._rh[0].ctypes.data_as(u32p), self._rh[1].ctypes.data_as(f32p),
+                                                                      self._rh[2].ctypes.data_as(f32p))
         d.n_colliders = len(colliders)
         d.colliders = C.cast(self._coll, C.POINTER(Collider))
         d.params = params
