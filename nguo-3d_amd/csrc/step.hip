@@ -1946,7 +1946,12 @@ float ccd_max_step(nguo_step *st, float max_disp) {
                              iters, st->tmin.p);
       });
       if (full || (float)t_self * max_disp <= allowed) { t = std::min(t, t_self); break; }
-      allowed = std::min(allowed * 4.0f, max_disp);
+      // implicit.rs:1942 widens x4. Widening to exactly the motion t_self permits returns the
+      // same t (the reference's own argument: a pass whose radius covers t * max_disp of motion
+      // is the full-radius query for any t it returns): the next pass holds every pair of this
+      // one, so its t_self' <= t_self and t_self' * max_disp <= allowed certifies it. At most two
+      // passes, the second with the smallest radius that certifies.
+      allowed = std::min((float)t_self * max_disp, max_disp);
     }
     NGUO_REQUIRE(read_u32(st, st->err.p) == 0, NGUO_E_CAPACITY, "LBVH traversal stack overflow");
   }
