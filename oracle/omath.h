@@ -10,6 +10,8 @@
 #include <cmath>
 #include <cstdint>
 
+#include "ojet.h"
+
 namespace oracle {
 
 template <class T> struct V3 {

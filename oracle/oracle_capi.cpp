@@ -271,15 +271,21 @@ double oracle_solver_energy(void *p, const double *x, const double *xt, double i
 }  // extern "C"
 namespace {
 // Assembly switches of the derivative checks: 1 = the frozen-direction gradient model of the
-// held quantities, 2 = unprojected shell and rod-segment blocks, 4 = no strand bending terms.
+// held quantities, 2 = unprojected shell and rod-segment blocks, 4 = no strand bending terms,
+// 8 (with 1) = the frozen-parameter model instead (directions of x, exact Jacobian, FP64 CSR).
 template <class S> struct Switches {
   S &s;
   Switches(S &solver, const typename S::Frozen *f, int flags) : s(solver) {
     s.gn = (flags & 1) ? f : nullptr;
     s.project_elastic = !(flags & 2);
     s.with_rod_hinges = !(flags & 4);
+    s.exact_jac = (flags & 9) == 9;
+    s.csr64 = (flags & 9) == 9;
   }
-  ~Switches() { s.gn = nullptr; s.project_elastic = true; s.with_rod_hinges = true; s.keep_blocks = nullptr; }
+  ~Switches() {
+    s.gn = nullptr; s.project_elastic = true; s.with_rod_hinges = true; s.keep_blocks = nullptr;
+    s.exact_jac = false; s.csr64 = false;
+  }
 };
 }  // namespace
 extern "C" {
@@ -352,6 +358,57 @@ int oracle_solver_hess_vec(void *p, const double *x, const double *xt, double in
         for (int b = 0; b < 3; ++b) hu[3 * r + a] += (double)v[3 * a + b] * u[3 * c + b];
     }
     return 0;
+  };
+  return h->fp64 ? go(*h->s64, 0.0) : go(*h->s32, 0.0f);
+}
+
+// Holds the quantities of an assembly at the last step's end state (its inertial target, its
+// start as the friction reference) for the frozen-parameter calls below.
+void oracle_solver_freeze_end(void *p) {
+  auto *h = (Handle *)p;
+  if (h->fp64) h->fz64.reset(new Solver<double>::Frozen(h->s64->freeze_end()));
+  else h->fz32.reset(new Solver<float>::Frozen(h->s32->freeze_end()));
+}
+
+// Equilibrium of the frozen-parameter gradient (held quantities of oracle_solver_freeze_end,
+// the current rest scales, the last step's inertial target and friction reference) by Newton
+// from x_in (null: the state's positions). x_out [3 n]; res: the final largest free gradient
+// entry. Returns the Newton updates taken, -1 without held quantities.
+int oracle_solver_equilibrium(void *p, const double *x_in, double tol, uint32_t max_iter, double *x_out, double *res) {
+  auto *h = (Handle *)p;
+  uint32_t n = oracle_solver_nverts(p);
+  auto go = [&](auto &s, auto tag) -> int {
+    using T = decltype(tag);
+    auto *f = frozen_of<T>(h);
+    if (!f) return -1;
+    Switches<Solver<T>> sw(s, f, 1 | 2 | 8);
+    std::vector<V3<T>> x = x_in ? load_d<T>(x_in, n) : s.pos;
+    int it = (int)s.equilibrium(x, tol, max_iter, res);
+    for (uint32_t i = 0; i < n; ++i)
+      for (int c = 0; c < 3; ++c) x_out[3 * i + c] = (double)x[i][c];
+    return it;
+  };
+  return h->fp64 ? go(*h->s64, 0.0) : go(*h->s32, 0.0f);
+}
+
+// The drape adjoint of the frozen-parameter equilibrium at xeval [3 n] (the exact Jacobian
+// of the held-quantity gradient, FP64 CG to cg_tol): dLds [nfaces]. Returns the CG
+// iterations, or -1 without held quantities.
+int oracle_solver_adjoint_frozen(void *p, const double *dLdx, uint32_t nfaces, double cg_tol, const double *xeval, double *dLds,
+                                 double *residual) {
+  auto *h = (Handle *)p;
+  uint32_t n = oracle_solver_nverts(p);
+  std::vector<double> g(dLdx, dLdx + 3 * (size_t)n);
+  auto go = [&](auto &s, auto tag) -> int {
+    using T = decltype(tag);
+    auto *f = frozen_of<T>(h);
+    if (!f) return -1;
+    Switches<Solver<T>> sw(s, f, 1 | 2 | 8);
+    CgStats st;
+    std::vector<double> out = s.adjoint(g, nfaces, cg_tol, &st, true, xeval);
+    for (uint32_t k = 0; k < nfaces; ++k) dLds[k] = out[k];
+    if (residual) *residual = st.residual;
+    return (int)st.iters;
   };
   return h->fp64 ? go(*h->s64, 0.0) : go(*h->s32, 0.0f);
 }

@@ -31,6 +31,7 @@ struct Csr {
   uint32_t n = 0;
   std::vector<uint32_t> row_ptr, col_idx;
   std::vector<float> values;
+  std::vector<double> values64;  // FP64 copies of the values (when the assembly keeps them)
 };
 
 template <class T> struct Stencil {
@@ -41,6 +42,12 @@ template <class T> struct Stencil {
   uint8_t ee = 0;      // 0: vertex-triangle (idx = v, t0, t1, t2), 1: edge-edge (ea0, ea1, eb0, eb1)
   uint8_t inside = 0;  // vertex-triangle against a collider facet with the vertex inside (signed gap)
 };
+
+// `count` points as Jet coordinates: point a's coordinate c is input 3 a + c.
+inline void jet_points(const V3<double> *p, int count, V3<Jet> *out) {
+  for (int a = 0; a < count; ++a)
+    for (int c = 0; c < 3; ++c) out[a][c] = Jet::var(p[a][c], 3 * a + c);
+}
 
 struct CgStats {
   uint32_t iters = 0;
@@ -69,11 +76,13 @@ inline bool invert3x3(const double m[3][3], double out[3][3]) {
 // iteration; FP64 right-hand side and result.
 inline std::vector<double> cg_block_jacobi64(const Csr &a, const std::vector<double> &bf, double tol, uint32_t max_iter, uint32_t *iters_out) {
   const uint32_t n = a.n;
+  const bool v64 = !a.values64.empty();
+  auto val = [&](uint32_t k) { return v64 ? a.values64[k] : (double)a.values[k]; };
   bool block3 = n % 3 == 0;
   std::vector<double> inv_diag(n, 1.0);
   for (uint32_t r = 0; r < n; ++r)
     for (uint32_t k = a.row_ptr[r]; k < a.row_ptr[r + 1]; ++k)
-      if (a.col_idx[k] == r) { float d = a.values[k]; inv_diag[r] = d != 0.0f ? 1.0 / (double)d : 1.0; }
+      if (a.col_idx[k] == r) { double d = val(k); inv_diag[r] = d != 0.0 ? 1.0 / d : 1.0; }
   std::vector<std::array<double, 9>> binv;
   if (block3) {
     binv.resize(n / 3);
@@ -83,7 +92,7 @@ inline std::vector<double> cg_block_jacobi64(const Csr &a, const std::vector<dou
         uint32_t row = 3 * i + r;
         for (uint32_t k = a.row_ptr[row]; k < a.row_ptr[row + 1]; ++k) {
           uint32_t c = a.col_idx[k];
-          if (c >= 3 * i && c < 3 * i + 3) m[r][c - 3 * i] = (double)a.values[k];
+          if (c >= 3 * i && c < 3 * i + 3) m[r][c - 3 * i] = val(k);
         }
       }
       if (invert3x3(m, o)) {
@@ -109,7 +118,7 @@ inline std::vector<double> cg_block_jacobi64(const Csr &a, const std::vector<dou
   auto spmv = [&](const std::vector<double> &x, std::vector<double> &out) {
     for (uint32_t row = 0; row < n; ++row) {
       double s = 0;
-      for (uint32_t k = a.row_ptr[row]; k < a.row_ptr[row + 1]; ++k) s += (double)a.values[k] * x[a.col_idx[k]];
+      for (uint32_t k = a.row_ptr[row]; k < a.row_ptr[row + 1]; ++k) s += val(k) * x[a.col_idx[k]];
       out[row] = s;
     }
   };
@@ -444,6 +453,15 @@ template <class T> class Solver {
     std::vector<T> col_c, col_lam;
   };
   const Frozen *gn = nullptr;  // non-null: assemble the frozen-direction gradient model
+  // With gn: the frozen-parameter model instead. Only the stencil set, Rayleigh stiffnesses,
+  // barrier kinds and the friction quantities (normal, weights, lambda) stay those of x_ref;
+  // distances, distance gradients, hinge and strand angles and collider normals are those of
+  // x, and the assembled Hessian is the exact Jacobian of the assembled gradient: the barrier's
+  // b'(d) d(grad d)/dx, b'' unclamped, the hinge's (theta - rest) d(grad theta)/dx, the exact
+  // strand Jacobian and a sphere collider's b'(d) (I - n n^T) / r join the Gauss-Newton terms.
+  // The adjoint of an equilibrium of this gradient is its exact derivative.
+  bool exact_jac = false;
+  bool csr64 = false;  // the assembled CSR keeps FP64 values (Csr::values64) as well
   ShellElem<T> scaled(const ShellElem<T> &el) const {
     if (face_scale.empty()) return el;
     const T sc = face_scale[el.face];
@@ -821,7 +839,10 @@ template <class T> class Solver {
     auto push = [&](uint32_t r, uint32_t c, T v) { blk.push(r, c, v); };
     std::vector<Stencil<T>> sc = gn ? gn->st : (prm.self_collision ? stencils(x) : std::vector<Stencil<T>>{});
     if (gn)
-      for (auto &s0 : sc) s0.d = stencil_gap(x, s0);  // the gap moves, its direction stays
+      for (auto &s0 : sc) {
+        if (exact_jac) stencil_eval(x, s0);  // gap and direction of x
+        else s0.d = stencil_gap(x, s0);      // the gap moves, its direction stays
+      }
     // inertia
     for (uint32_t i = 0; i < n; ++i) {
       float im = inv_mass[i];
@@ -857,6 +878,13 @@ template <class T> class Solver {
       strand_force_hessian(x[v[0]], x[v[1]], x[v[2]], h.rest, g, hh);
       for (int a = 0; a < 3; ++a)
         for (int c = 0; c < 3; ++c) grad[3 * v[a] + c] += h.stiff * g[a][c];
+      if (exact_jac) {
+        const V pv[3] = {x[v[0]], x[v[1]], x[v[2]]};
+        double J[9][9];
+        strand_jacobian(pv, h.rest, J);
+        for (int a = 0; a < 9; ++a)
+          for (int b = 0; b < 9; ++b) hh[a * 9 + b] = T(J[a][b]);
+      }
       for (int a = 0; a < 3; ++a)
         for (int b = 0; b < 3; ++b)
           for (int r = 0; r < 3; ++r)
@@ -912,11 +940,13 @@ template <class T> class Solver {
         V p[4] = {x[hg.v[0]], x[hg.v[1]], x[hg.v[2]], x[hg.v[3]]};
         T theta = dihedral_angle(p);
         V g[4];
-        if (gn) for (int a = 0; a < 4; ++a) g[a] = gn->hinge_g[hi][a];
+        if (gn && !exact_jac) for (int a = 0; a < 4; ++a) g[a] = gn->hinge_g[hi][a];
         else dihedral_grad(p, g);
         T kwt = kb * hg.w;
         if (kwt <= T(0)) continue;
         T gval = theta - hg.rest;
+        double dth[12], Jg[12][12];
+        if (exact_jac) hinge_jacobian(p, dth, Jg);
         for (int a = 0; a < 4; ++a) {
           uint32_t va = hg.v[a];
           for (int c = 0; c < 3; ++c) grad[3 * va + c] += kwt * gval * g[a][c];
@@ -924,7 +954,7 @@ template <class T> class Solver {
             uint32_t vb = hg.v[b];
             for (int r = 0; r < 3; ++r)
               for (int cc = 0; cc < 3; ++cc) {
-                T v = kwt * g[a][r] * g[b][cc];
+                T v = exact_jac ? kwt * (g[a][r] * T(dth[3 * b + cc]) + gval * T(Jg[3 * a + r][3 * b + cc])) : kwt * g[a][r] * g[b][cc];
                 if (v != T(0)) push(3 * va + r, 3 * vb + cc, v);
               }
           }
@@ -945,15 +975,22 @@ template <class T> class Solver {
           T d;
           V nn;
           collider_distance(c, x[i], d, nn);
-          if (gn) nn = gn->col_n[fi];
+          if (gn && !exact_jac) nn = gn->col_n[fi];
           T k = kappa;  // contact_dynamic = 0 -> dynamic_stiffness returns the base kappa
           T bo[3];
           if (!barrier_of(prm.barrier_kind, d, dh, k, bo)) continue;
           V f_c = nn * bo[1];
           gsum += f_c;
-          T hp = std::fmax(bo[2], T(0));
+          T hp = exact_jac ? bo[2] : std::fmax(bo[2], T(0));
           for (int r = 0; r < 3; ++r)
             for (int cc = 0; cc < 3; ++cc) hsum[3 * r + cc] += hp * nn[r] * nn[cc];
+          if (exact_jac && c.kind != NGUO_COLLIDER_HALFSPACE) {
+            // d n / d x of a sphere's outward normal: (I - n n^T) / |x - c|
+            const T l = length(x[i] - V(T(c.a[0]), T(c.a[1]), T(c.a[2])));
+            if (l >= T(1.0e-9))
+              for (int r = 0; r < 3; ++r)
+                for (int cc = 0; cc < 3; ++cc) hsum[3 * r + cc] += bo[1] * ((r == cc ? T(1) : T(0)) - nn[r] * nn[cc]) / l;
+          }
           if (gn) {
             if (gn->col_fric[fi]) {
               const V u = x[i] - pos[i], n0 = gn->col_n[fi];
@@ -988,7 +1025,7 @@ template <class T> class Solver {
         int kind = gn ? gn->kind[fi] : (s.d < T(0) ? BARRIER_CUBIC : prm.barrier_kind);
         T bo[3];
         if (!barrier_of(kind, s.d, s.ghat, k, bo)) continue;
-        T bg = bo[1], hp = std::fmax(bo[2], T(0));
+        T bg = bo[1], hp = exact_jac ? bo[2] : std::fmax(bo[2], T(0));
         for (int a = 0; a < 4; ++a) {
           for (int c = 0; c < 3; ++c) grad[3 * s.idx[a] + c] += bg * s.g[a][c];
           for (int b = 0; b < 4; ++b)
@@ -997,6 +1034,17 @@ template <class T> class Solver {
                 T vh = hp * s.g[a][r] * s.g[b][cc];
                 if (vh != T(0)) push(3 * s.idx[a] + r, 3 * s.idx[b] + cc, vh);
               }
+        }
+        if (exact_jac && bg != T(0)) {
+          double J[12][12];
+          stencil_jacobian(x, s, J);
+          for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b)
+              for (int r = 0; r < 3; ++r)
+                for (int cc = 0; cc < 3; ++cc) {
+                  T vh = bg * T(J[3 * a + r][3 * b + cc]);
+                  if (vh != T(0)) push(3 * s.idx[a] + r, 3 * s.idx[b] + cc, vh);
+                }
         }
         T mu_s = T(prm.friction);
         if (gn) {
@@ -1064,12 +1112,14 @@ template <class T> class Solver {
         if (pin_i) {
           a.col_idx.push_back(3 * i + r);
           a.values.push_back(1.0f);
+          if (csr64) a.values64.push_back(1.0);
         } else {
           for (auto &cb : rows[i]) {
             if (inv_mass[cb.first] <= 0.0f) continue;
             for (int c = 0; c < 3; ++c) {
               a.col_idx.push_back(3 * cb.first + c);
               a.values.push_back((float)blk.val[cb.second][3 * r + c]);
+              if (csr64) a.values64.push_back((double)blk.val[cb.second][3 * r + c]);
             }
           }
         }
@@ -1091,6 +1141,64 @@ template <class T> class Solver {
       if (l < T(1.0e-9)) { d = -T(c.radius); nrm = V(0, 1, 0); }
       else { d = l - T(c.radius); nrm = v / l; }
     }
+  }
+
+  // The gap and distance gradient of stencil s at positions x.
+  void stencil_eval(const std::vector<V> &x, Stencil<T> &s) const {
+    const V a = x[s.idx[0]], b = x[s.idx[1]], c = x[s.idx[2]], d = x[s.idx[3]];
+    if (s.ee) s.d = segment_segment_distance_grad(a, b, c, d, s.g);
+    else if (s.inside) s.d = signed_point_tri_distance_grad(a, b, c, d, true, s.g);
+    else s.d = point_tri_distance_grad(a, b, c, d, s.g);
+  }
+  // J[3 a + r][3 b + c] = d g[a][r] / d x[idx[b]][c], g the distance gradient of stencil s at x
+  // (forward mode through the same function, so exact inside its branch region).
+  void stencil_jacobian(const std::vector<V> &x, const Stencil<T> &s, double J[12][12]) const {
+    V3<double> p[4];
+    for (int a = 0; a < 4; ++a) p[a] = V3<double>((double)x[s.idx[a]].x, (double)x[s.idx[a]].y, (double)x[s.idx[a]].z);
+    V3<Jet> q[4], g[4];
+    jet_points(p, 4, q);
+    if (s.ee) segment_segment_distance_grad(q[0], q[1], q[2], q[3], g);
+    else if (s.inside) signed_point_tri_distance_grad(q[0], q[1], q[2], q[3], true, g);
+    else point_tri_distance_grad(q[0], q[1], q[2], q[3], g);
+    for (int a = 0; a < 4; ++a)
+      for (int r = 0; r < 3; ++r)
+        for (int j = 0; j < 12; ++j) J[3 * a + r][j] = g[a][r].g[j];
+  }
+  // Hinge: dth[j] = d theta / dx_j and Jg[3 a + r][j] = d g[a][r] / dx_j, with theta the dihedral
+  // angle and g the analytic gradient the assembly uses.
+  static void hinge_jacobian(const V p[4], double dth[12], double Jg[12][12]) {
+    V3<double> pd[4];
+    for (int a = 0; a < 4; ++a) pd[a] = V3<double>((double)p[a].x, (double)p[a].y, (double)p[a].z);
+    V3<Jet> q[4], g[4];
+    jet_points(pd, 4, q);
+    const Jet th = dihedral_angle(q);
+    dihedral_grad(q, g);
+    for (int j = 0; j < 12; ++j) dth[j] = th.g[j];
+    for (int a = 0; a < 4; ++a)
+      for (int r = 0; r < 3; ++r)
+        for (int j = 0; j < 12; ++j) Jg[3 * a + r][j] = g[a][r].g[j];
+  }
+  // The strand's gradient (unit stiffness) as strand_force_hessian returns it, and its exact
+  // Jacobian J[3 a + r][3 b + c] over the three vertices.
+  template <class S> static void strand_grad(V3<S> x0, V3<S> x1, V3<S> x2, S rest, V3<S> grad[3]) {
+    const V3<S> e0 = x0 - x1, e1 = x2 - x1;
+    const S theta = std::atan2(length(cross(e0, e1)), dot(e0, e1));
+    V3<S> gt[2];
+    strand_gradient_theta(e0, e1, gt);
+    const S w = theta - rest;
+    grad[0] = w * gt[0];
+    grad[1] = w * (-gt[0] - gt[1]);
+    grad[2] = w * gt[1];
+  }
+  static void strand_jacobian(const V p[3], T rest, double J[9][9]) {
+    V3<double> pd[3];
+    for (int a = 0; a < 3; ++a) pd[a] = V3<double>((double)p[a].x, (double)p[a].y, (double)p[a].z);
+    V3<Jet> q[3], g[3];
+    jet_points(pd, 3, q);
+    strand_grad(q[0], q[1], q[2], Jet((double)rest), g);
+    for (int a = 0; a < 3; ++a)
+      for (int r = 0; r < 3; ++r)
+        for (int j = 0; j < 9; ++j) J[3 * a + r][j] = g[a][r].g[j];
   }
 
   // The gap of stencil s at positions x (its distance function, signed for a collider facet
@@ -1407,8 +1515,14 @@ template <class T> class Solver {
     pos = prev;  // friction reference of the last step
     std::vector<float> rhs;
     std::vector<T> kap;
-    Csr a = assemble(xs, last_xt, inv_h2, rhs, nullptr, &kap);
-    if (exact) {
+    Csr a;
+    if (gn) {
+      // the caller's held quantities and switches (the frozen-parameter model)
+      a = assemble(xs, last_xt, inv_h2, rhs);
+    } else {
+      a = assemble(xs, last_xt, inv_h2, rhs, nullptr, &kap);
+    }
+    if (exact && !gn) {
       // contact stiffnesses stay those of the step (Rayleigh against the projected blocks)
       project_elastic = false;
       a = assemble(xs, last_xt, inv_h2, rhs, nullptr, nullptr, nullptr, &kap);
@@ -1426,7 +1540,8 @@ template <class T> class Solver {
       double num = 0, den = 0;
       for (uint32_t row = 0; row < a.n; ++row) {
         double sacc = 0;
-        for (uint32_t k = a.row_ptr[row]; k < a.row_ptr[row + 1]; ++k) sacc += (double)a.values[k] * lam[a.col_idx[k]];
+        for (uint32_t k = a.row_ptr[row]; k < a.row_ptr[row + 1]; ++k)
+          sacc += (a.values64.empty() ? (double)a.values[k] : a.values64[k]) * lam[a.col_idx[k]];
         num += (b[row] - sacc) * (b[row] - sacc);
         den += b[row] * b[row];
       }
@@ -1458,6 +1573,74 @@ template <class T> class Solver {
       out[el0.face] -= acc;
     }
     return out;
+  }
+
+  // The held quantities (Frozen) of an assembly at the last step's end state, with that step's
+  // inertial target and friction reference (its start).
+  Frozen freeze_end() {
+    const T h = T(prm.dt);
+    std::vector<V> keep = pos, x = pos;
+    pos = prev;
+    Frozen f = freeze(x, last_xt, T(1) / (h * h));
+    pos = keep;
+    return f;
+  }
+
+  // Newton on the assembled gradient under the caller's switches (meant for the
+  // frozen-parameter model: gn with exact_jac and csr64), from x, with the last step's inertial
+  // target and friction reference: the FP64 Jacobian solved by the FP64 block-Jacobi CG to
+  // 1e-13, the step halved until the largest free gradient entry decreases. Stops once the
+  // largest update is below tol or after max_iter updates; returns the updates taken and the
+  // final largest free gradient entry in *res.
+  uint32_t equilibrium(std::vector<V> &x, double tol, uint32_t max_iter, double *res) {
+    const uint32_t n = (uint32_t)pos.size();
+    const T h = T(prm.dt), inv_h2 = T(1) / (h * h);
+    std::vector<V> keep = pos;
+    pos = prev;
+    auto gmax = [&](const std::vector<T> &g) {
+      double m = 0;
+      for (uint32_t i = 0; i < n; ++i)
+        if (inv_mass[i] > 0.0f)
+          for (int c = 0; c < 3; ++c) m = std::isfinite((double)g[3 * i + c]) ? std::fmax(m, std::fabs((double)g[3 * i + c])) : INFINITY;
+      return m;
+    };
+    std::vector<float> rhs;
+    std::vector<T> gr;
+    Csr a = assemble(x, last_xt, inv_h2, rhs, nullptr, nullptr, &gr);
+    double g0 = gmax(gr);
+    uint32_t it = 0;
+    while (it < max_iter) {
+      std::vector<double> b(3 * n, 0.0);
+      for (uint32_t i = 0; i < n; ++i)
+        if (inv_mass[i] > 0.0f)
+          for (int c = 0; c < 3; ++c) b[3 * i + c] = -(double)gr[3 * i + c];
+      uint32_t cgi = 0;
+      const std::vector<double> dx = cg_block_jacobi64(a, b, 1e-13, 100000, &cgi);
+      double dmax = 0;
+      for (double v : dx) dmax = std::fmax(dmax, std::fabs(v));
+      double step = 1.0, gnew = g0;
+      std::vector<V> xn;
+      std::vector<T> grn;
+      Csr an;
+      for (int ls = 0; ls < 40; ++ls, step *= 0.5) {
+        xn = x;
+        for (uint32_t i = 0; i < n; ++i)
+          if (inv_mass[i] > 0.0f)
+            for (int c = 0; c < 3; ++c) xn[i][c] += T(step * dx[3 * i + c]);
+        an = assemble(xn, last_xt, inv_h2, rhs, nullptr, nullptr, &grn);
+        gnew = gmax(grn);
+        if (gnew < g0 || step * dmax < tol) break;
+      }
+      x = xn;
+      a = std::move(an);
+      gr = std::move(grn);
+      g0 = gnew;
+      ++it;
+      if (step * dmax < tol) break;
+    }
+    pos = keep;
+    if (res) *res = g0;
+    return it;
   }
 
   // implicit.rs:1951-2191 step.

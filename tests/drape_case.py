@@ -82,3 +82,57 @@ def draws():
     theta_t = np.random.default_rng(5).normal(1.0, 0.02, 64)
     pick = np.random.default_rng(6).choice(64, 8, replace=False)
     return theta, theta_t, pick
+
+
+class FrozenDrape:
+    """The drape equilibrium with the step's contact parameters held (the frozen-parameter
+    model): one FP64 oracle step from the fixed start under theta gives x*; the stencil set,
+    Rayleigh stiffnesses, barrier kinds and friction quantities (normal, weights, lambda) of an
+    assembly at x* are held, and x(theta') is the root of the step gradient with them (distances,
+    angles and normals of x itself), by Newton from x* to a largest free gradient entry of
+    ~1e-13. The loss and its central differences run through that root; the adjoint is the
+    oracle's exact Jacobian of the same gradient."""
+
+    def __init__(self, scene, warm, panel, theta, theta_target):
+        self.scene = scene
+        self.o = o = O.OracleSolver(scene, fp64=True)
+        o.step(warm)
+        xs, vs = o.positions(), o.velocities()
+        o.set_state(xs, vs)
+        o.set_rest_scale(theta_target[panel])
+        o.step(1)
+        self.target = o.positions_d()
+        o.set_state(xs, vs)
+        o.set_rest_scale(theta[panel])
+        o.step(1)
+        self.step_metrics = o.metrics()
+        self.xstar = o.positions_d()
+        o.freeze_end()
+        self.panel = panel
+        self.free = scene.inv_mass > 0
+        self.nf = scene.desc.n_shell_faces
+
+    def solve(self, theta):
+        self.o.set_rest_scale(theta[self.panel])
+        return self.o.equilibrium(self.xstar)
+
+    def loss(self, theta):
+        x = self.solve(theta)[0]
+        d = (x - self.target)[self.free]
+        return float((d * d).sum(axis=1).mean()), x
+
+    def dloss_dx(self, x):
+        g = np.zeros_like(x)
+        g[self.free] = 2.0 * (x - self.target)[self.free] / self.free.sum()
+        return g
+
+    def central_fd(self, theta, k, h):
+        e = np.zeros_like(theta)
+        e[k] = h
+        return (self.loss(theta + e)[0] - self.loss(theta - e)[0]) / (2.0 * h)
+
+    def adjoint(self, theta):
+        """(per-panel dL/dtheta, CG residual, the equilibrium) through the frozen-parameter root."""
+        _, x = self.loss(theta)
+        g_face, _, res = self.o.adjoint_frozen(self.dloss_dx(x), self.nf, x)
+        return np.bincount(self.panel, weights=g_face, minlength=64), res, x

@@ -100,6 +100,11 @@ def load():
         lib.oracle_solver_hess_vec.restype = C.c_int
         lib.oracle_solver_hess_vec.argtypes = [vp, D, D, C.c_double, C.c_int, D, D]
         lib.oracle_solver_x_tilde_d.argtypes = [vp, D]
+        lib.oracle_solver_freeze_end.argtypes = [vp]
+        lib.oracle_solver_equilibrium.restype = C.c_int
+        lib.oracle_solver_equilibrium.argtypes = [vp, D, C.c_double, C.c_uint32, D, D]
+        lib.oracle_solver_adjoint_frozen.restype = C.c_int
+        lib.oracle_solver_adjoint_frozen.argtypes = [vp, D, C.c_uint32, C.c_double, D, D, D]
         lib.oracle_solver_frozen_stencils.restype = C.c_uint32
         lib.oracle_solver_frozen_stencils.argtypes = [vp, C.c_uint32, U, D, D, D, C.POINTER(C.c_int32), D, C.POINTER(C.c_int32), D, D, D]
         lib.oracle_morton_codes.argtypes = [C.c_uint32, F, C.c_uint32, U, U]
@@ -253,13 +258,40 @@ class OracleSolver:
         self.lib.oracle_solver_x_tilde_d(self.h, dp(out))
         return out
 
-    # Assembly switches of the derivative checks (oracle_solver_model_gradient / hess_vec).
+    # Assembly switches of the derivative checks (oracle_solver_model_gradient / hess_vec);
+    # FROZEN_PARAMS = GN_MODEL | UNPROJECTED | 8: the frozen-parameter model with its exact Jacobian.
     GN_MODEL, UNPROJECTED, NO_ROD_BENDS = 1, 2, 4
+    FROZEN_PARAMS = 1 | 2 | 8
     TERMS = ("inertia", "shells", "hinges", "rods", "rod_bends", "barriers", "friction", "colliders", "springs")
 
     def freeze(self, x, xt, inv_h2):
         """Hold the contact quantities of an assembly at x for energy / model_gradient."""
         self.lib.oracle_solver_freeze(self.h, dp(d64(x)), dp(d64(xt)), inv_h2)
+
+    def freeze_end(self):
+        """Hold the quantities of an assembly at the last step's end state (its target, its
+        start as the friction reference) for equilibrium / adjoint_frozen / FROZEN_PARAMS."""
+        self.lib.oracle_solver_freeze_end(self.h)
+
+    def equilibrium(self, x=None, tol=1e-13, max_iter=50):
+        """(x, Newton updates, final largest free gradient entry) of the frozen-parameter
+        equilibrium under the current rest scales, by Newton from x (default: the state)."""
+        out = np.zeros(3 * self.n)
+        res = np.zeros(1)
+        xi = None if x is None else d64(x).reshape(-1)
+        it = self.lib.oracle_solver_equilibrium(self.h, None if xi is None else dp(xi), tol, max_iter, dp(out), dp(res))
+        assert it >= 0, "freeze_end first"
+        return out.reshape(-1, 3), it, float(res[0])
+
+    def adjoint_frozen(self, dLdx, nfaces, xeval, cg_tol=1e-13):
+        """(dL/ds per shell face, CG iterations, residual) through the frozen-parameter
+        equilibrium at xeval, with its exact Jacobian."""
+        g = d64(dLdx).reshape(-1)
+        out = np.zeros(nfaces)
+        res = np.zeros(1)
+        it = self.lib.oracle_solver_adjoint_frozen(self.h, dp(g), nfaces, cg_tol, dp(d64(xeval).reshape(-1)), dp(out), dp(res))
+        assert it >= 0, "freeze_end first"
+        return out, it, float(res[0])
 
     def energy(self, x, xt, inv_h2, frozen=False):
         """(total, terms[9]) of the step energy at x; frozen: with the held contact quantities."""
