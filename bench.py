@@ -53,8 +53,9 @@ def main():
     ap.add_argument("--c3-solve", choices=["both", "batched", "concurrent"], default="both",
                     help="batch solve modes to time (a single mode also skips the sequential runs)")
     ap.add_argument("--c4-iters", type=int, default=20, help="timed gradient-descent iterations of the drape adjoint (0 = skip)")
-    ap.add_argument("--c4-hessian", choices=["exact", "newton"], default="newton",
-                    help="adjoint Hessian: exact elastic second derivatives, or the Newton iteration's projected blocks")
+    ap.add_argument("--c4-hessian", choices=["frozen", "exact", "newton"], default="newton",
+                    help="adjoint Hessian: the exact Jacobian with the contact parameters held at x* (frozen), exact elastic "
+                         "second derivatives only (exact), or the Newton iteration's projected blocks (newton)")
     ap.add_argument("--schwarz-domain", type=int, default=None, help="override the scene's Schwarz domain size")
     ap.add_argument("--schwarz-weight", type=float, default=None, help="override the scene's two-level coarse weight (0 = one level)")
     # 17.94 MB read + 1.22 MB written per launch of the roofline kernel (the Schwarz apply) on
@@ -238,7 +239,7 @@ def main():
             return st.positions().astype(np.float64)
 
         target = forward(theta_t)
-        exact = args.c4_hessian == "exact"
+        mode = {"frozen": st.ADJ_EXACT, "exact": st.ADJ_EXACT_ELASTIC, "newton": st.ADJ_NEWTON}[args.c4_hessian]
 
         def loss_of(x):
             d = (x - target)[free]
@@ -249,10 +250,10 @@ def main():
             dLdx[free] = 2.0 * d / free.sum()
             nonlocal fallbacks
             try:
-                g_face, stats = st.adjoint(dLdx, tol=1e-6, exact_elastic=exact)
+                g_face, stats = st.adjoint(dLdx, tol=1e-6, hessian=mode)
             except nh.NguoError:
-                fallbacks += 1  # H not positive definite at this end state: the Newton blocks
-                g_face, stats = st.adjoint(dLdx, tol=1e-6, exact_elastic=False)
+                fallbacks += 1  # the exact form does not solve at this end state: the Newton blocks
+                g_face, stats = st.adjoint(dLdx, tol=1e-6, hessian=st.ADJ_NEWTON)
             adj_iters.append(stats.iters)
             return np.bincount(panel, weights=g_face.astype(np.float64), minlength=64)
 

@@ -249,6 +249,15 @@ int nguo_step_set_rest_scale(nguo_step *st, const float *face_scale, uint32_t nf
  * step must not have been followed by set_state or set_rest_scale. */
 #define NGUO_ADJ_NEWTON 0        /* H as the Newton iteration assembles it */
 #define NGUO_ADJ_EXACT_ELASTIC 1 /* shells and rod segments unprojected (their exact second derivative) */
+/* The exact Jacobian of the step's gradient with the contact parameters held at x* (the stencil
+ * set, Rayleigh stiffnesses, barrier kinds, friction normal / weights / lambda: the
+ * frozen-parameter drape, whose derivative this is): the unprojected elastic blocks plus
+ * kappa b'(d) d(grad d)/dx per stencil, kb (theta - rest) d(grad theta)/dx per hinge, the exact
+ * strand blocks and a sphere's normal curvature. Solved as the fixed point of
+ * lam = H0^-1 (dL/dx - H2 lam) (H0 the Newton Hessian, H2 = H - H0, each step one PCG solve to
+ * tol) with Anderson mixing: stats->iters sums the PCG iterations, stats->sweeps counts the
+ * solves, stats->residual is the final relative update; NGUO_E_UNSUPPORTED above 1e-3. */
+#define NGUO_ADJ_EXACT 2
 int nguo_step_adjoint(nguo_step *st, const float *dLdx, float tol, int hessian, float *dLds, nguo_stats *stats);
 
 /* ---- batch of independent scenes (the graded-size form, C3). Each instance is a full step

@@ -482,6 +482,57 @@ __device__ inline void dihedral_grad(const f3 p[4], f3 g[4]) {
   g[3] = g3;
 }
 
+// The analytic dihedral gradient above in FP64 for the exact adjoint's Jacobian; a degenerate
+// hinge (zero-area wing or zero-length edge) gets a zero gradient.
+__device__ inline void dihedral_grad64(const d3 p[4], d3 g[4]) {
+  const d3 m1 = cross(p[1] - p[0], p[2] - p[0]), m2 = cross(p[1] - p[2], p[3] - p[2]), e = p[2] - p[1];
+  const double a1 = len(m1), a2 = len(m2), l2 = dot(e, e);
+  if (a1 < 1.0e-12 || a2 < 1.0e-12 || l2 < 1.0e-24) {
+    for (int k = 0; k < 4; ++k) g[k] = d3{0, 0, 0};
+    return;
+  }
+  const double l = sqrt(l2);
+  const d3 g0 = (m1 / a1) * (-l / a1), g3 = (m2 / a2) * (-l / a2);
+  const double ta = dot(p[0] - p[1], e) / l2, tb = dot(p[3] - p[1], e) / l2;
+  g[0] = g0;
+  g[1] = g0 * (ta - 1.0) + g3 * (tb - 1.0);
+  g[2] = g0 * (-ta) + g3 * (-tb);
+  g[3] = g3;
+}
+
+// The strand's gradient (unit stiffness, strand_force_hessian's grad) in FP64.
+__device__ inline void strand_grad64(d3 x0, d3 x1, d3 x2, double rest_angle, d3 grad[3]) {
+  const d3 e0 = x0 - x1, e1 = x2 - x1, cr = cross(e0, e1);
+  const double theta = atan2(len(cr), dot(e0, e1));
+  const double cross_sq = dot(cr, cr), e0_sq = dot(e0, e0), e1_sq = dot(e1, e1);
+  d3 gt0{0, 0, 0}, gt1{0, 0, 0};
+  if (!(cross_sq <= 1.0e-12 * e0_sq * e1_sq)) {
+    const d3 n = cr / sqrt(cross_sq);
+    gt0 = cross(e0, n) / e0_sq;
+    gt1 = -(cross(e1, n)) / e1_sq;
+  }
+  const double w = theta - rest_angle;
+  grad[0] = w * gt0;
+  grad[1] = w * (-gt0 - gt1);
+  grad[2] = w * gt1;
+}
+
+// Central-difference Jacobian of an FP64 gradient function over NV points: J[(3 a + r) * 12 + j]
+// = d grad[a][r] / d coordinate j (j < 3 NV), step 1e-6 * scale.
+template <int NV, class Grad> __device__ inline void fd_jacobian64(const d3 p[NV], double scale, Grad grad, double *J) {
+  const double eps = 1.0e-6 * scale, inv = 1.0 / (2.0 * eps);
+  for (int j = 0; j < 3 * NV; ++j) {
+    d3 pp[NV], pm[NV], gp[NV], gm[NV];
+    for (int k = 0; k < NV; ++k) { pp[k] = p[k]; pm[k] = p[k]; }
+    setc(pp[j / 3], j % 3, comp(pp[j / 3], j % 3) + eps);
+    setc(pm[j / 3], j % 3, comp(pm[j / 3], j % 3) - eps);
+    grad(pp, gp);
+    grad(pm, gm);
+    for (int a = 0; a < NV; ++a)
+      for (int r = 0; r < 3; ++r) J[(3 * a + r) * 12 + j] = (comp(gp[a], r) - comp(gm[a], r)) * inv;
+  }
+}
+
 // implicit.rs:3287-3308 rod_stretch_block (project = false: compressed segments keep their
 // negative transverse stiffness, the exact second derivative).
 __device__ inline void rod_stretch_block(f3 d, float rest, float k, float h[9], f3 &ga, bool project = true) {

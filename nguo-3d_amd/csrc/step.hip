@@ -130,6 +130,216 @@ __global__ __launch_bounds__(BLOCK) void k_shell_dgds(ElemDev E, Params P, const
   out[face[e]] = -acc;
 }
 
+// A stencil's gap and distance gradient in FP64 (the FP64 scheme's own closest points),
+// signed like its FP32 gap d32 (negative for a vertex inside the static body).
+__device__ inline double stencil_geom64(bool vt, const uint32_t id[4], const float *__restrict__ x, float d32, d3 g[4]) {
+  const d3 p0 = to_d(ld3(x, id[0])), p1 = to_d(ld3(x, id[1])), p2 = to_d(ld3(x, id[2])), p3 = to_d(ld3(x, id[3]));
+  double d = vt ? point_tri_distance_grad(p0, p1, p2, p3, g) : segment_segment_distance_grad(p0, p1, p2, p3, g);
+  if (d32 < 0.0f) {
+    d = -d;
+    for (int a = 0; a < 4; ++a) g[a] = -g[a];
+  }
+  return d;
+}
+
+// ---- exact adjoint (NGUO_ADJ_EXACT): the Jacobian terms the Newton Hessian leaves out, as
+// dense FP64 blocks over up to four vertices (dof[4 k + a]: free block row, NONE: no row or
+// column; blk[144 k + (3 a + r) * 12 + 3 b + c]). Per contact stencil kappa b'(d) times the
+// Jacobian of the distance gradient (kappa, barrier kind and stencil set those of the step's
+// end state, as the step's assembly there holds them); per hinge kb (theta - rest) times the
+// Jacobian of the dihedral gradient; per strand the exact Jacobian of its gradient minus the
+// Wu-Kim block the Newton Hessian holds; per shell, rod segment and edge spring the
+// unprojected block minus the projected one; per vertex the spheres' b'(d) (I - n n^T) /
+// |x - c|. The Jacobians of the FP64 gradients are central differences (devmath
+// fd_jacobian64).
+struct AdjBlocks {
+  uint32_t *dof;
+  double *blk;
+};
+
+__device__ inline double span64(const d3 *p, int nv) {
+  double m = 0.0;
+  for (int a = 0; a < nv; ++a)
+    for (int b = a + 1; b < nv; ++b) m = fmax(m, len(p[a] - p[b]));
+  return m > 0.0 ? m : 1.0;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_adj_contacts(uint32_t ns, uint32_t nvt, const uint32_t *__restrict__ sidx, const float *__restrict__ sd,
+                                                       const float *__restrict__ kappa, float ghat_vt, float ghat_ee, Params P,
+                                                       const uint32_t *__restrict__ vdof, const float *__restrict__ x, AdjBlocks B) {
+  const uint32_t s = blockIdx.x * BLOCK + threadIdx.x;
+  if (s >= ns) return;
+  uint32_t id[4];
+  d3 p[4];
+  for (int a = 0; a < 4; ++a) {
+    id[a] = sidx[4 * (size_t)s + a];
+    p[a] = to_d(ld3(x, id[a]));
+    B.dof[4 * (size_t)s + a] = vdof[id[a]];
+  }
+  double *J = B.blk + 144 * (size_t)s;
+  const bool vt = s < nvt, neg = sd[s] < 0.0f;
+  auto grad = [&](const d3 *q, d3 *g) {
+    if (vt) (void)point_tri_distance_grad(q[0], q[1], q[2], q[3], g);
+    else (void)segment_segment_distance_grad(q[0], q[1], q[2], q[3], g);
+    if (neg)
+      for (int a = 0; a < 4; ++a) g[a] = -g[a];
+  };
+  d3 g[4];
+  const double d = stencil_geom64(vt, id, x, sd[s], g);
+  const int kind = neg ? 1 : P.barrier_kind;
+  double bo[3] = {0, 0, 0};
+  const bool act = barrier_of(kind, d, (double)(vt ? ghat_vt : ghat_ee), (double)kappa[s], bo);
+  if (!act || bo[1] == 0.0) {
+    for (int k = 0; k < 144; ++k) J[k] = 0.0;
+    return;
+  }
+  fd_jacobian64<4>(p, span64(p, 4), grad, J);
+  for (int k = 0; k < 144; ++k) J[k] *= bo[1];
+}
+
+__global__ __launch_bounds__(BLOCK) void k_adj_hinges(ElemDev E, Params P, const uint32_t *__restrict__ vdof, const float *__restrict__ x, AdjBlocks B) {
+  const uint32_t e = blockIdx.x * BLOCK + threadIdx.x;
+  if (e >= E.nh) return;
+  d3 p[4];
+  for (int a = 0; a < 4; ++a) {
+    const uint32_t v = E.hg_v[4 * (size_t)e + a];
+    p[a] = to_d(ld3(x, v));
+    B.dof[4 * (size_t)e + a] = vdof[v];
+  }
+  double *J = B.blk + 144 * (size_t)e;
+  const double c = (double)P.kb * (dihedral_angle(p) - (double)E.hg_rest[e]);
+  if (!(P.kb > 0.0f) || c == 0.0) {
+    for (int k = 0; k < 144; ++k) J[k] = 0.0;
+    return;
+  }
+  fd_jacobian64<4>(p, span64(p, 4), [](const d3 *q, d3 *g) { dihedral_grad64(q, g); }, J);
+  for (int k = 0; k < 144; ++k) J[k] *= c;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_adj_strands(ElemDev E, const uint32_t *__restrict__ vdof, const float *__restrict__ x, AdjBlocks B) {
+  const uint32_t e = blockIdx.x * BLOCK + threadIdx.x;
+  if (e >= E.nrh) return;
+  d3 p[3];
+  for (int a = 0; a < 3; ++a) {
+    const uint32_t v = E.rh_v[3 * (size_t)e + a];
+    p[a] = to_d(ld3(x, v));
+    B.dof[4 * (size_t)e + a] = vdof[v];
+  }
+  B.dof[4 * (size_t)e + 3] = NONE;
+  double *J = B.blk + 144 * (size_t)e;
+  for (int k = 0; k < 144; ++k) J[k] = 0.0;
+  const double rest = (double)E.rh_rest[e], st = (double)E.rh_stiff[e];
+  if (!(st > 0.0)) return;
+  fd_jacobian64<3>(p, span64(p, 3), [rest](const d3 *q, d3 *g) { strand_grad64(q[0], q[1], q[2], rest, g); }, J);
+  const float *ob = E.out_dense + 81 * (size_t)e;  // the Newton Hessian's block (stiffness included)
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+          double &v = J[(3 * a + r) * 12 + 3 * b + c];
+          v = st * v - (double)ob[(a * 3 + b) * 9 + 3 * r + c];
+        }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_adj_shells(ElemDev E, Params P, const uint32_t *__restrict__ vdof, const float *__restrict__ x, AdjBlocks B) {
+  const uint32_t e = blockIdx.x * BLOCK + threadIdx.x;
+  if (e >= E.ns) return;
+  const uint32_t *v = E.sh_v + 3 * (size_t)e;
+  for (int a = 0; a < 3; ++a) B.dof[4 * (size_t)e + a] = vdof[v[a]];
+  B.dof[4 * (size_t)e + 3] = NONE;
+  f3 p[3] = {ld3(x, v[0]), ld3(x, v[1]), ld3(x, v[2])};
+  const float *dm = E.sh_dm + 4 * (size_t)e;
+  const float sc = E.sh_scale[e];
+  Mat2 m{dm[0] / sc, dm[1] / sc, dm[2] / sc, dm[3] / sc};
+  f3 g[3];
+  float h[81];
+  shell_element(p, m, E.sh_area[e] * sc * sc, P.shell_model, P.shell_mu, P.shell_lambda, P.eig_eps, g, h, false);
+  const float *ob = E.out_dense + 81 * ((size_t)E.nrh + e);  // the projected block of the assembly
+  double *J = B.blk + 144 * (size_t)e;
+  for (int k = 0; k < 144; ++k) J[k] = 0.0;
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) J[(3 * a + r) * 12 + 3 * b + c] = (double)h[(3 * a + r) * 9 + 3 * b + c] - (double)ob[(a * 3 + b) * 9 + 3 * r + c];
+}
+
+__global__ __launch_bounds__(BLOCK) void k_adj_rods(ElemDev E, float ksp, const uint32_t *__restrict__ vdof, const float *__restrict__ x, AdjBlocks B) {
+  const uint32_t e = blockIdx.x * BLOCK + threadIdx.x;
+  if (e >= E.nr + E.nsp) return;
+  float h[9];
+  f3 ga;
+  uint32_t va, vb;
+  if (e < E.nr) {
+    va = E.rd_v[2 * (size_t)e];
+    vb = E.rd_v[2 * (size_t)e + 1];
+    rod_stretch_block(ld3(x, va) - ld3(x, vb), E.rd_rest[e], E.rd_k[e], h, ga, false);
+  } else {
+    const uint32_t sp = e - E.nr;
+    va = E.sp_v[2 * (size_t)sp];
+    vb = E.sp_v[2 * (size_t)sp + 1];
+    spring_block(ld3(x, va) - ld3(x, vb), E.sp_rest[sp], ksp, h, ga, false);
+  }
+  B.dof[4 * (size_t)e] = vdof[va];
+  B.dof[4 * (size_t)e + 1] = vdof[vb];
+  B.dof[4 * (size_t)e + 2] = NONE;
+  B.dof[4 * (size_t)e + 3] = NONE;
+  const float *o = E.out_rod + 12 * (size_t)e;  // the projected block of the assembly
+  double *J = B.blk + 144 * (size_t)e;
+  for (int k = 0; k < 144; ++k) J[k] = 0.0;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      const double dv = (double)h[3 * r + c] - (double)o[3 * r + c];
+      J[r * 12 + c] = dv;
+      J[(3 + r) * 12 + 3 + c] = dv;
+      J[r * 12 + 3 + c] = -dv;
+      J[(3 + r) * 12 + c] = -dv;
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_adj_spheres(uint32_t V, const float *__restrict__ im, const uint32_t *__restrict__ vdof, const float *__restrict__ x,
+                                                      Params P, AdjBlocks B) {
+  const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= V) return;
+  B.dof[4 * (size_t)i] = im[i] > 0.0f ? vdof[i] : NONE;
+  for (int a = 1; a < 4; ++a) B.dof[4 * (size_t)i + a] = NONE;
+  double *J = B.blk + 144 * (size_t)i;
+  for (int k = 0; k < 144; ++k) J[k] = 0.0;
+  if (!(im[i] > 0.0f)) return;
+  const d3 xi = to_d(ld3(x, i));
+  for (uint32_t k = 0; k < P.ncoll; ++k) {
+    const nguo_collider &c = P.coll[k];
+    if (c.kind == NGUO_COLLIDER_HALFSPACE) continue;
+    const d3 v = xi - d3{c.a[0], c.a[1], c.a[2]};
+    const double l = len(v);
+    if (l < 1.0e-9) continue;
+    const d3 n = v / l;
+    double bo[3];
+    if (!barrier_of(P.barrier_kind, l - (double)c.radius, (double)P.dhat, (double)P.kappa, bo)) continue;
+    for (int r = 0; r < 3; ++r)
+      for (int cc = 0; cc < 3; ++cc) J[r * 12 + cc] += bo[1] * ((r == cc ? 1.0 : 0.0) - comp(n, r) * comp(n, cc)) / l;
+  }
+}
+
+// y += H2 x over the blocks (FP64; blocks sharing a row add atomically).
+__global__ __launch_bounds__(BLOCK) void k_adj_apply(uint32_t nb, AdjBlocks B, const double *__restrict__ x, double *y) {
+  const uint32_t k = blockIdx.x * BLOCK + threadIdx.x;
+  if (k >= nb) return;
+  uint32_t dof[4];
+  for (int a = 0; a < 4; ++a) dof[a] = B.dof[4 * (size_t)k + a];
+  const double *J = B.blk + 144 * (size_t)k;
+  for (int a = 0; a < 4; ++a) {
+    if (dof[a] == NONE) continue;
+    for (int r = 0; r < 3; ++r) {
+      double acc = 0.0;
+      for (int b = 0; b < 4; ++b) {
+        if (dof[b] == NONE) continue;
+        for (int c = 0; c < 3; ++c) acc += J[(3 * a + r) * 12 + 3 * b + c] * x[3 * (size_t)dof[b] + c];
+      }
+      if (acc != 0.0) atomicAdd(&y[3 * (size_t)dof[a] + r], acc);
+    }
+  }
+}
+
 __global__ __launch_bounds__(BLOCK) void k_hinges(ElemDev E, Params P, const float *__restrict__ x) {
   uint32_t e = blockIdx.x * BLOCK + threadIdx.x;
   if (e >= E.nh) return;
@@ -1149,18 +1359,6 @@ __device__ inline double friction_potential(double c, d3 n, d3 u, double eps) {
   return c * (sl <= eps ? sl * sl / (2.0 * eps) : sl - 0.5 * eps);
 }
 
-// A stencil's gap and distance gradient in FP64 (the FP64 scheme's own closest points),
-// signed like its FP32 gap d32 (negative for a vertex inside the static body).
-__device__ inline double stencil_geom64(bool vt, const uint32_t id[4], const float *__restrict__ x, float d32, d3 g[4]) {
-  const d3 p0 = to_d(ld3(x, id[0])), p1 = to_d(ld3(x, id[1])), p2 = to_d(ld3(x, id[2])), p3 = to_d(ld3(x, id[3]));
-  double d = vt ? point_tri_distance_grad(p0, p1, p2, p3, g) : segment_segment_distance_grad(p0, p1, p2, p3, g);
-  if (d32 < 0.0f) {
-    d = -d;
-    for (int a = 0; a < 4; ++a) g[a] = -g[a];
-  }
-  return d;
-}
-
 // Inertia and analytic colliders, one thread per vertex.
 __global__ __launch_bounds__(BLOCK) void k_energy_verts(uint32_t V, const float *__restrict__ im, const float *__restrict__ x,
                                                        const float *__restrict__ xt, const float *__restrict__ xs, double inv_h2, Params P,
@@ -1458,6 +1656,8 @@ struct nguo_step {
   uint32_t n_shell_faces = 0;
   DevBuf<float> xprev, lam, adj_b, dgds;
   std::vector<float> h_adj_b;
+  DevBuf<uint32_t> adj_dof;  // exact adjoint: the Jacobian blocks the Newton Hessian leaves out
+  DevBuf<double> adj_blk, adj_x64, adj_y64;
   bool adj_valid = false;  // the last step can be differentiated (cleared by set_state / set_rest_scale)
   std::vector<float> trace_toi;
   bool own_stream = false;
@@ -2750,13 +2950,167 @@ int nguo_step_set_rest_scale(nguo_step *st, const float *face_scale, uint32_t nf
   });
 }
 
+// The exact adjoint solve (H0 + H2) lam = b, with H0 the matrix and contact operator
+// assembled at x* (the elastic blocks unprojected, or, with elastic_diffs, projected as the
+// Newton Hessian has them) and H2 the Jacobian blocks H0 leaves out (k_adj_*; with
+// elastic_diffs also the elastic blocks' unprojected-minus-projected differences). lam is the fixed point of G(lam) = H0^-1 (b - H2 lam), each G one
+// solve of the step's PCG to `tol`; Anderson mixing over the last EXACT_ADJ_DEPTH updates
+// (the affine fixed point's GMRES) converges where the fixed-point map's spectral radius
+// (that of H0^-1 H2) alone would be slow. Stops when the relative update |G(lam) - lam| /
+// |G(lam)| falls to tol, or after EXACT_ADJ_MAX solves. b = adj_b on entry (FP32, free dofs),
+// the solution in lam; returns the last solve's stats with iters summed over all solves.
+}  // extern "C"
+namespace {
+constexpr uint32_t EXACT_ADJ_DEPTH = 6, EXACT_ADJ_MAX = 60;
+constexpr float EXACT_ADJ_ACCEPT = 1e-3f;  // relative update above which the result is refused
+template <class Solve>
+PcgStats solve_exact_adjoint(nguo_step *st, float tol, bool elastic_diffs, Solve solve_h0, uint32_t *outer, double *outer_res) {
+  hipStream_t s = st->s;
+  const size_t n = 3 * (size_t)st->NF;
+  const ElemDev E = st->act.dev();
+  uint32_t n_sph = 0;
+  for (uint32_t k = 0; k < st->P.ncoll; ++k) n_sph += st->P.coll[k].kind != NGUO_COLLIDER_HALFSPACE;
+  const uint32_t ne_s = elastic_diffs ? E.ns : 0, ne_r = elastic_diffs ? E.nr + E.nsp : 0;
+  const uint32_t o_h = st->ns, o_r = o_h + E.nh, o_s = o_r + E.nrh, o_e = o_s + ne_s, o_v = o_e + ne_r;
+  const uint32_t nb = o_v + (n_sph ? st->V : 0);
+  st->adj_dof.reserve(4 * (size_t)nb);
+  st->adj_blk.reserve(144 * (size_t)nb);
+  st->adj_x64.reserve(n);
+  st->adj_y64.reserve(n);
+  auto at = [&](uint32_t off) { return AdjBlocks{st->adj_dof.p + 4 * (size_t)off, st->adj_blk.p + 144 * (size_t)off}; };
+  const float dhat = st->P.dhat;
+  if (st->ns)
+    hipLaunchKernelGGL(k_adj_contacts, dim3(div_up(st->ns, BLOCK)), dim3(BLOCK), 0, s, st->ns, st->nvt, (const uint32_t *)st->sidx.p,
+                       (const float *)st->sd.p, (const float *)st->ckappa.p, dhat, 0.5f * (dhat + dhat), st->P, (const uint32_t *)st->vdof.p,
+                       (const float *)st->x.p, at(0));
+  if (E.nh)
+    hipLaunchKernelGGL(k_adj_hinges, dim3(div_up(E.nh, BLOCK)), dim3(BLOCK), 0, s, E, st->P, (const uint32_t *)st->vdof.p, (const float *)st->x.p,
+                       at(o_h));
+  if (E.nrh)
+    hipLaunchKernelGGL(k_adj_strands, dim3(div_up(E.nrh, BLOCK)), dim3(BLOCK), 0, s, E, (const uint32_t *)st->vdof.p, (const float *)st->x.p,
+                       at(o_r));
+  if (ne_s)
+    hipLaunchKernelGGL(k_adj_shells, dim3(div_up(ne_s, BLOCK)), dim3(BLOCK), 0, s, E, st->P, (const uint32_t *)st->vdof.p, (const float *)st->x.p,
+                       at(o_s));
+  if (ne_r)
+    hipLaunchKernelGGL(k_adj_rods, dim3(div_up(ne_r, BLOCK)), dim3(BLOCK), 0, s, E, st->P.ksp, (const uint32_t *)st->vdof.p,
+                       (const float *)st->x.p, at(o_e));
+  if (n_sph)
+    hipLaunchKernelGGL(k_adj_spheres, dim3(div_up(st->V, BLOCK)), dim3(BLOCK), 0, s, st->V, (const float *)st->im.p,
+                       (const uint32_t *)st->vdof.p, (const float *)st->x.p, st->P, at(o_v));
+  NGUO_HIP(hipGetLastError());
+  std::vector<double> b(n), lam(n, 0.0), g(n), f(n), y(n), g_prev, f_prev;
+  for (size_t i = 0; i < n; ++i) b[i] = (double)st->h_adj_b[i];
+  std::vector<float> rf(n), lf(n);
+  std::vector<std::vector<double>> dG, dF;
+  PcgStats ps{};
+  uint32_t total = 0;
+  auto nrm = [](const std::vector<double> &v) {
+    double a = 0.0;
+    for (double e : v) a += e * e;
+    return std::sqrt(a);
+  };
+  *outer = 0;
+  *outer_res = INFINITY;
+  bool first = true;
+  while (*outer < EXACT_ADJ_MAX) {
+    // G(lam) = H0^-1 (b - H2 lam)
+    if (first) {
+      for (size_t i = 0; i < n; ++i) rf[i] = (float)b[i];
+    } else {
+      st->adj_x64.upload(lam.data(), n, s);
+      st->adj_y64.zero(n, s);
+      hipLaunchKernelGGL(k_adj_apply, dim3(div_up(nb, BLOCK)), dim3(BLOCK), 0, s, nb, at(0), (const double *)st->adj_x64.p, st->adj_y64.p);
+      NGUO_HIP(hipGetLastError());
+      st->adj_y64.download(y.data(), n, s);
+      NGUO_HIP(hipStreamSynchronize(s));
+      for (size_t i = 0; i < n; ++i) rf[i] = (float)(b[i] - y[i]);
+    }
+    st->adj_b.upload(rf.data(), n, s);
+    ps = solve_h0();
+    total += ps.iters;
+    ++*outer;
+    if (ps.breakdown) break;
+    st->lam.download(lf.data(), n, s);
+    NGUO_HIP(hipStreamSynchronize(s));
+    for (size_t i = 0; i < n; ++i) {
+      g[i] = (double)lf[i];
+      f[i] = g[i] - lam[i];
+    }
+    const double gn = nrm(g);
+    *outer_res = gn > 0.0 ? nrm(f) / gn : 0.0;
+    if (nb == 0 || *outer_res <= (double)tol) break;
+    if (!first) {
+      std::vector<double> df(n), dg(n);
+      for (size_t i = 0; i < n; ++i) { df[i] = f[i] - f_prev[i]; dg[i] = g[i] - g_prev[i]; }
+      dF.push_back(std::move(df));
+      dG.push_back(std::move(dg));
+      if (dF.size() > EXACT_ADJ_DEPTH) { dF.erase(dF.begin()); dG.erase(dG.begin()); }
+    }
+    f_prev = f;
+    g_prev = g;
+    first = false;
+    // lam = g - dG gamma, gamma = argmin |f - dF gamma| (normal equations, m <= depth)
+    const size_t m = dF.size();
+    std::vector<double> gam(m, 0.0);
+    if (m) {
+      std::vector<double> A(m * m), r(m);
+      double tr = 0.0;
+      for (size_t i = 0; i < m; ++i) {
+        for (size_t j = 0; j < m; ++j) {
+          double a = 0.0;
+          for (size_t k = 0; k < n; ++k) a += dF[i][k] * dF[j][k];
+          A[i * m + j] = a;
+        }
+        double a = 0.0;
+        for (size_t k = 0; k < n; ++k) a += dF[i][k] * f[k];
+        r[i] = a;
+        tr += A[i * m + i];
+      }
+      for (size_t i = 0; i < m; ++i) A[i * m + i] += 1e-14 * tr;
+      // Gaussian elimination with partial pivoting on the small system
+      for (size_t c = 0; c < m; ++c) {
+        size_t pv = c;
+        for (size_t i = c + 1; i < m; ++i)
+          if (std::fabs(A[i * m + c]) > std::fabs(A[pv * m + c])) pv = i;
+        if (pv != c) {
+          for (size_t j = 0; j < m; ++j) std::swap(A[c * m + j], A[pv * m + j]);
+          std::swap(r[c], r[pv]);
+        }
+        if (A[c * m + c] == 0.0) continue;
+        for (size_t i = c + 1; i < m; ++i) {
+          const double q = A[i * m + c] / A[c * m + c];
+          for (size_t j = c; j < m; ++j) A[i * m + j] -= q * A[c * m + j];
+          r[i] -= q * r[c];
+        }
+      }
+      for (size_t c = m; c-- > 0;) {
+        double a = r[c];
+        for (size_t j = c + 1; j < m; ++j) a -= A[c * m + j] * gam[j];
+        gam[c] = A[c * m + c] != 0.0 ? a / A[c * m + c] : 0.0;
+      }
+    }
+    for (size_t i = 0; i < n; ++i) {
+      double v = g[i];
+      for (size_t k = 0; k < m; ++k) v -= dG[k][i] * gam[k];
+      lam[i] = v;
+    }
+  }
+  // the last G(lam) is in st->lam (its solve's output)
+  ps.iters = total;
+  return ps;
+}
+}  // namespace
+extern "C" {
+
 int nguo_step_adjoint(nguo_step *st, const float *dLdx, float tol, int hessian, float *dLds, nguo_stats *stats) {
   return guarded([&]() -> int {
     NGUO_REQUIRE(st && dLdx && dLds, NGUO_E_INVALID, "nguo_step_adjoint: null argument");
     NGUO_REQUIRE(st->adj_valid, NGUO_E_INVALID,
                  "nguo_step_adjoint: no step to differentiate (advance first; set_state and set_rest_scale clear the last step)");
     NGUO_REQUIRE(tol > 0.0f, NGUO_E_INVALID, "nguo_step_adjoint: tol must be > 0");
-    NGUO_REQUIRE(hessian == NGUO_ADJ_NEWTON || hessian == NGUO_ADJ_EXACT_ELASTIC, NGUO_E_INVALID, "nguo_step_adjoint: unknown hessian mode");
+    NGUO_REQUIRE(hessian == NGUO_ADJ_NEWTON || hessian == NGUO_ADJ_EXACT_ELASTIC || hessian == NGUO_ADJ_EXACT, NGUO_E_INVALID,
+                 "nguo_step_adjoint: unknown hessian mode");
     hipStream_t s = st->s;
     const uint32_t nfc = st->n_shell_faces;
     if (nfc) std::memset(dLds, 0, nfc * sizeof(float));
@@ -2769,7 +3123,7 @@ int nguo_step_adjoint(nguo_step *st, const float *dLdx, float tol, int hessian, 
     NGUO_HIP(hipMemcpyAsync(st->x0.p, st->xprev.p, 3 * (size_t)st->V * sizeof(float), hipMemcpyDeviceToDevice, s));
     assemble_elastic(st, st->P, s);
     contacts(st);  // contact stiffnesses from the step's (projected) elastic blocks
-    if (hessian == NGUO_ADJ_EXACT_ELASTIC) {
+    if (hessian != NGUO_ADJ_NEWTON) {
       Params PA = st->P;
       PA.project = 0;
       assemble_elastic(st, PA, s);
@@ -2783,15 +3137,36 @@ int nguo_step_adjoint(nguo_step *st, const float *dLdx, float tol, int hessian, 
     st->lam.reserve(3 * (size_t)st->NF);
     PcgStats ps;
     prepare_schwarz(st);
-    if (uses_fp64(st)) {
-      if (!st->pcg64) st->pcg64.reset(new Pcg64(s));
-      ps = st->pcg64->solve(st->act.M, contact_view(st), *st->sz, st->adj_b.p, tol, st->prm.cg_max_iter, st->lam.p);
+    auto solve_h0 = [&]() {
+      if (uses_fp64(st)) {
+        if (!st->pcg64) st->pcg64.reset(new Pcg64(s));
+        return st->pcg64->solve(st->act.M, contact_view(st), *st->sz, st->adj_b.p, tol, st->prm.cg_max_iter, st->lam.p);
+      }
+      return st->pcg->solve(st->act.M, contact_view(st), st->adj_b.p, tol, st->prm.cg_max_iter, st->lam.p);
+    };
+    uint32_t outer = 0;
+    double outer_res = 0.0;
+    if (hessian == NGUO_ADJ_EXACT) {
+      // H0 with the unprojected elastic blocks is the closer splitting (fewer solves); where it
+      // is indefinite (its first solve breaks down) H0 becomes the Newton Hessian and the
+      // elastic blocks' differences join H2
+      ps = solve_exact_adjoint(st, tol, false, solve_h0, &outer, &outer_res);
+      if (ps.breakdown && outer == 1) {
+        const uint32_t first_iters = ps.iters;
+        assemble_elastic(st, st->P, s);
+        prepare_schwarz(st);
+        ps = solve_exact_adjoint(st, tol, true, solve_h0, &outer, &outer_res);
+        ps.iters += first_iters;
+      }
     } else {
-      ps = st->pcg->solve(st->act.M, contact_view(st), st->adj_b.p, tol, st->prm.cg_max_iter, st->lam.p);
+      ps = solve_h0();
     }
     NGUO_HIP(hipMemcpyAsync(st->x0.p, st->x.p, 3 * (size_t)st->V * sizeof(float), hipMemcpyDeviceToDevice, s));
     NGUO_REQUIRE(!ps.breakdown, NGUO_E_UNSUPPORTED,
                  "nguo_step_adjoint: the adjoint Hessian is not positive definite at x* (PCG breakdown); use the Newton Hessian");
+    NGUO_REQUIRE(hessian != NGUO_ADJ_EXACT || outer_res <= (double)EXACT_ADJ_ACCEPT, NGUO_E_UNSUPPORTED,
+                 "nguo_step_adjoint: the exact adjoint's outer iteration did not converge (relative update " + std::to_string(outer_res) +
+                     " after " + std::to_string(outer) + " solves); the exact Jacobian is far from the Newton Hessian at x*");
     st->dgds.zero(nfc, s);
     if (st->act.ns)
       hipLaunchKernelGGL(k_shell_dgds, dim3(div_up(st->act.ns, BLOCK)), dim3(BLOCK), 0, s, st->act.dev(), st->P, (const float *)st->x.p,
@@ -2801,8 +3176,8 @@ int nguo_step_adjoint(nguo_step *st, const float *dLdx, float tol, int hessian, 
     NGUO_HIP(hipStreamSynchronize(s));
     if (stats) {
       stats->iters = ps.iters;
-      stats->sweeps = ps.sweeps;
-      stats->residual = ps.residual;
+      stats->sweeps = hessian == NGUO_ADJ_EXACT ? outer : ps.sweeps;
+      stats->residual = hessian == NGUO_ADJ_EXACT ? outer_res : ps.residual;
     }
     return NGUO_OK;
   });

@@ -509,13 +509,18 @@ class Stepper:
         sc = np.ascontiguousarray(face_scale, dtype=np.float32)
         check(load().nguo_step_set_rest_scale(self.h, fptr(sc), len(sc)))
 
-    def adjoint(self, dLdx, tol=1e-6, exact_elastic=False):
+    ADJ_NEWTON, ADJ_EXACT_ELASTIC, ADJ_EXACT = 0, 1, 2
+
+    def adjoint(self, dLdx, tol=1e-6, exact_elastic=False, hessian=None):
         """dL/ds per shell face through the last step's equilibrium, given dL/dx [n, 3], and
-        the adjoint solve's stats. exact_elastic: unprojected shell / rod-segment blocks."""
+        the adjoint solve's stats. exact_elastic: unprojected shell / rod-segment blocks;
+        hessian (overrides it): ADJ_NEWTON, ADJ_EXACT_ELASTIC or ADJ_EXACT (the exact Jacobian
+        of the step's gradient with the contact parameters held at x*)."""
         g = np.ascontiguousarray(dLdx, dtype=np.float32).reshape(-1)
         out = np.zeros(self.nfaces, np.float32)
         st = Stats()
-        check(load().nguo_step_adjoint(self.h, fptr(g), float(tol), 1 if exact_elastic else 0, fptr(out), C.byref(st)))
+        mode = hessian if hessian is not None else (self.ADJ_EXACT_ELASTIC if exact_elastic else self.ADJ_NEWTON)
+        check(load().nguo_step_adjoint(self.h, fptr(g), float(tol), int(mode), fptr(out), C.byref(st)))
         return out, st
 
     ENERGY_TERMS = ("inertia", "shells", "hinges", "rods", "rod_bends", "barriers", "friction", "colliders", "springs")

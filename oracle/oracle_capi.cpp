@@ -272,7 +272,8 @@ double oracle_solver_energy(void *p, const double *x, const double *xt, double i
 namespace {
 // Assembly switches of the derivative checks: 1 = the frozen-direction gradient model of the
 // held quantities, 2 = unprojected shell and rod-segment blocks, 4 = no strand bending terms,
-// 8 (with 1) = the frozen-parameter model instead (directions of x, exact Jacobian, FP64 CSR).
+// 8 (with 1) = the frozen-parameter model instead (directions of x, exact Jacobian, FP64 CSR),
+// 16 (with 9) = that Jacobian without the barrier's b'(d) d(grad d)/dx blocks.
 template <class S> struct Switches {
   S &s;
   Switches(S &solver, const typename S::Frozen *f, int flags) : s(solver) {
@@ -281,10 +282,11 @@ template <class S> struct Switches {
     s.with_rod_hinges = !(flags & 4);
     s.exact_jac = (flags & 9) == 9;
     s.csr64 = (flags & 9) == 9;
+    s.skip_stencil_jac = (flags & 25) == 25;
   }
   ~Switches() {
     s.gn = nullptr; s.project_elastic = true; s.with_rod_hinges = true; s.keep_blocks = nullptr;
-    s.exact_jac = false; s.csr64 = false;
+    s.exact_jac = false; s.csr64 = false; s.skip_stencil_jac = false;
   }
 };
 }  // namespace
@@ -362,12 +364,13 @@ int oracle_solver_hess_vec(void *p, const double *x, const double *xt, double in
   return h->fp64 ? go(*h->s64, 0.0) : go(*h->s32, 0.0f);
 }
 
-// Holds the quantities of an assembly at the last step's end state (its inertial target, its
-// start as the friction reference) for the frozen-parameter calls below.
-void oracle_solver_freeze_end(void *p) {
+// Holds the quantities of an assembly at the last step's end state (x null) or at x [3 n],
+// with that step's inertial target and its start as the friction reference, for the
+// frozen-parameter calls below.
+void oracle_solver_freeze_end(void *p, const double *x) {
   auto *h = (Handle *)p;
-  if (h->fp64) h->fz64.reset(new Solver<double>::Frozen(h->s64->freeze_end()));
-  else h->fz32.reset(new Solver<float>::Frozen(h->s32->freeze_end()));
+  if (h->fp64) h->fz64.reset(new Solver<double>::Frozen(h->s64->freeze_end(x)));
+  else h->fz32.reset(new Solver<float>::Frozen(h->s32->freeze_end(x)));
 }
 
 // Equilibrium of the frozen-parameter gradient (held quantities of oracle_solver_freeze_end,

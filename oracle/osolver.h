@@ -461,6 +461,7 @@ template <class T> class Solver {
   // strand Jacobian and a sphere collider's b'(d) (I - n n^T) / r join the Gauss-Newton terms.
   // The adjoint of an equilibrium of this gradient is its exact derivative.
   bool exact_jac = false;
+  bool skip_stencil_jac = false;  // with exact_jac: leave the barrier's b'(d) d(grad d)/dx out
   bool csr64 = false;  // the assembled CSR keeps FP64 values (Csr::values64) as well
   ShellElem<T> scaled(const ShellElem<T> &el) const {
     if (face_scale.empty()) return el;
@@ -1035,7 +1036,7 @@ template <class T> class Solver {
                 if (vh != T(0)) push(3 * s.idx[a] + r, 3 * s.idx[b] + cc, vh);
               }
         }
-        if (exact_jac && bg != T(0)) {
+        if (exact_jac && !skip_stencil_jac && bg != T(0)) {
           double J[12][12];
           stencil_jacobian(x, s, J);
           for (int a = 0; a < 4; ++a)
@@ -1575,11 +1576,13 @@ template <class T> class Solver {
     return out;
   }
 
-  // The held quantities (Frozen) of an assembly at the last step's end state, with that step's
-  // inertial target and friction reference (its start).
-  Frozen freeze_end() {
+  // The held quantities (Frozen) of an assembly at the last step's end state (or at xat, 3 n,
+  // when given), with that step's inertial target and friction reference (its start).
+  Frozen freeze_end(const double *xat = nullptr) {
     const T h = T(prm.dt);
     std::vector<V> keep = pos, x = pos;
+    if (xat)
+      for (size_t i = 0; i < x.size(); ++i) x[i] = V(T(xat[3 * i]), T(xat[3 * i + 1]), T(xat[3 * i + 2]));
     pos = prev;
     Frozen f = freeze(x, last_xt, T(1) / (h * h));
     pos = keep;

@@ -100,7 +100,7 @@ def load():
         lib.oracle_solver_hess_vec.restype = C.c_int
         lib.oracle_solver_hess_vec.argtypes = [vp, D, D, C.c_double, C.c_int, D, D]
         lib.oracle_solver_x_tilde_d.argtypes = [vp, D]
-        lib.oracle_solver_freeze_end.argtypes = [vp]
+        lib.oracle_solver_freeze_end.argtypes = [vp, D]
         lib.oracle_solver_equilibrium.restype = C.c_int
         lib.oracle_solver_equilibrium.argtypes = [vp, D, C.c_double, C.c_uint32, D, D]
         lib.oracle_solver_adjoint_frozen.restype = C.c_int
@@ -268,10 +268,11 @@ class OracleSolver:
         """Hold the contact quantities of an assembly at x for energy / model_gradient."""
         self.lib.oracle_solver_freeze(self.h, dp(d64(x)), dp(d64(xt)), inv_h2)
 
-    def freeze_end(self):
-        """Hold the quantities of an assembly at the last step's end state (its target, its
-        start as the friction reference) for equilibrium / adjoint_frozen / FROZEN_PARAMS."""
-        self.lib.oracle_solver_freeze_end(self.h)
+    def freeze_end(self, x=None):
+        """Hold the quantities of an assembly at the last step's end state (or at x), with the
+        step's target and its start as the friction reference, for equilibrium /
+        adjoint_frozen / FROZEN_PARAMS."""
+        self.lib.oracle_solver_freeze_end(self.h, None if x is None else dp(d64(x).reshape(-1)))
 
     def equilibrium(self, x=None, tol=1e-13, max_iter=50):
         """(x, Newton updates, final largest free gradient entry) of the frozen-parameter

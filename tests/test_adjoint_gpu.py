@@ -157,3 +157,90 @@ def test_adjoint_needs_a_step(ctx):
     with pytest.raises(nguo_hip.NguoError):
         st.set_rest_scale(np.ones(3, np.float32))
     st.close()
+
+
+def _exact_vs_frozen_reference(ctx, s, warm, seed):
+    """`warm` oracle steps, then one step from that state on both sides, then the device's
+    exact adjoint (NGUO_ADJ_EXACT) against the oracle's frozen-parameter adjoint (parameters
+    held at the device's end state, exact Jacobian, FP64 CG) at the device's end state, with
+    the same dL/dx."""
+    panel = nguo_hip.drape_panels(s)
+    theta, _, _ = D.draws()
+    st = nguo_hip.Stepper(ctx, s)
+    o = O.OracleSolver(s, fp64=True)
+    o.step(warm)
+    xs, vs = o.positions(), o.velocities()
+    for obj in (o, st):
+        obj.set_rest_scale(theta[panel].astype(np.float32) if obj is st else theta[panel])
+        obj.set_state(xs, vs)
+    o.step(1)
+    st.advance(1)
+    m = st.metrics()
+    ns = m.num_stencils
+    print(f"differentiated step: newton {m.newton_iters}, last toi {m.last_toi}, penetration free {m.penetration_free}")
+    assert ns > 0, "the state must be in contact"
+    x = st.positions().astype(np.float64)
+    rng = np.random.default_rng(seed)
+    free = s.inv_mass > 0
+    dLdx = np.zeros((s.n_verts, 3))
+    dLdx[free] = rng.normal(size=(free.sum(), 3)) / free.sum()
+    g_dev, stats = st.adjoint(dLdx, tol=1e-7, hessian=st.ADJ_EXACT)
+    o.freeze_end(x)
+    g_ref, _, res = o.adjoint_frozen(dLdx, s.desc.n_shell_faces, x, cg_tol=1e-12)
+    assert res < 1e-10
+    g_newton, _ = st.adjoint(dLdx, tol=1e-7)
+    pd = np.bincount(panel, weights=g_dev.astype(np.float64), minlength=64)
+    pr = np.bincount(panel, weights=g_ref, minlength=64)
+    pn = np.bincount(panel, weights=g_newton.astype(np.float64), minlength=64)
+    rel = np.linalg.norm(pd - pr) / np.linalg.norm(pr)
+    reln = np.linalg.norm(pn - pr) / np.linalg.norm(pr)
+    print(f"stencils {ns}: exact panels rel {rel:.3e} (Newton Hessian {reln:.3e}), solves {stats.sweeps}, "
+          f"PCG iterations {stats.iters}, final update {stats.residual:.2e}")
+    st.close()
+    return rel, reln, stats
+
+
+@pytest.mark.gpu
+def test_exact_adjoint_matches_frozen_reference_in_contact(ctx):
+    # config 101 seven steps in (the state whose frozen-parameter adjoint the CPU tests pin
+    # against central differences at 1.2e-6)
+    s = D.tube_scene()
+    rel, reln, stats = _exact_vs_frozen_reference(ctx, s, 7, 13)
+    assert stats.residual <= 1e-6
+    assert rel <= REF_REL
+    assert reln > 10 * REF_REL  # the terms the Newton Hessian leaves out are resolved
+
+
+@pytest.mark.gpu
+def test_exact_adjoint_is_refused_where_the_jacobian_is_indefinite(ctx):
+    # The tube garment one step (the scene's 4-iteration Newton cap) from a device drape state
+    # 6 steps in: the step ends short of its root and the exact Jacobian there is indefinite.
+    # The device refuses the exact form rather than return a gradient, and the oracle's FP64
+    # CG on the same Jacobian (parameters held at the device's end state) breaks down too.
+    s = nguo_hip.Scene(1, seed=1)
+    panel = nguo_hip.drape_panels(s)
+    theta, _, _ = D.draws()
+    st = nguo_hip.Stepper(ctx, s)
+    st.advance(6)
+    xs, vs = st.positions(), st.velocities()
+    o = O.OracleSolver(s, fp64=True)
+    for obj in (o, st):
+        obj.set_rest_scale(theta[panel].astype(np.float32) if obj is st else theta[panel])
+        obj.set_state(xs, vs)
+    o.step(1)
+    st.advance(1)
+    x = st.positions().astype(np.float64)
+    free = s.inv_mass > 0
+    dLdx = np.zeros((s.n_verts, 3))
+    dLdx[free] = np.random.default_rng(14).normal(size=(free.sum(), 3)) / free.sum()
+    with pytest.raises(nguo_hip.NguoError) as e:
+        st.adjoint(dLdx, tol=1e-7, hessian=st.ADJ_EXACT)
+    assert "did not converge" in str(e.value) or "not positive definite" in str(e.value), str(e.value)
+    o.freeze_end(x)
+    _, it, res = o.adjoint_frozen(dLdx, s.desc.n_shell_faces, x, cg_tol=1e-12)
+    print(f"oracle CG on the exact Jacobian: {it} iterations, residual {res:.3e}")
+    assert res > 1e-3
+    # the Newton-Hessian adjoint still solves there
+    g, stats = st.adjoint(dLdx, tol=1e-7)
+    assert np.all(np.isfinite(g)) and stats.residual <= 1e-7
+    st.close()
