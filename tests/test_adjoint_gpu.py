@@ -244,3 +244,59 @@ def test_exact_adjoint_is_refused_where_the_jacobian_is_indefinite(ctx):
     g, stats = st.adjoint(dLdx, tol=1e-7)
     assert np.all(np.isfinite(g)) and stats.residual <= 1e-7
     st.close()
+
+
+FD_REL_FP32_STATE = 2e-4  # |g - fd| / max |fd| over the 8 panels, for the device's FP32 end state
+
+
+@pytest.mark.gpu
+def test_exact_adjoint_matches_central_fd_in_contact(ctx):
+    # The device's exact adjoint at its own end state on config 101 seven steps in, against
+    # FP64 central differences (h = 1e-5) through the frozen-parameter root with the
+    # parameters held at the device's end state (the oracle's Newton re-solves the root from
+    # there at theta +- h on the 8 checked panels). The loss is linear, L = c . x with c =
+    # 2 (x* - target) / n fixed (the distance loss's gradient at x*, target the drape of a
+    # second draw), so the device and the differences differentiate the same function. The
+    # device's FP32 end state sits ~1.6e-6 m from the FP64 root, which alone moves the
+    # derivative by ~1e-4 of its largest component (the oracle's adjoint at the root itself
+    # meets the differences to 4e-8 there), hence the bar relative to the largest derivative.
+    s = D.tube_scene()
+    panel = nguo_hip.drape_panels(s)
+    theta, theta_t, pick = D.draws()
+    o = O.OracleSolver(s, fp64=True)
+    o.step(7)
+    xs, vs = o.positions(), o.velocities()
+    o.set_rest_scale(theta_t[panel])
+    o.step(1)
+    target = o.positions_d()
+    st = nguo_hip.Stepper(ctx, s)
+    for obj in (o, st):
+        obj.set_rest_scale(theta[panel].astype(np.float32) if obj is st else theta[panel])
+        obj.set_state(xs, vs)
+    o.step(1)
+    st.advance(1)
+    assert st.metrics().num_stencils > 0
+    x = st.positions().astype(np.float64)
+    free = s.inv_mass > 0
+    c = np.zeros((s.n_verts, 3))
+    c[free] = 2.0 * (x - target)[free] / free.sum()
+    g_face, stats = st.adjoint(c, tol=1e-7, hessian=st.ADJ_EXACT)
+    g = np.bincount(panel, weights=g_face.astype(np.float64), minlength=64)
+    o.freeze_end(x)
+    h = 1e-5
+    fd = []
+    for k in pick:
+        e = np.zeros(64)
+        e[k] = h
+        lv = []
+        for th in (theta + e, theta - e):
+            o.set_rest_scale(th[panel])
+            xr, it, res = o.equilibrium(x)
+            assert res < 1e-10
+            lv.append(float((c * xr).sum()))
+        fd.append((lv[0] - lv[1]) / (2 * h))
+    fd = np.array(fd)
+    err = np.abs(g[pick] - fd).max() / np.abs(fd).max()
+    print(f"max |g - fd| / max |fd| = {err:.3e}; per panel", np.abs(g[pick] - fd) / np.abs(fd))
+    assert err <= FD_REL_FP32_STATE
+    st.close()
