@@ -1575,7 +1575,7 @@ struct PhaseClock {
     ms[ph] += std::chrono::duration<double, std::milli>(now - t).count();
     t = now;
   }
-  void report(double time) const {
+  void report(double time, uint32_t verts) const {
     if (!on) return;
     static const char *names[PH_N] = {"assemble", "contacts", "domains", "solve", "ccd", "update", "post"};
     double tot = 0.0;
@@ -1586,7 +1586,7 @@ struct PhaseClock {
       std::snprintf(buf, sizeof buf, " %s %.3f", names[i], ms[i]);
       line += buf;
     }
-    std::fprintf(stderr, "nguo phase ms t=%.4f:%s total %.3f\n", time, line.c_str(), tot);
+    std::fprintf(stderr, "nguo phase ms V=%u t=%.4f:%s total %.3f\n", verts, time, line.c_str(), tot);
   }
 };
 
@@ -2380,7 +2380,7 @@ void advance_one(nguo_step *st) {
   m.device_free = (uint32_t)(g_dev_frees - frees0);
   st->adj_valid = !bad;
   st->clock.mark(PH_POST, s);
-  st->clock.report(st->time);
+  st->clock.report(st->time, st->V);
 }
 
 }  // namespace
