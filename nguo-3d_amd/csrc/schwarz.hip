@@ -513,31 +513,36 @@ __global__ __launch_bounds__(BLOCK, 3) void k_schwarz_build(SellView A, ContactV
     }
   }
   __syncthreads();
-  // contact: hp g_a g_b^T + lambda w_a w_b (I - n n^T) for partners inside the domain (one
-  // thread per row: the row's incidences are added in list order); the row's incidence range
-  // is kept for the apply
+  // contact: hp g_a g_b^T + lambda w_a w_b (I - n n^T) for partners inside the domain. One
+  // thread per scalar row 3 p + r (so each entry of L has one writer): the row's incidences
+  // are added in list order, partners in stencil order; the row's incidence range is kept
+  // for the apply
   if (C.ns) {
-    for (uint32_t p = t; p < nv; p += BLOCK) {
+    for (uint32_t i = t; i < 3 * nv; i += BLOCK) {
+      const uint32_t p = i / 3, r = i % 3;
       const uint32_t row = dom_row[base + p];
-      inc_rng[2 * ((size_t)d * SCHWARZ_MAX_DOMAIN + p)] = C.inc_ptr[row];
-      inc_rng[2 * ((size_t)d * SCHWARZ_MAX_DOMAIN + p) + 1] = C.inc_ptr[row + 1];
-      for (uint32_t k = C.inc_ptr[row]; k < C.inc_ptr[row + 1]; ++k) {
-        uint32_t e = C.inc[k], s = e >> 2, a = e & 3u;
-        double hp = C.coef[2 * s], lam = C.coef[2 * s + 1];
-        const float *ga = C.g + 12 * (size_t)s + 3 * a;
+      const uint32_t k0 = C.inc_ptr[row], k1 = C.inc_ptr[row + 1];
+      if (r == 0) {
+        inc_rng[2 * ((size_t)d * SCHWARZ_MAX_DOMAIN + p)] = k0;
+        inc_rng[2 * ((size_t)d * SCHWARZ_MAX_DOMAIN + p) + 1] = k1;
+      }
+      double *Lr = L + tri(3 * p + r);
+      for (uint32_t k = k0; k < k1; ++k) {
+        const uint32_t e = C.inc[k], s = e >> 2, a = e & 3u;
+        const double hp = C.coef[2 * s], lam = C.coef[2 * s + 1];
+        const double gar = (double)C.g[12 * (size_t)s + 3 * a + r];
         const float *n = C.n + 3 * (size_t)s;
-        double wa = C.w[4 * (size_t)s + a];
+        const double nr = (double)n[r];
+        const double wa = C.w[4 * (size_t)s + a];
         for (uint32_t b = 0; b < 4; ++b) {
-          uint32_t db = C.dof[4 * (size_t)s + b];
+          const uint32_t db = C.dof[4 * (size_t)s + b];
           if (db == NONE_S || row_dom[db] != d) continue;
-          uint32_t q = row_pos[db];
+          const uint32_t q = row_pos[db];
           const float *gb = C.g + 12 * (size_t)s + 3 * b;
-          double wb = C.w[4 * (size_t)s + b];
-          for (int r = 0; r < 3; ++r)
-            for (int cc = 0; cc < 3; ++cc)
-              if (3 * q + cc <= 3 * p + r)
-                L[tri(3 * p + r) + 3 * q + cc] +=
-                    hp * (double)ga[r] * (double)gb[cc] + lam * wa * wb * ((r == cc ? 1.0 : 0.0) - (double)n[r] * (double)n[cc]);
+          const double wb = C.w[4 * (size_t)s + b];
+          for (uint32_t cc = 0; cc < 3; ++cc)
+            if (3 * q + cc <= 3 * p + r)
+              Lr[3 * q + cc] += hp * gar * (double)gb[cc] + lam * wa * wb * ((r == cc ? 1.0 : 0.0) - nr * (double)n[cc]);
         }
       }
     }
