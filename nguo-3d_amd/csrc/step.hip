@@ -2957,11 +2957,12 @@ int nguo_step_set_rest_scale(nguo_step *st, const float *face_scale, uint32_t nf
 // solve of the step's PCG to `tol`; Anderson mixing over the last EXACT_ADJ_DEPTH updates
 // (the affine fixed point's GMRES) converges where the fixed-point map's spectral radius
 // (that of H0^-1 H2) alone would be slow. Stops when the relative update |G(lam) - lam| /
-// |G(lam)| falls to tol, or after EXACT_ADJ_MAX solves. b = adj_b on entry (FP32, free dofs),
+// |G(lam)| falls to tol, after EXACT_ADJ_MAX solves, or after EXACT_ADJ_GIVE_UP solves while
+// still above 1e-2. b = adj_b on entry (FP32, free dofs),
 // the solution in lam; returns the last solve's stats with iters summed over all solves.
 }  // extern "C"
 namespace {
-constexpr uint32_t EXACT_ADJ_DEPTH = 6, EXACT_ADJ_MAX = 60;
+constexpr uint32_t EXACT_ADJ_DEPTH = 6, EXACT_ADJ_MAX = 60, EXACT_ADJ_GIVE_UP = 15;
 constexpr float EXACT_ADJ_ACCEPT = 1e-3f;  // relative update above which the result is refused
 template <class Solve>
 PcgStats solve_exact_adjoint(nguo_step *st, float tol, bool elastic_diffs, Solve solve_h0, uint32_t *outer, double *outer_res) {
@@ -3040,6 +3041,9 @@ PcgStats solve_exact_adjoint(nguo_step *st, float tol, bool elastic_diffs, Solve
     const double gn = nrm(g);
     *outer_res = gn > 0.0 ? nrm(f) / gn : 0.0;
     if (nb == 0 || *outer_res <= (double)tol) break;
+    // a mixing that has not reached 1e-2 in EXACT_ADJ_GIVE_UP solves is not converging (the
+    // exact Jacobian far from H0, typically indefinite): stop rather than spend the budget
+    if (*outer >= EXACT_ADJ_GIVE_UP && *outer_res > 1e-2) break;
     if (!first) {
       std::vector<double> df(n), dg(n);
       for (size_t i = 0; i < n; ++i) { df[i] = f[i] - f_prev[i]; dg[i] = g[i] - g_prev[i]; }
