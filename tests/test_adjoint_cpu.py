@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 from conftest import nguo_hip
+import cases
 import drape_case as D
 
 FD_REL = 1e-4  # |adjoint - central FD| / |central FD| per checked parameter
@@ -129,3 +130,34 @@ def test_newton_hessian_adjoint_misses_in_contact(tube):
     g = np.bincount(dr.panel, weights=g_face, minlength=64)
     rel = np.abs(g[pick] - fd) / np.abs(fd)
     assert rel.max() > 1e-2, rel
+
+
+@pytest.fixture(scope="module")
+def sphere():
+    """The 10 x 10 sheet on the analytic sphere after 20 steps (65 Newton iterations to converge
+    the next one; the sphere's barrier and friction hold the sheet, no self contact): the
+    curvature term b'(d) (I - n n^T) / r of the sphere's barrier and the hinges' second
+    derivatives. Checked on the first 8 panels that hold faces of the flat sheet."""
+    s = cases.collider_scene()
+    p = s.desc.params
+    p.newton_iters = 300
+    p.cg_tol = 1e-10
+    p.cg_max_iter = 100000
+    panel = nguo_hip.drape_panels(s)
+    pick = np.flatnonzero(np.bincount(panel, minlength=64) > 0)[:8]
+    theta, theta_t, _ = D.draws()
+    dr = D.FrozenDrape(s, 20, panel, theta, theta_t)
+    fd = np.array([dr.central_fd(theta, k, FD_H_CONTACT) for k in pick])
+    return dr, theta, pick, fd
+
+
+def test_frozen_parameter_adjoint_on_the_sphere_collider(sphere):
+    dr, theta, pick, fd = sphere
+    assert dr.step_metrics["newton_iters"] < 300
+    g, res, x = dr.adjoint(theta)
+    assert res < 1e-10 and np.all(np.abs(fd) > 0)
+    rel = np.abs(g[pick] - fd) / np.abs(fd)
+    assert rel.max() <= FD_REL, rel
+    g_face, _, _ = dr.o.adjoint(dr.dloss_dx(x), dr.nf, 1e-13, exact=True, xstar=x)
+    gn = np.bincount(dr.panel, weights=g_face, minlength=64)
+    assert (np.abs(gn[pick] - fd) / np.abs(fd)).max() > 1e-2

@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 from conftest import nguo_hip
+import cases
 import drape_case as D
 import oracle_py as O
 
@@ -178,7 +179,7 @@ def _exact_vs_frozen_reference(ctx, s, warm, seed):
     m = st.metrics()
     ns = m.num_stencils
     print(f"differentiated step: newton {m.newton_iters}, last toi {m.last_toi}, penetration free {m.penetration_free}")
-    assert ns > 0, "the state must be in contact"
+    assert ns > 0 or m.num_contacts > 0, "the state must be in contact"
     x = st.positions().astype(np.float64)
     rng = np.random.default_rng(seed)
     free = s.inv_mass > 0
@@ -209,6 +210,58 @@ def test_exact_adjoint_matches_frozen_reference_in_contact(ctx):
     assert stats.residual <= 1e-6
     assert rel <= REF_REL
     assert reln > 10 * REF_REL  # the terms the Newton Hessian leaves out are resolved
+
+
+@pytest.mark.gpu
+def test_exact_adjoint_matches_frozen_reference_on_the_sphere_collider(ctx):
+    # The 10 x 10 sheet on the analytic sphere 20 steps in: the sphere's curvature term and the
+    # hinges' second derivatives (k_adj_spheres, k_adj_hinges), no contact stencils. On this
+    # nearly isotropically stretched sheet the device's FP32 unprojected shell blocks already
+    # differ from the oracle's FP64 ones by ~2e-4 of the gradient (the exact-elastic form shows
+    # the same), so the bar is that shared part plus REF_REL, and the exact terms themselves
+    # (exact minus exact-elastic, on both sides) are compared separately.
+    s = cases.collider_scene()
+    p = s.desc.params
+    p.newton_iters = 300
+    p.cg_tol = 1e-10
+    p.cg_max_iter = 100000
+    panel = nguo_hip.drape_panels(s)
+    theta, _, _ = D.draws()
+    o = O.OracleSolver(s, fp64=True)
+    o.step(20)
+    xs, vs = o.positions(), o.velocities()
+    st = nguo_hip.Stepper(ctx, s)
+    for obj in (o, st):
+        obj.set_rest_scale(theta[panel].astype(np.float32) if obj is st else theta[panel])
+        obj.set_state(xs, vs)
+    o.step(1)
+    st.advance(1)
+    assert st.metrics().num_contacts > 0
+    x = st.positions().astype(np.float64)
+    free = s.inv_mass > 0
+    dLdx = np.zeros((s.n_verts, 3))
+    dLdx[free] = np.random.default_rng(15).normal(size=(free.sum(), 3)) / free.sum()
+    g_ex, stats = st.adjoint(dLdx, tol=1e-7, hessian=st.ADJ_EXACT)
+    g_el, _ = st.adjoint(dLdx, tol=1e-7, hessian=st.ADJ_EXACT_ELASTIC)
+    o.freeze_end(x)
+    r_ex, _, res = o.adjoint_frozen(dLdx, s.desc.n_shell_faces, x, cg_tol=1e-13)
+    r_el, _, res2 = o.adjoint(dLdx, s.desc.n_shell_faces, 1e-13, exact=True, xstar=x)
+    assert res < 1e-10 and res2 < 1e-10
+
+    def pan(g):
+        return np.bincount(panel, weights=np.asarray(g, np.float64), minlength=64)
+
+    rel_ex = np.linalg.norm(pan(g_ex) - pan(r_ex)) / np.linalg.norm(pan(r_ex))
+    rel_el = np.linalg.norm(pan(g_el) - pan(r_el)) / np.linalg.norm(pan(r_el))
+    d_dev, d_ref = pan(g_ex) - pan(g_el), pan(r_ex) - pan(r_el)
+    rel_terms = np.linalg.norm(d_dev - d_ref) / np.linalg.norm(d_ref)
+    size = np.linalg.norm(d_ref) / np.linalg.norm(pan(r_ex))
+    print(f"exact {rel_ex:.3e}, exact-elastic {rel_el:.3e}, the exact terms {rel_terms:.3e} (they are {size:.3e} of the gradient), "
+          f"solves {stats.sweeps}")
+    assert stats.residual <= 1e-6
+    assert rel_ex <= REF_REL + rel_el
+    assert size > 10 * REF_REL and rel_terms <= 1e-2
+    st.close()
 
 
 @pytest.mark.gpu
