@@ -3038,9 +3038,13 @@ PcgStats solve_exact_adjoint(nguo_step *st, float tol, bool elastic_diffs, Solve
       g[i] = (double)lf[i];
       f[i] = g[i] - lam[i];
     }
+    if (nb == 0) {  // no blocks: H2 = 0 and the one solve is the answer
+      *outer_res = 0.0;
+      break;
+    }
     const double gn = nrm(g);
     *outer_res = gn > 0.0 ? nrm(f) / gn : 0.0;
-    if (nb == 0 || *outer_res <= (double)tol) break;
+    if (*outer_res <= (double)tol) break;
     // a mixing that has not reached 1e-2 in EXACT_ADJ_GIVE_UP solves is not converging (the
     // exact Jacobian far from H0, typically indefinite): stop rather than spend the budget
     if (*outer >= EXACT_ADJ_GIVE_UP && *outer_res > 1e-2) break;

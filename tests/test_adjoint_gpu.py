@@ -144,6 +144,27 @@ def test_adjoint_matches_central_fd(ctx):
 
 
 @pytest.mark.gpu
+def test_exact_adjoint_reduces_to_exact_elastic_without_contact(ctx):
+    # the contact-free sheet with hinge bending off: the exact Jacobian's added blocks are all
+    # zero, so the exact form takes one solve and returns the exact-elastic gradient bitwise
+    s = D.sheet_scene()
+    theta, _, _ = D.draws()
+    panel = nguo_hip.drape_panels(s, axes=(0, 2))
+    st = nguo_hip.Stepper(ctx, s)
+    st.advance(5)
+    st.set_rest_scale(theta[panel].astype(np.float32))
+    st.advance(1)
+    free = s.inv_mass > 0
+    dLdx = np.zeros((s.n_verts, 3))
+    dLdx[free] = np.random.default_rng(16).normal(size=(free.sum(), 3)) / free.sum()
+    g_el, st_el = st.adjoint(dLdx, tol=1e-7, hessian=st.ADJ_EXACT_ELASTIC)
+    g_ex, st_ex = st.adjoint(dLdx, tol=1e-7, hessian=st.ADJ_EXACT)
+    assert st_ex.sweeps == 1 and st_ex.iters == st_el.iters
+    assert np.array_equal(g_ex, g_el)
+    st.close()
+
+
+@pytest.mark.gpu
 def test_adjoint_needs_a_step(ctx):
     s = nguo_hip.Scene(100, seed=1)
     st = nguo_hip.Stepper(ctx, s)
