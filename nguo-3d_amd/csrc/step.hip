@@ -1084,6 +1084,55 @@ __device__ inline float ccd_query_pad(float pad, float dq, float max_disp, uint3
   const float w = iters >= CCD_TIGHT_ITERS ? dq + max_disp : 2.0f * max_disp;
   return fminf(pad, 1.3f * w * (1.0f + 1.0e-5f) + 1.0e-9f);
 }
+// A second bound of the same kind from the displacements' spread: every vertex of a candidate
+// pair has each component of its dx within [lo, hi], the union of the query's own range and
+// the other side's (the movable primitives' range, k_dx_range, or 0 for a tree whose vertices
+// do not move), so u_max, the largest |dx_i - dx_j| over the pair's four vertices, is at most
+// U = |hi - lo|, and boxes more than 1.3 U apart are surely clear (accd_surely_clear) and
+// return the horizon. When the cloth moves nearly rigidly (falling as a whole), U is far below
+// 2 max_disp; the pairs it drops return the horizon either way, so the result is unchanged.
+__device__ inline float ccd_spread_pad(float pad, f3 qlo, f3 qhi, f3 olo, f3 ohi) {
+  if (!(olo.x <= ohi.x)) return pad;  // no movable vertex recorded: no bound
+  const f3 r{fmaxf(qhi.x, ohi.x) - fminf(qlo.x, olo.x), fmaxf(qhi.y, ohi.y) - fminf(qlo.y, olo.y),
+             fmaxf(qhi.z, ohi.z) - fminf(qlo.z, olo.z)};
+  return fminf(pad, 1.3f * len(r) * (1.0f + 1.0e-5f) + 1.0e-9f);
+}
+// Order-preserving map of a float to a u32 (atomic min / max on the keys).
+__device__ inline uint32_t fkey(float f) {
+  const uint32_t b = __float_as_uint(f);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ inline float fkey_inv(uint32_t k) { return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k); }
+// The movable primitives' displacement range from k_dx_range (keys: min xyz, max xyz).
+__device__ inline void dx_range(const uint32_t *keys, f3 &lo, f3 &hi) {
+  lo = f3{fkey_inv(keys[0]), fkey_inv(keys[1]), fkey_inv(keys[2])};
+  hi = f3{fkey_inv(keys[3]), fkey_inv(keys[4]), fkey_inv(keys[5])};
+}
+// keys = (+inf x3, -inf x3), or the range {0} when a movable primitive has a fixed vertex
+__global__ void k_dx_range_init(uint32_t *keys, int with_zero) {
+  const uint32_t t = threadIdx.x;
+  if (t < 6) keys[t] = with_zero ? fkey(0.0f) : fkey(t < 3 ? INFINITY : -INFINITY);
+}
+// Component-wise min / max of dx over the free vertices (every vertex of a movable primitive
+// is free or fixed with dx = 0, the latter covered by k_dx_range_init's with_zero).
+__global__ __launch_bounds__(BLOCK) void k_dx_range(uint32_t V, const uint32_t *__restrict__ vdof, const float *__restrict__ dx, uint32_t *keys) {
+  const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+  const bool ok = i < V && vdof[i] != NONE;
+  const f3 d = ok ? ld3(dx, i) : f3{0, 0, 0};
+  float lo[3] = {ok ? d.x : INFINITY, ok ? d.y : INFINITY, ok ? d.z : INFINITY};
+  float hi[3] = {ok ? d.x : -INFINITY, ok ? d.y : -INFINITY, ok ? d.z : -INFINITY};
+  for (int c = 0; c < 3; ++c) {
+    lo[c] = wave_min(lo[c]);
+    hi[c] = wave_max(hi[c]);
+  }
+  if ((threadIdx.x & (WAVE - 1)) == 0)
+    for (int c = 0; c < 3; ++c) {
+      if (lo[c] <= hi[c]) {
+        atomicMin(&keys[c], fkey(lo[c]));
+        atomicMax(&keys[3 + c], fkey(hi[c]));
+      }
+    }
+}
 struct CcdDefer {
   uint32_t *count;  // pairs appended
   uint32_t *pairs;  // [2 cap]: (a | kind << 31, b): vertex-triangle (v, t) or edge-edge (lo, hi)
@@ -1101,7 +1150,8 @@ __device__ inline bool defer_pair(const CcdDefer &D, uint32_t a, uint32_t b, uin
 // vertex ids come with the traversal record) and test the filters from registers: the
 // filters are predicates, so their order changes nothing but the latency.
 __global__ __launch_bounds__(BLOCK) void k_ccd_vt(Topo T, BvhView dyn, BvhView stat, const float *__restrict__ x, const float *__restrict__ dx,
-                                                 float pad0, float max_disp, uint32_t iters, CcdDefer D, unsigned long long *tmin, uint32_t *err) {
+                                                 float pad0, float max_disp, uint32_t iters, const uint32_t *__restrict__ dxr, CcdDefer D,
+                                                 unsigned long long *tmin, uint32_t *err) {
   __shared__ uint32_t tstack[TRAV_LDS * BLOCK];
   uint32_t v = blockIdx.x * BLOCK + threadIdx.x;
   double best = 1.0;
@@ -1112,7 +1162,12 @@ __global__ __launch_bounds__(BLOCK) void k_ccd_vt(Topo T, BvhView dyn, BvhView s
     const bool below = T.side ? T.side[v] < 0.0f : false;
     ExcList exl;
     exl.load(T, v);
-    const float pad = ccd_query_pad(pad0, len(dvv), max_disp, iters);
+    // parts 0 .. TRAV_PARTS-1: the movable tree's subtrees; the rest: the static tree's, whose
+    // vertices do not move
+    const uint32_t part = blockIdx.y;
+    f3 olo{0, 0, 0}, ohi{0, 0, 0};
+    if (part < TRAV_PARTS) dx_range(dxr, olo, ohi);
+    const float pad = ccd_spread_pad(ccd_query_pad(pad0, len(dvv), max_disp, iters), dvv, dvv, olo, ohi);
     auto leaf = [&](uint32_t t, uint3 vid) {
       const uint32_t t0 = vid.x, t1 = vid.y, t2 = vid.z;
       f3 a = ld3(x, t0), b = ld3(x, t1), c = ld3(x, t2);
@@ -1138,8 +1193,6 @@ __global__ __launch_bounds__(BLOCK) void k_ccd_vt(Topo T, BvhView dyn, BvhView s
       best = fmin(best, toi / 1.25);
     };
     auto ov = [&](f3 lo, f3 hi) { return inside_padded(p, lo, hi, pad); };
-    // parts 0 .. TRAV_PARTS-1: the movable tree's subtrees; the rest: the static tree's
-    const uint32_t part = blockIdx.y;
     if (part < TRAV_PARTS) bvh_traverse_part(dyn, part, err, tstack, ov, leaf);
     else if (mv) bvh_traverse_part(stat, part - TRAV_PARTS, err, tstack, ov, leaf);
   }
@@ -1150,8 +1203,8 @@ __global__ __launch_bounds__(BLOCK) void k_ccd_vt(Topo T, BvhView dyn, BvhView s
 
 // Edge-edge ACCD (implicit.rs:1902-1936).
 __global__ __launch_bounds__(BLOCK) void k_ccd_ee(Topo T, BvhView tree, const uint32_t *qids, uint32_t nq, int self_query, const float *__restrict__ x,
-                                                 const float *__restrict__ dx, float pad0, float max_disp, uint32_t iters, CcdDefer D,
-                                                 unsigned long long *tmin, uint32_t *err) {
+                                                 const float *__restrict__ dx, float pad0, float max_disp, uint32_t iters,
+                                                 const uint32_t *__restrict__ dxr, CcdDefer D, unsigned long long *tmin, uint32_t *err) {
   __shared__ uint32_t tstack[TRAV_LDS * BLOCK];
   uint32_t q = blockIdx.x * BLOCK + threadIdx.x;
   double best = 1.0;
@@ -1160,7 +1213,10 @@ __global__ __launch_bounds__(BLOCK) void k_ccd_ee(Topo T, BvhView tree, const ui
     uint32_t b0 = T.edges[2 * (size_t)ej], b1 = T.edges[2 * (size_t)ej + 1];
     f3 pb = ld3(x, b0), qb = ld3(x, b1);
     const f3 db0 = ld3(dx, b0), db1 = ld3(dx, b1);
-    const float pad = ccd_query_pad(pad0, fmaxf(len(db0), len(db1)), max_disp, iters);
+    // the other side: the movable edges (self query) or the static edges (dx = 0)
+    f3 olo{0, 0, 0}, ohi{0, 0, 0};
+    if (self_query) dx_range(dxr, olo, ohi);
+    const float pad = ccd_spread_pad(ccd_query_pad(pad0, fmaxf(len(db0), len(db1)), max_disp, iters), vmin(db0, db1), vmax(db0, db1), olo, ohi);
     f3 blo = vmin(pb, qb) - f3{pad, pad, pad}, bhi = vmax(pb, qb) + f3{pad, pad, pad};
     auto ov = [&](f3 lo, f3 hi) { return !(lo.x > bhi.x || lo.y > bhi.y || lo.z > bhi.z || hi.x < blo.x || hi.y < blo.y || hi.z < blo.z); };
     // a leaf loads its edge's positions and motions in one memory round trip (the vertex ids
@@ -1611,6 +1667,8 @@ struct nguo_step {
   DevBuf<unsigned long long> tmin;
   // trees and their primitive id lists
   DevBuf<uint32_t> tri_dyn_ids, tri_stat_ids, tri_coll_ids, edge_dyn_ids, edge_stat_ids;
+  DevBuf<uint32_t> dxr;       // the movable primitives' dx range per CCD call (k_dx_range keys)
+  bool dyn_has_fixed = false;  // some movable primitive has a fixed vertex (dx = 0 in the range)
   uint32_t n_tri_dyn = 0, n_tri_stat = 0, n_tri_coll = 0, n_edge_dyn = 0, n_edge_stat = 0;
   Lbvh tree_tri_dyn, tree_tri_stat, tree_tri_coll, tree_edge_dyn, tree_edge_stat;
   // elements: active (system), static (all pinned) and mixed (pinned and free: contact blocks)
@@ -2120,6 +2178,10 @@ float ccd_max_step(nguo_step *st, float max_disp) {
   }
   if (has_self && std::isfinite(max_disp)) {
     uint32_t iters = std::max(st->prm.ccd_max_iter, 1u);
+    // the movable primitives' displacement range (dx is fixed across the passes)
+    hipLaunchKernelGGL(k_dx_range_init, dim3(1), dim3(WAVE), 0, s, st->dxr.p, st->dyn_has_fixed ? 1 : 0);
+    hipLaunchKernelGGL(k_dx_range, dim3(div_up(st->V, BLOCK)), dim3(BLOCK), 0, s, st->V, (const uint32_t *)st->vdof.p, (const float *)st->dx.p,
+                       st->dxr.p);
     float fixed = st->prm.contact_dhat;
     float allowed = std::max(std::min(0.5f * st->char_len, max_disp), 1.0e-9f);
     for (int pass = 0; pass < 64; ++pass) {
@@ -2129,18 +2191,19 @@ float ccd_max_step(nguo_step *st, float max_disp) {
       double t_self = run_min([&] {
         NGUO_HIP(hipMemsetAsync(st->defer_count.p, 0, sizeof(uint32_t), s));
         hipLaunchKernelGGL(k_ccd_vt, dim3(div_up(st->V, BLOCK), 2 * TRAV_PARTS), dim3(BLOCK), 0, s, T, st->tree_tri_dyn.view(), st->tree_tri_stat.view(),
-                           (const float *)st->x.p, (const float *)st->dx.p, search, max_disp, iters, D, st->tmin.p, st->err.p);
+                           (const float *)st->x.p, (const float *)st->dx.p, search, max_disp, iters, (const uint32_t *)st->dxr.p, D, st->tmin.p,
+                           st->err.p);
         if (st->n_edge_dyn)
           hipLaunchKernelGGL(k_ccd_ee, dim3(div_up(st->n_edge_dyn, BLOCK), TRAV_PARTS), dim3(BLOCK), 0, s, T, st->tree_edge_dyn.view(),
                              (const uint32_t *)st->edge_dyn_ids.p, st->n_edge_dyn, 1, (const float *)st->x.p, (const float *)st->dx.p, search, max_disp,
-                             iters, D, st->tmin.p, st->err.p);
+                             iters, (const uint32_t *)st->dxr.p, D, st->tmin.p, st->err.p);
         // movable edges against the static edges' tree: a movable query's padding covers
         // every pair the static side's padding would (its own motion is part of the cap), and
         // a pair beyond the static side's cap returns the horizon, so the minimum is the same
         if (st->n_edge_stat && st->n_edge_dyn)
           hipLaunchKernelGGL(k_ccd_ee, dim3(div_up(st->n_edge_dyn, BLOCK), TRAV_PARTS), dim3(BLOCK), 0, s, T, st->tree_edge_stat.view(),
                              (const uint32_t *)st->edge_dyn_ids.p, st->n_edge_dyn, 0, (const float *)st->x.p, (const float *)st->dx.p, search,
-                             max_disp, iters, D, st->tmin.p, st->err.p);
+                             max_disp, iters, (const uint32_t *)st->dxr.p, D, st->tmin.p, st->err.p);
         if (iters > ACCD_INLINE)
           hipLaunchKernelGGL(k_ccd_deferred, dim3(div_up(CCD_DEFER_CAP, BLOCK)), dim3(BLOCK), 0, s, T, D, (const float *)st->x.p, (const float *)st->dx.p,
                              iters, st->tmin.p);
@@ -2726,6 +2789,11 @@ int nguo_step_create(nguo_ctx *ctx, const nguo_scene_desc *sc, nguo_step **out) 
       n = (uint32_t)v.size();
       b.upload(v.empty() ? std::vector<uint32_t>{0} : v, s);
     };
+    for (uint32_t f : tdyn)
+      for (int k = 0; k < 3; ++k) st->dyn_has_fixed = st->dyn_has_fixed || pinned(st->h_tris[3 * f + k]);
+    for (uint32_t e : edyn)
+      for (int k = 0; k < 2; ++k) st->dyn_has_fixed = st->dyn_has_fixed || pinned(st->h_edges[2 * e + k]);
+    st->dxr.reserve(6);
     upl(st->tri_dyn_ids, tdyn, st->n_tri_dyn);
     upl(st->tri_stat_ids, tstat, st->n_tri_stat);
     upl(st->tri_coll_ids, coll_ids, st->n_tri_coll);
