@@ -256,7 +256,9 @@ int nguo_step_set_rest_scale(nguo_step *st, const float *face_scale, uint32_t nf
  * strand blocks and a sphere's normal curvature. Solved as the fixed point of
  * lam = H0^-1 (dL/dx - H2 lam) (H0 the Newton Hessian, H2 = H - H0, each step one PCG solve to
  * tol) with Anderson mixing: stats->iters sums the PCG iterations, stats->sweeps counts the
- * solves, stats->residual is the final relative update; NGUO_E_UNSUPPORTED above 1e-3. */
+ * solves, stats->residual is the final relative update; NGUO_E_UNSUPPORTED above 1e-3 (the
+ * mixing also stops after 15 solves still above 1e-2: an indefinite exact Jacobian, as where
+ * the step ended short of its root). */
 #define NGUO_ADJ_EXACT 2
 int nguo_step_adjoint(nguo_step *st, const float *dLdx, float tol, int hessian, float *dLds, nguo_stats *stats);
 
