@@ -85,6 +85,10 @@ class Lbvh {
   // kind 0: triangles (3 indices per primitive), 1: edges (2 indices). `ids` selects the
   // primitives of this tree (device array of primitive indices into `idx`).
   void build(int kind, uint32_t n, const uint32_t *ids, const uint32_t *idx, const float *pos, hipStream_t s);
+  // The last build's topology (leaf order, children, ranges, parts) with every box recomputed
+  // from `pos`: a valid tree for any positions, as tight as a build where they moved little
+  // since it.
+  void refit(int kind, const uint32_t *idx, const float *pos, hipStream_t s);
   BvhView view() const {
     return BvhView{n_, left_.p, right_.p, node_box_.p, leaf_box_.p, prim_.p, (const float4 *)trav_.p, (const float4 *)front_.p, slot_.p};
   }

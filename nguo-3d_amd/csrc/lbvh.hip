@@ -604,6 +604,20 @@ __global__ __launch_bounds__(BLOCK) void k_gather_prim(uint32_t n, const uint32_
   if (i < n) prim[i] = ids ? ids[order[i]] : order[i];
 }
 
+void Lbvh::refit(int kind, const uint32_t *idx, const float *pos, hipStream_t s) {
+  const uint32_t n = n_;
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_leaf_boxes, dim3(div_up(n, BLOCK)), dim3(BLOCK), 0, s, kind, n, (const uint32_t *)prim_.p, idx, pos, leaf_box_.p);
+  const TravSrc src{kind, idx, range_.p, prim_.p, leaf_box_.p, node_box_.p};
+  if (n >= 2) {
+    aabb_from_ranges(n, range_.p, leaf_box_.p, node_box_.p, table_, s);
+    hipLaunchKernelGGL(k_pack_trav, dim3(div_up(n - 1, BLOCK)), dim3(BLOCK), 0, s, n, (const uint32_t *)left_.p, (const uint32_t *)right_.p, src,
+                       (float4 *)trav_.p);
+  }
+  hipLaunchKernelGGL(k_frontier, dim3(1), dim3(WAVE), 0, s, n, (const uint32_t *)left_.p, (const uint32_t *)right_.p, src, (float4 *)front_.p);
+  NGUO_HIP(hipGetLastError());
+}
+
 void Lbvh::build(int kind, uint32_t n, const uint32_t *ids, const uint32_t *idx, const float *pos, hipStream_t s) {
   n_ = n;
   if (n == 0) return;

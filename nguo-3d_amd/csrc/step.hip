@@ -1668,6 +1668,7 @@ struct nguo_step {
   // trees and their primitive id lists
   DevBuf<uint32_t> tri_dyn_ids, tri_stat_ids, tri_coll_ids, edge_dyn_ids, edge_stat_ids;
   DevBuf<uint32_t> dxr;       // the movable primitives' dx range per CCD call (k_dx_range keys)
+  uint64_t tree_epoch = 1, built_epoch = 0;  // the dynamic trees are rebuilt once per step (epoch)
   bool dyn_has_fixed = false;  // some movable primitive has a fixed vertex (dx = 0 in the range)
   uint32_t n_tri_dyn = 0, n_tri_stat = 0, n_tri_coll = 0, n_edge_dyn = 0, n_edge_stat = 0;
   Lbvh tree_tri_dyn, tree_tri_stat, tree_tri_coll, tree_edge_dyn, tree_edge_stat;
@@ -2082,9 +2083,17 @@ void detect(nguo_step *st) {
   if (!st->prm.self_collision) return;
   Topo T = topo(st);
   float dhat = st->prm.contact_dhat;
-  // dynamic trees at x
-  st->tree_tri_dyn.build(0, st->n_tri_dyn, st->tri_dyn_ids.p, st->tris.p, st->x.p, s);
-  st->tree_edge_dyn.build(1, st->n_edge_dyn, st->edge_dyn_ids.p, st->edges.p, st->x.p, s);
+  // dynamic trees at x: built at a step's first detection, refit at its later Newton
+  // iterations (the candidate sets, and so the sorted stencil lists, do not depend on the
+  // topology; only the traversal's pruning does)
+  if (st->tree_epoch != st->built_epoch) {
+    st->tree_tri_dyn.build(0, st->n_tri_dyn, st->tri_dyn_ids.p, st->tris.p, st->x.p, s);
+    st->tree_edge_dyn.build(1, st->n_edge_dyn, st->edge_dyn_ids.p, st->edges.p, st->x.p, s);
+    st->built_epoch = st->tree_epoch;
+  } else {
+    st->tree_tri_dyn.refit(0, st->tris.p, st->x.p, s);
+    st->tree_edge_dyn.refit(1, st->edges.p, st->x.p, s);
+  }
   if (st->has_collider) {
     NGUO_HIP(hipMemsetAsync(st->vt_cell.p, 0, sizeof(uint32_t), s));
     hipLaunchKernelGGL(k_sides, dim3(div_up(st->V, BLOCK)), dim3(BLOCK), 0, s, T, st->tree_tri_coll.view(), (const float *)st->x.p, dhat,
@@ -2336,6 +2345,7 @@ struct GateMembership {
 
 void advance_one(nguo_step *st) {
   hipStream_t s = st->s;
+  ++st->tree_epoch;
   const uint64_t allocs0 = g_dev_allocs, frees0 = g_dev_frees;
   GateMembership member(uses_fp64(st) ? nullptr : st->gate);  // FP64 solves run on the instance's own stream
   const uint32_t V = st->V;
@@ -2412,7 +2422,8 @@ void advance_one(nguo_step *st) {
     hit |= st->hu[5];
   }
   if (st->prm.self_collision && !bad) {
-    st->tree_tri_dyn.build(0, st->n_tri_dyn, st->tri_dyn_ids.p, st->tris.p, st->x.p, s);
+    if (st->built_epoch == st->tree_epoch) st->tree_tri_dyn.refit(0, st->tris.p, st->x.p, s);  // built this step
+    else st->tree_tri_dyn.build(0, st->n_tri_dyn, st->tri_dyn_ids.p, st->tris.p, st->x.p, s);
     Topo T = topo(st);
     hipLaunchKernelGGL(k_pierce, dim3(div_up(st->E, BLOCK)), dim3(BLOCK), 0, s, T, st->tree_tri_dyn.view(), st->tree_tri_stat.view(), st->E,
                        (const float *)st->x.p, st->flag.p + 1, st->err.p);
