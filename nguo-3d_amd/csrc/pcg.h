@@ -17,6 +17,7 @@ struct SellMatrix {
   // slots), so a slice's first slot is 64 s ell_w without reading slice_ptr
   uint32_t ell_w = 0;
   std::vector<uint32_t> h_slice_ptr, h_row_len, h_col;
+  uint64_t pattern_id = 0;  // distinct for every build_pattern call in the process
   DevBuf<uint32_t> slice_ptr, row_len, col, diag_slot;
   DevBuf<float> val;
   // cols[i] = sorted block columns of block row i (must contain i).
@@ -52,6 +53,10 @@ __host__ __device__ constexpr uint32_t sym_tile64(uint32_t P, uint32_t Q) { retu
 struct Schwarz {
   uint32_t nd = 0, domain = SCHWARZ_MAX_DOMAIN;
   std::vector<uint32_t> h_dom_ptr, h_dom_row;
+  // the matrix pattern's block neighbours of every row in row order (CSR), for the contact
+  // domains' growth; rebuilt when the pattern (its column array) is another
+  std::vector<uint32_t> h_nbr_ptr, h_nbr;
+  uint64_t nbr_of = 0;  // SellMatrix::pattern_id of the lists (0: none)
   DevBuf<uint32_t> dom_ptr, dom_row, row_dom, row_pos, fails;
   DevBuf<uint32_t> dom_dof;  // [nd * 96]: stride-4 vector index of dof t of domain d, or ~0u past m
   DevBuf<float> ainv;  // nd blocks of SCHWARZ_INV_FLOATS (upper 16 x 16 tiles, row-major each)

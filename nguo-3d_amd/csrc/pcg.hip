@@ -6,6 +6,7 @@
 // matrix is stored as 3x3 blocks (36 B values + 4 B column per block instead of 9 x 8 B),
 // the scalars alpha/beta/rz live on the device in FP64, vectors are FP32, and the true
 // residual of each refinement sweep is recomputed in FP64.
+#include <atomic>
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -22,6 +23,8 @@ constexpr uint32_t NONE = 0xffffffffu;
 constexpr double ELL_MAX_PAD = 1.15;
 
 void SellMatrix::build_pattern(uint32_t nblocks, const std::vector<std::vector<uint32_t>> &cols, hipStream_t s) {
+  static std::atomic<uint64_t> next_id{1};
+  pattern_id = next_id.fetch_add(1);
   nb = nblocks;
   nslices = (nb + 63) / 64;
   h_slice_ptr.assign(nslices + 1, 0);

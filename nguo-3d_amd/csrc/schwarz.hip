@@ -25,6 +25,15 @@ void Schwarz::build(const SellMatrix &M, uint32_t domain_size, hipStream_t s, fl
   h_dom_ptr.assign(1, 0);
   h_dom_row.clear();
   std::vector<uint32_t> dom;
+  if (nbr_of != M.pattern_id || h_nbr_ptr.size() != (size_t)nv + 1) {
+    h_nbr_ptr.assign(1, 0);
+    h_nbr.clear();
+    for (uint32_t v = 0; v < nv; ++v) {
+      for (uint32_t k = 0; k < M.h_row_len[v]; ++k) h_nbr.push_back(M.h_col[M.slot(v, k)]);
+      h_nbr_ptr.push_back((uint32_t)h_nbr.size());
+    }
+    nbr_of = M.pattern_id;
+  }
   for (uint32_t seed = 0; seed < nv; ++seed) {
     if (assigned[seed]) continue;
     dom.assign(1, seed);
@@ -153,8 +162,8 @@ void Schwarz::build_contact(const SellMatrix &M, uint32_t ns, const uint32_t *do
     while (dom.size() < domain && fi < dom.size()) {
       const uint32_t v = dom[fi++];
       for (uint32_t k = cptr[v]; k < cptr[v + 1] && dom.size() < domain; ++k) take(cadj[k].second);
-      for (uint32_t k = 0; k < M.h_row_len[v] && dom.size() < domain; ++k) {
-        const uint32_t nb = M.h_col[M.slot(v, k)];
+      for (uint32_t k = h_nbr_ptr[v]; k < h_nbr_ptr[v + 1] && dom.size() < domain; ++k) {
+        const uint32_t nb = h_nbr[k];
         if (nb != v) take(nb);
       }
     }
