@@ -34,9 +34,10 @@ __global__ __launch_bounds__(BLOCK) void k_radix_hist(uint32_t n, const K *__res
   hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
 }
 
-// Exclusive scan of one value per thread over the workgroup (fixed order); returns the
-// prefix, *total receives the workgroup sum.
-__device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t *wsum /* [BLOCK / WAVE] */, uint32_t *total) {
+// Exclusive scan of one value per thread over a workgroup of NT threads (fixed order); returns
+// the prefix, *total receives the workgroup sum.
+template <uint32_t NT = BLOCK>
+__device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t *wsum /* [NT / WAVE] */, uint32_t *total) {
   const uint32_t lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
   uint32_t x = v;
 #pragma unroll
@@ -47,7 +48,7 @@ __device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t *wsum /* [BLOCK 
   if (lane == WAVE - 1) wsum[w] = x;
   __syncthreads();
   uint32_t before = 0, all = 0;
-  for (uint32_t q = 0; q < BLOCK / WAVE; ++q) {
+  for (uint32_t q = 0; q < NT / WAVE; ++q) {
     if (q < w) before += wsum[q];
     all += wsum[q];
   }
@@ -59,25 +60,27 @@ __device__ inline uint32_t block_excl_scan(uint32_t v, uint32_t *wsum /* [BLOCK 
 // In-place exclusive scan of up to SCAN_EXCL_MAX counters by one workgroup: staged in LDS
 // with coalesced loads, each thread sums a contiguous segment, the segment sums are scanned
 // across the workgroup by wavefront shuffles, and each thread writes its segment's prefixes.
-constexpr uint32_t SCAN_EXCL_MAX = 16384;
-__global__ __launch_bounds__(BLOCK) void k_scan_excl(uint32_t total, uint32_t *a) {
+// 1024 threads: a quarter of the serial segment per thread that 256 would leave.
+constexpr uint32_t SCAN_EXCL_MAX = 16384, SCAN_EXCL_THREADS = 1024;
+__global__ __launch_bounds__(SCAN_EXCL_THREADS) void k_scan_excl(uint32_t total, uint32_t *a) {
+  constexpr uint32_t NT = SCAN_EXCL_THREADS;
   __shared__ uint32_t buf[SCAN_EXCL_MAX];
-  __shared__ uint32_t wsum[BLOCK / WAVE];
-  for (uint32_t i = threadIdx.x; i < total; i += BLOCK) buf[i] = a[i];
+  __shared__ uint32_t wsum[NT / WAVE];
+  for (uint32_t i = threadIdx.x; i < total; i += NT) buf[i] = a[i];
   __syncthreads();
-  const uint32_t seg = (total + BLOCK - 1) / BLOCK;
+  const uint32_t seg = (total + NT - 1) / NT;
   const uint32_t lo = min(total, threadIdx.x * seg), hi = min(total, lo + seg);
   uint32_t s = 0;
   for (uint32_t i = lo; i < hi; ++i) s += buf[i];
   uint32_t all;
-  uint32_t acc = block_excl_scan(s, wsum, &all);
+  uint32_t acc = block_excl_scan<NT>(s, wsum, &all);
   for (uint32_t i = lo; i < hi; ++i) {
     const uint32_t v = buf[i];
     buf[i] = acc;
     acc += v;
   }
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < total; i += BLOCK) a[i] = buf[i];
+  for (uint32_t i = threadIdx.x; i < total; i += NT) a[i] = buf[i];
 }
 
 // Large scans: each workgroup scans SCAN_TILE counters in place (exclusive, local) and
@@ -187,12 +190,12 @@ static void radix_passes(uint32_t n, K *keys, uint32_t *vals, K *ktmp, uint32_t 
   for (int sh : shifts) {
     hipLaunchKernelGGL((k_radix_hist<K, ITEMS>), dim3(nblocks), dim3(BLOCK), 0, s, n, (const K *)ka, sh, hist, nblocks);
     if (total <= SCAN_EXCL_MAX) {
-      hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(BLOCK), 0, s, total, hist);
+      hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(SCAN_EXCL_THREADS), 0, s, total, hist);
     } else {
       const uint32_t nt = div_up(total, SCAN_TILE);
       NGUO_REQUIRE(nt <= SCAN_EXCL_MAX, NGUO_E_CAPACITY, "radix sort: too many tiles for the tile-sum scan");
       hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(BLOCK), 0, s, total, hist, sums);
-      hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(BLOCK), 0, s, nt, sums);
+      hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(SCAN_EXCL_THREADS), 0, s, nt, sums);
       hipLaunchKernelGGL(k_scan_add, dim3(nt), dim3(BLOCK), 0, s, total, hist, (const uint32_t *)sums);
     }
     hipLaunchKernelGGL((k_radix_scatter<K, ITEMS>), dim3(nblocks), dim3(BLOCK), 0, s, n, (const K *)ka, (const uint32_t *)va, kb, vb, sh,
